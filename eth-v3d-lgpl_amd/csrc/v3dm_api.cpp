@@ -1,0 +1,525 @@
+// C ABI of the gfx950 putative matcher: handle state, buffer management, orchestration.
+//
+// Mirrors V3D_CUDA::SIFT_Matching (V3D/CUDA/v3d_cudamatching.h:15-67,
+// v3d_cudamatching.cu:439-694).  Device buffers are owned by the handle; descriptor
+// buffers hold a multiple of 128 rows so tile loads never leave the allocation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "../../include/v3dm.h"
+#include "v3dm_device.h"
+#include "v3dm_kernels.h"
+#include "v3dm_ext.h"
+
+using namespace v3dm;
+
+namespace {
+
+thread_local std::string g_createError;
+
+int roundUp(int n, int m) { return (n + m - 1) / m * m; }
+
+}  // namespace
+
+struct v3dm_context {
+    int device = 0;
+    hipStream_t ownStream = nullptr;
+    hipStream_t stream = nullptr;
+
+    bool warnOnRounding = true;     // h:18
+    bool useReductionPath = false;  // h:19
+    bool useGuided = false;
+    bool allocated = false;
+    int precision = V3DM_PREC_FP32;
+
+    int nMaxL = 0, nMaxR = 0;  // rounded maxima (cu:444-463)
+    int nL = 0, nR = 0;        // current counts (cu:502, cu:518)
+    int capL = 0, capR = 0;    // allocated rows (multiple of kRowPad)
+
+    float* dL = nullptr;
+    float* dR = nullptr;
+    float* normL = nullptr;
+    float* normR = nullptr;
+    unsigned int* maxNormBits = nullptr;  // [0] = left, [1] = right
+    Top2* colPart = nullptr;
+    Top2* rowPart = nullptr;
+    int* colMax = nullptr;
+    int* rowMax = nullptr;
+    int* flagList = nullptr;
+    int* results = nullptr;  // counters (kNumCounters ints) followed by int2 pairs
+    int* hostStage = nullptr;
+    int pairCap = 0;
+    uint8_t* dQ = nullptr;  // uint8 upload staging (capL or capR rows)
+    size_t dQBytes = 0;
+
+    // guided matching state: masks applied since the last findPutativeMatches
+    std::vector<GuidedMask> masks;
+    float* dLeftXY = nullptr;
+    float* dRightXY = nullptr;
+
+    // last result
+    bool haveResult = false;
+    v3dm_stats stats{};
+    int lastNL = 0, lastNR = 0;
+
+    // profiling of the dominant kernel
+    bool profiling = false;
+    std::vector<hipEvent_t> evStart, evStop;
+    int evPending = 0;
+    double kernelMs = 0.0;
+    int kernelLaunches = 0;
+
+    ExtState* ext = nullptr;
+
+    std::string err;
+};
+
+namespace {
+
+int fail(v3dm_context* h, int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (h) h->err = buf;
+    else g_createError = buf;
+    return code;
+}
+
+#define HIPCHECK(h, call)                                                                           \
+    do {                                                                                            \
+        hipError_t const e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                       \
+            return fail((h), (e_ == hipErrorOutOfMemory) ? V3DM_ERR_NOMEM : V3DM_ERR_HIP,          \
+                        "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+template <typename T>
+void freeDev(T*& p)
+{
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+void releaseBuffers(v3dm_context* h)
+{
+    freeDev(h->dL);
+    freeDev(h->dR);
+    freeDev(h->normL);
+    freeDev(h->normR);
+    freeDev(h->maxNormBits);
+    freeDev(h->colPart);
+    freeDev(h->rowPart);
+    freeDev(h->colMax);
+    freeDev(h->rowMax);
+    freeDev(h->flagList);
+    freeDev(h->results);
+    freeDev(h->dQ);
+    freeDev(h->dLeftXY);
+    freeDev(h->dRightXY);
+    if (h->hostStage) (void)hipHostFree(h->hostStage);
+    h->hostStage = nullptr;
+    h->dQBytes = 0;
+    h->allocated = false;
+    h->nL = h->nR = 0;
+    h->haveResult = false;
+    h->masks.clear();
+    ext_release(h->ext);
+}
+
+// The ring of event pairs is harvested when it is full and in v3dm_get_stats.
+int harvestEvents(v3dm_context* h)
+{
+    for (int e = 0; e < h->evPending; ++e) {
+        HIPCHECK(h, hipEventSynchronize(h->evStop[e]));
+        float ms = 0.0f;
+        HIPCHECK(h, hipEventElapsedTime(&ms, h->evStart[e], h->evStop[e]));
+        h->kernelMs += ms;
+        h->kernelLaunches += 1;
+    }
+    h->evPending = 0;
+    return V3DM_OK;
+}
+
+int roundWarn(v3dm_context* h, int n, const char* who, const char* what)
+{
+    if ((n % 32) != 0) {
+        if (h->warnOnRounding) {
+            std::cerr << "SIFT_Matching::" << who << " warning: " << what << " should be a multiple of 32"
+                      << std::endl;
+            std::cerr << "Rounding " << what << " down." << std::endl;
+        }
+        n = n - (n & 31);
+    }
+    return n;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------- internal API
+namespace v3dm {
+
+hipStream_t ctx_stream(v3dm_context* h) { return h->stream; }
+ExtState*& ctx_ext(v3dm_context* h) { return h->ext; }
+int ctx_fail(v3dm_context* h, int code, const char* msg) { return fail(h, code, "%s", msg); }
+
+// Enqueue the plain FP32 exact path for the current features; leaves colMax/rowMax
+// decided and the compacted pairs + counters in h->results.
+int enqueue_plain(v3dm_context* h, float minScore, float minRatio)
+{
+    hipStream_t const st = h->stream;
+    int const nL = h->nL, nR = h->nR;
+    launch_reset_counters(h->results, kNumCounters, st);
+    if (nL > 0 && nR > 0) {
+        FusedPlan const p = plan_fused_fp32(nL, nR, 512);
+        int const ev = h->evPending;
+        if (h->profiling) HIPCHECK(h, hipEventRecord(h->evStart[ev], st));
+        launch_fused_fp32(h->dL, h->dR, nL, nR, p, h->colPart, h->capL, h->rowPart, h->capR, st);
+        if (h->profiling) {
+            HIPCHECK(h, hipEventRecord(h->evStop[ev], st));
+            h->evPending += 1;
+        }
+        launch_merge_decide(h->colPart, p.splits, h->capL, h->rowPart, p.leftBlocks, h->capR, nL, nR, h->normL,
+                            h->normR, h->maxNormBits, kEpsRelFp32, minScore, minRatio, h->colMax, h->rowMax,
+                            h->flagList, h->results, st);
+        launch_recheck_exact(h->dL, h->dR, nL, nR, h->flagList, h->results, minScore, minRatio, h->colMax,
+                             h->rowMax, 512, st);
+        launch_mutual_compact(h->colMax, h->rowMax, nL, nR, reinterpret_cast<int2*>(h->results + kNumCounters),
+                              h->results, st);
+    }
+    HIPCHECK(h, hipGetLastError());
+    if (h->profiling && h->evPending == (int)h->evStart.size()) {
+        int const rc = harvestEvents(h);
+        if (rc) return rc;
+    }
+    return V3DM_OK;
+}
+
+}  // namespace v3dm
+
+// ---------------------------------------------------------------------------------------- C ABI
+extern "C" {
+
+int v3dm_create(v3dm_handle* out, int device)
+{
+    if (!out) return fail(nullptr, V3DM_ERR_INVALID, "v3dm_create: null handle pointer");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0)
+        return fail(nullptr, V3DM_ERR_HIP, "v3dm_create: no HIP device available (%s)", hipGetErrorString(e));
+    if (device < 0 || device >= count)
+        return fail(nullptr, V3DM_ERR_INVALID, "v3dm_create: device %d out of range [0, %d)", device, count);
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return fail(nullptr, V3DM_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+    v3dm_context* h = new v3dm_context();
+    h->device = device;
+    e = hipStreamCreateWithFlags(&h->ownStream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete h;
+        return fail(nullptr, V3DM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    h->stream = h->ownStream;
+    *out = h;
+    return V3DM_OK;
+}
+
+void v3dm_destroy(v3dm_handle h)
+{
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    (void)hipStreamSynchronize(h->stream);
+    releaseBuffers(h);
+    ext_destroy(h->ext);
+    for (auto e : h->evStart) (void)hipEventDestroy(e);
+    for (auto e : h->evStop) (void)hipEventDestroy(e);
+    if (h->ownStream) (void)hipStreamDestroy(h->ownStream);
+    delete h;
+}
+
+const char* v3dm_last_error(v3dm_handle h) { return h ? h->err.c_str() : g_createError.c_str(); }
+
+int v3dm_allocate(v3dm_handle h, int nMaxLeft, int nMaxRight, int useGuidedMatching)
+{
+    if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
+    if (nMaxLeft < 0 || nMaxRight < 0) return fail(h, V3DM_ERR_INVALID, "allocate: negative maximum");
+    if (nMaxLeft > (1 << 20) || nMaxRight > (1 << 20))
+        return fail(h, V3DM_ERR_INVALID, "allocate: maximum above 2^20 features");
+    HIPCHECK(h, hipSetDevice(h->device));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    releaseBuffers(h);
+    h->useGuided = useGuidedMatching != 0;
+    nMaxLeft = roundWarn(h, nMaxLeft, "allocate()", "nMaxLeftFeatures");
+    nMaxRight = roundWarn(h, nMaxRight, "allocate()", "nMaxRightFeatures");
+    h->nMaxL = nMaxLeft;
+    h->nMaxR = nMaxRight;
+    h->capL = std::max(kRowPad, roundUp(nMaxLeft, kRowPad));
+    h->capR = std::max(kRowPad, roundUp(nMaxRight, kRowPad));
+    size_t const cl = (size_t)h->capL, cr = (size_t)h->capR;
+    HIPCHECK(h, hipMalloc(&h->dL, cl * kDim * sizeof(float)));
+    HIPCHECK(h, hipMalloc(&h->dR, cr * kDim * sizeof(float)));
+    HIPCHECK(h, hipMemsetAsync(h->dL, 0, cl * kDim * sizeof(float), h->stream));
+    HIPCHECK(h, hipMemsetAsync(h->dR, 0, cr * kDim * sizeof(float), h->stream));
+    HIPCHECK(h, hipMalloc(&h->normL, cl * sizeof(float)));
+    HIPCHECK(h, hipMalloc(&h->normR, cr * sizeof(float)));
+    HIPCHECK(h, hipMalloc(&h->maxNormBits, 4 * sizeof(unsigned int)));
+    HIPCHECK(h, hipMemsetAsync(h->maxNormBits, 0, 4 * sizeof(unsigned int), h->stream));
+    HIPCHECK(h, hipMalloc(&h->colPart, (size_t)kMaxSplits * cl * sizeof(Top2)));
+    HIPCHECK(h, hipMalloc(&h->rowPart, (cl / 128) * cr * sizeof(Top2)));
+    HIPCHECK(h, hipMalloc(&h->colMax, cl * sizeof(int)));
+    HIPCHECK(h, hipMalloc(&h->rowMax, cr * sizeof(int)));
+    HIPCHECK(h, hipMalloc(&h->flagList, (cl + cr) * sizeof(int)));
+    h->pairCap = (int)std::min(cl, cr);
+    HIPCHECK(h, hipMalloc(&h->results, (kNumCounters + 2 * (size_t)h->pairCap) * sizeof(int)));
+    HIPCHECK(h, hipMemsetAsync(h->results, 0, (kNumCounters + 2 * (size_t)h->pairCap) * sizeof(int), h->stream));
+    HIPCHECK(h, hipHostMalloc(&h->hostStage, (kNumCounters + 2 * (size_t)h->pairCap) * sizeof(int)));
+    if (h->useGuided) {
+        HIPCHECK(h, hipMalloc(&h->dLeftXY, cl * 2 * sizeof(float)));
+        HIPCHECK(h, hipMalloc(&h->dRightXY, cr * 2 * sizeof(float)));
+    }
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    h->allocated = true;
+    return V3DM_OK;
+}
+
+int v3dm_deallocate(v3dm_handle h)
+{
+    if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
+    HIPCHECK(h, hipSetDevice(h->device));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    releaseBuffers(h);
+    h->nMaxL = h->nMaxR = 0;
+    return V3DM_OK;
+}
+
+int v3dm_set_flags(v3dm_handle h, int warnOnRounding, int useReductionPath)
+{
+    if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
+    h->warnOnRounding = warnOnRounding != 0;
+    h->useReductionPath = useReductionPath != 0;
+    return V3DM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+enum Source { kHostF32, kDeviceF32, kHostU8 };
+
+int setFeatures(v3dm_context* h, bool left, int n, const void* src, Source kind)
+{
+    if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
+    if (!h->allocated) return fail(h, V3DM_ERR_STATE, "set%sFeatures before allocate", left ? "Left" : "Right");
+    if (n < 0) return fail(h, V3DM_ERR_INVALID, "set%sFeatures: negative count %d", left ? "Left" : "Right", n);
+    n = roundWarn(h, n, left ? "setLeftFeatures()" : "setRightFeatures()", "nFeatures");
+    n = std::min(n, left ? h->nMaxL : h->nMaxR);
+    if (n > 0 && !src) return fail(h, V3DM_ERR_INVALID, "null descriptor pointer");
+    HIPCHECK(h, hipSetDevice(h->device));
+    float* dst = left ? h->dL : h->dR;
+    size_t const bytes = (size_t)n * kDim * sizeof(float);
+    if (n > 0) {
+        if (kind == kHostF32) {
+            HIPCHECK(h, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
+        } else if (kind == kDeviceF32) {
+            HIPCHECK(h, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, h->stream));
+        } else {
+            size_t const qb = (size_t)(left ? h->capL : h->capR) * kDim;
+            if (h->dQBytes < qb) {
+                freeDev(h->dQ);
+                HIPCHECK(h, hipMalloc(&h->dQ, qb));
+                h->dQBytes = qb;
+            }
+            HIPCHECK(h, hipMemcpyAsync(h->dQ, src, (size_t)n * kDim, hipMemcpyHostToDevice, h->stream));
+            launch_dequant_u8(h->dQ, n, dst, h->stream);
+        }
+    }
+    unsigned int* mx = h->maxNormBits + (left ? 0 : 1);
+    HIPCHECK(h, hipMemsetAsync(mx, 0, sizeof(unsigned int), h->stream));
+    launch_norms(dst, n, left ? h->normL : h->normR, mx, h->stream);
+    HIPCHECK(h, hipGetLastError());
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    if (left) h->nL = n;
+    else h->nR = n;
+    h->haveResult = false;
+    h->masks.clear();
+    ext_features_changed(h, left);
+    return V3DM_OK;
+}
+
+int copyResults(v3dm_context* h, int32_t* pairs, int cap, int* nPairs)
+{
+    if (!h->haveResult) return fail(h, V3DM_ERR_STATE, "no result: call findPutativeMatches first");
+    size_t const headBytes = kNumCounters * sizeof(int);
+    HIPCHECK(h, hipMemcpyAsync(h->hostStage, h->results, headBytes, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    int const n = h->hostStage[kCntPairs];
+    if (n > 0)
+        HIPCHECK(h, hipMemcpyAsync(h->hostStage + kNumCounters, h->results + kNumCounters,
+                                   (size_t)n * 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    h->stats.nMatches = n;
+    h->stats.nRechecked = h->hostStage[kCntFlag];
+    if (nPairs) *nPairs = n;
+    int const ncopy = std::min(n, std::max(cap, 0));
+    if (ncopy > 0) {
+        if (!pairs) return fail(h, V3DM_ERR_INVALID, "null pairs buffer");
+        memcpy(pairs, h->hostStage + kNumCounters, (size_t)ncopy * 2 * sizeof(int32_t));
+    }
+    if (n > cap) return fail(h, V3DM_ERR_CAPACITY, "pairs buffer holds %d pairs, %d matches found", cap, n);
+    return V3DM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int v3dm_set_left(v3dm_handle h, int n, const float* desc) { return setFeatures(h, true, n, desc, kHostF32); }
+int v3dm_set_right(v3dm_handle h, int n, const float* desc) { return setFeatures(h, false, n, desc, kHostF32); }
+int v3dm_set_left_device(v3dm_handle h, int n, const float* d) { return setFeatures(h, true, n, d, kDeviceF32); }
+int v3dm_set_right_device(v3dm_handle h, int n, const float* d) { return setFeatures(h, false, n, d, kDeviceF32); }
+int v3dm_set_left_u8(v3dm_handle h, int n, const uint8_t* q) { return setFeatures(h, true, n, q, kHostU8); }
+int v3dm_set_right_u8(v3dm_handle h, int n, const uint8_t* q) { return setFeatures(h, false, n, q, kHostU8); }
+
+int v3dm_set_precision(v3dm_handle h, int mode)
+{
+    if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
+    if (mode < V3DM_PREC_FP32 || mode > V3DM_PREC_LEARNED)
+        return fail(h, V3DM_ERR_INVALID, "unknown precision mode %d", mode);
+    if (!ext_supports(h, mode)) return fail(h, V3DM_ERR_UNSUPPORTED, "precision mode %d is not available", mode);
+    h->precision = mode;
+    return V3DM_OK;
+}
+
+int v3dm_set_stream(v3dm_handle h, void* stream)
+{
+    if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    h->stream = stream ? (hipStream_t)stream : h->ownStream;
+    return V3DM_OK;
+}
+
+int v3dm_find_putative_matches_async(v3dm_handle h, float minScore, float minRatio)
+{
+    if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
+    if (!h->allocated) return fail(h, V3DM_ERR_STATE, "findPutativeMatches before allocate");
+    HIPCHECK(h, hipSetDevice(h->device));
+    h->masks.clear();
+    int rc;
+    if (h->precision == V3DM_PREC_FP32) rc = enqueue_plain(h, minScore, minRatio);
+    else rc = ext_enqueue(h, h->precision, minScore, minRatio);
+    if (rc) return rc;
+    h->haveResult = true;
+    h->lastNL = h->nL;
+    h->lastNR = h->nR;
+    h->stats.nLeft = h->nL;
+    h->stats.nRight = h->nR;
+    h->stats.precision = h->precision;
+    return V3DM_OK;
+}
+
+int v3dm_get_matches(v3dm_handle h, int32_t* pairs, int cap, int* nPairs)
+{
+    if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
+    HIPCHECK(h, hipSetDevice(h->device));
+    return copyResults(h, pairs, cap, nPairs);
+}
+
+int v3dm_find_putative_matches(v3dm_handle h, float minScore, float minRatio, int32_t* pairs, int cap, int* nPairs)
+{
+    int const rc = v3dm_find_putative_matches_async(h, minScore, minRatio);
+    if (rc) return rc;
+    return copyResults(h, pairs, cap, nPairs);
+}
+
+int v3dm_device_results(v3dm_handle h, const int32_t** d_pairs, const int32_t** d_counters)
+{
+    if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
+    if (!h->allocated) return fail(h, V3DM_ERR_STATE, "not allocated");
+    if (d_pairs) *d_pairs = h->results + kNumCounters;
+    if (d_counters) *d_counters = h->results;
+    return V3DM_OK;
+}
+
+int v3dm_get_maxima(v3dm_handle h, int32_t* colMax, int32_t* rowMax)
+{
+    if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
+    if (!h->haveResult) return fail(h, V3DM_ERR_STATE, "no result: call findPutativeMatches first");
+    HIPCHECK(h, hipSetDevice(h->device));
+    if (colMax && h->lastNL > 0)
+        HIPCHECK(h, hipMemcpyAsync(colMax, h->colMax, (size_t)h->lastNL * sizeof(int), hipMemcpyDeviceToHost,
+                                   h->stream));
+    if (rowMax && h->lastNR > 0)
+        HIPCHECK(h, hipMemcpyAsync(rowMax, h->rowMax, (size_t)h->lastNR * sizeof(int), hipMemcpyDeviceToHost,
+                                   h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return V3DM_OK;
+}
+
+int v3dm_set_profiling(v3dm_handle h, int enable)
+{
+    if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
+    HIPCHECK(h, hipSetDevice(h->device));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    if (enable && h->evStart.empty()) {
+        h->evStart.resize(256);
+        h->evStop.resize(256);
+        for (int i = 0; i < 256; ++i) {
+            HIPCHECK(h, hipEventCreate(&h->evStart[i]));
+            HIPCHECK(h, hipEventCreate(&h->evStop[i]));
+        }
+    }
+    h->profiling = enable != 0;
+    h->evPending = 0;
+    h->kernelMs = 0.0;
+    h->kernelLaunches = 0;
+    return V3DM_OK;
+}
+
+int v3dm_get_stats(v3dm_handle h, v3dm_stats* out)
+{
+    if (!h || !out) return fail(h, V3DM_ERR_INVALID, "null argument");
+    HIPCHECK(h, hipSetDevice(h->device));
+    if (h->profiling) {
+        int const rc = harvestEvents(h);
+        if (rc) return rc;
+    }
+    if (h->haveResult) {
+        int cnt[kNumCounters];
+        HIPCHECK(h, hipMemcpyAsync(cnt, h->results, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
+        HIPCHECK(h, hipStreamSynchronize(h->stream));
+        h->stats.nMatches = cnt[kCntPairs];
+        h->stats.nRechecked = cnt[kCntFlag];
+    }
+    h->stats.nFallback = ext_fallbacks(h);
+    *out = h->stats;
+    out->kernelMs = h->kernelMs;
+    out->kernelLaunches = h->kernelLaunches;
+    return V3DM_OK;
+}
+
+int v3dm_run_guided_F(v3dm_handle h, const double F[9], float distanceThreshold, const float* leftXY,
+                      const float* rightXY, float minScore, float minRatio, int32_t* pairs, int cap, int* nPairs)
+{
+    (void)F; (void)distanceThreshold; (void)leftXY; (void)rightXY; (void)minScore; (void)minRatio;
+    (void)pairs; (void)cap; (void)nPairs;
+    return fail(h, V3DM_ERR_UNSUPPORTED, "guided matching is not built yet");
+}
+
+int v3dm_run_guided_H(v3dm_handle h, const double H[9], float distanceThreshold, const float* leftXY,
+                      const float* rightXY, float minScore, float minRatio, int32_t* pairs, int cap, int* nPairs)
+{
+    (void)H; (void)distanceThreshold; (void)leftXY; (void)rightXY; (void)minScore; (void)minRatio;
+    (void)pairs; (void)cap; (void)nPairs;
+    return fail(h, V3DM_ERR_UNSUPPORTED, "guided matching is not built yet");
+}
+
+}  // extern "C"

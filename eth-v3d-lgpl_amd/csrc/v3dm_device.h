@@ -1,0 +1,86 @@
+// Device-side helpers shared by the matcher kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace v3dm {
+
+constexpr int kDim = 128;          // descriptor length
+constexpr int kFlag = -2;          // decision still open: re-checked exactly
+constexpr float kInit = -1e30f;    // initial best / second best (cu:138-140)
+
+// Best, second best (multiset order statistics) and index of the best of one query.
+struct Top2 {
+    float b1, b2;
+    int a1;
+};
+
+__device__ __forceinline__ Top2 top2_init() { return Top2{kInit, kInit, -1}; }
+
+// One candidate, the update rule of cu:145-147.  NaN scores are skipped.
+__device__ __forceinline__ void top2_push(Top2& s, float v, int idx)
+{
+    bool const gt = v > s.b1;
+    s.a1 = gt ? idx : s.a1;
+    s.b2 = gt ? s.b1 : fmaxf(v, s.b2);
+    s.b1 = fmaxf(s.b1, v);
+}
+
+// Order-independent merge of two partial states: larger best wins, an equal best keeps
+// the smaller index (the first index of an ascending scan), the second best is the
+// second largest of the union.  Commutative and associative.
+__device__ __forceinline__ Top2 top2_merge(Top2 a, Top2 b)
+{
+    bool const takeB = (b.b1 > a.b1) || (b.b1 == a.b1 && b.a1 >= 0 && (a.a1 < 0 || b.a1 < a.a1));
+    Top2 r;
+    r.a1 = takeB ? b.a1 : a.a1;
+    r.b2 = fmaxf(fminf(a.b1, b.b1), fmaxf(a.b2, b.b2));
+    r.b1 = fmaxf(a.b1, b.b1);
+    return r;
+}
+
+__device__ __forceinline__ Top2 top2_shfl_xor(Top2 s, int mask)
+{
+    Top2 o;
+    o.b1 = __shfl_xor(s.b1, mask);
+    o.b2 = __shfl_xor(s.b2, mask);
+    o.a1 = __shfl_xor(s.a1, mask);
+    return o;
+}
+
+// Decision of cu:150-161 (ratio) / cu:48 (minRatio < 0), IEEE arithmetic.
+__device__ __forceinline__ bool accept_rule(float b1, float b2, float minScore, float minRatio)
+{
+    if (minRatio < 0.0f) return b1 > minScore;
+    float const nextDist = (float)fmax(0.001, (double)(2.0f - 2.0f * b2));
+    float const bestDist = 2.0f - 2.0f * b1;
+    float const ratio = bestDist / nextDist;
+    return b1 > minScore && ratio < minRatio;
+}
+
+__device__ __forceinline__ int decide_exact(Top2 s, float minScore, float minRatio)
+{
+    return accept_rule(s.b1, s.b2, minScore, minRatio) ? s.a1 : -1;
+}
+
+// Decision from approximate scores whose error against the exact scores is at most eps.
+// Returns the exact decision when every exact score set within the error box gives the
+// same answer, kFlag otherwise.  The rule is monotone (accept grows with b1 and falls
+// with b2) when 2 - 2*b1 > 0 over the box, so the two corners settle it; outside that
+// regime, with a non-finite bound, or with a best/second margin that could reorder the
+// argmax, the query is flagged.
+__device__ __forceinline__ int decide_certified(Top2 s, float eps, float minScore, float minRatio)
+{
+    if (!(eps < 3.0e38f) || !(s.b1 == s.b1) || !(s.b2 == s.b2)) return kFlag;
+    float const lo1 = s.b1 - eps, hi1 = s.b1 + eps;
+    float const lo2 = s.b2 - eps, hi2 = s.b2 + eps;
+    if (minRatio >= 0.0f && !(hi1 < 1.0f)) return kFlag;
+    if (!accept_rule(hi1, lo2, minScore, minRatio)) return -1;
+    if (accept_rule(lo1, hi2, minScore, minRatio) && (s.b1 - s.b2 > 2.0f * eps) && s.a1 >= 0) return s.a1;
+    return kFlag;
+}
+
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+}  // namespace v3dm

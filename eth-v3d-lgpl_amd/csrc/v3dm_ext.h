@@ -1,0 +1,34 @@
+// Internal interface between the C ABI and the shortlist / guided / batch extensions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/v3dm.h"
+
+namespace v3dm {
+
+// A guided-matching mask applied since the last findPutativeMatches (cu:554-694): kind 0 =
+// epipolar (Sampson distance with F), 1 = homography transfer distance with H.
+struct GuidedMask {
+    int kind;
+    float m[9];
+    float sqrThreshold;
+};
+
+struct ExtState;
+
+// accessors into the handle (v3dm_api.cpp)
+hipStream_t ctx_stream(v3dm_context* h);
+ExtState*& ctx_ext(v3dm_context* h);
+int ctx_fail(v3dm_context* h, int code, const char* msg);
+int enqueue_plain(v3dm_context* h, float minScore, float minRatio);
+
+// extension hooks (v3dm_ext.cpp)
+bool ext_supports(v3dm_context* h, int mode);
+int ext_enqueue(v3dm_context* h, int mode, float minScore, float minRatio);
+int ext_fallbacks(v3dm_context* h);
+void ext_features_changed(v3dm_context* h, bool left);
+void ext_release(ExtState* e);
+void ext_destroy(ExtState*& e);
+
+}  // namespace v3dm

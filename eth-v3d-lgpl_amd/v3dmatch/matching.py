@@ -1,0 +1,198 @@
+"""Python mirror of V3D_CUDA::SIFT_Matching over the C ABI (include/v3dm.h).
+
+Method names, argument meaning and defaults follow V3D/CUDA/v3d_cudamatching.h:15-67:
+allocate / deallocate / setLeftFeatures / setRightFeatures / findPutativeMatches
+(minScore=0.9, minRatio=0.8; minRatio < 0 disables the ratio test) /
+runGuidedMatching / runGuidedHomographyMatching / enableRoundingWarning /
+enableReductionPath.  Where the reference fills a std::vector<pair<int,int>> the Python
+methods return a list of (left, right) tuples sorted by left index.  Errors raise
+V3DMError (the reference exits the process, cu:12-27).
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._native import Stats, V3DMError, load_library
+
+PREC_FP32 = 0
+PREC_BF16 = 1
+PREC_FP8 = 2
+PREC_LEARNED = 3
+
+
+def _f32_rows(desc, n):
+    a = np.ascontiguousarray(desc, dtype=np.float32)
+    if a.ndim == 1:
+        a = a.reshape(-1, 128)
+    if a.ndim != 2 or a.shape[1] != 128:
+        raise ValueError("descriptors must have shape (n, 128)")
+    if a.shape[0] < n:
+        raise ValueError(f"descriptor array has {a.shape[0]} rows, nFeatures = {n}")
+    return a
+
+
+def _xy(pos, n):
+    a = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 2)
+    if a.shape[0] < n:
+        raise ValueError(f"position array has {a.shape[0]} rows, need {n}")
+    return a
+
+
+class SIFT_Matching:
+    """Putative matcher on one GPU (V3D_CUDA::SIFT_Matching, v3d_cudamatching.h:15)."""
+
+    def __init__(self, device=0):
+        self._lib = load_library()
+        h = C.c_void_p()
+        rc = self._lib.v3dm_create(C.byref(h), int(device))
+        if rc != 0:
+            raise V3DMError(rc, self._lib.v3dm_last_error(None).decode())
+        self._h = h
+        self._nL = 0
+        self._nR = 0
+        self._nMaxL = 0
+        self._nMaxR = 0
+        self._warn = True
+        self._reduce = False
+
+    # -- life cycle -----------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.v3dm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise V3DMError(rc, self._lib.v3dm_last_error(self._h).decode())
+
+    # -- reference API ---------------------------------------------------------------
+    def enableRoundingWarning(self, flag=True):
+        self._warn = bool(flag)
+        self._check(self._lib.v3dm_set_flags(self._h, int(self._warn), int(self._reduce)))
+
+    def enableReductionPath(self, flag=True):
+        self._reduce = bool(flag)
+        self._check(self._lib.v3dm_set_flags(self._h, int(self._warn), int(self._reduce)))
+
+    def allocate(self, nMaxLeftFeatures, nMaxRightFeatures=None, useGuidedMatching=False):
+        """allocate(nMaxLeft, nMaxRight, useGuided) (cu:439) or allocate(nMax, useGuided) (h:34)."""
+        if isinstance(nMaxRightFeatures, bool):
+            useGuidedMatching, nMaxRightFeatures = nMaxRightFeatures, None
+        if nMaxRightFeatures is None:
+            nMaxRightFeatures = nMaxLeftFeatures
+        self._check(self._lib.v3dm_allocate(self._h, int(nMaxLeftFeatures), int(nMaxRightFeatures),
+                                            int(bool(useGuidedMatching))))
+        self._nMaxL = int(nMaxLeftFeatures) - (int(nMaxLeftFeatures) & 31)
+        self._nMaxR = int(nMaxRightFeatures) - (int(nMaxRightFeatures) & 31)
+
+    def deallocate(self):
+        self._check(self._lib.v3dm_deallocate(self._h))
+
+    def setLeftFeatures(self, nFeatures, descriptors):
+        a = _f32_rows(descriptors, nFeatures) if nFeatures > 0 else None
+        self._check(self._lib.v3dm_set_left(self._h, int(nFeatures), a.ctypes.data if a is not None else None))
+        self._nL = min(int(nFeatures) - (int(nFeatures) & 31), self._nMaxL)
+
+    def setRightFeatures(self, nFeatures, descriptors):
+        a = _f32_rows(descriptors, nFeatures) if nFeatures > 0 else None
+        self._check(self._lib.v3dm_set_right(self._h, int(nFeatures), a.ctypes.data if a is not None else None))
+        self._nR = min(int(nFeatures) - (int(nFeatures) & 31), self._nMaxR)
+
+    def findPutativeMatches(self, minScore=0.9, minRatio=0.8):
+        return [tuple(p) for p in self.find_putative_matches_array(minScore, minRatio).tolist()]
+
+    def runGuidedMatching(self, F, distanceThreshold, leftPositions, rightPositions, minScore=0.9, minRatio=0.8):
+        return self._guided(self._lib.v3dm_run_guided_F, F, distanceThreshold, leftPositions, rightPositions,
+                            minScore, minRatio)
+
+    def runGuidedHomographyMatching(self, H, distanceThreshold, leftPositions, rightPositions, minScore=0.9,
+                                    minRatio=0.8):
+        return self._guided(self._lib.v3dm_run_guided_H, H, distanceThreshold, leftPositions, rightPositions,
+                            minScore, minRatio)
+
+    # -- array forms and extensions --------------------------------------------------
+    def _cap(self):
+        return max(1, min(max(self._nL, 0), max(self._nR, 0)))
+
+    def _pairs_out(self, fn, *args):
+        cap = self._cap()
+        out = np.empty((cap, 2), dtype=np.int32)
+        n = C.c_int(0)
+        self._check(fn(self._h, *args, out.ctypes.data, cap, C.byref(n)))
+        return out[: n.value].copy()
+
+    def find_putative_matches_array(self, minScore=0.9, minRatio=0.8):
+        """Matches as an (n, 2) int32 array of (left, right), sorted by left index."""
+        return self._pairs_out(self._lib.v3dm_find_putative_matches, C.c_float(minScore), C.c_float(minRatio))
+
+    def _guided(self, fn, M, thr, lpos, rpos, minScore, minRatio):
+        m = np.ascontiguousarray(np.asarray(M, dtype=np.float64).reshape(3, 3))
+        lp = _xy(lpos, self._nL)
+        rp = _xy(rpos, self._nR)
+        pairs = self._pairs_out(fn, m.ctypes.data, C.c_float(thr), lp.ctypes.data, rp.ctypes.data,
+                                C.c_float(minScore), C.c_float(minRatio))
+        return [tuple(p) for p in pairs.tolist()]
+
+    def set_precision(self, mode):
+        self._check(self._lib.v3dm_set_precision(self._h, int(mode)))
+
+    def set_stream(self, stream_ptr):
+        self._check(self._lib.v3dm_set_stream(self._h, C.c_void_p(stream_ptr) if stream_ptr else None))
+
+    def set_left_u8(self, q):
+        q = np.ascontiguousarray(q, dtype=np.uint8).reshape(-1, 128)
+        self._check(self._lib.v3dm_set_left_u8(self._h, q.shape[0], q.ctypes.data))
+        self._nL = min(q.shape[0] - (q.shape[0] & 31), self._nMaxL)
+
+    def set_right_u8(self, q):
+        q = np.ascontiguousarray(q, dtype=np.uint8).reshape(-1, 128)
+        self._check(self._lib.v3dm_set_right_u8(self._h, q.shape[0], q.ctypes.data))
+        self._nR = min(q.shape[0] - (q.shape[0] & 31), self._nMaxR)
+
+    def set_left_device(self, n, dptr):
+        self._check(self._lib.v3dm_set_left_device(self._h, int(n), C.c_void_p(dptr)))
+        self._nL = min(int(n) - (int(n) & 31), self._nMaxL)
+
+    def set_right_device(self, n, dptr):
+        self._check(self._lib.v3dm_set_right_device(self._h, int(n), C.c_void_p(dptr)))
+        self._nR = min(int(n) - (int(n) & 31), self._nMaxR)
+
+    def find_async(self, minScore=0.9, minRatio=0.8):
+        self._check(self._lib.v3dm_find_putative_matches_async(self._h, C.c_float(minScore), C.c_float(minRatio)))
+
+    def get_matches(self):
+        return self._pairs_out(self._lib.v3dm_get_matches)
+
+    def get_maxima(self):
+        cm = np.empty(max(self._nL, 1), dtype=np.int32)
+        rm = np.empty(max(self._nR, 1), dtype=np.int32)
+        self._check(self._lib.v3dm_get_maxima(self._h, cm.ctypes.data, rm.ctypes.data))
+        return cm[: self._nL], rm[: self._nR]
+
+    def set_profiling(self, enable=True):
+        self._check(self._lib.v3dm_set_profiling(self._h, int(bool(enable))))
+
+    def stats(self):
+        s = Stats()
+        self._check(self._lib.v3dm_get_stats(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in Stats._fields_}
+
+    @property
+    def nLeft(self):
+        return self._nL
+
+    @property
+    def nRight(self):
+        return self._nR

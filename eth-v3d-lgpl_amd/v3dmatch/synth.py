@@ -1,0 +1,80 @@
+"""Synthetic SIFT-like descriptor sets (ctypes binding of libv3dsynth.so).
+
+Generator definition (eth-v3d-lgpl_amd/csrc/v3dm_synth.c): Gamma(0.5,1) components,
+L2 normalise, clip at 0.2, renormalise, quantise round(512 x) clamped to 255 (uint8),
+dequantise as SIFT_Feature::normalizeDescriptor (V3D/Base/v3d_feature.h:30-38).
+Right sets mix noisy copies of left descriptors (sigma = 0.05, clamped at 0,
+re-quantised) with fresh distractors, randomly permuted.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from ._native import V3DMError, lib_path
+
+BASE_SEED = 0x5EED_E7A3D
+SIGMA = 0.05
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        p = lib_path("libv3dsynth.so")
+        if not os.path.exists(p):
+            raise V3DMError(5, f"{p} is missing: build it with eth-v3d-lgpl_amd/build.py")
+        lib = C.CDLL(p)
+        lib.v3ds_seed.restype = C.c_uint64
+        lib.v3ds_seed.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]
+        lib.v3ds_dequantise.restype = None
+        lib.v3ds_dequantise.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        lib.v3ds_fresh.restype = None
+        lib.v3ds_fresh.argtypes = [C.c_uint64, C.c_int, C.c_void_p]
+        lib.v3ds_make_pair.restype = C.c_int
+        lib.v3ds_make_pair.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_double,
+                                       C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.v3ds_batch_sizes.restype = None
+        lib.v3ds_batch_sizes.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_double,
+                                         C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib = lib
+    return _lib
+
+
+def seed(base, config, pair, side):
+    return int(_load().v3ds_seed(base, config, pair, side))
+
+
+def fresh(seed_value, n):
+    q = np.empty((n, 128), dtype=np.uint8)
+    _load().v3ds_fresh(C.c_uint64(seed_value), int(n), q.ctypes.data)
+    return q
+
+
+def dequantise(q):
+    """uint8 (n,128) -> float32 (n,128), as SIFT_Feature::normalizeDescriptor."""
+    q = np.ascontiguousarray(q, dtype=np.uint8).reshape(-1, 128)
+    out = np.empty(q.shape, dtype=np.float32)
+    _load().v3ds_dequantise(q.ctypes.data, q.shape[0], out.ctypes.data)
+    return out
+
+
+def make_pair(nL, nR, copy_frac=0.4, config=1, pair=0, base=BASE_SEED, sigma=SIGMA):
+    """One image pair: (qL uint8 (nL,128), qR uint8 (nR,128), truth int32 (nR,))."""
+    qL = np.empty((nL, 128), dtype=np.uint8)
+    qR = np.empty((nR, 128), dtype=np.uint8)
+    truth = np.empty(nR, dtype=np.int32)
+    _load().v3ds_make_pair(C.c_uint64(base), config, pair, int(nL), int(nR), float(copy_frac), float(sigma),
+                           qL.ctypes.data, qR.ctypes.data, truth.ctypes.data)
+    return qL, qR, truth
+
+
+def batch_sizes(n_pairs, lo=500, hi=16000, frac_lo=0.3, frac_hi=0.6, config=3, base=BASE_SEED):
+    """Per-pair (nL, nR, copy_frac) of an uneven batch: n = floor(exp(U(ln lo, ln hi)))."""
+    nL = np.empty(n_pairs, dtype=np.int32)
+    nR = np.empty(n_pairs, dtype=np.int32)
+    fr = np.empty(n_pairs, dtype=np.float64)
+    _load().v3ds_batch_sizes(C.c_uint64(base), config, int(n_pairs), int(lo), int(hi), float(frac_lo),
+                             float(frac_hi), nL.ctypes.data, nR.ctypes.data, fr.ctypes.data)
+    return nL, nR, fr

@@ -1,0 +1,129 @@
+/*
+ * v3dm -- C ABI of the MI355X (gfx950) SIFT putative matcher.
+ *
+ * Drop-in for the CUDA matcher V3D_CUDA::SIFT_Matching (V3D/CUDA/v3d_cudamatching.h:15-67,
+ * V3D/CUDA/v3d_cudamatching.cu:439-694).  One opaque handle per matcher object; every
+ * call is synchronous with respect to the host unless its name ends in _async.  A handle
+ * is not thread-safe; separate handles may be used from separate threads.
+ *
+ * Return value of every int function: V3DM_OK (0) or an error code; the message of the
+ * last error of a handle is returned by v3dm_last_error(handle) (v3dm_last_error(NULL)
+ * for v3dm_create).  The reference prints and exit()s on CUDA errors (cu:12-27); this
+ * library returns the code instead and leaves the process alive.
+ *
+ * Descriptors are 128 floats per feature, row-major (feature k at desc + 128*k), exactly
+ * the layout the reference copies with cublasSetMatrix (cu:503, cu:519).
+ */
+#ifndef V3DM_H
+#define V3DM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct v3dm_context* v3dm_handle;
+
+enum {
+    V3DM_OK = 0,
+    V3DM_ERR_INVALID = 1,      /* bad argument (negative count, null pointer, ...) */
+    V3DM_ERR_STATE = 2,        /* call out of order (e.g. features set before allocate) */
+    V3DM_ERR_CAPACITY = 3,     /* output buffer too small: *nPairs holds the required count */
+    V3DM_ERR_HIP = 4,          /* HIP runtime error */
+    V3DM_ERR_UNSUPPORTED = 5,  /* feature not available in this build / mode */
+    V3DM_ERR_NOMEM = 6         /* device or host allocation failed */
+};
+
+/* Precision modes of findPutativeMatches (library extension; the reference has FP32 only). */
+enum {
+    V3DM_PREC_FP32 = 0,    /* exact match set (certified FP32 MFMA + FP64 re-check) */
+    V3DM_PREC_BF16 = 1,    /* BF16 MFMA shortlist, exact rescoring */
+    V3DM_PREC_FP8 = 2,     /* FP8 e4m3 MFMA shortlist, exact rescoring */
+    V3DM_PREC_LEARNED = 3  /* learned 32-d code shortlist, exact rescoring */
+};
+
+/* Counters of the last findPutativeMatches / guided call (library extension). */
+typedef struct v3dm_stats {
+    int nLeft, nRight;        /* feature counts used (after rounding / clamping) */
+    int nMatches;             /* matches found */
+    int nRechecked;           /* queries whose decision was settled by the exact re-check */
+    int precision;            /* V3DM_PREC_* used */
+    int nFallback;            /* pairs that fell back to the exact path (shortlist modes) */
+    double kernelMs;          /* accumulated time of the dominant kernel since the last reset
+                                 (only when profiling is enabled) */
+    int kernelLaunches;       /* launches accumulated in kernelMs */
+} v3dm_stats;
+
+/* Handle life cycle (constructor / destructor of SIFT_Matching, h:17-21).  device is a HIP
+ * device ordinal. */
+int v3dm_create(v3dm_handle* h, int device);
+void v3dm_destroy(v3dm_handle h);
+const char* v3dm_last_error(v3dm_handle h);
+
+/* SIFT_Matching::allocate(nMaxLeft, nMaxRight, useGuidedMatching), cu:439-474.
+ * Both maxima are rounded down to a multiple of 32 (warning on stderr if enabled). */
+int v3dm_allocate(v3dm_handle h, int nMaxLeft, int nMaxRight, int useGuidedMatching);
+/* SIFT_Matching::deallocate(), cu:476-488. */
+int v3dm_deallocate(v3dm_handle h);
+/* enableRoundingWarning / enableReductionPath, h:23-31.  The reduction path is accepted
+ * and gives the canonical (non-reduction) selection semantics. */
+int v3dm_set_flags(v3dm_handle h, int warnOnRounding, int useReductionPath);
+
+/* setLeftFeatures / setRightFeatures, cu:490-520: n is rounded down to a multiple of 32 and
+ * clamped to the allocated maximum; desc is a host array of n*128 floats. */
+int v3dm_set_left(v3dm_handle h, int n, const float* desc);
+int v3dm_set_right(v3dm_handle h, int n, const float* desc);
+
+/* findPutativeMatches(matches, minScore, minRatio), cu:522-552.  Writes up to cap
+ * (left, right) index pairs to pairs[2*k], pairs[2*k+1], sorted by left index, and the
+ * number of matches to *nPairs.  minRatio < 0 disables the ratio test.  Returns
+ * V3DM_ERR_CAPACITY (with the full count in *nPairs) if cap is too small; cap >=
+ * min(nLeft, nRight) always suffices. */
+int v3dm_find_putative_matches(v3dm_handle h, float minScore, float minRatio, int32_t* pairs, int cap,
+                               int* nPairs);
+
+/* runGuidedMatching (F, Sampson distance) and runGuidedHomographyMatching (H, transfer
+ * distance), cu:554-694.  M is a row-major 3x3 double matrix; positions are AoS (x, y)
+ * floats of the current left / right features.  Candidates outside the distance
+ * threshold get the score -1e20 (cu:308, cu:340).  As in the reference, the mask applies
+ * to the scores of the last findPutativeMatches and repeated guided calls compound.
+ * Requires allocate(..., useGuidedMatching = 1). */
+int v3dm_run_guided_F(v3dm_handle h, const double F[9], float distanceThreshold, const float* leftXY,
+                      const float* rightXY, float minScore, float minRatio, int32_t* pairs, int cap, int* nPairs);
+int v3dm_run_guided_H(v3dm_handle h, const double H[9], float distanceThreshold, const float* leftXY,
+                      const float* rightXY, float minScore, float minRatio, int32_t* pairs, int cap, int* nPairs);
+
+/* ---- extensions (no counterpart in the reference) ---------------------------------- */
+
+/* Select the precision mode (V3DM_PREC_*) of later findPutativeMatches calls. */
+int v3dm_set_precision(v3dm_handle h, int mode);
+/* Run the library's kernels on this HIP stream (hipStream_t, NULL = the handle's own). */
+int v3dm_set_stream(v3dm_handle h, void* hipStream);
+/* Feature upload from device memory (n*128 floats, device pointer); no host copy. */
+int v3dm_set_left_device(v3dm_handle h, int n, const float* d_desc);
+int v3dm_set_right_device(v3dm_handle h, int n, const float* d_desc);
+/* Feature upload as uint8 SIFT descriptors (host array n*128 bytes), dequantised on the
+ * device exactly as SIFT_Feature::normalizeDescriptor (V3D/Base/v3d_feature.h:30-38). */
+int v3dm_set_left_u8(v3dm_handle h, int n, const uint8_t* q);
+int v3dm_set_right_u8(v3dm_handle h, int n, const uint8_t* q);
+/* findPutativeMatches without the host copy: enqueues the work on the stream and returns.
+ * The result stays on the device until v3dm_get_matches. */
+int v3dm_find_putative_matches_async(v3dm_handle h, float minScore, float minRatio);
+/* Waits for the last _async call and copies its matches (as v3dm_find_putative_matches). */
+int v3dm_get_matches(v3dm_handle h, int32_t* pairs, int cap, int* nPairs);
+/* Device pointers of the last result: int2 pairs[], int counters[] (counters[0] = count). */
+int v3dm_device_results(v3dm_handle h, const int32_t** d_pairs, const int32_t** d_counters);
+/* Per-query decisions of the last call (colMax: nLeft ints, rowMax: nRight ints, -1 = none),
+ * the arrays of cu:532-533 after the selection kernels. */
+int v3dm_get_maxima(v3dm_handle h, int32_t* colMax, int32_t* rowMax);
+/* Time the dominant kernel with HIP events on the handle's stream (1 = on, resets sums). */
+int v3dm_set_profiling(v3dm_handle h, int enable);
+int v3dm_get_stats(v3dm_handle h, v3dm_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* V3DM_H */
