@@ -38,6 +38,7 @@ struct v3dm_context {
     bool useGuided = false;
     bool allocated = false;
     int precision = V3DM_PREC_FP32;
+    float epsRel = kEpsRelFp32;  // 0: every query goes to the exact re-check
 
     int nMaxL = 0, nMaxR = 0;  // rounded maxima (cu:444-463)
     int nL = 0, nR = 0;        // current counts (cu:502, cu:518)
@@ -53,6 +54,7 @@ struct v3dm_context {
     int* colMax = nullptr;
     int* rowMax = nullptr;
     int* flagList = nullptr;
+    RecheckState* rstate = nullptr;
     int* results = nullptr;  // counters (kNumCounters ints) followed by int2 pairs
     int* hostStage = nullptr;
     int pairCap = 0;
@@ -122,6 +124,7 @@ void releaseBuffers(v3dm_context* h)
     freeDev(h->colMax);
     freeDev(h->rowMax);
     freeDev(h->flagList);
+    freeDev(h->rstate);
     freeDev(h->results);
     freeDev(h->dQ);
     freeDev(h->dLeftXY);
@@ -189,10 +192,10 @@ int enqueue_plain(v3dm_context* h, float minScore, float minRatio)
             h->evPending += 1;
         }
         launch_merge_decide(h->colPart, p.splits, h->capL, h->rowPart, p.leftBlocks, h->capR, nL, nR, h->normL,
-                            h->normR, h->maxNormBits, kEpsRelFp32, minScore, minRatio, h->colMax, h->rowMax,
-                            h->flagList, h->results, st);
+                            h->normR, h->maxNormBits, h->epsRel, minScore, minRatio, h->colMax, h->rowMax,
+                            h->flagList, h->results, h->rstate, st);
         launch_recheck_exact(h->dL, h->dR, nL, nR, h->flagList, h->results, minScore, minRatio, h->colMax,
-                             h->rowMax, 512, st);
+                             h->rowMax, h->rstate, kRecheckGrid, st);
         launch_mutual_compact(h->colMax, h->rowMax, nL, nR, reinterpret_cast<int2*>(h->results + kNumCounters),
                               h->results, st);
     }
@@ -278,6 +281,7 @@ int v3dm_allocate(v3dm_handle h, int nMaxLeft, int nMaxRight, int useGuidedMatch
     HIPCHECK(h, hipMalloc(&h->colMax, cl * sizeof(int)));
     HIPCHECK(h, hipMalloc(&h->rowMax, cr * sizeof(int)));
     HIPCHECK(h, hipMalloc(&h->flagList, (cl + cr) * sizeof(int)));
+    HIPCHECK(h, hipMalloc(&h->rstate, (cl + cr) * sizeof(RecheckState)));
     h->pairCap = (int)std::min(cl, cr);
     HIPCHECK(h, hipMalloc(&h->results, (kNumCounters + 2 * (size_t)h->pairCap) * sizeof(int)));
     HIPCHECK(h, hipMemsetAsync(h->results, 0, (kNumCounters + 2 * (size_t)h->pairCap) * sizeof(int), h->stream));
@@ -399,6 +403,14 @@ int v3dm_set_precision(v3dm_handle h, int mode)
     return V3DM_OK;
 }
 
+int v3dm_set_certification(v3dm_handle h, float epsRel)
+{
+    if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
+    if (!(epsRel >= 0.0f) || !(epsRel < 1.0f)) return fail(h, V3DM_ERR_INVALID, "epsRel must be in [0, 1)");
+    h->epsRel = epsRel;
+    return V3DM_OK;
+}
+
 int v3dm_set_stream(v3dm_handle h, void* stream)
 {
     if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
@@ -460,6 +472,14 @@ int v3dm_get_maxima(v3dm_handle h, int32_t* colMax, int32_t* rowMax)
     if (rowMax && h->lastNR > 0)
         HIPCHECK(h, hipMemcpyAsync(rowMax, h->rowMax, (size_t)h->lastNR * sizeof(int), hipMemcpyDeviceToHost,
                                    h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return V3DM_OK;
+}
+
+int v3dm_synchronize(v3dm_handle h)
+{
+    if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
+    HIPCHECK(h, hipSetDevice(h->device));
     HIPCHECK(h, hipStreamSynchronize(h->stream));
     return V3DM_OK;
 }

@@ -10,31 +10,6 @@
 
 namespace v3dm {
 
-// ------------------------------------------------------------------------------ norms
-// norm[r] = ||x_r||_2 (FP32), maxNorm = max over rows (as ordered uint bits: norms are
-// non-negative or NaN, and a NaN wins, which later flags every query).
-__global__ __launch_bounds__(256) void k_norms(float const* __restrict__ X, int n, float* __restrict__ norm,
-                                               unsigned int* __restrict__ maxNormBits)
-{
-    int const lane = threadIdx.x & 63;
-    int const row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= n) return;
-    float2 const v = reinterpret_cast<float2 const*>(X + (size_t)row * kDim)[lane];
-    float s = v.x * v.x + v.y * v.y;
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    float const nr = sqrtf(s);
-    if (lane == 0) {
-        norm[row] = nr;
-        atomicMax(maxNormBits, __float_as_uint(nr));
-    }
-}
-
-void launch_norms(float const* X, int n, float* norm, unsigned int* maxNormBits, hipStream_t st)
-{
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_norms, dim3((n + 3) / 4), dim3(256), 0, st, X, n, norm, maxNormBits);
-}
-
 // ------------------------------------------------------------------------------ fused tile kernel
 // Work-group: 4 waves, 128 left features (each wave keeps 32 of them as the B operand of
 // v_mfma_f32_32x32x2_f32 in 64 VGPRs), streaming 64-row tiles of right features through
@@ -191,187 +166,6 @@ void launch_fused_fp32(float const* L, float const* R, int nL, int nR, FusedPlan
     if (nL <= 0 || nR <= 0) return;
     hipLaunchKernelGGL(k_fused_fp32, dim3(p.leftBlocks, p.splits), dim3(256), 0, st, L, R, nL, nR,
                        p.tilesPerSplit, colPart, ldc, rowPart, ldr);
-}
-
-// ------------------------------------------------------------------------------ merge + decide
-__global__ __launch_bounds__(256) void k_merge_decide(Top2 const* __restrict__ colPart, int nColParts, int ldc,
-                                                      Top2 const* __restrict__ rowPart, int nRowParts, int ldr,
-                                                      int nL, int nR, float const* __restrict__ normL,
-                                                      float const* __restrict__ normR,
-                                                      unsigned int const* __restrict__ maxNormBits, float epsRel,
-                                                      float minScore, float minRatio, int* __restrict__ colMax,
-                                                      int* __restrict__ rowMax, int* __restrict__ flagList,
-                                                      int* __restrict__ counters)
-{
-    int const g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nL + nR) return;
-    bool const left = g < nL;
-    int const q = left ? g : g - nL;
-    Top2 const* part = left ? colPart : rowPart;
-    int const np = left ? nColParts : nRowParts;
-    int const ld = left ? ldc : ldr;
-    Top2 s = part[q];
-    for (int p = 1; p < np; ++p) s = top2_merge(s, part[(size_t)p * ld + q]);
-    float const nq = left ? normL[q] : normR[q];
-    float const mo = __uint_as_float(maxNormBits[left ? 1 : 0]);
-    float const eps = epsRel * nq * mo + kEpsAbs;
-    int d = decide_certified(s, eps, minScore, minRatio);
-    if (d == kFlag) {
-        int const slot = atomicAdd(&counters[kCntFlag], 1);
-        flagList[slot] = g;
-    }
-    (left ? colMax : rowMax)[q] = d;
-}
-
-void launch_merge_decide(Top2 const* colPart, int nColParts, int ldc, Top2 const* rowPart, int nRowParts,
-                         int ldr, int nL, int nR, float const* normL, float const* normR,
-                         unsigned int const* maxNormBits, float epsRel, float minScore, float minRatio,
-                         int* colMax, int* rowMax, int* flagList, int* counters, hipStream_t st)
-{
-    int const n = nL + nR;
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_merge_decide, dim3((n + 255) / 256), dim3(256), 0, st, colPart, nColParts, ldc, rowPart,
-                       nRowParts, ldr, nL, nR, normL, normR, maxNormBits, epsRel, minScore, minRatio, colMax,
-                       rowMax, flagList, counters);
-}
-
-// ------------------------------------------------------------------------------ exact re-check
-// One work-group per open query (grid-stride over the list): exact scores
-// fl32(sum_k double(q_k) * double(c_k)) in ascending k against every candidate, top-2
-// with the first-index tie rule, and the decision in IEEE FP32.
-__global__ __launch_bounds__(256) void k_recheck_exact(float const* __restrict__ L, float const* __restrict__ R,
-                                                       int nL, int nR, int const* __restrict__ flagList,
-                                                       int const* __restrict__ counters, float minScore,
-                                                       float minRatio, int* __restrict__ colMax,
-                                                       int* __restrict__ rowMax)
-{
-    __shared__ double qd[kDim];
-    __shared__ Top2 red[4];
-    int const tid = threadIdx.x;
-    int const cnt = counters[kCntFlag];
-    for (int e = blockIdx.x; e < cnt; e += gridDim.x) {
-        int const code = flagList[e];
-        bool const left = code < nL;
-        int const q = left ? code : code - nL;
-        float const* Q = (left ? L : R) + (size_t)q * kDim;
-        float const* C = left ? R : L;
-        int const nC = left ? nR : nL;
-        __syncthreads();
-        if (tid < kDim) qd[tid] = (double)Q[tid];
-        __syncthreads();
-        Top2 s = top2_init();
-        for (int c = tid; c < nC; c += 256) {
-            float4 const* cp = reinterpret_cast<float4 const*>(C + (size_t)c * kDim);
-            double acc = 0.0;
-#pragma unroll 8
-            for (int k4 = 0; k4 < 32; ++k4) {
-                float4 const v = cp[k4];
-                acc += qd[4 * k4 + 0] * (double)v.x;
-                acc += qd[4 * k4 + 1] * (double)v.y;
-                acc += qd[4 * k4 + 2] * (double)v.z;
-                acc += qd[4 * k4 + 3] * (double)v.w;
-            }
-            top2_push(s, (float)acc, c);
-        }
-        for (int o = 32; o > 0; o >>= 1) s = top2_merge(s, top2_shfl_xor(s, o));
-        if ((tid & 63) == 0) red[tid >> 6] = s;
-        __syncthreads();
-        if (tid == 0) {
-            Top2 t = red[0];
-            for (int w = 1; w < 4; ++w) t = top2_merge(t, red[w]);
-            (left ? colMax : rowMax)[q] = decide_exact(t, minScore, minRatio);
-        }
-    }
-}
-
-void launch_recheck_exact(float const* L, float const* R, int nL, int nR, int const* flagList,
-                          int const* counters, float minScore, float minRatio, int* colMax, int* rowMax,
-                          int grid, hipStream_t st)
-{
-    if (nL <= 0 || nR <= 0) return;
-    hipLaunchKernelGGL(k_recheck_exact, dim3(grid), dim3(256), 0, st, L, R, nL, nR, flagList, counters, minScore,
-                       minRatio, colMax, rowMax);
-}
-
-// ------------------------------------------------------------------------------ mutual + compaction
-// One work-group: (k, colMax[k]) is a match iff rowMax[colMax[k]] == k (cu:546-551);
-// matches are written in ascending k with a block-wide prefix count.
-__global__ __launch_bounds__(1024) void k_mutual_compact(int const* __restrict__ colMax,
-                                                         int const* __restrict__ rowMax, int nL, int nR,
-                                                         int2* __restrict__ pairs, int* __restrict__ counters)
-{
-    __shared__ int wsum[16];
-    int const tid = threadIdx.x;
-    int const lane = tid & 63;
-    int const wave = tid >> 6;
-    int base = 0;
-    for (int c0 = 0; c0 < nL; c0 += 1024) {
-        int const i = c0 + tid;
-        int l = -1;
-        bool m = false;
-        if (i < nL) {
-            l = colMax[i];
-            m = (l >= 0) && (l < nR) && (rowMax[l] == i);
-        }
-        unsigned long long const bal = __ballot(m);
-        int const pre = __popcll(bal & ((1ull << lane) - 1ull));
-        if (lane == 0) wsum[wave] = __popcll(bal);
-        __syncthreads();
-        int off = 0, tot = 0;
-#pragma unroll
-        for (int w = 0; w < 16; ++w) {
-            int const x = wsum[w];
-            off += (w < wave) ? x : 0;
-            tot += x;
-        }
-        if (m) pairs[base + off + pre] = make_int2(i, l);
-        base += tot;
-        __syncthreads();
-    }
-    if (tid == 0) counters[kCntPairs] = base;
-}
-
-void launch_mutual_compact(int const* colMax, int const* rowMax, int nL, int nR, int2* pairs, int* counters,
-                           hipStream_t st)
-{
-    hipLaunchKernelGGL(k_mutual_compact, dim3(1), dim3(1024), 0, st, colMax, rowMax, nL, nR, pairs, counters);
-}
-
-// ------------------------------------------------------------------------------ counters reset
-__global__ void k_reset_counters(int* counters, int n)
-{
-    if ((int)threadIdx.x < n) counters[threadIdx.x] = 0;
-}
-
-void launch_reset_counters(int* counters, int n, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, counters, n);
-}
-
-}  // namespace v3dm
-
-namespace v3dm {
-
-// ------------------------------------------------------------------------------ uint8 dequantisation
-// One wave per descriptor, 2 components per lane.  The sum of squares of integers <= 255
-// over 128 terms stays below 2^24, so it is exact in FP32 in any order.
-__global__ __launch_bounds__(256) void k_dequant_u8(uint8_t const* __restrict__ q, int n, float* __restrict__ out)
-{
-    int const lane = threadIdx.x & 63;
-    int const row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= n) return;
-    uchar2 const v = reinterpret_cast<uchar2 const*>(q + (size_t)row * kDim)[lane];
-    float const a = (float)v.x, b = (float)v.y;
-    float s = a * a + b * b;
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    float const denom = __fdiv_rn(1.0f, __fsqrt_rn(s));
-    reinterpret_cast<float2*>(out + (size_t)row * kDim)[lane] = make_float2(a * denom, b * denom);
-}
-
-void launch_dequant_u8(uint8_t const* q, int n, float* out, hipStream_t st)
-{
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_dequant_u8, dim3((n + 3) / 4), dim3(256), 0, st, q, n, out);
 }
 
 }  // namespace v3dm

@@ -6,15 +6,18 @@
 namespace v3dm {
 
 struct Top2;
+struct RecheckState;
 
 constexpr int kMaxSplits = 64;    // right-feature splits of the fused kernel
 constexpr int kCntPairs = 0;      // counters[0]: number of matches
 constexpr int kCntFlag = 1;       // counters[1]: number of queries re-checked exactly
 constexpr int kNumCounters = 4;
 constexpr int kRowPad = 128;      // device descriptor buffers hold a multiple of 128 rows
+constexpr int kRecheckGrid = 1024;
 // |approximate score - exact score| <= epsRel * ||q|| * max||c|| + kEpsAbs for the FP32
 // MFMA path: 128-term FP32 dot products err by at most ~129 * 2^-23 * sum|q_k c_k|, and
-// sum|q_k c_k| <= ||q|| ||c||; epsRel = 2^-15 leaves a factor 2 on top.
+// sum|q_k c_k| <= ||q|| ||c||; epsRel = 2^-15 leaves a factor 2 on top.  epsRel <= 0
+// sends every query to the exact re-check.
 constexpr float kEpsRelFp32 = 3.0517578125e-05f;  // 2^-15
 constexpr float kEpsAbs = 1e-30f;
 
@@ -24,24 +27,25 @@ struct FusedPlan {
     int tilesPerSplit = 0;
 };
 
+// selection stage (v3dm_select.hip)
 void launch_norms(float const* X, int n, float* norm, unsigned int* maxNormBits, hipStream_t st);
-FusedPlan plan_fused_fp32(int nL, int nR, int targetGroups);
-void launch_fused_fp32(float const* L, float const* R, int nL, int nR, FusedPlan const& p, Top2* colPart, int ldc,
-                       Top2* rowPart, int ldr, hipStream_t st);
 void launch_merge_decide(Top2 const* colPart, int nColParts, int ldc, Top2 const* rowPart, int nRowParts, int ldr,
                          int nL, int nR, float const* normL, float const* normR, unsigned int const* maxNormBits,
                          float epsRel, float minScore, float minRatio, int* colMax, int* rowMax, int* flagList,
-                         int* counters, hipStream_t st);
+                         int* counters, RecheckState* rstate, hipStream_t st);
 void launch_recheck_exact(float const* L, float const* R, int nL, int nR, int const* flagList, int const* counters,
-                          float minScore, float minRatio, int* colMax, int* rowMax, int grid, hipStream_t st);
+                          float minScore, float minRatio, int* colMax, int* rowMax, RecheckState* rstate, int grid,
+                          hipStream_t st);
 void launch_mutual_compact(int const* colMax, int const* rowMax, int nL, int nR, int2* pairs, int* counters,
                            hipStream_t st);
 void launch_reset_counters(int* counters, int n, hipStream_t st);
-
-}  // namespace v3dm
-
-namespace v3dm {
 // uint8 SIFT descriptors -> floats, as SIFT_Feature::normalizeDescriptor
 // (V3D/Base/v3d_feature.h:30-38): var = sum q^2, d = q * (1/sqrtf(var)), IEEE rounding.
 void launch_dequant_u8(uint8_t const* q, int n, float* out, hipStream_t st);
+
+// plain FP32 fused score + top-2 kernel (v3dm_fp32.hip)
+FusedPlan plan_fused_fp32(int nL, int nR, int targetGroups);
+void launch_fused_fp32(float const* L, float const* R, int nL, int nR, FusedPlan const& p, Top2* colPart, int ldc,
+                       Top2* rowPart, int ldr, hipStream_t st);
+
 }  // namespace v3dm

@@ -1,0 +1,305 @@
+// Selection stage shared by every precision form: per-feature norms, merging of partial
+// top-2 states, certified decisions, exact FP64 re-check of open queries, mutual check
+// and ordered compaction of the matches.
+//
+// Reference: the selection kernels and host loop of V3D/CUDA/v3d_cudamatching.cu
+// (cu:133-210 column/row top-2 with the ratio test, cu:33-85 without it, cu:546-551 the
+// mutual check).
+#include "v3dm_device.h"
+#include "v3dm_kernels.h"
+
+namespace v3dm {
+
+// ------------------------------------------------------------------------------ norms
+// norm[r] = ||x_r||_2 (FP32) and the maximum over rows.  Norms are >= 0 or NaN, so the
+// max can be taken on the uint bits; a NaN wins, which later flags every query.
+// One work-group handles 64 rows (16 per wave) and issues one atomic.
+__global__ __launch_bounds__(256) void k_norms(float const* __restrict__ X, int n, float* __restrict__ norm,
+                                               unsigned int* __restrict__ maxNormBits)
+{
+    __shared__ unsigned int wmax[4];
+    int const lane = threadIdx.x & 63;
+    int const wave = threadIdx.x >> 6;
+    unsigned int mx = 0;
+    for (int k = 0; k < 16; ++k) {
+        int const row = blockIdx.x * 64 + wave * 16 + k;
+        if (row >= n) break;
+        float2 const v = reinterpret_cast<float2 const*>(X + (size_t)row * kDim)[lane];
+        float s = v.x * v.x + v.y * v.y;
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        float const nr = sqrtf(s);
+        if (lane == 0) norm[row] = nr;
+        mx = max(mx, __float_as_uint(nr));
+    }
+    if (lane == 0) wmax[wave] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int const m = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+        atomicMax(maxNormBits, m);
+    }
+}
+
+void launch_norms(float const* X, int n, float* norm, unsigned int* maxNormBits, hipStream_t st)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_norms, dim3((n + 63) / 64), dim3(256), 0, st, X, n, norm, maxNormBits);
+}
+
+// ------------------------------------------------------------------------------ exact re-check state
+// Per open query, the exact top-2 is accumulated with atomics by the work-groups that
+// score its candidate chunks:
+//   key = ord(best) << 32 | ~argbest   (64-bit max: larger best, then smaller index)
+//   b2  = ord(second)                  (32-bit max)
+// A work-group whose key loses (or which displaces the previous holder) pushes the
+// losing best into b2, so b2 ends as the second largest of the multiset of scores.
+// ord() maps floats to uint32 monotonically.
+__device__ __forceinline__ unsigned int ordf(float f)
+{
+    unsigned int const u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ float unordf(unsigned int o)
+{
+    return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
+__device__ __forceinline__ unsigned long long make_key(float b1, int a1)
+{
+    return ((unsigned long long)ordf(b1) << 32) | (unsigned long long)(~(unsigned int)a1);
+}
+
+__device__ __forceinline__ void recheck_state_init(RecheckState& r)
+{
+    r.key = make_key(kInit, -1);
+    r.b2 = ordf(kInit);
+    r.done = 0;
+}
+
+// ------------------------------------------------------------------------------ merge + decide
+// 8 lanes per query (lane = 8*query + sub); each lane merges every 8th partial, then the
+// 8 lanes combine with xor shuffles.  Queries [0, nL) are left features, [nL, nL+nR)
+// right features.
+__global__ __launch_bounds__(256) void k_merge_decide(Top2 const* __restrict__ colPart, int nColParts, int ldc,
+                                                      Top2 const* __restrict__ rowPart, int nRowParts, int ldr,
+                                                      int nL, int nR, float const* __restrict__ normL,
+                                                      float const* __restrict__ normR,
+                                                      unsigned int const* __restrict__ maxNormBits, float epsRel,
+                                                      float minScore, float minRatio, int* __restrict__ colMax,
+                                                      int* __restrict__ rowMax, int* __restrict__ flagList,
+                                                      int* __restrict__ counters, RecheckState* __restrict__ rstate)
+{
+    int const sub = threadIdx.x & 7;
+    int const g = (int)((blockIdx.x * 256u + threadIdx.x) >> 3);
+    bool const valid = g < nL + nR;
+    int const gq = valid ? g : 0;
+    bool const left = gq < nL;
+    int const q = left ? gq : gq - nL;
+    Top2 const* part = left ? colPart : rowPart;
+    int const np = valid ? (left ? nColParts : nRowParts) : 0;
+    size_t const ld = left ? (size_t)ldc : (size_t)ldr;
+    Top2 s = top2_init();
+#pragma unroll 4
+    for (int p = sub; p < np; p += 8) s = top2_merge(s, part[(size_t)p * ld + q]);
+    s = top2_merge(s, top2_shfl_xor(s, 1));
+    s = top2_merge(s, top2_shfl_xor(s, 2));
+    s = top2_merge(s, top2_shfl_xor(s, 4));
+    if (!valid || sub != 0) return;
+    float const nq = left ? normL[q] : normR[q];
+    float const mo = __uint_as_float(maxNormBits[left ? 1 : 0]);
+    float const eps = epsRel * nq * mo + kEpsAbs;
+    int const d = (epsRel > 0.0f) ? decide_certified(s, eps, minScore, minRatio) : kFlag;
+    if (d == kFlag) {
+        int const slot = atomicAdd(&counters[kCntFlag], 1);
+        flagList[slot] = g;
+        recheck_state_init(rstate[slot]);
+    }
+    (left ? colMax : rowMax)[q] = d;
+}
+
+void launch_merge_decide(Top2 const* colPart, int nColParts, int ldc, Top2 const* rowPart, int nRowParts,
+                         int ldr, int nL, int nR, float const* normL, float const* normR,
+                         unsigned int const* maxNormBits, float epsRel, float minScore, float minRatio,
+                         int* colMax, int* rowMax, int* flagList, int* counters, RecheckState* rstate,
+                         hipStream_t st)
+{
+    int const n = nL + nR;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_merge_decide, dim3((n + 31) / 32), dim3(256), 0, st, colPart, nColParts, ldc, rowPart,
+                       nRowParts, ldr, nL, nR, normL, normR, maxNormBits, epsRel, minScore, minRatio, colMax,
+                       rowMax, flagList, counters, rstate);
+}
+
+// ------------------------------------------------------------------------------ exact re-check
+// Work item = (open query e, chunk c of 256 candidates); grid-stride over
+// counters[kCntFlag] * nChunks items.  Each thread scores one candidate exactly:
+// fl32(sum_k double(q_k) * double(c_k)) in ascending k (products of floats are exact in
+// FP64, so fused multiply-adds give the same sum).  The chunk's top-2 (first index on
+// ties) goes into the query's atomic state; the work-group that completes the last chunk
+// of a query takes the IEEE FP32 decision.
+__global__ __launch_bounds__(256) void k_recheck_exact(float const* __restrict__ L, float const* __restrict__ R,
+                                                       int nL, int nR, int const* __restrict__ flagList,
+                                                       int const* __restrict__ counters, int nChunks,
+                                                       float minScore, float minRatio, int* __restrict__ colMax,
+                                                       int* __restrict__ rowMax, RecheckState* __restrict__ rstate)
+{
+    __shared__ double qd[kDim];
+    __shared__ Top2 red[4];
+    int const tid = threadIdx.x;
+    int const items = counters[kCntFlag] * nChunks;
+    for (int w = blockIdx.x; w < items; w += gridDim.x) {
+        int const e = w / nChunks;
+        int const c = w - e * nChunks;
+        int const code = flagList[e];
+        bool const left = code < nL;
+        int const q = left ? code : code - nL;
+        float const* Q = (left ? L : R) + (size_t)q * kDim;
+        float const* C = left ? R : L;
+        int const nC = left ? nR : nL;
+        __syncthreads();
+        if (tid < kDim) qd[tid] = (double)Q[tid];
+        __syncthreads();
+        Top2 s = top2_init();
+        int const ci = c * 256 + tid;
+        if (ci < nC) {
+            float4 const* cp = reinterpret_cast<float4 const*>(C + (size_t)ci * kDim);
+            double acc = 0.0;
+#pragma unroll 8
+            for (int k4 = 0; k4 < 32; ++k4) {
+                float4 const v = cp[k4];
+                acc += qd[4 * k4 + 0] * (double)v.x;
+                acc += qd[4 * k4 + 1] * (double)v.y;
+                acc += qd[4 * k4 + 2] * (double)v.z;
+                acc += qd[4 * k4 + 3] * (double)v.w;
+            }
+            top2_push(s, (float)acc, ci);
+        }
+        for (int o = 32; o > 0; o >>= 1) s = top2_merge(s, top2_shfl_xor(s, o));
+        if ((tid & 63) == 0) red[tid >> 6] = s;
+        __syncthreads();
+        if (tid == 0) {
+            Top2 t = red[0];
+            for (int wv = 1; wv < 4; ++wv) t = top2_merge(t, red[wv]);
+            RecheckState* rs = rstate + e;
+            unsigned long long const key = make_key(t.b1, t.a1);
+            unsigned long long const old = atomicMax(&rs->key, key);
+            float const lost = (old > key) ? t.b1 : unordf((unsigned int)(old >> 32));
+            atomicMax(&rs->b2, ordf(fmaxf(t.b2, lost)));
+            __threadfence();
+            int const done = atomicAdd(&rs->done, 1);
+            if (done == nChunks - 1) {
+                __threadfence();
+                unsigned long long const fk = atomicMax(&rs->key, 0ull);
+                unsigned int const fb2 = atomicMax(&rs->b2, 0u);
+                Top2 f;
+                f.b1 = unordf((unsigned int)(fk >> 32));
+                f.a1 = (int)(~(unsigned int)(fk & 0xffffffffull));
+                f.b2 = unordf(fb2);
+                (left ? colMax : rowMax)[q] = decide_exact(f, minScore, minRatio);
+            }
+        }
+    }
+}
+
+void launch_recheck_exact(float const* L, float const* R, int nL, int nR, int const* flagList,
+                          int const* counters, float minScore, float minRatio, int* colMax, int* rowMax,
+                          RecheckState* rstate, int grid, hipStream_t st)
+{
+    if (nL <= 0 || nR <= 0) return;
+    int const nChunks = ((nL > nR ? nL : nR) + 255) / 256;
+    hipLaunchKernelGGL(k_recheck_exact, dim3(grid), dim3(256), 0, st, L, R, nL, nR, flagList, counters, nChunks,
+                       minScore, minRatio, colMax, rowMax, rstate);
+}
+
+// ------------------------------------------------------------------------------ mutual + compaction
+// One work-group of 1024 threads, 4 consecutive left features per thread: (k, colMax[k])
+// is a match iff rowMax[colMax[k]] == k (cu:546-551); matches are written in ascending k
+// with a wave-shuffle prefix count.
+__global__ __launch_bounds__(1024) void k_mutual_compact(int const* __restrict__ colMax,
+                                                         int const* __restrict__ rowMax, int nL, int nR,
+                                                         int2* __restrict__ pairs, int* __restrict__ counters)
+{
+    __shared__ int wsum[16];
+    int const tid = threadIdx.x;
+    int const lane = tid & 63;
+    int const wave = tid >> 6;
+    int base = 0;
+    for (int c0 = 0; c0 < nL; c0 += 4096) {
+        int const i0 = c0 + 4 * tid;
+        int l[4];
+        bool m[4];
+        int cnt = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            int const i = i0 + u;
+            l[u] = (i < nL) ? colMax[i] : -1;
+            m[u] = (l[u] >= 0) && (l[u] < nR) && (rowMax[l[u]] == i);
+            cnt += m[u] ? 1 : 0;
+        }
+        int incl = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            int const t = __shfl_up(incl, o);
+            incl += (lane >= o) ? t : 0;
+        }
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        int off = 0, tot = 0;
+#pragma unroll
+        for (int w = 0; w < 16; ++w) {
+            int const x = wsum[w];
+            off += (w < wave) ? x : 0;
+            tot += x;
+        }
+        int pos = base + off + incl - cnt;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (m[u]) pairs[pos++] = make_int2(i0 + u, l[u]);
+        base += tot;
+        __syncthreads();
+    }
+    if (tid == 0) counters[kCntPairs] = base;
+}
+
+void launch_mutual_compact(int const* colMax, int const* rowMax, int nL, int nR, int2* pairs, int* counters,
+                           hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mutual_compact, dim3(1), dim3(1024), 0, st, colMax, rowMax, nL, nR, pairs, counters);
+}
+
+// ------------------------------------------------------------------------------ counters reset
+__global__ void k_reset_counters(int* counters, int n)
+{
+    if ((int)threadIdx.x < n) counters[threadIdx.x] = 0;
+}
+
+void launch_reset_counters(int* counters, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, counters, n);
+}
+
+// ------------------------------------------------------------------------------ uint8 dequantisation
+// One wave per descriptor, 2 components per lane.  The sum of squares of integers <= 255
+// over 128 terms stays below 2^24, so it is exact in FP32 in any order; the reciprocal
+// square root is formed as 1/sqrt with IEEE rounding, as SIFT_Feature::normalizeDescriptor
+// (V3D/Base/v3d_feature.h:30-38) does.
+__global__ __launch_bounds__(256) void k_dequant_u8(uint8_t const* __restrict__ q, int n, float* __restrict__ out)
+{
+    int const lane = threadIdx.x & 63;
+    int const row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    uchar2 const v = reinterpret_cast<uchar2 const*>(q + (size_t)row * kDim)[lane];
+    float const a = (float)v.x, b = (float)v.y;
+    float s = a * a + b * b;
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    float const denom = __fdiv_rn(1.0f, __fsqrt_rn(s));
+    reinterpret_cast<float2*>(out + (size_t)row * kDim)[lane] = make_float2(a * denom, b * denom);
+}
+
+void launch_dequant_u8(uint8_t const* q, int n, float* out, hipStream_t st)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_dequant_u8, dim3((n + 3) / 4), dim3(256), 0, st, q, n, out);
+}
+
+}  // namespace v3dm

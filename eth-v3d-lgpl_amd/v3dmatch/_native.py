@@ -53,6 +53,7 @@ SIGNATURES = {
     "v3dm_run_guided_F": (_I, [_P, _FP, _F, _FP, _FP, _F, _F, _FP, _I, _IP]),
     "v3dm_run_guided_H": (_I, [_P, _FP, _F, _FP, _FP, _F, _F, _FP, _I, _IP]),
     "v3dm_set_precision": (_I, [_P, _I]),
+    "v3dm_set_certification": (_I, [_P, _F]),
     "v3dm_set_stream": (_I, [_P, _P]),
     "v3dm_set_left_device": (_I, [_P, _I, _FP]),
     "v3dm_set_right_device": (_I, [_P, _I, _FP]),
@@ -62,6 +63,7 @@ SIGNATURES = {
     "v3dm_get_matches": (_I, [_P, _FP, _I, _IP]),
     "v3dm_device_results": (_I, [_P, C.POINTER(_P), C.POINTER(_P)]),
     "v3dm_get_maxima": (_I, [_P, _FP, _FP]),
+    "v3dm_synchronize": (_I, [_P]),
     "v3dm_set_profiling": (_I, [_P, _I]),
     "v3dm_get_stats": (_I, [_P, C.POINTER(Stats)]),
 }

@@ -148,6 +148,10 @@ class SIFT_Matching:
     def set_precision(self, mode):
         self._check(self._lib.v3dm_set_precision(self._h, int(mode)))
 
+    def set_certification(self, eps_rel):
+        """Error-bound scale of certified decisions (0 = every query re-checked exactly)."""
+        self._check(self._lib.v3dm_set_certification(self._h, C.c_float(eps_rel)))
+
     def set_stream(self, stream_ptr):
         self._check(self._lib.v3dm_set_stream(self._h, C.c_void_p(stream_ptr) if stream_ptr else None))
 
@@ -180,6 +184,9 @@ class SIFT_Matching:
         rm = np.empty(max(self._nR, 1), dtype=np.int32)
         self._check(self._lib.v3dm_get_maxima(self._h, cm.ctypes.data, rm.ctypes.data))
         return cm[: self._nL], rm[: self._nR]
+
+    def synchronize(self):
+        self._check(self._lib.v3dm_synchronize(self._h))
 
     def set_profiling(self, enable=True):
         self._check(self._lib.v3dm_set_profiling(self._h, int(bool(enable))))

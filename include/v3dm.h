@@ -99,6 +99,11 @@ int v3dm_run_guided_H(v3dm_handle h, const double H[9], float distanceThreshold,
 
 /* Select the precision mode (V3DM_PREC_*) of later findPutativeMatches calls. */
 int v3dm_set_precision(v3dm_handle h, int mode);
+/* Error-bound scale of the certified FP32 decisions: a query is settled from the MFMA
+ * scores only when every exact score set within epsRel * ||q|| * max||c|| gives the same
+ * decision; otherwise it is re-checked exactly in FP64.  Default 2^-15; 0 re-checks every
+ * query (the whole selection then runs on exact FP64 scores). */
+int v3dm_set_certification(v3dm_handle h, float epsRel);
 /* Run the library's kernels on this HIP stream (hipStream_t, NULL = the handle's own). */
 int v3dm_set_stream(v3dm_handle h, void* hipStream);
 /* Feature upload from device memory (n*128 floats, device pointer); no host copy. */
@@ -118,6 +123,8 @@ int v3dm_device_results(v3dm_handle h, const int32_t** d_pairs, const int32_t** 
 /* Per-query decisions of the last call (colMax: nLeft ints, rowMax: nRight ints, -1 = none),
  * the arrays of cu:532-533 after the selection kernels. */
 int v3dm_get_maxima(v3dm_handle h, int32_t* colMax, int32_t* rowMax);
+/* Block until all work the handle enqueued on its stream has finished. */
+int v3dm_synchronize(v3dm_handle h);
 /* Time the dominant kernel with HIP events on the handle's stream (1 = on, resets sums). */
 int v3dm_set_profiling(v3dm_handle h, int enable);
 int v3dm_get_stats(v3dm_handle h, v3dm_stats* out);

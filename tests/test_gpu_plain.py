@@ -112,3 +112,17 @@ def test_c2_full_size(matcher):
     L, R, _ = _pair(16384, 16384, pair=0, config=2)
     got = _check(matcher, L, R, 16384, 16384)
     say(f"  C2 matches={len(got)} rechecked={matcher.stats()['nRechecked']}")
+
+
+@pytest.mark.parametrize("nL,nR,seed", [(512, 480, 51), (2048, 1024, 52), (4096, 4096, 53)])
+def test_every_query_rechecked_exactly(matcher, nL, nR, seed):
+    """Certification off: every decision comes from the exact FP64 re-check kernel."""
+    say(f"case recheck-all nL={nL} nR={nR}")
+    L, R, _ = _pair(nL, nR, pair=seed)
+    matcher.set_certification(0.0)
+    try:
+        _check(matcher, L, R, nL, nR)
+        _check(matcher, L, R, nL, nR, 0.9, -1.0)
+        assert matcher.stats()["nRechecked"] == nL + nR
+    finally:
+        matcher.set_certification(2.0 ** -15)
