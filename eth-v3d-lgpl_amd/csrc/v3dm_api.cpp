@@ -17,6 +17,7 @@
 #include "v3dm_device.h"
 #include "v3dm_kernels.h"
 #include "v3dm_ext.h"
+#include "v3dm_context.h"
 
 using namespace v3dm;
 
@@ -28,60 +29,6 @@ int roundUp(int n, int m) { return (n + m - 1) / m * m; }
 
 }  // namespace
 
-struct v3dm_context {
-    int device = 0;
-    hipStream_t ownStream = nullptr;
-    hipStream_t stream = nullptr;
-
-    bool warnOnRounding = true;     // h:18
-    bool useReductionPath = false;  // h:19
-    bool useGuided = false;
-    bool allocated = false;
-    int precision = V3DM_PREC_FP32;
-    float epsRel = kEpsRelFp32;  // 0: every query goes to the exact re-check
-
-    int nMaxL = 0, nMaxR = 0;  // rounded maxima (cu:444-463)
-    int nL = 0, nR = 0;        // current counts (cu:502, cu:518)
-    int capL = 0, capR = 0;    // allocated rows (multiple of kRowPad)
-
-    float* dL = nullptr;
-    float* dR = nullptr;
-    float* normL = nullptr;
-    float* normR = nullptr;
-    unsigned int* maxNormBits = nullptr;  // [0] = left, [1] = right
-    Top2* colPart = nullptr;
-    Top2* rowPart = nullptr;
-    int* colMax = nullptr;
-    int* rowMax = nullptr;
-    int* flagList = nullptr;
-    RecheckState* rstate = nullptr;
-    int* results = nullptr;  // counters (kNumCounters ints) followed by int2 pairs
-    int* hostStage = nullptr;
-    int pairCap = 0;
-    uint8_t* dQ = nullptr;  // uint8 upload staging (capL or capR rows)
-    size_t dQBytes = 0;
-
-    // guided matching state: masks applied since the last findPutativeMatches
-    std::vector<GuidedMask> masks;
-    float* dLeftXY = nullptr;
-    float* dRightXY = nullptr;
-
-    // last result
-    bool haveResult = false;
-    v3dm_stats stats{};
-    int lastNL = 0, lastNR = 0;
-
-    // profiling of the dominant kernel
-    bool profiling = false;
-    std::vector<hipEvent_t> evStart, evStop;
-    int evPending = 0;
-    double kernelMs = 0.0;
-    int kernelLaunches = 0;
-
-    ExtState* ext = nullptr;
-
-    std::string err;
-};
 
 namespace {
 
@@ -124,7 +71,9 @@ void releaseBuffers(v3dm_context* h)
     freeDev(h->colMax);
     freeDev(h->rowMax);
     freeDev(h->flagList);
-    freeDev(h->rstate);
+    freeDev(h->rpart);
+    freeDev(h->mtmp);
+    freeDev(h->mblock);
     freeDev(h->results);
     freeDev(h->dQ);
     freeDev(h->dLeftXY);
@@ -177,33 +126,45 @@ int ctx_fail(v3dm_context* h, int code, const char* msg) { return fail(h, code, 
 
 // Enqueue the plain FP32 exact path for the current features; leaves colMax/rowMax
 // decided and the compacted pairs + counters in h->results.
+int prof_begin(v3dm_context* h)
+{
+    if (!h->profiling) return V3DM_OK;
+    if (h->evPending == (int)h->evStart.size()) {
+        int const rc = harvestEvents(h);
+        if (rc) return rc;
+    }
+    HIPCHECK(h, hipEventRecord(h->evStart[h->evPending], h->stream));
+    return V3DM_OK;
+}
+
+int prof_end(v3dm_context* h)
+{
+    if (!h->profiling) return V3DM_OK;
+    HIPCHECK(h, hipEventRecord(h->evStop[h->evPending], h->stream));
+    h->evPending += 1;
+    return V3DM_OK;
+}
+
 int enqueue_plain(v3dm_context* h, float minScore, float minRatio)
 {
     hipStream_t const st = h->stream;
     int const nL = h->nL, nR = h->nR;
-    launch_reset_counters(h->results, kNumCounters, st);
-    if (nL > 0 && nR > 0) {
+    bool const work = nL > 0 && nR > 0;
+    int rc;
+    if (work) {
         FusedPlan const p = plan_fused_fp32(nL, nR, 512);
-        int const ev = h->evPending;
-        if (h->profiling) HIPCHECK(h, hipEventRecord(h->evStart[ev], st));
+        if ((rc = prof_begin(h))) return rc;
         launch_fused_fp32(h->dL, h->dR, nL, nR, p, h->colPart, h->capL, h->rowPart, h->capR, st);
-        if (h->profiling) {
-            HIPCHECK(h, hipEventRecord(h->evStop[ev], st));
-            h->evPending += 1;
-        }
+        if ((rc = prof_end(h))) return rc;
         launch_merge_decide(h->colPart, p.splits, h->capL, h->rowPart, p.leftBlocks, h->capR, nL, nR, h->normL,
                             h->normR, h->maxNormBits, h->epsRel, minScore, minRatio, h->colMax, h->rowMax,
-                            h->flagList, h->results, h->rstate, st);
+                            h->flagList, h->results, st);
         launch_recheck_exact(h->dL, h->dR, nL, nR, h->flagList, h->results, minScore, minRatio, h->colMax,
-                             h->rowMax, h->rstate, kRecheckGrid, st);
-        launch_mutual_compact(h->colMax, h->rowMax, nL, nR, reinterpret_cast<int2*>(h->results + kNumCounters),
-                              h->results, st);
+                             h->rowMax, h->rpart, kRecheckGrid, st);
     }
+    launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mtmp, h->mblock,
+                          reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st);
     HIPCHECK(h, hipGetLastError());
-    if (h->profiling && h->evPending == (int)h->evStart.size()) {
-        int const rc = harvestEvents(h);
-        if (rc) return rc;
-    }
     return V3DM_OK;
 }
 
@@ -281,7 +242,9 @@ int v3dm_allocate(v3dm_handle h, int nMaxLeft, int nMaxRight, int useGuidedMatch
     HIPCHECK(h, hipMalloc(&h->colMax, cl * sizeof(int)));
     HIPCHECK(h, hipMalloc(&h->rowMax, cr * sizeof(int)));
     HIPCHECK(h, hipMalloc(&h->flagList, (cl + cr) * sizeof(int)));
-    HIPCHECK(h, hipMalloc(&h->rstate, (cl + cr) * sizeof(RecheckState)));
+    HIPCHECK(h, hipMalloc(&h->rpart, (cl + cr) * kRecheckMaxChunks * sizeof(Top2)));
+    HIPCHECK(h, hipMalloc(&h->mtmp, cl * sizeof(int2)));
+    HIPCHECK(h, hipMalloc(&h->mblock, (cl / 1024 + 2) * sizeof(int)));
     h->pairCap = (int)std::min(cl, cr);
     HIPCHECK(h, hipMalloc(&h->results, (kNumCounters + 2 * (size_t)h->pairCap) * sizeof(int)));
     HIPCHECK(h, hipMemsetAsync(h->results, 0, (kNumCounters + 2 * (size_t)h->pairCap) * sizeof(int), h->stream));
@@ -371,7 +334,8 @@ int copyResults(v3dm_context* h, int32_t* pairs, int cap, int* nPairs)
                                    (size_t)n * 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHECK(h, hipStreamSynchronize(h->stream));
     h->stats.nMatches = n;
-    h->stats.nRechecked = h->hostStage[kCntFlag];
+    h->stats.nRechecked = h->hostStage[kCntFlagDone];
+    h->stats.nRescored = h->hostStage[kCntRescoredDone];
     if (nPairs) *nPairs = n;
     int const ncopy = std::min(n, std::max(cap, 0));
     if (ncopy > 0) {
@@ -517,7 +481,8 @@ int v3dm_get_stats(v3dm_handle h, v3dm_stats* out)
         HIPCHECK(h, hipMemcpyAsync(cnt, h->results, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
         HIPCHECK(h, hipStreamSynchronize(h->stream));
         h->stats.nMatches = cnt[kCntPairs];
-        h->stats.nRechecked = cnt[kCntFlag];
+        h->stats.nRechecked = cnt[kCntFlagDone];
+        h->stats.nRescored = cnt[kCntRescoredDone];
     }
     h->stats.nFallback = ext_fallbacks(h);
     *out = h->stats;

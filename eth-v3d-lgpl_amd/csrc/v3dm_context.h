@@ -1,0 +1,68 @@
+// Internal state of one matcher handle (shared by the C ABI and the precision modes).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/v3dm.h"
+#include "v3dm_device.h"
+#include "v3dm_ext.h"
+#include "v3dm_kernels.h"
+
+struct v3dm_context {
+    int device = 0;
+    hipStream_t ownStream = nullptr;
+    hipStream_t stream = nullptr;
+
+    bool warnOnRounding = true;     // h:18
+    bool useReductionPath = false;  // h:19
+    bool useGuided = false;
+    bool allocated = false;
+    int precision = V3DM_PREC_FP32;
+    float epsRel = v3dm::kEpsRelFp32;  // 0: every query goes to the exact re-check
+
+    int nMaxL = 0, nMaxR = 0;  // rounded maxima (cu:444-463)
+    int nL = 0, nR = 0;        // current counts (cu:502, cu:518)
+    int capL = 0, capR = 0;    // allocated rows (multiple of kRowPad)
+
+    float* dL = nullptr;
+    float* dR = nullptr;
+    float* normL = nullptr;
+    float* normR = nullptr;
+    unsigned int* maxNormBits = nullptr;  // [0] = left, [1] = right
+    v3dm::Top2* colPart = nullptr;
+    v3dm::Top2* rowPart = nullptr;
+    int* colMax = nullptr;
+    int* rowMax = nullptr;
+    int* flagList = nullptr;
+    v3dm::Top2* rpart = nullptr;       // re-check chunk partials: (capL + capR) x kRecheckMaxChunks
+    int2* mtmp = nullptr;        // mutual-check block slices: capL
+    int* mblock = nullptr;       // mutual-check block counts
+    int* results = nullptr;  // counters (kNumCounters ints) followed by int2 pairs
+    int* hostStage = nullptr;
+    int pairCap = 0;
+    uint8_t* dQ = nullptr;  // uint8 upload staging (capL or capR rows)
+    size_t dQBytes = 0;
+
+    // guided matching state: masks applied since the last findPutativeMatches
+    std::vector<v3dm::GuidedMask> masks;
+    float* dLeftXY = nullptr;
+    float* dRightXY = nullptr;
+
+    // last result
+    bool haveResult = false;
+    v3dm_stats stats{};
+    int lastNL = 0, lastNR = 0;
+
+    // profiling of the dominant kernel
+    bool profiling = false;
+    std::vector<hipEvent_t> evStart, evStop;
+    int evPending = 0;
+    double kernelMs = 0.0;
+    int kernelLaunches = 0;
+
+    v3dm::ExtState* ext = nullptr;
+
+    std::string err;
+};

@@ -17,12 +17,6 @@ struct Top2 {
 
 __device__ __forceinline__ Top2 top2_init() { return Top2{kInit, kInit, -1}; }
 
-// Exact top-2 of one open query, accumulated with atomics by the re-check work-groups.
-struct RecheckState {
-    unsigned long long key;  // ord(best) << 32 | ~argbest
-    unsigned int b2;         // ord(second best)
-    int done;                // candidate chunks finished
-};
 
 // One candidate, the update rule of cu:145-147.  NaN scores are skipped.
 __device__ __forceinline__ void top2_push(Top2& s, float v, int idx)

@@ -1,17 +1,86 @@
-// Extension hooks: precision modes other than the exact FP32 path.
+// Precision modes other than the exact FP32 path: the BF16 shortlist form.
 #include "v3dm_ext.h"
+
+#include "v3dm_context.h"
+#include "v3dm_short.h"
 
 namespace v3dm {
 
 struct ExtState {
+    int capL = 0, capR = 0;  // rows of the bf16 buffers (multiple of kShortRowPad)
+    __bf16* bL = nullptr;
+    __bf16* bR = nullptr;
+    bool bLValid = false, bRValid = false;
+    Cand2* partL = nullptr;  // kShortMaxSplits x capL x 2
+    Cand2* partR = nullptr;
     int fallbacks = 0;
 };
 
-bool ext_supports(v3dm_context*, int mode) { return mode == V3DM_PREC_FP32; }
+namespace {
 
-int ext_enqueue(v3dm_context* h, int mode, float, float)
+int roundUpTo(int n, int m) { return (n + m - 1) / m * m; }
+
+#define EXTCHECK(h, call)                                                    \
+    do {                                                                     \
+        hipError_t const e_ = (call);                                        \
+        if (e_ != hipSuccess) return ctx_fail((h), V3DM_ERR_HIP, #call);     \
+    } while (0)
+
+int ensure(v3dm_context* h, ExtState*& e)
 {
-    (void)mode;
+    if (!e) e = new ExtState();
+    int const cl = roundUpTo(h->capL, kShortRowPad), cr = roundUpTo(h->capR, kShortRowPad);
+    if (e->bL && e->capL == cl && e->capR == cr) return V3DM_OK;
+    ext_release(e);
+    e->capL = cl;
+    e->capR = cr;
+    EXTCHECK(h, hipMalloc(&e->bL, (size_t)cl * kDim * sizeof(__bf16)));
+    EXTCHECK(h, hipMalloc(&e->bR, (size_t)cr * kDim * sizeof(__bf16)));
+    EXTCHECK(h, hipMemsetAsync(e->bL, 0, (size_t)cl * kDim * sizeof(__bf16), h->stream));
+    EXTCHECK(h, hipMemsetAsync(e->bR, 0, (size_t)cr * kDim * sizeof(__bf16), h->stream));
+    EXTCHECK(h, hipMalloc(&e->partL, (size_t)kShortMaxSplits * cl * 2 * sizeof(Cand2)));
+    EXTCHECK(h, hipMalloc(&e->partR, (size_t)kShortMaxSplits * cr * 2 * sizeof(Cand2)));
+    e->bLValid = e->bRValid = false;
+    return V3DM_OK;
+}
+
+int enqueue_bf16(v3dm_context* h, float minScore, float minRatio)
+{
+    ExtState*& e = ctx_ext(h);
+    int rc = ensure(h, e);
+    if (rc) return rc;
+    hipStream_t const st = h->stream;
+    int const nL = h->nL, nR = h->nR;
+    bool const work = nL > 0 && nR > 0;
+    if (work) {
+        if (!e->bLValid) launch_to_bf16(h->dL, nL, e->bL, st);
+        if (!e->bRValid) launch_to_bf16(h->dR, nR, e->bR, st);
+        e->bLValid = e->bRValid = true;
+        ShortPlan const pl = plan_short(nL, nR, 512);
+        ShortPlan const pr = plan_short(nR, nL, 512);
+        if ((rc = prof_begin(h))) return rc;
+        launch_pass_bf16(e->bL, nL, e->bR, nR, pl, e->partL, e->capL, st);
+        if ((rc = prof_end(h))) return rc;
+        if ((rc = prof_begin(h))) return rc;
+        launch_pass_bf16(e->bR, nR, e->bL, nL, pr, e->partR, e->capR, st);
+        if ((rc = prof_end(h))) return rc;
+        launch_rescore(h->dL, h->dR, nL, nR, e->partL, pl.splits, e->capL, e->partR, pr.splits, e->capR, h->normL,
+                       h->normR, h->maxNormBits, kWinRelBf16, minScore, minRatio, h->colMax, h->rowMax, h->results,
+                       st);
+    }
+    launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mtmp, h->mblock,
+                          reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st);
+    EXTCHECK(h, hipGetLastError());
+    return V3DM_OK;
+}
+
+}  // namespace
+
+bool ext_supports(v3dm_context*, int mode) { return mode == V3DM_PREC_FP32 || mode == V3DM_PREC_BF16; }
+
+int ext_enqueue(v3dm_context* h, int mode, float minScore, float minRatio)
+{
+    if (mode == V3DM_PREC_BF16) return enqueue_bf16(h, minScore, minRatio);
     return ctx_fail(h, V3DM_ERR_UNSUPPORTED, "precision mode not available");
 }
 
@@ -21,12 +90,30 @@ int ext_fallbacks(v3dm_context* h)
     return e ? e->fallbacks : 0;
 }
 
-void ext_features_changed(v3dm_context*, bool) {}
+void ext_features_changed(v3dm_context* h, bool left)
+{
+    ExtState* e = ctx_ext(h);
+    if (!e) return;
+    if (left) e->bLValid = false;
+    else e->bRValid = false;
+}
 
-void ext_release(ExtState*) {}
+void ext_release(ExtState* e)
+{
+    if (!e) return;
+    if (e->bL) (void)hipFree(e->bL);
+    if (e->bR) (void)hipFree(e->bR);
+    if (e->partL) (void)hipFree(e->partL);
+    if (e->partR) (void)hipFree(e->partR);
+    e->bL = e->bR = nullptr;
+    e->partL = e->partR = nullptr;
+    e->capL = e->capR = 0;
+    e->bLValid = e->bRValid = false;
+}
 
 void ext_destroy(ExtState*& e)
 {
+    ext_release(e);
     delete e;
     e = nullptr;
 }

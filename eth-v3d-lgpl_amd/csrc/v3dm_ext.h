@@ -22,6 +22,9 @@ hipStream_t ctx_stream(v3dm_context* h);
 ExtState*& ctx_ext(v3dm_context* h);
 int ctx_fail(v3dm_context* h, int code, const char* msg);
 int enqueue_plain(v3dm_context* h, float minScore, float minRatio);
+// HIP events around the dominant kernel(s) of a call, when profiling is enabled
+int prof_begin(v3dm_context* h);
+int prof_end(v3dm_context* h);
 
 // extension hooks (v3dm_ext.cpp)
 bool ext_supports(v3dm_context* h, int mode);

@@ -6,12 +6,15 @@
 namespace v3dm {
 
 struct Top2;
-struct RecheckState;
 
 constexpr int kMaxSplits = 64;    // right-feature splits of the fused kernel
 constexpr int kCntPairs = 0;      // counters[0]: number of matches
-constexpr int kCntFlag = 1;       // counters[1]: number of queries re-checked exactly
-constexpr int kNumCounters = 4;
+constexpr int kCntFlag = 1;       // counters[1]: open queries of the running call (cleared at its end)
+constexpr int kCntFlagDone = 2;   // counters[2]: queries re-checked exactly by the last call
+constexpr int kCntRescored = 3;   // counters[3]: shortlist candidates rescored by the running call
+constexpr int kCntRescoredDone = 4;  // counters[4]: ... by the last call
+constexpr int kNumCounters = 8;
+constexpr int kRecheckMaxChunks = 64;  // candidate chunks per re-checked query
 constexpr int kRowPad = 128;      // device descriptor buffers hold a multiple of 128 rows
 constexpr int kRecheckGrid = 1024;
 // |approximate score - exact score| <= epsRel * ||q|| * max||c|| + kEpsAbs for the FP32
@@ -20,6 +23,11 @@ constexpr int kRecheckGrid = 1024;
 // sends every query to the exact re-check.
 constexpr float kEpsRelFp32 = 3.0517578125e-05f;  // 2^-15
 constexpr float kEpsAbs = 1e-30f;
+
+struct RecheckPlan {
+    int nChunks = 1;
+    int per = 1;  // candidates per thread of one chunk
+};
 
 struct FusedPlan {
     int leftBlocks = 0;
@@ -32,13 +40,14 @@ void launch_norms(float const* X, int n, float* norm, unsigned int* maxNormBits,
 void launch_merge_decide(Top2 const* colPart, int nColParts, int ldc, Top2 const* rowPart, int nRowParts, int ldr,
                          int nL, int nR, float const* normL, float const* normR, unsigned int const* maxNormBits,
                          float epsRel, float minScore, float minRatio, int* colMax, int* rowMax, int* flagList,
-                         int* counters, RecheckState* rstate, hipStream_t st);
+                         int* counters, hipStream_t st);
+RecheckPlan plan_recheck(int nL, int nR);
 void launch_recheck_exact(float const* L, float const* R, int nL, int nR, int const* flagList, int const* counters,
-                          float minScore, float minRatio, int* colMax, int* rowMax, RecheckState* rstate, int grid,
+                          float minScore, float minRatio, int* colMax, int* rowMax, Top2* part, int grid,
                           hipStream_t st);
-void launch_mutual_compact(int const* colMax, int const* rowMax, int nL, int nR, int2* pairs, int* counters,
-                           hipStream_t st);
-void launch_reset_counters(int* counters, int n, hipStream_t st);
+// nL = 0 publishes an empty result.  tmp: nL int2, blockCount: ceil(nL/1024) ints.
+void launch_mutual_compact(int const* colMax, int const* rowMax, int nL, int nR, int2* tmp, int* blockCount,
+                           int2* pairs, int* counters, hipStream_t st);
 // uint8 SIFT descriptors -> floats, as SIFT_Feature::normalizeDescriptor
 // (V3D/Base/v3d_feature.h:30-38): var = sum q^2, d = q * (1/sqrtf(var)), IEEE rounding.
 void launch_dequant_u8(uint8_t const* q, int n, float* out, hipStream_t st);

@@ -45,37 +45,6 @@ void launch_norms(float const* X, int n, float* norm, unsigned int* maxNormBits,
     hipLaunchKernelGGL(k_norms, dim3((n + 63) / 64), dim3(256), 0, st, X, n, norm, maxNormBits);
 }
 
-// ------------------------------------------------------------------------------ exact re-check state
-// Per open query, the exact top-2 is accumulated with atomics by the work-groups that
-// score its candidate chunks:
-//   key = ord(best) << 32 | ~argbest   (64-bit max: larger best, then smaller index)
-//   b2  = ord(second)                  (32-bit max)
-// A work-group whose key loses (or which displaces the previous holder) pushes the
-// losing best into b2, so b2 ends as the second largest of the multiset of scores.
-// ord() maps floats to uint32 monotonically.
-__device__ __forceinline__ unsigned int ordf(float f)
-{
-    unsigned int const u = __float_as_uint(f);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-
-__device__ __forceinline__ float unordf(unsigned int o)
-{
-    return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
-}
-
-__device__ __forceinline__ unsigned long long make_key(float b1, int a1)
-{
-    return ((unsigned long long)ordf(b1) << 32) | (unsigned long long)(~(unsigned int)a1);
-}
-
-__device__ __forceinline__ void recheck_state_init(RecheckState& r)
-{
-    r.key = make_key(kInit, -1);
-    r.b2 = ordf(kInit);
-    r.done = 0;
-}
-
 // ------------------------------------------------------------------------------ merge + decide
 // 8 lanes per query (lane = 8*query + sub); each lane merges every 8th partial, then the
 // 8 lanes combine with xor shuffles.  Queries [0, nL) are left features, [nL, nL+nR)
@@ -87,7 +56,7 @@ __global__ __launch_bounds__(256) void k_merge_decide(Top2 const* __restrict__ c
                                                       unsigned int const* __restrict__ maxNormBits, float epsRel,
                                                       float minScore, float minRatio, int* __restrict__ colMax,
                                                       int* __restrict__ rowMax, int* __restrict__ flagList,
-                                                      int* __restrict__ counters, RecheckState* __restrict__ rstate)
+                                                      int* __restrict__ counters)
 {
     int const sub = threadIdx.x & 7;
     int const g = (int)((blockIdx.x * 256u + threadIdx.x) >> 3);
@@ -112,7 +81,6 @@ __global__ __launch_bounds__(256) void k_merge_decide(Top2 const* __restrict__ c
     if (d == kFlag) {
         int const slot = atomicAdd(&counters[kCntFlag], 1);
         flagList[slot] = g;
-        recheck_state_init(rstate[slot]);
     }
     (left ? colMax : rowMax)[q] = d;
 }
@@ -120,28 +88,27 @@ __global__ __launch_bounds__(256) void k_merge_decide(Top2 const* __restrict__ c
 void launch_merge_decide(Top2 const* colPart, int nColParts, int ldc, Top2 const* rowPart, int nRowParts,
                          int ldr, int nL, int nR, float const* normL, float const* normR,
                          unsigned int const* maxNormBits, float epsRel, float minScore, float minRatio,
-                         int* colMax, int* rowMax, int* flagList, int* counters, RecheckState* rstate,
-                         hipStream_t st)
+                         int* colMax, int* rowMax, int* flagList, int* counters, hipStream_t st)
 {
     int const n = nL + nR;
     if (n <= 0) return;
     hipLaunchKernelGGL(k_merge_decide, dim3((n + 31) / 32), dim3(256), 0, st, colPart, nColParts, ldc, rowPart,
                        nRowParts, ldr, nL, nR, normL, normR, maxNormBits, epsRel, minScore, minRatio, colMax,
-                       rowMax, flagList, counters, rstate);
+                       rowMax, flagList, counters);
 }
 
 // ------------------------------------------------------------------------------ exact re-check
-// Work item = (open query e, chunk c of 256 candidates); grid-stride over
-// counters[kCntFlag] * nChunks items.  Each thread scores one candidate exactly:
+// Pass 1: work item = (open query e, chunk c of up to 256*per candidates); grid-stride
+// over counters[kCntFlag] * nChunks items.  Each thread scores its candidates exactly:
 // fl32(sum_k double(q_k) * double(c_k)) in ascending k (products of floats are exact in
-// FP64, so fused multiply-adds give the same sum).  The chunk's top-2 (first index on
-// ties) goes into the query's atomic state; the work-group that completes the last chunk
-// of a query takes the IEEE FP32 decision.
-__global__ __launch_bounds__(256) void k_recheck_exact(float const* __restrict__ L, float const* __restrict__ R,
-                                                       int nL, int nR, int const* __restrict__ flagList,
-                                                       int const* __restrict__ counters, int nChunks,
-                                                       float minScore, float minRatio, int* __restrict__ colMax,
-                                                       int* __restrict__ rowMax, RecheckState* __restrict__ rstate)
+// FP64, so fused multiply-adds give the same sum), 4 independent chains at a time.  The
+// chunk's top-2 (first index on ties) goes to part[e * nChunks + c].
+// Pass 2: one wave per open query merges its chunk partials and takes the IEEE FP32
+// decision.  The kernel boundary between the passes orders the partial writes.
+__global__ __launch_bounds__(256) void k_recheck_chunks(float const* __restrict__ L, float const* __restrict__ R,
+                                                        int nL, int nR, int const* __restrict__ flagList,
+                                                        int const* __restrict__ counters, int nChunks, int per,
+                                                        Top2* __restrict__ part)
 {
     __shared__ double qd[kDim];
     __shared__ Top2 red[4];
@@ -160,19 +127,31 @@ __global__ __launch_bounds__(256) void k_recheck_exact(float const* __restrict__
         if (tid < kDim) qd[tid] = (double)Q[tid];
         __syncthreads();
         Top2 s = top2_init();
-        int const ci = c * 256 + tid;
-        if (ci < nC) {
-            float4 const* cp = reinterpret_cast<float4 const*>(C + (size_t)ci * kDim);
-            double acc = 0.0;
-#pragma unroll 8
-            for (int k4 = 0; k4 < 32; ++k4) {
-                float4 const v = cp[k4];
-                acc += qd[4 * k4 + 0] * (double)v.x;
-                acc += qd[4 * k4 + 1] * (double)v.y;
-                acc += qd[4 * k4 + 2] * (double)v.z;
-                acc += qd[4 * k4 + 3] * (double)v.w;
+        int const cbeg = c * 256 * per;
+        for (int u0 = 0; u0 < per; u0 += 4) {
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+            float4 const* cp[4];
+            bool ok[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                int const ci = cbeg + (u0 + u) * 256 + tid;
+                ok[u] = (u0 + u < per) && (ci < nC);
+                cp[u] = reinterpret_cast<float4 const*>(C + (size_t)(ok[u] ? ci : 0) * kDim);
             }
-            top2_push(s, (float)acc, ci);
+#pragma unroll 4
+            for (int k4 = 0; k4 < 32; ++k4) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    float4 const v = cp[u][k4];
+                    acc[u] += qd[4 * k4 + 0] * (double)v.x;
+                    acc[u] += qd[4 * k4 + 1] * (double)v.y;
+                    acc[u] += qd[4 * k4 + 2] * (double)v.z;
+                    acc[u] += qd[4 * k4 + 3] * (double)v.w;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (ok[u]) top2_push(s, (float)acc[u], cbeg + (u0 + u) * 256 + tid);
         }
         for (int o = 32; o > 0; o >>= 1) s = top2_merge(s, top2_shfl_xor(s, o));
         if ((tid & 63) == 0) red[tid >> 6] = s;
@@ -180,102 +159,132 @@ __global__ __launch_bounds__(256) void k_recheck_exact(float const* __restrict__
         if (tid == 0) {
             Top2 t = red[0];
             for (int wv = 1; wv < 4; ++wv) t = top2_merge(t, red[wv]);
-            RecheckState* rs = rstate + e;
-            unsigned long long const key = make_key(t.b1, t.a1);
-            unsigned long long const old = atomicMax(&rs->key, key);
-            float const lost = (old > key) ? t.b1 : unordf((unsigned int)(old >> 32));
-            atomicMax(&rs->b2, ordf(fmaxf(t.b2, lost)));
-            __threadfence();
-            int const done = atomicAdd(&rs->done, 1);
-            if (done == nChunks - 1) {
-                __threadfence();
-                unsigned long long const fk = atomicMax(&rs->key, 0ull);
-                unsigned int const fb2 = atomicMax(&rs->b2, 0u);
-                Top2 f;
-                f.b1 = unordf((unsigned int)(fk >> 32));
-                f.a1 = (int)(~(unsigned int)(fk & 0xffffffffull));
-                f.b2 = unordf(fb2);
-                (left ? colMax : rowMax)[q] = decide_exact(f, minScore, minRatio);
-            }
+            part[(size_t)e * nChunks + c] = t;
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_recheck_finish(int nL, int const* __restrict__ flagList,
+                                                        int const* __restrict__ counters, int nChunks,
+                                                        Top2 const* __restrict__ part, float minScore,
+                                                        float minRatio, int* __restrict__ colMax,
+                                                        int* __restrict__ rowMax)
+{
+    int const lane = threadIdx.x & 63;
+    int const cnt = counters[kCntFlag];
+    for (int e = blockIdx.x * 4 + (threadIdx.x >> 6); e < cnt; e += gridDim.x * 4) {
+        Top2 s = top2_init();
+        for (int c = lane; c < nChunks; c += 64) s = top2_merge(s, part[(size_t)e * nChunks + c]);
+        for (int o = 32; o > 0; o >>= 1) s = top2_merge(s, top2_shfl_xor(s, o));
+        if (lane == 0) {
+            int const code = flagList[e];
+            bool const left = code < nL;
+            (left ? colMax : rowMax)[left ? code : code - nL] = decide_exact(s, minScore, minRatio);
+        }
+    }
+}
+
+RecheckPlan plan_recheck(int nL, int nR)
+{
+    RecheckPlan p;
+    int const n = nL > nR ? nL : nR;
+    p.per = (n + 256 * kRecheckMaxChunks - 1) / (256 * kRecheckMaxChunks);
+    if (p.per < 1) p.per = 1;
+    p.nChunks = (n + 256 * p.per - 1) / (256 * p.per);
+    if (p.nChunks < 1) p.nChunks = 1;
+    return p;
 }
 
 void launch_recheck_exact(float const* L, float const* R, int nL, int nR, int const* flagList,
                           int const* counters, float minScore, float minRatio, int* colMax, int* rowMax,
-                          RecheckState* rstate, int grid, hipStream_t st)
+                          Top2* part, int grid, hipStream_t st)
 {
     if (nL <= 0 || nR <= 0) return;
-    int const nChunks = ((nL > nR ? nL : nR) + 255) / 256;
-    hipLaunchKernelGGL(k_recheck_exact, dim3(grid), dim3(256), 0, st, L, R, nL, nR, flagList, counters, nChunks,
-                       minScore, minRatio, colMax, rowMax, rstate);
+    RecheckPlan const p = plan_recheck(nL, nR);
+    hipLaunchKernelGGL(k_recheck_chunks, dim3(grid), dim3(256), 0, st, L, R, nL, nR, flagList, counters, p.nChunks,
+                       p.per, part);
+    hipLaunchKernelGGL(k_recheck_finish, dim3(256), dim3(256), 0, st, nL, flagList, counters, p.nChunks, part,
+                       minScore, minRatio, colMax, rowMax);
 }
 
 // ------------------------------------------------------------------------------ mutual + compaction
-// One work-group of 1024 threads, 4 consecutive left features per thread: (k, colMax[k])
-// is a match iff rowMax[colMax[k]] == k (cu:546-551); matches are written in ascending k
-// with a wave-shuffle prefix count.
-__global__ __launch_bounds__(1024) void k_mutual_compact(int const* __restrict__ colMax,
-                                                         int const* __restrict__ rowMax, int nL, int nR,
-                                                         int2* __restrict__ pairs, int* __restrict__ counters)
+// (k, colMax[k]) is a match iff rowMax[colMax[k]] == k (cu:546-551).  Pass 1: each
+// work-group takes 1024 consecutive left features (4 per thread), compacts its matches in
+// ascending k into its own slice of tmp and writes its count.  Pass 2: one work-group
+// forms the exclusive prefix of the block counts and copies the slices into place, so
+// the output is sorted by k.  Pass 2 also publishes the re-check count and clears the
+// counter for the next call.
+__global__ __launch_bounds__(256) void k_mutual_local(int const* __restrict__ colMax, int const* __restrict__ rowMax,
+                                                      int nL, int nR, int2* __restrict__ tmp,
+                                                      int* __restrict__ blockCount)
 {
-    __shared__ int wsum[16];
+    __shared__ int wsum[4];
     int const tid = threadIdx.x;
     int const lane = tid & 63;
     int const wave = tid >> 6;
-    int base = 0;
-    for (int c0 = 0; c0 < nL; c0 += 4096) {
-        int const i0 = c0 + 4 * tid;
-        int l[4];
-        bool m[4];
-        int cnt = 0;
+    int const i0 = blockIdx.x * 1024 + 4 * tid;
+    int l[4];
+    bool m[4];
+    int cnt = 0;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            int const i = i0 + u;
-            l[u] = (i < nL) ? colMax[i] : -1;
-            m[u] = (l[u] >= 0) && (l[u] < nR) && (rowMax[l[u]] == i);
-            cnt += m[u] ? 1 : 0;
-        }
-        int incl = cnt;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            int const t = __shfl_up(incl, o);
-            incl += (lane >= o) ? t : 0;
-        }
-        if (lane == 63) wsum[wave] = incl;
-        __syncthreads();
-        int off = 0, tot = 0;
-#pragma unroll
-        for (int w = 0; w < 16; ++w) {
-            int const x = wsum[w];
-            off += (w < wave) ? x : 0;
-            tot += x;
-        }
-        int pos = base + off + incl - cnt;
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (m[u]) pairs[pos++] = make_int2(i0 + u, l[u]);
-        base += tot;
-        __syncthreads();
+    for (int u = 0; u < 4; ++u) {
+        int const i = i0 + u;
+        l[u] = (i < nL) ? colMax[i] : -1;
+        m[u] = (l[u] >= 0) && (l[u] < nR) && (rowMax[l[u]] == i);
+        cnt += m[u] ? 1 : 0;
     }
-    if (tid == 0) counters[kCntPairs] = base;
+    int incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int const t = __shfl_up(incl, o);
+        incl += (lane >= o) ? t : 0;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    int off = 0;
+    for (int w = 0; w < wave; ++w) off += wsum[w];
+    int pos = blockIdx.x * 1024 + off + incl - cnt;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+        if (m[u]) tmp[pos++] = make_int2(i0 + u, l[u]);
+    if (tid == 0) blockCount[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
-void launch_mutual_compact(int const* colMax, int const* rowMax, int nL, int nR, int2* pairs, int* counters,
-                           hipStream_t st)
+__global__ __launch_bounds__(1024) void k_mutual_gather(int2 const* __restrict__ tmp,
+                                                        int const* __restrict__ blockCount, int nBlocks,
+                                                        int2* __restrict__ pairs, int* __restrict__ counters)
 {
-    hipLaunchKernelGGL(k_mutual_compact, dim3(1), dim3(1024), 0, st, colMax, rowMax, nL, nR, pairs, counters);
+    __shared__ int offs[1025];
+    int const tid = threadIdx.x;
+    if (tid == 0) {
+        int a = 0;
+        for (int b = 0; b < nBlocks; ++b) {
+            offs[b] = a;
+            a += blockCount[b];
+        }
+        offs[nBlocks] = a;
+    }
+    __syncthreads();
+    for (int b = 0; b < nBlocks; ++b) {
+        int const n = offs[b + 1] - offs[b];
+        for (int t = tid; t < n; t += 1024) pairs[offs[b] + t] = tmp[(size_t)b * 1024 + t];
+    }
+    if (tid == 0) {
+        counters[kCntPairs] = offs[nBlocks];
+        counters[kCntFlagDone] = counters[kCntFlag];
+        counters[kCntFlag] = 0;
+        counters[kCntRescoredDone] = counters[kCntRescored];
+        counters[kCntRescored] = 0;
+    }
 }
 
-// ------------------------------------------------------------------------------ counters reset
-__global__ void k_reset_counters(int* counters, int n)
+void launch_mutual_compact(int const* colMax, int const* rowMax, int nL, int nR, int2* tmp, int* blockCount,
+                           int2* pairs, int* counters, hipStream_t st)
 {
-    if ((int)threadIdx.x < n) counters[threadIdx.x] = 0;
-}
-
-void launch_reset_counters(int* counters, int n, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, counters, n);
+    int const nb = (nL + 1023) / 1024;
+    if (nb > 0)
+        hipLaunchKernelGGL(k_mutual_local, dim3(nb), dim3(256), 0, st, colMax, rowMax, nL, nR, tmp, blockCount);
+    hipLaunchKernelGGL(k_mutual_gather, dim3(1), dim3(1024), 0, st, tmp, blockCount, nb, pairs, counters);
 }
 
 // ------------------------------------------------------------------------------ uint8 dequantisation

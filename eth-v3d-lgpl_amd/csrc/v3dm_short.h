@@ -1,0 +1,38 @@
+// Shortlist forms: BF16 one-direction passes and exact rescoring.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace v3dm {
+
+// Two best (approximate score, candidate index) of one query in one partition.
+struct Cand2 {
+    float v1;
+    int j1;
+    float v2;
+    int j2;
+};
+
+struct ShortPlan {
+    int queryBlocks = 0;
+    int splits = 0;
+    int tilesPerSplit = 0;
+};
+
+constexpr int kShortMaxSplits = 16;
+constexpr int kShortRowPad = 256;  // bf16 descriptor buffers hold a multiple of 256 rows
+// Per-score error bound of the BF16 scores: operands rounded to bf16 err by <= 2^-9
+// relative, so |s_bf16 - s| <= (2^-8 + 2^-16) * ||q|| * ||c|| including FP32 accumulation
+// and the 6 truncated key bits.  Candidates within twice the bound of the approximate
+// second best are rescored.
+constexpr float kWinRelBf16 = 2.0f * 1.01f * 0.00390625f;
+
+void launch_to_bf16(float const* X, int rows, __bf16* Y, hipStream_t st);
+ShortPlan plan_short(int nQ, int nC, int targetGroups);
+void launch_pass_bf16(__bf16 const* Q, int nQ, __bf16 const* C, int nC, ShortPlan const& p, Cand2* part, int ldq,
+                      hipStream_t st);
+void launch_rescore(float const* L, float const* R, int nL, int nR, Cand2 const* partL, int nPartL, int ldqL,
+                    Cand2 const* partR, int nPartR, int ldqR, float const* normL, float const* normR,
+                    unsigned int const* maxNormBits, float winRel, float minScore, float minRatio, int* colMax,
+                    int* rowMax, int* counters, hipStream_t st);
+
+}  // namespace v3dm

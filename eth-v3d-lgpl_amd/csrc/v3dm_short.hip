@@ -1,0 +1,306 @@
+// Reduced-precision shortlist forms of the putative matcher (BF16 MFMA).
+//
+// The reference computes every score in FP32 (cublasSgemm, V3D/CUDA/v3d_cudamatching.cu:
+// 529-530).  Here each direction is scanned once with v_mfma_f32_32x32x16_bf16 and FP32
+// accumulation; every query keeps, per partition of its candidates, the two best
+// approximate scores with their indices.  The union of the partition lists is the
+// shortlist; the candidates whose approximate score lies within twice the BF16 error bound
+// of the approximate second best are rescored exactly (FP64 products, the oracle's
+// score), and the reference's selection rule (cu:133-210) runs on those exact scores.
+#include "v3dm_device.h"
+#include "v3dm_kernels.h"
+#include "v3dm_short.h"
+
+namespace v3dm {
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+// ------------------------------------------------------------------------------ FP32 -> BF16
+// Round to nearest even; 4 components per thread.
+__global__ __launch_bounds__(256) void k_to_bf16(float const* __restrict__ X, int nElems, __bf16* __restrict__ Y)
+{
+    int const i = (blockIdx.x * 256 + threadIdx.x) * 4;
+    if (i >= nElems) return;
+    float4 const v = *reinterpret_cast<float4 const*>(X + i);
+    Y[i + 0] = (__bf16)v.x;
+    Y[i + 1] = (__bf16)v.y;
+    Y[i + 2] = (__bf16)v.z;
+    Y[i + 3] = (__bf16)v.w;
+}
+
+void launch_to_bf16(float const* X, int rows, __bf16* Y, hipStream_t st)
+{
+    int const n = rows * kDim;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_to_bf16, dim3((n / 4 + 255) / 256), dim3(256), 0, st, X, n, Y);
+}
+
+// ------------------------------------------------------------------------------ one-direction pass
+// Work-group: 4 waves; each wave keeps 64 queries (two 32-query blocks) as B operands
+// (lane l: query l%32 of the block, bf16 k = h*64 + 8*s + v for MFMA step s, h = l/32),
+// so the work-group covers 256 queries.  Candidates stream through double-buffered LDS
+// in tiles of 64 (two 32-row blocks, A operand with the same k map).  Lane l then holds,
+// per 32x32 block, the scores of query l%32 against candidate rows
+// jl = 8*(r/4) + 4*h + r%4, r = 0..15.
+// Per query and lane half the running top-2 is kept on keys: the FP32 score with its
+// 6 low mantissa bits replaced by the position (block*16 + r) inside the tile, so that a
+// max / med3 pair updates value and position together; the tile of each of the two keys
+// is tracked once per tile.  Masked candidates score -1e30.
+constexpr int kSQ = 256;          // queries per work-group
+constexpr int kST = 64;           // candidates per tile
+constexpr int kSRow = 136;        // bf16 per LDS row: 128 + 8 (conflict-free b128 reads)
+
+__device__ __forceinline__ float pack_key(float v, int pos)
+{
+    return __uint_as_float((__float_as_uint(v) & ~63u) | (unsigned int)pos);
+}
+
+struct KeyState {
+    float k1, k2;
+    int t1, t2;
+};
+
+__device__ __forceinline__ void key_tile_update(KeyState& s, float o1, float o2, int t)
+{
+    bool const c1 = s.k1 != o1;
+    bool const c2 = s.k2 != o2;
+    int const nt2 = (c1 && s.k2 == o1) ? s.t1 : (c2 ? t : s.t2);
+    s.t1 = c1 ? t : s.t1;
+    s.t2 = nt2;
+}
+
+__global__ __launch_bounds__(256, 2) void k_pass_bf16(__bf16 const* __restrict__ Q, int nQ,
+                                                      __bf16 const* __restrict__ C, int nC, int tilesPerSplit,
+                                                      Cand2* __restrict__ part, int ldq)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 lds[2][kST * kSRow];
+
+    int const tid = threadIdx.x;
+    int const lane = tid & 63;
+    int const wave = tid >> 6;
+    int const h = lane >> 5;
+    int const c32 = lane & 31;
+    int const q0 = blockIdx.x * kSQ + wave * 64;
+    int const split = blockIdx.y;
+    int const nTiles = (nC + kST - 1) / kST;
+    int const t0 = split * tilesPerSplit;
+    int const t1 = min(t0 + tilesPerSplit, nTiles);
+
+    v8bf bq[2][8];
+#pragma unroll
+    for (int qb = 0; qb < 2; ++qb) {
+        uint4 const* qp = reinterpret_cast<uint4 const*>(Q + (size_t)(q0 + qb * 32 + c32) * kDim + h * 64);
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            uint4 const u = qp[s];
+            bq[qb][s] = *reinterpret_cast<v8bf const*>(&u);
+        }
+    }
+
+    uint4 const* C16 = reinterpret_cast<uint4 const*>(C);  // 16 uint4 per candidate row
+    if (t0 < t1) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            int const idx = p * 256 + tid;
+            int const row = idx >> 4, c8 = idx & 15;
+            *reinterpret_cast<uint4*>(&lds[0][row * kSRow + c8 * 8]) = C16[(size_t)(t0 * kST + row) * 16 + c8];
+        }
+    }
+    __syncthreads();
+
+    KeyState st[2];
+#pragma unroll
+    for (int qb = 0; qb < 2; ++qb) st[qb] = KeyState{-INFINITY, -INFINITY, -1, -1};
+
+    for (int t = t0; t < t1; ++t) {
+        int const buf = (t - t0) & 1;
+        bool const hasNext = (t + 1) < t1;
+        uint4 pre[4];
+        if (hasNext) {
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                int const idx = p * 256 + tid;
+                int const row = idx >> 4, c8 = idx & 15;
+                pre[p] = C16[(size_t)((t + 1) * kST + row) * 16 + c8];
+            }
+        }
+
+        v16f acc[2][2];
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) {
+            acc[cb][0] = v16f{};
+            acc[cb][1] = v16f{};
+            __bf16 const* A = &lds[buf][(cb * 32 + c32) * kSRow + h * 64];
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                v8bf const a = *reinterpret_cast<v8bf const*>(A + 8 * s);
+                acc[cb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bq[0][s], acc[cb][0], 0, 0, 0);
+                acc[cb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bq[1][s], acc[cb][1], 0, 0, 0);
+            }
+        }
+
+        int const jt = t * kST;
+#pragma unroll
+        for (int qb = 0; qb < 2; ++qb) {
+            float const o1 = st[qb].k1, o2 = st[qb].k2;
+            float k1 = o1, k2 = o2;
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    int const jl = cb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+                    float const v = (jt + jl < nC) ? acc[cb][qb][r] : -1e30f;
+                    float const key = pack_key(v, cb * 16 + r);
+                    k2 = __builtin_amdgcn_fmed3f(k1, k2, key);
+                    k1 = fmaxf(k1, key);
+                }
+            }
+            st[qb].k1 = k1;
+            st[qb].k2 = k2;
+            key_tile_update(st[qb], o1, o2, t);
+        }
+
+        if (hasNext) {
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                int const idx = p * 256 + tid;
+                int const row = idx >> 4, c8 = idx & 15;
+                *reinterpret_cast<uint4*>(&lds[buf ^ 1][row * kSRow + c8 * 8]) = pre[p];
+            }
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int qb = 0; qb < 2; ++qb) {
+        int const q = q0 + qb * 32 + c32;
+        if (q >= nQ) continue;
+        Cand2 c;
+        int const p1 = (int)(__float_as_uint(st[qb].k1) & 63u);
+        int const p2 = (int)(__float_as_uint(st[qb].k2) & 63u);
+        c.v1 = st[qb].k1;
+        c.v2 = st[qb].k2;
+        c.j1 = (st[qb].t1 >= 0 && c.v1 > -1e29f)
+                   ? st[qb].t1 * kST + (p1 >> 4) * 32 + 8 * ((p1 & 15) >> 2) + 4 * h + (p1 & 3) : -1;
+        c.j2 = (st[qb].t2 >= 0 && c.v2 > -1e29f)
+                   ? st[qb].t2 * kST + (p2 >> 4) * 32 + 8 * ((p2 & 15) >> 2) + 4 * h + (p2 & 3) : -1;
+        part[((size_t)split * ldq + q) * 2 + h] = c;
+    }
+}
+
+ShortPlan plan_short(int nQ, int nC, int targetGroups)
+{
+    ShortPlan p;
+    p.queryBlocks = (nQ + kSQ - 1) / kSQ;
+    int const nTiles = (nC + kST - 1) / kST;
+    int splits = (targetGroups + p.queryBlocks - 1) / p.queryBlocks;
+    if (splits > kShortMaxSplits) splits = kShortMaxSplits;
+    if (splits > nTiles) splits = nTiles;
+    if (splits < 1) splits = 1;
+    p.tilesPerSplit = (nTiles + splits - 1) / splits;
+    p.splits = (nTiles + p.tilesPerSplit - 1) / p.tilesPerSplit;
+    return p;
+}
+
+void launch_pass_bf16(__bf16 const* Q, int nQ, __bf16 const* C, int nC, ShortPlan const& p, Cand2* part, int ldq,
+                      hipStream_t st)
+{
+    if (nQ <= 0 || nC <= 0) return;
+    hipLaunchKernelGGL(k_pass_bf16, dim3(p.queryBlocks, p.splits), dim3(256), 0, st, Q, nQ, C, nC, p.tilesPerSplit,
+                       part, ldq);
+}
+
+// ------------------------------------------------------------------------------ exact rescoring
+// 8 lanes per query.  Lane `sub` owns the partition lists e = sub, sub+8, ... of the
+// query (2 entries each).  The approximate second best v2 of the union sets the window
+// v >= v2 - win (win = twice the per-score error bound); every listed candidate inside it
+// is rescored exactly (ascending-k FP64 sum of exact products, rounded to FP32) and the
+// exact top-2 (first index on ties) is merged over the 8 lanes.
+__global__ __launch_bounds__(256) void k_rescore(float const* __restrict__ L, float const* __restrict__ R, int nL,
+                                                 int nR, Cand2 const* __restrict__ partL, int nPartL, int ldqL,
+                                                 Cand2 const* __restrict__ partR, int nPartR, int ldqR,
+                                                 float const* __restrict__ normL, float const* __restrict__ normR,
+                                                 unsigned int const* __restrict__ maxNormBits, float winRel,
+                                                 float minScore, float minRatio, int* __restrict__ colMax,
+                                                 int* __restrict__ rowMax, int* __restrict__ counters)
+{
+    int const sub = threadIdx.x & 7;
+    int const g = (int)((blockIdx.x * 256u + threadIdx.x) >> 3);
+    bool const valid = g < nL + nR;
+    int const gq = valid ? g : 0;
+    bool const left = gq < nL;
+    int const q = left ? gq : gq - nL;
+    Cand2 const* part = left ? partL : partR;
+    int const nEnt = valid ? 2 * (left ? nPartL : nPartR) : 0;  // (split, half) lists
+    size_t const ldq = left ? (size_t)ldqL : (size_t)ldqR;
+    float const* Qd = (left ? L : R) + (size_t)q * kDim;
+    float const* Cd = left ? R : L;
+
+    // approximate top-2 values of the union
+    float a1 = -INFINITY, a2 = -INFINITY;
+    for (int e = sub; e < nEnt; e += 8) {
+        Cand2 const c = part[((size_t)(e >> 1) * ldq + q) * 2 + (e & 1)];
+        a2 = __builtin_amdgcn_fmed3f(a1, a2, c.v1);
+        a1 = fmaxf(a1, c.v1);
+        a2 = __builtin_amdgcn_fmed3f(a1, a2, c.v2);
+        a1 = fmaxf(a1, c.v2);
+    }
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) {
+        float const b1 = __shfl_xor(a1, o), b2 = __shfl_xor(a2, o);
+        a2 = fmaxf(fminf(a1, b1), fmaxf(a2, b2));
+        a1 = fmaxf(a1, b1);
+    }
+    float const nq = left ? normL[q] : normR[q];
+    float const mo = __uint_as_float(maxNormBits[left ? 1 : 0]);
+    float const thr = a2 - winRel * nq * mo;
+
+    Top2 s = top2_init();
+    int nres = 0;
+    for (int e = sub; e < nEnt; e += 8) {
+        Cand2 const c = part[((size_t)(e >> 1) * ldq + q) * 2 + (e & 1)];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            int const j = u ? c.j2 : c.j1;
+            float const v = u ? c.v2 : c.v1;
+            if (j < 0 || !(v >= thr)) continue;
+            float4 const* qp = reinterpret_cast<float4 const*>(Qd);
+            float4 const* cp = reinterpret_cast<float4 const*>(Cd + (size_t)j * kDim);
+            double acc = 0.0;
+#pragma unroll 8
+            for (int k4 = 0; k4 < 32; ++k4) {
+                float4 const x = qp[k4];
+                float4 const y = cp[k4];
+                acc += (double)x.x * (double)y.x;
+                acc += (double)x.y * (double)y.y;
+                acc += (double)x.z * (double)y.z;
+                acc += (double)x.w * (double)y.w;
+            }
+            float const sc = (float)acc;
+            if (sc == sc) s = top2_merge(s, Top2{sc, kInit, j});
+            ++nres;
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) {
+        s = top2_merge(s, top2_shfl_xor(s, o));
+        nres += __shfl_xor(nres, o);
+    }
+    if (!valid || sub != 0) return;
+    (left ? colMax : rowMax)[q] = decide_exact(s, minScore, minRatio);
+    atomicAdd(&counters[kCntRescored], nres);
+}
+
+void launch_rescore(float const* L, float const* R, int nL, int nR, Cand2 const* partL, int nPartL, int ldqL,
+                    Cand2 const* partR, int nPartR, int ldqR, float const* normL, float const* normR,
+                    unsigned int const* maxNormBits, float winRel, float minScore, float minRatio, int* colMax,
+                    int* rowMax, int* counters, hipStream_t st)
+{
+    int const n = nL + nR;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_rescore, dim3((n + 31) / 32), dim3(256), 0, st, L, R, nL, nR, partL, nPartL, ldqL, partR,
+                       nPartR, ldqR, normL, normR, maxNormBits, winRel, minScore, minRatio, colMax, rowMax,
+                       counters);
+}
+
+}  // namespace v3dm

@@ -24,7 +24,7 @@ class Stats(C.Structure):
     _fields_ = [
         ("nLeft", C.c_int), ("nRight", C.c_int), ("nMatches", C.c_int), ("nRechecked", C.c_int),
         ("precision", C.c_int), ("nFallback", C.c_int), ("kernelMs", C.c_double),
-        ("kernelLaunches", C.c_int),
+        ("kernelLaunches", C.c_int), ("nRescored", C.c_int),
     ]
 
 

@@ -54,6 +54,7 @@ typedef struct v3dm_stats {
     double kernelMs;          /* accumulated time of the dominant kernel since the last reset
                                  (only when profiling is enabled) */
     int kernelLaunches;       /* launches accumulated in kernelMs */
+    int nRescored;            /* shortlist candidates rescored exactly (shortlist modes) */
 } v3dm_stats;
 
 /* Handle life cycle (constructor / destructor of SIFT_Matching, h:17-21).  device is a HIP
