@@ -158,9 +158,9 @@ int enqueue_plain(v3dm_context* h, float minScore, float minRatio)
         if ((rc = prof_end(h))) return rc;
         launch_merge_decide(h->colPart, p.splits, h->capL, h->rowPart, p.leftBlocks, h->capR, nL, nR, h->normL,
                             h->normR, h->maxNormBits, h->epsRel, minScore, minRatio, h->colMax, h->rowMax,
-                            h->flagList, h->results, st);
-        launch_recheck_exact(h->dL, h->dR, nL, nR, h->flagList, h->results, minScore, minRatio, h->colMax,
-                             h->rowMax, h->rpart, kRecheckGrid, st);
+                            h->flagList, h->capL, h->results, st);
+        launch_recheck_exact(h->dL, h->dR, nL, nR, h->flagList, h->capL, h->results, minScore, minRatio,
+                             h->colMax, h->rowMax, h->rpart, kRecheckGrid, st);
     }
     launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mtmp, h->mblock,
                           reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st);

@@ -27,6 +27,22 @@ __device__ __forceinline__ void top2_push(Top2& s, float v, int idx)
     s.b1 = fmaxf(s.b1, v);
 }
 
+// max / second-largest without the operand canonicalisation fmaxf needs for NaN:
+// v_med3_f32(a, b, +inf) = max(a, b) and, for b1 >= b2, med3(b1, b2, v) is the new
+// second best.  `inf` must be +infinity supplied at run time (a kernel argument): with a
+// literal the compiler folds med3 back into a canonicalising max.  Used on approximate
+// scores only; a NaN score there comes from a non-finite descriptor, whose error bound is
+// non-finite, so its queries are re-checked.
+__device__ __forceinline__ float fast_max(float a, float b, float inf) { return __builtin_amdgcn_fmed3f(a, b, inf); }
+
+__device__ __forceinline__ void top2_push_fast(Top2& s, float v, int idx, float inf)
+{
+    bool const gt = v > s.b1;
+    s.a1 = gt ? idx : s.a1;
+    s.b2 = __builtin_amdgcn_fmed3f(s.b1, s.b2, v);
+    s.b1 = fast_max(s.b1, v, inf);
+}
+
 // Order-independent merge of two partial states: larger best wins, an equal best keeps
 // the smaller index (the first index of an ascending scan), the second best is the
 // second largest of the union.  Commutative and associative.

@@ -28,7 +28,7 @@ constexpr int kStride = 132;  // LDS floats per tile row: 128 + 4 (conflict-free
 __global__ __launch_bounds__(256, 2) void k_fused_fp32(float const* __restrict__ L, float const* __restrict__ R,
                                                        int nL, int nR, int tilesPerSplit,
                                                        Top2* __restrict__ colPart, int ldc,
-                                                       Top2* __restrict__ rowPart, int ldr)
+                                                       Top2* __restrict__ rowPart, int ldr, float inf)
 {
     __shared__ __attribute__((aligned(16))) float lds[2][kBJ * kStride];
 
@@ -118,24 +118,43 @@ __global__ __launch_bounds__(256, 2) void k_fused_fp32(float const* __restrict__
 
         int const jt = t * kBJ;
         float* S = lds[buf];
+        if (jt + kBJ <= nR) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                int const jl = 8 * (r >> 2) + 4 * h + (r & 3);
+                top2_push_fast(colS, acc0[r], jt + jl, inf);
+                top2_push_fast(colS, acc1[r], jt + jl + 32, inf);
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                int const jl = 8 * (r >> 2) + 4 * h + (r & 3);
+                top2_push_fast(colS, (jt + jl < nR) ? acc0[r] : -INFINITY, jt + jl, inf);
+                top2_push_fast(colS, (jt + jl + 32 < nR) ? acc1[r] : -INFINITY, jt + jl + 32, inf);
+            }
+        }
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             int const jl = 8 * (r >> 2) + 4 * h + (r & 3);
-            float const v0 = acc0[r];
-            float const v1 = acc1[r];
-            top2_push(colS, (jt + jl < nR) ? v0 : -INFINITY, jt + jl);
-            top2_push(colS, (jt + jl + 32 < nR) ? v1 : -INFINITY, jt + jl + 32);
-            S[jl * kStride + wave * 32 + c32] = v0;
-            S[(jl + 32) * kStride + wave * 32 + c32] = v1;
+            S[jl * kStride + wave * 32 + c32] = acc0[r];
+            S[(jl + 32) * kStride + wave * 32 + c32] = acc1[r];
         }
         __syncthreads();
 
         Top2 rs = top2_init();
+        if (i0 + kBI <= nL) {
 #pragma unroll
-        for (int c = 0; c < 32; ++c) {
-            int const il = c * 4 + rq;
-            float const v = S[rj * kStride + il];
-            top2_push(rs, (i0 + il < nL) ? v : -INFINITY, i0 + il);
+            for (int c = 0; c < 32; ++c) {
+                int const il = c * 4 + rq;
+                top2_push_fast(rs, S[rj * kStride + il], i0 + il, inf);
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 32; ++c) {
+                int const il = c * 4 + rq;
+                float const v = S[rj * kStride + il];
+                top2_push_fast(rs, (i0 + il < nL) ? v : -INFINITY, i0 + il, inf);
+            }
         }
         rs = top2_merge(rs, top2_shfl_xor(rs, 1));
         rs = top2_merge(rs, top2_shfl_xor(rs, 2));
@@ -165,7 +184,7 @@ void launch_fused_fp32(float const* L, float const* R, int nL, int nR, FusedPlan
 {
     if (nL <= 0 || nR <= 0) return;
     hipLaunchKernelGGL(k_fused_fp32, dim3(p.leftBlocks, p.splits), dim3(256), 0, st, L, R, nL, nR,
-                       p.tilesPerSplit, colPart, ldc, rowPart, ldr);
+                       p.tilesPerSplit, colPart, ldc, rowPart, ldr, INFINITY);
 }
 
 }  // namespace v3dm

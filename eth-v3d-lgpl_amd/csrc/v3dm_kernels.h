@@ -9,10 +9,11 @@ struct Top2;
 
 constexpr int kMaxSplits = 64;    // right-feature splits of the fused kernel
 constexpr int kCntPairs = 0;      // counters[0]: number of matches
-constexpr int kCntFlag = 1;       // counters[1]: open queries of the running call (cleared at its end)
+constexpr int kCntFlag = 1;       // counters[1]: open left queries of the running call (cleared at its end)
 constexpr int kCntFlagDone = 2;   // counters[2]: queries re-checked exactly by the last call
 constexpr int kCntRescored = 3;   // counters[3]: shortlist candidates rescored by the running call
 constexpr int kCntRescoredDone = 4;  // counters[4]: ... by the last call
+constexpr int kCntFlagR = 5;      // counters[5]: open right queries of the running call
 constexpr int kNumCounters = 8;
 constexpr int kRecheckMaxChunks = 64;  // candidate chunks per re-checked query
 constexpr int kRowPad = 128;      // device descriptor buffers hold a multiple of 128 rows
@@ -40,11 +41,12 @@ void launch_norms(float const* X, int n, float* norm, unsigned int* maxNormBits,
 void launch_merge_decide(Top2 const* colPart, int nColParts, int ldc, Top2 const* rowPart, int nRowParts, int ldr,
                          int nL, int nR, float const* normL, float const* normR, unsigned int const* maxNormBits,
                          float epsRel, float minScore, float minRatio, int* colMax, int* rowMax, int* flagList,
-                         int* counters, hipStream_t st);
-RecheckPlan plan_recheck(int nL, int nR);
-void launch_recheck_exact(float const* L, float const* R, int nL, int nR, int const* flagList, int const* counters,
-                          float minScore, float minRatio, int* colMax, int* rowMax, Top2* part, int grid,
-                          hipStream_t st);
+                         int flagOffR, int* counters, hipStream_t st);
+RecheckPlan plan_recheck(int nC);
+// flagList: open left queries at [0, ...), open right queries at [flagOffR, ...)
+void launch_recheck_exact(float const* L, float const* R, int nL, int nR, int const* flagList, int flagOffR,
+                          int const* counters, float minScore, float minRatio, int* colMax, int* rowMax, Top2* part,
+                          int grid, hipStream_t st);
 // nL = 0 publishes an empty result.  tmp: nL int2, blockCount: ceil(nL/1024) ints.
 void launch_mutual_compact(int const* colMax, int const* rowMax, int nL, int nR, int2* tmp, int* blockCount,
                            int2* pairs, int* counters, hipStream_t st);

@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void k_merge_decide(Top2 const* __restrict__ c
                                                       unsigned int const* __restrict__ maxNormBits, float epsRel,
                                                       float minScore, float minRatio, int* __restrict__ colMax,
                                                       int* __restrict__ rowMax, int* __restrict__ flagList,
-                                                      int* __restrict__ counters)
+                                                      int flagOffR, int* __restrict__ counters)
 {
     int const sub = threadIdx.x & 7;
     int const g = (int)((blockIdx.x * 256u + threadIdx.x) >> 3);
@@ -79,8 +79,8 @@ __global__ __launch_bounds__(256) void k_merge_decide(Top2 const* __restrict__ c
     float const eps = epsRel * nq * mo + kEpsAbs;
     int const d = (epsRel > 0.0f) ? decide_certified(s, eps, minScore, minRatio) : kFlag;
     if (d == kFlag) {
-        int const slot = atomicAdd(&counters[kCntFlag], 1);
-        flagList[slot] = g;
+        int const slot = atomicAdd(&counters[left ? kCntFlag : kCntFlagR], 1);
+        flagList[left ? slot : flagOffR + slot] = q;
     }
     (left ? colMax : rowMax)[q] = d;
 }
@@ -88,123 +88,144 @@ __global__ __launch_bounds__(256) void k_merge_decide(Top2 const* __restrict__ c
 void launch_merge_decide(Top2 const* colPart, int nColParts, int ldc, Top2 const* rowPart, int nRowParts,
                          int ldr, int nL, int nR, float const* normL, float const* normR,
                          unsigned int const* maxNormBits, float epsRel, float minScore, float minRatio,
-                         int* colMax, int* rowMax, int* flagList, int* counters, hipStream_t st)
+                         int* colMax, int* rowMax, int* flagList, int flagOffR, int* counters, hipStream_t st)
 {
     int const n = nL + nR;
     if (n <= 0) return;
     hipLaunchKernelGGL(k_merge_decide, dim3((n + 31) / 32), dim3(256), 0, st, colPart, nColParts, ldc, rowPart,
                        nRowParts, ldr, nL, nR, normL, normR, maxNormBits, epsRel, minScore, minRatio, colMax,
-                       rowMax, flagList, counters);
+                       rowMax, flagList, flagOffR, counters);
 }
 
 // ------------------------------------------------------------------------------ exact re-check
-// Pass 1: work item = (open query e, chunk c of up to 256*per candidates); grid-stride
-// over counters[kCntFlag] * nChunks items.  Each thread scores its candidates exactly:
-// fl32(sum_k double(q_k) * double(c_k)) in ascending k (products of floats are exact in
-// FP64, so fused multiply-adds give the same sum), 4 independent chains at a time.  The
-// chunk's top-2 (first index on ties) goes to part[e * nChunks + c].
-// Pass 2: one wave per open query merges its chunk partials and takes the IEEE FP32
-// decision.  The kernel boundary between the passes orders the partial writes.
+// Open queries are listed per direction (left queries at flagList[0..), right queries at
+// flagList[flagOffR..)).  Pass 1: a work item is (group of up to kRecheckGroup queries of
+// one direction, chunk of 256*per candidates); grid-stride over all items.  Each thread
+// scores its candidate against every query of the group: fl32(sum_k double(q_k) *
+// double(c_k)) in ascending k (products of floats are exact in FP64, so fused
+// multiply-adds give the same sum), the group's queries being independent FP64 chains.
+// The chunk's top-2 per query (first index on ties) goes to part[(row) * kRecheckMaxChunks
+// + chunk].  Pass 2: one wave per open query merges its chunk partials and takes the IEEE
+// FP32 decision.  The kernel boundary between the passes orders the partial writes.
+constexpr int kRecheckGroup = 16;
+
 __global__ __launch_bounds__(256) void k_recheck_chunks(float const* __restrict__ L, float const* __restrict__ R,
                                                         int nL, int nR, int const* __restrict__ flagList,
-                                                        int const* __restrict__ counters, int nChunks, int per,
+                                                        int flagOffR, int const* __restrict__ counters,
+                                                        int nChunksL, int perL, int nChunksR, int perR,
                                                         Top2* __restrict__ part)
 {
-    __shared__ double qd[kDim];
-    __shared__ Top2 red[4];
+    __shared__ double qd[kRecheckGroup][kDim];
+    __shared__ Top2 red[4][kRecheckGroup];
     int const tid = threadIdx.x;
-    int const items = counters[kCntFlag] * nChunks;
+    int const lane = tid & 63;
+    int const wave = tid >> 6;
+    int const cntL = counters[kCntFlag];
+    int const cntR = counters[kCntFlagR];
+    int const grpL = (cntL + kRecheckGroup - 1) / kRecheckGroup;
+    int const grpR = (cntR + kRecheckGroup - 1) / kRecheckGroup;
+    int const itemsL = grpL * nChunksL;
+    int const items = itemsL + grpR * nChunksR;
     for (int w = blockIdx.x; w < items; w += gridDim.x) {
-        int const e = w / nChunks;
-        int const c = w - e * nChunks;
-        int const code = flagList[e];
-        bool const left = code < nL;
-        int const q = left ? code : code - nL;
-        float const* Q = (left ? L : R) + (size_t)q * kDim;
+        bool const left = w < itemsL;
+        int const wl = left ? w : w - itemsL;
+        int const nCh = left ? nChunksL : nChunksR;
+        int const per = left ? perL : perR;
+        int const g = wl / nCh;
+        int const c = wl - g * nCh;
+        int const cnt = left ? cntL : cntR;
+        int const base = left ? 0 : flagOffR;
+        int const nq = min(kRecheckGroup, cnt - g * kRecheckGroup);
+        float const* Qs = left ? L : R;
         float const* C = left ? R : L;
         int const nC = left ? nR : nL;
         __syncthreads();
-        if (tid < kDim) qd[tid] = (double)Q[tid];
+        for (int x = tid; x < kRecheckGroup * kDim; x += 256) {
+            int const qi = x / kDim, k = x - qi * kDim;
+            qd[qi][k] = (qi < nq) ? (double)Qs[(size_t)flagList[base + g * kRecheckGroup + qi] * kDim + k] : 0.0;
+        }
         __syncthreads();
-        Top2 s = top2_init();
+        Top2 s[kRecheckGroup];
+#pragma unroll
+        for (int qi = 0; qi < kRecheckGroup; ++qi) s[qi] = top2_init();
         int const cbeg = c * 256 * per;
-        for (int u0 = 0; u0 < per; u0 += 4) {
-            double acc[4] = {0.0, 0.0, 0.0, 0.0};
-            float4 const* cp[4];
-            bool ok[4];
+        for (int u = 0; u < per; ++u) {
+            int const ci = cbeg + u * 256 + tid;
+            if (ci >= nC) break;
+            float4 const* cp = reinterpret_cast<float4 const*>(C + (size_t)ci * kDim);
+            double acc[kRecheckGroup];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                int const ci = cbeg + (u0 + u) * 256 + tid;
-                ok[u] = (u0 + u < per) && (ci < nC);
-                cp[u] = reinterpret_cast<float4 const*>(C + (size_t)(ok[u] ? ci : 0) * kDim);
-            }
-#pragma unroll 4
+            for (int qi = 0; qi < kRecheckGroup; ++qi) acc[qi] = 0.0;
             for (int k4 = 0; k4 < 32; ++k4) {
+                float4 const v = cp[k4];
+                double const x0 = v.x, x1 = v.y, x2 = v.z, x3 = v.w;
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    float4 const v = cp[u][k4];
-                    acc[u] += qd[4 * k4 + 0] * (double)v.x;
-                    acc[u] += qd[4 * k4 + 1] * (double)v.y;
-                    acc[u] += qd[4 * k4 + 2] * (double)v.z;
-                    acc[u] += qd[4 * k4 + 3] * (double)v.w;
+                for (int qi = 0; qi < kRecheckGroup; ++qi) {
+                    acc[qi] += qd[qi][4 * k4 + 0] * x0;
+                    acc[qi] += qd[qi][4 * k4 + 1] * x1;
+                    acc[qi] += qd[qi][4 * k4 + 2] * x2;
+                    acc[qi] += qd[qi][4 * k4 + 3] * x3;
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (ok[u]) top2_push(s, (float)acc[u], cbeg + (u0 + u) * 256 + tid);
+            for (int qi = 0; qi < kRecheckGroup; ++qi) top2_push(s[qi], (float)acc[qi], ci);
         }
-        for (int o = 32; o > 0; o >>= 1) s = top2_merge(s, top2_shfl_xor(s, o));
-        if ((tid & 63) == 0) red[tid >> 6] = s;
+#pragma unroll
+        for (int qi = 0; qi < kRecheckGroup; ++qi) {
+            Top2 t = s[qi];
+            for (int o = 32; o > 0; o >>= 1) t = top2_merge(t, top2_shfl_xor(t, o));
+            if (lane == 0) red[wave][qi] = t;
+        }
         __syncthreads();
-        if (tid == 0) {
-            Top2 t = red[0];
-            for (int wv = 1; wv < 4; ++wv) t = top2_merge(t, red[wv]);
-            part[(size_t)e * nChunks + c] = t;
+        if (tid < nq) {
+            Top2 t = red[0][tid];
+            for (int wv = 1; wv < 4; ++wv) t = top2_merge(t, red[wv][tid]);
+            part[(size_t)(base + g * kRecheckGroup + tid) * kRecheckMaxChunks + c] = t;
         }
     }
 }
 
-__global__ __launch_bounds__(256) void k_recheck_finish(int nL, int const* __restrict__ flagList,
-                                                        int const* __restrict__ counters, int nChunks,
-                                                        Top2 const* __restrict__ part, float minScore,
-                                                        float minRatio, int* __restrict__ colMax,
+__global__ __launch_bounds__(256) void k_recheck_finish(int const* __restrict__ flagList, int flagOffR,
+                                                        int const* __restrict__ counters, int nChunksL,
+                                                        int nChunksR, Top2 const* __restrict__ part,
+                                                        float minScore, float minRatio, int* __restrict__ colMax,
                                                         int* __restrict__ rowMax)
 {
     int const lane = threadIdx.x & 63;
-    int const cnt = counters[kCntFlag];
+    int const cntL = counters[kCntFlag];
+    int const cnt = cntL + counters[kCntFlagR];
     for (int e = blockIdx.x * 4 + (threadIdx.x >> 6); e < cnt; e += gridDim.x * 4) {
+        bool const left = e < cntL;
+        int const row = left ? e : flagOffR + (e - cntL);
+        int const nCh = left ? nChunksL : nChunksR;
         Top2 s = top2_init();
-        for (int c = lane; c < nChunks; c += 64) s = top2_merge(s, part[(size_t)e * nChunks + c]);
+        for (int c = lane; c < nCh; c += 64) s = top2_merge(s, part[(size_t)row * kRecheckMaxChunks + c]);
         for (int o = 32; o > 0; o >>= 1) s = top2_merge(s, top2_shfl_xor(s, o));
-        if (lane == 0) {
-            int const code = flagList[e];
-            bool const left = code < nL;
-            (left ? colMax : rowMax)[left ? code : code - nL] = decide_exact(s, minScore, minRatio);
-        }
+        if (lane == 0) (left ? colMax : rowMax)[flagList[row]] = decide_exact(s, minScore, minRatio);
     }
 }
 
-RecheckPlan plan_recheck(int nL, int nR)
+RecheckPlan plan_recheck(int nC)
 {
     RecheckPlan p;
-    int const n = nL > nR ? nL : nR;
-    p.per = (n + 256 * kRecheckMaxChunks - 1) / (256 * kRecheckMaxChunks);
+    p.per = (nC + 256 * kRecheckMaxChunks - 1) / (256 * kRecheckMaxChunks);
     if (p.per < 1) p.per = 1;
-    p.nChunks = (n + 256 * p.per - 1) / (256 * p.per);
+    p.nChunks = (nC + 256 * p.per - 1) / (256 * p.per);
     if (p.nChunks < 1) p.nChunks = 1;
     return p;
 }
 
-void launch_recheck_exact(float const* L, float const* R, int nL, int nR, int const* flagList,
+void launch_recheck_exact(float const* L, float const* R, int nL, int nR, int const* flagList, int flagOffR,
                           int const* counters, float minScore, float minRatio, int* colMax, int* rowMax,
                           Top2* part, int grid, hipStream_t st)
 {
     if (nL <= 0 || nR <= 0) return;
-    RecheckPlan const p = plan_recheck(nL, nR);
-    hipLaunchKernelGGL(k_recheck_chunks, dim3(grid), dim3(256), 0, st, L, R, nL, nR, flagList, counters, p.nChunks,
-                       p.per, part);
-    hipLaunchKernelGGL(k_recheck_finish, dim3(256), dim3(256), 0, st, nL, flagList, counters, p.nChunks, part,
-                       minScore, minRatio, colMax, rowMax);
+    RecheckPlan const pl = plan_recheck(nR);  // left queries scan the right features
+    RecheckPlan const pr = plan_recheck(nL);
+    hipLaunchKernelGGL(k_recheck_chunks, dim3(grid), dim3(256), 0, st, L, R, nL, nR, flagList, flagOffR, counters,
+                       pl.nChunks, pl.per, pr.nChunks, pr.per, part);
+    hipLaunchKernelGGL(k_recheck_finish, dim3(256), dim3(256), 0, st, flagList, flagOffR, counters, pl.nChunks,
+                       pr.nChunks, part, minScore, minRatio, colMax, rowMax);
 }
 
 // ------------------------------------------------------------------------------ mutual + compaction
@@ -271,8 +292,9 @@ __global__ __launch_bounds__(1024) void k_mutual_gather(int2 const* __restrict__
     }
     if (tid == 0) {
         counters[kCntPairs] = offs[nBlocks];
-        counters[kCntFlagDone] = counters[kCntFlag];
+        counters[kCntFlagDone] = counters[kCntFlag] + counters[kCntFlagR];
         counters[kCntFlag] = 0;
+        counters[kCntFlagR] = 0;
         counters[kCntRescoredDone] = counters[kCntRescored];
         counters[kCntRescored] = 0;
     }

@@ -71,7 +71,7 @@ __device__ __forceinline__ void key_tile_update(KeyState& s, float o1, float o2,
 
 __global__ __launch_bounds__(256, 2) void k_pass_bf16(__bf16 const* __restrict__ Q, int nQ,
                                                       __bf16 const* __restrict__ C, int nC, int tilesPerSplit,
-                                                      Cand2* __restrict__ part, int ldq)
+                                                      Cand2* __restrict__ part, int ldq, float inf)
 {
     __shared__ __attribute__((aligned(16))) __bf16 lds[2][kST * kSRow];
 
@@ -140,19 +140,32 @@ __global__ __launch_bounds__(256, 2) void k_pass_bf16(__bf16 const* __restrict__
         }
 
         int const jt = t * kST;
+        bool const full = jt + kST <= nC;
 #pragma unroll
         for (int qb = 0; qb < 2; ++qb) {
             float const o1 = st[qb].k1, o2 = st[qb].k2;
             float k1 = o1, k2 = o2;
+            if (full) {
 #pragma unroll
-            for (int cb = 0; cb < 2; ++cb) {
+                for (int cb = 0; cb < 2; ++cb) {
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    int const jl = cb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-                    float const v = (jt + jl < nC) ? acc[cb][qb][r] : -1e30f;
-                    float const key = pack_key(v, cb * 16 + r);
-                    k2 = __builtin_amdgcn_fmed3f(k1, k2, key);
-                    k1 = fmaxf(k1, key);
+                    for (int r = 0; r < 16; ++r) {
+                        float const key = pack_key(acc[cb][qb][r], cb * 16 + r);
+                        k2 = __builtin_amdgcn_fmed3f(k1, k2, key);
+                        k1 = fast_max(k1, key, inf);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int cb = 0; cb < 2; ++cb) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        int const jl = cb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+                        float const v = (jt + jl < nC) ? acc[cb][qb][r] : -1e30f;
+                        float const key = pack_key(v, cb * 16 + r);
+                        k2 = __builtin_amdgcn_fmed3f(k1, k2, key);
+                        k1 = fast_max(k1, key, inf);
+                    }
                 }
             }
             st[qb].k1 = k1;
@@ -207,7 +220,7 @@ void launch_pass_bf16(__bf16 const* Q, int nQ, __bf16 const* C, int nC, ShortPla
 {
     if (nQ <= 0 || nC <= 0) return;
     hipLaunchKernelGGL(k_pass_bf16, dim3(p.queryBlocks, p.splits), dim3(256), 0, st, Q, nQ, C, nC, p.tilesPerSplit,
-                       part, ldq);
+                       part, ldq, INFINITY);
 }
 
 // ------------------------------------------------------------------------------ exact rescoring
@@ -282,13 +295,15 @@ __global__ __launch_bounds__(256) void k_rescore(float const* __restrict__ L, fl
         }
     }
 #pragma unroll
-    for (int o = 1; o < 8; o <<= 1) {
-        s = top2_merge(s, top2_shfl_xor(s, o));
-        nres += __shfl_xor(nres, o);
-    }
-    if (!valid || sub != 0) return;
-    (left ? colMax : rowMax)[q] = decide_exact(s, minScore, minRatio);
-    atomicAdd(&counters[kCntRescored], nres);
+    for (int o = 1; o < 8; o <<= 1) s = top2_merge(s, top2_shfl_xor(s, o));
+    if (valid && sub == 0) (left ? colMax : rowMax)[q] = decide_exact(s, minScore, minRatio);
+    // one counter update per work-group
+    __shared__ int wsum[4];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) nres += __shfl_xor(nres, o);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = nres;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(&counters[kCntRescored], wsum[0] + wsum[1] + wsum[2] + wsum[3]);
 }
 
 void launch_rescore(float const* L, float const* R, int nL, int nR, Cand2 const* partL, int nPartL, int ldqL,

@@ -23,6 +23,18 @@ def _gcc(src, out, extra):
     return os.path.join(HERE, out)
 
 
+def build_variant(define, out_dir):
+    """Oracle with a planted mistake (-DV3DO_PLANT_*), for tests that show the checks fail."""
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, f"liboracle_{define.lower()}.so")
+    cmd = ["gcc", "-fPIC", "-shared", "-std=gnu99", "-fopenmp", "-O2", "-ffp-contract=off", f"-D{define}",
+           os.path.join(HERE, "v3dm_oracle.c"), "-o", out, "-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(r.stderr)
+    return out
+
+
 def build_all():
     # exact oracle: no contraction, no fast-math; AVX2 only vectorises the FP64 lanes
     a = _gcc("v3dm_oracle.c", "liboracle.so", ["-O3", "-mavx2", "-ffp-contract=off", "-fno-fast-math"])

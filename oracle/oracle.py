@@ -13,6 +13,39 @@ _lib = None
 _cpu = None
 
 
+def _declare(lib):
+    vp, i, f = C.c_void_p, C.c_int, C.c_float
+    sig = {
+        "v3do_decide": (i, [f, f, i, f, f]),
+        "v3do_exact_dot": (f, [vp, vp]),
+        "v3do_scores": (None, [vp, i, vp, i, vp]),
+        "v3do_top2_scan": (None, [vp, i, vp, i, vp, vp, vp]),
+        "v3do_column_top2": (None, [vp, i, i, vp, vp, vp]),
+        "v3do_row_top2": (None, [vp, i, i, vp, vp, vp]),
+        "v3do_column_maxima": (None, [vp, i, i, f, f, vp]),
+        "v3do_row_maxima": (None, [vp, i, i, f, f, vp]),
+        "v3do_mutual": (i, [vp, i, vp, i, vp, i]),
+        "v3do_maxima": (None, [vp, i, vp, i, f, f, vp, vp]),
+        "v3do_find_putative_matches": (i, [vp, i, vp, i, f, f, vp, i]),
+        "v3do_round32": (i, [i]),
+        "v3do_effective_count": (i, [i, i]),
+        "v3do_mask_epipolar": (None, [vp, i, i, f, vp, vp, vp]),
+        "v3do_mask_homography": (None, [vp, i, i, f, vp, vp, vp]),
+        "v3do_guided": (i, [vp, i, i, i, vp, f, vp, vp, f, f, vp, i]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def use_library(path):
+    """Point this module at another build of the oracle (planted-mistake variants)."""
+    global _lib
+    _lib = _declare(C.CDLL(path))
+
+
 def _load():
     global _lib
     if _lib is None:
@@ -20,29 +53,7 @@ def _load():
         if not os.path.exists(p):
             raise RuntimeError(f"{p} missing: run oracle/build.py")
         lib = C.CDLL(p)
-        vp, i, f = C.c_void_p, C.c_int, C.c_float
-        sig = {
-            "v3do_decide": (i, [f, f, i, f, f]),
-            "v3do_exact_dot": (f, [vp, vp]),
-            "v3do_scores": (None, [vp, i, vp, i, vp]),
-            "v3do_top2_scan": (None, [vp, i, vp, i, vp, vp, vp]),
-            "v3do_column_top2": (None, [vp, i, i, vp, vp, vp]),
-            "v3do_row_top2": (None, [vp, i, i, vp, vp, vp]),
-            "v3do_column_maxima": (None, [vp, i, i, f, f, vp]),
-            "v3do_row_maxima": (None, [vp, i, i, f, f, vp]),
-            "v3do_mutual": (i, [vp, i, vp, i, vp, i]),
-            "v3do_maxima": (None, [vp, i, vp, i, f, f, vp, vp]),
-            "v3do_find_putative_matches": (i, [vp, i, vp, i, f, f, vp, i]),
-            "v3do_round32": (i, [i]),
-            "v3do_effective_count": (i, [i, i]),
-            "v3do_mask_epipolar": (None, [vp, i, i, f, vp, vp, vp]),
-            "v3do_mask_homography": (None, [vp, i, i, f, vp, vp, vp]),
-            "v3do_guided": (i, [vp, i, i, i, vp, f, vp, vp, f, f, vp, i]),
-        }
-        for name, (res, args) in sig.items():
-            fn = getattr(lib, name)
-            fn.restype = res
-            fn.argtypes = args
+        _declare(lib)
         _lib = lib
     return _lib
 

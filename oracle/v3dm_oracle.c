@@ -41,10 +41,17 @@
 
 /* ------------------------------------------------------------------ selection rule */
 
+/* Planted mistakes (tests/test_oracle.py builds these variants to show the checks fail
+ * on them): V3DO_PLANT_TIE_LAST keeps the last index on ties, V3DO_PLANT_NO_CLAMP drops
+ * the 0.001 floor of the ratio denominator, V3DO_PLANT_FP32_SUM sums in FP32. */
 static inline void top2_update(float v, int idx, float* maxVal, float* nextVal, int* argMax)
 {
     /* cu:145-147 */
+#ifdef V3DO_PLANT_TIE_LAST
+    *argMax = (v >= *maxVal) ? idx : *argMax;
+#else
     *argMax = (v > *maxVal) ? idx : *argMax;
+#endif
     *nextVal = (v > *maxVal) ? *maxVal : fmaxf(v, *nextVal);
     *maxVal = fmaxf(*maxVal, v);
 }
@@ -53,7 +60,11 @@ static inline void top2_update(float v, int idx, float* maxVal, float* nextVal, 
 int v3do_decide(float maxVal, float nextVal, int argMax, float minScore, float minRatio)
 {
     if (minRatio < 0.0f) return (maxVal > minScore) ? argMax : -1;
+#ifdef V3DO_PLANT_NO_CLAMP
+    float const nextDist = 2.0f - 2.0f * nextVal;
+#else
     float const nextDist = (float)fmax(0.001, (double)(2.0f - 2.0f * nextVal));
+#endif
     float const bestDist = 2.0f - 2.0f * maxVal;
     float const ratio = bestDist / nextDist;
     return (maxVal > minScore && ratio < minRatio) ? argMax : -1;
@@ -63,9 +74,15 @@ int v3do_decide(float maxVal, float nextVal, int argMax, float minScore, float m
 
 float v3do_exact_dot(float const* a, float const* b)
 {
+#ifdef V3DO_PLANT_FP32_SUM
+    float s = 0.0f;
+    for (int k = 0; k < D; ++k) s += a[k] * b[k];
+    return s;
+#else
     double s = 0.0;
     for (int k = 0; k < D; ++k) s += (double)a[k] * (double)b[k];
     return (float)s;
+#endif
 }
 
 /* S[j*nL + i] = s_ij  (row-major nR x nL, the layout of the reference's score buffer) */
@@ -101,10 +118,18 @@ void v3do_top2_scan(float const* Q, int nQ, float const* C, int nC,
             float const* cc = C + (size_t)c * D;
             double acc[QB];
             for (int t = 0; t < QB; ++t) acc[t] = 0.0;
+#ifdef V3DO_PLANT_FP32_SUM
+            for (int t = 0; t < QB; ++t) {
+                float f = 0.0f;
+                for (int k = 0; k < D; ++k) f += cc[k] * (float)Qt[k][t];
+                acc[t] = f;
+            }
+#else
             for (int k = 0; k < D; ++k) {
                 double const a = (double)cc[k];
                 for (int t = 0; t < QB; ++t) acc[t] += a * Qt[k][t];
             }
+#endif
             for (int t = 0; t < QB; ++t) top2_update((float)acc[t], c, &mx[t], &nx[t], &am[t]);
         }
         for (int t = 0; t < nq; ++t) {
