@@ -51,6 +51,22 @@ def time_form(m, steps, warmup):
     return ms_step, kernel_ms, st
 
 
+def copy_calibration(nbytes):
+    """One hipMemcpy device-to-device of nbytes (the runtime's copy kernel reads and writes
+    exactly nbytes), used to scale the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log(f"bench: copied {nbytes} bytes device to device")
+
+
 def cpu_baseline(L, R, budget_s=10.0, max_reps=20):
     from oracle import oracle
 
@@ -74,6 +90,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-oracle", action="store_true", help="skip the oracle match-set check")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run one device-to-device copy of MIB MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -81,6 +99,9 @@ def main():
 
     import v3dmatch
     from v3dmatch import synth
+
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration << 20)
 
     log(f"bench: generating {N_FEAT}x{N_FEAT} pair")
     qL, qR, _ = synth.make_pair(N_FEAT, N_FEAT, 0.4, config=2, pair=0)

@@ -13,28 +13,142 @@ namespace v3dm {
 // ------------------------------------------------------------------------------ fused tile kernel
 // Work-group: 4 waves, 128 left features (each wave keeps 32 of them as the B operand of
 // v_mfma_f32_32x32x2_f32 in 64 VGPRs), streaming 64-row tiles of right features through
-// double-buffered LDS.  Lane l of a wave holds column i = l%32 of the 32x32 blocks and
-// rows j = 8*(r/4) + 4*(l/32) + r%4, r = 0..15.
+// two LDS stages filled by direct global->LDS loads (global_load_lds_dwordx4), so tile
+// t+1 is in flight while tile t is multiplied.  Lane l of a wave holds column i = l%32 of
+// the 32x32 blocks and rows j = 8*(r/4) + 4*(l/32) + r%4, r = 0..15.
 //  * per left i: running top-2 over j in registers (rows of the lane), merged with the
 //    partner half-wave at the end -> colPart[split][i]
-//  * per right j: the tile is written to LDS as S[j][i] and 4 threads per row reduce it
-//    over the work-group's 128 left features -> rowPart[leftBlock][j]
-// The k index of MFMA step s is h*64 + s (h = l/32), so each lane reads 4 consecutive
-// floats per ds_read_b128; A and B use the same map, so the dot products are unchanged.
+//  * per right j: the tile's scores are written over the stage just consumed as S[j][i]
+//    and 4 threads per row reduce them over the work-group's 128 left features
+//    -> rowPart[leftBlock][j]
+// The k index of MFMA step s is h*64 + s (h = l/32); A and B use the same map, so the dot
+// products are unchanged.  Stage layout: row r, half h (k = 64h .. 64h+63, 256 bytes)
+// is segment h*64 + r, stored at 272*(h*64 + r): a lane's 16 A chunks are contiguous
+// (immediate offsets) and the 16-byte pad rotates consecutive rows across bank groups.
+// The direct loads fill the stage 1 KiB at a time (lane L of a load lands at +16L), so
+// each lane fetches whatever chunk belongs at its slot (pad slots refetch a neighbour).
+// After the multiply the stage is reused for the transposed tile S[j][i] (row stride 132
+// floats, conflict-free for the writes and the per-right reads).
 constexpr int kBI = 128;
 constexpr int kBJ = 64;
-constexpr int kStride = 132;  // LDS floats per tile row: 128 + 4 (conflict-free b128 reads)
+constexpr int kSeg = 272;             // bytes per stage segment (256 data + 16 pad)
+constexpr int kStageLoads = 9;        // 1 KiB direct loads per wave per tile (4 waves)
+constexpr int kStageBytes = kStageLoads * 4 * 1024;
+constexpr int kSRow = 132;            // floats per row of the transposed tile
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Byte offset (within a tile of R) fetched by slot `slot` of the stage.
+__device__ __forceinline__ int stage_src_fp32(int slot)
+{
+    int const seg = slot / 17, within = slot - seg * 17;
+    if (seg >= 2 * kBJ) return 0;  // tail of the stage: never read
+    int const h = seg / kBJ, row = seg - h * kBJ;
+    return row * 512 + (h * 16 + (within < 16 ? within : 15)) * 16;
+}
+
+__device__ __forceinline__ void stage_tile_fp32(float const* __restrict__ R, int t, unsigned char* stage,
+                                                int const (&src)[kStageLoads], int wv)
+{
+    unsigned char const* tile = reinterpret_cast<unsigned char const*>(R + (size_t)t * kBJ * kDim);
+#pragma unroll
+    for (int p = 0; p < kStageLoads; ++p)
+        __builtin_amdgcn_global_load_lds(tile + src[p], (lds_void*)(stage + (4 * p + wv) * 1024), 16, 0, 0);
+}
+
+template <bool kFullJ, bool kFullI>
+__device__ __forceinline__ void tile_fp32(unsigned char* stage, float const (&bfrag)[64], Top2& colS, Top2* rowPart,
+                                          int ldr, int lb, int t, int nL, int nR, int i0, int tid, float inf)
+{
+    int const lane = tid & 63;
+    int const wave = tid >> 6;
+    int const h = lane >> 5;
+    int const c32 = lane & 31;
+    v16f acc0 = {};
+    v16f acc1 = {};
+    unsigned char const* A0 = stage + (h * kBJ + c32) * kSeg;
+    unsigned char const* A1 = A0 + 32 * kSeg;
+#pragma unroll
+    for (int s4 = 0; s4 < 16; ++s4) {
+        float4 const a0 = *reinterpret_cast<float4 const*>(A0 + 16 * s4);
+        float4 const a1 = *reinterpret_cast<float4 const*>(A1 + 16 * s4);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, bfrag[4 * s4 + 0], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, bfrag[4 * s4 + 0], acc1, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, bfrag[4 * s4 + 1], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, bfrag[4 * s4 + 1], acc1, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.z, bfrag[4 * s4 + 2], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.z, bfrag[4 * s4 + 2], acc1, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.w, bfrag[4 * s4 + 3], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, bfrag[4 * s4 + 3], acc1, 0, 0, 0);
+    }
+    int const jt = t * kBJ;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        int const jl = 8 * (r >> 2) + 4 * h + (r & 3);
+        float v0 = acc0[r], v1 = acc1[r];
+        if (!kFullJ) {
+            v0 = (jt + jl < nR) ? v0 : -INFINITY;
+            v1 = (jt + jl + 32 < nR) ? v1 : -INFINITY;
+        }
+        top2_push_fast(colS, v0, jt + jl, inf);
+        top2_push_fast(colS, v1, jt + jl + 32, inf);
+    }
+    __syncthreads();  // every wave is done reading the stage as the A operand
+    float* S = reinterpret_cast<float*>(stage);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        int const jl = 8 * (r >> 2) + 4 * h + (r & 3);
+        S[jl * kSRow + wave * 32 + c32] = acc0[r];
+        S[(jl + 32) * kSRow + wave * 32 + c32] = acc1[r];
+    }
+    __syncthreads();
+    int const rq = tid & 3;   // quarter of the 128 left features
+    int const rj = tid >> 2;  // tile row
+    Top2 rs = top2_init();
+#pragma unroll
+    for (int c = 0; c < 32; ++c) {
+        int const il = c * 4 + rq;
+        float v = S[rj * kSRow + il];
+        if (!kFullI) v = (i0 + il < nL) ? v : -INFINITY;
+        top2_push_fast(rs, v, i0 + il, inf);
+    }
+    rs = top2_merge(rs, top2_shfl_xor(rs, 1));
+    rs = top2_merge(rs, top2_shfl_xor(rs, 2));
+    if (rq == 0 && jt + rj < nR) rowPart[(size_t)lb * ldr + jt + rj] = rs;
+}
+
+// The tiles [t0, t1) of one work-group, double-buffered; kMasked handles a partial last
+// tile of right features or a partial block of left features.
+template <bool kMasked>
+__device__ __forceinline__ void run_tiles_fp32(float const* __restrict__ R, unsigned char* stA, unsigned char* stB,
+                                               int const (&src)[kStageLoads], float const (&bfrag)[64], Top2& colS,
+                                               Top2* rowPart, int ldr, int lb, int t0, int t1, int nL, int nR, int i0,
+                                               int tid, int wv, float inf)
+{
+    // two tiles per trip so that each stage is a distinct LDS object
+    for (int t = t0; t < t1; t += 2) {
+        __syncthreads();  // tile t is in stA; every wave is done with stB
+        if (t + 1 < t1) stage_tile_fp32(R, t + 1, stB, src, wv);
+        tile_fp32<!kMasked, !kMasked>(stA, bfrag, colS, rowPart, ldr, lb, t, nL, nR, i0, tid, inf);
+        if (t + 1 >= t1) break;
+        __syncthreads();  // tile t+1 is in stB; every wave is done with stA
+        if (t + 2 < t1) stage_tile_fp32(R, t + 2, stA, src, wv);
+        tile_fp32<!kMasked, !kMasked>(stB, bfrag, colS, rowPart, ldr, lb, t + 1, nL, nR, i0, tid, inf);
+    }
+}
 
 __global__ __launch_bounds__(256, 2) void k_fused_fp32(float const* __restrict__ L, float const* __restrict__ R,
                                                        int nL, int nR, int tilesPerSplit,
                                                        Top2* __restrict__ colPart, int ldc,
                                                        Top2* __restrict__ rowPart, int ldr, float inf)
 {
-    __shared__ __attribute__((aligned(16))) float lds[2][kBJ * kStride];
+    __shared__ __attribute__((aligned(1024))) unsigned char stA[kStageBytes];
+    __shared__ __attribute__((aligned(1024))) unsigned char stB[kStageBytes];
 
     int const tid = threadIdx.x;
     int const lane = tid & 63;
     int const wave = tid >> 6;
+    int const wv = __builtin_amdgcn_readfirstlane(wave);
     int const h = lane >> 5;
     int const c32 = lane & 31;
     int const lb = blockIdx.x;
@@ -43,6 +157,11 @@ __global__ __launch_bounds__(256, 2) void k_fused_fp32(float const* __restrict__
     int const nTiles = (nR + kBJ - 1) / kBJ;
     int const t0 = split * tilesPerSplit;
     int const t1 = min(t0 + tilesPerSplit, nTiles);
+
+    int src[kStageLoads];
+#pragma unroll
+    for (int p = 0; p < kStageLoads; ++p) src[p] = stage_src_fp32((4 * p + wv) * 64 + lane);
+    if (t0 < t1) stage_tile_fp32(R, t0, stA, src, wv);
 
     int const myI = i0 + wave * 32 + c32;
     float bfrag[64];
@@ -58,108 +177,10 @@ __global__ __launch_bounds__(256, 2) void k_fused_fp32(float const* __restrict__
         }
     }
 
-    float4 const* R4 = reinterpret_cast<float4 const*>(R);
-    // first tile -> lds[0]
-    if (t0 < t1) {
-#pragma unroll
-        for (int p = 0; p < 8; ++p) {
-            int const idx = p * 256 + tid;
-            int const row = idx >> 5, c4 = idx & 31;
-            float4 const v = R4[(size_t)(t0 * kBJ + row) * 32 + c4];
-            *reinterpret_cast<float4*>(&lds[0][row * kStride + c4 * 4]) = v;
-        }
-    }
-    __syncthreads();
-
     Top2 colS = top2_init();
-    int const rq = tid & 3;   // per-right reduction: quarter of the 128 left features
-    int const rj = tid >> 2;  // per-right reduction: tile row
-
-    for (int t = t0; t < t1; ++t) {
-        int const buf = (t - t0) & 1;
-        bool const hasNext = (t + 1) < t1;
-        float4 pre[8];
-        if (hasNext) {
-#pragma unroll
-            for (int p = 0; p < 8; ++p) {
-                int const idx = p * 256 + tid;
-                int const row = idx >> 5, c4 = idx & 31;
-                pre[p] = R4[(size_t)((t + 1) * kBJ + row) * 32 + c4];
-            }
-        }
-
-        v16f acc0 = {};
-        v16f acc1 = {};
-        float const* A0 = &lds[buf][c32 * kStride + h * 64];
-        float const* A1 = A0 + 32 * kStride;
-#pragma unroll
-        for (int s4 = 0; s4 < 16; ++s4) {
-            float4 const a0 = *reinterpret_cast<float4 const*>(A0 + 4 * s4);
-            float4 const a1 = *reinterpret_cast<float4 const*>(A1 + 4 * s4);
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, bfrag[4 * s4 + 0], acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, bfrag[4 * s4 + 0], acc1, 0, 0, 0);
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, bfrag[4 * s4 + 1], acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, bfrag[4 * s4 + 1], acc1, 0, 0, 0);
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.z, bfrag[4 * s4 + 2], acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.z, bfrag[4 * s4 + 2], acc1, 0, 0, 0);
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.w, bfrag[4 * s4 + 3], acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, bfrag[4 * s4 + 3], acc1, 0, 0, 0);
-        }
-        __syncthreads();  // every wave is done reading lds[buf]
-
-        if (hasNext) {
-#pragma unroll
-            for (int p = 0; p < 8; ++p) {
-                int const idx = p * 256 + tid;
-                int const row = idx >> 5, c4 = idx & 31;
-                *reinterpret_cast<float4*>(&lds[buf ^ 1][row * kStride + c4 * 4]) = pre[p];
-            }
-        }
-
-        int const jt = t * kBJ;
-        float* S = lds[buf];
-        if (jt + kBJ <= nR) {
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                int const jl = 8 * (r >> 2) + 4 * h + (r & 3);
-                top2_push_fast(colS, acc0[r], jt + jl, inf);
-                top2_push_fast(colS, acc1[r], jt + jl + 32, inf);
-            }
-        } else {
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                int const jl = 8 * (r >> 2) + 4 * h + (r & 3);
-                top2_push_fast(colS, (jt + jl < nR) ? acc0[r] : -INFINITY, jt + jl, inf);
-                top2_push_fast(colS, (jt + jl + 32 < nR) ? acc1[r] : -INFINITY, jt + jl + 32, inf);
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            int const jl = 8 * (r >> 2) + 4 * h + (r & 3);
-            S[jl * kStride + wave * 32 + c32] = acc0[r];
-            S[(jl + 32) * kStride + wave * 32 + c32] = acc1[r];
-        }
-        __syncthreads();
-
-        Top2 rs = top2_init();
-        if (i0 + kBI <= nL) {
-#pragma unroll
-            for (int c = 0; c < 32; ++c) {
-                int const il = c * 4 + rq;
-                top2_push_fast(rs, S[rj * kStride + il], i0 + il, inf);
-            }
-        } else {
-#pragma unroll
-            for (int c = 0; c < 32; ++c) {
-                int const il = c * 4 + rq;
-                float const v = S[rj * kStride + il];
-                top2_push_fast(rs, (i0 + il < nL) ? v : -INFINITY, i0 + il, inf);
-            }
-        }
-        rs = top2_merge(rs, top2_shfl_xor(rs, 1));
-        rs = top2_merge(rs, top2_shfl_xor(rs, 2));
-        if (rq == 0 && jt + rj < nR) rowPart[(size_t)lb * ldr + jt + rj] = rs;
-    }
+    bool const masked = (i0 + kBI > nL) || (t1 * kBJ > nR);
+    if (masked) run_tiles_fp32<true>(R, stA, stB, src, bfrag, colS, rowPart, ldr, lb, t0, t1, nL, nR, i0, tid, wv, inf);
+    else run_tiles_fp32<false>(R, stA, stB, src, bfrag, colS, rowPart, ldr, lb, t0, t1, nL, nR, i0, tid, wv, inf);
 
     colS = top2_merge(colS, top2_shfl_xor(colS, 32));
     if (h == 0 && myI < nL) colPart[(size_t)split * ldc + myI] = colS;

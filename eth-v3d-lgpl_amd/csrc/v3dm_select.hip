@@ -117,6 +117,7 @@ __global__ __launch_bounds__(256) void k_recheck_chunks(float const* __restrict_
 {
     __shared__ double qd[kRecheckGroup][kDim];
     __shared__ Top2 red[4][kRecheckGroup];
+    __shared__ int qidx[kRecheckGroup];
     int const tid = threadIdx.x;
     int const lane = tid & 63;
     int const wave = tid >> 6;
@@ -140,9 +141,28 @@ __global__ __launch_bounds__(256) void k_recheck_chunks(float const* __restrict_
         float const* C = left ? R : L;
         int const nC = left ? nR : nL;
         __syncthreads();
-        for (int x = tid; x < kRecheckGroup * kDim; x += 256) {
-            int const qi = x / kDim, k = x - qi * kDim;
-            qd[qi][k] = (qi < nq) ? (double)Qs[(size_t)flagList[base + g * kRecheckGroup + qi] * kDim + k] : 0.0;
+        if (tid < kRecheckGroup) qidx[tid] = (tid < nq) ? flagList[base + g * kRecheckGroup + tid] : -1;
+        __syncthreads();
+        {
+            // 16 queries x 128 floats = 512 float4: two independent float4 loads per thread
+            float4 v[2];
+            int qi[2], k4[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                int const x = u * 256 + tid;
+                qi[u] = x >> 5;
+                k4[u] = x & 31;
+                int const row = qidx[qi[u]];
+                v[u] = (row >= 0) ? reinterpret_cast<float4 const*>(Qs + (size_t)row * kDim)[k4[u]]
+                                  : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                qd[qi[u]][4 * k4[u] + 0] = v[u].x;
+                qd[qi[u]][4 * k4[u] + 1] = v[u].y;
+                qd[qi[u]][4 * k4[u] + 2] = v[u].z;
+                qd[qi[u]][4 * k4[u] + 3] = v[u].w;
+            }
         }
         __syncthreads();
         Top2 s[kRecheckGroup];
@@ -156,6 +176,7 @@ __global__ __launch_bounds__(256) void k_recheck_chunks(float const* __restrict_
             double acc[kRecheckGroup];
 #pragma unroll
             for (int qi = 0; qi < kRecheckGroup; ++qi) acc[qi] = 0.0;
+#pragma unroll 8
             for (int k4 = 0; k4 < 32; ++k4) {
                 float4 const v = cp[k4];
                 double const x0 = v.x, x1 = v.y, x2 = v.z, x3 = v.w;

@@ -36,19 +36,10 @@ void launch_to_bf16(float const* X, int rows, __bf16* Y, hipStream_t st)
 }
 
 // ------------------------------------------------------------------------------ one-direction pass
-// Work-group: 4 waves; each wave keeps 64 queries (two 32-query blocks) as B operands
-// (lane l: query l%32 of the block, bf16 k = h*64 + 8*s + v for MFMA step s, h = l/32),
-// so the work-group covers 256 queries.  Candidates stream through double-buffered LDS
-// in tiles of 64 (two 32-row blocks, A operand with the same k map).  Lane l then holds,
-// per 32x32 block, the scores of query l%32 against candidate rows
-// jl = 8*(r/4) + 4*h + r%4, r = 0..15.
-// Per query and lane half the running top-2 is kept on keys: the FP32 score with its
-// 6 low mantissa bits replaced by the position (block*16 + r) inside the tile, so that a
-// max / med3 pair updates value and position together; the tile of each of the two keys
-// is tracked once per tile.  Masked candidates score -1e30.
 constexpr int kSQ = 256;          // queries per work-group
-constexpr int kST = 64;           // candidates per tile
-constexpr int kSRow = 136;        // bf16 per LDS row: 128 + 8 (conflict-free b128 reads)
+constexpr int kST = 64;           // candidates per tile (16 KB of bf16)
+
+typedef __attribute__((address_space(3))) void lds_void;
 
 __device__ __forceinline__ float pack_key(float v, int pos)
 {
@@ -69,15 +60,115 @@ __device__ __forceinline__ void key_tile_update(KeyState& s, float o1, float o2,
     s.t2 = nt2;
 }
 
+// Tile t of the candidates -> one LDS stage by direct global->LDS loads
+// (global_load_lds_dwordx4, lane L of a load landing at +16L).  Stage layout: row r, half
+// h (k = 64h .. 64h+63, 128 bytes) is segment h*64 + r at 144*(h*64 + r): a lane's 8 A
+// chunks are contiguous (immediate offsets) and the 16-byte pad rotates consecutive rows
+// across bank groups.  Each lane fetches the chunk that belongs at its slot; pad slots
+// refetch a neighbour.
+constexpr int kSegB = 144;
+constexpr int kStageLoadsB = 5;  // 1 KiB direct loads per wave per tile (4 waves)
+constexpr int kStageBytesB = kStageLoadsB * 4 * 1024;
+
+__device__ __forceinline__ int stage_src_bf16(int slot)
+{
+    int const seg = slot / 9, within = slot - seg * 9;
+    if (seg >= 2 * kST) return 0;  // tail of the stage: never read
+    int const h = seg / kST, row = seg - h * kST;
+    return row * 256 + (h * 8 + (within < 8 ? within : 7)) * 16;
+}
+
+__device__ __forceinline__ void stage_tile_bf16(__bf16 const* __restrict__ C, int t, unsigned char* stage,
+                                                int const (&src)[kStageLoadsB], int wv)
+{
+    unsigned char const* tile = reinterpret_cast<unsigned char const*>(C + (size_t)t * kST * kDim);
+#pragma unroll
+    for (int p = 0; p < kStageLoadsB; ++p)
+        __builtin_amdgcn_global_load_lds(tile + src[p], (lds_void*)(stage + (4 * p + wv) * 1024), 16, 0, 0);
+}
+
+template <bool kFull>
+__device__ __forceinline__ void tile_bf16(unsigned char const* stage, v8bf const (&bq)[2][8], KeyState (&st)[2], int t,
+                                          int nC, int h, int c32, float inf)
+{
+    v16f acc[2][2];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+        acc[cb][0] = v16f{};
+        acc[cb][1] = v16f{};
+        unsigned char const* rowp = stage + (h * kST + cb * 32 + c32) * kSegB;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            v8bf const a = *reinterpret_cast<v8bf const*>(rowp + 16 * s);
+            acc[cb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bq[0][s], acc[cb][0], 0, 0, 0);
+            acc[cb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bq[1][s], acc[cb][1], 0, 0, 0);
+        }
+    }
+    int const jt = t * kST;
+#pragma unroll
+    for (int qb = 0; qb < 2; ++qb) {
+        float const o1 = st[qb].k1, o2 = st[qb].k2;
+        float k1 = o1, k2 = o2;
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                float v = acc[cb][qb][r];
+                if (!kFull) {
+                    int const jl = cb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+                    v = (jt + jl < nC) ? v : -1e30f;
+                }
+                float const key = pack_key(v, cb * 16 + r);
+                k2 = __builtin_amdgcn_fmed3f(k1, k2, key);
+                k1 = fast_max(k1, key, inf);
+            }
+        }
+        st[qb].k1 = k1;
+        st[qb].k2 = k2;
+        key_tile_update(st[qb], o1, o2, t);
+    }
+}
+
+// The tiles [t0, t1) of one work-group, double-buffered (two tiles per trip so that each
+// stage is a distinct LDS object); kFull = no tile of the range is partial.
+template <bool kFull>
+__device__ __forceinline__ void run_tiles_bf16(__bf16 const* __restrict__ C, unsigned char* stA, unsigned char* stB,
+                                               int const (&src)[kStageLoadsB], v8bf const (&bq)[2][8],
+                                               KeyState (&st)[2], int t0, int t1, int nC, int h, int c32, int wv,
+                                               float inf)
+{
+    for (int t = t0; t < t1; t += 2) {
+        __syncthreads();  // tile t is in stA; every wave is done with stB
+        if (t + 1 < t1) stage_tile_bf16(C, t + 1, stB, src, wv);
+        tile_bf16<kFull>(stA, bq, st, t, nC, h, c32, inf);
+        if (t + 1 >= t1) break;
+        __syncthreads();  // tile t+1 is in stB; every wave is done with stA
+        if (t + 2 < t1) stage_tile_bf16(C, t + 2, stA, src, wv);
+        tile_bf16<kFull>(stB, bq, st, t + 1, nC, h, c32, inf);
+    }
+}
+
+// Work-group: 4 waves; each wave keeps 64 queries (two 32-query blocks) as B operands
+// (lane l: query l%32 of the block, bf16 k = h*64 + 8*s + v for MFMA step s, h = l/32),
+// so the work-group covers 256 queries.  Candidates stream through two LDS stages in
+// tiles of 64 (two 32-row blocks, A operand with the same k map): while tile t is
+// multiplied, tile t+1 is already in flight into the other stage.  Lane l then holds, per
+// 32x32 block, the scores of query l%32 against candidate rows jl = 8*(r/4) + 4*h + r%4,
+// r = 0..15.  Per query and lane half the running top-2 is kept on keys: the FP32 score
+// with its 6 low mantissa bits replaced by the position (block*16 + r) inside the tile,
+// so that a med3 pair updates value and position together; the tile of each of the two
+// keys is tracked once per tile.  Masked candidates score -1e30.
 __global__ __launch_bounds__(256, 2) void k_pass_bf16(__bf16 const* __restrict__ Q, int nQ,
                                                       __bf16 const* __restrict__ C, int nC, int tilesPerSplit,
                                                       Cand2* __restrict__ part, int ldq, float inf)
 {
-    __shared__ __attribute__((aligned(16))) __bf16 lds[2][kST * kSRow];
+    __shared__ __attribute__((aligned(1024))) unsigned char stA[kStageBytesB];
+    __shared__ __attribute__((aligned(1024))) unsigned char stB[kStageBytesB];
 
     int const tid = threadIdx.x;
     int const lane = tid & 63;
     int const wave = tid >> 6;
+    int const wv = __builtin_amdgcn_readfirstlane(wave);
     int const h = lane >> 5;
     int const c32 = lane & 31;
     int const q0 = blockIdx.x * kSQ + wave * 64;
@@ -85,6 +176,11 @@ __global__ __launch_bounds__(256, 2) void k_pass_bf16(__bf16 const* __restrict__
     int const nTiles = (nC + kST - 1) / kST;
     int const t0 = split * tilesPerSplit;
     int const t1 = min(t0 + tilesPerSplit, nTiles);
+
+    int src[kStageLoadsB];
+#pragma unroll
+    for (int p = 0; p < kStageLoadsB; ++p) src[p] = stage_src_bf16((4 * p + wv) * 64 + lane);
+    if (t0 < t1) stage_tile_bf16(C, t0, stA, src, wv);
 
     v8bf bq[2][8];
 #pragma unroll
@@ -97,92 +193,12 @@ __global__ __launch_bounds__(256, 2) void k_pass_bf16(__bf16 const* __restrict__
         }
     }
 
-    uint4 const* C16 = reinterpret_cast<uint4 const*>(C);  // 16 uint4 per candidate row
-    if (t0 < t1) {
-#pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            int const idx = p * 256 + tid;
-            int const row = idx >> 4, c8 = idx & 15;
-            *reinterpret_cast<uint4*>(&lds[0][row * kSRow + c8 * 8]) = C16[(size_t)(t0 * kST + row) * 16 + c8];
-        }
-    }
-    __syncthreads();
-
     KeyState st[2];
 #pragma unroll
     for (int qb = 0; qb < 2; ++qb) st[qb] = KeyState{-INFINITY, -INFINITY, -1, -1};
 
-    for (int t = t0; t < t1; ++t) {
-        int const buf = (t - t0) & 1;
-        bool const hasNext = (t + 1) < t1;
-        uint4 pre[4];
-        if (hasNext) {
-#pragma unroll
-            for (int p = 0; p < 4; ++p) {
-                int const idx = p * 256 + tid;
-                int const row = idx >> 4, c8 = idx & 15;
-                pre[p] = C16[(size_t)((t + 1) * kST + row) * 16 + c8];
-            }
-        }
-
-        v16f acc[2][2];
-#pragma unroll
-        for (int cb = 0; cb < 2; ++cb) {
-            acc[cb][0] = v16f{};
-            acc[cb][1] = v16f{};
-            __bf16 const* A = &lds[buf][(cb * 32 + c32) * kSRow + h * 64];
-#pragma unroll
-            for (int s = 0; s < 8; ++s) {
-                v8bf const a = *reinterpret_cast<v8bf const*>(A + 8 * s);
-                acc[cb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bq[0][s], acc[cb][0], 0, 0, 0);
-                acc[cb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bq[1][s], acc[cb][1], 0, 0, 0);
-            }
-        }
-
-        int const jt = t * kST;
-        bool const full = jt + kST <= nC;
-#pragma unroll
-        for (int qb = 0; qb < 2; ++qb) {
-            float const o1 = st[qb].k1, o2 = st[qb].k2;
-            float k1 = o1, k2 = o2;
-            if (full) {
-#pragma unroll
-                for (int cb = 0; cb < 2; ++cb) {
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        float const key = pack_key(acc[cb][qb][r], cb * 16 + r);
-                        k2 = __builtin_amdgcn_fmed3f(k1, k2, key);
-                        k1 = fast_max(k1, key, inf);
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int cb = 0; cb < 2; ++cb) {
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        int const jl = cb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-                        float const v = (jt + jl < nC) ? acc[cb][qb][r] : -1e30f;
-                        float const key = pack_key(v, cb * 16 + r);
-                        k2 = __builtin_amdgcn_fmed3f(k1, k2, key);
-                        k1 = fast_max(k1, key, inf);
-                    }
-                }
-            }
-            st[qb].k1 = k1;
-            st[qb].k2 = k2;
-            key_tile_update(st[qb], o1, o2, t);
-        }
-
-        if (hasNext) {
-#pragma unroll
-            for (int p = 0; p < 4; ++p) {
-                int const idx = p * 256 + tid;
-                int const row = idx >> 4, c8 = idx & 15;
-                *reinterpret_cast<uint4*>(&lds[buf ^ 1][row * kSRow + c8 * 8]) = pre[p];
-            }
-        }
-        __syncthreads();
-    }
+    if (t1 * kST > nC) run_tiles_bf16<false>(C, stA, stB, src, bq, st, t0, t1, nC, h, c32, wv, inf);
+    else run_tiles_bf16<true>(C, stA, stB, src, bq, st, t0, t1, nC, h, c32, wv, inf);
 
 #pragma unroll
     for (int qb = 0; qb < 2; ++qb) {
@@ -280,14 +296,22 @@ __global__ __launch_bounds__(256) void k_rescore(float const* __restrict__ L, fl
             float4 const* qp = reinterpret_cast<float4 const*>(Qd);
             float4 const* cp = reinterpret_cast<float4 const*>(Cd + (size_t)j * kDim);
             double acc = 0.0;
-#pragma unroll 8
-            for (int k4 = 0; k4 < 32; ++k4) {
-                float4 const x = qp[k4];
-                float4 const y = cp[k4];
-                acc += (double)x.x * (double)y.x;
-                acc += (double)x.y * (double)y.y;
-                acc += (double)x.z * (double)y.z;
-                acc += (double)x.w * (double)y.w;
+            // two halves of 16 float4 pairs, each half's loads issued together
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                float4 x[16], y[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    x[u] = qp[half * 16 + u];
+                    y[u] = cp[half * 16 + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    acc += (double)x[u].x * (double)y[u].x;
+                    acc += (double)x[u].y * (double)y[u].y;
+                    acc += (double)x[u].z * (double)y[u].z;
+                    acc += (double)x[u].w * (double)y[u].w;
+                }
             }
             float const sc = (float)acc;
             if (sc == sc) s = top2_merge(s, Top2{sc, kInit, j});

@@ -62,3 +62,11 @@ def test_header_compiles_standalone(tmp_path, compiler, std, src):
     r = subprocess.run([compiler, std, "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
                         str(f)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_dropin_header_builds_and_links(tmp_path):
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "tests", "cpp"))
+    from build_dropin import build
+    exe = build(str(tmp_path))
+    assert os.path.exists(exe)
