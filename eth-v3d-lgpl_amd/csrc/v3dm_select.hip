@@ -108,6 +108,8 @@ void launch_merge_decide(Top2 const* colPart, int nColParts, int ldc, Top2 const
 // + chunk].  Pass 2: one wave per open query merges its chunk partials and takes the IEEE
 // FP32 decision.  The kernel boundary between the passes orders the partial writes.
 constexpr int kRecheckGroup = 16;
+constexpr int kRecheckSub = 64;     // candidates staged in LDS at a time
+constexpr int kRecheckRow = 132;    // floats per staged candidate row (528 bytes: conflict-free b128 reads)
 
 __global__ __launch_bounds__(256) void k_recheck_chunks(float const* __restrict__ L, float const* __restrict__ R,
                                                         int nL, int nR, int const* __restrict__ flagList,
@@ -115,8 +117,8 @@ __global__ __launch_bounds__(256) void k_recheck_chunks(float const* __restrict_
                                                         int nChunksL, int perL, int nChunksR, int perR,
                                                         Top2* __restrict__ part)
 {
-    __shared__ double qd[kRecheckGroup][kDim];
-    __shared__ Top2 red[4][kRecheckGroup];
+    __shared__ __attribute__((aligned(16))) double qd[kRecheckGroup][kDim];
+    __shared__ __attribute__((aligned(16))) float cand[kRecheckSub * kRecheckRow];
     __shared__ int qidx[kRecheckGroup];
     int const tid = threadIdx.x;
     int const lane = tid & 63;
@@ -164,44 +166,58 @@ __global__ __launch_bounds__(256) void k_recheck_chunks(float const* __restrict_
                 qd[qi[u]][4 * k4[u] + 3] = v[u].w;
             }
         }
-        __syncthreads();
-        Top2 s[kRecheckGroup];
+        // wave w scores queries 4w .. 4w+3 of the group; lane = candidate of the sub-tile
+        Top2 s[4];
 #pragma unroll
-        for (int qi = 0; qi < kRecheckGroup; ++qi) s[qi] = top2_init();
+        for (int j = 0; j < 4; ++j) s[j] = top2_init();
         int const cbeg = c * 256 * per;
-        for (int u = 0; u < per; ++u) {
-            int const ci = cbeg + u * 256 + tid;
-            if (ci >= nC) break;
-            float4 const* cp = reinterpret_cast<float4 const*>(C + (size_t)ci * kDim);
-            double acc[kRecheckGroup];
+        int const cend = min(cbeg + 256 * per, nC);
+        for (int c0 = cbeg; c0 < cend; c0 += kRecheckSub) {
+            __syncthreads();  // previous sub-tile consumed (and qd written, first time)
+            {
+                float4 v[8];
 #pragma unroll
-            for (int qi = 0; qi < kRecheckGroup; ++qi) acc[qi] = 0.0;
-#pragma unroll 8
-            for (int k4 = 0; k4 < 32; ++k4) {
-                float4 const v = cp[k4];
-                double const x0 = v.x, x1 = v.y, x2 = v.z, x3 = v.w;
+                for (int u = 0; u < 8; ++u) {
+                    int const x = u * 256 + tid;
+                    int const row = x >> 5, k4 = x & 31;
+                    v[u] = (c0 + row < nC) ? reinterpret_cast<float4 const*>(C + (size_t)(c0 + row) * kDim)[k4]
+                                           : make_float4(0.f, 0.f, 0.f, 0.f);
+                }
 #pragma unroll
-                for (int qi = 0; qi < kRecheckGroup; ++qi) {
-                    acc[qi] += qd[qi][4 * k4 + 0] * x0;
-                    acc[qi] += qd[qi][4 * k4 + 1] * x1;
-                    acc[qi] += qd[qi][4 * k4 + 2] * x2;
-                    acc[qi] += qd[qi][4 * k4 + 3] * x3;
+                for (int u = 0; u < 8; ++u) {
+                    int const x = u * 256 + tid;
+                    int const row = x >> 5, k4 = x & 31;
+                    *reinterpret_cast<float4*>(&cand[row * kRecheckRow + 4 * k4]) = v[u];
                 }
             }
+            __syncthreads();
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+            float const* cr = &cand[lane * kRecheckRow];
+#pragma unroll 4
+            for (int k4 = 0; k4 < 32; ++k4) {
+                float4 const v = *reinterpret_cast<float4 const*>(cr + 4 * k4);
+                double const x0 = v.x, x1 = v.y, x2 = v.z, x3 = v.w;
 #pragma unroll
-            for (int qi = 0; qi < kRecheckGroup; ++qi) top2_push(s[qi], (float)acc[qi], ci);
+                for (int j = 0; j < 4; ++j) {
+                    double const* q = &qd[4 * wave + j][4 * k4];
+                    acc[j] += q[0] * x0;
+                    acc[j] += q[1] * x1;
+                    acc[j] += q[2] * x2;
+                    acc[j] += q[3] * x3;
+                }
+            }
+            int const ci = c0 + lane;
+            if (ci < nC) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) top2_push(s[j], (float)acc[j], ci);
+            }
         }
 #pragma unroll
-        for (int qi = 0; qi < kRecheckGroup; ++qi) {
-            Top2 t = s[qi];
+        for (int j = 0; j < 4; ++j) {
+            Top2 t = s[j];
             for (int o = 32; o > 0; o >>= 1) t = top2_merge(t, top2_shfl_xor(t, o));
-            if (lane == 0) red[wave][qi] = t;
-        }
-        __syncthreads();
-        if (tid < nq) {
-            Top2 t = red[0][tid];
-            for (int wv = 1; wv < 4; ++wv) t = top2_merge(t, red[wv][tid]);
-            part[(size_t)(base + g * kRecheckGroup + tid) * kRecheckMaxChunks + c] = t;
+            int const qg = 4 * wave + j;
+            if (lane == 0 && qg < nq) part[(size_t)(base + g * kRecheckGroup + qg) * kRecheckMaxChunks + c] = t;
         }
     }
 }

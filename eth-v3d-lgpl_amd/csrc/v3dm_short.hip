@@ -240,11 +240,24 @@ void launch_pass_bf16(__bf16 const* Q, int nQ, __bf16 const* C, int nC, ShortPla
 }
 
 // ------------------------------------------------------------------------------ exact rescoring
-// 8 lanes per query.  Lane `sub` owns the partition lists e = sub, sub+8, ... of the
-// query (2 entries each).  The approximate second best v2 of the union sets the window
-// v >= v2 - win (win = twice the per-score error bound); every listed candidate inside it
-// is rescored exactly (ascending-k FP64 sum of exact products, rounded to FP32) and the
-// exact top-2 (first index on ties) is merged over the 8 lanes.
+// One wave per query.  Lane e holds partition list e of the query (2 candidates each).
+// The approximate second best v2 of the union sets the window v >= v2 - win (win = twice
+// the per-score error bound); the candidates inside it are compacted into a list and
+// rescored exactly in batches of 8: the wave stages the query and 8 candidate rows in LDS
+// with coalesced loads, then lane k < 8 forms fl32(sum_k double(q_k) * double(c_k)) in
+// ascending k (the oracle's score).  The exact top-2 (first index on ties) of the rescored
+// candidates gives the reference's decision (cu:133-210).
+constexpr int kRescoreBatch = 8;
+
+// Orders the LDS accesses of one wave (each wave of k_rescore works in its own LDS slice,
+// and its trip counts differ from the other waves', so a work-group barrier is not used).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+constexpr int kRescoreRow = 132;  // floats per staged row (528 bytes: conflict-free b128 reads)
+
 __global__ __launch_bounds__(256) void k_rescore(float const* __restrict__ L, float const* __restrict__ R, int nL,
                                                  int nR, Cand2 const* __restrict__ partL, int nPartL, int ldqL,
                                                  Cand2 const* __restrict__ partR, int nPartR, int ldqR,
@@ -253,79 +266,84 @@ __global__ __launch_bounds__(256) void k_rescore(float const* __restrict__ L, fl
                                                  float minScore, float minRatio, int* __restrict__ colMax,
                                                  int* __restrict__ rowMax, int* __restrict__ counters)
 {
-    int const sub = threadIdx.x & 7;
-    int const g = (int)((blockIdx.x * 256u + threadIdx.x) >> 3);
+    __shared__ __attribute__((aligned(16))) float rows[4][(1 + kRescoreBatch) * kRescoreRow];
+    __shared__ int list[4][64];
+    __shared__ int wsum[4];
+    int const lane = threadIdx.x & 63;
+    int const wave = threadIdx.x >> 6;
+    int const g = blockIdx.x * 4 + wave;
     bool const valid = g < nL + nR;
     int const gq = valid ? g : 0;
     bool const left = gq < nL;
     int const q = left ? gq : gq - nL;
     Cand2 const* part = left ? partL : partR;
-    int const nEnt = valid ? 2 * (left ? nPartL : nPartR) : 0;  // (split, half) lists
+    int const nEnt = valid ? 2 * (left ? nPartL : nPartR) : 0;  // (split, half) lists, <= 64
     size_t const ldq = left ? (size_t)ldqL : (size_t)ldqR;
     float const* Qd = (left ? L : R) + (size_t)q * kDim;
     float const* Cd = left ? R : L;
+    float* st = rows[wave];
 
-    // approximate top-2 values of the union
-    float a1 = -INFINITY, a2 = -INFINITY;
-    for (int e = sub; e < nEnt; e += 8) {
-        Cand2 const c = part[((size_t)(e >> 1) * ldq + q) * 2 + (e & 1)];
-        a2 = __builtin_amdgcn_fmed3f(a1, a2, c.v1);
-        a1 = fmaxf(a1, c.v1);
-        a2 = __builtin_amdgcn_fmed3f(a1, a2, c.v2);
-        a1 = fmaxf(a1, c.v2);
-    }
+    // the query row -> LDS (row 0)
+    if (valid && lane < 32) *reinterpret_cast<float4*>(&st[4 * lane]) = reinterpret_cast<float4 const*>(Qd)[lane];
+
+    Cand2 c = {-INFINITY, -1, -INFINITY, -1};
+    if (lane < nEnt) c = part[((size_t)(lane >> 1) * ldq + q) * 2 + (lane & 1)];
+    float a1 = fmaxf(c.v1, c.v2), a2 = fminf(c.v1, c.v2);
 #pragma unroll
-    for (int o = 1; o < 8; o <<= 1) {
+    for (int o = 32; o > 0; o >>= 1) {
         float const b1 = __shfl_xor(a1, o), b2 = __shfl_xor(a2, o);
         a2 = fmaxf(fminf(a1, b1), fmaxf(a2, b2));
         a1 = fmaxf(a1, b1);
     }
-    float const nq = left ? normL[q] : normR[q];
+    float const nq = valid ? (left ? normL[q] : normR[q]) : 0.0f;
     float const mo = __uint_as_float(maxNormBits[left ? 1 : 0]);
     float const thr = a2 - winRel * nq * mo;
 
+    // compact the window candidates (each lane contributes up to 2, in lane order)
+    bool const in1 = c.j1 >= 0 && c.v1 >= thr;
+    bool const in2 = c.j2 >= 0 && c.v2 >= thr;
+    unsigned long long const b1m = __ballot(in1), b2m = __ballot(in2);
+    unsigned long long const lt = (1ull << lane) - 1ull;
+    int const pos = __popcll(b1m & lt) + __popcll(b2m & lt);
+    if (in1) list[wave][pos] = c.j1;
+    if (in2) list[wave][pos + (in1 ? 1 : 0)] = c.j2;
+    int const nWin = __popcll(b1m) + __popcll(b2m);
+
     Top2 s = top2_init();
-    int nres = 0;
-    for (int e = sub; e < nEnt; e += 8) {
-        Cand2 const c = part[((size_t)(e >> 1) * ldq + q) * 2 + (e & 1)];
+    for (int k0 = 0; k0 < nWin; k0 += kRescoreBatch) {
+        int const nb = min(kRescoreBatch, nWin - k0);
+        wave_sync();  // list[] and the query row are visible; the previous batch is consumed
+        // nb candidate rows -> LDS rows 1..nb (32 float4 each; lanes split by row parity)
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            int const j = u ? c.j2 : c.j1;
-            float const v = u ? c.v2 : c.v1;
-            if (j < 0 || !(v >= thr)) continue;
-            float4 const* qp = reinterpret_cast<float4 const*>(Qd);
-            float4 const* cp = reinterpret_cast<float4 const*>(Cd + (size_t)j * kDim);
+        for (int u = 0; u < kRescoreBatch / 2; ++u) {
+            int const r = 2 * u + (lane >> 5);
+            if (r < nb) {
+                int const j = list[wave][k0 + r];
+                *reinterpret_cast<float4*>(&st[(1 + r) * kRescoreRow + 4 * (lane & 31)]) =
+                    reinterpret_cast<float4 const*>(Cd + (size_t)j * kDim)[lane & 31];
+            }
+        }
+        wave_sync();
+        if (lane < nb) {
+            float const* cr = &st[(1 + lane) * kRescoreRow];
             double acc = 0.0;
-            // two halves of 16 float4 pairs, each half's loads issued together
-#pragma unroll
-            for (int half = 0; half < 2; ++half) {
-                float4 x[16], y[16];
-#pragma unroll
-                for (int u = 0; u < 16; ++u) {
-                    x[u] = qp[half * 16 + u];
-                    y[u] = cp[half * 16 + u];
-                }
-#pragma unroll
-                for (int u = 0; u < 16; ++u) {
-                    acc += (double)x[u].x * (double)y[u].x;
-                    acc += (double)x[u].y * (double)y[u].y;
-                    acc += (double)x[u].z * (double)y[u].z;
-                    acc += (double)x[u].w * (double)y[u].w;
-                }
+#pragma unroll 8
+            for (int k4 = 0; k4 < 32; ++k4) {
+                float4 const x = *reinterpret_cast<float4 const*>(&st[4 * k4]);
+                float4 const y = *reinterpret_cast<float4 const*>(cr + 4 * k4);
+                acc += (double)x.x * (double)y.x;
+                acc += (double)x.y * (double)y.y;
+                acc += (double)x.z * (double)y.z;
+                acc += (double)x.w * (double)y.w;
             }
             float const sc = (float)acc;
-            if (sc == sc) s = top2_merge(s, Top2{sc, kInit, j});
-            ++nres;
+            if (sc == sc) s = top2_merge(s, Top2{sc, kInit, list[wave][k0 + lane]});
         }
     }
 #pragma unroll
-    for (int o = 1; o < 8; o <<= 1) s = top2_merge(s, top2_shfl_xor(s, o));
-    if (valid && sub == 0) (left ? colMax : rowMax)[q] = decide_exact(s, minScore, minRatio);
-    // one counter update per work-group
-    __shared__ int wsum[4];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) nres += __shfl_xor(nres, o);
-    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = nres;
+    for (int o = 32; o > 0; o >>= 1) s = top2_merge(s, top2_shfl_xor(s, o));
+    if (valid && lane == 0) (left ? colMax : rowMax)[q] = decide_exact(s, minScore, minRatio);
+    if (lane == 0) wsum[wave] = valid ? nWin : 0;
     __syncthreads();
     if (threadIdx.x == 0) atomicAdd(&counters[kCntRescored], wsum[0] + wsum[1] + wsum[2] + wsum[3]);
 }
@@ -337,7 +355,7 @@ void launch_rescore(float const* L, float const* R, int nL, int nR, Cand2 const*
 {
     int const n = nL + nR;
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_rescore, dim3((n + 31) / 32), dim3(256), 0, st, L, R, nL, nR, partL, nPartL, ldqL, partR,
+    hipLaunchKernelGGL(k_rescore, dim3((n + 3) / 4), dim3(256), 0, st, L, R, nL, nR, partL, nPartL, ldqL, partR,
                        nPartR, ldqR, normL, normR, maxNormBits, winRel, minScore, minRatio, colMax, rowMax,
                        counters);
 }
