@@ -1,4 +1,4 @@
-// Precision modes other than the exact FP32 path: the BF16 shortlist form.
+// Precision modes other than the exact FP32 path: the BF16 and FP8 shortlist forms.
 #include "v3dm_ext.h"
 
 #include "v3dm_context.h"
@@ -11,6 +11,11 @@ struct ExtState {
     __bf16* bL = nullptr;
     __bf16* bR = nullptr;
     bool bLValid = false, bRValid = false;
+    unsigned char* f8L = nullptr;  // e4m3 descriptors, capL x 128 / capR x 128
+    unsigned char* f8R = nullptr;
+    unsigned char* eL = nullptr;  // E8M0 scale byte per descriptor
+    unsigned char* eR = nullptr;
+    bool f8LValid = false, f8RValid = false;
     Cand2* partL = nullptr;  // kShortMaxSplits x capL x 2
     Cand2* partR = nullptr;
     int fallbacks = 0;
@@ -44,6 +49,25 @@ int ensure(v3dm_context* h, ExtState*& e)
     return V3DM_OK;
 }
 
+int ensure_fp8(v3dm_context* h, ExtState*& e)
+{
+    int rc = ensure(h, e);
+    if (rc) return rc;
+    if (e->f8L) return V3DM_OK;
+    size_t const bl = (size_t)e->capL * kDim, br = (size_t)e->capR * kDim;
+    EXTCHECK(h, hipMalloc(&e->f8L, bl));
+    EXTCHECK(h, hipMalloc(&e->f8R, br));
+    EXTCHECK(h, hipMalloc(&e->eL, e->capL));
+    EXTCHECK(h, hipMalloc(&e->eR, e->capR));
+    // padding rows stay zero with scale 2^0 (they are masked out of every decision)
+    EXTCHECK(h, hipMemsetAsync(e->f8L, 0, bl, h->stream));
+    EXTCHECK(h, hipMemsetAsync(e->f8R, 0, br, h->stream));
+    EXTCHECK(h, hipMemsetAsync(e->eL, 127, e->capL, h->stream));
+    EXTCHECK(h, hipMemsetAsync(e->eR, 127, e->capR, h->stream));
+    e->f8LValid = e->f8RValid = false;
+    return V3DM_OK;
+}
+
 int enqueue_bf16(v3dm_context* h, float minScore, float minRatio)
 {
     ExtState*& e = ctx_ext(h);
@@ -74,13 +98,47 @@ int enqueue_bf16(v3dm_context* h, float minScore, float minRatio)
     return V3DM_OK;
 }
 
+int enqueue_fp8(v3dm_context* h, float minScore, float minRatio)
+{
+    ExtState*& e = ctx_ext(h);
+    int rc = ensure_fp8(h, e);
+    if (rc) return rc;
+    hipStream_t const st = h->stream;
+    int const nL = h->nL, nR = h->nR;
+    bool const work = nL > 0 && nR > 0;
+    if (work) {
+        if (!e->f8LValid) launch_to_fp8(h->dL, nL, e->f8L, e->eL, st);
+        if (!e->f8RValid) launch_to_fp8(h->dR, nR, e->f8R, e->eR, st);
+        e->f8LValid = e->f8RValid = true;
+        ShortPlan const pl = plan_short(nL, nR, 512);
+        ShortPlan const pr = plan_short(nR, nL, 512);
+        if ((rc = prof_begin(h))) return rc;
+        launch_pass_fp8(e->f8L, e->eL, nL, e->f8R, e->eR, nR, pl, e->partL, e->capL, st);
+        if ((rc = prof_end(h))) return rc;
+        if ((rc = prof_begin(h))) return rc;
+        launch_pass_fp8(e->f8R, e->eR, nR, e->f8L, e->eL, nL, pr, e->partR, e->capR, st);
+        if ((rc = prof_end(h))) return rc;
+        launch_rescore(h->dL, h->dR, nL, nR, e->partL, pl.splits, e->capL, e->partR, pr.splits, e->capR, h->normL,
+                       h->normR, h->maxNormBits, kWinRelFp8, minScore, minRatio, h->colMax, h->rowMax, h->results,
+                       st);
+    }
+    launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mtmp, h->mblock,
+                          reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st);
+    EXTCHECK(h, hipGetLastError());
+    return V3DM_OK;
+}
+
 }  // namespace
 
-bool ext_supports(v3dm_context*, int mode) { return mode == V3DM_PREC_FP32 || mode == V3DM_PREC_BF16; }
+bool ext_supports(v3dm_context*, int mode)
+{
+    return mode == V3DM_PREC_FP32 || mode == V3DM_PREC_BF16 || mode == V3DM_PREC_FP8;
+}
 
 int ext_enqueue(v3dm_context* h, int mode, float minScore, float minRatio)
 {
     if (mode == V3DM_PREC_BF16) return enqueue_bf16(h, minScore, minRatio);
+    if (mode == V3DM_PREC_FP8) return enqueue_fp8(h, minScore, minRatio);
     return ctx_fail(h, V3DM_ERR_UNSUPPORTED, "precision mode not available");
 }
 
@@ -94,8 +152,8 @@ void ext_features_changed(v3dm_context* h, bool left)
 {
     ExtState* e = ctx_ext(h);
     if (!e) return;
-    if (left) e->bLValid = false;
-    else e->bRValid = false;
+    if (left) e->bLValid = e->f8LValid = false;
+    else e->bRValid = e->f8RValid = false;
 }
 
 void ext_release(ExtState* e)
@@ -105,6 +163,10 @@ void ext_release(ExtState* e)
     if (e->bR) (void)hipFree(e->bR);
     if (e->partL) (void)hipFree(e->partL);
     if (e->partR) (void)hipFree(e->partR);
+    for (unsigned char* p : {e->f8L, e->f8R, e->eL, e->eR})
+        if (p) (void)hipFree(p);
+    e->f8L = e->f8R = e->eL = e->eR = nullptr;
+    e->f8LValid = e->f8RValid = false;
     e->bL = e->bR = nullptr;
     e->partL = e->partR = nullptr;
     e->capL = e->capR = 0;

@@ -26,10 +26,20 @@ constexpr int kShortRowPad = 256;  // bf16 descriptor buffers hold a multiple of
 // second best are rescored.
 constexpr float kWinRelBf16 = 2.0f * 1.01f * 0.00390625f;
 
+// FP8 form: e4m3 operands with per-descriptor power-of-two scales err by <= 2^-4 relative
+// each, so the worst case |s_fp8 - s| <= (2^-3 + 2^-8) * sum_k |q_k c_k| would rescore every
+// listed candidate.  On normalised SIFT descriptors the error of a 128-term dot product has
+// a standard deviation near 0.006 * ||q|| * ||c||, so the window is 2^-5 * ||q|| * max||c||
+// (about five standard deviations); the recall it gives is measured against the exact form.
+constexpr float kWinRelFp8 = 0.03125f;
+
 void launch_to_bf16(float const* X, int rows, __bf16* Y, hipStream_t st);
 ShortPlan plan_short(int nQ, int nC, int targetGroups);
 void launch_pass_bf16(__bf16 const* Q, int nQ, __bf16 const* C, int nC, ShortPlan const& p, Cand2* part, int ldq,
                       hipStream_t st);
+void launch_to_fp8(float const* X, int rows, unsigned char* Y, unsigned char* E, hipStream_t st);
+void launch_pass_fp8(unsigned char const* Q, unsigned char const* EQ, int nQ, unsigned char const* C,
+                     unsigned char const* EC, int nC, ShortPlan const& p, Cand2* part, int ldq, hipStream_t st);
 void launch_rescore(float const* L, float const* R, int nL, int nR, Cand2 const* partL, int nPartL, int ldqL,
                     Cand2 const* partR, int nPartR, int ldqR, float const* normL, float const* normR,
                     unsigned int const* maxNormBits, float winRel, float minScore, float minRatio, int* colMax,
