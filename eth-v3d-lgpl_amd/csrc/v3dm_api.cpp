@@ -122,6 +122,7 @@ namespace v3dm {
 
 hipStream_t ctx_stream(v3dm_context* h) { return h->stream; }
 ExtState*& ctx_ext(v3dm_context* h) { return h->ext; }
+BatchState*& ctx_batch(v3dm_context* h) { return h->batch; }
 int ctx_fail(v3dm_context* h, int code, const char* msg) { return fail(h, code, "%s", msg); }
 
 // Enqueue the plain FP32 exact path for the current features; leaves colMax/rowMax
@@ -204,6 +205,7 @@ void v3dm_destroy(v3dm_handle h)
     (void)hipStreamSynchronize(h->stream);
     releaseBuffers(h);
     ext_destroy(h->ext);
+    batch_destroy(h->batch);
     for (auto e : h->evStart) (void)hipEventDestroy(e);
     for (auto e : h->evStop) (void)hipEventDestroy(e);
     if (h->ownStream) (void)hipStreamDestroy(h->ownStream);

@@ -1,0 +1,732 @@
+// Batched matching host side: image upload, pair list, work-item plans, autotuning of the
+// item granularity and persistent grid, hipGraph capture of a batch run, result download.
+//
+// Replaces the caller loop of the reference (V3D/Apps/SfM/match_sequence.cpp:165-382,
+// V3D/Apps/SfM/match_using_voctree.cpp:264-609), which sets the two feature sets of each
+// pair and calls findPutativeMatches (V3D/CUDA/v3d_cudamatching.cu:490-552) once per pair.
+// Per pair the result is the same: counts rounded down to a multiple of 32 (cu:493-502),
+// matches sorted by left index.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <vector>
+
+#include "v3dm_batch.h"
+#include "v3dm_context.h"
+
+namespace v3dm {
+
+void launch_batch_load_u8(uint8_t const* src, int const* srcRow, int const* nImg, int const* blkImg,
+                          int const* imgBase, int rows, float* img, hipStream_t st);
+void launch_batch_load_f32(float const* src, int const* srcRow, int const* nImg, int const* blkImg,
+                           int const* imgBase, int rows, float* img, hipStream_t st);
+
+namespace {
+
+int roundUp(int n, int m) { return (n + m - 1) / m * m; }
+
+struct Plan {
+    std::vector<int4> items;
+    std::vector<BatchSeg> seg;
+    size_t partSlots = 0;
+    int tiles = 0;  // tiles per item asked for
+    int grid = 0;
+    float passMs = -1.0f;  // autotuner measurement
+};
+
+}  // namespace
+
+struct BatchState {
+    // images
+    int nImages = 0;
+    std::vector<int> imgN;     // rounded counts
+    std::vector<int> imgBaseH;
+    int imgRows = 0, imgCapRows = 0;
+    float* img = nullptr;
+    __bf16* imgB = nullptr;
+    bool bfValid = false;
+    int* imgBase = nullptr;
+    int* blkImg = nullptr;
+    int* nImgD = nullptr;
+    int* srcRow = nullptr;
+    float* norm = nullptr;
+    unsigned int* maxNorm = nullptr;
+    void* upload = nullptr;  // device staging of the raw input
+    size_t uploadCap = 0;
+    void* pinned = nullptr;  // host staging
+    size_t pinnedCap = 0;
+    int imgCap = 0;
+
+    // pairs
+    int nPairs = 0;
+    std::vector<int> pa, pb;
+    int qRows = 0, leftRows = 0;
+    std::vector<int> segOfBlockH;
+    int* segOfBlock = nullptr;
+    int segBlkCap = 0;
+
+    // plans (0 = plain FP32, 1 = shortlist) and their device copies
+    Plan plan[2];
+    bool planValid[2] = {false, false};
+    int4* items[2] = {nullptr, nullptr};
+    int itemCap[2] = {0, 0};
+    BatchSeg* segD[2] = {nullptr, nullptr};
+    int segCap[2] = {0, 0};
+    Top2* partF = nullptr;
+    size_t partFCap = 0;
+    Cand2* partB = nullptr;
+    size_t partBCap = 0;
+
+    // selection / results
+    int* sel = nullptr;
+    int* flagList = nullptr;
+    int selCap = 0;
+    int* counters = nullptr;  // kBatchCounters ints
+    int* pairCount = nullptr;
+    int2* tmp = nullptr;
+    int* blockCount = nullptr;
+    int2* out = nullptr;
+    int outCap = 0, pairCap = 0;
+    int* hostRes = nullptr;  // pinned: counters + pair counts
+    int hostResCap = 0;
+
+    // autotuning and graphs
+    bool autotune = true;
+    int forceTiles[2] = {0, 0}, forceGrid[2] = {0, 0};
+    struct GraphKey {
+        int mode;
+        float minScore, minRatio, epsRel;
+        int version;
+        bool operator<(GraphKey const& o) const
+        {
+            return std::tie(mode, minScore, minRatio, epsRel, version) <
+                   std::tie(o.mode, o.minScore, o.minRatio, o.epsRel, o.version);
+        }
+    };
+    std::map<GraphKey, hipGraphExec_t> graphs;
+    int version = 0;  // bumped when the layout or a plan changes
+    bool useGraph = true;
+
+    // last run
+    bool pending = false;
+    int lastMode = 0;
+    hipEvent_t evA = nullptr, evB = nullptr;
+    float lastMs = 0.0f;
+};
+
+namespace {
+
+#define BCHECK(h, call)                                                       \
+    do {                                                                      \
+        hipError_t const e_ = (call);                                         \
+        if (e_ != hipSuccess) return ctx_fail((h), V3DM_ERR_HIP, #call);      \
+    } while (0)
+
+template <class T>
+int grow(v3dm_context* h, T*& p, size_t& cap, size_t need)
+{
+    if (p && cap >= need) return V3DM_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t const n = std::max<size_t>(need, 1);
+    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return ctx_fail(h, V3DM_ERR_NOMEM, "device allocation failed");
+    cap = n;
+    return V3DM_OK;
+}
+
+template <class T>
+int growi(v3dm_context* h, T*& p, int& cap, size_t need)
+{
+    size_t c = (size_t)cap;
+    int const rc = grow(h, p, c, need);
+    cap = (int)c;
+    return rc;
+}
+
+void drop_graphs(BatchState* b)
+{
+    for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+    b->graphs.clear();
+}
+
+BatchDev dev_view(BatchState* b, int form)
+{
+    BatchDev d;
+    d.img = b->img;
+    d.imgB = b->imgB;
+    d.imgBase = b->imgBase;
+    d.norm = b->norm;
+    d.maxNorm = b->maxNorm;
+    d.seg = b->segD[form];
+    d.segOfBlock = b->segOfBlock;
+    d.nPairs = b->nPairs;
+    d.qRows = b->qRows;
+    return d;
+}
+
+// Query space and the segment skeleton (partials filled in by the plans).
+void layout_segments(BatchState* b, std::vector<BatchSeg>& seg)
+{
+    int const P = b->nPairs;
+    seg.assign(2 * P, BatchSeg{});
+    int q = 0;
+    for (int side = 0; side < 2; ++side) {
+        if (side == 1) b->leftRows = q;
+        for (int p = 0; p < P; ++p) {
+            BatchSeg& s = seg[side * P + p];
+            s.img = side == 0 ? b->pa[p] : b->pb[p];
+            s.oImg = side == 0 ? b->pb[p] : b->pa[p];
+            s.n = b->imgN[s.img];
+            s.nO = b->imgN[s.oImg];
+            s.pair = p;
+            s.qOff = q;
+            q += roundUp(std::max(s.n, 1), kBatchRowAlign);
+        }
+    }
+    b->qRows = q;
+}
+
+// Plain FP32 plan: items (pair, left block, split, tiles) with `tiles` 64-row right tiles
+// per item; partials: left queries one per split, right queries one per left block.
+Plan plan_plain(BatchState* b, int tiles, int grid)
+{
+    Plan pl;
+    pl.tiles = tiles;
+    pl.grid = grid;
+    layout_segments(b, pl.seg);
+    int const P = b->nPairs;
+    size_t off = 0;
+    std::vector<std::pair<int, int4>> work;
+    for (int p = 0; p < P; ++p) {
+        BatchSeg& sl = pl.seg[p];
+        BatchSeg& sr = pl.seg[P + p];
+        int const nL = sl.n, nR = sr.n;
+        int const padL = roundUp(std::max(nL, 1), kBatchRowAlign), padR = roundUp(std::max(nR, 1), kBatchRowAlign);
+        sl.ld = padL;
+        sr.ld = padR;
+        if (nL <= 0 || nR <= 0) {
+            sl.nParts = sr.nParts = 0;
+            sl.partOff = sr.partOff = 0;
+            continue;
+        }
+        int const nTiles = (nR + 63) / 64;
+        int const splits = (nTiles + tiles - 1) / tiles;
+        int const blocks = (nL + 127) / 128;
+        sl.nParts = splits;
+        sl.partOff = (int)off;
+        off += (size_t)splits * padL;
+        sr.nParts = blocks;
+        sr.partOff = (int)off;
+        off += (size_t)blocks * padR;
+        for (int lb = 0; lb < blocks; ++lb)
+            for (int s = 0; s < splits; ++s) {
+                int const cost = std::min(tiles, nTiles - s * tiles);
+                work.push_back({cost, make_int4(p, lb, s, tiles)});
+            }
+    }
+    std::stable_sort(work.begin(), work.end(),
+                     [](std::pair<int, int4> const& x, std::pair<int, int4> const& y) { return x.first > y.first; });
+    pl.items.reserve(work.size());
+    for (auto& w : work) pl.items.push_back(w.second);
+    pl.partSlots = off;
+    return pl;
+}
+
+// Shortlist plan: items (segment, query block, split, tiles) with at least `tiles` 64-row
+// candidate tiles per item and at most kShortMaxSplits splits per query.
+Plan plan_short_batch(BatchState* b, int tiles, int grid)
+{
+    Plan pl;
+    pl.tiles = tiles;
+    pl.grid = grid;
+    layout_segments(b, pl.seg);
+    size_t off = 0;
+    std::vector<std::pair<int, int4>> work;
+    for (int si = 0; si < (int)pl.seg.size(); ++si) {
+        BatchSeg& s = pl.seg[si];
+        s.ld = roundUp(std::max(s.n, 1), kBatchRowAlign);
+        if (s.n <= 0 || s.nO <= 0) {
+            s.nParts = 0;
+            s.partOff = 0;
+            continue;
+        }
+        int const nTiles = (s.nO + 63) / 64;
+        int const tps = std::max(tiles, (nTiles + kShortMaxSplits - 1) / kShortMaxSplits);
+        int const splits = (nTiles + tps - 1) / tps;
+        s.nParts = splits;
+        s.partOff = (int)off;
+        off += (size_t)splits * s.ld;
+        int const qBlocks = (s.n + 255) / 256;
+        for (int qb = 0; qb < qBlocks; ++qb)
+            for (int sp = 0; sp < splits; ++sp) {
+                int const cost = std::min(tps, nTiles - sp * tps);
+                work.push_back({cost, make_int4(si, qb, sp, tps)});
+            }
+    }
+    std::stable_sort(work.begin(), work.end(),
+                     [](std::pair<int, int4> const& x, std::pair<int, int4> const& y) { return x.first > y.first; });
+    pl.items.reserve(work.size());
+    for (auto& w : work) pl.items.push_back(w.second);
+    pl.partSlots = off;
+    return pl;
+}
+
+int upload_plan(v3dm_context* h, BatchState* b, int form, Plan const& pl)
+{
+    int rc;
+    if ((rc = growi(h, b->items[form], b->itemCap[form], pl.items.size()))) return rc;
+    if ((rc = growi(h, b->segD[form], b->segCap[form], pl.seg.size()))) return rc;
+    if (form == 0) {
+        if ((rc = grow(h, b->partF, b->partFCap, pl.partSlots))) return rc;
+    } else {
+        if ((rc = grow(h, b->partB, b->partBCap, pl.partSlots * 2))) return rc;
+    }
+    hipStream_t const st = h->stream;
+    if (!pl.items.empty())
+        BCHECK(h, hipMemcpyAsync(b->items[form], pl.items.data(), pl.items.size() * sizeof(int4),
+                                 hipMemcpyHostToDevice, st));
+    if (!pl.seg.empty())
+        BCHECK(h, hipMemcpyAsync(b->segD[form], pl.seg.data(), pl.seg.size() * sizeof(BatchSeg),
+                                 hipMemcpyHostToDevice, st));
+    BCHECK(h, hipStreamSynchronize(st));  // host vectors may change after return
+    return V3DM_OK;
+}
+
+int ensure_query_space(v3dm_context* h, BatchState* b)
+{
+    std::vector<BatchSeg> seg;
+    layout_segments(b, seg);
+    int const nBlk = b->qRows / kBatchRowAlign;
+    b->segOfBlockH.assign(nBlk, 0);
+    for (int si = 0; si < (int)seg.size(); ++si) {
+        int const nb = roundUp(std::max(seg[si].n, 1), kBatchRowAlign) / kBatchRowAlign;
+        for (int k = 0; k < nb; ++k) b->segOfBlockH[seg[si].qOff / kBatchRowAlign + k] = si;
+    }
+    int rc;
+    if ((rc = growi(h, b->segOfBlock, b->segBlkCap, nBlk))) return rc;
+    if ((rc = growi(h, b->sel, b->selCap, b->qRows))) return rc;
+    if (b->flagList) (void)hipFree(b->flagList);
+    b->flagList = nullptr;
+    BCHECK(h, hipMalloc(&b->flagList, (size_t)std::max(b->qRows, 1) * sizeof(int)));
+    int const nbMutual = (b->leftRows + 1023) / 1024;
+    if (b->tmp) (void)hipFree(b->tmp);
+    if (b->blockCount) (void)hipFree(b->blockCount);
+    b->tmp = nullptr;
+    b->blockCount = nullptr;
+    BCHECK(h, hipMalloc(&b->tmp, (size_t)std::max(nbMutual, 1) * 1024 * sizeof(int2)));
+    BCHECK(h, hipMalloc(&b->blockCount, (size_t)std::max(nbMutual, 1) * sizeof(int)));
+    // every left query gives at most one match
+    if ((rc = growi(h, b->out, b->outCap, std::max(b->leftRows, 1)))) return rc;
+    if ((rc = growi(h, b->pairCount, b->pairCap, std::max(b->nPairs, 1)))) return rc;
+    if (!b->counters) BCHECK(h, hipMalloc(&b->counters, kBatchCounters * sizeof(int)));
+    BCHECK(h, hipMemsetAsync(b->counters, 0, kBatchCounters * sizeof(int), h->stream));
+    if (b->hostRes && b->hostResCap < kBatchCounters + b->nPairs) {
+        (void)hipHostFree(b->hostRes);
+        b->hostRes = nullptr;
+    }
+    if (!b->hostRes) {
+        b->hostResCap = kBatchCounters + std::max(b->nPairs, 1);
+        BCHECK(h, hipHostMalloc(&b->hostRes, b->hostResCap * sizeof(int), hipHostMallocDefault));
+    }
+    if (nBlk > 0)
+        BCHECK(h, hipMemcpyAsync(b->segOfBlock, b->segOfBlockH.data(), nBlk * sizeof(int), hipMemcpyHostToDevice,
+                                 h->stream));
+    BCHECK(h, hipStreamSynchronize(h->stream));
+    return V3DM_OK;
+}
+
+int form_of(int mode) { return mode == V3DM_PREC_FP32 ? 0 : 1; }
+
+// Enqueue one batch run of `mode` with the current plan of its form.
+int enqueue_run(v3dm_context* h, BatchState* b, int mode, float minScore, float minRatio)
+{
+    int const form = form_of(mode);
+    hipStream_t const st = h->stream;
+    BatchDev const d = dev_view(b, form);
+    Plan const& pl = b->plan[form];
+    launch_batch_reset(b->counters, b->pairCount, b->nPairs, st);
+    if (form == 0) {
+        launch_batch_fused_fp32(d, b->items[0], (int)pl.items.size(), b->counters + 0, b->partF, pl.grid, st);
+        launch_batch_merge_decide(d, b->partF, h->epsRel, minScore, minRatio, b->sel, b->flagList, b->counters, st);
+        launch_batch_recheck(d, b->flagList, b->counters, minScore, minRatio, b->sel, 1024, st);
+    } else {
+        launch_batch_pass_bf16(d, b->items[1], (int)pl.items.size(), b->counters + 1, b->partB, pl.grid, st);
+        launch_batch_rescore(d, b->partB, kWinRelBf16, minScore, minRatio, b->sel, b->counters, st);
+    }
+    launch_batch_mutual(d, b->sel, b->leftRows, b->tmp, b->blockCount, b->out, b->pairCount, b->counters, st);
+    BCHECK(h, hipGetLastError());
+    return V3DM_OK;
+}
+
+// Time the pass kernel of `form` under plan `pl` (best of two runs), in ms.
+int time_pass(v3dm_context* h, BatchState* b, int form, Plan const& pl, float& ms)
+{
+    int rc = upload_plan(h, b, form, pl);
+    if (rc) return rc;
+    b->plan[form] = pl;
+    hipStream_t const st = h->stream;
+    BatchDev const d = dev_view(b, form);
+    float best = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
+        launch_batch_reset(b->counters, b->pairCount, b->nPairs, st);
+        BCHECK(h, hipEventRecord(b->evA, st));
+        if (form == 0)
+            launch_batch_fused_fp32(d, b->items[0], (int)pl.items.size(), b->counters + 0, b->partF, pl.grid, st);
+        else
+            launch_batch_pass_bf16(d, b->items[1], (int)pl.items.size(), b->counters + 1, b->partB, pl.grid, st);
+        BCHECK(h, hipEventRecord(b->evB, st));
+        BCHECK(h, hipEventSynchronize(b->evB));
+        float t = 0.0f;
+        BCHECK(h, hipEventElapsedTime(&t, b->evA, b->evB));
+        if (rep > 0) best = std::min(best, t);  // the first run warms caches and code
+    }
+    ms = best;
+    return V3DM_OK;
+}
+
+// Choose (tiles per item, persistent grid) for a form: the given override, or the fastest
+// candidate measured on this batch, or the default when tuning is off.
+int ensure_plan(v3dm_context* h, BatchState* b, int form)
+{
+    if (b->planValid[form]) return V3DM_OK;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    auto make = [&](int tiles, int grid) {
+        return form == 0 ? plan_plain(b, tiles, grid) : plan_short_batch(b, tiles, grid);
+    };
+    std::vector<int> tileCands = form == 0 ? std::vector<int>{8, 16, 32, 64} : std::vector<int>{4, 8, 16, 32};
+    std::vector<int> gridCands = {cus, 2 * cus};
+    int bestT = form == 0 ? 16 : 8, bestG = 2 * cus;
+    float bestMs = -1.0f;
+    if (b->forceTiles[form] > 0 || b->forceGrid[form] > 0) {
+        bestT = b->forceTiles[form] > 0 ? b->forceTiles[form] : bestT;
+        bestG = b->forceGrid[form] > 0 ? b->forceGrid[form] : bestG;
+    } else if (b->autotune) {
+        for (int t : tileCands)
+            for (int g : gridCands) {
+                float ms = 0.0f;
+                int const rc = time_pass(h, b, form, make(t, g), ms);
+                if (rc) return rc;
+                if (bestMs < 0.0f || ms < bestMs) {
+                    bestMs = ms;
+                    bestT = t;
+                    bestG = g;
+                }
+            }
+    }
+    drop_graphs(b);
+    Plan pl = make(bestT, bestG);
+    pl.passMs = bestMs;
+    int const rc = upload_plan(h, b, form, pl);
+    if (rc) return rc;
+    b->plan[form] = std::move(pl);
+    b->planValid[form] = true;
+    b->version += 1;
+    return V3DM_OK;
+}
+
+int check_batch(v3dm_context* h, BatchState*& b)
+{
+    if (!h) return V3DM_ERR_INVALID;
+    b = ctx_batch(h);
+    if (!b || b->nImages <= 0) return ctx_fail(h, V3DM_ERR_STATE, "no batch images set");
+    if (b->nPairs <= 0) return ctx_fail(h, V3DM_ERR_STATE, "no batch pairs set");
+    return V3DM_OK;
+}
+
+int set_images(v3dm_context* h, int nImages, int32_t const* counts, void const* desc, bool u8)
+{
+    if (!h) return V3DM_ERR_INVALID;
+    if (nImages <= 0 || !counts || !desc) return ctx_fail(h, V3DM_ERR_INVALID, "invalid batch images");
+    BatchState*& b = ctx_batch(h);
+    if (!b) {
+        b = new BatchState();
+        BCHECK(h, hipEventCreate(&b->evA));
+        BCHECK(h, hipEventCreate(&b->evB));
+    }
+    size_t totalRaw = 0;
+    std::vector<int> srcRowH(nImages), nH(nImages), baseH(nImages);
+    int rows = 0;
+    for (int m = 0; m < nImages; ++m) {
+        if (counts[m] < 0) return ctx_fail(h, V3DM_ERR_INVALID, "negative feature count");
+        srcRowH[m] = (int)totalRaw;
+        totalRaw += (size_t)counts[m];
+        nH[m] = counts[m] / 32 * 32;  // cu:493-502
+        baseH[m] = rows;
+        rows += roundUp(std::max(nH[m], 1), kBatchRowAlign);
+    }
+    if (totalRaw > (size_t)0x7fffffff / kDim) return ctx_fail(h, V3DM_ERR_INVALID, "batch too large");
+    drop_graphs(b);
+    b->planValid[0] = b->planValid[1] = false;
+    b->nPairs = 0;
+    b->pending = false;
+    b->nImages = nImages;
+    b->imgN = nH;
+    b->imgBaseH = baseH;
+    b->imgRows = rows;
+    int rc;
+    size_t imgCap = (size_t)b->imgCapRows;
+    if (imgCap < (size_t)rows) {
+        if (b->img) (void)hipFree(b->img);
+        if (b->imgB) (void)hipFree(b->imgB);
+        if (b->norm) (void)hipFree(b->norm);
+        b->img = nullptr;
+        b->imgB = nullptr;
+        b->norm = nullptr;
+        BCHECK(h, hipMalloc(&b->img, (size_t)rows * kDim * sizeof(float)));
+        BCHECK(h, hipMalloc(&b->imgB, (size_t)rows * kDim * sizeof(__bf16)));
+        BCHECK(h, hipMalloc(&b->norm, (size_t)rows * sizeof(float)));
+        b->imgCapRows = rows;
+    }
+    if ((rc = growi(h, b->imgBase, b->imgCap, nImages))) return rc;
+    if (b->nImgD) (void)hipFree(b->nImgD);
+    if (b->srcRow) (void)hipFree(b->srcRow);
+    if (b->maxNorm) (void)hipFree(b->maxNorm);
+    if (b->blkImg) (void)hipFree(b->blkImg);
+    b->nImgD = b->srcRow = b->blkImg = nullptr;
+    b->maxNorm = nullptr;
+    BCHECK(h, hipMalloc(&b->nImgD, nImages * sizeof(int)));
+    BCHECK(h, hipMalloc(&b->srcRow, nImages * sizeof(int)));
+    BCHECK(h, hipMalloc(&b->maxNorm, nImages * sizeof(unsigned int)));
+    int const nBlk = rows / kBatchRowAlign;
+    std::vector<int> blkImgH(nBlk);
+    for (int m = 0; m < nImages; ++m)
+        for (int k = 0; k < roundUp(std::max(nH[m], 1), kBatchRowAlign) / kBatchRowAlign; ++k)
+            blkImgH[baseH[m] / kBatchRowAlign + k] = m;
+    BCHECK(h, hipMalloc(&b->blkImg, nBlk * sizeof(int)));
+
+    size_t const elt = u8 ? 1 : sizeof(float);
+    size_t const bytes = totalRaw * kDim * elt;
+    if (b->pinnedCap < bytes) {
+        if (b->pinned) (void)hipHostFree(b->pinned);
+        b->pinned = nullptr;
+        b->pinnedCap = 0;
+        BCHECK(h, hipHostMalloc(&b->pinned, std::max<size_t>(bytes, 1), hipHostMallocDefault));
+        b->pinnedCap = std::max<size_t>(bytes, 1);
+    }
+    if (b->uploadCap < bytes) {
+        if (b->upload) (void)hipFree(b->upload);
+        b->upload = nullptr;
+        b->uploadCap = 0;
+        BCHECK(h, hipMalloc(&b->upload, std::max<size_t>(bytes, 1)));
+        b->uploadCap = std::max<size_t>(bytes, 1);
+    }
+    hipStream_t const st = h->stream;
+    BCHECK(h, hipStreamSynchronize(st));  // the staging buffers may still feed an earlier upload
+    std::memcpy(b->pinned, desc, bytes);
+    BCHECK(h, hipMemcpyAsync(b->upload, b->pinned, bytes, hipMemcpyHostToDevice, st));
+    BCHECK(h, hipMemcpyAsync(b->imgBase, baseH.data(), nImages * sizeof(int), hipMemcpyHostToDevice, st));
+    BCHECK(h, hipMemcpyAsync(b->nImgD, nH.data(), nImages * sizeof(int), hipMemcpyHostToDevice, st));
+    BCHECK(h, hipMemcpyAsync(b->srcRow, srcRowH.data(), nImages * sizeof(int), hipMemcpyHostToDevice, st));
+    BCHECK(h, hipMemcpyAsync(b->blkImg, blkImgH.data(), nBlk * sizeof(int), hipMemcpyHostToDevice, st));
+    BCHECK(h, hipMemsetAsync(b->maxNorm, 0, nImages * sizeof(unsigned int), st));
+    if (u8)
+        launch_batch_load_u8(static_cast<uint8_t const*>(b->upload), b->srcRow, b->nImgD, b->blkImg, b->imgBase, rows,
+                             b->img, st);
+    else
+        launch_batch_load_f32(static_cast<float const*>(b->upload), b->srcRow, b->nImgD, b->blkImg, b->imgBase, rows,
+                              b->img, st);
+    launch_batch_norms(b->img, b->blkImg, rows, b->norm, b->maxNorm, st);
+    b->bfValid = false;
+    BCHECK(h, hipGetLastError());
+    BCHECK(h, hipStreamSynchronize(st));  // host vectors above go out of scope
+    return V3DM_OK;
+}
+
+}  // namespace
+
+void batch_destroy(BatchState*& b)
+{
+    if (!b) return;
+    drop_graphs(b);
+    for (void* p : {(void*)b->img, (void*)b->imgB, (void*)b->imgBase, (void*)b->blkImg, (void*)b->nImgD,
+                    (void*)b->srcRow, (void*)b->norm, (void*)b->maxNorm, b->upload, (void*)b->segOfBlock,
+                    (void*)b->items[0], (void*)b->items[1], (void*)b->segD[0], (void*)b->segD[1], (void*)b->partF,
+                    (void*)b->partB, (void*)b->sel, (void*)b->flagList, (void*)b->counters, (void*)b->pairCount,
+                    (void*)b->tmp, (void*)b->blockCount, (void*)b->out})
+        if (p) (void)hipFree(p);
+    if (b->pinned) (void)hipHostFree(b->pinned);
+    if (b->hostRes) (void)hipHostFree(b->hostRes);
+    if (b->evA) (void)hipEventDestroy(b->evA);
+    if (b->evB) (void)hipEventDestroy(b->evB);
+    delete b;
+    b = nullptr;
+}
+
+}  // namespace v3dm
+
+using namespace v3dm;
+
+extern "C" {
+
+int v3dm_batch_set_images_u8(v3dm_handle h, int nImages, const int32_t* counts, const uint8_t* desc)
+{
+    return set_images(h, nImages, counts, desc, true);
+}
+
+int v3dm_batch_set_images(v3dm_handle h, int nImages, const int32_t* counts, const float* desc)
+{
+    return set_images(h, nImages, counts, desc, false);
+}
+
+int v3dm_batch_set_pairs(v3dm_handle h, int nPairs, const int32_t* left, const int32_t* right)
+{
+    if (!h) return V3DM_ERR_INVALID;
+    BatchState* b = ctx_batch(h);
+    if (!b || b->nImages <= 0) return ctx_fail(h, V3DM_ERR_STATE, "set the batch images first");
+    if (nPairs <= 0 || !left || !right) return ctx_fail(h, V3DM_ERR_INVALID, "invalid pair list");
+    for (int p = 0; p < nPairs; ++p)
+        if (left[p] < 0 || left[p] >= b->nImages || right[p] < 0 || right[p] >= b->nImages)
+            return ctx_fail(h, V3DM_ERR_INVALID, "pair refers to an image outside the batch");
+    drop_graphs(b);
+    b->planValid[0] = b->planValid[1] = false;
+    b->pending = false;
+    b->nPairs = nPairs;
+    b->pa.assign(left, left + nPairs);
+    b->pb.assign(right, right + nPairs);
+    return ensure_query_space(h, b);
+}
+
+int v3dm_batch_set_tuning(v3dm_handle h, int autotune, int mode, int tilesPerItem, int grid)
+{
+    if (!h) return V3DM_ERR_INVALID;
+    BatchState* b = ctx_batch(h);
+    if (!b) return ctx_fail(h, V3DM_ERR_STATE, "set the batch images first");
+    if (mode < 0 || mode > V3DM_PREC_LEARNED || tilesPerItem < 0 || grid < 0)
+        return ctx_fail(h, V3DM_ERR_INVALID, "invalid tuning");
+    int const form = form_of(mode);
+    b->autotune = autotune != 0;
+    b->forceTiles[form] = tilesPerItem;
+    b->forceGrid[form] = grid;
+    b->planValid[form] = false;
+    drop_graphs(b);
+    return V3DM_OK;
+}
+
+int v3dm_batch_get_tuning(v3dm_handle h, int mode, int* tilesPerItem, int* grid, int* nItems, float* passMs)
+{
+    if (!h) return V3DM_ERR_INVALID;
+    BatchState* b = ctx_batch(h);
+    if (!b) return ctx_fail(h, V3DM_ERR_STATE, "no batch");
+    int const form = form_of(mode);
+    if (!b->planValid[form]) return ctx_fail(h, V3DM_ERR_STATE, "no plan for this mode yet");
+    Plan const& pl = b->plan[form];
+    if (tilesPerItem) *tilesPerItem = pl.tiles;
+    if (grid) *grid = pl.grid;
+    if (nItems) *nItems = (int)pl.items.size();
+    if (passMs) *passMs = pl.passMs;
+    return V3DM_OK;
+}
+
+int v3dm_batch_use_graph(v3dm_handle h, int enable)
+{
+    if (!h) return V3DM_ERR_INVALID;
+    BatchState* b = ctx_batch(h);
+    if (!b) return ctx_fail(h, V3DM_ERR_STATE, "no batch");
+    b->useGraph = enable != 0;
+    return V3DM_OK;
+}
+
+int v3dm_batch_match_async(v3dm_handle h, float minScore, float minRatio)
+{
+    BatchState* b = nullptr;
+    int rc = check_batch(h, b);
+    if (rc) return rc;
+    int const mode = h->precision;
+    if (mode != V3DM_PREC_FP32 && mode != V3DM_PREC_BF16)
+        return ctx_fail(h, V3DM_ERR_UNSUPPORTED, "batched matching supports the FP32 and BF16 modes");
+    int const form = form_of(mode);
+    hipStream_t const st = h->stream;
+    if (form == 1 && !b->bfValid) {
+        launch_to_bf16(b->img, b->imgRows, b->imgB, st);
+        b->bfValid = true;
+    }
+    if ((rc = ensure_plan(h, b, form))) return rc;
+    BCHECK(h, hipEventRecord(b->evA, st));
+    if (b->useGraph) {
+        BatchState::GraphKey const key{mode, minScore, minRatio, h->epsRel, b->version};
+        auto it = b->graphs.find(key);
+        if (it == b->graphs.end()) {
+            hipGraph_t g = nullptr;
+            BCHECK(h, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            int const rc2 = enqueue_run(h, b, mode, minScore, minRatio);
+            hipError_t const ec = hipStreamEndCapture(st, &g);
+            if (rc2) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc2;
+            }
+            if (ec != hipSuccess) return ctx_fail(h, V3DM_ERR_HIP, "hipStreamEndCapture failed");
+            hipGraphExec_t ex = nullptr;
+            hipError_t const ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) return ctx_fail(h, V3DM_ERR_HIP, "hipGraphInstantiate failed");
+            it = b->graphs.emplace(key, ex).first;
+        }
+        BCHECK(h, hipGraphLaunch(it->second, st));
+    } else if ((rc = enqueue_run(h, b, mode, minScore, minRatio))) {
+        return rc;
+    }
+    BCHECK(h, hipEventRecord(b->evB, st));
+    BCHECK(h, hipMemcpyAsync(b->hostRes, b->counters, kBatchCounters * sizeof(int), hipMemcpyDeviceToHost, st));
+    BCHECK(h, hipMemcpyAsync(b->hostRes + kBatchCounters, b->pairCount, b->nPairs * sizeof(int),
+                             hipMemcpyDeviceToHost, st));
+    b->pending = true;
+    b->lastMode = mode;
+    return V3DM_OK;
+}
+
+int v3dm_batch_get_matches(v3dm_handle h, int32_t* offsets, int32_t* pairs, int cap, int* nTotal)
+{
+    BatchState* b = nullptr;
+    int rc = check_batch(h, b);
+    if (rc) return rc;
+    if (!b->pending) return ctx_fail(h, V3DM_ERR_STATE, "no batch run pending");
+    if (!nTotal || cap < 0 || (cap > 0 && !pairs)) return ctx_fail(h, V3DM_ERR_INVALID, "invalid output");
+    BCHECK(h, hipStreamSynchronize(h->stream));
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, b->evA, b->evB) == hipSuccess) b->lastMs = ms;
+    int const total = b->hostRes[kBCntPairs];
+    *nTotal = total;
+    if (offsets) {
+        int acc = 0;
+        for (int p = 0; p < b->nPairs; ++p) {
+            offsets[p] = acc;
+            acc += b->hostRes[kBatchCounters + p];
+        }
+        offsets[b->nPairs] = acc;
+    }
+    h->stats.nMatches = total;
+    h->stats.nRechecked = b->hostRes[kBCntFlagDone];
+    h->stats.nRescored = b->hostRes[kBCntRescoredDone];
+    h->stats.precision = b->lastMode;
+    h->stats.nLeft = h->stats.nRight = 0;
+    if (total > cap) return ctx_fail(h, V3DM_ERR_CAPACITY, "pair buffer too small");
+    if (total > 0) BCHECK(h, hipMemcpy(pairs, b->out, (size_t)total * sizeof(int2), hipMemcpyDeviceToHost));
+    return V3DM_OK;
+}
+
+int v3dm_batch_match(v3dm_handle h, float minScore, float minRatio, int32_t* offsets, int32_t* pairs, int cap,
+                     int* nTotal)
+{
+    int const rc = v3dm_batch_match_async(h, minScore, minRatio);
+    if (rc) return rc;
+    return v3dm_batch_get_matches(h, offsets, pairs, cap, nTotal);
+}
+
+int v3dm_batch_last_ms(v3dm_handle h, float* ms)
+{
+    if (!h || !ms) return V3DM_ERR_INVALID;
+    BatchState* b = ctx_batch(h);
+    if (!b) return ctx_fail(h, V3DM_ERR_STATE, "no batch");
+    *ms = b->lastMs;
+    return V3DM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// Batched matching: many image pairs of uneven sizes in one pass over a device work queue.
+//
+// Device layout of a batch (all buffers owned by the handle, sized for the batch):
+//  * images: the descriptors of every image, FP32 rows of 128 floats; image m starts at
+//    row imgBase[m], a multiple of kBatchRowAlign, and its padding rows are zero.  A BF16
+//    copy uses the same row numbering.
+//  * query space: every (pair, side) is a segment of query rows; segment p (< nPairs) holds
+//    the left queries of pair p, segment nPairs + p the right queries.  Segments start at
+//    multiples of kBatchRowAlign; segOfBlock[q / kBatchRowAlign] names the segment of query
+//    row q.  The per-query decisions (sel) and flags live in this space.
+//  * work items: int4 records in a device array, taken in order by persistent work-groups
+//    through an atomic counter; the host sorts them by decreasing cost.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "v3dm_device.h"
+#include "v3dm_short.h"
+
+namespace v3dm {
+
+constexpr int kBatchRowAlign = 256;
+constexpr int kBatchQueues = 4;  // work counters: 0 plain FP32 pass, 1 BF16 / learned pass
+// Counters of a batch (an int array of kBatchCounters): queue heads, open-query count,
+// rescored-candidate count, totals published by the gather kernel.
+constexpr int kBCntFlag = 4;
+constexpr int kBCntRescored = 5;
+constexpr int kBCntPairs = 6;
+constexpr int kBCntFlagDone = 7;
+constexpr int kBCntRescoredDone = 8;
+constexpr int kBatchCounters = 16;
+
+struct BatchSeg {
+    int qOff;     // first query row of the segment
+    int n;        // queries (feature count of the query image, rounded down to 32)
+    int img;      // query image
+    int oImg;     // candidate image
+    int nO;       // candidates
+    int pair;
+    int partOff;  // first partial of the segment (Top2 units for the FP32 form, Cand2 for shortlists)
+    int nParts;   // partials per query
+    int ld;       // stride between the partials of one query (padded n)
+    int pad0, pad1, pad2;
+};
+
+// Plain FP32 item: x = pair, y = left block (128 rows), z = first tile, w = last tile (excl.)
+// Shortlist item:  x = segment, y = query block (256 rows), z = first tile, w = last tile
+// (64-row tiles of candidates in both forms).
+
+struct BatchDev {
+    float const* img;
+    __bf16 const* imgB;
+    int const* imgBase;
+    float const* norm;            // per image row
+    unsigned int const* maxNorm;  // per image (float bits)
+    BatchSeg const* seg;
+    int const* segOfBlock;
+    int nPairs;
+    int qRows;                    // size of the query space
+};
+
+// kernels of the plain FP32 form (v3dm_fp32.hip)
+void launch_batch_fused_fp32(BatchDev const& b, int4 const* items, int nItems, int* queue, Top2* part, int grid,
+                             hipStream_t st);
+// shortlist pass (v3dm_short.hip)
+void launch_batch_pass_bf16(BatchDev const& b, int4 const* items, int nItems, int* queue, Cand2* part, int grid,
+                            hipStream_t st);
+void launch_batch_rescore(BatchDev const& b, Cand2 const* part, float winRel, float minScore, float minRatio,
+                          int* sel, int* counters, hipStream_t st);
+// selection and bookkeeping (v3dm_batch.hip)
+void launch_batch_reset(int* counters, int* pairCount, int nPairs, hipStream_t st);
+void launch_batch_norms(float const* img, int const* rowImg, int rows, float* norm, unsigned int* maxNorm,
+                        hipStream_t st);
+void launch_batch_merge_decide(BatchDev const& b, Top2 const* part, float epsRel, float minScore, float minRatio,
+                               int* sel, int* flagList, int* counters, hipStream_t st);
+void launch_batch_recheck(BatchDev const& b, int const* flagList, int* counters, float minScore, float minRatio,
+                          int* sel, int grid, hipStream_t st);
+void launch_batch_mutual(BatchDev const& b, int const* sel, int leftRows, int2* tmp, int* blockCount, int2* pairs,
+                         int* pairCount, int* counters, hipStream_t st);
+
+}  // namespace v3dm

@@ -1,0 +1,346 @@
+// Selection and bookkeeping kernels of batched matching: per-row norms, merge of the FP32
+// partial top-2 with certified decisions, exact FP64 re-check of open queries, and the
+// mutual check with stable compaction of every pair's matches.
+//
+// Per pair these compute what V3D_CUDA::SIFT_Matching::findPutativeMatches does
+// (V3D/CUDA/v3d_cudamatching.cu:522-552) for that pair alone, as the reference's callers
+// do one pair at a time (V3D/Apps/SfM/match_sequence.cpp:165-382).
+#include "v3dm_batch.h"
+#include "v3dm_kernels.h"
+
+namespace v3dm {
+
+__global__ void k_batch_reset(int* __restrict__ counters, int* __restrict__ pairCount, int nPairs)
+{
+    int const t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < kBatchCounters) {
+        // totals of the previous run stay readable until the gather overwrites them
+        if (t != kBCntPairs && t != kBCntFlagDone && t != kBCntRescoredDone) counters[t] = 0;
+    }
+    if (t < nPairs) pairCount[t] = 0;
+}
+
+void launch_batch_reset(int* counters, int* pairCount, int nPairs, hipStream_t st)
+{
+    int const n = nPairs > kBatchCounters ? nPairs : kBatchCounters;
+    hipLaunchKernelGGL(k_batch_reset, dim3((n + 255) / 256), dim3(256), 0, st, counters, pairCount, nPairs);
+}
+
+// Row norms of the image buffer and per-image maximum.  blkImg[row / kBatchRowAlign] =
+// image of the row; a work-group's 64 rows lie in one image (images are aligned to 256
+// rows), so one atomicMax per work-group suffices.
+__global__ __launch_bounds__(256) void k_batch_norms(float const* __restrict__ X, int const* __restrict__ blkImg,
+                                                     int rows, float* __restrict__ norm,
+                                                     unsigned int* __restrict__ maxNorm)
+{
+    __shared__ unsigned int wmax[4];
+    int const lane = threadIdx.x & 63;
+    int const wave = threadIdx.x >> 6;
+    unsigned int mx = 0;
+    for (int k = 0; k < 16; ++k) {
+        int const row = blockIdx.x * 64 + wave * 16 + k;
+        if (row >= rows) break;
+        float2 const v = reinterpret_cast<float2 const*>(X + (size_t)row * kDim)[lane];
+        float s = v.x * v.x + v.y * v.y;
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        float const nr = sqrtf(s);
+        if (lane == 0) norm[row] = nr;
+        mx = max(mx, __float_as_uint(nr));
+    }
+    if (lane == 0) wmax[wave] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0 && blockIdx.x * 64 < rows) {
+        unsigned int const m = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+        atomicMax(&maxNorm[blkImg[(blockIdx.x * 64) / kBatchRowAlign]], m);
+    }
+}
+
+void launch_batch_norms(float const* img, int const* blkImg, int rows, float* norm, unsigned int* maxNorm,
+                        hipStream_t st)
+{
+    if (rows <= 0) return;
+    hipLaunchKernelGGL(k_batch_norms, dim3((rows + 63) / 64), dim3(256), 0, st, img, blkImg, rows, norm, maxNorm);
+}
+
+// 8 lanes per query row: merge the segment's partials, then the certified decision (as
+// the single-pair merge).  Open queries are appended to flagList (query-space rows).
+__global__ __launch_bounds__(256) void k_batch_merge_decide(BatchDev b, Top2 const* __restrict__ part, float epsRel,
+                                                            float minScore, float minRatio, int* __restrict__ sel,
+                                                            int* __restrict__ flagList, int* __restrict__ counters)
+{
+    int const sub = threadIdx.x & 7;
+    int const g = (int)((blockIdx.x * 256u + threadIdx.x) >> 3);
+    if (g >= b.qRows) return;  // uniform over the 8 lanes of a query
+    BatchSeg const sg = b.seg[b.segOfBlock[g / kBatchRowAlign]];
+    int const q = g - sg.qOff;
+    int const np = (q < sg.n) ? sg.nParts : 0;
+    Top2 const* pp = part + sg.partOff + q;
+    Top2 s = top2_init();
+#pragma unroll 4
+    for (int p = sub; p < np; p += 8) s = top2_merge(s, pp[(size_t)p * sg.ld]);
+    s = top2_merge(s, top2_shfl_xor(s, 1));
+    s = top2_merge(s, top2_shfl_xor(s, 2));
+    s = top2_merge(s, top2_shfl_xor(s, 4));
+    if (sub != 0) return;
+    if (q >= sg.n || sg.nO <= 0) {
+        sel[g] = -1;
+        return;
+    }
+    float const nq = b.norm[b.imgBase[sg.img] + q];
+    float const mo = __uint_as_float(b.maxNorm[sg.oImg]);
+    float const eps = epsRel * nq * mo + kEpsAbs;
+    int const d = (epsRel > 0.0f) ? decide_certified(s, eps, minScore, minRatio) : kFlag;
+    if (d == kFlag) flagList[atomicAdd(&counters[kBCntFlag], 1)] = g;
+    sel[g] = d;
+}
+
+void launch_batch_merge_decide(BatchDev const& b, Top2 const* part, float epsRel, float minScore, float minRatio,
+                               int* sel, int* flagList, int* counters, hipStream_t st)
+{
+    if (b.qRows <= 0) return;
+    hipLaunchKernelGGL(k_batch_merge_decide, dim3((b.qRows + 31) / 32), dim3(256), 0, st, b, part, epsRel, minScore,
+                       minRatio, sel, flagList, counters);
+}
+
+// Exact decision of one open query per work-group (grid-stride over the flag list).
+// Thread t scores candidates t, t + 256, ... as fl32(sum_k double(q_k) * double(c_k)) in
+// ascending k: the candidates pass through LDS 256 rows at a time, in two halves of 64
+// components, and each thread carries its FP64 sum across the halves, so the order of the
+// sum is the oracle's.  The per-thread top-2 states (ascending candidates) are merged by
+// an order-independent merge.
+constexpr int kBRecheckRows = 256;
+constexpr int kBRecheckStride = 68;  // floats per staged half row (64 + 4 pad)
+
+__global__ __launch_bounds__(256) void k_batch_recheck(BatchDev b, int const* __restrict__ flagList,
+                                                       int const* __restrict__ counters, float minScore,
+                                                       float minRatio, int* __restrict__ sel)
+{
+    __shared__ __attribute__((aligned(16))) float cand[kBRecheckRows * kBRecheckStride];
+    __shared__ double qd[kDim];
+    __shared__ Top2 wtop[4];
+    int const tid = threadIdx.x;
+    int const lane = tid & 63;
+    int const wave = tid >> 6;
+    int const nFlag = counters[kBCntFlag];
+    for (int w = blockIdx.x; w < nFlag; w += gridDim.x) {
+        int const g = flagList[w];
+        BatchSeg const sg = b.seg[b.segOfBlock[g / kBatchRowAlign]];
+        int const q = g - sg.qOff;
+        float const* Qd = b.img + (size_t)(b.imgBase[sg.img] + q) * kDim;
+        float const* Cd = b.img + (size_t)b.imgBase[sg.oImg] * kDim;
+        __syncthreads();  // the previous query's LDS contents are consumed
+        if (tid < kDim) qd[tid] = (double)Qd[tid];
+        Top2 s = top2_init();
+        for (int c0 = 0; c0 < sg.nO; c0 += kBRecheckRows) {
+            double acc = 0.0;
+            for (int half = 0; half < 2; ++half) {
+                __syncthreads();  // every thread is done with the staged half
+                // 256 rows x 64 floats = 4096 float4: 16 per thread, row-contiguous
+#pragma unroll 4
+                for (int u = 0; u < 16; ++u) {
+                    int const f = u * 256 + tid;
+                    int const r = f >> 4, c4 = f & 15;
+                    // rows past nO are the image's zero padding (images are padded to
+                    // multiples of 256 rows): in bounds, and masked below
+                    float4 const v =
+                        reinterpret_cast<float4 const*>(Cd + (size_t)(c0 + r) * kDim + half * 64)[c4];
+                    *reinterpret_cast<float4*>(&cand[r * kBRecheckStride + 4 * c4]) = v;
+                }
+                __syncthreads();
+                float const* cr = &cand[tid * kBRecheckStride];
+                double const* qh = qd + half * 64;
+#pragma unroll 8
+                for (int k4 = 0; k4 < 16; ++k4) {
+                    float4 const y = *reinterpret_cast<float4 const*>(cr + 4 * k4);
+                    acc += qh[4 * k4 + 0] * (double)y.x;
+                    acc += qh[4 * k4 + 1] * (double)y.y;
+                    acc += qh[4 * k4 + 2] * (double)y.z;
+                    acc += qh[4 * k4 + 3] * (double)y.w;
+                }
+            }
+            int const j = c0 + tid;
+            float const sc = (float)acc;
+            if (j < sg.nO && sc == sc) top2_push(s, sc, j);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s = top2_merge(s, top2_shfl_xor(s, o));
+        if (lane == 0) wtop[wave] = s;
+        __syncthreads();
+        if (tid == 0) {
+            Top2 t = top2_merge(top2_merge(wtop[0], wtop[1]), top2_merge(wtop[2], wtop[3]));
+            sel[g] = decide_exact(t, minScore, minRatio);
+        }
+    }
+}
+
+void launch_batch_recheck(BatchDev const& b, int const* flagList, int* counters, float minScore, float minRatio,
+                          int* sel, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_batch_recheck, dim3(grid), dim3(256), 0, st, b, flagList, counters, minScore, minRatio, sel);
+}
+
+// Mutual check over the left-query rows [0, leftRows) of the query space (segments 0 ..
+// nPairs-1 in pair order): left i of pair p with sel = l >= 0 is kept iff the right query
+// l of pair p chose i (cu:546-551).  Stable compaction in query-space order keeps every
+// pair's matches contiguous and sorted by left index; pairCount[p] counts them.
+__global__ __launch_bounds__(256) void k_batch_mutual_local(BatchDev b, int const* __restrict__ sel, int leftRows,
+                                                            int2* __restrict__ tmp, int* __restrict__ blockCount,
+                                                            int* __restrict__ pairCount)
+{
+    __shared__ int wsum[4];
+    int const tid = threadIdx.x;
+    int const lane = tid & 63;
+    int const wave = tid >> 6;
+    int const g0 = blockIdx.x * 1024 + 4 * tid;  // 4 rows per thread, one 256-row block per wave
+    int li[4], lr[4];
+    bool m[4];
+    int cnt = 0;
+    int pairId = -1;
+    if (g0 < leftRows) {
+        int const si = b.segOfBlock[g0 / kBatchRowAlign];
+        BatchSeg const sl = b.seg[si];
+        BatchSeg const sr = b.seg[b.nPairs + si];
+        pairId = si;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            int const i = g0 + u - sl.qOff;
+            int const l = (i < sl.n) ? sel[g0 + u] : -1;
+            li[u] = i;
+            lr[u] = l;
+            m[u] = (l >= 0) && (l < sr.n) && (sel[sr.qOff + l] == i);
+            cnt += m[u] ? 1 : 0;
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) m[u] = false;
+    }
+    int incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int const t = __shfl_up(incl, o);
+        incl += (lane >= o) ? t : 0;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    int off = 0;
+    for (int w = 0; w < wave; ++w) off += wsum[w];
+    int pos = blockIdx.x * 1024 + off + incl - cnt;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+        if (m[u]) tmp[pos++] = make_int2(li[u], lr[u]);
+    // one atomic per wave: its 256 rows are one segment
+    int wcnt = cnt;
+    for (int o = 32; o > 0; o >>= 1) wcnt += __shfl_xor(wcnt, o);
+    if (lane == 0 && wcnt > 0) atomicAdd(&pairCount[pairId], wcnt);
+    if (tid == 0) blockCount[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+// Single work-group: exclusive prefix over the block counts (each thread sums a run of
+// blocks, then a work-group scan), then the copy, and the run's totals.
+__global__ __launch_bounds__(1024) void k_batch_mutual_gather(int2 const* __restrict__ tmp,
+                                                              int const* __restrict__ blockCount, int nBlocks,
+                                                              int2* __restrict__ pairs, int* __restrict__ counters)
+{
+    __shared__ int offs[1024];
+    __shared__ int wsum[16];
+    int const tid = threadIdx.x;
+    int const lane = tid & 63;
+    int const wave = tid >> 6;
+    int const per = (nBlocks + 1023) / 1024;
+    int const b0 = tid * per;
+    int mine = 0;
+    for (int k = 0; k < per; ++k)
+        if (b0 + k < nBlocks) mine += blockCount[b0 + k];
+    int incl = mine;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int const t = __shfl_up(incl, o);
+        incl += (lane >= o) ? t : 0;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    int woff = 0;
+    for (int w = 0; w < wave; ++w) woff += wsum[w];
+    offs[tid] = woff + incl - mine;
+    __syncthreads();
+    int total = 0;
+    for (int w = 0; w < 16; ++w) total += wsum[w];
+    // copy: wave-cooperative per block, blocks in order of the owning thread
+    for (int t = wave; t < 1024; t += 16) {
+        int o = offs[t];
+        for (int k = 0; k < per; ++k) {
+            int const blk = t * per + k;
+            if (blk >= nBlocks) break;
+            int const n = blockCount[blk];
+            for (int e = lane; e < n; e += 64) pairs[o + e] = tmp[(size_t)blk * 1024 + e];
+            o += n;
+        }
+    }
+    if (tid == 0) {
+        counters[kBCntPairs] = total;
+        counters[kBCntFlagDone] = counters[kBCntFlag];
+        counters[kBCntRescoredDone] = counters[kBCntRescored];
+    }
+}
+
+void launch_batch_mutual(BatchDev const& b, int const* sel, int leftRows, int2* tmp, int* blockCount, int2* pairs,
+                         int* pairCount, int* counters, hipStream_t st)
+{
+    int const nb = (leftRows + 1023) / 1024;
+    if (nb > 0)
+        hipLaunchKernelGGL(k_batch_mutual_local, dim3(nb), dim3(256), 0, st, b, sel, leftRows, tmp, blockCount,
+                           pairCount);
+    hipLaunchKernelGGL(k_batch_mutual_gather, dim3(1), dim3(1024), 0, st, tmp, blockCount, nb, pairs, counters);
+}
+
+// Input rows -> the aligned image buffer (one wave per destination row).  Row r of the
+// buffer belongs to image m = blkImg[r / kBatchRowAlign]; its local index r - imgBase[m] is
+// a feature when below nImg[m] (the count rounded down to 32) and zero padding otherwise.
+__global__ __launch_bounds__(256) void k_batch_load_u8(uint8_t const* __restrict__ src, int const* __restrict__ srcRow,
+                                                       int const* __restrict__ nImg, int const* __restrict__ blkImg,
+                                                       int const* __restrict__ imgBase, int rows,
+                                                       float* __restrict__ img)
+{
+    int const lane = threadIdx.x & 63;
+    int const row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= rows) return;
+    int const m = blkImg[row / kBatchRowAlign];
+    int const local = row - imgBase[m];
+    float2 v = make_float2(0.0f, 0.0f);
+    if (local < nImg[m])
+        v = dequant_u8_wave(reinterpret_cast<uchar2 const*>(src + (size_t)(srcRow[m] + local) * kDim)[lane]);
+    reinterpret_cast<float2*>(img + (size_t)row * kDim)[lane] = v;
+}
+
+__global__ __launch_bounds__(256) void k_batch_load_f32(float const* __restrict__ src, int const* __restrict__ srcRow,
+                                                        int const* __restrict__ nImg, int const* __restrict__ blkImg,
+                                                        int const* __restrict__ imgBase, int rows,
+                                                        float* __restrict__ img)
+{
+    int const lane = threadIdx.x & 63;
+    int const row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= rows) return;
+    int const m = blkImg[row / kBatchRowAlign];
+    int const local = row - imgBase[m];
+    float2 v = make_float2(0.0f, 0.0f);
+    if (local < nImg[m]) v = reinterpret_cast<float2 const*>(src + (size_t)(srcRow[m] + local) * kDim)[lane];
+    reinterpret_cast<float2*>(img + (size_t)row * kDim)[lane] = v;
+}
+
+void launch_batch_load_u8(uint8_t const* src, int const* srcRow, int const* nImg, int const* blkImg,
+                          int const* imgBase, int rows, float* img, hipStream_t st)
+{
+    if (rows <= 0) return;
+    hipLaunchKernelGGL(k_batch_load_u8, dim3((rows + 3) / 4), dim3(256), 0, st, src, srcRow, nImg, blkImg, imgBase,
+                       rows, img);
+}
+
+void launch_batch_load_f32(float const* src, int const* srcRow, int const* nImg, int const* blkImg,
+                           int const* imgBase, int rows, float* img, hipStream_t st)
+{
+    if (rows <= 0) return;
+    hipLaunchKernelGGL(k_batch_load_f32, dim3((rows + 3) / 4), dim3(256), 0, st, src, srcRow, nImg, blkImg, imgBase,
+                       rows, img);
+}
+
+}  // namespace v3dm

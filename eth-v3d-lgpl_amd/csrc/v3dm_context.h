@@ -63,6 +63,7 @@ struct v3dm_context {
     int kernelLaunches = 0;
 
     v3dm::ExtState* ext = nullptr;
+    v3dm::BatchState* batch = nullptr;
 
     std::string err;
 };

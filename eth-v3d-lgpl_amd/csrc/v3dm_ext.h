@@ -16,10 +16,12 @@ struct GuidedMask {
 };
 
 struct ExtState;
+struct BatchState;
 
 // accessors into the handle (v3dm_api.cpp)
 hipStream_t ctx_stream(v3dm_context* h);
 ExtState*& ctx_ext(v3dm_context* h);
+BatchState*& ctx_batch(v3dm_context* h);
 int ctx_fail(v3dm_context* h, int code, const char* msg);
 int enqueue_plain(v3dm_context* h, float minScore, float minRatio);
 // HIP events around the dominant kernel(s) of a call, when profiling is enabled
@@ -33,5 +35,7 @@ int ext_fallbacks(v3dm_context* h);
 void ext_features_changed(v3dm_context* h, bool left);
 void ext_release(ExtState* e);
 void ext_destroy(ExtState*& e);
+// batched matching (v3dm_batch.cpp)
+void batch_destroy(BatchState*& b);
 
 }  // namespace v3dm

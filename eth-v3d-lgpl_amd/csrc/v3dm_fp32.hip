@@ -7,6 +7,7 @@
 // ever lives in accumulator registers and LDS.
 #include "v3dm_device.h"
 #include "v3dm_kernels.h"
+#include "v3dm_batch.h"
 
 namespace v3dm {
 
@@ -137,22 +138,22 @@ __device__ __forceinline__ void run_tiles_fp32(float const* __restrict__ R, unsi
     }
 }
 
-__global__ __launch_bounds__(256, 2) void k_fused_fp32(float const* __restrict__ L, float const* __restrict__ R,
-                                                       int nL, int nR, int tilesPerSplit,
-                                                       Top2* __restrict__ colPart, int ldc,
-                                                       Top2* __restrict__ rowPart, int ldr, float inf)
+// One work unit: left block `lb` (128 features) against right tiles [split * tilesPerSplit,
+// ...) -> colPart[split * ldc + i], rowPart[lb * ldr + j].
+__device__ __forceinline__ void fused_block(float const* __restrict__ L, float const* __restrict__ R, int nL, int nR,
+                                            int lb, int split, int tilesPerSplit, Top2* __restrict__ colPart,
+                                            int ldc, Top2* __restrict__ rowPart, int ldr, unsigned char* stA,
+                                            unsigned char* stB, float inf)
 {
-    __shared__ __attribute__((aligned(1024))) unsigned char stA[kStageBytes];
-    __shared__ __attribute__((aligned(1024))) unsigned char stB[kStageBytes];
-
-    int const tid = threadIdx.x;
+    // an opaque copy of the thread index: in the persistent kernel it keeps the per-lane
+    // offsets derived from it inside the item loop instead of live across it (spills)
+    int tid = threadIdx.x;
+    __asm__ volatile("" : "+v"(tid));
     int const lane = tid & 63;
     int const wave = tid >> 6;
     int const wv = __builtin_amdgcn_readfirstlane(wave);
     int const h = lane >> 5;
     int const c32 = lane & 31;
-    int const lb = blockIdx.x;
-    int const split = blockIdx.y;
     int const i0 = lb * kBI;
     int const nTiles = (nR + kBJ - 1) / kBJ;
     int const t0 = split * tilesPerSplit;
@@ -184,6 +185,49 @@ __global__ __launch_bounds__(256, 2) void k_fused_fp32(float const* __restrict__
 
     colS = top2_merge(colS, top2_shfl_xor(colS, 32));
     if (h == 0 && myI < nL) colPart[(size_t)split * ldc + myI] = colS;
+}
+
+__global__ __launch_bounds__(256, 2) void k_fused_fp32(float const* __restrict__ L, float const* __restrict__ R,
+                                                       int nL, int nR, int tilesPerSplit,
+                                                       Top2* __restrict__ colPart, int ldc,
+                                                       Top2* __restrict__ rowPart, int ldr, float inf)
+{
+    __shared__ __attribute__((aligned(1024))) unsigned char stA[kStageBytes];
+    __shared__ __attribute__((aligned(1024))) unsigned char stB[kStageBytes];
+    fused_block(L, R, nL, nR, blockIdx.x, blockIdx.y, tilesPerSplit, colPart, ldc, rowPart, ldr, stA, stB, inf);
+}
+
+// Persistent form over a batch: each work-group takes items from the queue until it runs
+// dry.  Item = (pair, left block, split, tiles per split); the left queries of pair p are
+// segment p, its right queries segment nPairs + p.
+__global__ __launch_bounds__(256, 2) void k_batch_fused_fp32(BatchDev b, int4 const* __restrict__ items, int nItems,
+                                                             int* __restrict__ queue, Top2* __restrict__ part,
+                                                             float inf)
+{
+    __shared__ __attribute__((aligned(1024))) unsigned char stA[kStageBytes];
+    __shared__ __attribute__((aligned(1024))) unsigned char stB[kStageBytes];
+    __shared__ int sItem;
+    for (;;) {
+        if (threadIdx.x == 0) sItem = atomicAdd(queue, 1);
+        __syncthreads();  // also: every wave is done with the stages of the previous item
+        int const it = __builtin_amdgcn_readfirstlane(sItem);  // uniform: keeps the item in SGPRs
+        __syncthreads();
+        if (it >= nItems) break;
+        int4 const w = items[it];
+        BatchSeg const sl = b.seg[w.x];
+        BatchSeg const sr = b.seg[b.nPairs + w.x];
+        float const* L = b.img + (size_t)b.imgBase[sl.img] * kDim;
+        float const* R = b.img + (size_t)b.imgBase[sr.img] * kDim;
+        fused_block(L, R, sl.n, sr.n, w.y, w.z, w.w, part + sl.partOff, sl.ld, part + sr.partOff, sr.ld, stA, stB,
+                    inf);
+    }
+}
+
+void launch_batch_fused_fp32(BatchDev const& b, int4 const* items, int nItems, int* queue, Top2* part, int grid,
+                             hipStream_t st)
+{
+    if (nItems <= 0) return;
+    hipLaunchKernelGGL(k_batch_fused_fp32, dim3(grid), dim3(256), 0, st, b, items, nItems, queue, part, INFINITY);
 }
 
 FusedPlan plan_fused_fp32(int nL, int nR, int targetGroups)

@@ -347,21 +347,14 @@ void launch_mutual_compact(int const* colMax, int const* rowMax, int nL, int nR,
 }
 
 // ------------------------------------------------------------------------------ uint8 dequantisation
-// One wave per descriptor, 2 components per lane.  The sum of squares of integers <= 255
-// over 128 terms stays below 2^24, so it is exact in FP32 in any order; the reciprocal
-// square root is formed as 1/sqrt with IEEE rounding, as SIFT_Feature::normalizeDescriptor
-// (V3D/Base/v3d_feature.h:30-38) does.
+// One wave per descriptor, 2 components per lane (dequant_u8_wave).
 __global__ __launch_bounds__(256) void k_dequant_u8(uint8_t const* __restrict__ q, int n, float* __restrict__ out)
 {
     int const lane = threadIdx.x & 63;
     int const row = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= n) return;
     uchar2 const v = reinterpret_cast<uchar2 const*>(q + (size_t)row * kDim)[lane];
-    float const a = (float)v.x, b = (float)v.y;
-    float s = a * a + b * b;
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    float const denom = __fdiv_rn(1.0f, __fsqrt_rn(s));
-    reinterpret_cast<float2*>(out + (size_t)row * kDim)[lane] = make_float2(a * denom, b * denom);
+    reinterpret_cast<float2*>(out + (size_t)row * kDim)[lane] = dequant_u8_wave(v);
 }
 
 void launch_dequant_u8(uint8_t const* q, int n, float* out, hipStream_t st)

@@ -10,6 +10,7 @@
 #include "v3dm_device.h"
 #include "v3dm_kernels.h"
 #include "v3dm_short.h"
+#include "v3dm_batch.h"
 
 namespace v3dm {
 
@@ -158,21 +159,20 @@ __device__ __forceinline__ void run_tiles_bf16(__bf16 const* __restrict__ C, uns
 // with its 6 low mantissa bits replaced by the position (block*16 + r) inside the tile,
 // so that a med3 pair updates value and position together; the tile of each of the two
 // keys is tracked once per tile.  Masked candidates score -1e30.
-__global__ __launch_bounds__(256, 2) void k_pass_bf16(__bf16 const* __restrict__ Q, int nQ,
-                                                      __bf16 const* __restrict__ C, int nC, int tilesPerSplit,
-                                                      Cand2* __restrict__ part, int ldq, float inf)
+// One work unit: query block `qblk` (256 queries) against candidate tiles [split *
+// tilesPerSplit, ...) -> part[((split * ldq) + q) * 2 + h].
+__device__ __forceinline__ void pass_bf16_block(__bf16 const* __restrict__ Q, int nQ, __bf16 const* __restrict__ C,
+                                                int nC, int qblk, int split, int tilesPerSplit,
+                                                Cand2* __restrict__ part, int ldq, unsigned char* stA,
+                                                unsigned char* stB, float inf)
 {
-    __shared__ __attribute__((aligned(1024))) unsigned char stA[kStageBytesB];
-    __shared__ __attribute__((aligned(1024))) unsigned char stB[kStageBytesB];
-
     int const tid = threadIdx.x;
     int const lane = tid & 63;
     int const wave = tid >> 6;
     int const wv = __builtin_amdgcn_readfirstlane(wave);
     int const h = lane >> 5;
     int const c32 = lane & 31;
-    int const q0 = blockIdx.x * kSQ + wave * 64;
-    int const split = blockIdx.y;
+    int const q0 = qblk * kSQ + wave * 64;
     int const nTiles = (nC + kST - 1) / kST;
     int const t0 = split * tilesPerSplit;
     int const t1 = min(t0 + tilesPerSplit, nTiles);
@@ -217,6 +217,45 @@ __global__ __launch_bounds__(256, 2) void k_pass_bf16(__bf16 const* __restrict__
     }
 }
 
+__global__ __launch_bounds__(256, 2) void k_pass_bf16(__bf16 const* __restrict__ Q, int nQ,
+                                                      __bf16 const* __restrict__ C, int nC, int tilesPerSplit,
+                                                      Cand2* __restrict__ part, int ldq, float inf)
+{
+    __shared__ __attribute__((aligned(1024))) unsigned char stA[kStageBytesB];
+    __shared__ __attribute__((aligned(1024))) unsigned char stB[kStageBytesB];
+    pass_bf16_block(Q, nQ, C, nC, blockIdx.x, blockIdx.y, tilesPerSplit, part, ldq, stA, stB, inf);
+}
+
+// Persistent form over a batch: each work-group takes items from the queue until it runs
+// dry.  Item = (segment, query block, split, tiles per split).
+__global__ __launch_bounds__(256, 2) void k_batch_pass_bf16(BatchDev b, int4 const* __restrict__ items, int nItems,
+                                                            int* __restrict__ queue, Cand2* __restrict__ part,
+                                                            float inf)
+{
+    __shared__ __attribute__((aligned(1024))) unsigned char stA[kStageBytesB];
+    __shared__ __attribute__((aligned(1024))) unsigned char stB[kStageBytesB];
+    __shared__ int sItem;
+    for (;;) {
+        if (threadIdx.x == 0) sItem = atomicAdd(queue, 1);
+        __syncthreads();  // also: every wave is done with the stages of the previous item
+        int const it = __builtin_amdgcn_readfirstlane(sItem);  // uniform: keeps the item in SGPRs
+        __syncthreads();
+        if (it >= nItems) break;
+        int4 const w = items[it];
+        BatchSeg const sg = b.seg[w.x];
+        __bf16 const* Q = b.imgB + (size_t)b.imgBase[sg.img] * kDim;
+        __bf16 const* C = b.imgB + (size_t)b.imgBase[sg.oImg] * kDim;
+        pass_bf16_block(Q, sg.n, C, sg.nO, w.y, w.z, w.w, part + (size_t)sg.partOff * 2, sg.ld, stA, stB, inf);
+    }
+}
+
+void launch_batch_pass_bf16(BatchDev const& b, int4 const* items, int nItems, int* queue, Cand2* part, int grid,
+                            hipStream_t st)
+{
+    if (nItems <= 0) return;
+    hipLaunchKernelGGL(k_batch_pass_bf16, dim3(grid), dim3(256), 0, st, b, items, nItems, queue, part, INFINITY);
+}
+
 ShortPlan plan_short(int nQ, int nC, int targetGroups)
 {
     ShortPlan p;
@@ -258,33 +297,16 @@ __device__ __forceinline__ void wave_sync()
 }
 constexpr int kRescoreRow = 132;  // floats per staged row (528 bytes: conflict-free b128 reads)
 
-__global__ __launch_bounds__(256) void k_rescore(float const* __restrict__ L, float const* __restrict__ R, int nL,
-                                                 int nR, Cand2 const* __restrict__ partL, int nPartL, int ldqL,
-                                                 Cand2 const* __restrict__ partR, int nPartR, int ldqR,
-                                                 float const* __restrict__ normL, float const* __restrict__ normR,
-                                                 unsigned int const* __restrict__ maxNormBits, float winRel,
-                                                 float minScore, float minRatio, int* __restrict__ colMax,
-                                                 int* __restrict__ rowMax, int* __restrict__ counters)
+// Exact rescoring of one query by one wave; st = the wave's LDS rows, list = its 64-entry
+// candidate list.  Returns the decision; nWinOut = candidates rescored.
+__device__ __forceinline__ int rescore_query(float const* __restrict__ Qd, float const* __restrict__ Cd,
+                                             Cand2 const* __restrict__ part, size_t ldq, int q, int nEnt, float nq,
+                                             float mo, float winRel, float minScore, float minRatio, float* st,
+                                             int* list, int& nWinOut)
 {
-    __shared__ __attribute__((aligned(16))) float rows[4][(1 + kRescoreBatch) * kRescoreRow];
-    __shared__ int list[4][64];
-    __shared__ int wsum[4];
     int const lane = threadIdx.x & 63;
-    int const wave = threadIdx.x >> 6;
-    int const g = blockIdx.x * 4 + wave;
-    bool const valid = g < nL + nR;
-    int const gq = valid ? g : 0;
-    bool const left = gq < nL;
-    int const q = left ? gq : gq - nL;
-    Cand2 const* part = left ? partL : partR;
-    int const nEnt = valid ? 2 * (left ? nPartL : nPartR) : 0;  // (split, half) lists, <= 64
-    size_t const ldq = left ? (size_t)ldqL : (size_t)ldqR;
-    float const* Qd = (left ? L : R) + (size_t)q * kDim;
-    float const* Cd = left ? R : L;
-    float* st = rows[wave];
-
     // the query row -> LDS (row 0)
-    if (valid && lane < 32) *reinterpret_cast<float4*>(&st[4 * lane]) = reinterpret_cast<float4 const*>(Qd)[lane];
+    if (lane < 32) *reinterpret_cast<float4*>(&st[4 * lane]) = reinterpret_cast<float4 const*>(Qd)[lane];
 
     Cand2 c = {-INFINITY, -1, -INFINITY, -1};
     if (lane < nEnt) c = part[((size_t)(lane >> 1) * ldq + q) * 2 + (lane & 1)];
@@ -295,8 +317,6 @@ __global__ __launch_bounds__(256) void k_rescore(float const* __restrict__ L, fl
         a2 = fmaxf(fminf(a1, b1), fmaxf(a2, b2));
         a1 = fmaxf(a1, b1);
     }
-    float const nq = valid ? (left ? normL[q] : normR[q]) : 0.0f;
-    float const mo = __uint_as_float(maxNormBits[left ? 1 : 0]);
     float const thr = a2 - winRel * nq * mo;
 
     // compact the window candidates (each lane contributes up to 2, in lane order)
@@ -305,8 +325,8 @@ __global__ __launch_bounds__(256) void k_rescore(float const* __restrict__ L, fl
     unsigned long long const b1m = __ballot(in1), b2m = __ballot(in2);
     unsigned long long const lt = (1ull << lane) - 1ull;
     int const pos = __popcll(b1m & lt) + __popcll(b2m & lt);
-    if (in1) list[wave][pos] = c.j1;
-    if (in2) list[wave][pos + (in1 ? 1 : 0)] = c.j2;
+    if (in1) list[pos] = c.j1;
+    if (in2) list[pos + (in1 ? 1 : 0)] = c.j2;
     int const nWin = __popcll(b1m) + __popcll(b2m);
 
     Top2 s = top2_init();
@@ -318,7 +338,7 @@ __global__ __launch_bounds__(256) void k_rescore(float const* __restrict__ L, fl
         for (int u = 0; u < kRescoreBatch / 2; ++u) {
             int const r = 2 * u + (lane >> 5);
             if (r < nb) {
-                int const j = list[wave][k0 + r];
+                int const j = list[k0 + r];
                 *reinterpret_cast<float4*>(&st[(1 + r) * kRescoreRow + 4 * (lane & 31)]) =
                     reinterpret_cast<float4 const*>(Cd + (size_t)j * kDim)[lane & 31];
             }
@@ -337,13 +357,46 @@ __global__ __launch_bounds__(256) void k_rescore(float const* __restrict__ L, fl
                 acc += (double)x.w * (double)y.w;
             }
             float const sc = (float)acc;
-            if (sc == sc) s = top2_merge(s, Top2{sc, kInit, list[wave][k0 + lane]});
+            if (sc == sc) s = top2_merge(s, Top2{sc, kInit, list[k0 + lane]});
         }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s = top2_merge(s, top2_shfl_xor(s, o));
-    if (valid && lane == 0) (left ? colMax : rowMax)[q] = decide_exact(s, minScore, minRatio);
-    if (lane == 0) wsum[wave] = valid ? nWin : 0;
+    wave_sync();  // the list and rows may be reused by the caller's next query
+    nWinOut = nWin;
+    return decide_exact(s, minScore, minRatio);
+}
+
+__global__ __launch_bounds__(256) void k_rescore(float const* __restrict__ L, float const* __restrict__ R, int nL,
+                                                 int nR, Cand2 const* __restrict__ partL, int nPartL, int ldqL,
+                                                 Cand2 const* __restrict__ partR, int nPartR, int ldqR,
+                                                 float const* __restrict__ normL, float const* __restrict__ normR,
+                                                 unsigned int const* __restrict__ maxNormBits, float winRel,
+                                                 float minScore, float minRatio, int* __restrict__ colMax,
+                                                 int* __restrict__ rowMax, int* __restrict__ counters)
+{
+    __shared__ __attribute__((aligned(16))) float rows[4][(1 + kRescoreBatch) * kRescoreRow];
+    __shared__ int list[4][64];
+    __shared__ int wsum[4];
+    int const lane = threadIdx.x & 63;
+    int const wave = threadIdx.x >> 6;
+    int const g = blockIdx.x * 4 + wave;
+    int nWin = 0;
+    if (g < nL + nR) {
+        bool const left = g < nL;
+        int const q = left ? g : g - nL;
+        Cand2 const* part = left ? partL : partR;
+        int const nEnt = 2 * (left ? nPartL : nPartR);  // (split, half) lists, <= 64
+        size_t const ldq = left ? (size_t)ldqL : (size_t)ldqR;
+        float const* Qd = (left ? L : R) + (size_t)q * kDim;
+        float const* Cd = left ? R : L;
+        float const nq = left ? normL[q] : normR[q];
+        float const mo = __uint_as_float(maxNormBits[left ? 1 : 0]);
+        int const d = rescore_query(Qd, Cd, part, ldq, q, nEnt, nq, mo, winRel, minScore, minRatio, rows[wave],
+                                    list[wave], nWin);
+        if (lane == 0) (left ? colMax : rowMax)[q] = d;
+    }
+    if (lane == 0) wsum[wave] = nWin;
     __syncthreads();
     if (threadIdx.x == 0) atomicAdd(&counters[kCntRescored], wsum[0] + wsum[1] + wsum[2] + wsum[3]);
 }
@@ -358,6 +411,46 @@ void launch_rescore(float const* L, float const* R, int nL, int nR, Cand2 const*
     hipLaunchKernelGGL(k_rescore, dim3((n + 3) / 4), dim3(256), 0, st, L, R, nL, nR, partL, nPartL, ldqL, partR,
                        nPartR, ldqR, normL, normR, maxNormBits, winRel, minScore, minRatio, colMax, rowMax,
                        counters);
+}
+
+// Batched form: one wave per query row of the query space (padding rows idle).
+__global__ __launch_bounds__(256) void k_batch_rescore(BatchDev b, Cand2 const* __restrict__ part, float winRel,
+                                                       float minScore, float minRatio, int* __restrict__ sel,
+                                                       int* __restrict__ counters)
+{
+    __shared__ __attribute__((aligned(16))) float rows[4][(1 + kRescoreBatch) * kRescoreRow];
+    __shared__ int list[4][64];
+    __shared__ int wsum[4];
+    int const lane = threadIdx.x & 63;
+    int const wave = threadIdx.x >> 6;
+    int const g = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(wave);
+    int nWin = 0;
+    if (g < b.qRows) {
+        BatchSeg const sg = b.seg[b.segOfBlock[g / kBatchRowAlign]];
+        int const q = g - sg.qOff;
+        if (q < sg.n && sg.nO > 0) {
+            int const row = b.imgBase[sg.img] + q;
+            float const* Qd = b.img + (size_t)row * kDim;
+            float const* Cd = b.img + (size_t)b.imgBase[sg.oImg] * kDim;
+            float const mo = __uint_as_float(b.maxNorm[sg.oImg]);
+            int const d = rescore_query(Qd, Cd, part + (size_t)sg.partOff * 2, (size_t)sg.ld, q, 2 * sg.nParts,
+                                        b.norm[row], mo, winRel, minScore, minRatio, rows[wave], list[wave], nWin);
+            if (lane == 0) sel[g] = d;
+        } else if (lane == 0) {
+            sel[g] = -1;
+        }
+    }
+    if (lane == 0) wsum[wave] = nWin;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(&counters[kBCntRescored], wsum[0] + wsum[1] + wsum[2] + wsum[3]);
+}
+
+void launch_batch_rescore(BatchDev const& b, Cand2 const* part, float winRel, float minScore, float minRatio,
+                          int* sel, int* counters, hipStream_t st)
+{
+    if (b.qRows <= 0) return;
+    hipLaunchKernelGGL(k_batch_rescore, dim3((b.qRows + 3) / 4), dim3(256), 0, st, b, part, winRel, minScore,
+                       minRatio, sel, counters);
 }
 
 }  // namespace v3dm

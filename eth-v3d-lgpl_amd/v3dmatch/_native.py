@@ -66,6 +66,16 @@ SIGNATURES = {
     "v3dm_synchronize": (_I, [_P]),
     "v3dm_set_profiling": (_I, [_P, _I]),
     "v3dm_get_stats": (_I, [_P, C.POINTER(Stats)]),
+    "v3dm_batch_set_images_u8": (_I, [_P, _I, _FP, _FP]),
+    "v3dm_batch_set_images": (_I, [_P, _I, _FP, _FP]),
+    "v3dm_batch_set_pairs": (_I, [_P, _I, _FP, _FP]),
+    "v3dm_batch_match_async": (_I, [_P, _F, _F]),
+    "v3dm_batch_get_matches": (_I, [_P, _FP, _FP, _I, _IP]),
+    "v3dm_batch_match": (_I, [_P, _F, _F, _FP, _FP, _I, _IP]),
+    "v3dm_batch_set_tuning": (_I, [_P, _I, _I, _I, _I]),
+    "v3dm_batch_get_tuning": (_I, [_P, _I, _IP, _IP, _IP, C.POINTER(C.c_float)]),
+    "v3dm_batch_use_graph": (_I, [_P, _I]),
+    "v3dm_batch_last_ms": (_I, [_P, C.POINTER(C.c_float)]),
 }
 
 

@@ -196,6 +196,67 @@ class SIFT_Matching:
         self._check(self._lib.v3dm_get_stats(self._h, C.byref(s)))
         return {f: getattr(s, f) for f, _ in Stats._fields_}
 
+    # -- batched matching ----------------------------------------------------------------
+    def batch_set_images(self, counts, descriptors):
+        """Images of a batch: counts[m] features each, descriptors concatenated row-wise
+        (uint8 SIFT descriptors, dequantised on the GPU, or float32)."""
+        counts = np.ascontiguousarray(counts, dtype=np.int32)
+        d = np.asarray(descriptors)
+        if d.dtype == np.uint8:
+            d = np.ascontiguousarray(d).reshape(-1, 128)
+            fn = self._lib.v3dm_batch_set_images_u8
+        else:
+            d = np.ascontiguousarray(d, dtype=np.float32).reshape(-1, 128)
+            fn = self._lib.v3dm_batch_set_images
+        if d.shape[0] != int(counts.sum()):
+            raise ValueError("descriptor rows do not match the counts")
+        self._check(fn(self._h, counts.shape[0], counts.ctypes.data, d.ctypes.data))
+        self._bcounts = counts.copy()
+
+    def batch_set_pairs(self, left, right):
+        left = np.ascontiguousarray(left, dtype=np.int32)
+        right = np.ascontiguousarray(right, dtype=np.int32)
+        self._check(self._lib.v3dm_batch_set_pairs(self._h, left.shape[0], left.ctypes.data, right.ctypes.data))
+        rounded = self._bcounts - (self._bcounts & 31)
+        self._bcap = max(1, int(rounded[left].sum()))
+        self._bpairs = left.shape[0]
+
+    def batch_match_async(self, minScore=0.9, minRatio=0.8):
+        self._check(self._lib.v3dm_batch_match_async(self._h, C.c_float(minScore), C.c_float(minRatio)))
+
+    def batch_get_matches(self):
+        """(offsets int32 (nPairs+1,), pairs int32 (total, 2)) of the last batch run."""
+        offs = np.empty(self._bpairs + 1, dtype=np.int32)
+        out = np.empty((self._bcap, 2), dtype=np.int32)
+        n = C.c_int(0)
+        self._check(self._lib.v3dm_batch_get_matches(self._h, offs.ctypes.data, out.ctypes.data, self._bcap,
+                                                      C.byref(n)))
+        return offs, out[: n.value].copy()
+
+    def batch_match(self, minScore=0.9, minRatio=0.8):
+        """Matches of every pair of the batch: a list of (n_p, 2) int32 arrays."""
+        self.batch_match_async(minScore, minRatio)
+        offs, pairs = self.batch_get_matches()
+        return [pairs[offs[p]:offs[p + 1]] for p in range(self._bpairs)]
+
+    def batch_set_tuning(self, autotune=True, mode=0, tiles_per_item=0, grid=0):
+        self._check(self._lib.v3dm_batch_set_tuning(self._h, int(bool(autotune)), int(mode), int(tiles_per_item),
+                                                     int(grid)))
+
+    def batch_get_tuning(self, mode=0):
+        t, g, n, ms = C.c_int(0), C.c_int(0), C.c_int(0), C.c_float(0)
+        self._check(self._lib.v3dm_batch_get_tuning(self._h, int(mode), C.byref(t), C.byref(g), C.byref(n),
+                                                     C.byref(ms)))
+        return {"tiles_per_item": t.value, "grid": g.value, "items": n.value, "pass_ms": ms.value}
+
+    def batch_use_graph(self, enable=True):
+        self._check(self._lib.v3dm_batch_use_graph(self._h, int(bool(enable))))
+
+    def batch_last_ms(self):
+        ms = C.c_float(0)
+        self._check(self._lib.v3dm_batch_last_ms(self._h, C.byref(ms)))
+        return ms.value
+
     @property
     def nLeft(self):
         return self._nL

@@ -78,3 +78,23 @@ def batch_sizes(n_pairs, lo=500, hi=16000, frac_lo=0.3, frac_hi=0.6, config=3, b
     _load().v3ds_batch_sizes(C.c_uint64(base), config, int(n_pairs), int(lo), int(hi), float(frac_lo),
                              float(frac_hi), nL.ctypes.data, nR.ctypes.data, fr.ctypes.data)
     return nL, nR, fr
+
+
+def make_batch(n_pairs, lo=500, hi=16000, frac_lo=0.3, frac_hi=0.6, config=3, base=BASE_SEED):
+    """An uneven batch of image pairs in the layout of the batch API: pair p matches image
+    2p (left) against image 2p + 1 (right).  Returns (counts int32 (2P,), uint8 descriptors
+    (sum(counts), 128), left int32 (P,), right int32 (P,), copy fractions (P,))."""
+    nL, nR, fr = batch_sizes(n_pairs, lo, hi, frac_lo, frac_hi, config=config, base=base)
+    counts = np.empty(2 * n_pairs, dtype=np.int32)
+    counts[0::2] = nL
+    counts[1::2] = nR
+    desc = np.empty((int(counts.sum()), 128), dtype=np.uint8)
+    row = 0
+    for p in range(n_pairs):
+        qL, qR, _ = make_pair(int(nL[p]), int(nR[p]), float(fr[p]), config=config, pair=p, base=base)
+        desc[row:row + nL[p]] = qL
+        row += int(nL[p])
+        desc[row:row + nR[p]] = qR
+        row += int(nR[p])
+    left = np.arange(0, 2 * n_pairs, 2, dtype=np.int32)
+    return counts, desc, left, left + 1, fr

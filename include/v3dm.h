@@ -130,6 +130,42 @@ int v3dm_synchronize(v3dm_handle h);
 int v3dm_set_profiling(v3dm_handle h, int enable);
 int v3dm_get_stats(v3dm_handle h, v3dm_stats* out);
 
+/* ---- batched matching (extension) ----------------------------------------------------
+ * Replaces the caller loop that matches many image pairs one at a time
+ * (V3D/Apps/SfM/match_sequence.cpp:165-382, match_using_voctree.cpp:264-609: per pair
+ * setLeftFeatures / setRightFeatures / findPutativeMatches, cu:490-552).  The images of a
+ * batch are uploaded once; a pair list names (left image, right image) per pair; one call
+ * matches every pair with the handle's precision mode (FP32 exact or BF16 shortlist)
+ * through persistent kernels over a device work queue, replayed from a captured hipGraph.
+ * Per pair the result is what findPutativeMatches gives for that pair: counts rounded down
+ * to a multiple of 32, matches (left, right) sorted by left index.  Independent of
+ * v3dm_allocate. */
+/* nImages images, counts[m] features each, descriptors concatenated (image m starts at row
+ * counts[0] + ... + counts[m-1]); uint8 descriptors are dequantised as
+ * SIFT_Feature::normalizeDescriptor (V3D/Base/v3d_feature.h:30-38). */
+int v3dm_batch_set_images_u8(v3dm_handle h, int nImages, const int32_t* counts, const uint8_t* desc);
+int v3dm_batch_set_images(v3dm_handle h, int nImages, const int32_t* counts, const float* desc);
+/* Pair p matches image left[p] (left features) against image right[p]. */
+int v3dm_batch_set_pairs(v3dm_handle h, int nPairs, const int32_t* left, const int32_t* right);
+/* Enqueue the batch (no host synchronisation). */
+int v3dm_batch_match_async(v3dm_handle h, float minScore, float minRatio);
+/* Wait for the last batch run; offsets[0..nPairs] (may be NULL) delimit each pair's matches
+ * in pairs[2*k], pairs[2*k+1]; *nTotal = total count (V3DM_ERR_CAPACITY if cap < total). */
+int v3dm_batch_get_matches(v3dm_handle h, int32_t* offsets, int32_t* pairs, int cap, int* nTotal);
+/* v3dm_batch_match_async followed by v3dm_batch_get_matches. */
+int v3dm_batch_match(v3dm_handle h, float minScore, float minRatio, int32_t* offsets, int32_t* pairs, int cap,
+                     int* nTotal);
+/* Work-item tuning of the persistent kernels: autotune != 0 measures the candidate
+ * (tiles per item, grid) settings on the batch at its first run; tilesPerItem / grid > 0
+ * fix them for the form of `mode` instead. */
+int v3dm_batch_set_tuning(v3dm_handle h, int autotune, int mode, int tilesPerItem, int grid);
+/* The setting in use for `mode` and the autotuner's measured pass time (-1 if not tuned). */
+int v3dm_batch_get_tuning(v3dm_handle h, int mode, int* tilesPerItem, int* grid, int* nItems, float* passMs);
+/* 1 (default): replay batch runs from a captured hipGraph; 0: launch kernel by kernel. */
+int v3dm_batch_use_graph(v3dm_handle h, int enable);
+/* GPU time of the last batch run (HIP events around the whole run), ms. */
+int v3dm_batch_last_ms(v3dm_handle h, float* ms);
+
 #ifdef __cplusplus
 }
 #endif
