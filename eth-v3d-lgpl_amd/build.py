@@ -23,7 +23,8 @@ HIP_FLAGS = [
     # packed FP32 instructions are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-HIP_SOURCES = ["v3dm_fp32.hip", "v3dm_select.hip", "v3dm_short.hip", "v3dm_api.cpp", "v3dm_ext.cpp", "v3dm_batch.cpp"]
+HIP_SOURCES = ["v3dm_fp32.hip", "v3dm_select.hip", "v3dm_short.hip", "v3dm_api.cpp", "v3dm_ext.cpp", "v3dm_batch.cpp",
+               "v3dm_learned.cpp"]
 
 
 def _run(cmd):

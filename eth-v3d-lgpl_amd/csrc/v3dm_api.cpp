@@ -74,6 +74,7 @@ void releaseBuffers(v3dm_context* h)
     freeDev(h->rpart);
     freeDev(h->mtmp);
     freeDev(h->mblock);
+    freeDev(h->stripes);
     freeDev(h->results);
     freeDev(h->dQ);
     freeDev(h->dLeftXY);
@@ -123,6 +124,7 @@ namespace v3dm {
 hipStream_t ctx_stream(v3dm_context* h) { return h->stream; }
 ExtState*& ctx_ext(v3dm_context* h) { return h->ext; }
 BatchState*& ctx_batch(v3dm_context* h) { return h->batch; }
+LearnedState*& ctx_learned(v3dm_context* h) { return h->learned; }
 int ctx_fail(v3dm_context* h, int code, const char* msg) { return fail(h, code, "%s", msg); }
 
 // Enqueue the plain FP32 exact path for the current features; leaves colMax/rowMax
@@ -163,7 +165,7 @@ int enqueue_plain(v3dm_context* h, float minScore, float minRatio)
         launch_recheck_exact(h->dL, h->dR, nL, nR, h->flagList, h->capL, h->results, minScore, minRatio,
                              h->colMax, h->rowMax, h->rpart, kRecheckGrid, st);
     }
-    launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mtmp, h->mblock,
+    launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mtmp, h->mblock, h->stripes,
                           reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st);
     HIPCHECK(h, hipGetLastError());
     return V3DM_OK;
@@ -206,6 +208,7 @@ void v3dm_destroy(v3dm_handle h)
     releaseBuffers(h);
     ext_destroy(h->ext);
     batch_destroy(h->batch);
+    learned_destroy(h->learned);
     for (auto e : h->evStart) (void)hipEventDestroy(e);
     for (auto e : h->evStop) (void)hipEventDestroy(e);
     if (h->ownStream) (void)hipStreamDestroy(h->ownStream);
@@ -247,6 +250,8 @@ int v3dm_allocate(v3dm_handle h, int nMaxLeft, int nMaxRight, int useGuidedMatch
     HIPCHECK(h, hipMalloc(&h->rpart, (cl + cr) * kRecheckMaxChunks * sizeof(Top2)));
     HIPCHECK(h, hipMalloc(&h->mtmp, cl * sizeof(int2)));
     HIPCHECK(h, hipMalloc(&h->mblock, (cl / 1024 + 2) * sizeof(int)));
+    HIPCHECK(h, hipMalloc(&h->stripes, kStripes * kStripeStride * sizeof(int)));
+    HIPCHECK(h, hipMemsetAsync(h->stripes, 0, kStripes * kStripeStride * sizeof(int), h->stream));
     h->pairCap = (int)std::min(cl, cr);
     HIPCHECK(h, hipMalloc(&h->results, (kNumCounters + 2 * (size_t)h->pairCap) * sizeof(int)));
     HIPCHECK(h, hipMemsetAsync(h->results, 0, (kNumCounters + 2 * (size_t)h->pairCap) * sizeof(int), h->stream));
@@ -377,6 +382,21 @@ int v3dm_set_certification(v3dm_handle h, float epsRel)
     return V3DM_OK;
 }
 
+int v3dm_set_learned_topk(v3dm_handle h, int k)
+{
+    if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
+    if (k < 1 || k > 32) return fail(h, V3DM_ERR_INVALID, "topK must be in [1, 32]");
+    h->learnedTopK = k;
+    return V3DM_OK;
+}
+
+int v3dm_set_derived_cache(v3dm_handle h, int enable)
+{
+    if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
+    h->cacheDerived = enable != 0;
+    return V3DM_OK;
+}
+
 int v3dm_set_stream(v3dm_handle h, void* stream)
 {
     if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
@@ -391,6 +411,10 @@ int v3dm_find_putative_matches_async(v3dm_handle h, float minScore, float minRat
     if (!h->allocated) return fail(h, V3DM_ERR_STATE, "findPutativeMatches before allocate");
     HIPCHECK(h, hipSetDevice(h->device));
     h->masks.clear();
+    if (!h->cacheDerived) {
+        ext_features_changed(h, true);
+        ext_features_changed(h, false);
+    }
     int rc;
     if (h->precision == V3DM_PREC_FP32) rc = enqueue_plain(h, minScore, minRatio);
     else rc = ext_enqueue(h, h->precision, minScore, minRatio);
@@ -468,6 +492,12 @@ int v3dm_set_profiling(v3dm_handle h, int enable)
     h->kernelMs = 0.0;
     h->kernelLaunches = 0;
     return V3DM_OK;
+}
+
+int v3dm_get_second_best(v3dm_handle h, int32_t* colSecond, int32_t* rowSecond)
+{
+    if (!h) return V3DM_ERR_INVALID;
+    return ext_second_best(h, colSecond, rowSecond);
 }
 
 int v3dm_get_stats(v3dm_handle h, v3dm_stats* out)

@@ -86,6 +86,7 @@ struct BatchState {
     int* flagList = nullptr;
     int selCap = 0;
     int* counters = nullptr;  // kBatchCounters ints
+    int* stripes = nullptr;   // striped counts, cleared by the gather kernel
     int* pairCount = nullptr;
     int2* tmp = nullptr;
     int* blockCount = nullptr;
@@ -324,6 +325,10 @@ int ensure_query_space(v3dm_context* h, BatchState* b)
     if ((rc = growi(h, b->out, b->outCap, std::max(b->leftRows, 1)))) return rc;
     if ((rc = growi(h, b->pairCount, b->pairCap, std::max(b->nPairs, 1)))) return rc;
     if (!b->counters) BCHECK(h, hipMalloc(&b->counters, kBatchCounters * sizeof(int)));
+    if (!b->stripes) {
+        BCHECK(h, hipMalloc(&b->stripes, kStripes * kStripeStride * sizeof(int)));
+        BCHECK(h, hipMemsetAsync(b->stripes, 0, kStripes * kStripeStride * sizeof(int), h->stream));
+    }
     BCHECK(h, hipMemsetAsync(b->counters, 0, kBatchCounters * sizeof(int), h->stream));
     if (b->hostRes && b->hostResCap < kBatchCounters + b->nPairs) {
         (void)hipHostFree(b->hostRes);
@@ -356,9 +361,10 @@ int enqueue_run(v3dm_context* h, BatchState* b, int mode, float minScore, float 
         launch_batch_recheck(d, b->flagList, b->counters, minScore, minRatio, b->sel, 1024, st);
     } else {
         launch_batch_pass_bf16(d, b->items[1], (int)pl.items.size(), b->counters + 1, b->partB, pl.grid, st);
-        launch_batch_rescore(d, b->partB, kWinRelBf16, minScore, minRatio, b->sel, b->counters, st);
+        launch_batch_rescore(d, b->partB, kWinRelBf16, 0, minScore, minRatio, b->sel, b->stripes, st);
     }
-    launch_batch_mutual(d, b->sel, b->leftRows, b->tmp, b->blockCount, b->out, b->pairCount, b->counters, st);
+    launch_batch_mutual(d, b->sel, b->leftRows, b->tmp, b->blockCount, b->out, b->pairCount, b->stripes,
+                        b->counters, st);
     BCHECK(h, hipGetLastError());
     return V3DM_OK;
 }
@@ -549,7 +555,7 @@ void batch_destroy(BatchState*& b)
     for (void* p : {(void*)b->img, (void*)b->imgB, (void*)b->imgBase, (void*)b->blkImg, (void*)b->nImgD,
                     (void*)b->srcRow, (void*)b->norm, (void*)b->maxNorm, b->upload, (void*)b->segOfBlock,
                     (void*)b->items[0], (void*)b->items[1], (void*)b->segD[0], (void*)b->segD[1], (void*)b->partF,
-                    (void*)b->partB, (void*)b->sel, (void*)b->flagList, (void*)b->counters, (void*)b->pairCount,
+                    (void*)b->partB, (void*)b->sel, (void*)b->flagList, (void*)b->counters, (void*)b->stripes, (void*)b->pairCount,
                     (void*)b->tmp, (void*)b->blockCount, (void*)b->out})
         if (p) (void)hipFree(p);
     if (b->pinned) (void)hipHostFree(b->pinned);

@@ -23,7 +23,6 @@ constexpr int kBatchQueues = 4;  // work counters: 0 plain FP32 pass, 1 BF16 / l
 // Counters of a batch (an int array of kBatchCounters): queue heads, open-query count,
 // rescored-candidate count, totals published by the gather kernel.
 constexpr int kBCntFlag = 4;
-constexpr int kBCntRescored = 5;
 constexpr int kBCntPairs = 6;
 constexpr int kBCntFlagDone = 7;
 constexpr int kBCntRescoredDone = 8;
@@ -64,8 +63,8 @@ void launch_batch_fused_fp32(BatchDev const& b, int4 const* items, int nItems, i
 // shortlist pass (v3dm_short.hip)
 void launch_batch_pass_bf16(BatchDev const& b, int4 const* items, int nItems, int* queue, Cand2* part, int grid,
                             hipStream_t st);
-void launch_batch_rescore(BatchDev const& b, Cand2 const* part, float winRel, float minScore, float minRatio,
-                          int* sel, int* counters, hipStream_t st);
+void launch_batch_rescore(BatchDev const& b, Cand2 const* part, float winRel, int topK, float minScore,
+                          float minRatio, int* sel, int* stripes, hipStream_t st);
 // selection and bookkeeping (v3dm_batch.hip)
 void launch_batch_reset(int* counters, int* pairCount, int nPairs, hipStream_t st);
 void launch_batch_norms(float const* img, int const* rowImg, int rows, float* norm, unsigned int* maxNorm,
@@ -75,6 +74,6 @@ void launch_batch_merge_decide(BatchDev const& b, Top2 const* part, float epsRel
 void launch_batch_recheck(BatchDev const& b, int const* flagList, int* counters, float minScore, float minRatio,
                           int* sel, int grid, hipStream_t st);
 void launch_batch_mutual(BatchDev const& b, int const* sel, int leftRows, int2* tmp, int* blockCount, int2* pairs,
-                         int* pairCount, int* counters, hipStream_t st);
+                         int* pairCount, int* stripes, int* counters, hipStream_t st);
 
 }  // namespace v3dm

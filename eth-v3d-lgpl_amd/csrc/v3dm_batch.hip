@@ -239,13 +239,20 @@ __global__ __launch_bounds__(256) void k_batch_mutual_local(BatchDev b, int cons
 // blocks, then a work-group scan), then the copy, and the run's totals.
 __global__ __launch_bounds__(1024) void k_batch_mutual_gather(int2 const* __restrict__ tmp,
                                                               int const* __restrict__ blockCount, int nBlocks,
-                                                              int2* __restrict__ pairs, int* __restrict__ counters)
+                                                              int* __restrict__ stripes, int2* __restrict__ pairs,
+                                                              int* __restrict__ counters)
 {
     __shared__ int offs[1024];
     __shared__ int wsum[16];
     int const tid = threadIdx.x;
     int const lane = tid & 63;
     int const wave = tid >> 6;
+    int rescored = 0;
+    if (tid < kStripes) {
+        rescored = stripes[tid * kStripeStride];
+        stripes[tid * kStripeStride] = 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) rescored += __shfl_xor(rescored, o);  // wave 0 holds the stripes
     int const per = (nBlocks + 1023) / 1024;
     int const b0 = tid * per;
     int mine = 0;
@@ -279,18 +286,19 @@ __global__ __launch_bounds__(1024) void k_batch_mutual_gather(int2 const* __rest
     if (tid == 0) {
         counters[kBCntPairs] = total;
         counters[kBCntFlagDone] = counters[kBCntFlag];
-        counters[kBCntRescoredDone] = counters[kBCntRescored];
+        counters[kBCntRescoredDone] = rescored;
     }
 }
 
 void launch_batch_mutual(BatchDev const& b, int const* sel, int leftRows, int2* tmp, int* blockCount, int2* pairs,
-                         int* pairCount, int* counters, hipStream_t st)
+                         int* pairCount, int* stripes, int* counters, hipStream_t st)
 {
     int const nb = (leftRows + 1023) / 1024;
     if (nb > 0)
         hipLaunchKernelGGL(k_batch_mutual_local, dim3(nb), dim3(256), 0, st, b, sel, leftRows, tmp, blockCount,
                            pairCount);
-    hipLaunchKernelGGL(k_batch_mutual_gather, dim3(1), dim3(1024), 0, st, tmp, blockCount, nb, pairs, counters);
+    hipLaunchKernelGGL(k_batch_mutual_gather, dim3(1), dim3(1024), 0, st, tmp, blockCount, nb, stripes, pairs,
+                       counters);
 }
 
 // Input rows -> the aligned image buffer (one wave per destination row).  Row r of the

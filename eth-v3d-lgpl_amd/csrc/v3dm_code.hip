@@ -1,0 +1,176 @@
+// Code pass of the learned shortlist: integer MFMA over 32-d codes with the candidate
+// position folded into the accumulator, top-2 per query and partition on integer keys.
+//
+// Codes: the unit-norm network output c quantised to q = round(c * scale) in [-31, 31] and
+// stored as the int8 value 4q (32 bytes per feature).  v_mfma_i32_32x32x32_i8 with the
+// accumulator initialised to r (the element index 0..15 of the D block) gives, for lane l
+// and element r, key = 16 * sum_k q_c q_q + r: the score with the candidate's row inside
+// the block in its low 4 bits (row 8*(r/4) + 4*(l/32) + r%4, query l%32).  Keys order as
+// (score, row), so a top-2 over keys is three integer ops per two elements (max, med3,
+// max3); the block of each kept key is tracked once per block.
+#include "v3dm_device.h"
+#include "v3dm_learned.h"
+#include "v3dm_short.h"
+
+namespace v3dm {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+constexpr int kCQ = 256;   // queries per work-group (4 waves x 2 blocks of 32)
+constexpr int kCT = 64;    // candidates per tile (2 blocks of 32)
+constexpr int kCodeBytes = 32;
+
+// Unit-norm codes -> int8 4q; rows >= n -> zero.  One thread per row.
+__global__ __launch_bounds__(256) void k_code_quant(float const* __restrict__ Z, int n, int rows, float scale,
+                                                    signed char* __restrict__ codes)
+{
+    int const r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= rows) return;
+    float z[kCode];
+    float s = 0.0f;
+#pragma unroll
+    for (int e = 0; e < kCode; ++e) {
+        z[e] = r < n ? Z[(size_t)r * kCode + e] : 0.0f;
+        s += z[e] * z[e];
+    }
+    float const inv = scale / fmaxf(sqrtf(s), 1e-12f);
+    int w[kCode / 4];
+#pragma unroll
+    for (int g = 0; g < kCode / 4; ++g) {
+        unsigned int packed = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            float const v = z[4 * g + e] * inv;
+            int q = (v == v) ? (int)rintf(fminf(fmaxf(v, -31.0f), 31.0f)) : 0;
+            packed |= ((unsigned int)(q * 4) & 0xffu) << (8 * e);
+        }
+        w[g] = (int)packed;
+    }
+    int4* out = reinterpret_cast<int4*>(codes + (size_t)r * kCodeBytes);
+    out[0] = make_int4(w[0], w[1], w[2], w[3]);
+    out[1] = make_int4(w[4], w[5], w[6], w[7]);
+}
+
+void launch_code_quant(float const* Z, int n, int rows, float scale, signed char* codes, hipStream_t st)
+{
+    if (rows <= 0) return;
+    hipLaunchKernelGGL(k_code_quant, dim3((rows + 255) / 256), dim3(256), 0, st, Z, n, rows, scale, codes);
+}
+
+struct KeyStateI {
+    int k1, k2;
+    int b1, b2;  // block (32-candidate) of k1 / k2
+};
+
+// One 32x32 block of keys (16 per lane) into the running top-2; blk = block index.
+template <bool kFull>
+__device__ __forceinline__ void key_block_update(KeyStateI& s, v16i const& d, int blk, int jBase, int nC, int h)
+{
+    int const o1 = s.k1, o2 = s.k2;
+    int k1 = o1, k2 = o2;
+#pragma unroll
+    for (int r = 0; r < 16; r += 2) {
+        int x = d[r], y = d[r + 1];
+        if (!kFull) {
+            int const jx = jBase + 8 * (r >> 2) + 4 * h + (r & 3);
+            x = (jx < nC) ? x : INT_MIN;
+            y = (jx + 1 < nC) ? y : INT_MIN;
+        }
+        // t = med3(x, y, k1), the second largest of {k1, x, y}; written as an instruction so
+        // that the max(x, y) inside the generic pattern is not shared with the k1 update,
+        // which then folds into one v_max3_i32
+        int t;
+        __asm__("v_med3_i32 %0, %1, %2, %3" : "=v"(t) : "v"(x), "v"(y), "v"(k1));
+        k1 = max(max(k1, x), y);
+        k2 = max(k2, t);  // two consecutive pairs fold into v_max3_i32
+    }
+    bool const c1 = k1 != o1;
+    bool const c2 = k2 != o2;
+    s.b2 = (c1 && k2 == o1) ? s.b1 : (c2 ? blk : s.b2);
+    s.b1 = c1 ? blk : s.b1;
+    s.k1 = k1;
+    s.k2 = k2;
+}
+
+template <bool kFull>
+__device__ __forceinline__ void code_tile(signed char const* __restrict__ C, int t, v4i const (&bq)[2],
+                                          v16i const& pos, KeyStateI (&st)[2], int nC, int h, int c32)
+{
+    v4i a[2];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+        a[cb] = *reinterpret_cast<v4i const*>(C + (size_t)(t * kCT + cb * 32 + c32) * kCodeBytes + 16 * h);
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+        int const blk = 2 * t + cb;
+#pragma unroll
+        for (int qb = 0; qb < 2; ++qb) {
+            v16i const d = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[cb], bq[qb], pos, 0, 0, 0);
+            key_block_update<kFull>(st[qb], d, blk, blk * 32, nC, h);
+        }
+    }
+}
+
+// One work unit: query block qblk (256 queries) against candidate tiles [split *
+// tilesPerSplit, ...) -> part[((split * ldq) + q) * 2 + h] (code scores as floats).
+__device__ __forceinline__ void code_block(signed char const* __restrict__ Q, int nQ,
+                                           signed char const* __restrict__ C, int nC, int qblk, int split,
+                                           int tilesPerSplit, Cand2* __restrict__ part, int ldq)
+{
+    int const lane = threadIdx.x & 63;
+    int const wave = threadIdx.x >> 6;
+    int const h = lane >> 5;
+    int const c32 = lane & 31;
+    int const q0 = qblk * kCQ + wave * 64;
+    int const nTiles = (nC + kCT - 1) / kCT;
+    int const t0 = split * tilesPerSplit;
+    int const t1 = min(t0 + tilesPerSplit, nTiles);
+
+    v4i bq[2];
+#pragma unroll
+    for (int qb = 0; qb < 2; ++qb)
+        bq[qb] = *reinterpret_cast<v4i const*>(Q + (size_t)(q0 + qb * 32 + c32) * kCodeBytes + 16 * h);
+    v16i pos;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) pos[r] = r;
+    KeyStateI st[2];
+#pragma unroll
+    for (int qb = 0; qb < 2; ++qb) st[qb] = KeyStateI{INT_MIN, INT_MIN, -1, -1};
+
+    int const tFull = (t1 * kCT > nC) ? t1 - 1 : t1;  // the last tile may be partial
+    for (int t = t0; t < tFull; ++t) code_tile<true>(C, t, bq, pos, st, nC, h, c32);
+    if (tFull < t1) code_tile<false>(C, tFull, bq, pos, st, nC, h, c32);
+
+#pragma unroll
+    for (int qb = 0; qb < 2; ++qb) {
+        int const q = q0 + qb * 32 + c32;
+        if (q >= nQ) continue;
+        Cand2 c;
+        int const r1 = st[qb].k1 & 15, r2 = st[qb].k2 & 15;
+        bool const ok1 = st[qb].b1 >= 0 && st[qb].k1 != INT_MIN;
+        bool const ok2 = st[qb].b2 >= 0 && st[qb].k2 != INT_MIN;
+        c.v1 = ok1 ? (float)(st[qb].k1 >> 4) : -INFINITY;
+        c.v2 = ok2 ? (float)(st[qb].k2 >> 4) : -INFINITY;
+        c.j1 = ok1 ? st[qb].b1 * 32 + 8 * (r1 >> 2) + 4 * h + (r1 & 3) : -1;
+        c.j2 = ok2 ? st[qb].b2 * 32 + 8 * (r2 >> 2) + 4 * h + (r2 & 3) : -1;
+        part[((size_t)split * ldq + q) * 2 + h] = c;
+    }
+}
+
+__global__ __launch_bounds__(256, 4) void k_pass_code(signed char const* __restrict__ Q, int nQ,
+                                                   signed char const* __restrict__ C, int nC, int tilesPerSplit,
+                                                   Cand2* __restrict__ part, int ldq)
+{
+    code_block(Q, nQ, C, nC, blockIdx.x, blockIdx.y, tilesPerSplit, part, ldq);
+}
+
+void launch_pass_code(signed char const* Q, int nQ, signed char const* C, int nC, ShortPlan const& p, Cand2* part,
+                      int ldq, hipStream_t st)
+{
+    if (nQ <= 0 || nC <= 0) return;
+    hipLaunchKernelGGL(k_pass_code, dim3(p.queryBlocks, p.splits), dim3(256), 0, st, Q, nQ, C, nC, p.tilesPerSplit,
+                       part, ldq);
+}
+
+}  // namespace v3dm

@@ -21,6 +21,8 @@ struct v3dm_context {
     bool allocated = false;
     int precision = V3DM_PREC_FP32;
     float epsRel = v3dm::kEpsRelFp32;  // 0: every query goes to the exact re-check
+    int learnedTopK = 8;               // candidates per query rescored in the learned mode
+    bool cacheDerived = true;          // keep bf16 / fp8 / code copies while features are unchanged
 
     int nMaxL = 0, nMaxR = 0;  // rounded maxima (cu:444-463)
     int nL = 0, nR = 0;        // current counts (cu:502, cu:518)
@@ -39,6 +41,7 @@ struct v3dm_context {
     v3dm::Top2* rpart = nullptr;       // re-check chunk partials: (capL + capR) x kRecheckMaxChunks
     int2* mtmp = nullptr;        // mutual-check block slices: capL
     int* mblock = nullptr;       // mutual-check block counts
+    int* stripes = nullptr;      // striped counts (kStripes slots), cleared by the gather kernel
     int* results = nullptr;  // counters (kNumCounters ints) followed by int2 pairs
     int* hostStage = nullptr;
     int pairCap = 0;
@@ -64,6 +67,7 @@ struct v3dm_context {
 
     v3dm::ExtState* ext = nullptr;
     v3dm::BatchState* batch = nullptr;
+    v3dm::LearnedState* learned = nullptr;
 
     std::string err;
 };

@@ -1,7 +1,9 @@
-// Precision modes other than the exact FP32 path: the BF16 and FP8 shortlist forms.
+// Precision modes other than the exact FP32 path: the BF16, FP8 and learned-code shortlist
+// forms.
 #include "v3dm_ext.h"
 
 #include "v3dm_context.h"
+#include "v3dm_learned.h"
 #include "v3dm_short.h"
 
 namespace v3dm {
@@ -11,11 +13,20 @@ struct ExtState {
     __bf16* bL = nullptr;
     __bf16* bR = nullptr;
     bool bLValid = false, bRValid = false;
+    int* secL = nullptr;  // exact second best candidate per query (shortlist forms)
+    int* secR = nullptr;
+    bool haveSecond = false;
     unsigned char* f8L = nullptr;  // e4m3 descriptors, capL x 128 / capR x 128
     unsigned char* f8R = nullptr;
     unsigned char* eL = nullptr;  // E8M0 scale byte per descriptor
     unsigned char* eR = nullptr;
     bool f8LValid = false, f8RValid = false;
+    signed char* cL = nullptr;  // learned codes (int8, 32 per feature)
+    signed char* cR = nullptr;
+    float* zL = nullptr;  // network output before quantisation
+    float* zR = nullptr;
+    bool cLValid = false, cRValid = false;
+    float cScale = 0.0f;  // scale the codes were quantised with
     Cand2* partL = nullptr;  // kShortMaxSplits x capL x 2
     Cand2* partR = nullptr;
     int fallbacks = 0;
@@ -45,6 +56,8 @@ int ensure(v3dm_context* h, ExtState*& e)
     EXTCHECK(h, hipMemsetAsync(e->bR, 0, (size_t)cr * kDim * sizeof(__bf16), h->stream));
     EXTCHECK(h, hipMalloc(&e->partL, (size_t)kShortMaxSplits * cl * 2 * sizeof(Cand2)));
     EXTCHECK(h, hipMalloc(&e->partR, (size_t)kShortMaxSplits * cr * 2 * sizeof(Cand2)));
+    EXTCHECK(h, hipMalloc(&e->secL, (size_t)cl * sizeof(int)));
+    EXTCHECK(h, hipMalloc(&e->secR, (size_t)cr * sizeof(int)));
     e->bLValid = e->bRValid = false;
     return V3DM_OK;
 }
@@ -89,10 +102,68 @@ int enqueue_bf16(v3dm_context* h, float minScore, float minRatio)
         launch_pass_bf16(e->bR, nR, e->bL, nL, pr, e->partR, e->capR, st);
         if ((rc = prof_end(h))) return rc;
         launch_rescore(h->dL, h->dR, nL, nR, e->partL, pl.splits, e->capL, e->partR, pr.splits, e->capR, h->normL,
-                       h->normR, h->maxNormBits, kWinRelBf16, minScore, minRatio, h->colMax, h->rowMax, h->results,
-                       st);
+                       h->normR, h->maxNormBits, kWinRelBf16, 0, minScore, minRatio, h->colMax, h->rowMax, e->secL,
+                       e->secR, h->stripes, st);
     }
-    launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mtmp, h->mblock,
+    e->haveSecond = work;
+    launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mtmp, h->mblock, h->stripes,
+                          reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st);
+    EXTCHECK(h, hipGetLastError());
+    return V3DM_OK;
+}
+
+int ensure_codes(v3dm_context* h, ExtState*& e)
+{
+    int rc = ensure(h, e);
+    if (rc) return rc;
+    if (e->cL) return V3DM_OK;
+    EXTCHECK(h, hipMalloc(&e->cL, (size_t)e->capL * kCode));
+    EXTCHECK(h, hipMalloc(&e->cR, (size_t)e->capR * kCode));
+    EXTCHECK(h, hipMalloc(&e->zL, (size_t)e->capL * kCode * sizeof(float)));
+    EXTCHECK(h, hipMalloc(&e->zR, (size_t)e->capR * kCode * sizeof(float)));
+    e->cLValid = e->cRValid = false;
+    return V3DM_OK;
+}
+
+int enqueue_learned(v3dm_context* h, float minScore, float minRatio)
+{
+    MlpWeights const* w = learned_weights(ctx_learned(h));
+    float const scale = learned_code_scale(ctx_learned(h));
+    if (!w || !(scale > 0.0f)) return ctx_fail(h, V3DM_ERR_STATE, "learned network or code scale not set");
+    ExtState*& e = ctx_ext(h);
+    int rc = ensure_codes(h, e);
+    if (rc) return rc;
+    hipStream_t const st = h->stream;
+    int const nL = h->nL, nR = h->nR;
+    bool const work = nL > 0 && nR > 0;
+    if (work) {
+        if (e->cScale != scale) e->cLValid = e->cRValid = false;
+        e->cScale = scale;
+        // embedding: network forward on the FP32 rows (read up to the next multiple of 64,
+        // inside the allocation), then normalise + quantise; code rows >= n are zero
+        if (!e->cLValid) {
+            launch_mlp_forward_f32(h->dL, (nL + 63) / 64 * 64, *w, e->zL, st);
+            launch_code_quant(e->zL, nL, e->capL, scale, e->cL, st);
+        }
+        if (!e->cRValid) {
+            launch_mlp_forward_f32(h->dR, (nR + 63) / 64 * 64, *w, e->zR, st);
+            launch_code_quant(e->zR, nR, e->capR, scale, e->cR, st);
+        }
+        e->cLValid = e->cRValid = true;
+        ShortPlan const pl = plan_short(nL, nR, 512);
+        ShortPlan const pr = plan_short(nR, nL, 512);
+        if ((rc = prof_begin(h))) return rc;
+        launch_pass_code(e->cL, nL, e->cR, nR, pl, e->partL, e->capL, st);
+        if ((rc = prof_end(h))) return rc;
+        if ((rc = prof_begin(h))) return rc;
+        launch_pass_code(e->cR, nR, e->cL, nL, pr, e->partR, e->capR, st);
+        if ((rc = prof_end(h))) return rc;
+        launch_rescore(h->dL, h->dR, nL, nR, e->partL, pl.splits, e->capL, e->partR, pr.splits, e->capR, h->normL,
+                       h->normR, h->maxNormBits, 0.0f, h->learnedTopK, minScore, minRatio, h->colMax, h->rowMax,
+                       e->secL, e->secR, h->stripes, st);
+    }
+    e->haveSecond = work;
+    launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mtmp, h->mblock, h->stripes,
                           reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st);
     EXTCHECK(h, hipGetLastError());
     return V3DM_OK;
@@ -119,10 +190,11 @@ int enqueue_fp8(v3dm_context* h, float minScore, float minRatio)
         launch_pass_fp8(e->f8R, e->eR, nR, e->f8L, e->eL, nL, pr, e->partR, e->capR, st);
         if ((rc = prof_end(h))) return rc;
         launch_rescore(h->dL, h->dR, nL, nR, e->partL, pl.splits, e->capL, e->partR, pr.splits, e->capR, h->normL,
-                       h->normR, h->maxNormBits, kWinRelFp8, minScore, minRatio, h->colMax, h->rowMax, h->results,
-                       st);
+                       h->normR, h->maxNormBits, kWinRelFp8, 0, minScore, minRatio, h->colMax, h->rowMax, e->secL,
+                       e->secR, h->stripes, st);
     }
-    launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mtmp, h->mblock,
+    e->haveSecond = work;
+    launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mtmp, h->mblock, h->stripes,
                           reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st);
     EXTCHECK(h, hipGetLastError());
     return V3DM_OK;
@@ -132,14 +204,27 @@ int enqueue_fp8(v3dm_context* h, float minScore, float minRatio)
 
 bool ext_supports(v3dm_context*, int mode)
 {
-    return mode == V3DM_PREC_FP32 || mode == V3DM_PREC_BF16 || mode == V3DM_PREC_FP8;
+    return mode == V3DM_PREC_FP32 || mode == V3DM_PREC_BF16 || mode == V3DM_PREC_FP8 || mode == V3DM_PREC_LEARNED;
 }
 
 int ext_enqueue(v3dm_context* h, int mode, float minScore, float minRatio)
 {
     if (mode == V3DM_PREC_BF16) return enqueue_bf16(h, minScore, minRatio);
     if (mode == V3DM_PREC_FP8) return enqueue_fp8(h, minScore, minRatio);
+    if (mode == V3DM_PREC_LEARNED) return enqueue_learned(h, minScore, minRatio);
     return ctx_fail(h, V3DM_ERR_UNSUPPORTED, "precision mode not available");
+}
+
+int ext_second_best(v3dm_context* h, int* colSecond, int* rowSecond)
+{
+    ExtState* e = ctx_ext(h);
+    if (!e || !e->haveSecond) return ctx_fail(h, V3DM_ERR_STATE, "no shortlist result");
+    EXTCHECK(h, hipStreamSynchronize(h->stream));
+    if (colSecond && h->nL > 0)
+        EXTCHECK(h, hipMemcpy(colSecond, e->secL, (size_t)h->nL * sizeof(int), hipMemcpyDeviceToHost));
+    if (rowSecond && h->nR > 0)
+        EXTCHECK(h, hipMemcpy(rowSecond, e->secR, (size_t)h->nR * sizeof(int), hipMemcpyDeviceToHost));
+    return V3DM_OK;
 }
 
 int ext_fallbacks(v3dm_context* h)
@@ -152,8 +237,8 @@ void ext_features_changed(v3dm_context* h, bool left)
 {
     ExtState* e = ctx_ext(h);
     if (!e) return;
-    if (left) e->bLValid = e->f8LValid = false;
-    else e->bRValid = e->f8RValid = false;
+    if (left) e->bLValid = e->f8LValid = e->cLValid = false;
+    else e->bRValid = e->f8RValid = e->cRValid = false;
 }
 
 void ext_release(ExtState* e)
@@ -163,8 +248,17 @@ void ext_release(ExtState* e)
     if (e->bR) (void)hipFree(e->bR);
     if (e->partL) (void)hipFree(e->partL);
     if (e->partR) (void)hipFree(e->partR);
+    if (e->secL) (void)hipFree(e->secL);
+    if (e->secR) (void)hipFree(e->secR);
+    e->secL = e->secR = nullptr;
+    e->haveSecond = false;
     for (unsigned char* p : {e->f8L, e->f8R, e->eL, e->eR})
         if (p) (void)hipFree(p);
+    for (void* p : {(void*)e->cL, (void*)e->cR, (void*)e->zL, (void*)e->zR})
+        if (p) (void)hipFree(p);
+    e->cL = e->cR = nullptr;
+    e->zL = e->zR = nullptr;
+    e->cLValid = e->cRValid = false;
     e->f8L = e->f8R = e->eL = e->eR = nullptr;
     e->f8LValid = e->f8RValid = false;
     e->bL = e->bR = nullptr;

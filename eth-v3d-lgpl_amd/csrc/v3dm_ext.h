@@ -17,11 +17,14 @@ struct GuidedMask {
 
 struct ExtState;
 struct BatchState;
+struct LearnedState;
+struct MlpWeights;
 
 // accessors into the handle (v3dm_api.cpp)
 hipStream_t ctx_stream(v3dm_context* h);
 ExtState*& ctx_ext(v3dm_context* h);
 BatchState*& ctx_batch(v3dm_context* h);
+LearnedState*& ctx_learned(v3dm_context* h);
 int ctx_fail(v3dm_context* h, int code, const char* msg);
 int enqueue_plain(v3dm_context* h, float minScore, float minRatio);
 // HIP events around the dominant kernel(s) of a call, when profiling is enabled
@@ -32,10 +35,16 @@ int prof_end(v3dm_context* h);
 bool ext_supports(v3dm_context* h, int mode);
 int ext_enqueue(v3dm_context* h, int mode, float minScore, float minRatio);
 int ext_fallbacks(v3dm_context* h);
+// exact second best candidate of every query of the last shortlist call (host arrays)
+int ext_second_best(v3dm_context* h, int* colSecond, int* rowSecond);
 void ext_features_changed(v3dm_context* h, bool left);
 void ext_release(ExtState* e);
 void ext_destroy(ExtState*& e);
 // batched matching (v3dm_batch.cpp)
 void batch_destroy(BatchState*& b);
+// learned network (v3dm_learned.cpp)
+void learned_destroy(LearnedState*& L);
+MlpWeights const* learned_weights(LearnedState* L);
+float learned_code_scale(LearnedState* L);
 
 }  // namespace v3dm

@@ -11,10 +11,13 @@ constexpr int kMaxSplits = 64;    // right-feature splits of the fused kernel
 constexpr int kCntPairs = 0;      // counters[0]: number of matches
 constexpr int kCntFlag = 1;       // counters[1]: open left queries of the running call (cleared at its end)
 constexpr int kCntFlagDone = 2;   // counters[2]: queries re-checked exactly by the last call
-constexpr int kCntRescored = 3;   // counters[3]: shortlist candidates rescored by the running call
 constexpr int kCntRescoredDone = 4;  // counters[4]: ... by the last call
 constexpr int kCntFlagR = 5;      // counters[5]: open right queries of the running call
 constexpr int kNumCounters = 8;
+// Counts summed by many work-groups go to kStripes slots one cache line apart (atomics on a
+// single address serialise in L2); the gather kernel sums and clears them.
+constexpr int kStripes = 64;
+constexpr int kStripeStride = 32;  // ints (128 bytes)
 constexpr int kRecheckMaxChunks = 64;  // candidate chunks per re-checked query
 constexpr int kRowPad = 128;      // device descriptor buffers hold a multiple of 128 rows
 constexpr int kRecheckGrid = 1024;
@@ -49,7 +52,7 @@ void launch_recheck_exact(float const* L, float const* R, int nL, int nR, int co
                           int grid, hipStream_t st);
 // nL = 0 publishes an empty result.  tmp: nL int2, blockCount: ceil(nL/1024) ints.
 void launch_mutual_compact(int const* colMax, int const* rowMax, int nL, int nR, int2* tmp, int* blockCount,
-                           int2* pairs, int* counters, hipStream_t st);
+                           int* stripes, int2* pairs, int* counters, hipStream_t st);
 // uint8 SIFT descriptors -> floats, as SIFT_Feature::normalizeDescriptor
 // (V3D/Base/v3d_feature.h:30-38): var = sum q^2, d = q * (1/sqrtf(var)), IEEE rounding.
 void launch_dequant_u8(uint8_t const* q, int n, float* out, hipStream_t st);

@@ -1,0 +1,570 @@
+// Learned shortlist network: 128 -> 256 -> 32 ReLU MLP, BF16 MFMA forward and backward
+// kernels with FP32 accumulation, contrastive loss, Adam on FP32 master weights.
+//
+// The reference matcher has no learned stage; this network only proposes candidates, whose
+// scores are then computed exactly.  Row-major parameter vector (FP32 masters):
+//   W1 [256][128] at 0, b1 [256] at 32768, W2 [32][256] at 33024, b2 [32] at 41216.
+//
+// Activation layouts of a minibatch of R rows (padded to Rp, a multiple of 64):
+//   X  bf16 [Rp][128]    XT  bf16 [128][Rp]     HT  bf16 [256][Rp] (after ReLU)
+//   Z  f32  [Rp][32]     dC  f32  [Rp][32]      dZ  bf16 [Rp][32], dZT bf16 [32][Rp]
+//   dZf f32 [Rp][32]     dHT bf16 [256][Rp]
+// The weight-gradient GEMMs reduce over the rows, so they read the transposed activations
+// (contiguous along the reduction); their per-chunk partials are summed by the Adam kernel
+// in a fixed order (no atomics: the gradients are deterministic).
+//
+// MFMA v_mfma_f32_32x32x16_bf16: lane l supplies row (A) / column (B) l%32 and k = 8*(l/32)
+// + 0..7 of the 16-deep step; D element r of lane l is (row 8*(r/4) + 4*(l/32) + r%4,
+// column l%32).
+#include "v3dm_device.h"
+#include "v3dm_learned.h"
+
+namespace v3dm {
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ v16f mfma_bf16(v8bf a, v8bf b, v16f c)
+{
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ v8bf load8(__bf16 const* p) { return *reinterpret_cast<v8bf const*>(p); }
+
+// 4 consecutive rows (r%4 = 0..3) of one column of a D block, as 4 bf16 (8 bytes)
+__device__ __forceinline__ void store4_bf16(__bf16* p, float a, float b, float c, float d)
+{
+    typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+    v4bf v = {(__bf16)a, (__bf16)b, (__bf16)c, (__bf16)d};
+    *reinterpret_cast<v4bf*>(p) = v;
+}
+
+// ------------------------------------------------------------------------------ gather
+// One wave per row: row r of the minibatch is pool row tri[3*(r % B) + r / B] (anchors,
+// then positives, then negatives); rows >= R are zero.
+__global__ __launch_bounds__(256) void k_mlp_gather(float const* __restrict__ pool, int const* __restrict__ tri,
+                                                    int const* __restrict__ order, int orderOff, int B, int Rp,
+                                                    __bf16* __restrict__ X, __bf16* __restrict__ XT)
+{
+    int const lane = threadIdx.x & 63;
+    int const r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= Rp) return;
+    float2 v = make_float2(0.0f, 0.0f);
+    if (r < 3 * B) {
+        int const t = order[orderOff + r % B];
+        int const row = tri[3 * t + r / B];
+        v = reinterpret_cast<float2 const*>(pool + (size_t)row * kDim)[lane];
+    }
+    __bf16 const a = (__bf16)v.x, b = (__bf16)v.y;
+    X[(size_t)r * kDim + 2 * lane] = a;
+    X[(size_t)r * kDim + 2 * lane + 1] = b;
+    XT[(size_t)(2 * lane) * Rp + r] = a;
+    XT[(size_t)(2 * lane + 1) * Rp + r] = b;
+}
+
+// FP32 rows -> bf16 rows (inference input), rows >= n zero.
+__global__ __launch_bounds__(256) void k_mlp_rows_bf16(float const* __restrict__ src, int n, int Rp,
+                                                       __bf16* __restrict__ X)
+{
+    int const lane = threadIdx.x & 63;
+    int const r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= Rp) return;
+    float2 v = make_float2(0.0f, 0.0f);
+    if (r < n) v = reinterpret_cast<float2 const*>(src + (size_t)r * kDim)[lane];
+    X[(size_t)r * kDim + 2 * lane] = (__bf16)v.x;
+    X[(size_t)r * kDim + 2 * lane + 1] = (__bf16)v.y;
+}
+
+// ------------------------------------------------------------------------------ forward
+// 64 rows per work-group.  Layer 1: wave w computes hidden units [64w, 64w + 64) of both
+// 32-row blocks (4 D blocks, 8 k-steps); bias + ReLU; h -> LDS (bf16, [row][unit]) and,
+// when HT is given, -> HT.  Layer 2: waves 0 and 1 compute the 32 codes of row block w
+// (16 k-steps over the units, A from LDS).
+constexpr int kHsStride = 264;  // bf16 per LDS row of h (256 + 8: rows start 16 bytes apart mod 128)
+
+// A-operand rows: bf16 (training minibatch) or FP32 (descriptors, converted in registers)
+__device__ __forceinline__ v8bf load_row8(__bf16 const* p) { return load8(p); }
+__device__ __forceinline__ v8bf load_row8(float const* p)
+{
+    float4 const u = reinterpret_cast<float4 const*>(p)[0];
+    float4 const v = reinterpret_cast<float4 const*>(p)[1];
+    return v8bf{(__bf16)u.x, (__bf16)u.y, (__bf16)u.z, (__bf16)u.w,
+                (__bf16)v.x, (__bf16)v.y, (__bf16)v.z, (__bf16)v.w};
+}
+
+template <typename TIn>
+__global__ __launch_bounds__(256) void k_mlp_forward(TIn const* __restrict__ X, int Rp,
+                                                     __bf16 const* __restrict__ W1b, float const* __restrict__ b1,
+                                                     __bf16 const* __restrict__ W2b, float const* __restrict__ b2,
+                                                     __bf16* __restrict__ HT, float* __restrict__ Z)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 Hs[64 * kHsStride];
+    int const lane = threadIdx.x & 63;
+    int const wave = threadIdx.x >> 6;
+    int const h = lane >> 5;
+    int const c32 = lane & 31;
+    int const row0 = blockIdx.x * 64;
+
+    v16f acc[2][2];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) acc[rb][u] = v16f{};
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+        int const k = ks * 16 + 8 * h;
+        v8bf a[2], b[2];
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) a[rb] = load_row8(X + (size_t)(row0 + rb * 32 + c32) * kDim + k);
+#pragma unroll
+        for (int u = 0; u < 2; ++u) b[u] = load8(W1b + (size_t)((2 * wave + u) * 32 + c32) * kDim + k);
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int u = 0; u < 2; ++u) acc[rb][u] = mfma_bf16(a[rb], b[u], acc[rb][u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        int const unit = (2 * wave + u) * 32 + c32;
+        float const bias = b1[unit];
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) {
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                float v[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) v[e] = fmaxf(acc[rb][u][4 * g + e] + bias, 0.0f);
+                int const rl = rb * 32 + 8 * g + 4 * h;  // first of 4 consecutive rows
+#pragma unroll
+                for (int e = 0; e < 4; ++e) Hs[(rl + e) * kHsStride + unit] = (__bf16)v[e];
+                if (HT) store4_bf16(HT + (size_t)unit * Rp + row0 + rl, v[0], v[1], v[2], v[3]);
+            }
+        }
+    }
+    __syncthreads();
+    if (wave >= 2) return;
+    int const rb = wave;
+    v16f z = v16f{};
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) {
+        int const k = ks * 16 + 8 * h;
+        v8bf const a = *reinterpret_cast<v8bf const*>(&Hs[(rb * 32 + c32) * kHsStride + k]);
+        v8bf const b = load8(W2b + (size_t)c32 * kHid + k);
+        z = mfma_bf16(a, b, z);
+    }
+    float const bias = b2[c32];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        int const row = row0 + rb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+        Z[(size_t)row * kCode + c32] = z[r] + bias;
+    }
+}
+
+// ------------------------------------------------------------------------------ loss
+// 64 anchors per work-group, 4 threads per anchor (quarter q of each 64-row positive tile).
+// Codes are normalised on the fly.  The in-batch hardest positive of anchor t is the
+// argmax over t' != t of c_t . c_{p_t'} (first index on ties).  Gradients with respect to
+// the normalised codes go to dC: anchor and negative rows are owned by their triplet,
+// positive rows also receive the hardest-negative terms of other anchors (atomic adds on a
+// zeroed dC).
+__device__ __forceinline__ float softplus_f(float x) { return x > 20.0f ? x : log1pf(expf(x)); }
+__device__ __forceinline__ float sigmoid_f(float x) { return 1.0f / (1.0f + expf(-x)); }
+
+__global__ __launch_bounds__(256) void k_mlp_loss(float const* __restrict__ Z, int B, float invTau,
+                                                  float* __restrict__ dC, int* __restrict__ hardOut,
+                                                  float* __restrict__ lossPart)
+{
+    __shared__ float tile[64][kCode + 1];
+    __shared__ float wl[4];
+    int const tid = threadIdx.x;
+    int const q = tid & 3;
+    int const al = tid >> 2;
+    int const t = blockIdx.x * 64 + al;
+    bool const valid = t < B;
+
+    // normalised anchor code (8 components per thread, gathered to all 32 by shuffles)
+    float ca[kCode];
+    {
+        int const tt = valid ? t : 0;
+        float s = 0.0f;
+        float part[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            part[e] = Z[(size_t)tt * kCode + 8 * q + e];
+            s += part[e] * part[e];
+        }
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        float const inv = 1.0f / fmaxf(sqrtf(s), 1e-12f);
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                float const v = __shfl(part[e], ((tid & 63) & ~3) | qq) * inv;
+                ca[8 * qq + e] = v;
+            }
+    }
+    float best = -INFINITY;
+    int bestIdx = -1;
+    for (int p0 = 0; p0 < B; p0 += 64) {
+        __syncthreads();
+        {
+            // stage positives p0 .. p0+63, normalised (4 threads per row, 8 components each)
+            int const pr = tid >> 2;
+            int const pi = p0 + pr;
+            float part[8];
+            float s = 0.0f;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                part[e] = (pi < B) ? Z[(size_t)(B + pi) * kCode + 8 * q + e] : 0.0f;
+                s += part[e] * part[e];
+            }
+            s += __shfl_xor(s, 1);
+            s += __shfl_xor(s, 2);
+            float const inv = 1.0f / fmaxf(sqrtf(s), 1e-12f);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) tile[pr][8 * q + e] = part[e] * inv;
+        }
+        __syncthreads();
+        for (int j = q; j < 64; j += 4) {
+            int const pj = p0 + j;
+            if (pj >= B || pj == t) continue;
+            float d = 0.0f;
+#pragma unroll
+            for (int e = 0; e < kCode; ++e) d += ca[e] * tile[j][e];
+            if (d > best) {
+                best = d;
+                bestIdx = pj;
+            }
+        }
+    }
+    // combine the 4 quarters (first index on ties)
+#pragma unroll
+    for (int o = 1; o < 4; o <<= 1) {
+        float const ob = __shfl_xor(best, o);
+        int const oi = __shfl_xor(bestIdx, o);
+        bool const take = ob > best || (ob == best && oi >= 0 && (bestIdx < 0 || oi < bestIdx));
+        best = take ? ob : best;
+        bestIdx = take ? oi : bestIdx;
+    }
+    float lossT = 0.0f;
+    if (valid) {
+        // positive and negative codes of triplet t
+        float cp[kCode], cn[kCode];
+        float sp2 = 0.0f, sn2 = 0.0f;
+#pragma unroll
+        for (int e = 0; e < kCode; ++e) {
+            cp[e] = Z[(size_t)(B + t) * kCode + e];
+            cn[e] = Z[(size_t)(2 * B + t) * kCode + e];
+            sp2 += cp[e] * cp[e];
+            sn2 += cn[e] * cn[e];
+        }
+        float const ip = 1.0f / fmaxf(sqrtf(sp2), 1e-12f), in = 1.0f / fmaxf(sqrtf(sn2), 1e-12f);
+        float sp = 0.0f, sn = 0.0f;
+#pragma unroll
+        for (int e = 0; e < kCode; ++e) {
+            cp[e] *= ip;
+            cn[e] *= in;
+            sp += ca[e] * cp[e];
+            sn += ca[e] * cn[e];
+        }
+        float const xh = (best - sp) * invTau, xn = (sn - sp) * invTau;
+        lossT = softplus_f(xn) + (bestIdx >= 0 ? softplus_f(xh) : 0.0f);
+        float const scale = invTau / (float)B;
+        float const gn = sigmoid_f(xn) * scale;
+        float const gh = bestIdx >= 0 ? sigmoid_f(xh) * scale : 0.0f;
+        // quarter q writes components 8q .. 8q+7
+        float chq[8];
+        if (bestIdx >= 0) {
+            float s2 = 0.0f;
+            float full[kCode];
+#pragma unroll
+            for (int e = 0; e < kCode; ++e) {
+                full[e] = Z[(size_t)(B + bestIdx) * kCode + e];
+                s2 += full[e] * full[e];
+            }
+            float const ih = 1.0f / fmaxf(sqrtf(s2), 1e-12f);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) chq[e] = full[8 * q + e] * ih;
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) chq[e] = 0.0f;
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            int const k = 8 * q + e;
+            float const a = ca[k];
+            dC[(size_t)t * kCode + k] = gn * (cn[k] - cp[k]) + gh * (chq[e] - cp[k]);
+            atomicAdd(&dC[(size_t)(B + t) * kCode + k], -(gn + gh) * a);
+            dC[(size_t)(2 * B + t) * kCode + k] = gn * a;
+            if (bestIdx >= 0) atomicAdd(&dC[(size_t)(B + bestIdx) * kCode + k], gh * a);
+        }
+        if (q == 0 && hardOut) hardOut[t] = bestIdx;
+    }
+    // work-group loss sum (quarter 0 of each anchor contributes)
+    float v = (q == 0) ? lossT : 0.0f;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((tid & 63) == 0) wl[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) lossPart[blockIdx.x] = wl[0] + wl[1] + wl[2] + wl[3];
+}
+
+// Code gradient through the normalisation: dz = (dc - c (c . dc)) / |z|.  One thread per
+// row; rows >= R get zeros.
+__global__ __launch_bounds__(256) void k_mlp_norm_backward(float const* __restrict__ Z, float const* __restrict__ dC,
+                                                           int R, int Rp, float* __restrict__ dZf,
+                                                           __bf16* __restrict__ dZ, __bf16* __restrict__ dZT)
+{
+    int const r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= Rp) return;
+    float z[kCode], d[kCode];
+    float s2 = 0.0f, cd = 0.0f;
+#pragma unroll
+    for (int e = 0; e < kCode; ++e) {
+        z[e] = r < R ? Z[(size_t)r * kCode + e] : 0.0f;
+        d[e] = r < R ? dC[(size_t)r * kCode + e] : 0.0f;
+        s2 += z[e] * z[e];
+    }
+    float const n = fmaxf(sqrtf(s2), 1e-12f);
+    float const inv = 1.0f / n;
+#pragma unroll
+    for (int e = 0; e < kCode; ++e) cd += z[e] * inv * d[e];
+#pragma unroll
+    for (int e = 0; e < kCode; ++e) {
+        float const g = (d[e] - z[e] * inv * cd) * inv;
+        dZf[(size_t)r * kCode + e] = g;
+        dZ[(size_t)r * kCode + e] = (__bf16)g;
+        dZT[(size_t)e * Rp + r] = (__bf16)g;
+    }
+}
+
+// ------------------------------------------------------------------------------ backward
+// dH = dZ W2 (64 rows per work-group; wave w: units [64w, 64w+64) of both row blocks, two
+// 16-deep steps over the 32 codes, B operand from W2T [unit][code]), masked by h > 0,
+// written transposed (dHT).
+__global__ __launch_bounds__(256) void k_mlp_backward_h(__bf16 const* __restrict__ dZ, int Rp,
+                                                        __bf16 const* __restrict__ W2Tb,
+                                                        __bf16 const* __restrict__ HT, __bf16* __restrict__ dHT)
+{
+    int const lane = threadIdx.x & 63;
+    int const wave = threadIdx.x >> 6;
+    int const h = lane >> 5;
+    int const c32 = lane & 31;
+    int const row0 = blockIdx.x * 64;
+    v16f acc[2][2];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) acc[rb][u] = v16f{};
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+        int const k = ks * 16 + 8 * h;
+        v8bf a[2], b[2];
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) a[rb] = load8(dZ + (size_t)(row0 + rb * 32 + c32) * kCode + k);
+#pragma unroll
+        for (int u = 0; u < 2; ++u) b[u] = load8(W2Tb + (size_t)((2 * wave + u) * 32 + c32) * kCode + k);
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int u = 0; u < 2; ++u) acc[rb][u] = mfma_bf16(a[rb], b[u], acc[rb][u]);
+    }
+    typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        int const unit = (2 * wave + u) * 32 + c32;
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                int const rl = row0 + rb * 32 + 8 * g + 4 * h;
+                v4bf const hv = *reinterpret_cast<v4bf const*>(HT + (size_t)unit * Rp + rl);
+                float v[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) v[e] = ((float)hv[e] > 0.0f) ? acc[rb][u][4 * g + e] : 0.0f;
+                store4_bf16(dHT + (size_t)unit * Rp + rl, v[0], v[1], v[2], v[3]);
+            }
+    }
+}
+
+// Weight gradients over a chunk of kWgChunk rows: grid (chunks, 5); y < 4: dW1 blocks (unit
+// block 2y + w/2 ... ) -- each wave computes 2 of the 32 dW1 blocks of its y, y == 4: the 8
+// dW2 blocks (2 per wave).  Bias gradients: y == 4 also sums dZf (db2) and dHT (db1) over
+// the chunk.  Partials -> part[chunk][param].
+constexpr int kWgChunk = 512;
+
+__global__ __launch_bounds__(256) void k_mlp_wgrad(__bf16 const* __restrict__ XT, __bf16 const* __restrict__ HT,
+                                                   __bf16 const* __restrict__ dZT, __bf16 const* __restrict__ dHT,
+                                                   float const* __restrict__ dZf, int Rp, float* __restrict__ part)
+{
+    int const lane = threadIdx.x & 63;
+    int const wave = threadIdx.x >> 6;
+    int const h = lane >> 5;
+    int const c32 = lane & 31;
+    int const chunk = blockIdx.x;
+    int const y = blockIdx.y;
+    int const i0 = chunk * kWgChunk;
+    float* out = part + (size_t)chunk * kParams;
+    v16f acc[2] = {v16f{}, v16f{}};
+    if (y < 4) {
+        // dW1 blocks b = 8y + 2*wave + s, s = 0,1: unit block ub = b / 4, input block db = b % 4
+        int ub[2], db[2];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            int const b = 8 * y + 2 * wave + s;
+            ub[s] = b >> 2;
+            db[s] = b & 3;
+        }
+        for (int ks = 0; ks < kWgChunk / 16; ++ks) {
+            int const i = i0 + ks * 16 + 8 * h;
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                v8bf const a = load8(dHT + (size_t)(ub[s] * 32 + c32) * Rp + i);
+                v8bf const b = load8(XT + (size_t)(db[s] * 32 + c32) * Rp + i);
+                acc[s] = mfma_bf16(a, b, acc[s]);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                int const u = ub[s] * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+                int const d = db[s] * 32 + c32;
+                out[kOffW1 + u * kDim + d] = acc[s][r];
+            }
+    } else {
+        // dW2 blocks: unit block ub = 2*wave + s; D[code][unit]
+        for (int ks = 0; ks < kWgChunk / 16; ++ks) {
+            int const i = i0 + ks * 16 + 8 * h;
+            v8bf const a = load8(dZT + (size_t)c32 * Rp + i);
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                v8bf const b = load8(HT + (size_t)((2 * wave + s) * 32 + c32) * Rp + i);
+                acc[s] = mfma_bf16(a, b, acc[s]);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                int const c = 8 * (r >> 2) + 4 * h + (r & 3);
+                int const u = (2 * wave + s) * 32 + c32;
+                out[kOffW2 + c * kHid + u] = acc[s][r];
+            }
+        // bias gradients of the chunk: thread t < 256 -> db1[t], t < 32 also db2[t]
+        int const t = threadIdx.x;
+        float s1 = 0.0f;
+        for (int i = 0; i < kWgChunk; ++i) s1 += (float)dHT[(size_t)t * Rp + i0 + i];
+        out[kOffB1 + t] = s1;
+        if (t < kCode) {
+            float s2 = 0.0f;
+            for (int i = 0; i < kWgChunk; ++i) s2 += dZf[(size_t)(i0 + i) * kCode + t];
+            out[kOffB2 + t] = s2;
+        }
+    }
+}
+
+// Sum of the chunk partials (fixed order) -> grad; Adam on the FP32 masters; refresh the
+// bf16 copies (W1b, W2b, W2Tb).  One thread per parameter.
+__global__ __launch_bounds__(256) void k_mlp_adam(float const* __restrict__ part, int nChunks,
+                                                  float* __restrict__ grad, float* __restrict__ P,
+                                                  float* __restrict__ M, float* __restrict__ V, float lr,
+                                                  float beta1, float beta2, float eps, float bc1, float bc2,
+                                                  int apply, __bf16* __restrict__ W1b, __bf16* __restrict__ W2b,
+                                                  __bf16* __restrict__ W2Tb)
+{
+    int const p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= kParams) return;
+    float g = 0.0f;
+    for (int c = 0; c < nChunks; ++c) g += part[(size_t)c * kParams + p];
+    grad[p] = g;
+    float w = P[p];
+    if (apply) {
+        float const m = beta1 * M[p] + (1.0f - beta1) * g;
+        float const v = beta2 * V[p] + (1.0f - beta2) * g * g;
+        M[p] = m;
+        V[p] = v;
+        w = w - lr * (m / bc1) / (sqrtf(v / bc2) + eps);
+        P[p] = w;
+    }
+    if (p < kOffB1) {
+        W1b[p] = (__bf16)w;
+    } else if (p >= kOffW2 && p < kOffB2) {
+        int const q = p - kOffW2;
+        int const c = q / kHid, u = q - c * kHid;
+        W2b[q] = (__bf16)w;
+        W2Tb[u * kCode + c] = (__bf16)w;
+    }
+}
+
+__global__ void k_fill_f32(float* __restrict__ p, size_t n, float v)
+{
+    size_t const i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------------------------ launchers
+void launch_mlp_gather(float const* pool, int const* tri, int const* order, int orderOff, int B, int Rp,
+                       __bf16* X, __bf16* XT, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mlp_gather, dim3((Rp + 3) / 4), dim3(256), 0, st, pool, tri, order, orderOff, B, Rp, X, XT);
+}
+
+void launch_mlp_rows_bf16(float const* src, int n, int Rp, __bf16* X, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mlp_rows_bf16, dim3((Rp + 3) / 4), dim3(256), 0, st, src, n, Rp, X);
+}
+
+void launch_mlp_forward(__bf16 const* X, int Rp, MlpWeights const& w, __bf16* HT, float* Z, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mlp_forward<__bf16>, dim3(Rp / 64), dim3(256), 0, st, X, Rp, w.W1b, w.P + kOffB1, w.W2b,
+                       w.P + kOffB2, HT, Z);
+}
+
+void launch_mlp_forward_f32(float const* X, int rows64, MlpWeights const& w, float* Z, hipStream_t st)
+{
+    if (rows64 <= 0) return;
+    hipLaunchKernelGGL(k_mlp_forward<float>, dim3(rows64 / 64), dim3(256), 0, st, X, rows64, w.W1b, w.P + kOffB1,
+                       w.W2b, w.P + kOffB2, (__bf16*)nullptr, Z);
+}
+
+void launch_mlp_loss(float const* Z, int B, float tau, float* dC, int* hard, float* lossPart, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mlp_loss, dim3((B + 63) / 64), dim3(256), 0, st, Z, B, 1.0f / tau, dC, hard, lossPart);
+}
+
+void launch_mlp_norm_backward(float const* Z, float const* dC, int R, int Rp, float* dZf, __bf16* dZ, __bf16* dZT,
+                              hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mlp_norm_backward, dim3((Rp + 255) / 256), dim3(256), 0, st, Z, dC, R, Rp, dZf, dZ, dZT);
+}
+
+void launch_mlp_backward_h(__bf16 const* dZ, int Rp, MlpWeights const& w, __bf16 const* HT, __bf16* dHT,
+                           hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mlp_backward_h, dim3(Rp / 64), dim3(256), 0, st, dZ, Rp, w.W2Tb, HT, dHT);
+}
+
+int mlp_wgrad_chunks(int Rp) { return (Rp + kWgChunk - 1) / kWgChunk; }
+
+void launch_mlp_wgrad(__bf16 const* XT, __bf16 const* HT, __bf16 const* dZT, __bf16 const* dHT, float const* dZf,
+                      int Rp, float* part, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mlp_wgrad, dim3(Rp / kWgChunk, 5), dim3(256), 0, st, XT, HT, dZT, dHT, dZf, Rp, part);
+}
+
+void launch_mlp_adam(float const* part, int nChunks, float* grad, MlpWeights const& w, float* M, float* V, float lr,
+                     int step, int apply, hipStream_t st)
+{
+    float const beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+    float const bc1 = 1.0f - powf(beta1, (float)step), bc2 = 1.0f - powf(beta2, (float)step);
+    hipLaunchKernelGGL(k_mlp_adam, dim3((kParams + 255) / 256), dim3(256), 0, st, part, nChunks, grad, w.P, M, V,
+                       lr, beta1, beta2, eps, apply ? bc1 : 1.0f, apply ? bc2 : 1.0f, apply, w.W1b, w.W2b, w.W2Tb);
+}
+
+void launch_fill(float* p, size_t n, float v, hipStream_t st)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+}
+
+}  // namespace v3dm

@@ -310,10 +310,17 @@ __global__ __launch_bounds__(256) void k_mutual_local(int const* __restrict__ co
 
 __global__ __launch_bounds__(1024) void k_mutual_gather(int2 const* __restrict__ tmp,
                                                         int const* __restrict__ blockCount, int nBlocks,
-                                                        int2* __restrict__ pairs, int* __restrict__ counters)
+                                                        int* __restrict__ stripes, int2* __restrict__ pairs,
+                                                        int* __restrict__ counters)
 {
     __shared__ int offs[1025];
     int const tid = threadIdx.x;
+    int rescored = 0;
+    if (tid < kStripes) {
+        rescored = stripes[tid * kStripeStride];
+        stripes[tid * kStripeStride] = 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) rescored += __shfl_xor(rescored, o);  // wave 0 holds the stripes
     if (tid == 0) {
         int a = 0;
         for (int b = 0; b < nBlocks; ++b) {
@@ -332,18 +339,17 @@ __global__ __launch_bounds__(1024) void k_mutual_gather(int2 const* __restrict__
         counters[kCntFlagDone] = counters[kCntFlag] + counters[kCntFlagR];
         counters[kCntFlag] = 0;
         counters[kCntFlagR] = 0;
-        counters[kCntRescoredDone] = counters[kCntRescored];
-        counters[kCntRescored] = 0;
+        counters[kCntRescoredDone] = rescored;
     }
 }
 
 void launch_mutual_compact(int const* colMax, int const* rowMax, int nL, int nR, int2* tmp, int* blockCount,
-                           int2* pairs, int* counters, hipStream_t st)
+                           int* stripes, int2* pairs, int* counters, hipStream_t st)
 {
     int const nb = (nL + 1023) / 1024;
     if (nb > 0)
         hipLaunchKernelGGL(k_mutual_local, dim3(nb), dim3(256), 0, st, colMax, rowMax, nL, nR, tmp, blockCount);
-    hipLaunchKernelGGL(k_mutual_gather, dim3(1), dim3(1024), 0, st, tmp, blockCount, nb, pairs, counters);
+    hipLaunchKernelGGL(k_mutual_gather, dim3(1), dim3(1024), 0, st, tmp, blockCount, nb, stripes, pairs, counters);
 }
 
 // ------------------------------------------------------------------------------ uint8 dequantisation

@@ -42,7 +42,11 @@ void launch_pass_fp8(unsigned char const* Q, unsigned char const* EQ, int nQ, un
                      unsigned char const* EC, int nC, ShortPlan const& p, Cand2* part, int ldq, hipStream_t st);
 void launch_rescore(float const* L, float const* R, int nL, int nR, Cand2 const* partL, int nPartL, int ldqL,
                     Cand2 const* partR, int nPartR, int ldqR, float const* normL, float const* normR,
-                    unsigned int const* maxNormBits, float winRel, float minScore, float minRatio, int* colMax,
-                    int* rowMax, int* counters, hipStream_t st);
+                    unsigned int const* maxNormBits, float winRel, int topK, float minScore, float minRatio,
+                    int* colMax, int* rowMax, int* colSecond, int* rowSecond, int* stripes, hipStream_t st);
+// learned code pass (v3dm_code.hip)
+void launch_code_quant(float const* Z, int n, int rows, float scale, signed char* codes, hipStream_t st);
+void launch_pass_code(signed char const* Q, int nQ, signed char const* C, int nC, ShortPlan const& p, Cand2* part,
+                      int ldq, hipStream_t st);
 
 }  // namespace v3dm

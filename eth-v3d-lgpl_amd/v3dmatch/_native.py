@@ -55,6 +55,8 @@ SIGNATURES = {
     "v3dm_set_precision": (_I, [_P, _I]),
     "v3dm_set_certification": (_I, [_P, _F]),
     "v3dm_set_stream": (_I, [_P, _P]),
+    "v3dm_set_learned_topk": (_I, [_P, _I]),
+    "v3dm_set_derived_cache": (_I, [_P, _I]),
     "v3dm_set_left_device": (_I, [_P, _I, _FP]),
     "v3dm_set_right_device": (_I, [_P, _I, _FP]),
     "v3dm_set_left_u8": (_I, [_P, _I, _FP]),
@@ -66,6 +68,7 @@ SIGNATURES = {
     "v3dm_synchronize": (_I, [_P]),
     "v3dm_set_profiling": (_I, [_P, _I]),
     "v3dm_get_stats": (_I, [_P, C.POINTER(Stats)]),
+    "v3dm_get_second_best": (_I, [_P, _FP, _FP]),
     "v3dm_batch_set_images_u8": (_I, [_P, _I, _FP, _FP]),
     "v3dm_batch_set_images": (_I, [_P, _I, _FP, _FP]),
     "v3dm_batch_set_pairs": (_I, [_P, _I, _FP, _FP]),
@@ -76,6 +79,13 @@ SIGNATURES = {
     "v3dm_batch_get_tuning": (_I, [_P, _I, _IP, _IP, _IP, C.POINTER(C.c_float)]),
     "v3dm_batch_use_graph": (_I, [_P, _I]),
     "v3dm_batch_last_ms": (_I, [_P, C.POINTER(C.c_float)]),
+    "v3dm_learned_set_state": (_I, [_P, _FP, _FP, _FP, _I]),
+    "v3dm_learned_get_state": (_I, [_P, _FP, _FP, _FP, _IP]),
+    "v3dm_learned_set_code_scale": (_I, [_P, _F]),
+    "v3dm_learned_set_training": (_I, [_P, _I, _FP, _I, _FP]),
+    "v3dm_learned_train": (_I, [_P, _I, _I, _F, _F, C.c_uint64, _FP]),
+    "v3dm_learned_gradients": (_I, [_P, _I, _FP, _F, C.POINTER(C.c_float), _FP, _FP]),
+    "v3dm_learned_embed": (_I, [_P, _I, _FP, _FP]),
 }
 
 

@@ -152,6 +152,12 @@ class SIFT_Matching:
         """Error-bound scale of certified decisions (0 = every query re-checked exactly)."""
         self._check(self._lib.v3dm_set_certification(self._h, C.c_float(eps_rel)))
 
+    def set_learned_topk(self, k):
+        self._check(self._lib.v3dm_set_learned_topk(self._h, int(k)))
+
+    def set_derived_cache(self, enable=True):
+        self._check(self._lib.v3dm_set_derived_cache(self._h, int(bool(enable))))
+
     def set_stream(self, stream_ptr):
         self._check(self._lib.v3dm_set_stream(self._h, C.c_void_p(stream_ptr) if stream_ptr else None))
 
@@ -184,6 +190,14 @@ class SIFT_Matching:
         rm = np.empty(max(self._nR, 1), dtype=np.int32)
         self._check(self._lib.v3dm_get_maxima(self._h, cm.ctypes.data, rm.ctypes.data))
         return cm[: self._nL], rm[: self._nR]
+
+    def get_second_best(self):
+        """(colSecond, rowSecond) of the last shortlist-mode call: per feature, the feature
+        of the other set with the second best exact score among the rescored candidates."""
+        cs = np.empty(max(self._nL, 1), dtype=np.int32)
+        rs = np.empty(max(self._nR, 1), dtype=np.int32)
+        self._check(self._lib.v3dm_get_second_best(self._h, cs.ctypes.data, rs.ctypes.data))
+        return cs[: self._nL], rs[: self._nR]
 
     def synchronize(self):
         self._check(self._lib.v3dm_synchronize(self._h))

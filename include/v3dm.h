@@ -105,6 +105,13 @@ int v3dm_set_precision(v3dm_handle h, int mode);
  * decision; otherwise it is re-checked exactly in FP64.  Default 2^-15; 0 re-checks every
  * query (the whole selection then runs on exact FP64 scores). */
 int v3dm_set_certification(v3dm_handle h, float epsRel);
+/* Candidates per query scored exactly in the learned mode (the best k by code score;
+ * default 8, at most 32). */
+int v3dm_set_learned_topk(v3dm_handle h, int k);
+/* 1 (default): the bf16 / fp8 / code copies of the features are kept between calls while
+ * the features are unchanged; 0: every call derives them again (so that a call's time
+ * includes that work). */
+int v3dm_set_derived_cache(v3dm_handle h, int enable);
 /* Run the library's kernels on this HIP stream (hipStream_t, NULL = the handle's own). */
 int v3dm_set_stream(v3dm_handle h, void* hipStream);
 /* Feature upload from device memory (n*128 floats, device pointer); no host copy. */
@@ -129,6 +136,10 @@ int v3dm_synchronize(v3dm_handle h);
 /* Time the dominant kernel with HIP events on the handle's stream (1 = on, resets sums). */
 int v3dm_set_profiling(v3dm_handle h, int enable);
 int v3dm_get_stats(v3dm_handle h, v3dm_stats* out);
+/* After a shortlist-mode call (BF16 / FP8): for every left (right) feature the feature of
+ * the other set with the second best exact score among the rescored candidates (-1 if
+ * none) -- the hardest distractor, used to label training triplets. */
+int v3dm_get_second_best(v3dm_handle h, int32_t* colSecond, int32_t* rowSecond);
 
 /* ---- batched matching (extension) ----------------------------------------------------
  * Replaces the caller loop that matches many image pairs one at a time
@@ -165,6 +176,32 @@ int v3dm_batch_get_tuning(v3dm_handle h, int mode, int* tilesPerItem, int* grid,
 int v3dm_batch_use_graph(v3dm_handle h, int enable);
 /* GPU time of the last batch run (HIP events around the whole run), ms. */
 int v3dm_batch_last_ms(v3dm_handle h, float* ms);
+
+/* ---- learned shortlist network (extension; the reference has no learned stage) -------
+ * A 128 -> 256 -> 32 ReLU MLP embeds descriptors into 32-d codes whose dot products
+ * propose candidates; the proposed candidates are then scored exactly.  Parameters are
+ * one packed FP32 vector of V3DM_LEARNED_PARAMS floats: W1 [256][128], b1 [256],
+ * W2 [32][256], b2 [32] (row-major, in that order).  Training runs on the GPU with BF16
+ * MFMA forward / backward kernels, FP32 master weights and Adam. */
+#define V3DM_LEARNED_PARAMS 41248
+/* Parameters and Adam moments (m, v may be NULL: zero) at Adam step count `step`. */
+int v3dm_learned_set_state(v3dm_handle h, const float* params, const float* m, const float* v, int step);
+int v3dm_learned_get_state(v3dm_handle h, float* params, float* m, float* v, int* step);
+/* Quantisation scale of the unit-norm codes used by the code pass (codes * scale are
+ * rounded to integers in [-31, 31]). */
+int v3dm_learned_set_code_scale(v3dm_handle h, float scale);
+/* Training set: nPool descriptor rows (host, 128 floats each) and nTriplets triplets
+ * (anchor, positive, negative) of pool rows (3 int32 each). */
+int v3dm_learned_set_training(v3dm_handle h, int nPool, const float* pool, int nTriplets, const int32_t* triplets);
+/* `steps` Adam steps on minibatches of `batch` triplets (reshuffled every epoch with
+ * `seed`); loss temperature tau; losses[s] = mean minibatch loss of step s (may be NULL). */
+int v3dm_learned_train(v3dm_handle h, int steps, int batch, float lr, float tau, uint64_t seed, float* losses);
+/* Loss and FP32 gradient (packed layout) of one minibatch (indices into the training
+ * triplets), without an update; hard[t] = in-batch hardest positive chosen for anchor t. */
+int v3dm_learned_gradients(v3dm_handle h, int nBatch, const int32_t* tripletIdx, float tau, float* loss,
+                           float* grads, int32_t* hard);
+/* Codes before normalisation (n x 32 floats) of n host descriptors. */
+int v3dm_learned_embed(v3dm_handle h, int n, const float* desc, float* codes);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,107 @@
+"""GPU tests of the learned-code shortlist form: with the trained network
+(v3dmatch/weights/shortlist_mlp.npz) the match set keeps >= 99.5% recall and precision
+against the exact set; the same pipeline with an untrained network loses recall (what the
+training gains); and the per-query candidate count trades recall for work."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import v3dmatch
+from oracle import oracle
+from v3dmatch import learned, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _pairs_set(a):
+    return set(map(tuple, a.tolist()))
+
+
+@pytest.fixture(scope="module")
+def matcher():
+    assert os.path.exists(learned.WEIGHTS_PATH), "trained weights missing"
+    m = v3dmatch.SIFT_Matching(0)
+    m.enableRoundingWarning(False)
+    m.allocate(16384, 16384)
+    net = learned.Network(m).load(learned.WEIGHTS_PATH)
+    yield m, net
+    m.close()
+
+
+def _recall(m, L, R, want):
+    m.setLeftFeatures(L.shape[0], L)
+    m.setRightFeatures(R.shape[0], R)
+    m.set_precision(v3dmatch.PREC_LEARNED)
+    got = _pairs_set(m.find_putative_matches_array(0.9, 0.8))
+    inter = len(got & want)
+    return inter / max(len(want), 1), inter / max(len(got), 1), m.stats()
+
+
+@pytest.mark.parametrize("n,seed", [(1024, 3), (4096, 0), (4096, 7)])
+def test_learned_recall_oracle(matcher, n, seed):
+    m, net = matcher
+    qL, qR, _ = synth.make_pair(n, n, 0.4, config=1, pair=seed)
+    L, R = synth.dequantise(qL), synth.dequantise(qR)
+    want = _pairs_set(oracle.find_putative_matches(L, R, 0.9, 0.8))
+    rec, prec, st = _recall(m, L, R, want)
+    say(f"case learned n={n} seed={seed}: exact={len(want)} recall={rec:.5f} precision={prec:.5f} "
+        f"rescored={st['nRescored']}")
+    assert rec >= 0.995 and prec >= 0.995
+
+
+def test_learned_recall_full_size(matcher):
+    m, net = matcher
+    n = 16384
+    qL, qR, _ = synth.make_pair(n, n, 0.4, config=2, pair=0)
+    L, R = synth.dequantise(qL), synth.dequantise(qR)
+    m.setLeftFeatures(n, L)
+    m.setRightFeatures(n, R)
+    m.set_precision(v3dmatch.PREC_FP32)
+    want = _pairs_set(m.find_putative_matches_array(0.9, 0.8))
+    rec, prec, st = _recall(m, L, R, want)
+    say(f"case learned full size: exact={len(want)} recall={rec:.5f} precision={prec:.5f}")
+    assert rec >= 0.995 and prec >= 0.995
+
+
+def test_training_gain(matcher):
+    """The trained network against the same pipeline with its initial (untrained) weights."""
+    m, net = matcher
+    n = 4096
+    qL, qR, _ = synth.make_pair(n, n, 0.4, config=1, pair=11)
+    L, R = synth.dequantise(qL), synth.dequantise(qR)
+    want = _pairs_set(oracle.find_putative_matches(L, R, 0.9, 0.8))
+    rec_t, _, _ = _recall(m, L, R, want)
+    other = v3dmatch.SIFT_Matching(0)
+    other.enableRoundingWarning(False)
+    other.allocate(n, n)
+    raw = learned.Network(other)
+    raw.set_params(learned.init_params(1))
+    raw.calibrate_code_scale(np.concatenate([L[:2048], R[:2048]]))
+    rec_u, _, _ = _recall(other, L, R, want)
+    other.close()
+    say(f"case training gain: trained recall {rec_t:.5f}, untrained recall {rec_u:.5f}")
+    assert rec_t >= 0.995
+    assert rec_u < rec_t - 0.01
+
+
+def test_topk_tradeoff(matcher):
+    m, net = matcher
+    n = 4096
+    qL, qR, _ = synth.make_pair(n, n, 0.4, config=1, pair=12)
+    L, R = synth.dequantise(qL), synth.dequantise(qR)
+    want = _pairs_set(oracle.find_putative_matches(L, R, 0.9, 0.8))
+    res = {}
+    for k in (2, 4, 8, 16):
+        m.set_learned_topk(k)
+        res[k] = _recall(m, L, R, want)
+    m.set_learned_topk(8)
+    say("case top-k: " + " ".join(f"k={k}: recall {r[0]:.5f} rescored {r[2]['nRescored']}" for k, r in res.items()))
+    assert res[16][0] >= res[2][0]
+    assert res[16][2]["nRescored"] > res[2][2]["nRescored"]
