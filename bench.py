@@ -104,8 +104,14 @@ def bench_batch(steps, warmup, n_pairs=256):
            "features": int(counts.sum()),
            "score_pairs": int(sum(int(counts[a] // 32 * 32) * int(counts[b] // 32 * 32)
                                   for a, b in zip(left, right)))}
+    m.set_derived_cache(False)  # each run derives its bf16 / code copies of the images
+    forms = [("plain_fp32", v3dmatch.PREC_FP32), ("bf16_shortlist", v3dmatch.PREC_BF16)]
+    from v3dmatch import learned
+    if os.path.exists(learned.WEIGHTS_PATH):
+        learned.Network(m).load(learned.WEIGHTS_PATH)
+        forms.append(("learned_shortlist", v3dmatch.PREC_LEARNED))
     sets = {}
-    for name, prec in (("plain_fp32", v3dmatch.PREC_FP32), ("bf16_shortlist", v3dmatch.PREC_BF16)):
+    for name, prec in forms:
         log(f"bench: batch {name}")
         m.set_precision(prec)
         m.batch_match(MIN_SCORE, MIN_RATIO)  # first run: autotuning and graph capture
@@ -123,11 +129,13 @@ def bench_batch(steps, warmup, n_pairs=256):
         res[name] = {"ms_per_batch": dt * 1e3, "gpu_ms_per_batch": gpu_ms / steps,
                      "pairs_per_s": n_pairs / dt, "matches": len(sets[name]),
                      "tuning": m.batch_get_tuning(prec)}
-    ex, gs = sets["plain_fp32"], sets["bf16_shortlist"]
-    inter = len(ex & gs)
-    res["bf16_shortlist"]["recall"] = inter / max(len(ex), 1)
-    res["bf16_shortlist"]["precision"] = inter / max(len(gs), 1)
-    res["bf16_shortlist"]["speedup_vs_plain"] = res["plain_fp32"]["ms_per_batch"] / res["bf16_shortlist"]["ms_per_batch"]
+    ex = sets["plain_fp32"]
+    for name, _ in forms[1:]:
+        gs = sets[name]
+        inter = len(ex & gs)
+        res[name]["recall"] = inter / max(len(ex), 1)
+        res[name]["precision"] = inter / max(len(gs), 1)
+        res[name]["speedup_vs_plain"] = res["plain_fp32"]["ms_per_batch"] / res[name]["ms_per_batch"]
     m.close()
     return res
 

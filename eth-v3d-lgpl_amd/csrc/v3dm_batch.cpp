@@ -17,6 +17,7 @@
 
 #include "v3dm_batch.h"
 #include "v3dm_context.h"
+#include "v3dm_learned.h"
 
 namespace v3dm {
 
@@ -49,6 +50,9 @@ struct BatchState {
     float* img = nullptr;
     __bf16* imgB = nullptr;
     bool bfValid = false;
+    signed char* codes = nullptr;  // learned codes of every image row
+    bool codesValid = false;
+    float codesScale = 0.0f;
     int* imgBase = nullptr;
     int* blkImg = nullptr;
     int* nImgD = nullptr;
@@ -69,13 +73,13 @@ struct BatchState {
     int* segOfBlock = nullptr;
     int segBlkCap = 0;
 
-    // plans (0 = plain FP32, 1 = shortlist) and their device copies
-    Plan plan[2];
-    bool planValid[2] = {false, false};
-    int4* items[2] = {nullptr, nullptr};
-    int itemCap[2] = {0, 0};
-    BatchSeg* segD[2] = {nullptr, nullptr};
-    int segCap[2] = {0, 0};
+    // plans (0 = plain FP32, 1 = BF16 shortlist, 2 = learned codes) and their device copies
+    Plan plan[3];
+    bool planValid[3] = {false, false, false};
+    int4* items[3] = {nullptr, nullptr, nullptr};
+    int itemCap[3] = {0, 0, 0};
+    BatchSeg* segD[3] = {nullptr, nullptr, nullptr};
+    int segCap[3] = {0, 0, 0};
     Top2* partF = nullptr;
     size_t partFCap = 0;
     Cand2* partB = nullptr;
@@ -84,6 +88,8 @@ struct BatchState {
     // selection / results
     int* sel = nullptr;
     int* flagList = nullptr;
+    int* slots = nullptr;      // rescoring lists (qRows x kRescoreSlots), shortlist forms
+    int* slotCount = nullptr;  // qRows
     int selCap = 0;
     int* counters = nullptr;  // kBatchCounters ints
     int* stripes = nullptr;   // striped counts, cleared by the gather kernel
@@ -97,15 +103,16 @@ struct BatchState {
 
     // autotuning and graphs
     bool autotune = true;
-    int forceTiles[2] = {0, 0}, forceGrid[2] = {0, 0};
+    int forceTiles[3] = {0, 0, 0}, forceGrid[3] = {0, 0, 0};
     struct GraphKey {
         int mode;
         float minScore, minRatio, epsRel;
         int version;
+        int topK;
         bool operator<(GraphKey const& o) const
         {
-            return std::tie(mode, minScore, minRatio, epsRel, version) <
-                   std::tie(o.mode, o.minScore, o.minRatio, o.epsRel, o.version);
+            return std::tie(mode, minScore, minRatio, epsRel, version, topK) <
+                   std::tie(o.mode, o.minScore, o.minRatio, o.epsRel, o.version, o.topK);
         }
     };
     std::map<GraphKey, hipGraphExec_t> graphs;
@@ -160,6 +167,7 @@ BatchDev dev_view(BatchState* b, int form)
     BatchDev d;
     d.img = b->img;
     d.imgB = b->imgB;
+    d.codes = b->codes;
     d.imgBase = b->imgBase;
     d.norm = b->norm;
     d.maxNorm = b->maxNorm;
@@ -239,8 +247,8 @@ Plan plan_plain(BatchState* b, int tiles, int grid)
 }
 
 // Shortlist plan: items (segment, query block, split, tiles) with at least `tiles` 64-row
-// candidate tiles per item and at most kShortMaxSplits splits per query.
-Plan plan_short_batch(BatchState* b, int tiles, int grid)
+// candidate tiles per item and at most maxSplits (<= kShortMaxSplits) splits per query.
+Plan plan_short_batch(BatchState* b, int tiles, int grid, int maxSplits)
 {
     Plan pl;
     pl.tiles = tiles;
@@ -257,7 +265,7 @@ Plan plan_short_batch(BatchState* b, int tiles, int grid)
             continue;
         }
         int const nTiles = (s.nO + 63) / 64;
-        int const tps = std::max(tiles, (nTiles + kShortMaxSplits - 1) / kShortMaxSplits);
+        int const tps = std::max(tiles, (nTiles + maxSplits - 1) / maxSplits);
         int const splits = (nTiles + tps - 1) / tps;
         s.nParts = splits;
         s.partOff = (int)off;
@@ -311,6 +319,11 @@ int ensure_query_space(v3dm_context* h, BatchState* b)
     int rc;
     if ((rc = growi(h, b->segOfBlock, b->segBlkCap, nBlk))) return rc;
     if ((rc = growi(h, b->sel, b->selCap, b->qRows))) return rc;
+    if (b->slots) (void)hipFree(b->slots);
+    if (b->slotCount) (void)hipFree(b->slotCount);
+    b->slots = b->slotCount = nullptr;
+    BCHECK(h, hipMalloc(&b->slots, (size_t)std::max(b->qRows, 1) * kRescoreSlots * sizeof(int)));
+    BCHECK(h, hipMalloc(&b->slotCount, (size_t)std::max(b->qRows, 1) * sizeof(int)));
     if (b->flagList) (void)hipFree(b->flagList);
     b->flagList = nullptr;
     BCHECK(h, hipMalloc(&b->flagList, (size_t)std::max(b->qRows, 1) * sizeof(int)));
@@ -320,7 +333,7 @@ int ensure_query_space(v3dm_context* h, BatchState* b)
     b->tmp = nullptr;
     b->blockCount = nullptr;
     BCHECK(h, hipMalloc(&b->tmp, (size_t)std::max(nbMutual, 1) * 1024 * sizeof(int2)));
-    BCHECK(h, hipMalloc(&b->blockCount, (size_t)std::max(nbMutual, 1) * sizeof(int)));
+    BCHECK(h, hipMalloc(&b->blockCount, (size_t)std::max(nbMutual, 1) * 2 * sizeof(int)));  // counts, offsets
     // every left query gives at most one match
     if ((rc = growi(h, b->out, b->outCap, std::max(b->leftRows, 1)))) return rc;
     if ((rc = growi(h, b->pairCount, b->pairCap, std::max(b->nPairs, 1)))) return rc;
@@ -345,7 +358,19 @@ int ensure_query_space(v3dm_context* h, BatchState* b)
     return V3DM_OK;
 }
 
-int form_of(int mode) { return mode == V3DM_PREC_FP32 ? 0 : 1; }
+int form_of(int mode) { return mode == V3DM_PREC_FP32 ? 0 : (mode == V3DM_PREC_LEARNED ? 2 : 1); }
+
+// Derived copies of the images for a form: bf16 rows (BF16 form) or learned codes.
+void enqueue_derived(v3dm_context* h, BatchState* b, int form)
+{
+    hipStream_t const st = h->stream;
+    if (form == 1) {
+        launch_to_bf16(b->img, b->imgRows, b->imgB, st);
+    } else if (form == 2) {
+        launch_mlp_codes_f32(b->img, b->imgRows, *learned_weights(ctx_learned(h)), learned_code_scale(ctx_learned(h)),
+                             b->codes, st);
+    }
+}
 
 // Enqueue one batch run of `mode` with the current plan of its form.
 int enqueue_run(v3dm_context* h, BatchState* b, int mode, float minScore, float minRatio)
@@ -355,13 +380,19 @@ int enqueue_run(v3dm_context* h, BatchState* b, int mode, float minScore, float 
     BatchDev const d = dev_view(b, form);
     Plan const& pl = b->plan[form];
     launch_batch_reset(b->counters, b->pairCount, b->nPairs, st);
+    if (!h->cacheDerived) enqueue_derived(h, b, form);
     if (form == 0) {
         launch_batch_fused_fp32(d, b->items[0], (int)pl.items.size(), b->counters + 0, b->partF, pl.grid, st);
         launch_batch_merge_decide(d, b->partF, h->epsRel, minScore, minRatio, b->sel, b->flagList, b->counters, st);
         launch_batch_recheck(d, b->flagList, b->counters, minScore, minRatio, b->sel, 1024, st);
-    } else {
+    } else if (form == 1) {
         launch_batch_pass_bf16(d, b->items[1], (int)pl.items.size(), b->counters + 1, b->partB, pl.grid, st);
-        launch_batch_rescore(d, b->partB, kWinRelBf16, 0, minScore, minRatio, b->sel, b->stripes, st);
+        launch_batch_rescore(d, b->partB, kWinRelBf16, 0, minScore, minRatio, b->slots, b->slotCount, b->sel,
+                             b->stripes, st);
+    } else {
+        launch_batch_pass_code(d, b->items[2], (int)pl.items.size(), b->counters + 2, b->partB, pl.grid, st);
+        launch_batch_rescore(d, b->partB, 0.0f, h->learnedTopK, minScore, minRatio, b->slots, b->slotCount, b->sel,
+                             b->stripes, st);
     }
     launch_batch_mutual(d, b->sel, b->leftRows, b->tmp, b->blockCount, b->out, b->pairCount, b->stripes,
                         b->counters, st);
@@ -383,8 +414,10 @@ int time_pass(v3dm_context* h, BatchState* b, int form, Plan const& pl, float& m
         BCHECK(h, hipEventRecord(b->evA, st));
         if (form == 0)
             launch_batch_fused_fp32(d, b->items[0], (int)pl.items.size(), b->counters + 0, b->partF, pl.grid, st);
-        else
+        else if (form == 1)
             launch_batch_pass_bf16(d, b->items[1], (int)pl.items.size(), b->counters + 1, b->partB, pl.grid, st);
+        else
+            launch_batch_pass_code(d, b->items[2], (int)pl.items.size(), b->counters + 2, b->partB, pl.grid, st);
         BCHECK(h, hipEventRecord(b->evB, st));
         BCHECK(h, hipEventSynchronize(b->evB));
         float t = 0.0f;
@@ -405,10 +438,11 @@ int ensure_plan(v3dm_context* h, BatchState* b, int form)
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     auto make = [&](int tiles, int grid) {
-        return form == 0 ? plan_plain(b, tiles, grid) : plan_short_batch(b, tiles, grid);
+        return form == 0 ? plan_plain(b, tiles, grid) : plan_short_batch(b, tiles, grid, form == 2 ? 8 : 16);
     };
     std::vector<int> tileCands = form == 0 ? std::vector<int>{8, 16, 32, 64} : std::vector<int>{4, 8, 16, 32};
-    std::vector<int> gridCands = {cus, 2 * cus};
+    // the code pass runs 4 work-groups per CU, the others 2
+    std::vector<int> gridCands = form == 2 ? std::vector<int>{2 * cus, 4 * cus} : std::vector<int>{cus, 2 * cus};
     int bestT = form == 0 ? 16 : 8, bestG = 2 * cus;
     float bestMs = -1.0f;
     if (b->forceTiles[form] > 0 || b->forceGrid[form] > 0) {
@@ -470,7 +504,7 @@ int set_images(v3dm_context* h, int nImages, int32_t const* counts, void const* 
     }
     if (totalRaw > (size_t)0x7fffffff / kDim) return ctx_fail(h, V3DM_ERR_INVALID, "batch too large");
     drop_graphs(b);
-    b->planValid[0] = b->planValid[1] = false;
+    b->planValid[0] = b->planValid[1] = b->planValid[2] = false;
     b->nPairs = 0;
     b->pending = false;
     b->nImages = nImages;
@@ -483,6 +517,8 @@ int set_images(v3dm_context* h, int nImages, int32_t const* counts, void const* 
         if (b->img) (void)hipFree(b->img);
         if (b->imgB) (void)hipFree(b->imgB);
         if (b->norm) (void)hipFree(b->norm);
+        if (b->codes) (void)hipFree(b->codes);  // sized with the image rows: allocated again on use
+        b->codes = nullptr;
         b->img = nullptr;
         b->imgB = nullptr;
         b->norm = nullptr;
@@ -541,6 +577,7 @@ int set_images(v3dm_context* h, int nImages, int32_t const* counts, void const* 
                               b->img, st);
     launch_batch_norms(b->img, b->blkImg, rows, b->norm, b->maxNorm, st);
     b->bfValid = false;
+    b->codesValid = false;
     BCHECK(h, hipGetLastError());
     BCHECK(h, hipStreamSynchronize(st));  // host vectors above go out of scope
     return V3DM_OK;
@@ -552,10 +589,10 @@ void batch_destroy(BatchState*& b)
 {
     if (!b) return;
     drop_graphs(b);
-    for (void* p : {(void*)b->img, (void*)b->imgB, (void*)b->imgBase, (void*)b->blkImg, (void*)b->nImgD,
+    for (void* p : {(void*)b->img, (void*)b->imgB, (void*)b->codes, (void*)b->imgBase, (void*)b->blkImg, (void*)b->nImgD,
                     (void*)b->srcRow, (void*)b->norm, (void*)b->maxNorm, b->upload, (void*)b->segOfBlock,
                     (void*)b->items[0], (void*)b->items[1], (void*)b->segD[0], (void*)b->segD[1], (void*)b->partF,
-                    (void*)b->partB, (void*)b->sel, (void*)b->flagList, (void*)b->counters, (void*)b->stripes, (void*)b->pairCount,
+                    (void*)b->partB, (void*)b->sel, (void*)b->flagList, (void*)b->slots, (void*)b->slotCount, (void*)b->counters, (void*)b->stripes, (void*)b->pairCount,
                     (void*)b->tmp, (void*)b->blockCount, (void*)b->out})
         if (p) (void)hipFree(p);
     if (b->pinned) (void)hipHostFree(b->pinned);
@@ -592,7 +629,7 @@ int v3dm_batch_set_pairs(v3dm_handle h, int nPairs, const int32_t* left, const i
         if (left[p] < 0 || left[p] >= b->nImages || right[p] < 0 || right[p] >= b->nImages)
             return ctx_fail(h, V3DM_ERR_INVALID, "pair refers to an image outside the batch");
     drop_graphs(b);
-    b->planValid[0] = b->planValid[1] = false;
+    b->planValid[0] = b->planValid[1] = b->planValid[2] = false;
     b->pending = false;
     b->nPairs = nPairs;
     b->pa.assign(left, left + nPairs);
@@ -646,18 +683,33 @@ int v3dm_batch_match_async(v3dm_handle h, float minScore, float minRatio)
     int rc = check_batch(h, b);
     if (rc) return rc;
     int const mode = h->precision;
-    if (mode != V3DM_PREC_FP32 && mode != V3DM_PREC_BF16)
-        return ctx_fail(h, V3DM_ERR_UNSUPPORTED, "batched matching supports the FP32 and BF16 modes");
+    if (mode != V3DM_PREC_FP32 && mode != V3DM_PREC_BF16 && mode != V3DM_PREC_LEARNED)
+        return ctx_fail(h, V3DM_ERR_UNSUPPORTED, "batched matching supports the FP32, BF16 and learned modes");
     int const form = form_of(mode);
     hipStream_t const st = h->stream;
-    if (form == 1 && !b->bfValid) {
-        launch_to_bf16(b->img, b->imgRows, b->imgB, st);
+    if (form == 2) {
+        LearnedState* L = ctx_learned(h);
+        if (!learned_weights(L) || !(learned_code_scale(L) > 0.0f))
+            return ctx_fail(h, V3DM_ERR_STATE, "learned network or code scale not set");
+        if (!b->codes) BCHECK(h, hipMalloc(&b->codes, (size_t)b->imgCapRows * kCode));
+        if (b->codesScale != learned_code_scale(L)) b->codesValid = false;
+        b->codesScale = learned_code_scale(L);
+    }
+    // derived copies outside the run when they are cached (else every run derives them)
+    if (h->cacheDerived && form == 1 && !b->bfValid) {
+        enqueue_derived(h, b, 1);
         b->bfValid = true;
     }
+    if (h->cacheDerived && form == 2 && !b->codesValid) {
+        enqueue_derived(h, b, 2);
+        b->codesValid = true;
+    }
+    if (!h->cacheDerived && form != 0 && !b->planValid[form]) enqueue_derived(h, b, form);  // for the tuner
     if ((rc = ensure_plan(h, b, form))) return rc;
     BCHECK(h, hipEventRecord(b->evA, st));
     if (b->useGraph) {
-        BatchState::GraphKey const key{mode, minScore, minRatio, h->epsRel, b->version};
+        BatchState::GraphKey const key{mode, minScore, minRatio, h->epsRel, b->version * 2 + (h->cacheDerived ? 1 : 0),
+                                       h->learnedTopK};
         auto it = b->graphs.find(key);
         if (it == b->graphs.end()) {
             hipGraph_t g = nullptr;

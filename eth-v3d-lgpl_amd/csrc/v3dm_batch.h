@@ -48,6 +48,7 @@ struct BatchSeg {
 struct BatchDev {
     float const* img;
     __bf16 const* imgB;
+    signed char const* codes;     // learned codes, 32 bytes per image row
     int const* imgBase;
     float const* norm;            // per image row
     unsigned int const* maxNorm;  // per image (float bits)
@@ -64,7 +65,10 @@ void launch_batch_fused_fp32(BatchDev const& b, int4 const* items, int nItems, i
 void launch_batch_pass_bf16(BatchDev const& b, int4 const* items, int nItems, int* queue, Cand2* part, int grid,
                             hipStream_t st);
 void launch_batch_rescore(BatchDev const& b, Cand2 const* part, float winRel, int topK, float minScore,
-                          float minRatio, int* sel, int* stripes, hipStream_t st);
+                          float minRatio, int* slots, int* count, int* sel, int* stripes, hipStream_t st);
+// learned code pass (v3dm_code.hip)
+void launch_batch_pass_code(BatchDev const& b, int4 const* items, int nItems, int* queue, Cand2* part, int grid,
+                            hipStream_t st);
 // selection and bookkeeping (v3dm_batch.hip)
 void launch_batch_reset(int* counters, int* pairCount, int nPairs, hipStream_t st);
 void launch_batch_norms(float const* img, int const* rowImg, int rows, float* norm, unsigned int* maxNorm,

@@ -235,14 +235,13 @@ __global__ __launch_bounds__(256) void k_batch_mutual_local(BatchDev b, int cons
     if (tid == 0) blockCount[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
-// Single work-group: exclusive prefix over the block counts (each thread sums a run of
-// blocks, then a work-group scan), then the copy, and the run's totals.
-__global__ __launch_bounds__(1024) void k_batch_mutual_gather(int2 const* __restrict__ tmp,
-                                                              int const* __restrict__ blockCount, int nBlocks,
-                                                              int* __restrict__ stripes, int2* __restrict__ pairs,
+// Single work-group: exclusive prefix of the block counts -> blockOff (each thread sums a
+// run of blocks, then a work-group scan), and the run's totals.  The copy into place is a
+// separate kernel with one work-group per block.
+__global__ __launch_bounds__(1024) void k_batch_mutual_prefix(int const* __restrict__ blockCount, int nBlocks,
+                                                              int* __restrict__ blockOff, int* __restrict__ stripes,
                                                               int* __restrict__ counters)
 {
-    __shared__ int offs[1024];
     __shared__ int wsum[16];
     int const tid = threadIdx.x;
     int const lane = tid & 63;
@@ -266,39 +265,44 @@ __global__ __launch_bounds__(1024) void k_batch_mutual_gather(int2 const* __rest
     }
     if (lane == 63) wsum[wave] = incl;
     __syncthreads();
-    int woff = 0;
-    for (int w = 0; w < wave; ++w) woff += wsum[w];
-    offs[tid] = woff + incl - mine;
-    __syncthreads();
-    int total = 0;
-    for (int w = 0; w < 16; ++w) total += wsum[w];
-    // copy: wave-cooperative per block, blocks in order of the owning thread
-    for (int t = wave; t < 1024; t += 16) {
-        int o = offs[t];
-        for (int k = 0; k < per; ++k) {
-            int const blk = t * per + k;
-            if (blk >= nBlocks) break;
-            int const n = blockCount[blk];
-            for (int e = lane; e < n; e += 64) pairs[o + e] = tmp[(size_t)blk * 1024 + e];
-            o += n;
-        }
+    int off = incl - mine;
+    for (int w = 0; w < wave; ++w) off += wsum[w];
+    for (int k = 0; k < per; ++k) {
+        int const blk = b0 + k;
+        if (blk >= nBlocks) break;
+        blockOff[blk] = off;
+        off += blockCount[blk];
     }
     if (tid == 0) {
+        int total = 0;
+        for (int w = 0; w < 16; ++w) total += wsum[w];
         counters[kBCntPairs] = total;
         counters[kBCntFlagDone] = counters[kBCntFlag];
         counters[kBCntRescoredDone] = rescored;
     }
 }
 
+__global__ __launch_bounds__(256) void k_batch_mutual_copy(int2 const* __restrict__ tmp,
+                                                           int const* __restrict__ blockCount,
+                                                           int const* __restrict__ blockOff, int2* __restrict__ pairs)
+{
+    int const blk = blockIdx.x;
+    int const n = blockCount[blk];
+    int const o = blockOff[blk];
+    for (int e = threadIdx.x; e < n; e += 256) pairs[o + e] = tmp[(size_t)blk * 1024 + e];
+}
+
 void launch_batch_mutual(BatchDev const& b, int const* sel, int leftRows, int2* tmp, int* blockCount, int2* pairs,
                          int* pairCount, int* stripes, int* counters, hipStream_t st)
 {
     int const nb = (leftRows + 1023) / 1024;
+    int* blockOff = blockCount + nb;  // the count buffer holds 2 * nb ints
     if (nb > 0)
         hipLaunchKernelGGL(k_batch_mutual_local, dim3(nb), dim3(256), 0, st, b, sel, leftRows, tmp, blockCount,
                            pairCount);
-    hipLaunchKernelGGL(k_batch_mutual_gather, dim3(1), dim3(1024), 0, st, tmp, blockCount, nb, stripes, pairs,
+    hipLaunchKernelGGL(k_batch_mutual_prefix, dim3(1), dim3(1024), 0, st, blockCount, nb, blockOff, stripes,
                        counters);
+    if (nb > 0) hipLaunchKernelGGL(k_batch_mutual_copy, dim3(nb), dim3(256), 0, st, tmp, blockCount, blockOff, pairs);
 }
 
 // Input rows -> the aligned image buffer (one wave per destination row).  Row r of the

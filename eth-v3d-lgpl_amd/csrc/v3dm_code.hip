@@ -8,6 +8,7 @@
 // the block in its low 4 bits (row 8*(r/4) + 4*(l/32) + r%4, query l%32).  Keys order as
 // (score, row), so a top-2 over keys is three integer ops per two elements (max, med3,
 // max3); the block of each kept key is tracked once per block.
+#include "v3dm_batch.h"
 #include "v3dm_device.h"
 #include "v3dm_learned.h"
 #include "v3dm_short.h"
@@ -163,6 +164,32 @@ __global__ __launch_bounds__(256, 4) void k_pass_code(signed char const* __restr
                                                    Cand2* __restrict__ part, int ldq)
 {
     code_block(Q, nQ, C, nC, blockIdx.x, blockIdx.y, tilesPerSplit, part, ldq);
+}
+
+// Persistent form over a batch: item = (segment, query block, split, tiles per split).
+__global__ __launch_bounds__(256, 4) void k_batch_pass_code(BatchDev b, int4 const* __restrict__ items, int nItems,
+                                                            int* __restrict__ queue, Cand2* __restrict__ part)
+{
+    __shared__ int sItem;
+    for (;;) {
+        if (threadIdx.x == 0) sItem = atomicAdd(queue, 1);
+        __syncthreads();
+        int const it = __builtin_amdgcn_readfirstlane(sItem);
+        __syncthreads();
+        if (it >= nItems) break;
+        int4 const w = items[it];
+        BatchSeg const sg = b.seg[w.x];
+        signed char const* Q = b.codes + (size_t)b.imgBase[sg.img] * kCodeBytes;
+        signed char const* C = b.codes + (size_t)b.imgBase[sg.oImg] * kCodeBytes;
+        code_block(Q, sg.n, C, sg.nO, w.y, w.z, w.w, part + (size_t)sg.partOff * 2, sg.ld);
+    }
+}
+
+void launch_batch_pass_code(BatchDev const& b, int4 const* items, int nItems, int* queue, Cand2* part, int grid,
+                            hipStream_t st)
+{
+    if (nItems <= 0) return;
+    hipLaunchKernelGGL(k_batch_pass_code, dim3(grid), dim3(256), 0, st, b, items, nItems, queue, part);
 }
 
 void launch_pass_code(signed char const* Q, int nQ, signed char const* C, int nC, ShortPlan const& p, Cand2* part,

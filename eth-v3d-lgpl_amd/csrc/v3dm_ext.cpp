@@ -13,6 +13,8 @@ struct ExtState {
     __bf16* bL = nullptr;
     __bf16* bR = nullptr;
     bool bLValid = false, bRValid = false;
+    int* slots = nullptr;      // rescoring candidate lists: (capL + capR) x kRescoreSlots
+    int* slotCount = nullptr;  // (capL + capR)
     int* secL = nullptr;  // exact second best candidate per query (shortlist forms)
     int* secR = nullptr;
     bool haveSecond = false;
@@ -23,8 +25,6 @@ struct ExtState {
     bool f8LValid = false, f8RValid = false;
     signed char* cL = nullptr;  // learned codes (int8, 32 per feature)
     signed char* cR = nullptr;
-    float* zL = nullptr;  // network output before quantisation
-    float* zR = nullptr;
     bool cLValid = false, cRValid = false;
     float cScale = 0.0f;  // scale the codes were quantised with
     Cand2* partL = nullptr;  // kShortMaxSplits x capL x 2
@@ -56,6 +56,8 @@ int ensure(v3dm_context* h, ExtState*& e)
     EXTCHECK(h, hipMemsetAsync(e->bR, 0, (size_t)cr * kDim * sizeof(__bf16), h->stream));
     EXTCHECK(h, hipMalloc(&e->partL, (size_t)kShortMaxSplits * cl * 2 * sizeof(Cand2)));
     EXTCHECK(h, hipMalloc(&e->partR, (size_t)kShortMaxSplits * cr * 2 * sizeof(Cand2)));
+    EXTCHECK(h, hipMalloc(&e->slots, (size_t)(cl + cr) * kRescoreSlots * sizeof(int)));
+    EXTCHECK(h, hipMalloc(&e->slotCount, (size_t)(cl + cr) * sizeof(int)));
     EXTCHECK(h, hipMalloc(&e->secL, (size_t)cl * sizeof(int)));
     EXTCHECK(h, hipMalloc(&e->secR, (size_t)cr * sizeof(int)));
     e->bLValid = e->bRValid = false;
@@ -102,8 +104,8 @@ int enqueue_bf16(v3dm_context* h, float minScore, float minRatio)
         launch_pass_bf16(e->bR, nR, e->bL, nL, pr, e->partR, e->capR, st);
         if ((rc = prof_end(h))) return rc;
         launch_rescore(h->dL, h->dR, nL, nR, e->partL, pl.splits, e->capL, e->partR, pr.splits, e->capR, h->normL,
-                       h->normR, h->maxNormBits, kWinRelBf16, 0, minScore, minRatio, h->colMax, h->rowMax, e->secL,
-                       e->secR, h->stripes, st);
+                       h->normR, h->maxNormBits, kWinRelBf16, 0, minScore, minRatio, e->slots, e->slotCount,
+                       h->colMax, h->rowMax, e->secL, e->secR, h->stripes, st);
     }
     e->haveSecond = work;
     launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mtmp, h->mblock, h->stripes,
@@ -119,8 +121,6 @@ int ensure_codes(v3dm_context* h, ExtState*& e)
     if (e->cL) return V3DM_OK;
     EXTCHECK(h, hipMalloc(&e->cL, (size_t)e->capL * kCode));
     EXTCHECK(h, hipMalloc(&e->cR, (size_t)e->capR * kCode));
-    EXTCHECK(h, hipMalloc(&e->zL, (size_t)e->capL * kCode * sizeof(float)));
-    EXTCHECK(h, hipMalloc(&e->zR, (size_t)e->capR * kCode * sizeof(float)));
     e->cLValid = e->cRValid = false;
     return V3DM_OK;
 }
@@ -140,15 +140,10 @@ int enqueue_learned(v3dm_context* h, float minScore, float minRatio)
         if (e->cScale != scale) e->cLValid = e->cRValid = false;
         e->cScale = scale;
         // embedding: network forward on the FP32 rows (read up to the next multiple of 64,
-        // inside the allocation), then normalise + quantise; code rows >= n are zero
-        if (!e->cLValid) {
-            launch_mlp_forward_f32(h->dL, (nL + 63) / 64 * 64, *w, e->zL, st);
-            launch_code_quant(e->zL, nL, e->capL, scale, e->cL, st);
-        }
-        if (!e->cRValid) {
-            launch_mlp_forward_f32(h->dR, (nR + 63) / 64 * 64, *w, e->zR, st);
-            launch_code_quant(e->zR, nR, e->capR, scale, e->cR, st);
-        }
+        // inside the allocation) with the quantisation fused; code rows past n are never
+        // used (candidates past n are masked, queries past n not written)
+        if (!e->cLValid) launch_mlp_codes_f32(h->dL, (nL + 63) / 64 * 64, *w, scale, e->cL, st);
+        if (!e->cRValid) launch_mlp_codes_f32(h->dR, (nR + 63) / 64 * 64, *w, scale, e->cR, st);
         e->cLValid = e->cRValid = true;
         ShortPlan const pl = plan_short(nL, nR, 512);
         ShortPlan const pr = plan_short(nR, nL, 512);
@@ -159,8 +154,8 @@ int enqueue_learned(v3dm_context* h, float minScore, float minRatio)
         launch_pass_code(e->cR, nR, e->cL, nL, pr, e->partR, e->capR, st);
         if ((rc = prof_end(h))) return rc;
         launch_rescore(h->dL, h->dR, nL, nR, e->partL, pl.splits, e->capL, e->partR, pr.splits, e->capR, h->normL,
-                       h->normR, h->maxNormBits, 0.0f, h->learnedTopK, minScore, minRatio, h->colMax, h->rowMax,
-                       e->secL, e->secR, h->stripes, st);
+                       h->normR, h->maxNormBits, 0.0f, h->learnedTopK, minScore, minRatio, e->slots, e->slotCount,
+                       h->colMax, h->rowMax, e->secL, e->secR, h->stripes, st);
     }
     e->haveSecond = work;
     launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mtmp, h->mblock, h->stripes,
@@ -190,8 +185,8 @@ int enqueue_fp8(v3dm_context* h, float minScore, float minRatio)
         launch_pass_fp8(e->f8R, e->eR, nR, e->f8L, e->eL, nL, pr, e->partR, e->capR, st);
         if ((rc = prof_end(h))) return rc;
         launch_rescore(h->dL, h->dR, nL, nR, e->partL, pl.splits, e->capL, e->partR, pr.splits, e->capR, h->normL,
-                       h->normR, h->maxNormBits, kWinRelFp8, 0, minScore, minRatio, h->colMax, h->rowMax, e->secL,
-                       e->secR, h->stripes, st);
+                       h->normR, h->maxNormBits, kWinRelFp8, 0, minScore, minRatio, e->slots, e->slotCount,
+                       h->colMax, h->rowMax, e->secL, e->secR, h->stripes, st);
     }
     e->haveSecond = work;
     launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mtmp, h->mblock, h->stripes,
@@ -248,16 +243,18 @@ void ext_release(ExtState* e)
     if (e->bR) (void)hipFree(e->bR);
     if (e->partL) (void)hipFree(e->partL);
     if (e->partR) (void)hipFree(e->partR);
+    if (e->slots) (void)hipFree(e->slots);
+    if (e->slotCount) (void)hipFree(e->slotCount);
+    e->slots = e->slotCount = nullptr;
     if (e->secL) (void)hipFree(e->secL);
     if (e->secR) (void)hipFree(e->secR);
     e->secL = e->secR = nullptr;
     e->haveSecond = false;
     for (unsigned char* p : {e->f8L, e->f8R, e->eL, e->eR})
         if (p) (void)hipFree(p);
-    for (void* p : {(void*)e->cL, (void*)e->cR, (void*)e->zL, (void*)e->zR})
+    for (void* p : {(void*)e->cL, (void*)e->cR})
         if (p) (void)hipFree(p);
     e->cL = e->cR = nullptr;
-    e->zL = e->zR = nullptr;
     e->cLValid = e->cRValid = false;
     e->f8L = e->f8R = e->eL = e->eR = nullptr;
     e->f8LValid = e->f8RValid = false;

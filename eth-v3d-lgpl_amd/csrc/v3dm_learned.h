@@ -29,6 +29,9 @@ void launch_mlp_forward(__bf16 const* X, int Rp, MlpWeights const& w, __bf16* HT
 // Inference: codes z of FP32 descriptor rows [0, rows64) (rows64 a multiple of 64; the rows
 // must be readable, rows past the caller's count give ignored codes).
 void launch_mlp_forward_f32(float const* X, int rows64, MlpWeights const& w, float* Z, hipStream_t st);
+// Inference: quantised codes (int8 4q, 32 per row) of FP32 rows [0, rows64) in one kernel.
+void launch_mlp_codes_f32(float const* X, int rows64, MlpWeights const& w, float scale, signed char* codes,
+                          hipStream_t st);
 void launch_mlp_loss(float const* Z, int B, float tau, float* dC, int* hard, float* lossPart, hipStream_t st);
 void launch_mlp_norm_backward(float const* Z, float const* dC, int R, int Rp, float* dZf, __bf16* dZ, __bf16* dZT,
                               hipStream_t st);

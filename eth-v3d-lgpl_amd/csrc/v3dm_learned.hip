@@ -95,7 +95,8 @@ template <typename TIn>
 __global__ __launch_bounds__(256) void k_mlp_forward(TIn const* __restrict__ X, int Rp,
                                                      __bf16 const* __restrict__ W1b, float const* __restrict__ b1,
                                                      __bf16 const* __restrict__ W2b, float const* __restrict__ b2,
-                                                     __bf16* __restrict__ HT, float* __restrict__ Z)
+                                                     __bf16* __restrict__ HT, float* __restrict__ Z,
+                                                     signed char* __restrict__ codes, float codeScale)
 {
     __shared__ __attribute__((aligned(16))) __bf16 Hs[64 * kHsStride];
     int const lane = threadIdx.x & 63;
@@ -155,7 +156,18 @@ __global__ __launch_bounds__(256) void k_mlp_forward(TIn const* __restrict__ X, 
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         int const row = row0 + rb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-        Z[(size_t)row * kCode + c32] = z[r] + bias;
+        float const v = z[r] + bias;
+        if (Z) Z[(size_t)row * kCode + c32] = v;
+        if (codes) {
+            // unit-norm code quantised to q = rint(c * scale) in [-31, 31], stored as 4q; the
+            // row's 32 components are the 32 lanes of this half-wave
+            float s = v * v;
+#pragma unroll
+            for (int o = 16; o > 0; o >>= 1) s += __shfl_xor(s, o);
+            float const c = v * codeScale / fmaxf(sqrtf(s), 1e-12f);
+            int const q = (c == c) ? (int)rintf(fminf(fmaxf(c, -31.0f), 31.0f)) : 0;
+            codes[(size_t)row * kCode + c32] = (signed char)(4 * q);
+        }
     }
 }
 
@@ -517,14 +529,22 @@ void launch_mlp_rows_bf16(float const* src, int n, int Rp, __bf16* X, hipStream_
 void launch_mlp_forward(__bf16 const* X, int Rp, MlpWeights const& w, __bf16* HT, float* Z, hipStream_t st)
 {
     hipLaunchKernelGGL(k_mlp_forward<__bf16>, dim3(Rp / 64), dim3(256), 0, st, X, Rp, w.W1b, w.P + kOffB1, w.W2b,
-                       w.P + kOffB2, HT, Z);
+                       w.P + kOffB2, HT, Z, (signed char*)nullptr, 0.0f);
 }
 
 void launch_mlp_forward_f32(float const* X, int rows64, MlpWeights const& w, float* Z, hipStream_t st)
 {
     if (rows64 <= 0) return;
     hipLaunchKernelGGL(k_mlp_forward<float>, dim3(rows64 / 64), dim3(256), 0, st, X, rows64, w.W1b, w.P + kOffB1,
-                       w.W2b, w.P + kOffB2, (__bf16*)nullptr, Z);
+                       w.W2b, w.P + kOffB2, (__bf16*)nullptr, Z, (signed char*)nullptr, 0.0f);
+}
+
+void launch_mlp_codes_f32(float const* X, int rows64, MlpWeights const& w, float scale, signed char* codes,
+                          hipStream_t st)
+{
+    if (rows64 <= 0) return;
+    hipLaunchKernelGGL(k_mlp_forward<float>, dim3(rows64 / 64), dim3(256), 0, st, X, rows64, w.W1b, w.P + kOffB1,
+                       w.W2b, w.P + kOffB2, (__bf16*)nullptr, (float*)nullptr, codes, scale);
 }
 
 void launch_mlp_loss(float const* Z, int B, float tau, float* dC, int* hard, float* lossPart, hipStream_t st)

@@ -5,8 +5,8 @@
 // accumulation; every query keeps, per partition of its candidates, the two best
 // approximate scores with their indices.  The union of the partition lists is the
 // shortlist; the candidates whose approximate score lies within twice the BF16 error bound
-// of the approximate second best are rescored exactly (FP64 products, the oracle's
-// score), and the reference's selection rule (cu:133-210) runs on those exact scores.
+// of the approximate second best are rescored exactly (v3dm_rescore.hip), and the
+// reference's selection rule (cu:133-210) runs on those exact scores.
 #include "v3dm_device.h"
 #include "v3dm_kernels.h"
 #include "v3dm_short.h"
@@ -278,231 +278,6 @@ void launch_pass_bf16(__bf16 const* Q, int nQ, __bf16 const* C, int nC, ShortPla
                        part, ldq, INFINITY);
 }
 
-// ------------------------------------------------------------------------------ exact rescoring
-// One wave per query.  Lane e holds partition list e of the query (2 candidates each).
-// The approximate second best v2 of the union sets the window v >= v2 - win (win = twice
-// the per-score error bound); the candidates inside it are compacted into a list and
-// rescored exactly in batches of 8: the wave stages the query and 8 candidate rows in LDS
-// with coalesced loads, then lane k < 8 forms fl32(sum_k double(q_k) * double(c_k)) in
-// ascending k (the oracle's score).  The exact top-2 (first index on ties) of the rescored
-// candidates gives the reference's decision (cu:133-210).
-constexpr int kRescoreBatch = 8;
 
-// Orders the LDS accesses of one wave (each wave of k_rescore works in its own LDS slice,
-// and its trip counts differ from the other waves', so a work-group barrier is not used).
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
-constexpr int kRescoreRow = 132;  // floats per staged row (528 bytes: conflict-free b128 reads)
-
-// Top-2 with the index of the second best as well (same order rules as top2_merge; the
-// second's index prefers the smaller index among equal values).
-struct Top2x {
-    float b1, b2;
-    int a1, a2;
-};
-
-__device__ __forceinline__ Top2x top2x_merge(Top2x a, Top2x b)
-{
-    bool const takeB = (b.b1 > a.b1) || (b.b1 == a.b1 && b.a1 >= 0 && (a.a1 < 0 || b.a1 < a.a1));
-    Top2x const w = takeB ? b : a;
-    Top2x const l = takeB ? a : b;
-    bool const lo = (l.b1 > w.b2) || (l.b1 == w.b2 && l.a1 >= 0 && (w.a2 < 0 || l.a1 < w.a2));
-    return Top2x{w.b1, lo ? l.b1 : w.b2, w.a1, lo ? l.a1 : w.a2};
-}
-
-__device__ __forceinline__ Top2x top2x_shfl_xor(Top2x s, int mask)
-{
-    return Top2x{__shfl_xor(s.b1, mask), __shfl_xor(s.b2, mask), __shfl_xor(s.a1, mask), __shfl_xor(s.a2, mask)};
-}
-
-// Exact rescoring of one query by one wave; st = the wave's LDS rows, list = its 64-entry
-// candidate list.  Returns the decision; nWinOut = candidates rescored, secondOut = the
-// candidate of second best exact score among them (-1 if none).
-__device__ __forceinline__ int rescore_query(float const* __restrict__ Qd, float const* __restrict__ Cd,
-                                             Cand2 const* __restrict__ part, size_t ldq, int q, int nEnt, float nq,
-                                             float mo, float winRel, int topK, float minScore, float minRatio,
-                                             float* st, int* list, int& nWinOut, int& secondOut)
-{
-    int const lane = threadIdx.x & 63;
-    // the query row -> LDS (row 0)
-    if (lane < 32) *reinterpret_cast<float4*>(&st[4 * lane]) = reinterpret_cast<float4 const*>(Qd)[lane];
-
-    Cand2 c = {-INFINITY, -1, -INFINITY, -1};
-    if (lane < nEnt) c = part[((size_t)(lane >> 1) * ldq + q) * 2 + (lane & 1)];
-    float a1 = fmaxf(c.v1, c.v2), a2 = fminf(c.v1, c.v2);
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        float const b1 = __shfl_xor(a1, o), b2 = __shfl_xor(a2, o);
-        a2 = fmaxf(fminf(a1, b1), fmaxf(a2, b2));
-        a1 = fmaxf(a1, b1);
-    }
-    float thr = a2 - winRel * nq * mo;
-    if (topK > 0) {
-        // the topK-th largest listed value (each lane's two values are in order v1 >= v2)
-        float x1 = (c.j1 >= 0) ? c.v1 : -INFINITY;
-        float x2 = (c.j2 >= 0) ? c.v2 : -INFINITY;
-        float kth = -INFINITY;
-        for (int k = 0; k < topK; ++k) {
-            float m = x1;
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-            if (m == -INFINITY) break;
-            kth = m;
-            unsigned long long const own = __ballot(x1 == m);
-            if (lane == __ffsll((long long)own) - 1) {
-                x1 = x2;
-                x2 = -INFINITY;
-            }
-        }
-        thr = kth;
-    }
-
-    // compact the window candidates (each lane contributes up to 2, in lane order)
-    bool const in1 = c.j1 >= 0 && c.v1 >= thr;
-    bool const in2 = c.j2 >= 0 && c.v2 >= thr;
-    unsigned long long const b1m = __ballot(in1), b2m = __ballot(in2);
-    unsigned long long const lt = (1ull << lane) - 1ull;
-    int const pos = __popcll(b1m & lt) + __popcll(b2m & lt);
-    if (in1) list[pos] = c.j1;
-    if (in2) list[pos + (in1 ? 1 : 0)] = c.j2;
-    int const nWin = __popcll(b1m) + __popcll(b2m);
-
-    Top2x s = {kInit, kInit, -1, -1};
-    for (int k0 = 0; k0 < nWin; k0 += kRescoreBatch) {
-        int const nb = min(kRescoreBatch, nWin - k0);
-        wave_sync();  // list[] and the query row are visible; the previous batch is consumed
-        // nb candidate rows -> LDS rows 1..nb (32 float4 each; lanes split by row parity)
-#pragma unroll
-        for (int u = 0; u < kRescoreBatch / 2; ++u) {
-            int const r = 2 * u + (lane >> 5);
-            if (r < nb) {
-                int const j = list[k0 + r];
-                *reinterpret_cast<float4*>(&st[(1 + r) * kRescoreRow + 4 * (lane & 31)]) =
-                    reinterpret_cast<float4 const*>(Cd + (size_t)j * kDim)[lane & 31];
-            }
-        }
-        wave_sync();
-        if (lane < nb) {
-            float const* cr = &st[(1 + lane) * kRescoreRow];
-            double acc = 0.0;
-#pragma unroll 8
-            for (int k4 = 0; k4 < 32; ++k4) {
-                float4 const x = *reinterpret_cast<float4 const*>(&st[4 * k4]);
-                float4 const y = *reinterpret_cast<float4 const*>(cr + 4 * k4);
-                acc += (double)x.x * (double)y.x;
-                acc += (double)x.y * (double)y.y;
-                acc += (double)x.z * (double)y.z;
-                acc += (double)x.w * (double)y.w;
-            }
-            float const sc = (float)acc;
-            if (sc == sc) s = top2x_merge(s, Top2x{sc, kInit, list[k0 + lane], -1});
-        }
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) s = top2x_merge(s, top2x_shfl_xor(s, o));
-    wave_sync();  // the list and rows may be reused by the caller's next query
-    nWinOut = nWin;
-    secondOut = s.a2;
-    return decide_exact(Top2{s.b1, s.b2, s.a1}, minScore, minRatio);
-}
-
-__global__ __launch_bounds__(256) void k_rescore(float const* __restrict__ L, float const* __restrict__ R, int nL,
-                                                 int nR, Cand2 const* __restrict__ partL, int nPartL, int ldqL,
-                                                 Cand2 const* __restrict__ partR, int nPartR, int ldqR,
-                                                 float const* __restrict__ normL, float const* __restrict__ normR,
-                                                 unsigned int const* __restrict__ maxNormBits, float winRel,
-                                                 int topK, float minScore, float minRatio, int* __restrict__ colMax,
-                                                 int* __restrict__ rowMax, int* __restrict__ colSecond,
-                                                 int* __restrict__ rowSecond, int* __restrict__ stripes)
-{
-    __shared__ __attribute__((aligned(16))) float rows[4][(1 + kRescoreBatch) * kRescoreRow];
-    __shared__ int list[4][64];
-    __shared__ int wsum[4];
-    int const lane = threadIdx.x & 63;
-    int const wave = threadIdx.x >> 6;
-    int const g = blockIdx.x * 4 + wave;
-    int nWin = 0;
-    if (g < nL + nR) {
-        bool const left = g < nL;
-        int const q = left ? g : g - nL;
-        Cand2 const* part = left ? partL : partR;
-        int const nEnt = 2 * (left ? nPartL : nPartR);  // (split, half) lists, <= 64
-        size_t const ldq = left ? (size_t)ldqL : (size_t)ldqR;
-        float const* Qd = (left ? L : R) + (size_t)q * kDim;
-        float const* Cd = left ? R : L;
-        float const nq = left ? normL[q] : normR[q];
-        float const mo = __uint_as_float(maxNormBits[left ? 1 : 0]);
-        int second = -1;
-        int const d = rescore_query(Qd, Cd, part, ldq, q, nEnt, nq, mo, winRel, topK, minScore, minRatio,
-                                    rows[wave], list[wave], nWin, second);
-        if (lane == 0) {
-            (left ? colMax : rowMax)[q] = d;
-            (left ? colSecond : rowSecond)[q] = second;
-        }
-    }
-    if (lane == 0) wsum[wave] = nWin;
-    __syncthreads();
-    if (threadIdx.x == 0)
-        atomicAdd(&stripes[(blockIdx.x % kStripes) * kStripeStride], wsum[0] + wsum[1] + wsum[2] + wsum[3]);
-}
-
-void launch_rescore(float const* L, float const* R, int nL, int nR, Cand2 const* partL, int nPartL, int ldqL,
-                    Cand2 const* partR, int nPartR, int ldqR, float const* normL, float const* normR,
-                    unsigned int const* maxNormBits, float winRel, int topK, float minScore, float minRatio,
-                    int* colMax, int* rowMax, int* colSecond, int* rowSecond, int* stripes, hipStream_t st)
-{
-    int const n = nL + nR;
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_rescore, dim3((n + 3) / 4), dim3(256), 0, st, L, R, nL, nR, partL, nPartL, ldqL, partR,
-                       nPartR, ldqR, normL, normR, maxNormBits, winRel, topK, minScore, minRatio, colMax,
-                       rowMax, colSecond, rowSecond, stripes);
-}
-
-// Batched form: one wave per query row of the query space (padding rows idle).
-__global__ __launch_bounds__(256) void k_batch_rescore(BatchDev b, Cand2 const* __restrict__ part, float winRel,
-                                                       int topK, float minScore, float minRatio,
-                                                       int* __restrict__ sel,
-                                                       int* __restrict__ stripes)
-{
-    __shared__ __attribute__((aligned(16))) float rows[4][(1 + kRescoreBatch) * kRescoreRow];
-    __shared__ int list[4][64];
-    __shared__ int wsum[4];
-    int const lane = threadIdx.x & 63;
-    int const wave = threadIdx.x >> 6;
-    int const g = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(wave);
-    int nWin = 0;
-    if (g < b.qRows) {
-        BatchSeg const sg = b.seg[b.segOfBlock[g / kBatchRowAlign]];
-        int const q = g - sg.qOff;
-        if (q < sg.n && sg.nO > 0) {
-            int const row = b.imgBase[sg.img] + q;
-            float const* Qd = b.img + (size_t)row * kDim;
-            float const* Cd = b.img + (size_t)b.imgBase[sg.oImg] * kDim;
-            float const mo = __uint_as_float(b.maxNorm[sg.oImg]);
-            int second = -1;
-            int const d = rescore_query(Qd, Cd, part + (size_t)sg.partOff * 2, (size_t)sg.ld, q, 2 * sg.nParts,
-                                        b.norm[row], mo, winRel, topK, minScore, minRatio, rows[wave], list[wave],
-                                        nWin, second);
-            if (lane == 0) sel[g] = d;
-        } else if (lane == 0) {
-            sel[g] = -1;
-        }
-    }
-    if (lane == 0) wsum[wave] = nWin;
-    __syncthreads();
-    if (threadIdx.x == 0)
-        atomicAdd(&stripes[(blockIdx.x % kStripes) * kStripeStride], wsum[0] + wsum[1] + wsum[2] + wsum[3]);
-}
-
-void launch_batch_rescore(BatchDev const& b, Cand2 const* part, float winRel, int topK, float minScore,
-                          float minRatio, int* sel, int* stripes, hipStream_t st)
-{
-    if (b.qRows <= 0) return;
-    hipLaunchKernelGGL(k_batch_rescore, dim3((b.qRows + 3) / 4), dim3(256), 0, st, b, part, winRel, topK, minScore,
-                       minRatio, sel, stripes);
-}
 
 }  // namespace v3dm

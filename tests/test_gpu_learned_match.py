@@ -105,3 +105,23 @@ def test_topk_tradeoff(matcher):
     say("case top-k: " + " ".join(f"k={k}: recall {r[0]:.5f} rescored {r[2]['nRescored']}" for k, r in res.items()))
     assert res[16][0] >= res[2][0]
     assert res[16][2]["nRescored"] > res[2][2]["nRescored"]
+
+
+def test_learned_batch(matcher):
+    """The learned form through the batch API on an uneven batch: recall and precision
+    against the plain batch (identical to the single-pair exact path)."""
+    m, net = matcher
+    counts, desc, left, right, _ = synth.make_batch(24, lo=300, hi=6000, config=41)
+    m.batch_set_images(counts, desc)
+    m.batch_set_pairs(left, right)
+    m.set_precision(v3dmatch.PREC_FP32)
+    want = m.batch_match(0.9, 0.8)
+    m.set_precision(v3dmatch.PREC_LEARNED)
+    got = m.batch_match(0.9, 0.8)
+    ws = set((p, a, b) for p in range(len(left)) for a, b in want[p].tolist())
+    gs = set((p, a, b) for p in range(len(left)) for a, b in got[p].tolist())
+    inter = len(ws & gs)
+    say(f"case learned batch: exact={len(ws)} learned={len(gs)} recall={inter / len(ws):.5f} "
+        f"precision={inter / max(len(gs), 1):.5f} tuning {m.batch_get_tuning(v3dmatch.PREC_LEARNED)}")
+    assert inter >= 0.995 * len(ws) and inter >= 0.995 * len(gs)
+    m.set_precision(v3dmatch.PREC_FP32)
