@@ -22,43 +22,6 @@ constexpr int kCQ = 256;   // queries per work-group (4 waves x 2 blocks of 32)
 constexpr int kCT = 64;    // candidates per tile (2 blocks of 32)
 constexpr int kCodeBytes = 32;
 
-// Unit-norm codes -> int8 4q; rows >= n -> zero.  One thread per row.
-__global__ __launch_bounds__(256) void k_code_quant(float const* __restrict__ Z, int n, int rows, float scale,
-                                                    signed char* __restrict__ codes)
-{
-    int const r = blockIdx.x * 256 + threadIdx.x;
-    if (r >= rows) return;
-    float z[kCode];
-    float s = 0.0f;
-#pragma unroll
-    for (int e = 0; e < kCode; ++e) {
-        z[e] = r < n ? Z[(size_t)r * kCode + e] : 0.0f;
-        s += z[e] * z[e];
-    }
-    float const inv = scale / fmaxf(sqrtf(s), 1e-12f);
-    int w[kCode / 4];
-#pragma unroll
-    for (int g = 0; g < kCode / 4; ++g) {
-        unsigned int packed = 0;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            float const v = z[4 * g + e] * inv;
-            int q = (v == v) ? (int)rintf(fminf(fmaxf(v, -31.0f), 31.0f)) : 0;
-            packed |= ((unsigned int)(q * 4) & 0xffu) << (8 * e);
-        }
-        w[g] = (int)packed;
-    }
-    int4* out = reinterpret_cast<int4*>(codes + (size_t)r * kCodeBytes);
-    out[0] = make_int4(w[0], w[1], w[2], w[3]);
-    out[1] = make_int4(w[4], w[5], w[6], w[7]);
-}
-
-void launch_code_quant(float const* Z, int n, int rows, float scale, signed char* codes, hipStream_t st)
-{
-    if (rows <= 0) return;
-    hipLaunchKernelGGL(k_code_quant, dim3((rows + 255) / 256), dim3(256), 0, st, Z, n, rows, scale, codes);
-}
-
 struct KeyStateI {
     int k1, k2;
     int b1, b2;  // block (32-candidate) of k1 / k2
@@ -94,22 +57,29 @@ __device__ __forceinline__ void key_block_update(KeyStateI& s, v16i const& d, in
     s.k2 = k2;
 }
 
-template <bool kFull>
-__device__ __forceinline__ void code_tile(signed char const* __restrict__ C, int t, v4i const (&bq)[2],
-                                          v16i const& pos, KeyStateI (&st)[2], int nC, int h, int c32)
+__device__ __forceinline__ void load_code_tile(signed char const* __restrict__ C, int t, int h, int c32, v4i (&a)[2])
 {
-    v4i a[2];
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb)
         a[cb] = *reinterpret_cast<v4i const*>(C + (size_t)(t * kCT + cb * 32 + c32) * kCodeBytes + 16 * h);
+}
+
+// The four 32x32 blocks of a tile (2 candidate blocks x 2 query blocks): all four MFMAs
+// issue before the first update, so each update finds its block done.
+template <bool kFull>
+__device__ __forceinline__ void code_tile(v4i const (&a)[2], int t, v4i const (&bq)[2], v16i const& pos,
+                                          KeyStateI (&st)[2], int nC, int h)
+{
+    v16i d[2][2];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+        for (int qb = 0; qb < 2; ++qb) d[cb][qb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[cb], bq[qb], pos, 0, 0, 0);
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb) {
         int const blk = 2 * t + cb;
 #pragma unroll
-        for (int qb = 0; qb < 2; ++qb) {
-            v16i const d = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[cb], bq[qb], pos, 0, 0, 0);
-            key_block_update<kFull>(st[qb], d, blk, blk * 32, nC, h);
-        }
+        for (int qb = 0; qb < 2; ++qb) key_block_update<kFull>(st[qb], d[cb][qb], blk, blk * 32, nC, h);
     }
 }
 
@@ -140,8 +110,17 @@ __device__ __forceinline__ void code_block(signed char const* __restrict__ Q, in
     for (int qb = 0; qb < 2; ++qb) st[qb] = KeyStateI{INT_MIN, INT_MIN, -1, -1};
 
     int const tFull = (t1 * kCT > nC) ? t1 - 1 : t1;  // the last tile may be partial
-    for (int t = t0; t < tFull; ++t) code_tile<true>(C, t, bq, pos, st, nC, h, c32);
-    if (tFull < t1) code_tile<false>(C, tFull, bq, pos, st, nC, h, c32);
+    // the next tile's A operands are loaded while the current tile is processed (the index
+    // is clamped to the split's last tile, so every load stays in range)
+    v4i a[2], an[2];
+    load_code_tile(C, t0, h, c32, a);
+    for (int t = t0; t < tFull; ++t) {
+        load_code_tile(C, min(t + 1, t1 - 1), h, c32, an);
+        code_tile<true>(a, t, bq, pos, st, nC, h);
+        a[0] = an[0];
+        a[1] = an[1];
+    }
+    if (tFull < t1) code_tile<false>(a, tFull, bq, pos, st, nC, h);
 
 #pragma unroll
     for (int qb = 0; qb < 2; ++qb) {

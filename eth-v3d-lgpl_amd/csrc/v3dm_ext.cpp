@@ -139,14 +139,15 @@ int enqueue_learned(v3dm_context* h, float minScore, float minRatio)
     if (work) {
         if (e->cScale != scale) e->cLValid = e->cRValid = false;
         e->cScale = scale;
-        // embedding: network forward on the FP32 rows (read up to the next multiple of 64,
+        // embedding: network forward on the FP32 rows (read up to the next multiple of 128,
         // inside the allocation) with the quantisation fused; code rows past n are never
         // used (candidates past n are masked, queries past n not written)
-        if (!e->cLValid) launch_mlp_codes_f32(h->dL, (nL + 63) / 64 * 64, *w, scale, e->cL, st);
-        if (!e->cRValid) launch_mlp_codes_f32(h->dR, (nR + 63) / 64 * 64, *w, scale, e->cR, st);
+        if (!e->cLValid) launch_mlp_codes_f32(h->dL, (nL + 127) / 128 * 128, *w, scale, e->cL, st);
+        if (!e->cRValid) launch_mlp_codes_f32(h->dR, (nR + 127) / 128 * 128, *w, scale, e->cR, st);
         e->cLValid = e->cRValid = true;
-        ShortPlan const pl = plan_short(nL, nR, 512);
-        ShortPlan const pr = plan_short(nR, nL, 512);
+        // 1024 work-groups: the code pass runs four per CU
+        ShortPlan const pl = plan_short(nL, nR, 1024);
+        ShortPlan const pr = plan_short(nR, nL, 1024);
         if ((rc = prof_begin(h))) return rc;
         launch_pass_code(e->cL, nL, e->cR, nR, pl, e->partL, e->capL, st);
         if ((rc = prof_end(h))) return rc;

@@ -62,6 +62,7 @@ int ensure_state(v3dm_context* h, LearnedState*& L)
     LCHECK(h, hipMalloc(&L->w.W1b, kHid * 128 * sizeof(__bf16)));
     LCHECK(h, hipMalloc(&L->w.W2b, kCode * kHid * sizeof(__bf16)));
     LCHECK(h, hipMalloc(&L->w.W2Tb, kHid * kCode * sizeof(__bf16)));
+    LCHECK(h, hipMalloc(&L->w.W2p, kHid * kCode * sizeof(__bf16)));
     LCHECK(h, hipMalloc(&L->M, kParams * sizeof(float)));
     LCHECK(h, hipMalloc(&L->V, kParams * sizeof(float)));
     LCHECK(h, hipMalloc(&L->grad, kParams * sizeof(float)));
@@ -131,6 +132,7 @@ void learned_destroy(LearnedState*& L)
     freeDev(L->w.W1b);
     freeDev(L->w.W2b);
     freeDev(L->w.W2Tb);
+    freeDev(L->w.W2p);
     for (float** p : {&L->M, &L->V, &L->grad, &L->pool, &L->Z, &L->dC, &L->dZf, &L->part, &L->lossPart}) freeDev(*p);
     for (__bf16** p : {&L->X, &L->XT, &L->HT, &L->dZ, &L->dZT, &L->dHT}) freeDev(*p);
     freeDev(L->tri);

@@ -20,17 +20,16 @@ struct MlpWeights {
     __bf16* W1b = nullptr;   // [256][128]
     __bf16* W2b = nullptr;   // [32][256]
     __bf16* W2Tb = nullptr;  // [256][32]
+    __bf16* W2p = nullptr;   // W2 as the A operands of the inference kernel's second layer
 };
 
 void launch_mlp_gather(float const* pool, int const* tri, int const* order, int orderOff, int B, int Rp,
                        __bf16* X, __bf16* XT, hipStream_t st);
 void launch_mlp_rows_bf16(float const* src, int n, int Rp, __bf16* X, hipStream_t st);
 void launch_mlp_forward(__bf16 const* X, int Rp, MlpWeights const& w, __bf16* HT, float* Z, hipStream_t st);
-// Inference: codes z of FP32 descriptor rows [0, rows64) (rows64 a multiple of 64; the rows
-// must be readable, rows past the caller's count give ignored codes).
-void launch_mlp_forward_f32(float const* X, int rows64, MlpWeights const& w, float* Z, hipStream_t st);
-// Inference: quantised codes (int8 4q, 32 per row) of FP32 rows [0, rows64) in one kernel.
-void launch_mlp_codes_f32(float const* X, int rows64, MlpWeights const& w, float scale, signed char* codes,
+// Inference: quantised codes of FP32 rows [0, rows128) (a multiple of 128 rows, all
+// readable), 32 bytes per row in the kernel's own (fixed) order of the code dimensions.
+void launch_mlp_codes_f32(float const* X, int rows128, MlpWeights const& w, float scale, signed char* codes,
                           hipStream_t st);
 void launch_mlp_loss(float const* Z, int B, float tau, float* dC, int* hard, float* lossPart, hipStream_t st);
 void launch_mlp_norm_backward(float const* Z, float const* dC, int R, int Rp, float* dZf, __bf16* dZ, __bf16* dZT,

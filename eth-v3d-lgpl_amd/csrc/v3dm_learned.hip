@@ -95,8 +95,7 @@ template <typename TIn>
 __global__ __launch_bounds__(256) void k_mlp_forward(TIn const* __restrict__ X, int Rp,
                                                      __bf16 const* __restrict__ W1b, float const* __restrict__ b1,
                                                      __bf16 const* __restrict__ W2b, float const* __restrict__ b2,
-                                                     __bf16* __restrict__ HT, float* __restrict__ Z,
-                                                     signed char* __restrict__ codes, float codeScale)
+                                                     __bf16* __restrict__ HT, float* __restrict__ Z)
 {
     __shared__ __attribute__((aligned(16))) __bf16 Hs[64 * kHsStride];
     int const lane = threadIdx.x & 63;
@@ -156,18 +155,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward(TIn const* __restrict__ X, 
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         int const row = row0 + rb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-        float const v = z[r] + bias;
-        if (Z) Z[(size_t)row * kCode + c32] = v;
-        if (codes) {
-            // unit-norm code quantised to q = rint(c * scale) in [-31, 31], stored as 4q; the
-            // row's 32 components are the 32 lanes of this half-wave
-            float s = v * v;
-#pragma unroll
-            for (int o = 16; o > 0; o >>= 1) s += __shfl_xor(s, o);
-            float const c = v * codeScale / fmaxf(sqrtf(s), 1e-12f);
-            int const q = (c == c) ? (int)rintf(fminf(fmaxf(c, -31.0f), 31.0f)) : 0;
-            codes[(size_t)row * kCode + c32] = (signed char)(4 * q);
-        }
+        Z[(size_t)row * kCode + c32] = z[r] + bias;
     }
 }
 
@@ -482,7 +470,7 @@ __global__ __launch_bounds__(256) void k_mlp_adam(float const* __restrict__ part
                                                   float* __restrict__ M, float* __restrict__ V, float lr,
                                                   float beta1, float beta2, float eps, float bc1, float bc2,
                                                   int apply, __bf16* __restrict__ W1b, __bf16* __restrict__ W2b,
-                                                  __bf16* __restrict__ W2Tb)
+                                                  __bf16* __restrict__ W2Tb, __bf16* __restrict__ W2p)
 {
     int const p = blockIdx.x * 256 + threadIdx.x;
     if (p >= kParams) return;
@@ -505,7 +493,122 @@ __global__ __launch_bounds__(256) void k_mlp_adam(float const* __restrict__ part
         int const c = q / kHid, u = q - c * kHid;
         W2b[q] = (__bf16)w;
         W2Tb[u * kCode + c] = (__bf16)w;
+        // W2p: A operand (code c, lane half hh, element v) of step s of unit block ub, for
+        // the unit that element r = 8s + v of the first layer's D block holds:
+        // u % 32 = 8 (r / 4) + 4 hh + r % 4
+        int const ub = u >> 5, ul = u & 31;
+        int const r = 4 * (ul >> 3) + (ul & 3), hh = (ul >> 2) & 1;
+        int const s = r >> 3, v = r & 7;
+        W2p[((ub * 2 + s) * 64 + hh * 32 + c) * 8 + v] = (__bf16)w;
     }
+}
+
+// ------------------------------------------------------------------------------ inference codes
+// Codes of descriptor rows, persistent over 128-row blocks (grid-stride).  W1 is staged once
+// per work-group in LDS (rows padded to 272 bytes: the 32 rows one A-operand read touches
+// fall on distinct bank groups).  Each wave takes 32 rows:
+//   layer 1, transposed: D1[unit][row] = W1 . x (A = W1 rows from LDS, B = the wave's rows,
+//     converted to bf16 once per block and kept in registers), one 32-unit block at a time;
+//   bias + ReLU in registers; the 16 hidden units a lane holds are the B operand of layer 2
+//     as they are (k order permuted), with W2 read in the same permutation (W2p);
+//   layer 2: D2[code][row]; lane (row, half) holds 16 of the row's codes: bias, norm over
+//     the two halves, quantisation, one 16-byte store (bytes 16*half .. +15 of the row, in
+//     element order: a fixed permutation of the code dimensions, the same for every row).
+constexpr int kW1Stride = 136;  // bf16 per LDS row of W1 (128 + 8)
+
+__global__ __launch_bounds__(256, 2) void k_mlp_codes(float const* __restrict__ X, int nBlocks,
+                                                      __bf16 const* __restrict__ W1b, float const* __restrict__ b1,
+                                                      __bf16 const* __restrict__ W2p, float const* __restrict__ b2,
+                                                      float scale, signed char* __restrict__ codes)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 W1s[kHid * kW1Stride];
+    __shared__ __attribute__((aligned(16))) float b1s[kHid];
+    int const tid = threadIdx.x;
+    int const lane = tid & 63;
+    int const wave = tid >> 6;
+    int const h = lane >> 5;
+    int const c32 = lane & 31;
+    // W1 -> LDS: 256 rows x 16 chunks of 16 bytes
+    for (int f = tid; f < kHid * 16; f += 256) {
+        int const row = f >> 4, ch = f & 15;
+        *reinterpret_cast<uint4*>(&W1s[row * kW1Stride + ch * 8]) =
+            reinterpret_cast<uint4 const*>(W1b + (size_t)row * 128)[ch];
+    }
+    b1s[tid] = b1[tid];
+    __syncthreads();
+    float bz[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) bz[r] = b2[8 * (r >> 2) + 4 * h + (r & 3)];
+
+    for (int blk = blockIdx.x; blk < nBlocks; blk += gridDim.x) {
+        int const row = blk * 128 + wave * 32 + c32;
+        v8bf xb[8];
+        {
+            float4 const* xp = reinterpret_cast<float4 const*>(X + (size_t)row * kDim);
+#pragma unroll
+            for (int ks = 0; ks < 8; ++ks) {
+                float4 const u = xp[4 * ks + 2 * h];
+                float4 const v = xp[4 * ks + 2 * h + 1];
+                xb[ks] = v8bf{(__bf16)u.x, (__bf16)u.y, (__bf16)u.z, (__bf16)u.w,
+                              (__bf16)v.x, (__bf16)v.y, (__bf16)v.z, (__bf16)v.w};
+            }
+        }
+        v16f z = v16f{};
+#pragma unroll 2
+        for (int ub = 0; ub < 8; ++ub) {
+            v16f a = v16f{};
+            __bf16 const* wrow = &W1s[(ub * 32 + c32) * kW1Stride + 8 * h];
+#pragma unroll
+            for (int ks = 0; ks < 8; ++ks)
+                a = mfma_bf16(*reinterpret_cast<v8bf const*>(wrow + 16 * ks), xb[ks], a);
+            float4 const* bp = reinterpret_cast<float4 const*>(&b1s[ub * 32 + 4 * h]);
+            v8bf hb[2];
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                float4 const bb = bp[2 * g];  // units ub*32 + 8g + 4h + 0..3
+                hb[g >> 1][4 * (g & 1) + 0] = (__bf16)fmaxf(a[4 * g + 0] + bb.x, 0.0f);
+                hb[g >> 1][4 * (g & 1) + 1] = (__bf16)fmaxf(a[4 * g + 1] + bb.y, 0.0f);
+                hb[g >> 1][4 * (g & 1) + 2] = (__bf16)fmaxf(a[4 * g + 2] + bb.z, 0.0f);
+                hb[g >> 1][4 * (g & 1) + 3] = (__bf16)fmaxf(a[4 * g + 3] + bb.w, 0.0f);
+            }
+#pragma unroll
+            for (int s = 0; s < 2; ++s) z = mfma_bf16(load8(W2p + ((ub * 2 + s) * 64 + lane) * 8), hb[s], z);
+        }
+        float s2 = 0.0f;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            z[r] += bz[r];
+            s2 += z[r] * z[r];
+        }
+        s2 += __shfl_xor(s2, 32);
+        float const inv = scale / fmaxf(sqrtf(s2), 1e-12f);
+        unsigned int w[4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            unsigned int p = 0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                float const c = z[4 * g + e] * inv;
+                int const q = (c == c) ? (int)rintf(fminf(fmaxf(c, -31.0f), 31.0f)) : 0;
+                p |= ((unsigned int)(4 * q) & 0xffu) << (8 * e);
+            }
+            w[g] = p;
+        }
+        *reinterpret_cast<uint4*>(codes + (size_t)row * kCode + 16 * h) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+
+void launch_mlp_codes_f32(float const* X, int rows128, MlpWeights const& w, float scale, signed char* codes,
+                          hipStream_t st)
+{
+    int const nBlocks = rows128 / 128;
+    if (nBlocks <= 0) return;
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    int const grid = nBlocks < 2 * cus ? nBlocks : 2 * cus;
+    hipLaunchKernelGGL(k_mlp_codes, dim3(grid), dim3(256), 0, st, X, nBlocks, w.W1b, w.P + kOffB1, w.W2p,
+                       w.P + kOffB2, scale, codes);
 }
 
 __global__ void k_fill_f32(float* __restrict__ p, size_t n, float v)
@@ -529,23 +632,10 @@ void launch_mlp_rows_bf16(float const* src, int n, int Rp, __bf16* X, hipStream_
 void launch_mlp_forward(__bf16 const* X, int Rp, MlpWeights const& w, __bf16* HT, float* Z, hipStream_t st)
 {
     hipLaunchKernelGGL(k_mlp_forward<__bf16>, dim3(Rp / 64), dim3(256), 0, st, X, Rp, w.W1b, w.P + kOffB1, w.W2b,
-                       w.P + kOffB2, HT, Z, (signed char*)nullptr, 0.0f);
+                       w.P + kOffB2, HT, Z);
 }
 
-void launch_mlp_forward_f32(float const* X, int rows64, MlpWeights const& w, float* Z, hipStream_t st)
-{
-    if (rows64 <= 0) return;
-    hipLaunchKernelGGL(k_mlp_forward<float>, dim3(rows64 / 64), dim3(256), 0, st, X, rows64, w.W1b, w.P + kOffB1,
-                       w.W2b, w.P + kOffB2, (__bf16*)nullptr, Z, (signed char*)nullptr, 0.0f);
-}
 
-void launch_mlp_codes_f32(float const* X, int rows64, MlpWeights const& w, float scale, signed char* codes,
-                          hipStream_t st)
-{
-    if (rows64 <= 0) return;
-    hipLaunchKernelGGL(k_mlp_forward<float>, dim3(rows64 / 64), dim3(256), 0, st, X, rows64, w.W1b, w.P + kOffB1,
-                       w.W2b, w.P + kOffB2, (__bf16*)nullptr, (float*)nullptr, codes, scale);
-}
 
 void launch_mlp_loss(float const* Z, int B, float tau, float* dC, int* hard, float* lossPart, hipStream_t st)
 {
@@ -578,7 +668,8 @@ void launch_mlp_adam(float const* part, int nChunks, float* grad, MlpWeights con
     float const beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
     float const bc1 = 1.0f - powf(beta1, (float)step), bc2 = 1.0f - powf(beta2, (float)step);
     hipLaunchKernelGGL(k_mlp_adam, dim3((kParams + 255) / 256), dim3(256), 0, st, part, nChunks, grad, w.P, M, V,
-                       lr, beta1, beta2, eps, apply ? bc1 : 1.0f, apply ? bc2 : 1.0f, apply, w.W1b, w.W2b, w.W2Tb);
+                       lr, beta1, beta2, eps, apply ? bc1 : 1.0f, apply ? bc2 : 1.0f, apply, w.W1b, w.W2b, w.W2Tb,
+                       w.W2p);
 }
 
 void launch_fill(float* p, size_t n, float v, hipStream_t st)

@@ -50,7 +50,6 @@ void launch_rescore(float const* L, float const* R, int nL, int nR, Cand2 const*
                     int* slots, int* count, int* colMax, int* rowMax, int* colSecond, int* rowSecond, int* stripes,
                     hipStream_t st);
 // learned code pass (v3dm_code.hip)
-void launch_code_quant(float const* Z, int n, int rows, float scale, signed char* codes, hipStream_t st);
 void launch_pass_code(signed char const* Q, int nQ, signed char const* C, int nC, ShortPlan const& p, Cand2* part,
                       int ldq, hipStream_t st);
 
