@@ -197,6 +197,9 @@ def main():
                        "dtype": dtype, "peak": peak, "matches": len(gs),
                        "recall": inter / max(len(ex), 1), "precision": inter / max(len(gs), 1),
                        "rescored_candidates": int(st["nRescored"]), "speedup_vs_plain": ms_plain / ms}
+        if prec == v3dmatch.PREC_LEARNED:
+            forms[name]["sampled_check"] = {"samples": int(st["nSampled"]), "missed": int(st["nSampleMissed"]),
+                                            "fallback_to_plain": int(st["nFallback"])}
 
     time_shortlist(v3dmatch.PREC_BF16, "bf16_shortlist", "k_pass_bf16", flop, "bf16", PEAK_BF16_MFMA)
     time_shortlist(v3dmatch.PREC_FP8, "fp8_shortlist", "k_pass_fp8", flop, "fp8", PEAK_FP8_MFMA)

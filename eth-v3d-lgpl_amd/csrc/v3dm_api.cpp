@@ -332,6 +332,8 @@ int setFeatures(v3dm_context* h, bool left, int n, const void* src, Source kind)
 int copyResults(v3dm_context* h, int32_t* pairs, int cap, int* nPairs)
 {
     if (!h->haveResult) return fail(h, V3DM_ERR_STATE, "no result: call findPutativeMatches first");
+    int const rcf = ext_finish(h);
+    if (rcf) return rcf;
     size_t const headBytes = kNumCounters * sizeof(int);
     HIPCHECK(h, hipMemcpyAsync(h->hostStage, h->results, headBytes, hipMemcpyDeviceToHost, h->stream));
     HIPCHECK(h, hipStreamSynchronize(h->stream));
@@ -390,6 +392,16 @@ int v3dm_set_learned_topk(v3dm_handle h, int k)
     return V3DM_OK;
 }
 
+int v3dm_set_fallback(v3dm_handle h, int samples, float minRecall)
+{
+    if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
+    if (samples < 0 || samples > kSampleRows || !(minRecall >= 0.0f) || !(minRecall <= 1.0f))
+        return fail(h, V3DM_ERR_INVALID, "samples must be in [0, %d] and minRecall in [0, 1]", kSampleRows);
+    h->fallbackSamples = samples;
+    h->fallbackRecall = minRecall;
+    return V3DM_OK;
+}
+
 int v3dm_set_derived_cache(v3dm_handle h, int enable)
 {
     if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
@@ -410,6 +422,11 @@ int v3dm_find_putative_matches_async(v3dm_handle h, float minScore, float minRat
     if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
     if (!h->allocated) return fail(h, V3DM_ERR_STATE, "findPutativeMatches before allocate");
     HIPCHECK(h, hipSetDevice(h->device));
+    h->lastMinScore = minScore;
+    h->lastMinRatio = minRatio;
+    h->stats.nFallback = 0;
+    h->stats.nSampled = 0;
+    h->stats.nSampleMissed = 0;
     h->masks.clear();
     if (!h->cacheDerived) {
         ext_features_changed(h, true);
@@ -456,6 +473,8 @@ int v3dm_get_maxima(v3dm_handle h, int32_t* colMax, int32_t* rowMax)
     if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
     if (!h->haveResult) return fail(h, V3DM_ERR_STATE, "no result: call findPutativeMatches first");
     HIPCHECK(h, hipSetDevice(h->device));
+    int const rcf = ext_finish(h);
+    if (rcf) return rcf;
     if (colMax && h->lastNL > 0)
         HIPCHECK(h, hipMemcpyAsync(colMax, h->colMax, (size_t)h->lastNL * sizeof(int), hipMemcpyDeviceToHost,
                                    h->stream));
@@ -509,6 +528,8 @@ int v3dm_get_stats(v3dm_handle h, v3dm_stats* out)
         if (rc) return rc;
     }
     if (h->haveResult) {
+        int const rcf = ext_finish(h);
+        if (rcf) return rcf;
         int cnt[kNumCounters];
         HIPCHECK(h, hipMemcpyAsync(cnt, h->results, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
         HIPCHECK(h, hipStreamSynchronize(h->stream));
@@ -516,7 +537,6 @@ int v3dm_get_stats(v3dm_handle h, v3dm_stats* out)
         h->stats.nRechecked = cnt[kCntFlagDone];
         h->stats.nRescored = cnt[kCntRescoredDone];
     }
-    h->stats.nFallback = ext_fallbacks(h);
     *out = h->stats;
     out->kernelMs = h->kernelMs;
     out->kernelLaunches = h->kernelLaunches;

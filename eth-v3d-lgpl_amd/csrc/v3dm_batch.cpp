@@ -88,8 +88,6 @@ struct BatchState {
     // selection / results
     int* sel = nullptr;
     int* flagList = nullptr;
-    int* slots = nullptr;      // rescoring lists (qRows x kRescoreSlots), shortlist forms
-    int* slotCount = nullptr;  // qRows
     int selCap = 0;
     int* counters = nullptr;  // kBatchCounters ints
     int* stripes = nullptr;   // striped counts, cleared by the gather kernel
@@ -319,11 +317,6 @@ int ensure_query_space(v3dm_context* h, BatchState* b)
     int rc;
     if ((rc = growi(h, b->segOfBlock, b->segBlkCap, nBlk))) return rc;
     if ((rc = growi(h, b->sel, b->selCap, b->qRows))) return rc;
-    if (b->slots) (void)hipFree(b->slots);
-    if (b->slotCount) (void)hipFree(b->slotCount);
-    b->slots = b->slotCount = nullptr;
-    BCHECK(h, hipMalloc(&b->slots, (size_t)std::max(b->qRows, 1) * kRescoreSlots * sizeof(int)));
-    BCHECK(h, hipMalloc(&b->slotCount, (size_t)std::max(b->qRows, 1) * sizeof(int)));
     if (b->flagList) (void)hipFree(b->flagList);
     b->flagList = nullptr;
     BCHECK(h, hipMalloc(&b->flagList, (size_t)std::max(b->qRows, 1) * sizeof(int)));
@@ -387,12 +380,10 @@ int enqueue_run(v3dm_context* h, BatchState* b, int mode, float minScore, float 
         launch_batch_recheck(d, b->flagList, b->counters, minScore, minRatio, b->sel, 1024, st);
     } else if (form == 1) {
         launch_batch_pass_bf16(d, b->items[1], (int)pl.items.size(), b->counters + 1, b->partB, pl.grid, st);
-        launch_batch_rescore(d, b->partB, kWinRelBf16, 0, minScore, minRatio, b->slots, b->slotCount, b->sel,
-                             b->stripes, st);
+        launch_batch_rescore(d, b->partB, kWinRelBf16, 0, minScore, minRatio, b->sel, b->stripes, st);
     } else {
         launch_batch_pass_code(d, b->items[2], (int)pl.items.size(), b->counters + 2, b->partB, pl.grid, st);
-        launch_batch_rescore(d, b->partB, 0.0f, h->learnedTopK, minScore, minRatio, b->slots, b->slotCount, b->sel,
-                             b->stripes, st);
+        launch_batch_rescore(d, b->partB, 0.0f, h->learnedTopK, minScore, minRatio, b->sel, b->stripes, st);
     }
     launch_batch_mutual(d, b->sel, b->leftRows, b->tmp, b->blockCount, b->out, b->pairCount, b->stripes,
                         b->counters, st);
@@ -592,7 +583,7 @@ void batch_destroy(BatchState*& b)
     for (void* p : {(void*)b->img, (void*)b->imgB, (void*)b->codes, (void*)b->imgBase, (void*)b->blkImg, (void*)b->nImgD,
                     (void*)b->srcRow, (void*)b->norm, (void*)b->maxNorm, b->upload, (void*)b->segOfBlock,
                     (void*)b->items[0], (void*)b->items[1], (void*)b->segD[0], (void*)b->segD[1], (void*)b->partF,
-                    (void*)b->partB, (void*)b->sel, (void*)b->flagList, (void*)b->slots, (void*)b->slotCount, (void*)b->counters, (void*)b->stripes, (void*)b->pairCount,
+                    (void*)b->partB, (void*)b->sel, (void*)b->flagList,  (void*)b->counters, (void*)b->stripes, (void*)b->pairCount,
                     (void*)b->tmp, (void*)b->blockCount, (void*)b->out})
         if (p) (void)hipFree(p);
     if (b->pinned) (void)hipHostFree(b->pinned);

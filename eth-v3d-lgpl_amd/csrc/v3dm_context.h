@@ -22,6 +22,9 @@ struct v3dm_context {
     int precision = V3DM_PREC_FP32;
     float epsRel = v3dm::kEpsRelFp32;  // 0: every query goes to the exact re-check
     int learnedTopK = 8;               // candidates per query rescored in the learned mode
+    int fallbackSamples = v3dm::kSampleRows;  // sampled check of learned results (0: off)
+    float fallbackRecall = 0.995f;     // sampled recall below this -> the plain form is run
+    float lastMinScore = 0.9f, lastMinRatio = 0.8f;
     bool cacheDerived = true;          // keep bf16 / fp8 / code copies while features are unchanged
 
     int nMaxL = 0, nMaxR = 0;  // rounded maxima (cu:444-463)

@@ -13,8 +13,6 @@ struct ExtState {
     __bf16* bL = nullptr;
     __bf16* bR = nullptr;
     bool bLValid = false, bRValid = false;
-    int* slots = nullptr;      // rescoring candidate lists: (capL + capR) x kRescoreSlots
-    int* slotCount = nullptr;  // (capL + capR)
     int* secL = nullptr;  // exact second best candidate per query (shortlist forms)
     int* secR = nullptr;
     bool haveSecond = false;
@@ -27,9 +25,25 @@ struct ExtState {
     signed char* cR = nullptr;
     bool cLValid = false, cRValid = false;
     float cScale = 0.0f;  // scale the codes were quantised with
+
+    // sampled exact check of a learned result (kSampleRows left features)
+    float* sL = nullptr;       // sampled rows
+    float* sNorm = nullptr;
+    int* sIdx = nullptr;
+    Top2* sColPart = nullptr;  // kMaxSplits x kSampleRows
+    Top2* sRowPart = nullptr;  // capR (written by the fused kernel, not used)
+    int* sDec = nullptr;       // exact decisions of the samples (+1 unused right slot)
+    int* sFlags = nullptr;     // open-query list of the samples
+    Top2* sRpart = nullptr;    // re-check partials
+    int* sCounters = nullptr;  // kNumCounters
+    int* sOut = nullptr;       // accepted, missed, wrongly accepted
+    int* sHost = nullptr;      // pinned copy of sOut
+    hipStream_t side = nullptr;  // the check runs beside the learned passes
+    hipEvent_t evFork = nullptr, evJoin = nullptr;
+    bool sPending = false;
+    int sSamples = 0;
     Cand2* partL = nullptr;  // kShortMaxSplits x capL x 2
     Cand2* partR = nullptr;
-    int fallbacks = 0;
 };
 
 namespace {
@@ -56,8 +70,6 @@ int ensure(v3dm_context* h, ExtState*& e)
     EXTCHECK(h, hipMemsetAsync(e->bR, 0, (size_t)cr * kDim * sizeof(__bf16), h->stream));
     EXTCHECK(h, hipMalloc(&e->partL, (size_t)kShortMaxSplits * cl * 2 * sizeof(Cand2)));
     EXTCHECK(h, hipMalloc(&e->partR, (size_t)kShortMaxSplits * cr * 2 * sizeof(Cand2)));
-    EXTCHECK(h, hipMalloc(&e->slots, (size_t)(cl + cr) * kRescoreSlots * sizeof(int)));
-    EXTCHECK(h, hipMalloc(&e->slotCount, (size_t)(cl + cr) * sizeof(int)));
     EXTCHECK(h, hipMalloc(&e->secL, (size_t)cl * sizeof(int)));
     EXTCHECK(h, hipMalloc(&e->secR, (size_t)cr * sizeof(int)));
     e->bLValid = e->bRValid = false;
@@ -104,7 +116,7 @@ int enqueue_bf16(v3dm_context* h, float minScore, float minRatio)
         launch_pass_bf16(e->bR, nR, e->bL, nL, pr, e->partR, e->capR, st);
         if ((rc = prof_end(h))) return rc;
         launch_rescore(h->dL, h->dR, nL, nR, e->partL, pl.splits, e->capL, e->partR, pr.splits, e->capR, h->normL,
-                       h->normR, h->maxNormBits, kWinRelBf16, 0, minScore, minRatio, e->slots, e->slotCount,
+                       h->normR, h->maxNormBits, kWinRelBf16, 0, minScore, minRatio,
                        h->colMax, h->rowMax, e->secL, e->secR, h->stripes, st);
     }
     e->haveSecond = work;
@@ -122,6 +134,62 @@ int ensure_codes(v3dm_context* h, ExtState*& e)
     EXTCHECK(h, hipMalloc(&e->cL, (size_t)e->capL * kCode));
     EXTCHECK(h, hipMalloc(&e->cR, (size_t)e->capR * kCode));
     e->cLValid = e->cRValid = false;
+    return V3DM_OK;
+}
+
+int ensure_sample(v3dm_context* h, ExtState* e)
+{
+    if (e->sL) return V3DM_OK;
+    EXTCHECK(h, hipMalloc(&e->sL, (size_t)kSampleRows * kDim * sizeof(float)));
+    EXTCHECK(h, hipMalloc(&e->sNorm, kSampleRows * sizeof(float)));
+    EXTCHECK(h, hipMalloc(&e->sIdx, kSampleRows * sizeof(int)));
+    EXTCHECK(h, hipMalloc(&e->sColPart, (size_t)kSampleMaxSplits * kSampleRows * sizeof(Top2)));
+    EXTCHECK(h, hipMalloc(&e->sRowPart, (size_t)e->capR * sizeof(Top2)));
+    EXTCHECK(h, hipMalloc(&e->sDec, (kSampleRows + 1) * sizeof(int)));
+    EXTCHECK(h, hipMalloc(&e->sFlags, (kSampleRows + 1) * sizeof(int)));
+    EXTCHECK(h, hipMalloc(&e->sRpart, (size_t)(kSampleRows + 1) * kRecheckMaxChunks * sizeof(Top2)));
+    EXTCHECK(h, hipMalloc(&e->sCounters, kNumCounters * sizeof(int)));
+    EXTCHECK(h, hipMalloc(&e->sOut, 4 * sizeof(int)));
+    EXTCHECK(h, hipHostMalloc(&e->sHost, 4 * sizeof(int), hipHostMallocDefault));
+    EXTCHECK(h, hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking));
+    EXTCHECK(h, hipEventCreateWithFlags(&e->evFork, hipEventDisableTiming));
+    EXTCHECK(h, hipEventCreateWithFlags(&e->evJoin, hipEventDisableTiming));
+    return V3DM_OK;
+}
+
+// The plain exact path on S sampled left features against all right features, on a side
+// stream from the fork point (it needs only the features); the comparison with the learned
+// decisions joins the handle's stream and lands in sOut (read back with the result).
+int enqueue_sample_fork(v3dm_context* h, ExtState* e, float minScore, float minRatio)
+{
+    int rc = ensure_sample(h, e);
+    if (rc) return rc;
+    EXTCHECK(h, hipEventRecord(e->evFork, h->stream));
+    EXTCHECK(h, hipStreamWaitEvent(e->side, e->evFork, 0));
+    hipStream_t const st = e->side;
+    int const nL = h->nL, nR = h->nR;
+    int const S = std::min(std::min(h->fallbackSamples, kSampleRows), nL);
+    launch_sample_gather(h->dL, h->normL, nL, S, e->sL, e->sNorm, e->sIdx, st);
+    FusedPlan const p = plan_fused_fp32(S, nR, kSampleMaxSplits, kSampleMaxSplits);
+    launch_fused_fp32(e->sL, h->dR, S, nR, p, e->sColPart, kSampleRows, e->sRowPart, e->capR, st);
+    EXTCHECK(h, hipMemsetAsync(e->sCounters, 0, kNumCounters * sizeof(int), st));
+    launch_merge_decide(e->sColPart, p.splits, kSampleRows, e->sRowPart, p.leftBlocks, e->capR, S, 0, e->sNorm,
+                        h->normR, h->maxNormBits, h->epsRel, minScore, minRatio, e->sDec, e->sDec + kSampleRows,
+                        e->sFlags, kSampleRows, e->sCounters, st);
+    launch_recheck_exact(e->sL, h->dR, S, nR, e->sFlags, kSampleRows, e->sCounters, minScore, minRatio, e->sDec,
+                         e->sDec + kSampleRows, e->sRpart, kRecheckGrid, st);
+    EXTCHECK(h, hipEventRecord(e->evJoin, st));
+    e->sSamples = S;
+    return V3DM_OK;
+}
+
+int enqueue_sample_join(v3dm_context* h, ExtState* e)
+{
+    hipStream_t const st = h->stream;
+    EXTCHECK(h, hipStreamWaitEvent(st, e->evJoin, 0));
+    launch_sample_compare(e->sDec, e->sIdx, e->sSamples, h->colMax, e->sOut, st);
+    EXTCHECK(h, hipMemcpyAsync(e->sHost, e->sOut, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+    e->sPending = true;
     return V3DM_OK;
 }
 
@@ -145,6 +213,9 @@ int enqueue_learned(v3dm_context* h, float minScore, float minRatio)
         if (!e->cLValid) launch_mlp_codes_f32(h->dL, (nL + 127) / 128 * 128, *w, scale, e->cL, st);
         if (!e->cRValid) launch_mlp_codes_f32(h->dR, (nR + 127) / 128 * 128, *w, scale, e->cR, st);
         e->cLValid = e->cRValid = true;
+        e->sPending = false;
+        bool const check = h->fallbackSamples > 0;
+        if (check && (rc = enqueue_sample_fork(h, e, minScore, minRatio))) return rc;
         // 1024 work-groups: the code pass runs four per CU
         ShortPlan const pl = plan_short(nL, nR, 1024);
         ShortPlan const pr = plan_short(nR, nL, 1024);
@@ -155,8 +226,9 @@ int enqueue_learned(v3dm_context* h, float minScore, float minRatio)
         launch_pass_code(e->cR, nR, e->cL, nL, pr, e->partR, e->capR, st);
         if ((rc = prof_end(h))) return rc;
         launch_rescore(h->dL, h->dR, nL, nR, e->partL, pl.splits, e->capL, e->partR, pr.splits, e->capR, h->normL,
-                       h->normR, h->maxNormBits, 0.0f, h->learnedTopK, minScore, minRatio, e->slots, e->slotCount,
+                       h->normR, h->maxNormBits, 0.0f, h->learnedTopK, minScore, minRatio,
                        h->colMax, h->rowMax, e->secL, e->secR, h->stripes, st);
+        if (check && (rc = enqueue_sample_join(h, e))) return rc;
     }
     e->haveSecond = work;
     launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mtmp, h->mblock, h->stripes,
@@ -186,7 +258,7 @@ int enqueue_fp8(v3dm_context* h, float minScore, float minRatio)
         launch_pass_fp8(e->f8R, e->eR, nR, e->f8L, e->eL, nL, pr, e->partR, e->capR, st);
         if ((rc = prof_end(h))) return rc;
         launch_rescore(h->dL, h->dR, nL, nR, e->partL, pl.splits, e->capL, e->partR, pr.splits, e->capR, h->normL,
-                       h->normR, h->maxNormBits, kWinRelFp8, 0, minScore, minRatio, e->slots, e->slotCount,
+                       h->normR, h->maxNormBits, kWinRelFp8, 0, minScore, minRatio,
                        h->colMax, h->rowMax, e->secL, e->secR, h->stripes, st);
     }
     e->haveSecond = work;
@@ -211,6 +283,25 @@ int ext_enqueue(v3dm_context* h, int mode, float minScore, float minRatio)
     return ctx_fail(h, V3DM_ERR_UNSUPPORTED, "precision mode not available");
 }
 
+// Called before a result is read: a learned result whose sampled check shows a recall below
+// the handle's threshold is replaced by the plain form's (stats.nFallback = 1).
+int ext_finish(v3dm_context* h)
+{
+    ExtState* e = ctx_ext(h);
+    if (!e || !e->sPending) return V3DM_OK;
+    e->sPending = false;
+    EXTCHECK(h, hipStreamSynchronize(h->stream));
+    int const accepted = e->sHost[0], missed = e->sHost[1];
+    h->stats.nSampled = e->sSamples;
+    h->stats.nSampleMissed = missed;
+    if (accepted > 0 && (float)(accepted - missed) < h->fallbackRecall * (float)accepted) {
+        int const rc = enqueue_plain(h, h->lastMinScore, h->lastMinRatio);
+        if (rc) return rc;
+        h->stats.nFallback = 1;
+    }
+    return V3DM_OK;
+}
+
 int ext_second_best(v3dm_context* h, int* colSecond, int* rowSecond)
 {
     ExtState* e = ctx_ext(h);
@@ -221,12 +312,6 @@ int ext_second_best(v3dm_context* h, int* colSecond, int* rowSecond)
     if (rowSecond && h->nR > 0)
         EXTCHECK(h, hipMemcpy(rowSecond, e->secR, (size_t)h->nR * sizeof(int), hipMemcpyDeviceToHost));
     return V3DM_OK;
-}
-
-int ext_fallbacks(v3dm_context* h)
-{
-    ExtState* e = ctx_ext(h);
-    return e ? e->fallbacks : 0;
 }
 
 void ext_features_changed(v3dm_context* h, bool left)
@@ -244,9 +329,6 @@ void ext_release(ExtState* e)
     if (e->bR) (void)hipFree(e->bR);
     if (e->partL) (void)hipFree(e->partL);
     if (e->partR) (void)hipFree(e->partR);
-    if (e->slots) (void)hipFree(e->slots);
-    if (e->slotCount) (void)hipFree(e->slotCount);
-    e->slots = e->slotCount = nullptr;
     if (e->secL) (void)hipFree(e->secL);
     if (e->secR) (void)hipFree(e->secR);
     e->secL = e->secR = nullptr;
@@ -257,6 +339,19 @@ void ext_release(ExtState* e)
         if (p) (void)hipFree(p);
     e->cL = e->cR = nullptr;
     e->cLValid = e->cRValid = false;
+    for (void* p : {(void*)e->sL, (void*)e->sNorm, (void*)e->sIdx, (void*)e->sColPart, (void*)e->sRowPart,
+                    (void*)e->sDec, (void*)e->sFlags, (void*)e->sRpart, (void*)e->sCounters, (void*)e->sOut})
+        if (p) (void)hipFree(p);
+    if (e->sHost) (void)hipHostFree(e->sHost);
+    if (e->side) (void)hipStreamDestroy(e->side);
+    if (e->evFork) (void)hipEventDestroy(e->evFork);
+    if (e->evJoin) (void)hipEventDestroy(e->evJoin);
+    e->side = nullptr;
+    e->evFork = e->evJoin = nullptr;
+    e->sL = e->sNorm = nullptr;
+    e->sIdx = e->sDec = e->sFlags = e->sCounters = e->sOut = e->sHost = nullptr;
+    e->sColPart = e->sRowPart = e->sRpart = nullptr;
+    e->sPending = false;
     e->f8L = e->f8R = e->eL = e->eR = nullptr;
     e->f8LValid = e->f8RValid = false;
     e->bL = e->bR = nullptr;

@@ -34,7 +34,8 @@ int prof_end(v3dm_context* h);
 // extension hooks (v3dm_ext.cpp)
 bool ext_supports(v3dm_context* h, int mode);
 int ext_enqueue(v3dm_context* h, int mode, float minScore, float minRatio);
-int ext_fallbacks(v3dm_context* h);
+// before a result is read: sampled check of a learned result, plain rerun if it fails
+int ext_finish(v3dm_context* h);
 // exact second best candidate of every query of the last shortlist call (host arrays)
 int ext_second_best(v3dm_context* h, int* colSecond, int* rowSecond);
 void ext_features_changed(v3dm_context* h, bool left);

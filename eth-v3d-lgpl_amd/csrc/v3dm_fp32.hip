@@ -230,13 +230,13 @@ void launch_batch_fused_fp32(BatchDev const& b, int4 const* items, int nItems, i
     hipLaunchKernelGGL(k_batch_fused_fp32, dim3(grid), dim3(256), 0, st, b, items, nItems, queue, part, INFINITY);
 }
 
-FusedPlan plan_fused_fp32(int nL, int nR, int targetGroups)
+FusedPlan plan_fused_fp32(int nL, int nR, int targetGroups, int maxSplits)
 {
     FusedPlan p;
     p.leftBlocks = (nL + kBI - 1) / kBI;
     int const nTiles = (nR + kBJ - 1) / kBJ;
     int splits = (targetGroups + p.leftBlocks - 1) / p.leftBlocks;
-    if (splits > kMaxSplits) splits = kMaxSplits;
+    if (splits > maxSplits) splits = maxSplits;
     if (splits > nTiles) splits = nTiles;
     if (splits < 1) splits = 1;
     p.tilesPerSplit = (nTiles + splits - 1) / splits;

@@ -16,6 +16,8 @@ constexpr int kCntFlagR = 5;      // counters[5]: open right queries of the runn
 constexpr int kNumCounters = 8;
 // Counts summed by many work-groups go to kStripes slots one cache line apart (atomics on a
 // single address serialise in L2); the gather kernel sums and clears them.
+constexpr int kSampleRows = 128;  // sampled check of a shortlist result: at most one left block
+constexpr int kSampleMaxSplits = 256;  // ... its right features split into up to 256 work-groups
 constexpr int kStripes = 64;
 constexpr int kStripeStride = 32;  // ints (128 bytes)
 constexpr int kRecheckMaxChunks = 64;  // candidate chunks per re-checked query
@@ -58,8 +60,13 @@ void launch_mutual_compact(int const* colMax, int const* rowMax, int nL, int nR,
 void launch_dequant_u8(uint8_t const* q, int n, float* out, hipStream_t st);
 
 // plain FP32 fused score + top-2 kernel (v3dm_fp32.hip)
-FusedPlan plan_fused_fp32(int nL, int nR, int targetGroups);
+FusedPlan plan_fused_fp32(int nL, int nR, int targetGroups, int maxSplits = kMaxSplits);
 void launch_fused_fp32(float const* L, float const* R, int nL, int nR, FusedPlan const& p, Top2* colPart, int ldc,
                        Top2* rowPart, int ldr, hipStream_t st);
+
+// sampled exact check of a shortlist result (v3dm_select.hip)
+void launch_sample_gather(float const* L, float const* normL, int nL, int S, float* Ls, float* normS, int* idx,
+                          hipStream_t st);
+void launch_sample_compare(int const* exactS, int const* idx, int S, int const* colMax, int* out, hipStream_t st);
 
 }  // namespace v3dm

@@ -1,15 +1,16 @@
-// Exact rescoring of the shortlist forms (BF16, FP8, learned codes), in two kernels.
+// Exact rescoring of the shortlist forms (BF16, FP8, learned codes): one kernel, eight
+// lanes per query.
 //
-//  select: one lane per query scans the query's partition lists (two candidates per
-//          (split, half) list, approximate scores), fixes the threshold -- either the
-//          window v >= a2 - winRel * |q| * max|c| below the approximate second best a2, or
-//          the topK-th largest approximate score -- and writes the candidates at or above
-//          it to the query's slot row (at most kRescoreSlots, which every list fits).
-//  score:  eight lanes per query split the query's slots; each lane forms the exact score
-//          fl32(sum_k double(q_k) * double(c_k)) of its candidate in ascending k (one FP64
-//          chain per lane, rows read straight from memory); the eight partial top-2 states
-//          merge by xor shuffles; the decision is the reference's rule on the exact top-2.
-// The second best candidate (by exact score) is kept as well: it labels training data.
+// Lane `sub` of a query's group holds the query's partition lists e = sub, sub + 8, ...
+// (two candidates with approximate scores per (split, half) list).  The group fixes the
+// threshold -- either the window v >= a2 - winRel * |q| * max|c| below the approximate
+// second best a2, or the topK-th largest approximate score (a tournament over the group:
+// each round takes the largest remaining value) -- compacts the candidates at or above it
+// into an LDS row, and the lanes share them out: each forms the exact score
+// fl32(sum_k double(q_k) * double(c_k)) (ascending k, one FP64 chain, rows read straight
+// from memory) of every eighth one.  The lanes' top-2 states merge by
+// xor shuffles and the reference's rule decides on the exact top-2.  The second best
+// candidate by exact score is kept as well: it labels training data.
 #include "v3dm_batch.h"
 #include "v3dm_kernels.h"
 #include "v3dm_short.h"
@@ -39,221 +40,220 @@ __device__ __forceinline__ Top2x top2x_shfl_xor(Top2x s, int mask)
     return Top2x{__shfl_xor(s.b1, mask), __shfl_xor(s.b2, mask), __shfl_xor(s.a1, mask), __shfl_xor(s.a2, mask)};
 }
 
-// Threshold and slot list of one query (lane-per-query).  part points at the query's
-// (split 0, half 0) entry; entries of split s are at part[s * ldq * 2 + h].  kMax bounds
-// topK (the sorted register array of the topK largest values).
-template <int kMax>
-__device__ __forceinline__ int select_query(Cand2 const* __restrict__ part, size_t ldq2, int nParts, float nq,
-                                            float mo, float winRel, int topK, int* __restrict__ slots)
+constexpr int kPerLane = kRescoreSlots / 8;  // candidates one lane holds (kShortMaxSplits lists)
+
+__device__ __forceinline__ double exact_dot(float const* __restrict__ q, float const* __restrict__ c)
 {
+    float4 const* qp = reinterpret_cast<float4 const*>(q);
+    float4 const* cp = reinterpret_cast<float4 const*>(c);
+    double acc = 0.0;
+#pragma unroll 8
+    for (int k4 = 0; k4 < 32; ++k4) {
+        float4 const x = qp[k4];
+        float4 const y = cp[k4];
+        acc += (double)x.x * (double)y.x;
+        acc += (double)x.y * (double)y.y;
+        acc += (double)x.z * (double)y.z;
+        acc += (double)x.w * (double)y.w;
+    }
+    return acc;
+}
+
+// One query, eight lanes (sub = lane & 7).  part: the query's (split 0, half 0) list; the
+// lists of split s are at part[s * ldq2 + h]; slots: the group's kRescoreSlots ints of LDS.
+// Returns the exact top-2 (all lanes) and the number of candidates scored by this lane.
+__device__ __forceinline__ Top2x finish_query(Cand2 const* __restrict__ part, size_t ldq2, int nParts, float nq,
+                                              float mo, float winRel, int topK, float const* __restrict__ Qd,
+                                              float const* __restrict__ Cd, int sub, int* slots, int& nScored)
+{
+    float v[kPerLane];
+    int j[kPerLane];
+#pragma unroll
+    for (int u = 0; u < kPerLane / 2; ++u) {
+        int const e = sub + 8 * u;  // list (split e / 2, half e % 2)
+        Cand2 c = {-INFINITY, -1, -INFINITY, -1};
+        if (e < 2 * nParts) c = part[(size_t)(e >> 1) * ldq2 + (e & 1)];
+        v[2 * u] = c.j1 >= 0 ? c.v1 : -INFINITY;
+        j[2 * u] = c.j1;
+        v[2 * u + 1] = c.j2 >= 0 ? c.v2 : -INFINITY;
+        j[2 * u + 1] = c.j2;
+    }
     float thr;
     if (topK > 0) {
-        float top[kMax];
+        // tournament: round k takes the largest remaining value of the group; thr = the
+        // value taken in round topK (or the last finite one)
+        float w[kPerLane];
 #pragma unroll
-        for (int i = 0; i < kMax; ++i) top[i] = -INFINITY;
-        for (int s = 0; s < nParts; ++s)
-            for (int h = 0; h < 2; ++h) {
-                Cand2 const c = part[s * ldq2 + h];
-                float const vs[2] = {c.j1 >= 0 ? c.v1 : -INFINITY, c.j2 >= 0 ? c.v2 : -INFINITY};
+        for (int u = 0; u < kPerLane; ++u) w[u] = v[u];
+        float kth = -INFINITY;
+        for (int k = 0; k < topK; ++k) {
+            float m = w[0];
 #pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    float v = vs[u];
+            for (int u = 1; u < kPerLane; ++u) m = fmaxf(m, w[u]);
+            float g = m;
+            g = fmaxf(g, __shfl_xor(g, 1));
+            g = fmaxf(g, __shfl_xor(g, 2));
+            g = fmaxf(g, __shfl_xor(g, 4));
+            if (g == -INFINITY) break;  // uniform over the group
+            kth = g;
+            // the lowest lane holding g removes one copy
+            int own = (m == g) ? sub : 8;
+            own = min(own, __shfl_xor(own, 1));
+            own = min(own, __shfl_xor(own, 2));
+            own = min(own, __shfl_xor(own, 4));
+            if (own == sub) {
+                bool done = false;
 #pragma unroll
-                    for (int i = 0; i < kMax; ++i) {  // insertion: keeps top[] descending
-                        float const hi = fmaxf(top[i], v);
-                        v = fminf(top[i], v);
-                        top[i] = hi;
-                    }
+                for (int u = 0; u < kPerLane; ++u) {
+                    bool const hit = !done && w[u] == g;
+                    w[u] = hit ? -INFINITY : w[u];
+                    done = done || hit;
                 }
             }
-        float kth = top[0];
-#pragma unroll
-        for (int i = 1; i < kMax; ++i) kth = (i < topK && top[i] > -INFINITY) ? top[i] : kth;
+        }
         thr = kth;
     } else {
         float a1 = -INFINITY, a2 = -INFINITY;
-        for (int s = 0; s < nParts; ++s)
-            for (int h = 0; h < 2; ++h) {
-                Cand2 const c = part[s * ldq2 + h];
-                float const x = c.j1 >= 0 ? c.v1 : -INFINITY;
-                float const y = c.j2 >= 0 ? c.v2 : -INFINITY;
-                a2 = fmaxf(a2, fminf(a1, x));
-                a1 = fmaxf(a1, x);
-                a2 = fmaxf(a2, fminf(a1, y));
-                a1 = fmaxf(a1, y);
-            }
+#pragma unroll
+        for (int u = 0; u < kPerLane; ++u) {
+            a2 = fmaxf(a2, fminf(a1, v[u]));
+            a1 = fmaxf(a1, v[u]);
+        }
+#pragma unroll
+        for (int o = 1; o < 8; o <<= 1) {
+            float const b1 = __shfl_xor(a1, o), b2 = __shfl_xor(a2, o);
+            a2 = fmaxf(fminf(a1, b1), fmaxf(a2, b2));
+            a1 = fmaxf(a1, b1);
+        }
         thr = a2 - winRel * nq * mo;
     }
-    int n = 0;
-    for (int s = 0; s < nParts; ++s)
-        for (int h = 0; h < 2; ++h) {
-            Cand2 const c = part[s * ldq2 + h];
-            if (c.j1 >= 0 && c.v1 >= thr && n < kRescoreSlots) slots[n++] = c.j1;
-            if (c.j2 >= 0 && c.v2 >= thr && n < kRescoreSlots) slots[n++] = c.j2;
-        }
-    return n;
-}
-
-__device__ __forceinline__ int select_query_any(Cand2 const* __restrict__ part, size_t ldq2, int nParts, float nq,
-                                                float mo, float winRel, int topK, int* __restrict__ slots)
-{
-    if (topK <= 8) return select_query<8>(part, ldq2, nParts, nq, mo, winRel, topK, slots);
-    return select_query<kRescoreTopKMax>(part, ldq2, nParts, nq, mo, winRel, topK, slots);
-}
-
-// Exact top-2 of one query over its slots; 8 lanes (sub = lane & 7) per query.  Returns
-// the merged state in every lane of the group.
-__device__ __forceinline__ Top2x score_query(float const* __restrict__ Qd, float const* __restrict__ Cd,
-                                             int const* __restrict__ slots, int n, int sub)
-{
+    // compact the group's candidates at or above the threshold into its LDS slot row (in
+    // lane order), then lane sub scores slots sub, sub + 8, ... (balanced over the lanes)
+    int mine = 0;
+#pragma unroll
+    for (int u = 0; u < kPerLane; ++u) mine += (j[u] >= 0 && v[u] >= thr) ? 1 : 0;
+    int incl = mine;
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) {
+        int const t = __shfl_up(incl, o, 8);
+        incl += (sub >= o) ? t : 0;
+    }
+    int const total = __shfl(incl, 7, 8);
+    int pos = incl - mine;
+#pragma unroll
+    for (int u = 0; u < kPerLane; ++u)
+        if (j[u] >= 0 && v[u] >= thr) slots[pos++] = j[u];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // the group's slot writes are visible
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     Top2x s = {kInit, kInit, -1, -1};
-    for (int k = sub; k < n; k += 8) {
-        int const j = slots[k];
-        float4 const* qp = reinterpret_cast<float4 const*>(Qd);
-        float4 const* cp = reinterpret_cast<float4 const*>(Cd + (size_t)j * kDim);
-        double acc = 0.0;
-#pragma unroll 8
-        for (int k4 = 0; k4 < 32; ++k4) {
-            float4 const x = qp[k4];
-            float4 const y = cp[k4];
-            acc += (double)x.x * (double)y.x;
-            acc += (double)x.y * (double)y.y;
-            acc += (double)x.z * (double)y.z;
-            acc += (double)x.w * (double)y.w;
-        }
-        float const sc = (float)acc;
-        if (sc == sc) s = top2x_merge(s, Top2x{sc, kInit, j, -1});
+    int n = 0;
+    for (int k = sub; k < total; k += 8) {
+        int const jj = slots[k];
+        float const sc = (float)exact_dot(Qd, Cd + (size_t)jj * kDim);
+        if (sc == sc) s = top2x_merge(s, Top2x{sc, kInit, jj, -1});
+        ++n;
     }
     s = top2x_merge(s, top2x_shfl_xor(s, 1));
     s = top2x_merge(s, top2x_shfl_xor(s, 2));
     s = top2x_merge(s, top2x_shfl_xor(s, 4));
+    nScored = n;
     return s;
+}
+
+__device__ __forceinline__ void add_stripe(int n, int* __restrict__ stripes)
+{
+    __shared__ int wsum[4];
+    int const tid = threadIdx.x;
+    int c = n;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if ((tid & 63) == 0) wsum[tid >> 6] = c;
+    __syncthreads();
+    if (tid == 0) atomicAdd(&stripes[(blockIdx.x % kStripes) * kStripeStride], wsum[0] + wsum[1] + wsum[2] + wsum[3]);
 }
 
 }  // namespace
 
 // ------------------------------------------------------------------------------ one pair
-// Queries [0, nL) are left features (lists partL), [nL, nL + nR) right features.
-__global__ __launch_bounds__(256) void k_rescore_select(Cand2 const* __restrict__ partL, int nPartL, int ldqL,
-                                                        Cand2 const* __restrict__ partR, int nPartR, int ldqR,
-                                                        int nL, int nR, float const* __restrict__ normL,
-                                                        float const* __restrict__ normR,
-                                                        unsigned int const* __restrict__ maxNormBits, float winRel,
-                                                        int topK, int* __restrict__ slots, int* __restrict__ count)
+// Queries [0, nL) are left features (lists partL), [nL, nL + nR) right features; 32
+// queries per work-group.
+__global__ __launch_bounds__(256) void k_shortlist_finish(float const* __restrict__ L, float const* __restrict__ R,
+                                                          int nL, int nR, Cand2 const* __restrict__ partL, int nPartL,
+                                                          int ldqL, Cand2 const* __restrict__ partR, int nPartR,
+                                                          int ldqR, float const* __restrict__ normL,
+                                                          float const* __restrict__ normR,
+                                                          unsigned int const* __restrict__ maxNormBits, float winRel,
+                                                          int topK, float minScore, float minRatio,
+                                                          int* __restrict__ colMax, int* __restrict__ rowMax,
+                                                          int* __restrict__ colSecond, int* __restrict__ rowSecond,
+                                                          int* __restrict__ stripes)
 {
-    int const g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= nL + nR) return;
-    bool const left = g < nL;
-    int const q = left ? g : g - nL;
-    Cand2 const* part = (left ? partL : partR) + (size_t)q * 2;
-    size_t const ldq2 = 2 * (size_t)(left ? ldqL : ldqR);
-    float const nq = left ? normL[q] : normR[q];
-    float const mo = __uint_as_float(maxNormBits[left ? 1 : 0]);
-    count[g] = select_query_any(part, ldq2, left ? nPartL : nPartR, nq, mo, winRel, topK,
-                                slots + (size_t)g * kRescoreSlots);
-}
-
-__global__ __launch_bounds__(256) void k_rescore_score(float const* __restrict__ L, float const* __restrict__ R,
-                                                       int nL, int nR, int const* __restrict__ slots,
-                                                       int const* __restrict__ count, float minScore,
-                                                       float minRatio, int* __restrict__ colMax,
-                                                       int* __restrict__ rowMax, int* __restrict__ colSecond,
-                                                       int* __restrict__ rowSecond, int* __restrict__ stripes)
-{
-    __shared__ int wsum[4];
-    int const tid = threadIdx.x;
-    int const sub = tid & 7;
-    int const g = blockIdx.x * 32 + (tid >> 3);
+    __shared__ int slots[32][kRescoreSlots];
+    int const sub = threadIdx.x & 7;
+    int const g = blockIdx.x * 32 + (threadIdx.x >> 3);
     int n = 0;
     if (g < nL + nR) {  // uniform over the 8 lanes of a query
         bool const left = g < nL;
         int const q = left ? g : g - nL;
-        n = count[g];
-        Top2x const s = score_query((left ? L : R) + (size_t)q * kDim, left ? R : L,
-                                    slots + (size_t)g * kRescoreSlots, n, sub);
+        Top2x const s = finish_query((left ? partL : partR) + (size_t)q * 2, 2 * (size_t)(left ? ldqL : ldqR),
+                                     left ? nPartL : nPartR, left ? normL[q] : normR[q],
+                                     __uint_as_float(maxNormBits[left ? 1 : 0]), winRel, topK,
+                                     (left ? L : R) + (size_t)q * kDim, left ? R : L, sub, slots[threadIdx.x >> 3],
+                                     n);
         if (sub == 0) {
             (left ? colMax : rowMax)[q] = decide_exact(Top2{s.b1, s.b2, s.a1}, minScore, minRatio);
             if (colSecond) (left ? colSecond : rowSecond)[q] = s.a2;
         }
     }
-    int c = (sub == 0) ? n : 0;
-    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-    if ((tid & 63) == 0) wsum[tid >> 6] = c;
-    __syncthreads();
-    if (tid == 0) atomicAdd(&stripes[(blockIdx.x % kStripes) * kStripeStride], wsum[0] + wsum[1] + wsum[2] + wsum[3]);
+    add_stripe(n, stripes);
 }
 
 void launch_rescore(float const* L, float const* R, int nL, int nR, Cand2 const* partL, int nPartL, int ldqL,
                     Cand2 const* partR, int nPartR, int ldqR, float const* normL, float const* normR,
                     unsigned int const* maxNormBits, float winRel, int topK, float minScore, float minRatio,
-                    int* slots, int* count, int* colMax, int* rowMax, int* colSecond, int* rowSecond, int* stripes,
-                    hipStream_t st)
+                    int* colMax, int* rowMax, int* colSecond, int* rowSecond, int* stripes, hipStream_t st)
 {
     int const n = nL + nR;
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_rescore_select, dim3((n + 255) / 256), dim3(256), 0, st, partL, nPartL, ldqL, partR,
-                       nPartR, ldqR, nL, nR, normL, normR, maxNormBits, winRel, topK, slots, count);
-    hipLaunchKernelGGL(k_rescore_score, dim3((n + 31) / 32), dim3(256), 0, st, L, R, nL, nR, slots, count, minScore,
-                       minRatio, colMax, rowMax, colSecond, rowSecond, stripes);
+    hipLaunchKernelGGL(k_shortlist_finish, dim3((n + 31) / 32), dim3(256), 0, st, L, R, nL, nR, partL, nPartL, ldqL,
+                       partR, nPartR, ldqR, normL, normR, maxNormBits, winRel, topK, minScore, minRatio, colMax,
+                       rowMax, colSecond, rowSecond, stripes);
 }
 
 // ------------------------------------------------------------------------------ batch
-// Query-space rows (padding rows get count 0 and decision -1).
-__global__ __launch_bounds__(256) void k_batch_rescore_select(BatchDev b, Cand2 const* __restrict__ part,
-                                                              float winRel, int topK, int* __restrict__ slots,
-                                                              int* __restrict__ count)
+// Query-space rows; padding rows decide -1.
+__global__ __launch_bounds__(256) void k_batch_shortlist_finish(BatchDev b, Cand2 const* __restrict__ part,
+                                                                float winRel, int topK, float minScore,
+                                                                float minRatio, int* __restrict__ sel,
+                                                                int* __restrict__ stripes)
 {
-    int const g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= b.qRows) return;
-    BatchSeg const sg = b.seg[b.segOfBlock[g / kBatchRowAlign]];
-    int const q = g - sg.qOff;
-    int n = 0;
-    if (q < sg.n && sg.nO > 0 && sg.nParts > 0) {
-        int const row = b.imgBase[sg.img] + q;
-        n = select_query_any(part + ((size_t)sg.partOff + q) * 2, 2 * (size_t)sg.ld, sg.nParts, b.norm[row],
-                         __uint_as_float(b.maxNorm[sg.oImg]), winRel, topK, slots + (size_t)g * kRescoreSlots);
-    }
-    count[g] = n;
-}
-
-__global__ __launch_bounds__(256) void k_batch_rescore_score(BatchDev b, int const* __restrict__ slots,
-                                                             int const* __restrict__ count, float minScore,
-                                                             float minRatio, int* __restrict__ sel,
-                                                             int* __restrict__ stripes)
-{
-    __shared__ int wsum[4];
-    int const tid = threadIdx.x;
-    int const sub = tid & 7;
-    int const g = blockIdx.x * 32 + (tid >> 3);
+    __shared__ int slots[32][kRescoreSlots];
+    int const sub = threadIdx.x & 7;
+    int const g = blockIdx.x * 32 + (threadIdx.x >> 3);
     int n = 0;
     if (g < b.qRows) {
         BatchSeg const sg = b.seg[b.segOfBlock[g / kBatchRowAlign]];
         int const q = g - sg.qOff;
-        n = count[g];
         int d = -1;
-        if (q < sg.n && n > 0) {
-            float const* Qd = b.img + (size_t)(b.imgBase[sg.img] + q) * kDim;
-            float const* Cd = b.img + (size_t)b.imgBase[sg.oImg] * kDim;
-            Top2x const s = score_query(Qd, Cd, slots + (size_t)g * kRescoreSlots, n, sub);
+        if (q < sg.n && sg.nO > 0 && sg.nParts > 0) {
+            int const row = b.imgBase[sg.img] + q;
+            Top2x const s = finish_query(part + ((size_t)sg.partOff + q) * 2, 2 * (size_t)sg.ld, sg.nParts,
+                                         b.norm[row], __uint_as_float(b.maxNorm[sg.oImg]), winRel, topK,
+                                         b.img + (size_t)row * kDim, b.img + (size_t)b.imgBase[sg.oImg] * kDim, sub,
+                                         slots[threadIdx.x >> 3], n);
             d = decide_exact(Top2{s.b1, s.b2, s.a1}, minScore, minRatio);
         }
         if (sub == 0) sel[g] = d;
     }
-    int c = (sub == 0) ? n : 0;
-    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-    if ((tid & 63) == 0) wsum[tid >> 6] = c;
-    __syncthreads();
-    if (tid == 0) atomicAdd(&stripes[(blockIdx.x % kStripes) * kStripeStride], wsum[0] + wsum[1] + wsum[2] + wsum[3]);
+    add_stripe(n, stripes);
 }
 
 void launch_batch_rescore(BatchDev const& b, Cand2 const* part, float winRel, int topK, float minScore,
-                          float minRatio, int* slots, int* count, int* sel, int* stripes, hipStream_t st)
+                          float minRatio, int* sel, int* stripes, hipStream_t st)
 {
     if (b.qRows <= 0) return;
-    hipLaunchKernelGGL(k_batch_rescore_select, dim3((b.qRows + 255) / 256), dim3(256), 0, st, b, part, winRel, topK,
-                       slots, count);
-    hipLaunchKernelGGL(k_batch_rescore_score, dim3((b.qRows + 31) / 32), dim3(256), 0, st, b, slots, count, minScore,
-                       minRatio, sel, stripes);
+    hipLaunchKernelGGL(k_batch_shortlist_finish, dim3((b.qRows + 31) / 32), dim3(256), 0, st, b, part, winRel, topK,
+                       minScore, minRatio, sel, stripes);
 }
 
 }  // namespace v3dm

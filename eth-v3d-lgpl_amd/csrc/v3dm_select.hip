@@ -369,4 +369,62 @@ void launch_dequant_u8(uint8_t const* q, int n, float* out, hipStream_t st)
     hipLaunchKernelGGL(k_dequant_u8, dim3((n + 3) / 4), dim3(256), 0, st, q, n, out);
 }
 
+// ------------------------------------------------------------------------------ sampled check
+// A shortlist result is checked on S evenly spaced left features (S <= 128): their rows and
+// norms are gathered into a 128-row block (zero rows past S) and decided by the plain exact
+// path; the comparison counts the sampled features the exact rule accepts, those whose
+// shortlist decision differs from it (missed), and those the shortlist accepts wrongly.
+__global__ __launch_bounds__(256) void k_sample_gather(float const* __restrict__ L, float const* __restrict__ normL,
+                                                       int nL, int S, float* __restrict__ Ls,
+                                                       float* __restrict__ normS, int* __restrict__ idx)
+{
+    int const lane = threadIdx.x & 63;
+    int const k = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (k >= kSampleRows) return;
+    int const i = (k < S) ? (int)(((long long)k * nL) / S) : -1;
+    float2 const v = (i >= 0) ? reinterpret_cast<float2 const*>(L + (size_t)i * kDim)[lane] : make_float2(0.f, 0.f);
+    reinterpret_cast<float2*>(Ls + (size_t)k * kDim)[lane] = v;
+    if (lane == 0) {
+        normS[k] = (i >= 0) ? normL[i] : 0.0f;
+        idx[k] = i;
+    }
+}
+
+__global__ __launch_bounds__(128) void k_sample_compare(int const* __restrict__ exactS, int const* __restrict__ idx,
+                                                        int S, int const* __restrict__ colMax, int* __restrict__ out)
+{
+    __shared__ int red[2][3];
+    int const k = threadIdx.x;
+    int acc = 0, miss = 0, extra = 0;
+    if (k < S) {
+        int const e = exactS[k], d = colMax[idx[k]];
+        acc = e >= 0;
+        miss = (e >= 0 && d != e);
+        extra = (e < 0 && d >= 0);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        acc += __shfl_xor(acc, o);
+        miss += __shfl_xor(miss, o);
+        extra += __shfl_xor(extra, o);
+    }
+    if ((k & 63) == 0) {
+        red[k >> 6][0] = acc;
+        red[k >> 6][1] = miss;
+        red[k >> 6][2] = extra;
+    }
+    __syncthreads();
+    if (k < 3) out[k] = red[0][k] + red[1][k];
+}
+
+void launch_sample_gather(float const* L, float const* normL, int nL, int S, float* Ls, float* normS, int* idx,
+                          hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sample_gather, dim3(kSampleRows / 4), dim3(256), 0, st, L, normL, nL, S, Ls, normS, idx);
+}
+
+void launch_sample_compare(int const* exactS, int const* idx, int S, int const* colMax, int* out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sample_compare, dim3(1), dim3(128), 0, st, exactS, idx, S, colMax, out);
+}
+
 }  // namespace v3dm

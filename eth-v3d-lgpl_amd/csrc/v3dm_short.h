@@ -19,7 +19,7 @@ struct ShortPlan {
 };
 
 constexpr int kShortMaxSplits = 16;
-constexpr int kRescoreSlots = 2 * 2 * kShortMaxSplits;  // every candidate of every list
+constexpr int kRescoreSlots = 2 * 2 * kShortMaxSplits;  // candidates in every list of a query
 constexpr int kRescoreTopKMax = 32;
 constexpr int kShortRowPad = 256;  // bf16 descriptor buffers hold a multiple of 256 rows
 // Per-score error bound of the BF16 scores: operands rounded to bf16 err by <= 2^-9
@@ -42,13 +42,11 @@ void launch_pass_bf16(__bf16 const* Q, int nQ, __bf16 const* C, int nC, ShortPla
 void launch_to_fp8(float const* X, int rows, unsigned char* Y, unsigned char* E, hipStream_t st);
 void launch_pass_fp8(unsigned char const* Q, unsigned char const* EQ, int nQ, unsigned char const* C,
                      unsigned char const* EC, int nC, ShortPlan const& p, Cand2* part, int ldq, hipStream_t st);
-// Exact rescoring (v3dm_rescore.hip): slots holds kRescoreSlots ints per query, count one
-// int per query (queries = nL + nR, left first); colSecond / rowSecond may be NULL.
+// Exact rescoring (v3dm_rescore.hip); colSecond / rowSecond may be NULL.
 void launch_rescore(float const* L, float const* R, int nL, int nR, Cand2 const* partL, int nPartL, int ldqL,
                     Cand2 const* partR, int nPartR, int ldqR, float const* normL, float const* normR,
                     unsigned int const* maxNormBits, float winRel, int topK, float minScore, float minRatio,
-                    int* slots, int* count, int* colMax, int* rowMax, int* colSecond, int* rowSecond, int* stripes,
-                    hipStream_t st);
+                    int* colMax, int* rowMax, int* colSecond, int* rowSecond, int* stripes, hipStream_t st);
 // learned code pass (v3dm_code.hip)
 void launch_pass_code(signed char const* Q, int nQ, signed char const* C, int nC, ShortPlan const& p, Cand2* part,
                       int ldq, hipStream_t st);

@@ -24,7 +24,7 @@ class Stats(C.Structure):
     _fields_ = [
         ("nLeft", C.c_int), ("nRight", C.c_int), ("nMatches", C.c_int), ("nRechecked", C.c_int),
         ("precision", C.c_int), ("nFallback", C.c_int), ("kernelMs", C.c_double),
-        ("kernelLaunches", C.c_int), ("nRescored", C.c_int),
+        ("kernelLaunches", C.c_int), ("nRescored", C.c_int), ("nSampled", C.c_int), ("nSampleMissed", C.c_int),
     ]
 
 
@@ -57,6 +57,7 @@ SIGNATURES = {
     "v3dm_set_stream": (_I, [_P, _P]),
     "v3dm_set_learned_topk": (_I, [_P, _I]),
     "v3dm_set_derived_cache": (_I, [_P, _I]),
+    "v3dm_set_fallback": (_I, [_P, _I, _F]),
     "v3dm_set_left_device": (_I, [_P, _I, _FP]),
     "v3dm_set_right_device": (_I, [_P, _I, _FP]),
     "v3dm_set_left_u8": (_I, [_P, _I, _FP]),

@@ -155,6 +155,9 @@ class SIFT_Matching:
     def set_learned_topk(self, k):
         self._check(self._lib.v3dm_set_learned_topk(self._h, int(k)))
 
+    def set_fallback(self, samples=128, min_recall=0.995):
+        self._check(self._lib.v3dm_set_fallback(self._h, int(samples), C.c_float(min_recall)))
+
     def set_derived_cache(self, enable=True):
         self._check(self._lib.v3dm_set_derived_cache(self._h, int(bool(enable))))
 

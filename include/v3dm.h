@@ -55,6 +55,8 @@ typedef struct v3dm_stats {
                                  (only when profiling is enabled) */
     int kernelLaunches;       /* launches accumulated in kernelMs */
     int nRescored;            /* shortlist candidates rescored exactly (shortlist modes) */
+    int nSampled;             /* learned mode: left features decided exactly as a check */
+    int nSampleMissed;        /* ... of which the learned decision missed the exact match */
 } v3dm_stats;
 
 /* Handle life cycle (constructor / destructor of SIFT_Matching, h:17-21).  device is a HIP
@@ -112,6 +114,12 @@ int v3dm_set_learned_topk(v3dm_handle h, int k);
  * the features are unchanged; 0: every call derives them again (so that a call's time
  * includes that work). */
 int v3dm_set_derived_cache(v3dm_handle h, int enable);
+/* Learned mode: every call also decides `samples` evenly spaced left features (at most
+ * 128; 0 = off) with the exact plain path; when the recall of the learned decisions on them
+ * (exactly accepted features whose learned decision agrees / exactly accepted) is below
+ * minRecall, the result is replaced by the plain form's before it is returned
+ * (stats.nFallback = 1).  Default 128 and 0.995. */
+int v3dm_set_fallback(v3dm_handle h, int samples, float minRecall);
 /* Run the library's kernels on this HIP stream (hipStream_t, NULL = the handle's own). */
 int v3dm_set_stream(v3dm_handle h, void* hipStream);
 /* Feature upload from device memory (n*128 floats, device pointer); no host copy. */

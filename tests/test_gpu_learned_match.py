@@ -125,3 +125,44 @@ def test_learned_batch(matcher):
         f"precision={inter / max(len(gs), 1):.5f} tuning {m.batch_get_tuning(v3dmatch.PREC_LEARNED)}")
     assert inter >= 0.995 * len(ws) and inter >= 0.995 * len(gs)
     m.set_precision(v3dmatch.PREC_FP32)
+
+
+def test_sampled_check_and_fallback(matcher):
+    """Every learned call decides 128 sampled left features exactly; with the trained
+    network the check passes (no fallback), with a poor shortlist (untrained network, one
+    candidate per query) the sampled recall drops and the result is the plain form's."""
+    m, net = matcher
+    n = 4096
+    qL, qR, _ = synth.make_pair(n, n, 0.4, config=1, pair=13)
+    L, R = synth.dequantise(qL), synth.dequantise(qR)
+    want = oracle.find_putative_matches(L, R, 0.9, 0.8)
+    m.setLeftFeatures(n, L)
+    m.setRightFeatures(n, R)
+    m.set_precision(v3dmatch.PREC_LEARNED)
+    got = m.find_putative_matches_array(0.9, 0.8)
+    st = m.stats()
+    say(f"case sampled check (trained): sampled={st['nSampled']} missed={st['nSampleMissed']} "
+        f"fallback={st['nFallback']}")
+    assert st["nSampled"] == 128 and st["nFallback"] == 0
+    other = v3dmatch.SIFT_Matching(0)
+    other.enableRoundingWarning(False)
+    other.allocate(n, n)
+    raw = learned.Network(other)
+    raw.set_params(learned.init_params(2))
+    raw.calibrate_code_scale(np.concatenate([L[:2048], R[:2048]]))
+    other.setLeftFeatures(n, L)
+    other.setRightFeatures(n, R)
+    other.set_precision(v3dmatch.PREC_LEARNED)
+    other.set_learned_topk(1)
+    other.set_fallback(128, 1.0)
+    got = other.find_putative_matches_array(0.9, 0.8)
+    st = other.stats()
+    say(f"case sampled check (untrained, top-1): sampled={st['nSampled']} missed={st['nSampleMissed']} "
+        f"fallback={st['nFallback']}")
+    assert st["nSampleMissed"] > 0 and st["nFallback"] == 1
+    assert np.array_equal(got, want)  # the plain form's (exact) result
+    other.set_fallback(0, 0.995)
+    got = other.find_putative_matches_array(0.9, 0.8)
+    st = other.stats()
+    assert st["nSampled"] == 0 and st["nFallback"] == 0
+    other.close()
