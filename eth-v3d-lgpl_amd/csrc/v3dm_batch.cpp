@@ -22,7 +22,7 @@
 namespace v3dm {
 
 void launch_batch_load_u8(uint8_t const* src, int const* srcRow, int const* nImg, int const* blkImg,
-                          int const* imgBase, int rows, float* img, hipStream_t st);
+                          int const* imgBase, int rows, float* img, uint8_t* imgU8, float* denom, hipStream_t st);
 void launch_batch_load_f32(float const* src, int const* srcRow, int const* nImg, int const* blkImg,
                            int const* imgBase, int rows, float* img, hipStream_t st);
 
@@ -51,6 +51,9 @@ struct BatchState {
     __bf16* imgB = nullptr;
     bool bfValid = false;
     signed char* codes = nullptr;  // learned codes of every image row
+    uint8_t* imgU8 = nullptr;      // uint8 rows (uint8 uploads), aligned layout
+    float* denom = nullptr;
+    bool haveU8 = false;
     bool codesValid = false;
     float codesScale = 0.0f;
     int* imgBase = nullptr;
@@ -99,6 +102,23 @@ struct BatchState {
     int* hostRes = nullptr;  // pinned: counters + pair counts
     int hostResCap = 0;
 
+    // sampled exact check of learned runs (kSampleRows left features per pair) and the
+    // per-pair fallback to the plain form
+    float* sRows = nullptr;
+    float* sNorm = nullptr;
+    int* sIdx = nullptr;
+    int* sCount = nullptr;
+    int* sNSplits = nullptr;
+    Top2* sPart = nullptr;
+    int4* sItems = nullptr;
+    int nSItems = 0, sMaxSplits = 1;
+    int* pairFlag = nullptr;  // nPairs flags followed by 4 counts per pair
+    int* hostChk = nullptr;   // pinned copy
+    int4* fbItems = nullptr;
+    int fbItemCap = 0;
+    int* segMask = nullptr;
+    bool checkPending = false;
+
     // autotuning and graphs
     bool autotune = true;
     int forceTiles[3] = {0, 0, 0}, forceGrid[3] = {0, 0, 0};
@@ -107,10 +127,12 @@ struct BatchState {
         float minScore, minRatio, epsRel;
         int version;
         int topK;
+        int samples;
+        float minRecall;
         bool operator<(GraphKey const& o) const
         {
-            return std::tie(mode, minScore, minRatio, epsRel, version, topK) <
-                   std::tie(o.mode, o.minScore, o.minRatio, o.epsRel, o.version, o.topK);
+            return std::tie(mode, minScore, minRatio, epsRel, version, topK, samples, minRecall) <
+                   std::tie(o.mode, o.minScore, o.minRatio, o.epsRel, o.version, o.topK, o.samples, o.minRecall);
         }
     };
     std::map<GraphKey, hipGraphExec_t> graphs;
@@ -120,6 +142,7 @@ struct BatchState {
     // last run
     bool pending = false;
     int lastMode = 0;
+    float lastMinScore = 0.9f, lastMinRatio = 0.8f;
     hipEvent_t evA = nullptr, evB = nullptr;
     float lastMs = 0.0f;
 };
@@ -166,6 +189,8 @@ BatchDev dev_view(BatchState* b, int form)
     d.img = b->img;
     d.imgB = b->imgB;
     d.codes = b->codes;
+    d.u8 = b->haveU8 ? b->imgU8 : nullptr;
+    d.denom = b->denom;
     d.imgBase = b->imgBase;
     d.norm = b->norm;
     d.maxNorm = b->maxNorm;
@@ -304,6 +329,53 @@ int upload_plan(v3dm_context* h, BatchState* b, int form, Plan const& pl)
     return V3DM_OK;
 }
 
+// Items of the sampled check: (pair, 0, split, 8 tiles of 64 right features).
+int ensure_sample_plan(v3dm_context* h, BatchState* b)
+{
+    int const P = b->nPairs;
+    std::vector<BatchSeg> seg;
+    layout_segments(b, seg);
+    std::vector<std::pair<int, int4>> work;
+    std::vector<int> nSplits(P, 0);
+    int const tps = 8;
+    int maxS = 1;
+    for (int p = 0; p < P; ++p) {
+        int const nL = seg[p].n, nR = seg[P + p].n;
+        if (nL <= 0 || nR <= 0) continue;
+        int const nTiles = (nR + 63) / 64;
+        int const splits = (nTiles + tps - 1) / tps;
+        nSplits[p] = splits;
+        maxS = std::max(maxS, splits);
+        for (int s = 0; s < splits; ++s) work.push_back({std::min(tps, nTiles - s * tps), make_int4(p, 0, s, tps)});
+    }
+    std::stable_sort(work.begin(), work.end(),
+                     [](std::pair<int, int4> const& x, std::pair<int, int4> const& y) { return x.first > y.first; });
+    std::vector<int4> items;
+    for (auto& w : work) items.push_back(w.second);
+    for (void* ptr : {(void*)b->sRows, (void*)b->sNorm, (void*)b->sIdx, (void*)b->sCount, (void*)b->sNSplits,
+                      (void*)b->sPart, (void*)b->sItems, (void*)b->pairFlag, (void*)b->segMask})
+        if (ptr) (void)hipFree(ptr);
+    if (b->hostChk) (void)hipHostFree(b->hostChk);
+    size_t const rows = (size_t)P * kSampleRows;
+    BCHECK(h, hipMalloc(&b->sRows, rows * kDim * sizeof(float)));
+    BCHECK(h, hipMalloc(&b->sNorm, rows * sizeof(float)));
+    BCHECK(h, hipMalloc(&b->sIdx, rows * sizeof(int)));
+    BCHECK(h, hipMalloc(&b->sCount, P * sizeof(int)));
+    BCHECK(h, hipMalloc(&b->sNSplits, P * sizeof(int)));
+    BCHECK(h, hipMalloc(&b->sPart, rows * maxS * sizeof(Top2)));
+    BCHECK(h, hipMalloc(&b->sItems, std::max<size_t>(items.size(), 1) * sizeof(int4)));
+    BCHECK(h, hipMalloc(&b->pairFlag, 5 * P * sizeof(int)));
+    BCHECK(h, hipMalloc(&b->segMask, 2 * P * sizeof(int)));
+    BCHECK(h, hipHostMalloc(&b->hostChk, 5 * P * sizeof(int), hipHostMallocDefault));
+    BCHECK(h, hipMemcpy(b->sNSplits, nSplits.data(), P * sizeof(int), hipMemcpyHostToDevice));
+    if (!items.empty())
+        BCHECK(h, hipMemcpy(b->sItems, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+    b->nSItems = (int)items.size();
+    b->sMaxSplits = maxS;
+    b->checkPending = false;
+    return V3DM_OK;
+}
+
 int ensure_query_space(v3dm_context* h, BatchState* b)
 {
     std::vector<BatchSeg> seg;
@@ -376,7 +448,8 @@ int enqueue_run(v3dm_context* h, BatchState* b, int mode, float minScore, float 
     if (!h->cacheDerived) enqueue_derived(h, b, form);
     if (form == 0) {
         launch_batch_fused_fp32(d, b->items[0], (int)pl.items.size(), b->counters + 0, b->partF, pl.grid, st);
-        launch_batch_merge_decide(d, b->partF, h->epsRel, minScore, minRatio, b->sel, b->flagList, b->counters, st);
+        launch_batch_merge_decide(d, b->partF, h->epsRel, minScore, minRatio, b->sel, b->flagList, b->counters,
+                                  nullptr, st);
         launch_batch_recheck(d, b->flagList, b->counters, minScore, minRatio, b->sel, 1024, st);
     } else if (form == 1) {
         launch_batch_pass_bf16(d, b->items[1], (int)pl.items.size(), b->counters + 1, b->partB, pl.grid, st);
@@ -384,6 +457,17 @@ int enqueue_run(v3dm_context* h, BatchState* b, int mode, float minScore, float 
     } else {
         launch_batch_pass_code(d, b->items[2], (int)pl.items.size(), b->counters + 2, b->partB, pl.grid, st);
         launch_batch_rescore(d, b->partB, 0.0f, h->learnedTopK, minScore, minRatio, b->sel, b->stripes, st);
+        if (h->fallbackSamples > 0) {
+            int dev = 0, cus = 256;
+            (void)hipGetDevice(&dev);
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            launch_batch_sample_gather(d, h->fallbackSamples, b->sRows, b->sNorm, b->sIdx, b->sCount, st);
+            launch_batch_sample_fp32(d, b->sRows, b->sCount, b->sItems, b->nSItems, b->counters + 3, b->sPart,
+                                     b->sMaxSplits, 2 * cus, st);
+            launch_batch_sample_decide(d, b->sPart, b->sMaxSplits, b->sNSplits, b->sNorm, b->sIdx, b->sCount,
+                                       h->epsRel, minScore, minRatio, h->fallbackRecall, b->sel, b->pairFlag,
+                                       b->pairFlag + b->nPairs, st);
+        }
     }
     launch_batch_mutual(d, b->sel, b->leftRows, b->tmp, b->blockCount, b->out, b->pairCount, b->stripes,
                         b->counters, st);
@@ -463,6 +547,63 @@ int ensure_plan(v3dm_context* h, BatchState* b, int form)
     return V3DM_OK;
 }
 
+// After a learned run: pairs whose sampled recall fell below the threshold get the plain
+// form's decisions (plain items of those pairs only, merge masked to their segments,
+// re-check, mutual check of the whole batch), then the result counts are read again.
+int run_fallback(v3dm_context* h, BatchState* b)
+{
+    int const P = b->nPairs;
+    int const* flag = b->hostChk;
+    int const* cnt = b->hostChk + P;
+    std::vector<int> mask(2 * P, 0);
+    int nFall = 0, sampled = 0, missed = 0;
+    for (int p = 0; p < P; ++p) {
+        sampled += cnt[4 * p + 3];
+        missed += cnt[4 * p + 1];
+        if (flag[p]) {
+            mask[p] = mask[P + p] = 1;
+            ++nFall;
+        }
+    }
+    h->stats.nSampled = sampled;
+    h->stats.nSampleMissed = missed;
+    h->stats.nFallback = nFall;
+    if (nFall == 0) return V3DM_OK;
+    int rc;
+    if (!b->planValid[0]) {
+        int dev = 0, cus = 256;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        drop_graphs(b);
+        Plan pl = plan_plain(b, 32, 2 * cus);
+        if ((rc = upload_plan(h, b, 0, pl))) return rc;
+        b->plan[0] = std::move(pl);
+        b->planValid[0] = true;
+        b->version += 1;
+    }
+    Plan const& pl = b->plan[0];
+    std::vector<int4> items;
+    for (int4 const& w : pl.items)
+        if (mask[w.x]) items.push_back(w);
+    if ((rc = growi(h, b->fbItems, b->fbItemCap, std::max<size_t>(items.size(), 1)))) return rc;
+    hipStream_t const st = h->stream;
+    BCHECK(h, hipMemcpyAsync(b->fbItems, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, st));
+    BCHECK(h, hipMemcpyAsync(b->segMask, mask.data(), mask.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    BatchDev const d = dev_view(b, 0);
+    launch_batch_reset(b->counters, b->pairCount, P, st);
+    launch_batch_fused_fp32(d, b->fbItems, (int)items.size(), b->counters + 0, b->partF, pl.grid, st);
+    launch_batch_merge_decide(d, b->partF, h->epsRel, b->lastMinScore, b->lastMinRatio, b->sel, b->flagList,
+                              b->counters, b->segMask, st);
+    launch_batch_recheck(d, b->flagList, b->counters, b->lastMinScore, b->lastMinRatio, b->sel, 1024, st);
+    launch_batch_mutual(d, b->sel, b->leftRows, b->tmp, b->blockCount, b->out, b->pairCount, b->stripes, b->counters,
+                        st);
+    BCHECK(h, hipGetLastError());
+    BCHECK(h, hipMemcpyAsync(b->hostRes, b->counters, kBatchCounters * sizeof(int), hipMemcpyDeviceToHost, st));
+    BCHECK(h, hipMemcpyAsync(b->hostRes + kBatchCounters, b->pairCount, P * sizeof(int), hipMemcpyDeviceToHost, st));
+    BCHECK(h, hipStreamSynchronize(st));  // host vectors above go out of scope
+    return V3DM_OK;
+}
+
 int check_batch(v3dm_context* h, BatchState*& b)
 {
     if (!h) return V3DM_ERR_INVALID;
@@ -510,6 +651,10 @@ int set_images(v3dm_context* h, int nImages, int32_t const* counts, void const* 
         if (b->norm) (void)hipFree(b->norm);
         if (b->codes) (void)hipFree(b->codes);  // sized with the image rows: allocated again on use
         b->codes = nullptr;
+        if (b->imgU8) (void)hipFree(b->imgU8);
+        if (b->denom) (void)hipFree(b->denom);
+        b->imgU8 = nullptr;
+        b->denom = nullptr;
         b->img = nullptr;
         b->imgB = nullptr;
         b->norm = nullptr;
@@ -560,9 +705,14 @@ int set_images(v3dm_context* h, int nImages, int32_t const* counts, void const* 
     BCHECK(h, hipMemcpyAsync(b->srcRow, srcRowH.data(), nImages * sizeof(int), hipMemcpyHostToDevice, st));
     BCHECK(h, hipMemcpyAsync(b->blkImg, blkImgH.data(), nBlk * sizeof(int), hipMemcpyHostToDevice, st));
     BCHECK(h, hipMemsetAsync(b->maxNorm, 0, nImages * sizeof(unsigned int), st));
+    if (u8 && !b->imgU8) {
+        BCHECK(h, hipMalloc(&b->imgU8, (size_t)b->imgCapRows * kDim));
+        BCHECK(h, hipMalloc(&b->denom, (size_t)b->imgCapRows * sizeof(float)));
+    }
+    b->haveU8 = u8;
     if (u8)
         launch_batch_load_u8(static_cast<uint8_t const*>(b->upload), b->srcRow, b->nImgD, b->blkImg, b->imgBase, rows,
-                             b->img, st);
+                             b->img, b->imgU8, b->denom, st);
     else
         launch_batch_load_f32(static_cast<float const*>(b->upload), b->srcRow, b->nImgD, b->blkImg, b->imgBase, rows,
                               b->img, st);
@@ -580,14 +730,17 @@ void batch_destroy(BatchState*& b)
 {
     if (!b) return;
     drop_graphs(b);
-    for (void* p : {(void*)b->img, (void*)b->imgB, (void*)b->codes, (void*)b->imgBase, (void*)b->blkImg, (void*)b->nImgD,
+    for (void* p : {(void*)b->img, (void*)b->imgB, (void*)b->codes, (void*)b->imgU8, (void*)b->denom, (void*)b->imgBase, (void*)b->blkImg, (void*)b->nImgD,
                     (void*)b->srcRow, (void*)b->norm, (void*)b->maxNorm, b->upload, (void*)b->segOfBlock,
                     (void*)b->items[0], (void*)b->items[1], (void*)b->segD[0], (void*)b->segD[1], (void*)b->partF,
-                    (void*)b->partB, (void*)b->sel, (void*)b->flagList,  (void*)b->counters, (void*)b->stripes, (void*)b->pairCount,
+                    (void*)b->partB, (void*)b->sel, (void*)b->flagList, (void*)b->sRows, (void*)b->sNorm, (void*)b->sIdx, (void*)b->sCount,
+                    (void*)b->sNSplits, (void*)b->sPart, (void*)b->sItems, (void*)b->pairFlag, (void*)b->fbItems,
+                    (void*)b->segMask,  (void*)b->counters, (void*)b->stripes, (void*)b->pairCount,
                     (void*)b->tmp, (void*)b->blockCount, (void*)b->out})
         if (p) (void)hipFree(p);
     if (b->pinned) (void)hipHostFree(b->pinned);
     if (b->hostRes) (void)hipHostFree(b->hostRes);
+    if (b->hostChk) (void)hipHostFree(b->hostChk);
     if (b->evA) (void)hipEventDestroy(b->evA);
     if (b->evB) (void)hipEventDestroy(b->evB);
     delete b;
@@ -625,7 +778,9 @@ int v3dm_batch_set_pairs(v3dm_handle h, int nPairs, const int32_t* left, const i
     b->nPairs = nPairs;
     b->pa.assign(left, left + nPairs);
     b->pb.assign(right, right + nPairs);
-    return ensure_query_space(h, b);
+    int const rc = ensure_query_space(h, b);
+    if (rc) return rc;
+    return ensure_sample_plan(h, b);
 }
 
 int v3dm_batch_set_tuning(v3dm_handle h, int autotune, int mode, int tilesPerItem, int grid)
@@ -700,7 +855,8 @@ int v3dm_batch_match_async(v3dm_handle h, float minScore, float minRatio)
     BCHECK(h, hipEventRecord(b->evA, st));
     if (b->useGraph) {
         BatchState::GraphKey const key{mode, minScore, minRatio, h->epsRel, b->version * 2 + (h->cacheDerived ? 1 : 0),
-                                       h->learnedTopK};
+                                       h->learnedTopK, form == 2 ? h->fallbackSamples : 0,
+                                       form == 2 ? h->fallbackRecall : 0.0f};
         auto it = b->graphs.find(key);
         if (it == b->graphs.end()) {
             hipGraph_t g = nullptr;
@@ -726,6 +882,11 @@ int v3dm_batch_match_async(v3dm_handle h, float minScore, float minRatio)
     BCHECK(h, hipMemcpyAsync(b->hostRes, b->counters, kBatchCounters * sizeof(int), hipMemcpyDeviceToHost, st));
     BCHECK(h, hipMemcpyAsync(b->hostRes + kBatchCounters, b->pairCount, b->nPairs * sizeof(int),
                              hipMemcpyDeviceToHost, st));
+    b->checkPending = form == 2 && h->fallbackSamples > 0;
+    if (b->checkPending)
+        BCHECK(h, hipMemcpyAsync(b->hostChk, b->pairFlag, 5 * b->nPairs * sizeof(int), hipMemcpyDeviceToHost, st));
+    b->lastMinScore = minScore;
+    b->lastMinRatio = minRatio;
     b->pending = true;
     b->lastMode = mode;
     return V3DM_OK;
@@ -741,6 +902,11 @@ int v3dm_batch_get_matches(v3dm_handle h, int32_t* offsets, int32_t* pairs, int 
     BCHECK(h, hipStreamSynchronize(h->stream));
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, b->evA, b->evB) == hipSuccess) b->lastMs = ms;
+    h->stats.nFallback = h->stats.nSampled = h->stats.nSampleMissed = 0;
+    if (b->checkPending) {
+        b->checkPending = false;
+        if ((rc = run_fallback(h, b))) return rc;
+    }
     int const total = b->hostRes[kBCntPairs];
     *nTotal = total;
     if (offsets) {
@@ -767,6 +933,15 @@ int v3dm_batch_match(v3dm_handle h, float minScore, float minRatio, int32_t* off
     int const rc = v3dm_batch_match_async(h, minScore, minRatio);
     if (rc) return rc;
     return v3dm_batch_get_matches(h, offsets, pairs, cap, nTotal);
+}
+
+int v3dm_batch_get_fallback(v3dm_handle h, int32_t* flags)
+{
+    if (!h || !flags) return V3DM_ERR_INVALID;
+    BatchState* b = ctx_batch(h);
+    if (!b || b->nPairs <= 0 || !b->hostChk) return ctx_fail(h, V3DM_ERR_STATE, "no batch");
+    for (int p = 0; p < b->nPairs; ++p) flags[p] = b->hostChk[p];
+    return V3DM_OK;
 }
 
 int v3dm_batch_last_ms(v3dm_handle h, float* ms)

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include "v3dm_device.h"
+#include "v3dm_kernels.h"
 #include "v3dm_short.h"
 
 namespace v3dm {
@@ -49,6 +50,8 @@ struct BatchDev {
     float const* img;
     __bf16 const* imgB;
     signed char const* codes;     // learned codes, 32 bytes per image row
+    uint8_t const* u8;            // uint8 rows (images uploaded as uint8) or null
+    float const* denom;           // their scales: element k = (float)u8[k] * denom
     int const* imgBase;
     float const* norm;            // per image row
     unsigned int const* maxNorm;  // per image (float bits)
@@ -73,8 +76,18 @@ void launch_batch_pass_code(BatchDev const& b, int4 const* items, int nItems, in
 void launch_batch_reset(int* counters, int* pairCount, int nPairs, hipStream_t st);
 void launch_batch_norms(float const* img, int const* rowImg, int rows, float* norm, unsigned int* maxNorm,
                         hipStream_t st);
+// segMask (may be null): segments whose mask is 0 keep their decisions
 void launch_batch_merge_decide(BatchDev const& b, Top2 const* part, float epsRel, float minScore, float minRatio,
-                               int* sel, int* flagList, int* counters, hipStream_t st);
+                               int* sel, int* flagList, int* counters, int const* segMask, hipStream_t st);
+// sampled exact check of a shortlist run (v3dm_batch.hip, v3dm_fp32.hip)
+void launch_batch_sample_gather(BatchDev const& b, int samples, float* sRows, float* sNorm, int* sIdx, int* sCount,
+                                hipStream_t st);
+void launch_batch_sample_fp32(BatchDev const& b, float const* sRows, int const* sCount, int4 const* items,
+                              int nItems, int* queue, Top2* sPart, int maxSplits, int grid, hipStream_t st);
+void launch_batch_sample_decide(BatchDev const& b, Top2 const* sPart, int maxSplits, int const* nSplits,
+                                float const* sNorm, int const* sIdx, int const* sCount, float epsRel, float minScore,
+                                float minRatio, float minRecall, int const* sel, int* pairFlag, int* counts,
+                                hipStream_t st);
 void launch_batch_recheck(BatchDev const& b, int const* flagList, int* counters, float minScore, float minRatio,
                           int* sel, int grid, hipStream_t st);
 void launch_batch_mutual(BatchDev const& b, int const* sel, int leftRows, int2* tmp, int* blockCount, int2* pairs,

@@ -66,12 +66,15 @@ void launch_batch_norms(float const* img, int const* blkImg, int rows, float* no
 // the single-pair merge).  Open queries are appended to flagList (query-space rows).
 __global__ __launch_bounds__(256) void k_batch_merge_decide(BatchDev b, Top2 const* __restrict__ part, float epsRel,
                                                             float minScore, float minRatio, int* __restrict__ sel,
-                                                            int* __restrict__ flagList, int* __restrict__ counters)
+                                                            int* __restrict__ flagList, int* __restrict__ counters,
+                                                            int const* __restrict__ segMask)
 {
     int const sub = threadIdx.x & 7;
     int const g = (int)((blockIdx.x * 256u + threadIdx.x) >> 3);
     if (g >= b.qRows) return;  // uniform over the 8 lanes of a query
-    BatchSeg const sg = b.seg[b.segOfBlock[g / kBatchRowAlign]];
+    int const si = b.segOfBlock[g / kBatchRowAlign];
+    if (segMask && !segMask[si]) return;  // a segment the run leaves as it is
+    BatchSeg const sg = b.seg[si];
     int const q = g - sg.qOff;
     int const np = (q < sg.n) ? sg.nParts : 0;
     Top2 const* pp = part + sg.partOff + q;
@@ -95,11 +98,11 @@ __global__ __launch_bounds__(256) void k_batch_merge_decide(BatchDev b, Top2 con
 }
 
 void launch_batch_merge_decide(BatchDev const& b, Top2 const* part, float epsRel, float minScore, float minRatio,
-                               int* sel, int* flagList, int* counters, hipStream_t st)
+                               int* sel, int* flagList, int* counters, int const* segMask, hipStream_t st)
 {
     if (b.qRows <= 0) return;
     hipLaunchKernelGGL(k_batch_merge_decide, dim3((b.qRows + 31) / 32), dim3(256), 0, st, b, part, epsRel, minScore,
-                       minRatio, sel, flagList, counters);
+                       minRatio, sel, flagList, counters, segMask);
 }
 
 // Exact decision of one open query per work-group (grid-stride over the flag list).
@@ -308,10 +311,14 @@ void launch_batch_mutual(BatchDev const& b, int const* sel, int leftRows, int2* 
 // Input rows -> the aligned image buffer (one wave per destination row).  Row r of the
 // buffer belongs to image m = blkImg[r / kBatchRowAlign]; its local index r - imgBase[m] is
 // a feature when below nImg[m] (the count rounded down to 32) and zero padding otherwise.
+// The uint8 rows are also kept in the aligned layout with their scale (imgU8, denom): the
+// exact rescoring of the shortlist forms reads 128 bytes per candidate instead of 512 and
+// forms the same float values.
 __global__ __launch_bounds__(256) void k_batch_load_u8(uint8_t const* __restrict__ src, int const* __restrict__ srcRow,
                                                        int const* __restrict__ nImg, int const* __restrict__ blkImg,
                                                        int const* __restrict__ imgBase, int rows,
-                                                       float* __restrict__ img)
+                                                       float* __restrict__ img, uint8_t* __restrict__ imgU8,
+                                                       float* __restrict__ denom)
 {
     int const lane = threadIdx.x & 63;
     int const row = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -319,9 +326,16 @@ __global__ __launch_bounds__(256) void k_batch_load_u8(uint8_t const* __restrict
     int const m = blkImg[row / kBatchRowAlign];
     int const local = row - imgBase[m];
     float2 v = make_float2(0.0f, 0.0f);
-    if (local < nImg[m])
-        v = dequant_u8_wave(reinterpret_cast<uchar2 const*>(src + (size_t)(srcRow[m] + local) * kDim)[lane]);
+    uchar2 q = make_uchar2(0, 0);
+    float d = 0.0f;
+    if (local < nImg[m]) {
+        q = reinterpret_cast<uchar2 const*>(src + (size_t)(srcRow[m] + local) * kDim)[lane];
+        d = dequant_u8_denom(q);
+        v = make_float2(dequant_u8_value(q.x, d), dequant_u8_value(q.y, d));
+    }
     reinterpret_cast<float2*>(img + (size_t)row * kDim)[lane] = v;
+    reinterpret_cast<uchar2*>(imgU8 + (size_t)row * kDim)[lane] = q;
+    if (lane == 0) denom[row] = d;
 }
 
 __global__ __launch_bounds__(256) void k_batch_load_f32(float const* __restrict__ src, int const* __restrict__ srcRow,
@@ -340,11 +354,11 @@ __global__ __launch_bounds__(256) void k_batch_load_f32(float const* __restrict_
 }
 
 void launch_batch_load_u8(uint8_t const* src, int const* srcRow, int const* nImg, int const* blkImg,
-                          int const* imgBase, int rows, float* img, hipStream_t st)
+                          int const* imgBase, int rows, float* img, uint8_t* imgU8, float* denom, hipStream_t st)
 {
     if (rows <= 0) return;
     hipLaunchKernelGGL(k_batch_load_u8, dim3((rows + 3) / 4), dim3(256), 0, st, src, srcRow, nImg, blkImg, imgBase,
-                       rows, img);
+                       rows, img, imgU8, denom);
 }
 
 void launch_batch_load_f32(float const* src, int const* srcRow, int const* nImg, int const* blkImg,
@@ -353,6 +367,106 @@ void launch_batch_load_f32(float const* src, int const* srcRow, int const* nImg,
     if (rows <= 0) return;
     hipLaunchKernelGGL(k_batch_load_f32, dim3((rows + 3) / 4), dim3(256), 0, st, src, srcRow, nImg, blkImg, imgBase,
                        rows, img);
+}
+
+// ------------------------------------------------------------------------------ sampled check
+// Per pair p, S_p = min(samples, nL_p) evenly spaced left features: rows, norms and indices
+// gathered into kSampleRows-row blocks (zero rows past S_p).  One wave per sample row.
+__global__ __launch_bounds__(256) void k_batch_sample_gather(BatchDev b, int samples, float* __restrict__ sRows,
+                                                             float* __restrict__ sNorm, int* __restrict__ sIdx,
+                                                             int* __restrict__ sCount)
+{
+    int const lane = threadIdx.x & 63;
+    int const r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    int const p = r / kSampleRows, k = r - p * kSampleRows;
+    if (p >= b.nPairs) return;
+    BatchSeg const sl = b.seg[p];
+    int const S = min(samples, sl.n);
+    int const i = (k < S) ? (int)(((long long)k * sl.n) / S) : -1;
+    int const row = b.imgBase[sl.img] + (i >= 0 ? i : 0);
+    float2 const v = (i >= 0) ? reinterpret_cast<float2 const*>(b.img + (size_t)row * kDim)[lane]
+                              : make_float2(0.f, 0.f);
+    reinterpret_cast<float2*>(sRows + (size_t)r * kDim)[lane] = v;
+    if (lane == 0) {
+        sNorm[r] = (i >= 0) ? b.norm[row] : 0.0f;
+        sIdx[r] = i;
+        if (k == 0) sCount[p] = S;
+    }
+}
+
+// One work-group (kSampleRows threads) per pair: merge the sample's split partials, take
+// the certified FP32 decision (samples whose decision is not certified -- near ties, well
+// under 1% -- are left out of the estimate), compare with the shortlist decision sel of the
+// same left feature.  pairFlag[p] = 1 when the sampled recall (accepted - missed) /
+// accepted falls below minRecall; counts[4p..] = accepted, missed, left out.
+__global__ __launch_bounds__(kSampleRows) void k_batch_sample_decide(BatchDev b, Top2 const* __restrict__ sPart,
+                                                                     int maxSplits, int const* __restrict__ nSplits,
+                                                                     float const* __restrict__ sNorm,
+                                                                     int const* __restrict__ sIdx,
+                                                                     int const* __restrict__ sCount, float epsRel,
+                                                                     float minScore, float minRatio,
+                                                                     float minRecall, int const* __restrict__ sel,
+                                                                     int* __restrict__ pairFlag,
+                                                                     int* __restrict__ counts)
+{
+    __shared__ int red[2][3];
+    int const p = blockIdx.x;
+    int const k = threadIdx.x;
+    BatchSeg const sl = b.seg[p];
+    int const S = sCount[p];
+    int acc = 0, miss = 0, out = 0;
+    if (k < S && sl.nO > 0) {
+        Top2 s = top2_init();
+        Top2 const* pp = sPart + (size_t)p * maxSplits * kSampleRows + k;
+        for (int sp = 0; sp < nSplits[p]; ++sp) s = top2_merge(s, pp[(size_t)sp * kSampleRows]);
+        float const eps = epsRel * sNorm[p * kSampleRows + k] * __uint_as_float(b.maxNorm[sl.oImg]) + kEpsAbs;
+        int const e = (epsRel > 0.0f) ? decide_certified(s, eps, minScore, minRatio) : kFlag;
+        if (e == kFlag) {
+            out = 1;
+        } else {
+            int const d = sel[sl.qOff + sIdx[p * kSampleRows + k]];
+            acc = e >= 0;
+            miss = (e >= 0 && d != e);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        acc += __shfl_xor(acc, o);
+        miss += __shfl_xor(miss, o);
+        out += __shfl_xor(out, o);
+    }
+    if ((k & 63) == 0) {
+        red[k >> 6][0] = acc;
+        red[k >> 6][1] = miss;
+        red[k >> 6][2] = out;
+    }
+    __syncthreads();
+    if (k == 0) {
+        int const a = red[0][0] + red[1][0], m = red[0][1] + red[1][1], x = red[0][2] + red[1][2];
+        counts[4 * p + 0] = a;
+        counts[4 * p + 1] = m;
+        counts[4 * p + 2] = x;
+        counts[4 * p + 3] = S;
+        pairFlag[p] = (a > 0 && (float)(a - m) < minRecall * (float)a) ? 1 : 0;
+    }
+}
+
+void launch_batch_sample_gather(BatchDev const& b, int samples, float* sRows, float* sNorm, int* sIdx, int* sCount,
+                                hipStream_t st)
+{
+    int const rows = b.nPairs * kSampleRows;
+    if (rows <= 0) return;
+    hipLaunchKernelGGL(k_batch_sample_gather, dim3((rows + 3) / 4), dim3(256), 0, st, b, samples, sRows, sNorm, sIdx,
+                       sCount);
+}
+
+void launch_batch_sample_decide(BatchDev const& b, Top2 const* sPart, int maxSplits, int const* nSplits,
+                                float const* sNorm, int const* sIdx, int const* sCount, float epsRel, float minScore,
+                                float minRatio, float minRecall, int const* sel, int* pairFlag, int* counts,
+                                hipStream_t st)
+{
+    if (b.nPairs <= 0) return;
+    hipLaunchKernelGGL(k_batch_sample_decide, dim3(b.nPairs), dim3(kSampleRows), 0, st, b, sPart, maxSplits, nSplits,
+                       sNorm, sIdx, sCount, epsRel, minScore, minRatio, minRecall, sel, pairFlag, counts);
 }
 
 }  // namespace v3dm

@@ -101,13 +101,21 @@ __device__ __forceinline__ int decide_certified(Top2 s, float eps, float minScor
 // as SIFT_Feature::normalizeDescriptor (V3D/Base/v3d_feature.h:30-38): the sum of squares
 // of integers <= 255 over 128 terms stays below 2^24, so it is exact in FP32 in any order,
 // and the scale is 1/sqrt with IEEE rounding.
-__device__ __forceinline__ float2 dequant_u8_wave(uchar2 v)
+__device__ __forceinline__ float dequant_u8_denom(uchar2 v)
 {
     float const a = (float)v.x, b = (float)v.y;
     float s = a * a + b * b;
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    float const denom = __fdiv_rn(1.0f, __fsqrt_rn(s));
-    return make_float2(a * denom, b * denom);
+    return __fdiv_rn(1.0f, __fsqrt_rn(s));
+}
+
+// element value: the IEEE product (float)q * denom, the same in every kernel that forms it
+__device__ __forceinline__ float dequant_u8_value(unsigned int q, float denom) { return __fmul_rn((float)q, denom); }
+
+__device__ __forceinline__ float2 dequant_u8_wave(uchar2 v)
+{
+    float const denom = dequant_u8_denom(v);
+    return make_float2(dequant_u8_value(v.x, denom), dequant_u8_value(v.y, denom));
 }
 
 typedef float v16f __attribute__((ext_vector_type(16)));

@@ -94,6 +94,7 @@ __device__ __forceinline__ void tile_fp32(unsigned char* stage, float const (&bf
         top2_push_fast(colS, v0, jt + jl, inf);
         top2_push_fast(colS, v1, jt + jl + 32, inf);
     }
+    if (!rowPart) return;  // left decisions only (uniform: a kernel argument)
     __syncthreads();  // every wave is done reading the stage as the A operand
     float* S = reinterpret_cast<float*>(stage);
 #pragma unroll
@@ -221,6 +222,39 @@ __global__ __launch_bounds__(256, 2) void k_batch_fused_fp32(BatchDev b, int4 co
         fused_block(L, R, sl.n, sr.n, w.y, w.z, w.w, part + sl.partOff, sl.ld, part + sr.partOff, sr.ld, stA, stB,
                     inf);
     }
+}
+
+// Sampled check of a batch: item = (pair, -, split, tiles per split); the left block is the
+// pair's gathered sample rows (kSampleRows), left decisions only.
+__global__ __launch_bounds__(256, 2) void k_batch_sample_fp32(BatchDev b, float const* __restrict__ sRows,
+                                                              int const* __restrict__ sCount,
+                                                              int4 const* __restrict__ items, int nItems,
+                                                              int* __restrict__ queue, Top2* __restrict__ sPart,
+                                                              int maxSplits, float inf)
+{
+    __shared__ __attribute__((aligned(1024))) unsigned char stA[kStageBytes];
+    __shared__ __attribute__((aligned(1024))) unsigned char stB[kStageBytes];
+    __shared__ int sItem;
+    for (;;) {
+        if (threadIdx.x == 0) sItem = atomicAdd(queue, 1);
+        __syncthreads();
+        int const it = __builtin_amdgcn_readfirstlane(sItem);
+        __syncthreads();
+        if (it >= nItems) break;
+        int4 const w = items[it];
+        BatchSeg const sr = b.seg[b.nPairs + w.x];
+        float const* R = b.img + (size_t)b.imgBase[sr.img] * kDim;
+        fused_block(sRows + (size_t)w.x * kSampleRows * kDim, R, sCount[w.x], sr.n, 0, w.z, w.w,
+                    sPart + (size_t)w.x * maxSplits * kSampleRows, kSampleRows, nullptr, 0, stA, stB, inf);
+    }
+}
+
+void launch_batch_sample_fp32(BatchDev const& b, float const* sRows, int const* sCount, int4 const* items,
+                              int nItems, int* queue, Top2* sPart, int maxSplits, int grid, hipStream_t st)
+{
+    if (nItems <= 0) return;
+    hipLaunchKernelGGL(k_batch_sample_fp32, dim3(grid), dim3(256), 0, st, b, sRows, sCount, items, nItems, queue,
+                       sPart, maxSplits, INFINITY);
 }
 
 void launch_batch_fused_fp32(BatchDev const& b, int4 const* items, int nItems, int* queue, Top2* part, int grid,

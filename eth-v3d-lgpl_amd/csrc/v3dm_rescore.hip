@@ -59,12 +59,36 @@ __device__ __forceinline__ double exact_dot(float const* __restrict__ q, float c
     return acc;
 }
 
+// Exact score against a uint8 candidate row: element k = (float)c_k * denom, the value the
+// FP32 image holds (same IEEE product).
+__device__ __forceinline__ double exact_dot_u8(float const* __restrict__ q, uint8_t const* __restrict__ c, float d)
+{
+    float4 const* qp = reinterpret_cast<float4 const*>(q);
+    uint4 const* cp = reinterpret_cast<uint4 const*>(c);
+    double acc = 0.0;
+#pragma unroll 2
+    for (int k16 = 0; k16 < 8; ++k16) {
+        uint4 const w = cp[k16];
+        unsigned int const ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            float4 const x = qp[4 * k16 + u];
+            acc += (double)x.x * (double)dequant_u8_value(ws[u] & 0xffu, d);
+            acc += (double)x.y * (double)dequant_u8_value((ws[u] >> 8) & 0xffu, d);
+            acc += (double)x.z * (double)dequant_u8_value((ws[u] >> 16) & 0xffu, d);
+            acc += (double)x.w * (double)dequant_u8_value(ws[u] >> 24, d);
+        }
+    }
+    return acc;
+}
+
 // One query, eight lanes (sub = lane & 7).  part: the query's (split 0, half 0) list; the
 // lists of split s are at part[s * ldq2 + h]; slots: the group's kRescoreSlots ints of LDS.
 // Returns the exact top-2 (all lanes) and the number of candidates scored by this lane.
 __device__ __forceinline__ Top2x finish_query(Cand2 const* __restrict__ part, size_t ldq2, int nParts, float nq,
                                               float mo, float winRel, int topK, float const* __restrict__ Qd,
-                                              float const* __restrict__ Cd, int sub, int* slots, int& nScored)
+                                              float const* __restrict__ Cd, uint8_t const* __restrict__ Cu8,
+                                              float const* __restrict__ Cdenom, int sub, int* slots, int& nScored)
 {
     float v[kPerLane];
     int j[kPerLane];
@@ -150,7 +174,8 @@ __device__ __forceinline__ Top2x finish_query(Cand2 const* __restrict__ part, si
     int n = 0;
     for (int k = sub; k < total; k += 8) {
         int const jj = slots[k];
-        float const sc = (float)exact_dot(Qd, Cd + (size_t)jj * kDim);
+        float const sc = Cu8 ? (float)exact_dot_u8(Qd, Cu8 + (size_t)jj * kDim, Cdenom[jj])
+                             : (float)exact_dot(Qd, Cd + (size_t)jj * kDim);
         if (sc == sc) s = top2x_merge(s, Top2x{sc, kInit, jj, -1});
         ++n;
     }
@@ -198,8 +223,8 @@ __global__ __launch_bounds__(256) void k_shortlist_finish(float const* __restric
         Top2x const s = finish_query((left ? partL : partR) + (size_t)q * 2, 2 * (size_t)(left ? ldqL : ldqR),
                                      left ? nPartL : nPartR, left ? normL[q] : normR[q],
                                      __uint_as_float(maxNormBits[left ? 1 : 0]), winRel, topK,
-                                     (left ? L : R) + (size_t)q * kDim, left ? R : L, sub, slots[threadIdx.x >> 3],
-                                     n);
+                                     (left ? L : R) + (size_t)q * kDim, left ? R : L, nullptr, nullptr, sub,
+                                     slots[threadIdx.x >> 3], n);
         if (sub == 0) {
             (left ? colMax : rowMax)[q] = decide_exact(Top2{s.b1, s.b2, s.a1}, minScore, minRatio);
             if (colSecond) (left ? colSecond : rowSecond)[q] = s.a2;
@@ -237,9 +262,11 @@ __global__ __launch_bounds__(256) void k_batch_shortlist_finish(BatchDev b, Cand
         int d = -1;
         if (q < sg.n && sg.nO > 0 && sg.nParts > 0) {
             int const row = b.imgBase[sg.img] + q;
+            size_t const cb = (size_t)b.imgBase[sg.oImg];
             Top2x const s = finish_query(part + ((size_t)sg.partOff + q) * 2, 2 * (size_t)sg.ld, sg.nParts,
                                          b.norm[row], __uint_as_float(b.maxNorm[sg.oImg]), winRel, topK,
-                                         b.img + (size_t)row * kDim, b.img + (size_t)b.imgBase[sg.oImg] * kDim, sub,
+                                         b.img + (size_t)row * kDim, b.img + cb * kDim,
+                                         b.u8 ? b.u8 + cb * kDim : nullptr, b.u8 ? b.denom + cb : nullptr, sub,
                                          slots[threadIdx.x >> 3], n);
             d = decide_exact(Top2{s.b1, s.b2, s.a1}, minScore, minRatio);
         }

@@ -80,6 +80,7 @@ SIGNATURES = {
     "v3dm_batch_get_tuning": (_I, [_P, _I, _IP, _IP, _IP, C.POINTER(C.c_float)]),
     "v3dm_batch_use_graph": (_I, [_P, _I]),
     "v3dm_batch_last_ms": (_I, [_P, C.POINTER(C.c_float)]),
+    "v3dm_batch_get_fallback": (_I, [_P, _FP]),
     "v3dm_learned_set_state": (_I, [_P, _FP, _FP, _FP, _I]),
     "v3dm_learned_get_state": (_I, [_P, _FP, _FP, _FP, _IP]),
     "v3dm_learned_set_code_scale": (_I, [_P, _F]),

@@ -269,6 +269,13 @@ class SIFT_Matching:
     def batch_use_graph(self, enable=True):
         self._check(self._lib.v3dm_batch_use_graph(self._h, int(bool(enable))))
 
+    def batch_get_fallback(self):
+        """Per pair of the last learned batch run: 1 if its sampled check sent it to the
+        plain form."""
+        out = np.zeros(self._bpairs, dtype=np.int32)
+        self._check(self._lib.v3dm_batch_get_fallback(self._h, out.ctypes.data))
+        return out
+
     def batch_last_ms(self):
         ms = C.c_float(0)
         self._check(self._lib.v3dm_batch_last_ms(self._h, C.byref(ms)))

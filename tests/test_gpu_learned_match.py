@@ -166,3 +166,39 @@ def test_sampled_check_and_fallback(matcher):
     st = other.stats()
     assert st["nSampled"] == 0 and st["nFallback"] == 0
     other.close()
+
+
+def test_learned_batch_fallback(matcher):
+    """Batch: with the trained network no pair falls back; with an untrained network and a
+    single candidate per query, the pairs whose sampled recall falls short get exactly the
+    plain form's matches."""
+    m, net = matcher
+    counts, desc, left, right, _ = synth.make_batch(12, lo=600, hi=5000, config=43)
+    m.batch_set_images(counts, desc)
+    m.batch_set_pairs(left, right)
+    m.set_precision(v3dmatch.PREC_FP32)
+    plain = m.batch_match(0.9, 0.8)
+    m.set_precision(v3dmatch.PREC_LEARNED)
+    m.batch_match(0.9, 0.8)
+    st = m.stats()
+    say(f"case batch check (trained): sampled={st['nSampled']} missed={st['nSampleMissed']} "
+        f"fallback pairs={st['nFallback']}")
+    assert st["nSampled"] > 0 and st["nFallback"] == 0
+    other = v3dmatch.SIFT_Matching(0)
+    raw = learned.Network(other)
+    raw.set_params(learned.init_params(3))
+    raw.calibrate_code_scale(synth.dequantise(desc[:4096]))
+    other.batch_set_images(counts, desc)
+    other.batch_set_pairs(left, right)
+    other.set_precision(v3dmatch.PREC_LEARNED)
+    other.set_learned_topk(1)
+    other.set_fallback(128, 1.0)
+    got = other.batch_match(0.9, 0.8)
+    flags = other.batch_get_fallback()
+    st = other.stats()
+    say(f"case batch check (untrained, top-1): missed={st['nSampleMissed']} fallback pairs={int(flags.sum())}")
+    assert flags.sum() > 0 and st["nFallback"] == int(flags.sum())
+    for p in range(len(left)):
+        if flags[p]:
+            assert np.array_equal(got[p], plain[p]), p
+    other.close()
