@@ -79,6 +79,7 @@ void releaseBuffers(v3dm_context* h)
     freeDev(h->dQ);
     freeDev(h->dLeftXY);
     freeDev(h->dRightXY);
+    freeDev(h->guidedBits);
     if (h->hostStage) (void)hipHostFree(h->hostStage);
     h->hostStage = nullptr;
     h->dQBytes = 0;
@@ -259,6 +260,10 @@ int v3dm_allocate(v3dm_handle h, int nMaxLeft, int nMaxRight, int useGuidedMatch
     if (h->useGuided) {
         HIPCHECK(h, hipMalloc(&h->dLeftXY, cl * 2 * sizeof(float)));
         HIPCHECK(h, hipMalloc(&h->dRightXY, cr * 2 * sizeof(float)));
+        // one bit per (left, right) pair: 1/32 of the reference's score matrix (cu:467)
+        size_t const bitBytes = cl * (cr / 32) * sizeof(unsigned);
+        HIPCHECK(h, hipMalloc(&h->guidedBits, bitBytes));
+        HIPCHECK(h, hipMemsetAsync(h->guidedBits, 0, bitBytes, h->stream));
     }
     HIPCHECK(h, hipStreamSynchronize(h->stream));
     h->allocated = true;
@@ -353,6 +358,72 @@ int copyResults(v3dm_context* h, int32_t* pairs, int cap, int* nPairs)
     }
     if (n > cap) return fail(h, V3DM_ERR_CAPACITY, "pairs buffer holds %d pairs, %d matches found", cap, n);
     return V3DM_OK;
+}
+
+// runGuidedMatching / runGuidedHomographyMatching (cu:554-694).  The positions are
+// uploaded (AoS x, y per feature), the call's mask is added to the bitmap of removed pairs
+// (set on the first guided call after findPutativeMatches, so masks compound as the
+// reference's in-place overwrites of its score matrix do), then the exact FP32 matcher runs
+// over the current descriptors with removed pairs scored -1e20.  Guided calls always score
+// in FP32, whatever the precision mode.
+int runGuided(v3dm_context* h, int kind, const double M[9], float distanceThreshold, const float* leftXY,
+              const float* rightXY, float minScore, float minRatio, int32_t* pairs, int cap, int* nPairs)
+{
+    if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
+    if (!h->allocated) return fail(h, V3DM_ERR_STATE, "guided matching before allocate");
+    if (!h->useGuided || !h->guidedBits)
+        return fail(h, V3DM_ERR_STATE, "guided matching needs allocate(..., useGuidedMatching = true)");
+    if (!M) return fail(h, V3DM_ERR_INVALID, "null matrix");
+    int const nL = h->nL, nR = h->nR;
+    if ((nL > 0 && !leftXY) || (nR > 0 && !rightXY)) return fail(h, V3DM_ERR_INVALID, "null position array");
+    HIPCHECK(h, hipSetDevice(h->device));
+    hipStream_t const st = h->stream;
+    // a result still pending (e.g. the sampled check of a learned result) is superseded
+    ext_discard_result(h);
+    float m[9];
+    for (int t = 0; t < 9; ++t) m[t] = (float)M[t];  // cu:592-595 (double -> float3)
+    float const sqrThreshold = distanceThreshold * distanceThreshold;
+    bool const work = nL > 0 && nR > 0;
+    if (work) {
+        HIPCHECK(h, hipMemcpyAsync(h->dLeftXY, leftXY, (size_t)nL * 2 * sizeof(float), hipMemcpyHostToDevice, st));
+        HIPCHECK(h, hipMemcpyAsync(h->dRightXY, rightXY, (size_t)nR * 2 * sizeof(float), hipMemcpyHostToDevice,
+                                   st));
+        launch_guided_mask(kind, m, sqrThreshold, h->dLeftXY, h->dRightXY, nL, nR, h->guidedBits, h->capL,
+                           h->masks.empty(), st);
+    }
+    GuidedMask gm;
+    gm.kind = kind;
+    for (int t = 0; t < 9; ++t) gm.m[t] = m[t];
+    gm.sqrThreshold = sqrThreshold;
+    h->masks.push_back(gm);
+    int rc;
+    if (work) {
+        FusedPlan const p = plan_fused_fp32(nL, nR, 512);
+        if ((rc = prof_begin(h))) return rc;
+        launch_guided_fp32(h->dL, h->dR, nL, nR, p, h->colPart, h->capL, h->rowPart, h->capR, h->guidedBits,
+                           h->capL, st);
+        if ((rc = prof_end(h))) return rc;
+        launch_merge_decide(h->colPart, p.splits, h->capL, h->rowPart, p.leftBlocks, h->capR, nL, nR, h->normL,
+                            h->normR, h->maxNormBits, h->epsRel, minScore, minRatio, h->colMax, h->rowMax,
+                            h->flagList, h->capL, h->results, st);
+        launch_recheck_exact(h->dL, h->dR, nL, nR, h->flagList, h->capL, h->results, minScore, minRatio, h->colMax,
+                             h->rowMax, h->rpart, kRecheckGrid, st, h->guidedBits, h->capL);
+    }
+    launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mtmp, h->mblock, h->stripes,
+                          reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st);
+    HIPCHECK(h, hipGetLastError());
+    h->lastMinScore = minScore;
+    h->lastMinRatio = minRatio;
+    h->haveResult = true;
+    h->lastNL = nL;
+    h->lastNR = nR;
+    h->stats.nLeft = nL;
+    h->stats.nRight = nR;
+    h->stats.precision = V3DM_PREC_FP32;
+    h->stats.nFallback = 0;
+    h->stats.nSampled = 0;
+    h->stats.nSampleMissed = 0;
+    return copyResults(h, pairs, cap, nPairs);
 }
 
 }  // namespace
@@ -546,17 +617,13 @@ int v3dm_get_stats(v3dm_handle h, v3dm_stats* out)
 int v3dm_run_guided_F(v3dm_handle h, const double F[9], float distanceThreshold, const float* leftXY,
                       const float* rightXY, float minScore, float minRatio, int32_t* pairs, int cap, int* nPairs)
 {
-    (void)F; (void)distanceThreshold; (void)leftXY; (void)rightXY; (void)minScore; (void)minRatio;
-    (void)pairs; (void)cap; (void)nPairs;
-    return fail(h, V3DM_ERR_UNSUPPORTED, "guided matching is not built yet");
+    return runGuided(h, 0, F, distanceThreshold, leftXY, rightXY, minScore, minRatio, pairs, cap, nPairs);
 }
 
 int v3dm_run_guided_H(v3dm_handle h, const double H[9], float distanceThreshold, const float* leftXY,
                       const float* rightXY, float minScore, float minRatio, int32_t* pairs, int cap, int* nPairs)
 {
-    (void)H; (void)distanceThreshold; (void)leftXY; (void)rightXY; (void)minScore; (void)minRatio;
-    (void)pairs; (void)cap; (void)nPairs;
-    return fail(h, V3DM_ERR_UNSUPPORTED, "guided matching is not built yet");
+    return runGuided(h, 1, H, distanceThreshold, leftXY, rightXY, minScore, minRatio, pairs, cap, nPairs);
 }
 
 }  // extern "C"

@@ -55,6 +55,7 @@ struct v3dm_context {
     std::vector<v3dm::GuidedMask> masks;
     float* dLeftXY = nullptr;
     float* dRightXY = nullptr;
+    unsigned* guidedBits = nullptr;  // removed pairs: bit j%32 of word (j/32) * capL + i
 
     // last result
     bool haveResult = false;

@@ -314,6 +314,14 @@ int ext_second_best(v3dm_context* h, int* colSecond, int* rowSecond)
     return V3DM_OK;
 }
 
+void ext_discard_result(v3dm_context* h)
+{
+    ExtState* e = ctx_ext(h);
+    if (!e) return;
+    e->sPending = false;
+    e->haveSecond = false;
+}
+
 void ext_features_changed(v3dm_context* h, bool left)
 {
     ExtState* e = ctx_ext(h);

@@ -39,6 +39,8 @@ int ext_finish(v3dm_context* h);
 // exact second best candidate of every query of the last shortlist call (host arrays)
 int ext_second_best(v3dm_context* h, int* colSecond, int* rowSecond);
 void ext_features_changed(v3dm_context* h, bool left);
+// a guided call replaces the last result: drop its pending sampled check and second bests
+void ext_discard_result(v3dm_context* h);
 void ext_release(ExtState* e);
 void ext_destroy(ExtState*& e);
 // batched matching (v3dm_batch.cpp)

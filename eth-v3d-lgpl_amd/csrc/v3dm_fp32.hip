@@ -57,14 +57,29 @@ __device__ __forceinline__ void stage_tile_fp32(float const* __restrict__ R, int
         __builtin_amdgcn_global_load_lds(tile + src[p], (lds_void*)(stage + (4 * p + wv) * 1024), 16, 0, 0);
 }
 
-template <bool kFullJ, bool kFullI>
+// Removed (left, right) pairs of guided matching: bit (j & 31) of bits[(j >> 5) * ldi + i]
+// set means the score of left i and right j is replaced by -1e20 (cu:308, cu:340).
+struct GuidedBits {
+    unsigned const* bits;
+    int ldi;
+};
+
+template <bool kFullJ, bool kFullI, bool kGuided>
 __device__ __forceinline__ void tile_fp32(unsigned char* stage, float const (&bfrag)[64], Top2& colS, Top2* rowPart,
-                                          int ldr, int lb, int t, int nL, int nR, int i0, int tid, float inf)
+                                          int ldr, int lb, int t, int nL, int nR, int i0, int tid, float inf,
+                                          GuidedBits gb)
 {
     int const lane = tid & 63;
     int const wave = tid >> 6;
     int const h = lane >> 5;
     int const c32 = lane & 31;
+    unsigned gw0 = 0, gw1 = 0;
+    if (kGuided) {
+        // words of rows 64t .. 64t+31 and 64t+32 .. 64t+63 for this lane's left feature
+        size_t const gi = (size_t)(i0 + wave * 32 + c32);
+        gw0 = gb.bits[(size_t)(2 * t) * gb.ldi + gi];
+        gw1 = gb.bits[(size_t)(2 * t + 1) * gb.ldi + gi];
+    }
     v16f acc0 = {};
     v16f acc1 = {};
     unsigned char const* A0 = stage + (h * kBJ + c32) * kSeg;
@@ -83,6 +98,14 @@ __device__ __forceinline__ void tile_fp32(unsigned char* stage, float const (&bf
         acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, bfrag[4 * s4 + 3], acc1, 0, 0, 0);
     }
     int const jt = t * kBJ;
+    if (kGuided) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            int const jl = 8 * (r >> 2) + 4 * h + (r & 3);
+            acc0[r] = ((gw0 >> jl) & 1u) ? -1e20f : acc0[r];
+            acc1[r] = ((gw1 >> jl) & 1u) ? -1e20f : acc1[r];
+        }
+    }
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         int const jl = 8 * (r >> 2) + 4 * h + (r & 3);
@@ -121,30 +144,32 @@ __device__ __forceinline__ void tile_fp32(unsigned char* stage, float const (&bf
 
 // The tiles [t0, t1) of one work-group, double-buffered; kMasked handles a partial last
 // tile of right features or a partial block of left features.
-template <bool kMasked>
+template <bool kMasked, bool kGuided>
 __device__ __forceinline__ void run_tiles_fp32(float const* __restrict__ R, unsigned char* stA, unsigned char* stB,
                                                int const (&src)[kStageLoads], float const (&bfrag)[64], Top2& colS,
                                                Top2* rowPart, int ldr, int lb, int t0, int t1, int nL, int nR, int i0,
-                                               int tid, int wv, float inf)
+                                               int tid, int wv, float inf, GuidedBits gb)
 {
     // two tiles per trip so that each stage is a distinct LDS object
     for (int t = t0; t < t1; t += 2) {
         __syncthreads();  // tile t is in stA; every wave is done with stB
         if (t + 1 < t1) stage_tile_fp32(R, t + 1, stB, src, wv);
-        tile_fp32<!kMasked, !kMasked>(stA, bfrag, colS, rowPart, ldr, lb, t, nL, nR, i0, tid, inf);
+        tile_fp32<!kMasked, !kMasked, kGuided>(stA, bfrag, colS, rowPart, ldr, lb, t, nL, nR, i0, tid, inf, gb);
         if (t + 1 >= t1) break;
         __syncthreads();  // tile t+1 is in stB; every wave is done with stA
         if (t + 2 < t1) stage_tile_fp32(R, t + 2, stA, src, wv);
-        tile_fp32<!kMasked, !kMasked>(stB, bfrag, colS, rowPart, ldr, lb, t + 1, nL, nR, i0, tid, inf);
+        tile_fp32<!kMasked, !kMasked, kGuided>(stB, bfrag, colS, rowPart, ldr, lb, t + 1, nL, nR, i0, tid, inf,
+                                                gb);
     }
 }
 
 // One work unit: left block `lb` (128 features) against right tiles [split * tilesPerSplit,
 // ...) -> colPart[split * ldc + i], rowPart[lb * ldr + j].
+template <bool kGuided = false>
 __device__ __forceinline__ void fused_block(float const* __restrict__ L, float const* __restrict__ R, int nL, int nR,
                                             int lb, int split, int tilesPerSplit, Top2* __restrict__ colPart,
                                             int ldc, Top2* __restrict__ rowPart, int ldr, unsigned char* stA,
-                                            unsigned char* stB, float inf)
+                                            unsigned char* stB, float inf, GuidedBits gb = GuidedBits{nullptr, 0})
 {
     // an opaque copy of the thread index: in the persistent kernel it keeps the per-lane
     // offsets derived from it inside the item loop instead of live across it (spills)
@@ -181,8 +206,12 @@ __device__ __forceinline__ void fused_block(float const* __restrict__ L, float c
 
     Top2 colS = top2_init();
     bool const masked = (i0 + kBI > nL) || (t1 * kBJ > nR);
-    if (masked) run_tiles_fp32<true>(R, stA, stB, src, bfrag, colS, rowPart, ldr, lb, t0, t1, nL, nR, i0, tid, wv, inf);
-    else run_tiles_fp32<false>(R, stA, stB, src, bfrag, colS, rowPart, ldr, lb, t0, t1, nL, nR, i0, tid, wv, inf);
+    if (masked)
+        run_tiles_fp32<true, kGuided>(R, stA, stB, src, bfrag, colS, rowPart, ldr, lb, t0, t1, nL, nR, i0, tid, wv,
+                                      inf, gb);
+    else
+        run_tiles_fp32<false, kGuided>(R, stA, stB, src, bfrag, colS, rowPart, ldr, lb, t0, t1, nL, nR, i0, tid, wv,
+                                       inf, gb);
 
     colS = top2_merge(colS, top2_shfl_xor(colS, 32));
     if (h == 0 && myI < nL) colPart[(size_t)split * ldc + myI] = colS;
@@ -196,6 +225,19 @@ __global__ __launch_bounds__(256, 2) void k_fused_fp32(float const* __restrict__
     __shared__ __attribute__((aligned(1024))) unsigned char stA[kStageBytes];
     __shared__ __attribute__((aligned(1024))) unsigned char stB[kStageBytes];
     fused_block(L, R, nL, nR, blockIdx.x, blockIdx.y, tilesPerSplit, colPart, ldc, rowPart, ldr, stA, stB, inf);
+}
+
+// Guided matching: the same tiles with the removed pairs of the bitmap scored -1e20.
+__global__ __launch_bounds__(256, 2) void k_guided_fp32(float const* __restrict__ L, float const* __restrict__ R,
+                                                        int nL, int nR, int tilesPerSplit,
+                                                        Top2* __restrict__ colPart, int ldc,
+                                                        Top2* __restrict__ rowPart, int ldr,
+                                                        unsigned const* __restrict__ bits, int ldi, float inf)
+{
+    __shared__ __attribute__((aligned(1024))) unsigned char stA[kStageBytes];
+    __shared__ __attribute__((aligned(1024))) unsigned char stB[kStageBytes];
+    fused_block<true>(L, R, nL, nR, blockIdx.x, blockIdx.y, tilesPerSplit, colPart, ldc, rowPart, ldr, stA, stB, inf,
+                      GuidedBits{bits, ldi});
 }
 
 // Persistent form over a batch: each work-group takes items from the queue until it runs
@@ -284,6 +326,14 @@ void launch_fused_fp32(float const* L, float const* R, int nL, int nR, FusedPlan
     if (nL <= 0 || nR <= 0) return;
     hipLaunchKernelGGL(k_fused_fp32, dim3(p.leftBlocks, p.splits), dim3(256), 0, st, L, R, nL, nR,
                        p.tilesPerSplit, colPart, ldc, rowPart, ldr, INFINITY);
+}
+
+void launch_guided_fp32(float const* L, float const* R, int nL, int nR, FusedPlan const& p, Top2* colPart,
+                        int ldc, Top2* rowPart, int ldr, unsigned const* bits, int ldi, hipStream_t st)
+{
+    if (nL <= 0 || nR <= 0) return;
+    hipLaunchKernelGGL(k_guided_fp32, dim3(p.leftBlocks, p.splits), dim3(256), 0, st, L, R, nL, nR,
+                       p.tilesPerSplit, colPart, ldc, rowPart, ldr, bits, ldi, INFINITY);
 }
 
 }  // namespace v3dm

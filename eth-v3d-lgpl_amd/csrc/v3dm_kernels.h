@@ -48,10 +48,11 @@ void launch_merge_decide(Top2 const* colPart, int nColParts, int ldc, Top2 const
                          float epsRel, float minScore, float minRatio, int* colMax, int* rowMax, int* flagList,
                          int flagOffR, int* counters, hipStream_t st);
 RecheckPlan plan_recheck(int nC);
-// flagList: open left queries at [0, ...), open right queries at [flagOffR, ...)
+// flagList: open left queries at [0, ...), open right queries at [flagOffR, ...).  gbits:
+// guided-matching bitmap (pairs scored -1e20), or null.
 void launch_recheck_exact(float const* L, float const* R, int nL, int nR, int const* flagList, int flagOffR,
                           int const* counters, float minScore, float minRatio, int* colMax, int* rowMax, Top2* part,
-                          int grid, hipStream_t st);
+                          int grid, hipStream_t st, unsigned const* gbits = nullptr, int ldi = 0);
 // nL = 0 publishes an empty result.  tmp: nL int2, blockCount: ceil(nL/1024) ints.
 void launch_mutual_compact(int const* colMax, int const* rowMax, int nL, int nR, int2* tmp, int* blockCount,
                            int* stripes, int2* pairs, int* counters, hipStream_t st);
@@ -63,6 +64,15 @@ void launch_dequant_u8(uint8_t const* q, int n, float* out, hipStream_t st);
 FusedPlan plan_fused_fp32(int nL, int nR, int targetGroups, int maxSplits = kMaxSplits);
 void launch_fused_fp32(float const* L, float const* R, int nL, int nR, FusedPlan const& p, Top2* colPart, int ldc,
                        Top2* rowPart, int ldr, hipStream_t st);
+
+// guided matching (v3dm_guided.hip, v3dm_fp32.hip).  Bitmap of removed (left i, right j)
+// pairs: bit (j & 31) of bits[(j >> 5) * ldi + i]; ldi >= the left rows, a multiple of 128.
+// kind 0: Sampson distance with F (cu:277-311), 1: transfer distance with H (cu:313-343);
+// m: the matrix rounded to float, row-major.  first: set the bitmap, else add to it.
+void launch_guided_mask(int kind, float const (&m)[9], float sqrThreshold, float const* leftXY,
+                        float const* rightXY, int nL, int nR, unsigned* bits, int ldi, bool first, hipStream_t st);
+void launch_guided_fp32(float const* L, float const* R, int nL, int nR, FusedPlan const& p, Top2* colPart, int ldc,
+                        Top2* rowPart, int ldr, unsigned const* bits, int ldi, hipStream_t st);
 
 // sampled exact check of a shortlist result (v3dm_select.hip)
 void launch_sample_gather(float const* L, float const* normL, int nL, int S, float* Ls, float* normS, int* idx,

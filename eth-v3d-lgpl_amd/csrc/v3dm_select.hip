@@ -115,7 +115,8 @@ __global__ __launch_bounds__(256) void k_recheck_chunks(float const* __restrict_
                                                         int nL, int nR, int const* __restrict__ flagList,
                                                         int flagOffR, int const* __restrict__ counters,
                                                         int nChunksL, int perL, int nChunksR, int perR,
-                                                        Top2* __restrict__ part)
+                                                        Top2* __restrict__ part, unsigned const* __restrict__ gbits,
+                                                        int ldi)
 {
     __shared__ __attribute__((aligned(16))) double qd[kRecheckGroup][kDim];
     __shared__ __attribute__((aligned(16))) float cand[kRecheckSub * kRecheckRow];
@@ -209,7 +210,17 @@ __global__ __launch_bounds__(256) void k_recheck_chunks(float const* __restrict_
             int const ci = c0 + lane;
             if (ci < nC) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) top2_push(s[j], (float)acc[j], ci);
+                for (int j = 0; j < 4; ++j) {
+                    float v = (float)acc[j];
+                    int const qrow = qidx[4 * wave + j];
+                    if (gbits && qrow >= 0) {
+                        // guided matching: a removed (left, right) pair scores -1e20
+                        int const li = left ? qrow : ci;
+                        int const rj = left ? ci : qrow;
+                        v = ((gbits[(size_t)(rj >> 5) * ldi + li] >> (rj & 31)) & 1u) ? -1e20f : v;
+                    }
+                    top2_push(s[j], v, ci);
+                }
             }
         }
 #pragma unroll
@@ -254,13 +265,13 @@ RecheckPlan plan_recheck(int nC)
 
 void launch_recheck_exact(float const* L, float const* R, int nL, int nR, int const* flagList, int flagOffR,
                           int const* counters, float minScore, float minRatio, int* colMax, int* rowMax,
-                          Top2* part, int grid, hipStream_t st)
+                          Top2* part, int grid, hipStream_t st, unsigned const* gbits, int ldi)
 {
     if (nL <= 0 || nR <= 0) return;
     RecheckPlan const pl = plan_recheck(nR);  // left queries scan the right features
     RecheckPlan const pr = plan_recheck(nL);
     hipLaunchKernelGGL(k_recheck_chunks, dim3(grid), dim3(256), 0, st, L, R, nL, nR, flagList, flagOffR, counters,
-                       pl.nChunks, pl.per, pr.nChunks, pr.per, part);
+                       pl.nChunks, pl.per, pr.nChunks, pr.per, part, gbits, ldi);
     hipLaunchKernelGGL(k_recheck_finish, dim3(256), dim3(256), 0, st, flagList, flagOffR, counters, pl.nChunks,
                        pr.nChunks, part, minScore, minRatio, colMax, rowMax);
 }

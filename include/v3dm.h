@@ -91,8 +91,10 @@ int v3dm_find_putative_matches(v3dm_handle h, float minScore, float minRatio, in
  * distance), cu:554-694.  M is a row-major 3x3 double matrix; positions are AoS (x, y)
  * floats of the current left / right features.  Candidates outside the distance
  * threshold get the score -1e20 (cu:308, cu:340).  As in the reference, the mask applies
- * to the scores of the last findPutativeMatches and repeated guided calls compound.
- * Requires allocate(..., useGuidedMatching = 1). */
+ * to the scores of the last findPutativeMatches and repeated guided calls compound (until
+ * the next findPutativeMatches or set*Features).  The scores are recomputed from the
+ * current descriptors in exact FP32, whatever the precision mode.  Requires
+ * allocate(..., useGuidedMatching = 1): it holds one bit per (left, right) pair. */
 int v3dm_run_guided_F(v3dm_handle h, const double F[9], float distanceThreshold, const float* leftXY,
                       const float* rightXY, float minScore, float minRatio, int32_t* pairs, int cap, int* nPairs);
 int v3dm_run_guided_H(v3dm_handle h, const double H[9], float distanceThreshold, const float* leftXY,
