@@ -2,6 +2,7 @@
 
   libv3dmatch.so  HIP kernels + C ABI (hipcc, --offload-arch=gfx950)
   libv3dsynth.so  synthetic descriptor generator (gcc, host only)
+  v3dm_match_sequence  batched matching of an image sequence (g++, over the C ABI)
 
 Both land in v3dmatch/_lib/ next to the Python host package.  Run as a script or call
 build_all().  Every call recompiles.
@@ -24,7 +25,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 HIP_SOURCES = ["v3dm_fp32.hip", "v3dm_select.hip", "v3dm_short.hip", "v3dm_api.cpp", "v3dm_ext.cpp", "v3dm_batch.cpp",
-               "v3dm_learned.cpp"]
+               "v3dm_learned.cpp", "v3dm_features.cpp"]
 
 
 def _run(cmd):
@@ -67,8 +68,18 @@ def build_hip(jobs=8):
     return out
 
 
+def build_apps():
+    """Host programs over the C ABI (g++, linked against the in-tree libv3dmatch.so)."""
+    out = os.path.join(OUT, "v3dm_match_sequence")
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-I", os.path.join(os.path.dirname(HERE), "include"),
+          os.path.join(HERE, "apps", "match_sequence.cpp"), "-o", out + ".tmp", "-L", OUT, "-lv3dmatch",
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all():
-    return [build_synth(), build_hip()]
+    return [build_synth(), build_hip(), build_apps()]
 
 
 if __name__ == "__main__":

@@ -10,7 +10,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_DIR = os.path.join(_HERE, "_lib")
 
 V3DM_OK = 0
-ERROR_NAMES = {1: "INVALID", 2: "STATE", 3: "CAPACITY", 4: "HIP", 5: "UNSUPPORTED", 6: "NOMEM"}
+ERROR_NAMES = {1: "INVALID", 2: "STATE", 3: "CAPACITY", 4: "HIP", 5: "UNSUPPORTED", 6: "NOMEM", 7: "IO"}
 
 
 class V3DMError(RuntimeError):
@@ -88,6 +88,9 @@ SIGNATURES = {
     "v3dm_learned_train": (_I, [_P, _I, _I, _F, _F, C.c_uint64, _FP]),
     "v3dm_learned_gradients": (_I, [_P, _I, _FP, _F, C.POINTER(C.c_float), _FP, _FP]),
     "v3dm_learned_embed": (_I, [_P, _I, _FP, _FP]),
+    "v3dm_features_count": (_I, [C.c_char_p, _IP, _IP]),
+    "v3dm_features_read": (_I, [C.c_char_p, _I, _FP, _FP, _FP, _FP, _FP, _FP, _IP]),
+    "v3dm_features_write": (_I, [C.c_char_p, _I, _FP, _FP, _FP, _FP, _FP, _FP, _I]),
 }
 
 

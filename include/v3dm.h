@@ -33,7 +33,8 @@ enum {
     V3DM_ERR_CAPACITY = 3,     /* output buffer too small: *nPairs holds the required count */
     V3DM_ERR_HIP = 4,          /* HIP runtime error */
     V3DM_ERR_UNSUPPORTED = 5,  /* feature not available in this build / mode */
-    V3DM_ERR_NOMEM = 6         /* device or host allocation failed */
+    V3DM_ERR_NOMEM = 6,        /* device or host allocation failed */
+    V3DM_ERR_IO = 7            /* file missing, unreadable or malformed */
 };
 
 /* Precision modes of findPutativeMatches (library extension; the reference has FP32 only). */
@@ -151,13 +152,34 @@ int v3dm_get_stats(v3dm_handle h, v3dm_stats* out);
  * none) -- the hardest distractor, used to label training triplets. */
 int v3dm_get_second_best(v3dm_handle h, int32_t* colSecond, int32_t* rowSecond);
 
+/* ---- SIFT feature archives (".features" files) ------------------------------------------
+ * The caller-side input of the matcher: V3D::serializeDataToFile / serializeDataFromFile of
+ * a SerializableVector<SIFT_Feature> (V3D/Base/v3d_serialization.h:953-1017, record
+ * V3D/Base/v3d_feature.h:41-49), as loadFeatures reads them
+ * (V3D/Apps/SfM/match_sequence.cpp:25-46).  Binary ("V3DBIN" + uint32 count + 540 bytes per
+ * feature: int32 id, float x, y, direction x, y, scale, magnitude, 128 descriptor floats,
+ * little-endian) and text ("V3DTXT") archives.  Host only (no handle, no GPU); errors are
+ * reported by v3dm_last_error(NULL).  An unknown tag is V3DM_ERR_IO, as the reference's
+ * "Unknown archive magic tag" (v3d_serialization.h:1000). */
+/* Number of features and the format (1 binary, 0 text) of an archive. */
+int v3dm_features_count(const char* path, int* nFeatures, int* binary);
+/* The first min(count, cap) features; each output may be NULL (field skipped): ids[n],
+ * xy[2n] (x, y), dir[2n], scale[n], magnitude[n], desc[128n] (row-major).  *n = features
+ * read. */
+int v3dm_features_read(const char* path, int cap, int32_t* ids, float* xy, float* dir, float* scale,
+                       float* magnitude, float* desc, int* n);
+/* Writes n features (binary = 1: V3DBIN, 0: V3DTXT with 6 significant digits, as the
+ * reference's text archive); NULL ids / xy / dir / scale / magnitude write k / zeros. */
+int v3dm_features_write(const char* path, int n, const int32_t* ids, const float* xy, const float* dir,
+                        const float* scale, const float* magnitude, const float* desc, int binary);
+
 /* ---- batched matching (extension) ----------------------------------------------------
  * Replaces the caller loop that matches many image pairs one at a time
  * (V3D/Apps/SfM/match_sequence.cpp:165-382, match_using_voctree.cpp:264-609: per pair
  * setLeftFeatures / setRightFeatures / findPutativeMatches, cu:490-552).  The images of a
  * batch are uploaded once; a pair list names (left image, right image) per pair; one call
- * matches every pair with the handle's precision mode (FP32 exact or BF16 shortlist)
- * through persistent kernels over a device work queue, replayed from a captured hipGraph.
+ * matches every pair with the handle's precision mode (FP32 exact, BF16 shortlist or
+ * learned codes) through persistent kernels over a device work queue, replayed from a captured hipGraph.
  * Per pair the result is what findPutativeMatches gives for that pair: counts rounded down
  * to a multiple of 32, matches (left, right) sorted by left index.  Independent of
  * v3dm_allocate. */
