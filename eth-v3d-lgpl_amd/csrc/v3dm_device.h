@@ -97,6 +97,29 @@ __device__ __forceinline__ int decide_certified(Top2 s, float eps, float minScor
     return kFlag;
 }
 
+// IEEE FP32 product / sum / difference that the compiler may not fuse into an FMA, for
+// code that must reproduce the reference's plain float arithmetic.  (__fmul_rn / __fadd_rn
+// are plain operators in this HIP version, so after inlining they can be contracted under
+// the default -ffp-contract=fast-honor-pragmas; the pragma here takes the contract flag off
+// the operation itself.)
+__device__ __forceinline__ float mul_nc(float a, float b)
+{
+#pragma clang fp contract(off)
+    return a * b;
+}
+
+__device__ __forceinline__ float add_nc(float a, float b)
+{
+#pragma clang fp contract(off)
+    return a + b;
+}
+
+__device__ __forceinline__ float sub_nc(float a, float b)
+{
+#pragma clang fp contract(off)
+    return a - b;
+}
+
 // One uint8 SIFT descriptor per wave, lane l holding components 2l and 2l+1, dequantised
 // as SIFT_Feature::normalizeDescriptor (V3D/Base/v3d_feature.h:30-38): the sum of squares
 // of integers <= 255 over 128 terms stays below 2^24, so it is exact in FP32 in any order,
@@ -110,7 +133,7 @@ __device__ __forceinline__ float dequant_u8_denom(uchar2 v)
 }
 
 // element value: the IEEE product (float)q * denom, the same in every kernel that forms it
-__device__ __forceinline__ float dequant_u8_value(unsigned int q, float denom) { return __fmul_rn((float)q, denom); }
+__device__ __forceinline__ float dequant_u8_value(unsigned int q, float denom) { return mul_nc((float)q, denom); }
 
 __device__ __forceinline__ float2 dequant_u8_wave(uchar2 v)
 {

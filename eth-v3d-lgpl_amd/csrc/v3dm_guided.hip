@@ -6,8 +6,8 @@
 // bit is set.  Masks of repeated guided calls are OR-ed into the bitmap, which is what
 // overwriting the same stored matrix again amounts to.
 //
-// Arithmetic: plain IEEE FP32 in the reference's operand order, no contraction (explicit
-// _rn operations), correctly rounded division.
+// Arithmetic: plain IEEE FP32 in the reference's operand order, no contraction (mul_nc /
+// add_nc / sub_nc), correctly rounded division.
 #include "v3dm_device.h"
 #include "v3dm_kernels.h"
 
@@ -19,9 +19,9 @@ struct Mat3f {
     float m[9];
 };
 
-__device__ __forceinline__ float fm(float a, float b) { return __fmul_rn(a, b); }
-__device__ __forceinline__ float fa(float a, float b) { return __fadd_rn(a, b); }
-__device__ __forceinline__ float fs(float a, float b) { return __fsub_rn(a, b); }
+__device__ __forceinline__ float fm(float a, float b) { return mul_nc(a, b); }
+__device__ __forceinline__ float fa(float a, float b) { return add_nc(a, b); }
+__device__ __forceinline__ float fs(float a, float b) { return sub_nc(a, b); }
 
 // Work-group: 256 left features (one per thread) x 256 right features staged in LDS.
 // Epipolar (cu:277-311): Fp = F p per left feature; per right feature q the squares of

@@ -91,6 +91,14 @@ SIGNATURES = {
     "v3dm_features_count": (_I, [C.c_char_p, _IP, _IP]),
     "v3dm_features_read": (_I, [C.c_char_p, _I, _FP, _FP, _FP, _FP, _FP, _FP, _IP]),
     "v3dm_features_write": (_I, [C.c_char_p, _I, _FP, _FP, _FP, _FP, _FP, _FP, _I]),
+    "v3dm_voctree_sizes": (_I, [_I, _I, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "v3dm_voctree_create": (_I, [C.POINTER(_P), _I, _I, _I]),
+    "v3dm_voctree_destroy": (None, [_P]),
+    "v3dm_voctree_set_nodes": (_I, [_P, _FP]),
+    "v3dm_voctree_load": (_I, [_P, C.c_char_p]),
+    "v3dm_voctree_traverse": (_I, [_P, _I, _FP, _FP, _I]),
+    "v3dm_voctree_traverse_device": (_I, [_P, _I, _FP, _FP, _I]),
+    "v3dm_voctree_synchronize": (_I, [_P, _IP]),
 }
 
 

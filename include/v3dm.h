@@ -173,6 +173,39 @@ int v3dm_features_read(const char* path, int cap, int32_t* ids, float* xy, float
 int v3dm_features_write(const char* path, int n, const int32_t* ids, const float* xy, const float* dir,
                         const float* scale, const float* magnitude, const float* desc, int binary);
 
+/* ---- vocabulary-tree quantisation ------------------------------------------------------
+ * VocabularyTreeTraversal<BranchFactor, Depth> (V3D/CUDA/v3d_cudamatching.h:69-105,
+ * cu:705-869) and VocabularyTree<Depth, BranchFactor>::getVisualWord / findBestLeafIndex
+ * (V3D/Base/v3d_voctree.h:19-99, the per-feature loop of
+ * V3D/Apps/SfM/match_using_voctree.cpp:189-190).  Nodes: getNodeCount() x 128 floats,
+ * level after level, the children of node x at (x + 1) * branchFactor (the tree file's
+ * layout).  Per level the descent takes the first child of maximal score, the score being
+ * the sequential FP32 sum of the 128 products (the CPU tree's rule; the CUDA kernel's
+ * summation order is not reproduced).  Errors via v3dm_last_error(NULL). */
+typedef struct v3dm_voctree_s* v3dm_voctree;
+/* getNodeCount / getLeaveCount (h:84-101). */
+int v3dm_voctree_sizes(int branchFactor, int depth, int64_t* nodeCount, int64_t* leafCount);
+/* Constructor + allocate (cu:795-817); the key capacity grows on demand. */
+int v3dm_voctree_create(v3dm_voctree* t, int device, int branchFactor, int depth);
+/* deallocate (cu:819-826) + destructor. */
+void v3dm_voctree_destroy(v3dm_voctree t);
+/* setNodes (cu:828-838): host array of nodeCount x 128 floats. */
+int v3dm_voctree_set_nodes(v3dm_voctree t, const float* nodes);
+/* loadTreeFromFile (v3d_voctree.h:38-51): raw floats; a short file leaves the remaining
+ * nodes zero, as the reference does.  V3DM_ERR_IO if the file cannot be opened. */
+int v3dm_voctree_load(v3dm_voctree t, const char* path);
+/* traverseTree (cu:840-860) for n keys (host arrays, n x 128 floats).  globalIds = 1:
+ * node ids, as the CUDA traversal writes them (cu:784-791); 0: visual words in
+ * [0, leafCount), as getVisualWord returns them (v3d_voctree.h:78).  A key whose descent
+ * leaves the node array (possible only when no child scores above -1e30, e.g. NaN keys,
+ * where the reference reads out of range) gets INT32_MIN. */
+int v3dm_voctree_traverse(v3dm_voctree t, int n, const float* keys, int32_t* leaves, int globalIds);
+/* The same on device-resident keys and output, enqueued on the tree's stream (no host
+ * synchronisation); v3dm_voctree_synchronize waits for it and returns the number of keys
+ * the last call decided by the exact (sequential FP32) path. */
+int v3dm_voctree_traverse_device(v3dm_voctree t, int n, const float* d_keys, int32_t* d_leaves, int globalIds);
+int v3dm_voctree_synchronize(v3dm_voctree t, int* nExact);
+
 /* ---- batched matching (extension) ----------------------------------------------------
  * Replaces the caller loop that matches many image pairs one at a time
  * (V3D/Apps/SfM/match_sequence.cpp:165-382, match_using_voctree.cpp:264-609: per pair

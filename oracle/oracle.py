@@ -32,6 +32,8 @@ def _declare(lib):
         "v3do_mask_epipolar": (None, [vp, i, i, f, vp, vp, vp]),
         "v3do_mask_homography": (None, [vp, i, i, f, vp, vp, vp]),
         "v3do_guided": (i, [vp, i, i, i, vp, f, vp, vp, f, f, vp, i]),
+        "v3do_voctree_leaf": (C.c_int32, [vp, i, i, vp, i]),
+        "v3do_voctree_leaves": (None, [vp, i, i, vp, i, vp, i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -187,6 +189,22 @@ def guided(S, kind, M, thr, left_xy, right_xy, min_score=0.9, min_ratio=0.8):
     n = _load().v3do_guided(S.ctypes.data, nL, nR, kind, M.ctypes.data, thr, lx.ctypes.data, rx.ctypes.data,
                             min_score, min_ratio, out.ctypes.data, cap)
     return out[:n].copy()
+
+
+def voctree_node_count(branch, depth):
+    return sum(branch ** (d + 1) for d in range(depth))
+
+
+def voctree_leaves(tree, branch, depth, keys, global_ids=False):
+    """findBestLeafIndex of every key (v3d_voctree.h:63-99); global_ids: node ids as the
+    GPU traversal writes them (cu:784-791)."""
+    tree = _f32(tree).reshape(-1, 128)
+    assert tree.shape[0] == voctree_node_count(branch, depth)
+    keys = _f32(keys).reshape(-1, 128)
+    out = np.empty(keys.shape[0], np.int32)
+    _load().v3do_voctree_leaves(tree.ctypes.data, branch, depth, keys.ctypes.data, keys.shape[0], out.ctypes.data,
+                                1 if global_ids else 0)
+    return out
 
 
 # ------------------------------------------------------------------ CPU baseline

@@ -315,3 +315,52 @@ int v3do_guided(float* S, int nL, int nR, int kind, double const* M, float dista
     free(colMax); free(rowMax);
     return n;
 }
+
+/* ------------------------------------------------------------------ vocabulary tree */
+
+/*
+ * VocabularyTree<Depth, BranchFactor, 128>::findBestLeafIndex (V3D/Base/v3d_voctree.h:63-99)
+ * on the raw node array of the tree file (v3d_voctree.h:21-51: levels one after the other,
+ * the children of node x at (x + 1) * branch, the root's children at 0).  Per level the
+ * child with the largest score wins, score = sequential FP32 sum of the 128 products
+ * (v3d_voctree.h:90, no contraction), strict '>' against a running maximum that starts at
+ * -1e30 (first index on ties, NaN never wins, no winner = -1).  globalId = 0 returns the
+ * visual word leafId - levelStart[depth - 1] (v3d_voctree.h:78); 1 returns leafId, what
+ * VocabularyTreeTraversal::traverseTree writes (cu:784-791).  A child range outside the
+ * node array (the reference would read past its buffer) returns INT32_MIN.
+ */
+int32_t v3do_voctree_leaf(float const* tree, int branch, int depth, float const* key, int globalId)
+{
+    long long nNodes = 0, len = 1, lastStart = 0;
+    for (int d = 0; d < depth; ++d) {
+        lastStart = nNodes;
+        len *= branch;
+        nNodes += len;
+    }
+    long long leafId = 0, next = 0;
+    for (int d = 0; d < depth; ++d) {
+        if (next < 0 || next + branch > nNodes) return INT32_MIN;
+        float const* nodes = tree + next * D;
+        int best = -1;
+        float maxScore = -1e30f;
+        for (int k = 0; k < branch; ++k) {
+            float const* c = nodes + (size_t)k * D;
+            float score = 0.0f;
+            for (int i = 0; i < D; ++i) score += key[i] * c[i];
+            if (score > maxScore) {
+                best = k;
+                maxScore = score;
+            }
+        }
+        leafId = next + best;
+        next = (leafId + 1) * branch;
+    }
+    return (int32_t)(globalId ? leafId : leafId - lastStart);
+}
+
+void v3do_voctree_leaves(float const* tree, int branch, int depth, float const* keys, int n, int32_t* out,
+                         int globalId)
+{
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int q = 0; q < n; ++q) out[q] = v3do_voctree_leaf(tree, branch, depth, keys + (size_t)q * D, globalId);
+}

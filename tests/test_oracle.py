@@ -265,3 +265,95 @@ def test_guided_homography_closed_form():
             px, py = 2 * lxy[i, 0] + 1, 2 * lxy[i, 1] - 1
             d = (px - rxy[j, 0]) ** 2 + (py - rxy[j, 1]) ** 2
             assert (M[j, i] == np.float32(-1e20)) == (d > 0.5)
+
+
+# ------------------------------------------------------------------ vocabulary tree
+def _tree_levels(branch, depth):
+    starts, s, ln = [], 0, 1
+    for _ in range(depth):
+        starts.append(s)
+        ln *= branch
+        s += ln
+    return starts, s
+
+
+def test_voctree_oracle_follows_planted_paths():
+    # nodes are distinct scaled basis directions; a key built as the sum of the planted
+    # path's directions (weights falling with the level) descends exactly along that path
+    branch, depth = 3, 3
+    starts, n_nodes = _tree_levels(branch, depth)
+    rng = np.random.default_rng(0)
+    dims = rng.permutation(128)
+    tree = np.zeros((n_nodes, 128), np.float32)
+    for x in range(n_nodes):
+        tree[x, dims[x % 128]] = 1.0
+        tree[x, dims[(x * 7 + 3) % 128]] += 0.01 * (x % 5)
+    for leaf in range(branch ** depth):
+        path, x = [], leaf
+        for d in range(depth):
+            path.append(x % branch)
+            x //= branch
+        path = path[::-1]
+        key = np.zeros(128, np.float32)
+        node, nxt = 0, 0
+        for d, k in enumerate(path):
+            node = nxt + k
+            key += np.float32(4.0 ** (depth - d)) * tree[node]
+            nxt = (node + 1) * branch
+        got = oracle.voctree_leaves(tree, branch, depth, key[None], global_ids=False)[0]
+        glob = oracle.voctree_leaves(tree, branch, depth, key[None], global_ids=True)[0]
+        assert glob == node and got == node - starts[-1], (leaf, path, got, glob)
+
+
+def test_voctree_oracle_against_float64_descent():
+    # well-separated random cases: a float64 descent (children at (x + 1) * branch) gives the
+    # same leaves whenever every level's best leads its second by a clear margin
+    rng = np.random.default_rng(1)
+    for branch, depth in [(10, 3), (4, 5), (30, 2)]:
+        starts, n_nodes = _tree_levels(branch, depth)
+        tree = rng.standard_normal((n_nodes, 128)).astype(np.float32)
+        keys = rng.standard_normal((200, 128)).astype(np.float32)
+        got = oracle.voctree_leaves(tree, branch, depth, keys, global_ids=True)
+        checked = 0
+        for q in range(keys.shape[0]):
+            nxt, node, clear = 0, 0, True
+            for d in range(depth):
+                s = tree[nxt:nxt + branch].astype(np.float64) @ keys[q].astype(np.float64)
+                o = np.sort(s)
+                clear &= (o[-1] - o[-2]) > 1e-3
+                node = nxt + int(np.argmax(s))
+                nxt = (node + 1) * branch
+            if clear:
+                assert got[q] == node
+                checked += 1
+        assert checked > 150
+
+
+def test_voctree_oracle_ties_nan_and_floor():
+    branch, depth = 4, 2
+    starts, n_nodes = _tree_levels(branch, depth)
+    tree = np.zeros((n_nodes, 128), np.float32)
+    tree[:, 0] = 1.0  # every child identical: ties everywhere -> first child at each level
+    key = np.zeros(128, np.float32)
+    key[0] = 1.0
+    assert oracle.voctree_leaves(tree, branch, depth, key[None], True)[0] == (0 + 1) * branch + 0
+    # NaN key: no child ever beats -1e30 -> leafId = next - 1 with next = 0 at every level
+    nan = np.full((1, 128), np.nan, np.float32)
+    assert oracle.voctree_leaves(tree, branch, depth, nan, True)[0] == -1
+    assert oracle.voctree_leaves(tree, branch, depth, nan, False)[0] == -1 - starts[-1]
+    # scores at or below the -1e30 floor never win either
+    far = np.zeros((1, 128), np.float32)
+    far[0, 0] = -1e35
+    assert oracle.voctree_leaves(tree, branch, depth, far, True)[0] == -1
+
+
+def test_voctree_oracle_sums_sequentially_in_fp32():
+    # child 0 scores 2^24 + 1 - 2^24: 0 in sequential FP32 (2^24 + 1 rounds to 2^24), 1
+    # exactly; child 1 scores 0.5 either way.  The reference's float loop picks child 1.
+    branch, depth = 2, 1
+    tree = np.zeros((2, 128), np.float32)
+    tree[0, :3] = [2.0 ** 24, 1.0, -(2.0 ** 24)]
+    tree[1, 0] = 0.5
+    key = np.zeros((1, 128), np.float32)
+    key[0, :3] = 1.0
+    assert oracle.voctree_leaves(tree, branch, depth, key, True)[0] == 1
