@@ -4,9 +4,15 @@ Workload: one image pair of 16384 left x 16384 right SIFT-like descriptors (128-
 synthetic: Gamma(0.5,1) components, 0.2 clip, uint8 quantisation, 40% noisy copies),
 minScore 0.9, minRatio 0.8.  A step is one findPutativeMatches on the pair with the
 descriptors resident in HBM and the match list left in HBM.  The plain FP32 form and the
-BF16 and FP8 shortlist forms are timed; the plain form's match set is checked against the
-exact CPU oracle once per run, and the shortlist forms' recall and precision are measured
-against it.  The headline is the BF16 form when its recall is at least 99.5%.
+BF16, FP8 and learned-code shortlist forms are timed; the plain form's match set is
+checked against the exact CPU oracle once per run, and the shortlist forms' recall and
+precision are measured against it.  The headline is the fastest of the learned and BF16
+forms whose recall and precision are at least 99.5%.
+
+Also reported (not the headline): the 256-pair uneven batch, guided matching on the same
+pair, and vocabulary-tree quantisation of 10^6 keys on a <50, 3> tree.  The HBM traffic of
+the headline kernel comes from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in
+profiles/traffic.json (null when that file has no entry for the kernel).
 
 Prints one JSON line (see the keys below).  --gpus above 1 is refused: this build runs
 on one GPU.
@@ -38,19 +44,24 @@ def log(msg):
 
 
 def time_form(m, steps, warmup):
+    """ms per step over `steps` back-to-back calls (no event recording in this loop), then
+    the same number of calls again with HIP events around the dominant kernel on the
+    handle's stream: its average launch time."""
     for _ in range(warmup):
         m.find_async(MIN_SCORE, MIN_RATIO)
-    m.synchronize()
-    m.set_profiling(True)
     m.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         m.find_async(MIN_SCORE, MIN_RATIO)
     m.synchronize()
     t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / steps
+    m.set_profiling(True)
+    for _ in range(steps):
+        m.find_async(MIN_SCORE, MIN_RATIO)
+    m.synchronize()
     st = m.stats()
     m.set_profiling(False)
-    ms_step = (t1 - t0) * 1e3 / steps
     kernel_ms = st["kernelMs"] / max(st["kernelLaunches"], 1)  # per launch of the dominant kernel
     return ms_step, kernel_ms, st
 
@@ -140,6 +151,136 @@ def bench_batch(steps, warmup, n_pairs=256):
     return res
 
 
+class _Hip:
+    """Device buffers from the HIP runtime the library is linked with."""
+
+    def __init__(self):
+        import ctypes as C
+        self.C = C
+        self.rt = C.CDLL("libamdhip64.so")
+        self.rt.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.rt.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.rt.hipFree.argtypes = [C.c_void_p]
+
+    def upload(self, a):
+        p = self.C.c_void_p()
+        assert self.rt.hipMalloc(self.C.byref(p), a.nbytes) == 0
+        assert self.rt.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0
+        return p.value
+
+    def alloc(self, nbytes):
+        p = self.C.c_void_p()
+        assert self.rt.hipMalloc(self.C.byref(p), nbytes) == 0
+        return p.value
+
+    def download(self, a, p):
+        assert self.rt.hipMemcpy(a.ctypes.data, p, a.nbytes, 2) == 0
+
+    def free(self, p):
+        self.rt.hipFree(p)
+
+
+def synthetic_tree(branch, depth, seed=21, spread=0.35):
+    """k-means-like vocabulary tree: every node's children perturb its direction."""
+    rng = np.random.default_rng(seed)
+    n_nodes = sum(branch ** (d + 1) for d in range(depth))
+    tree = np.zeros((n_nodes, 128), np.float32)
+    unit = lambda x: (x / np.linalg.norm(x, axis=-1, keepdims=True)).astype(np.float32)  # noqa: E731
+    tree[:branch] = unit(np.abs(rng.standard_normal((branch, 128))))
+    start, ln = 0, branch
+    for _ in range(1, depth):
+        par = tree[start:start + ln]
+        kids = par[:, None, :] + spread * np.abs(rng.standard_normal((ln, branch, 128))) / np.sqrt(128)
+        c0 = (start + 1) * branch
+        tree[c0:c0 + ln * branch] = unit(kids).reshape(-1, 128)
+        start, ln = start + ln, ln * branch
+    leaves = tree[n_nodes - branch ** depth:]
+    return tree, leaves, unit
+
+
+def bench_voctree(steps, warmup, cpu, branch=50, depth=3, n=1_000_000):
+    """Vocabulary-tree quantisation (VocabularyTree::getVisualWord for every key): n keys
+    near the leaves of a synthetic <branch, depth> tree, keys and words in HBM."""
+    from v3dmatch import voctree
+    tree, leaves, unit = synthetic_tree(branch, depth)
+    rng = np.random.default_rng(22)
+    keys = unit(np.abs(leaves[rng.integers(0, leaves.shape[0], n)] + 0.08 * rng.standard_normal((n, 128))))
+    hip = _Hip()
+    dk, dw = hip.upload(keys), hip.alloc(4 * n)
+    res = {"workload": f"{n} keys, <{branch},{depth}> tree ({tree.shape[0]} nodes), keys near leaves"}
+    with voctree.VocabularyTree(branch, depth) as t:
+        t.setNodes(tree)
+        for _ in range(warmup):
+            t.traverse_device(n, dk, dw)
+        t.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            t.traverse_device(n, dk, dw)
+        exact = t.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        words = np.empty(n, np.int32)
+        hip.download(words, dw)
+    hip.free(dk)
+    hip.free(dw)
+    flop = 2.0 * n * branch * depth * 128
+    res.update({"ms_per_run": dt * 1e3, "keys_per_s": n / dt, "exact_keys": int(exact),
+                "tflops": flop / dt / 1e12, "frac_fp32_mfma": flop / dt / PEAK_FP32_MFMA})
+    if cpu:
+        from oracle import oracle
+        s = 20000
+        t0 = time.perf_counter()
+        want = oracle.voctree_leaves(tree, branch, depth, keys[:s])
+        cdt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"keys_per_s": s / cdt, "cores": oracle.cpu_max_threads(), "kind": "port",
+                               "sample": f"first {s} keys, oracle descent (sequential FP32 scores), OpenMP"}
+        res["words_identical_to_oracle_on_sample"] = bool(np.array_equal(words[:s], want))
+    return res
+
+
+def bench_guided(L, R, steps, warmup):
+    """Guided matching (runGuidedHomographyMatching) on the single pair: copies sit at H p +
+    noise, distractors anywhere; each call uploads the positions, masks, rescores the pair
+    exactly and returns the match list to the host."""
+    import v3dmatch
+    from v3dmatch import synth
+    _, _, truth = synth.make_pair(N_FEAT, N_FEAT, 0.4, config=2, pair=0)
+    rng = np.random.default_rng(5)
+    H = np.array([[1.02, 0.03, 12.0], [-0.02, 0.99, -7.0], [1e-5, -2e-5, 1.0]])
+    lxy = np.stack([rng.uniform(0, 1024, N_FEAT), rng.uniform(0, 768, N_FEAT)], 1)
+    rxy = np.stack([rng.uniform(0, 1024, N_FEAT), rng.uniform(0, 768, N_FEAT)], 1)
+    cp = truth >= 0
+    hp = (H @ np.c_[lxy[truth[cp]], np.ones(cp.sum())].T).T
+    rxy[cp] = hp[:, :2] / hp[:, 2:3] + rng.standard_normal((cp.sum(), 2))
+    lxy, rxy = lxy.astype(np.float32), rxy.astype(np.float32)
+    m = v3dmatch.SIFT_Matching(0)
+    m.enableRoundingWarning(False)
+    m.allocate(N_FEAT, N_FEAT, True)
+    m.setLeftFeatures(N_FEAT, L)
+    m.setRightFeatures(N_FEAT, R)
+    put = m.find_putative_matches_array(MIN_SCORE, MIN_RATIO)
+    for _ in range(warmup):
+        m.runGuidedHomographyMatching(H, 3.0, lxy, rxy, MIN_SCORE, MIN_RATIO)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g = m.runGuidedHomographyMatching(H, 3.0, lxy, rxy, MIN_SCORE, MIN_RATIO)
+    dt = (time.perf_counter() - t0) / steps
+    m.close()
+    return {"workload": f"{N_FEAT}x{N_FEAT} pair, homography mask, 3 px", "ms_per_call_host_inclusive": dt * 1e3,
+            "calls_per_s": 1.0 / dt, "putative_matches": int(len(put)), "guided_matches": len(g)}
+
+
+def traffic_for(kernel):
+    """HBM bytes per launch of `kernel` from profiles/traffic.json (rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes scaled by the copy calibration), or None."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(p) as f:
+            t = json.load(f)
+        return t["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +290,7 @@ def main():
     ap.add_argument("--no-oracle", action="store_true", help="skip the oracle match-set check")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch measurement")
     ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--no-extras", action="store_true", help="skip the guided and vocabulary-tree measurements")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -206,8 +348,9 @@ def main():
     from v3dmatch import learned
     if os.path.exists(learned.WEIGHTS_PATH):
         learned.Network(m).load(learned.WEIGHTS_PATH)
-        time_shortlist(v3dmatch.PREC_LEARNED, "learned_shortlist", "k_pass_code",
-                       2.0 * N_FEAT * N_FEAT * 32, "int8", PEAK_INT8_MFMA)
+        # one k_pass_code2 launch covers both directions
+        time_shortlist(v3dmatch.PREC_LEARNED, "learned_shortlist", "k_pass_code2",
+                       2 * 2.0 * N_FEAT * N_FEAT * 32, "int8", PEAK_INT8_MFMA)
     forms["plain_fp32"].update({"flop_per_launch": flop, "dtype": "fp32", "peak": PEAK_FP32_MFMA})
 
     # headline: the fastest of the learned and BF16 shortlist forms that keeps >= 99.5%
@@ -236,7 +379,7 @@ def main():
                    "n_left": N_FEAT, "n_right": N_FEAT, "form": head},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TOP/s" if hf["dtype"] == "int8" else "TFLOP/s", "frac": achieved / peak,
-                     "traffic": None, "kernel": hf["kernel"], "kernel_ms": hf["kernel_ms"],
+                     "traffic": traffic_for(hf["kernel"]), "kernel": hf["kernel"], "kernel_ms": hf["kernel_ms"],
                      "flop_per_launch": hf["flop_per_launch"]},
         "forms": forms,
     }
@@ -258,6 +401,11 @@ def main():
     m.close()
     if not args.no_batch:
         out["batch"] = bench_batch(args.batch_steps, 2)
+    if not args.no_extras:
+        log("bench: guided matching")
+        out["guided"] = bench_guided(L, R, 10, 2)
+        log("bench: vocabulary tree")
+        out["voctree"] = bench_voctree(10, 2, not args.no_cpu)
     print(json.dumps(out), flush=True)
     return 0
 

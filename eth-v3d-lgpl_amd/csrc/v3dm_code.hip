@@ -28,11 +28,17 @@ struct KeyStateI {
 };
 
 // One 32x32 block of keys (16 per lane) into the running top-2; blk = block index.
+// The block's accumulators come straight from the MFMA.  An instruction the compiler emits
+// itself gets the wait states that a VALU read of an MFMA result needs; inline assembly
+// does not.  So a compiler-emitted v_readfirstlane of the first accumulator (after its
+// wait, every register of the MFMA result is complete) feeds each v_med3_i32 below as an
+// operand, which orders every such read after it.
 template <bool kFull>
 __device__ __forceinline__ void key_block_update(KeyStateI& s, v16i const& d, int blk, int jBase, int nC, int h)
 {
     int const o1 = s.k1, o2 = s.k2;
     int k1 = o1, k2 = o2;
+    int const ready = __builtin_amdgcn_readfirstlane(d[0]);
 #pragma unroll
     for (int r = 0; r < 16; r += 2) {
         int x = d[r], y = d[r + 1];
@@ -43,9 +49,9 @@ __device__ __forceinline__ void key_block_update(KeyStateI& s, v16i const& d, in
         }
         // t = med3(x, y, k1), the second largest of {k1, x, y}; written as an instruction so
         // that the max(x, y) inside the generic pattern is not shared with the k1 update,
-        // which then folds into one v_max3_i32
+        // which then folds into one v_max3_i32 (`ready`: the ordering operand above)
         int t;
-        __asm__("v_med3_i32 %0, %1, %2, %3" : "=v"(t) : "v"(x), "v"(y), "v"(k1));
+        __asm__("v_med3_i32 %0, %1, %2, %3" : "=v"(t) : "v"(x), "v"(y), "v"(k1), "s"(ready));
         k1 = max(max(k1, x), y);
         k2 = max(k2, t);  // two consecutive pairs fold into v_max3_i32
     }
@@ -145,6 +151,26 @@ __global__ __launch_bounds__(256, 4) void k_pass_code(signed char const* __restr
     code_block(Q, nQ, C, nC, blockIdx.x, blockIdx.y, tilesPerSplit, part, ldq);
 }
 
+// Both directions of a pair in one launch: work-groups [0, queryBlocks0 * splits0) run
+// direction 0 (query block fastest), the rest direction 1.
+struct CodeDir {
+    signed char const* Q;
+    signed char const* C;
+    Cand2* part;
+    int nQ, nC, tilesPerSplit, ldq, queryBlocks, splits;
+};
+
+__global__ __launch_bounds__(256, 4) void k_pass_code2(CodeDir d0, CodeDir d1)
+{
+    int b = blockIdx.x;
+    int const n0 = d0.queryBlocks * d0.splits;
+    bool const second = b >= n0;  // uniform per work-group
+    CodeDir const d = second ? d1 : d0;
+    if (second) b -= n0;
+    int const split = b / d.queryBlocks;
+    code_block(d.Q, d.nQ, d.C, d.nC, b - split * d.queryBlocks, split, d.tilesPerSplit, d.part, d.ldq);
+}
+
 // Persistent form over a batch: item = (segment, query block, split, tiles per split).
 __global__ __launch_bounds__(256, 4) void k_batch_pass_code(BatchDev b, int4 const* __restrict__ items, int nItems,
                                                             int* __restrict__ queue, Cand2* __restrict__ part)
@@ -169,6 +195,16 @@ void launch_batch_pass_code(BatchDev const& b, int4 const* items, int nItems, in
 {
     if (nItems <= 0) return;
     hipLaunchKernelGGL(k_batch_pass_code, dim3(grid), dim3(256), 0, st, b, items, nItems, queue, part);
+}
+
+void launch_pass_code2(signed char const* A, int nA, signed char const* B, int nB, ShortPlan const& pa,
+                       Cand2* partA, int ldA, ShortPlan const& pb, Cand2* partB, int ldB, hipStream_t st)
+{
+    if (nA <= 0 || nB <= 0) return;
+    CodeDir const d0{A, B, partA, nA, nB, pa.tilesPerSplit, ldA, pa.queryBlocks, pa.splits};
+    CodeDir const d1{B, A, partB, nB, nA, pb.tilesPerSplit, ldB, pb.queryBlocks, pb.splits};
+    hipLaunchKernelGGL(k_pass_code2, dim3(pa.queryBlocks * pa.splits + pb.queryBlocks * pb.splits), dim3(256), 0, st,
+                       d0, d1);
 }
 
 void launch_pass_code(signed char const* Q, int nQ, signed char const* C, int nC, ShortPlan const& p, Cand2* part,

@@ -21,7 +21,7 @@ struct v3dm_context {
     bool allocated = false;
     int precision = V3DM_PREC_FP32;
     float epsRel = v3dm::kEpsRelFp32;  // 0: every query goes to the exact re-check
-    int learnedTopK = 8;               // candidates per query rescored in the learned mode
+    int learnedTopK = 6;               // candidates per query rescored in the learned mode
     int fallbackSamples = v3dm::kSampleRows;  // sampled check of learned results (0: off)
     float fallbackRecall = 0.995f;     // sampled recall below this -> the plain form is run
     float lastMinScore = 0.9f, lastMinRatio = 0.8f;

@@ -31,7 +31,6 @@ struct ExtState {
     float* sNorm = nullptr;
     int* sIdx = nullptr;
     Top2* sColPart = nullptr;  // kMaxSplits x kSampleRows
-    Top2* sRowPart = nullptr;  // capR (written by the fused kernel, not used)
     int* sDec = nullptr;       // exact decisions of the samples (+1 unused right slot)
     int* sFlags = nullptr;     // open-query list of the samples
     Top2* sRpart = nullptr;    // re-check partials
@@ -39,7 +38,7 @@ struct ExtState {
     int* sOut = nullptr;       // accepted, missed, wrongly accepted
     int* sHost = nullptr;      // pinned copy of sOut
     hipStream_t side = nullptr;  // the check runs beside the learned passes
-    hipEvent_t evFork = nullptr, evJoin = nullptr;
+    hipEvent_t evFork = nullptr, evJoin = nullptr, evMain = nullptr;
     bool sPending = false;
     int sSamples = 0;
     Cand2* partL = nullptr;  // kShortMaxSplits x capL x 2
@@ -144,7 +143,6 @@ int ensure_sample(v3dm_context* h, ExtState* e)
     EXTCHECK(h, hipMalloc(&e->sNorm, kSampleRows * sizeof(float)));
     EXTCHECK(h, hipMalloc(&e->sIdx, kSampleRows * sizeof(int)));
     EXTCHECK(h, hipMalloc(&e->sColPart, (size_t)kSampleMaxSplits * kSampleRows * sizeof(Top2)));
-    EXTCHECK(h, hipMalloc(&e->sRowPart, (size_t)e->capR * sizeof(Top2)));
     EXTCHECK(h, hipMalloc(&e->sDec, (kSampleRows + 1) * sizeof(int)));
     EXTCHECK(h, hipMalloc(&e->sFlags, (kSampleRows + 1) * sizeof(int)));
     EXTCHECK(h, hipMalloc(&e->sRpart, (size_t)(kSampleRows + 1) * kRecheckMaxChunks * sizeof(Top2)));
@@ -154,12 +152,15 @@ int ensure_sample(v3dm_context* h, ExtState* e)
     EXTCHECK(h, hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking));
     EXTCHECK(h, hipEventCreateWithFlags(&e->evFork, hipEventDisableTiming));
     EXTCHECK(h, hipEventCreateWithFlags(&e->evJoin, hipEventDisableTiming));
+    EXTCHECK(h, hipEventCreateWithFlags(&e->evMain, hipEventDisableTiming));
     return V3DM_OK;
 }
 
 // The plain exact path on S sampled left features against all right features, on a side
-// stream from the fork point (it needs only the features); the comparison with the learned
-// decisions joins the handle's stream and lands in sOut (read back with the result).
+// stream from the fork point (it needs only the features).  The comparison with the learned
+// decisions also runs on the side stream, after the handle's stream has decided (evMain),
+// while the handle's stream runs the mutual check; the handle's stream waits for the side
+// stream at the end of the call, so later work on it is ordered after the check.
 int enqueue_sample_fork(v3dm_context* h, ExtState* e, float minScore, float minRatio)
 {
     int rc = ensure_sample(h, e);
@@ -169,27 +170,33 @@ int enqueue_sample_fork(v3dm_context* h, ExtState* e, float minScore, float minR
     hipStream_t const st = e->side;
     int const nL = h->nL, nR = h->nR;
     int const S = std::min(std::min(h->fallbackSamples, kSampleRows), nL);
-    launch_sample_gather(h->dL, h->normL, nL, S, e->sL, e->sNorm, e->sIdx, st);
+    launch_sample_gather(h->dL, h->normL, nL, S, e->sL, e->sNorm, e->sIdx, e->sCounters, st);
     FusedPlan const p = plan_fused_fp32(S, nR, kSampleMaxSplits, kSampleMaxSplits);
-    launch_fused_fp32(e->sL, h->dR, S, nR, p, e->sColPart, kSampleRows, e->sRowPart, e->capR, st);
-    EXTCHECK(h, hipMemsetAsync(e->sCounters, 0, kNumCounters * sizeof(int), st));
-    launch_merge_decide(e->sColPart, p.splits, kSampleRows, e->sRowPart, p.leftBlocks, e->capR, S, 0, e->sNorm,
-                        h->normR, h->maxNormBits, h->epsRel, minScore, minRatio, e->sDec, e->sDec + kSampleRows,
-                        e->sFlags, kSampleRows, e->sCounters, st);
+    launch_fused_fp32(e->sL, h->dR, S, nR, p, e->sColPart, kSampleRows, nullptr, 0, st);  // left decisions only
+    launch_sample_decide(e->sColPart, p.splits, kSampleRows, S, e->sNorm, h->maxNormBits, h->epsRel, minScore,
+                         minRatio, e->sDec, e->sFlags, e->sCounters, st);
     launch_recheck_exact(e->sL, h->dR, S, nR, e->sFlags, kSampleRows, e->sCounters, minScore, minRatio, e->sDec,
                          e->sDec + kSampleRows, e->sRpart, kRecheckGrid, st);
-    EXTCHECK(h, hipEventRecord(e->evJoin, st));
     e->sSamples = S;
     return V3DM_OK;
 }
 
+// after the learned decisions: the comparison on the side stream
+int enqueue_sample_compare(v3dm_context* h, ExtState* e)
+{
+    EXTCHECK(h, hipEventRecord(e->evMain, h->stream));
+    EXTCHECK(h, hipStreamWaitEvent(e->side, e->evMain, 0));
+    launch_sample_compare(e->sDec, e->sIdx, e->sSamples, h->colMax, e->sOut, e->side);
+    EXTCHECK(h, hipMemcpyAsync(e->sHost, e->sOut, 4 * sizeof(int), hipMemcpyDeviceToHost, e->side));
+    EXTCHECK(h, hipEventRecord(e->evJoin, e->side));
+    e->sPending = true;
+    return V3DM_OK;
+}
+
+// end of the call: the handle's stream waits for the side stream
 int enqueue_sample_join(v3dm_context* h, ExtState* e)
 {
-    hipStream_t const st = h->stream;
-    EXTCHECK(h, hipStreamWaitEvent(st, e->evJoin, 0));
-    launch_sample_compare(e->sDec, e->sIdx, e->sSamples, h->colMax, e->sOut, st);
-    EXTCHECK(h, hipMemcpyAsync(e->sHost, e->sOut, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
-    e->sPending = true;
+    EXTCHECK(h, hipStreamWaitEvent(h->stream, e->evJoin, 0));
     return V3DM_OK;
 }
 
@@ -210,8 +217,8 @@ int enqueue_learned(v3dm_context* h, float minScore, float minRatio)
         // embedding: network forward on the FP32 rows (read up to the next multiple of 128,
         // inside the allocation) with the quantisation fused; code rows past n are never
         // used (candidates past n are masked, queries past n not written)
-        if (!e->cLValid) launch_mlp_codes_f32(h->dL, (nL + 127) / 128 * 128, *w, scale, e->cL, st);
-        if (!e->cRValid) launch_mlp_codes_f32(h->dR, (nR + 127) / 128 * 128, *w, scale, e->cR, st);
+        launch_mlp_codes2_f32(h->dL, e->cLValid ? 0 : (nL + 127) / 128 * 128, e->cL, h->dR,
+                              e->cRValid ? 0 : (nR + 127) / 128 * 128, e->cR, *w, scale, st);
         e->cLValid = e->cRValid = true;
         e->sPending = false;
         bool const check = h->fallbackSamples > 0;
@@ -219,20 +226,23 @@ int enqueue_learned(v3dm_context* h, float minScore, float minRatio)
         // 1024 work-groups: the code pass runs four per CU
         ShortPlan const pl = plan_short(nL, nR, 1024);
         ShortPlan const pr = plan_short(nR, nL, 1024);
+        // both directions in one launch (the profiled "dominant kernel" covers both)
         if ((rc = prof_begin(h))) return rc;
-        launch_pass_code(e->cL, nL, e->cR, nR, pl, e->partL, e->capL, st);
-        if ((rc = prof_end(h))) return rc;
-        if ((rc = prof_begin(h))) return rc;
-        launch_pass_code(e->cR, nR, e->cL, nL, pr, e->partR, e->capR, st);
+        launch_pass_code2(e->cL, nL, e->cR, nR, pl, e->partL, e->capL, pr, e->partR, e->capR, st);
         if ((rc = prof_end(h))) return rc;
         launch_rescore(h->dL, h->dR, nL, nR, e->partL, pl.splits, e->capL, e->partR, pr.splits, e->capR, h->normL,
                        h->normR, h->maxNormBits, 0.0f, h->learnedTopK, minScore, minRatio,
                        h->colMax, h->rowMax, e->secL, e->secR, h->stripes, st);
+        if (check && (rc = enqueue_sample_compare(h, e))) return rc;
+        e->haveSecond = true;
+        launch_mutual_compact(h->colMax, h->rowMax, nL, nR, h->mtmp, h->mblock, h->stripes,
+                              reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st);
         if (check && (rc = enqueue_sample_join(h, e))) return rc;
+    } else {
+        e->haveSecond = false;
+        launch_mutual_compact(h->colMax, h->rowMax, 0, nR, h->mtmp, h->mblock, h->stripes,
+                              reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st);
     }
-    e->haveSecond = work;
-    launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mtmp, h->mblock, h->stripes,
-                          reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st);
     EXTCHECK(h, hipGetLastError());
     return V3DM_OK;
 }
@@ -347,18 +357,19 @@ void ext_release(ExtState* e)
         if (p) (void)hipFree(p);
     e->cL = e->cR = nullptr;
     e->cLValid = e->cRValid = false;
-    for (void* p : {(void*)e->sL, (void*)e->sNorm, (void*)e->sIdx, (void*)e->sColPart, (void*)e->sRowPart,
+    for (void* p : {(void*)e->sL, (void*)e->sNorm, (void*)e->sIdx, (void*)e->sColPart,
                     (void*)e->sDec, (void*)e->sFlags, (void*)e->sRpart, (void*)e->sCounters, (void*)e->sOut})
         if (p) (void)hipFree(p);
     if (e->sHost) (void)hipHostFree(e->sHost);
     if (e->side) (void)hipStreamDestroy(e->side);
     if (e->evFork) (void)hipEventDestroy(e->evFork);
     if (e->evJoin) (void)hipEventDestroy(e->evJoin);
+    if (e->evMain) (void)hipEventDestroy(e->evMain);
     e->side = nullptr;
-    e->evFork = e->evJoin = nullptr;
+    e->evFork = e->evJoin = e->evMain = nullptr;
     e->sL = e->sNorm = nullptr;
     e->sIdx = e->sDec = e->sFlags = e->sCounters = e->sOut = e->sHost = nullptr;
-    e->sColPart = e->sRowPart = e->sRpart = nullptr;
+    e->sColPart = e->sRpart = nullptr;
     e->sPending = false;
     e->f8L = e->f8R = e->eL = e->eR = nullptr;
     e->f8LValid = e->f8RValid = false;

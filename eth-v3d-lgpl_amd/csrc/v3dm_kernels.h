@@ -13,6 +13,7 @@ constexpr int kCntFlag = 1;       // counters[1]: open left queries of the runni
 constexpr int kCntFlagDone = 2;   // counters[2]: queries re-checked exactly by the last call
 constexpr int kCntRescoredDone = 4;  // counters[4]: ... by the last call
 constexpr int kCntFlagR = 5;      // counters[5]: open right queries of the running call
+constexpr int kCntTicket = 6;     // counters[6]: work-groups done in the mutual-check kernel (0 between calls)
 constexpr int kNumCounters = 8;
 // Counts summed by many work-groups go to kStripes slots one cache line apart (atomics on a
 // single address serialise in L2); the gather kernel sums and clears them.
@@ -75,8 +76,13 @@ void launch_guided_fp32(float const* L, float const* R, int nL, int nR, FusedPla
                         Top2* rowPart, int ldr, unsigned const* bits, int ldi, hipStream_t st);
 
 // sampled exact check of a shortlist result (v3dm_select.hip)
+// (also zeroes the check's counters)
 void launch_sample_gather(float const* L, float const* normL, int nL, int S, float* Ls, float* normS, int* idx,
-                          hipStream_t st);
+                          int* counters, hipStream_t st);
+// merge + certified decision of the S sampled rows (maxNormBits[1]: max right norm)
+void launch_sample_decide(Top2 const* colPart, int nParts, int ldc, int S, float const* normS,
+                          unsigned int const* maxNormBits, float epsRel, float minScore, float minRatio, int* dec,
+                          int* flagList, int* counters, hipStream_t st);
 void launch_sample_compare(int const* exactS, int const* idx, int S, int const* colMax, int* out, hipStream_t st);
 
 }  // namespace v3dm

@@ -516,10 +516,12 @@ __global__ __launch_bounds__(256) void k_mlp_adam(float const* __restrict__ part
 //     element order: a fixed permutation of the code dimensions, the same for every row).
 constexpr int kW1Stride = 136;  // bf16 per LDS row of W1 (128 + 8)
 
-__global__ __launch_bounds__(256, 2) void k_mlp_codes(float const* __restrict__ X, int nBlocks,
+__global__ __launch_bounds__(256, 2) void k_mlp_codes(float const* __restrict__ X0, int nBlocks0,
+                                                      signed char* __restrict__ codes0, float const* __restrict__ X1,
+                                                      int nBlocks1, signed char* __restrict__ codes1,
                                                       __bf16 const* __restrict__ W1b, float const* __restrict__ b1,
                                                       __bf16 const* __restrict__ W2p, float const* __restrict__ b2,
-                                                      float scale, signed char* __restrict__ codes)
+                                                      float scale)
 {
     __shared__ __attribute__((aligned(16))) __bf16 W1s[kHid * kW1Stride];
     __shared__ __attribute__((aligned(16))) float b1s[kHid];
@@ -540,8 +542,12 @@ __global__ __launch_bounds__(256, 2) void k_mlp_codes(float const* __restrict__ 
 #pragma unroll
     for (int r = 0; r < 16; ++r) bz[r] = b2[8 * (r >> 2) + 4 * h + (r & 3)];
 
-    for (int blk = blockIdx.x; blk < nBlocks; blk += gridDim.x) {
-        int const row = blk * 128 + wave * 32 + c32;
+    // row blocks of the first feature set, then of the second
+    for (int blk = blockIdx.x; blk < nBlocks0 + nBlocks1; blk += gridDim.x) {
+        bool const second = blk >= nBlocks0;
+        float const* __restrict__ X = second ? X1 : X0;
+        signed char* __restrict__ codes = second ? codes1 : codes0;
+        int const row = (second ? blk - nBlocks0 : blk) * 128 + wave * 32 + c32;
         v8bf xb[8];
         {
             float4 const* xp = reinterpret_cast<float4 const*>(X + (size_t)row * kDim);
@@ -598,17 +604,23 @@ __global__ __launch_bounds__(256, 2) void k_mlp_codes(float const* __restrict__ 
     }
 }
 
-void launch_mlp_codes_f32(float const* X, int rows128, MlpWeights const& w, float scale, signed char* codes,
-                          hipStream_t st)
+void launch_mlp_codes2_f32(float const* X0, int rows0, signed char* codes0, float const* X1, int rows1,
+                           signed char* codes1, MlpWeights const& w, float scale, hipStream_t st)
 {
-    int const nBlocks = rows128 / 128;
-    if (nBlocks <= 0) return;
+    int const nb0 = rows0 / 128, nb1 = rows1 / 128;
+    if (nb0 + nb1 <= 0) return;
     int dev = 0, cus = 256;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    int const grid = nBlocks < 2 * cus ? nBlocks : 2 * cus;
-    hipLaunchKernelGGL(k_mlp_codes, dim3(grid), dim3(256), 0, st, X, nBlocks, w.W1b, w.P + kOffB1, w.W2p,
-                       w.P + kOffB2, scale, codes);
+    int const grid = nb0 + nb1 < 2 * cus ? nb0 + nb1 : 2 * cus;
+    hipLaunchKernelGGL(k_mlp_codes, dim3(grid), dim3(256), 0, st, X0, nb0, codes0, X1, nb1, codes1, w.W1b,
+                       w.P + kOffB1, w.W2p, w.P + kOffB2, scale);
+}
+
+void launch_mlp_codes_f32(float const* X, int rows128, MlpWeights const& w, float scale, signed char* codes,
+                          hipStream_t st)
+{
+    launch_mlp_codes2_f32(X, rows128, codes, nullptr, 0, nullptr, w, scale, st);
 }
 
 __global__ void k_fill_f32(float* __restrict__ p, size_t n, float v)

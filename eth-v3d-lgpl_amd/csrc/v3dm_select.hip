@@ -283,11 +283,19 @@ void launch_recheck_exact(float const* L, float const* R, int nL, int nR, int co
 // forms the exclusive prefix of the block counts and copies the slices into place, so
 // the output is sorted by k.  Pass 2 also publishes the re-check count and clears the
 // counter for the next call.
-__global__ __launch_bounds__(256) void k_mutual_local(int const* __restrict__ colMax, int const* __restrict__ rowMax,
-                                                      int nL, int nR, int2* __restrict__ tmp,
-                                                      int* __restrict__ blockCount)
+// One launch: every work-group takes 1024 consecutive left features (4 per thread),
+// compacts its matches in ascending k into its own slice of tmp and writes its count; the
+// last work-group to finish (atomic ticket after a device-scope fence) forms the exclusive
+// prefix of the counts and copies the slices into place, so the output is sorted by k.  It
+// also publishes the re-check and rescoring counts and clears the per-call counters.
+__global__ __launch_bounds__(256) void k_mutual(int const* __restrict__ colMax, int const* __restrict__ rowMax,
+                                                int nL, int nR, int2* __restrict__ tmp, int* __restrict__ blockCount,
+                                                int* __restrict__ stripes, int2* __restrict__ pairs,
+                                                int* __restrict__ counters)
 {
     __shared__ int wsum[4];
+    __shared__ int isLast;
+    __shared__ int offs[1025];  // up to 2^20 left features: 1024 slices
     int const tid = threadIdx.x;
     int const lane = tid & 63;
     int const wave = tid >> 6;
@@ -317,33 +325,36 @@ __global__ __launch_bounds__(256) void k_mutual_local(int const* __restrict__ co
     for (int u = 0; u < 4; ++u)
         if (m[u]) tmp[pos++] = make_int2(i0 + u, l[u]);
     if (tid == 0) blockCount[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-}
-
-__global__ __launch_bounds__(1024) void k_mutual_gather(int2 const* __restrict__ tmp,
-                                                        int const* __restrict__ blockCount, int nBlocks,
-                                                        int* __restrict__ stripes, int2* __restrict__ pairs,
-                                                        int* __restrict__ counters)
-{
-    __shared__ int offs[1025];
-    int const tid = threadIdx.x;
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) isLast = (atomicAdd(&counters[kCntTicket], 1) == (int)gridDim.x - 1);
+    __syncthreads();
+    if (!isLast) return;
+    __threadfence();
+    int const nBlocks = gridDim.x;
     int rescored = 0;
     if (tid < kStripes) {
         rescored = stripes[tid * kStripeStride];
         stripes[tid * kStripeStride] = 0;
     }
     for (int o = 32; o > 0; o >>= 1) rescored += __shfl_xor(rescored, o);  // wave 0 holds the stripes
+    for (int b = tid; b < nBlocks; b += 256) offs[b + 1] = blockCount[b];  // loads in parallel
+    __syncthreads();
     if (tid == 0) {
-        int a = 0;
-        for (int b = 0; b < nBlocks; ++b) {
-            offs[b] = a;
-            a += blockCount[b];
-        }
-        offs[nBlocks] = a;
+        offs[0] = 0;
+        for (int b = 0; b < nBlocks; ++b) offs[b + 1] += offs[b];
     }
     __syncthreads();
-    for (int b = 0; b < nBlocks; ++b) {
-        int const n = offs[b + 1] - offs[b];
-        for (int t = tid; t < n; t += 1024) pairs[offs[b] + t] = tmp[(size_t)b * 1024 + t];
+    // every output entry independently: its slice by binary search over the offsets
+    int const total = offs[nBlocks];
+    for (int i = tid; i < total; i += 256) {
+        int lo = 0, hi = nBlocks;  // offs[lo] <= i < offs[hi]
+        while (hi - lo > 1) {
+            int const mid = (lo + hi) >> 1;
+            if (offs[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        pairs[i] = tmp[(size_t)lo * 1024 + (i - offs[lo])];
     }
     if (tid == 0) {
         counters[kCntPairs] = offs[nBlocks];
@@ -351,6 +362,7 @@ __global__ __launch_bounds__(1024) void k_mutual_gather(int2 const* __restrict__
         counters[kCntFlag] = 0;
         counters[kCntFlagR] = 0;
         counters[kCntRescoredDone] = rescored;
+        counters[kCntTicket] = 0;
     }
 }
 
@@ -358,9 +370,8 @@ void launch_mutual_compact(int const* colMax, int const* rowMax, int nL, int nR,
                            int* stripes, int2* pairs, int* counters, hipStream_t st)
 {
     int const nb = (nL + 1023) / 1024;
-    if (nb > 0)
-        hipLaunchKernelGGL(k_mutual_local, dim3(nb), dim3(256), 0, st, colMax, rowMax, nL, nR, tmp, blockCount);
-    hipLaunchKernelGGL(k_mutual_gather, dim3(1), dim3(1024), 0, st, tmp, blockCount, nb, stripes, pairs, counters);
+    hipLaunchKernelGGL(k_mutual, dim3(nb > 0 ? nb : 1), dim3(256), 0, st, colMax, rowMax, nL, nR, tmp, blockCount,
+                       stripes, pairs, counters);
 }
 
 // ------------------------------------------------------------------------------ uint8 dequantisation
@@ -387,8 +398,10 @@ void launch_dequant_u8(uint8_t const* q, int n, float* out, hipStream_t st)
 // shortlist decision differs from it (missed), and those the shortlist accepts wrongly.
 __global__ __launch_bounds__(256) void k_sample_gather(float const* __restrict__ L, float const* __restrict__ normL,
                                                        int nL, int S, float* __restrict__ Ls,
-                                                       float* __restrict__ normS, int* __restrict__ idx)
+                                                       float* __restrict__ normS, int* __restrict__ idx,
+                                                       int* __restrict__ counters)
 {
+    if (blockIdx.x == 0 && threadIdx.x < kNumCounters) counters[threadIdx.x] = 0;  // the check's own counters
     int const lane = threadIdx.x & 63;
     int const k = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (k >= kSampleRows) return;
@@ -428,9 +441,40 @@ __global__ __launch_bounds__(128) void k_sample_compare(int const* __restrict__ 
 }
 
 void launch_sample_gather(float const* L, float const* normL, int nL, int S, float* Ls, float* normS, int* idx,
-                          hipStream_t st)
+                          int* counters, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_sample_gather, dim3(kSampleRows / 4), dim3(256), 0, st, L, normL, nL, S, Ls, normS, idx);
+    hipLaunchKernelGGL(k_sample_gather, dim3(kSampleRows / 4), dim3(256), 0, st, L, normL, nL, S, Ls, normS, idx,
+                       counters);
+}
+
+// The sampled rows' decisions: one wave per sampled left feature merges its split
+// partials (lane l takes every 64th), then the certified decision as in k_merge_decide.
+__global__ __launch_bounds__(256) void k_sample_decide(Top2 const* __restrict__ colPart, int nParts, int ldc, int S,
+                                                       float const* __restrict__ normS,
+                                                       unsigned int const* __restrict__ maxNormBits, float epsRel,
+                                                       float minScore, float minRatio, int* __restrict__ dec,
+                                                       int* __restrict__ flagList, int* __restrict__ counters)
+{
+    int const lane = threadIdx.x & 63;
+    int const q = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= S) return;  // uniform per wave
+    Top2 s = top2_init();
+    for (int p = lane; p < nParts; p += 64) s = top2_merge(s, colPart[(size_t)p * ldc + q]);
+    for (int o = 32; o > 0; o >>= 1) s = top2_merge(s, top2_shfl_xor(s, o));
+    if (lane != 0) return;
+    float const eps = epsRel * normS[q] * __uint_as_float(maxNormBits[1]) + kEpsAbs;
+    int const d = (epsRel > 0.0f) ? decide_certified(s, eps, minScore, minRatio) : kFlag;
+    if (d == kFlag) flagList[atomicAdd(&counters[kCntFlag], 1)] = q;
+    dec[q] = d;
+}
+
+void launch_sample_decide(Top2 const* colPart, int nParts, int ldc, int S, float const* normS,
+                          unsigned int const* maxNormBits, float epsRel, float minScore, float minRatio, int* dec,
+                          int* flagList, int* counters, hipStream_t st)
+{
+    if (S <= 0) return;
+    hipLaunchKernelGGL(k_sample_decide, dim3((S + 3) / 4), dim3(256), 0, st, colPart, nParts, ldc, S, normS,
+                       maxNormBits, epsRel, minScore, minRatio, dec, flagList, counters);
 }
 
 void launch_sample_compare(int const* exactS, int const* idx, int S, int const* colMax, int* out, hipStream_t st)

@@ -50,5 +50,8 @@ void launch_rescore(float const* L, float const* R, int nL, int nR, Cand2 const*
 // learned code pass (v3dm_code.hip)
 void launch_pass_code(signed char const* Q, int nQ, signed char const* C, int nC, ShortPlan const& p, Cand2* part,
                       int ldq, hipStream_t st);
+// both directions in one launch: A as queries against B (plan pa -> partA), B against A
+void launch_pass_code2(signed char const* A, int nA, signed char const* B, int nB, ShortPlan const& pa,
+                       Cand2* partA, int ldA, ShortPlan const& pb, Cand2* partB, int ldB, hipStream_t st);
 
 }  // namespace v3dm

@@ -42,7 +42,6 @@ struct v3dm_voctree_s {
     int* exactList = nullptr;
     int32_t* dOut = nullptr;
     int* cnt = nullptr;
-    int* offs = nullptr;
     int* cursor = nullptr;
     int* counters = nullptr;
     int lastExact = 0;
@@ -120,8 +119,7 @@ int enqueueTraverse(v3dm_voctree_s* t, int n, float const* dKeys, int32_t* dLeav
             lv.parentStart = t->levelStarts[d - 1];
             lv.nParents = (int)(t->levelStarts[d] - t->levelStarts[d - 1]);
             long long const oc = orderCapacity(n, lv.nParents) - 32;
-            launch_vt_bucket(t->node, n, lv, t->cnt, t->offs, t->cursor, t->order, (int)oc, t->exactList, t->counters,
-                             st);
+            launch_vt_bucket(t->node, n, lv, t->cnt, t->cursor, t->order, (int)oc, t->exactList, t->counters, st);
             launch_vt_score(dKeys, t->keyNorm, n, t->dNodes, t->maxNodeNorm, lv, t->order, false,
                             (int)((oc + 31) / 32), t->node, t->exactList, t->counters, st);
         }
@@ -178,9 +176,8 @@ int v3dm_voctree_create(v3dm_voctree* out, int device, int branchFactor, int dep
     hipError_t e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&t->dNodes, (size_t)(nodes + 32) * kDim * sizeof(float));
     if (e == hipSuccess) e = hipMemset(t->dNodes, 0, (size_t)(nodes + 32) * kDim * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&t->cnt, (size_t)t->maxParents * sizeof(int));
-    if (e == hipSuccess) e = hipMalloc(&t->offs, (size_t)t->maxParents * sizeof(int));
-    if (e == hipSuccess) e = hipMalloc(&t->cursor, (size_t)t->maxParents * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&t->cnt, (size_t)t->maxParents * kVtBinStride * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&t->cursor, (size_t)t->maxParents * kVtBinStride * sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&t->counters, kVtCounters * sizeof(int));
     if (e != hipSuccess) {
         v3dm_voctree_destroy(t);
@@ -199,7 +196,6 @@ void v3dm_voctree_destroy(v3dm_voctree t)
     releaseKeys(t);
     vfree(t->dNodes);
     vfree(t->cnt);
-    vfree(t->offs);
     vfree(t->cursor);
     vfree(t->counters);
     if (t->stream) (void)hipStreamDestroy(t->stream);

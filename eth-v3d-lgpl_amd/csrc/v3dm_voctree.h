@@ -16,6 +16,8 @@ constexpr int kVtCntExact = 1;   // counters[1]: keys decided exactly at the run
 constexpr int kVtCntFlagged = 2; // counters[2]: keys decided exactly over the whole call
 constexpr int kVtCounters = 4;
 constexpr int kVtNone = INT32_MIN;  // child range outside the node array
+constexpr int kVtBinStride = 32;    // ints between bucket counters (one 128-byte line each)
+constexpr int kVtLdsBins = 16384;   // parent levels up to this many nodes are counted in LDS
 
 struct VtLevel {
     int branch;
@@ -29,8 +31,9 @@ struct VtLevel {
 void launch_vt_reset(int* counters, bool all, hipStream_t st);
 // bucket keys by their parent node (level >= 1); keys whose parent is not a node of level
 // - 1 go straight to the exact list
-void launch_vt_bucket(int const* node, int n, VtLevel const& lv, int* cnt, int* offs, int* cursor, int* order,
-                      int orderCap, int* exactList, int* counters, hipStream_t st);
+// cnt, cursor: nParents * kVtBinStride ints
+void launch_vt_bucket(int const* node, int n, VtLevel const& lv, int* cnt, int* cursor, int* order, int orderCap,
+                      int* exactList, int* counters, hipStream_t st);
 // certified MFMA scores of every key against the children of its parent; settled keys get
 // their child, the others go to the exact list
 void launch_vt_score(float const* keys, float const* keyNorm, int n, float const* nodes, float maxNodeNorm,

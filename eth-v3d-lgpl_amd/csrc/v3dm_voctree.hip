@@ -39,30 +39,102 @@ __global__ __launch_bounds__(256) void k_vt_zero(int* a, int n, int value)
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) a[i] = value;
 }
 
+// Bucket counts live kVtBinStride ints apart (one L2 line per bucket): the global atomics
+// of different buckets then never share a line.  For up to kVtLdsBins parents every
+// work-group first counts its kVtKeysPerGroup keys in LDS and adds one total per bucket it
+// saw; the scatter reserves each bucket's range for the work-group with one atomic and
+// places the keys by their LDS rank.  Wider parent levels use one global atomic per key.
+constexpr int kVtKeysPerThread = 16;
+constexpr int kVtKeysPerGroup = 256 * kVtKeysPerThread;
+
+__device__ __forceinline__ int vt_bucket(int const* __restrict__ node, int k, long long parentStart, int nParents,
+                                         int* __restrict__ exactList, int* __restrict__ counters)
+{
+    int const nd = node[k];
+    if (nd == kVtNone) return -1;
+    long long const b = (long long)nd - parentStart;
+    if (b >= 0 && b < nParents) return (int)b;
+    exactList[atomicAdd(&counters[kVtCntExact], 1)] = k;  // not a node of the parent level
+    return -1;
+}
+
+__global__ __launch_bounds__(256) void k_vt_count_lds(int const* __restrict__ node, int n, long long parentStart,
+                                                      int nParents, int* __restrict__ cnt,
+                                                      int* __restrict__ exactList, int* __restrict__ counters)
+{
+    extern __shared__ int hist[];
+    int const tid = threadIdx.x;
+    for (int b = tid; b < nParents; b += 256) hist[b] = 0;
+    __syncthreads();
+    int const k0 = blockIdx.x * kVtKeysPerGroup;
+    for (int i = 0; i < kVtKeysPerThread; ++i) {
+        int const k = k0 + i * 256 + tid;
+        if (k >= n) break;
+        int const b = vt_bucket(node, k, parentStart, nParents, exactList, counters);
+        if (b >= 0) atomicAdd(&hist[b], 1);
+    }
+    __syncthreads();
+    for (int b = tid; b < nParents; b += 256) {
+        int const c = hist[b];
+        if (c) atomicAdd(&cnt[(size_t)b * kVtBinStride], c);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_vt_scatter_lds(int const* __restrict__ node, int n, long long parentStart,
+                                                        int nParents, int* __restrict__ cursor,
+                                                        int* __restrict__ order)
+{
+    extern __shared__ int hist[];
+    int const tid = threadIdx.x;
+    for (int b = tid; b < nParents; b += 256) hist[b] = 0;
+    __syncthreads();
+    int const k0 = blockIdx.x * kVtKeysPerGroup;
+    int bin[kVtKeysPerThread], rank[kVtKeysPerThread];
+#pragma unroll
+    for (int i = 0; i < kVtKeysPerThread; ++i) {
+        int const k = k0 + i * 256 + tid;
+        bin[i] = -1;
+        if (k < n) {
+            int const nd = node[k];
+            long long const b = (long long)nd - parentStart;
+            if (nd != kVtNone && b >= 0 && b < nParents) {
+                bin[i] = (int)b;
+                rank[i] = atomicAdd(&hist[b], 1);
+            }
+        }
+    }
+    __syncthreads();
+    for (int b = tid; b < nParents; b += 256) {
+        int const c = hist[b];
+        if (c) hist[b] = atomicAdd(&cursor[(size_t)b * kVtBinStride], c);  // the group's range in the bucket
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kVtKeysPerThread; ++i)
+        if (bin[i] >= 0) order[hist[bin[i]] + rank[i]] = k0 + i * 256 + tid;
+}
+
 __global__ __launch_bounds__(256) void k_vt_count(int const* __restrict__ node, int n, long long parentStart,
                                                   int nParents, int* __restrict__ cnt, int* __restrict__ exactList,
                                                   int* __restrict__ counters)
 {
     int const k = blockIdx.x * 256 + threadIdx.x;
     if (k >= n) return;
-    int const nd = node[k];
-    if (nd == kVtNone) return;
-    long long const b = (long long)nd - parentStart;
-    if (b >= 0 && b < nParents) atomicAdd(&cnt[b], 1);
-    else exactList[atomicAdd(&counters[kVtCntExact], 1)] = k;  // not a node of the parent level
+    int const b = vt_bucket(node, k, parentStart, nParents, exactList, counters);
+    if (b >= 0) atomicAdd(&cnt[(size_t)b * kVtBinStride], 1);
 }
 
 // One work-group: exclusive prefix of the bucket sizes rounded up to 32 (so that no slot of
 // 32 keys spans two buckets), slot count for the score kernel.
-__global__ __launch_bounds__(1024) void k_vt_scan(int const* __restrict__ cnt, int nParents, int* __restrict__ offs,
-                                                  int* __restrict__ cursor, int* __restrict__ counters)
+__global__ __launch_bounds__(1024) void k_vt_scan(int const* __restrict__ cnt, int nParents, int* __restrict__ cursor,
+                                                  int* __restrict__ counters)
 {
     __shared__ int part[1024];
     int const t = threadIdx.x;
     int const per = (nParents + 1023) / 1024;
     int const b0 = t * per, b1 = min(b0 + per, nParents);
     int s = 0;
-    for (int b = b0; b < b1; ++b) s += (cnt[b] + 31) & ~31;
+    for (int b = b0; b < b1; ++b) s += (cnt[(size_t)b * kVtBinStride] + 31) & ~31;
     part[t] = s;
     __syncthreads();
     for (int o = 1; o < 1024; o <<= 1) {  // Hillis-Steele inclusive scan
@@ -73,9 +145,8 @@ __global__ __launch_bounds__(1024) void k_vt_scan(int const* __restrict__ cnt, i
     }
     int run = part[t] - s;
     for (int b = b0; b < b1; ++b) {
-        offs[b] = run;
-        cursor[b] = run;
-        run += (cnt[b] + 31) & ~31;
+        cursor[(size_t)b * kVtBinStride] = run;
+        run += (cnt[(size_t)b * kVtBinStride] + 31) & ~31;
     }
     if (t == 1023) counters[kVtCntSlots] = part[1023] / 32;
 }
@@ -88,21 +159,32 @@ __global__ __launch_bounds__(256) void k_vt_scatter(int const* __restrict__ node
     int const nd = node[k];
     if (nd == kVtNone) return;
     long long const b = (long long)nd - parentStart;
-    if (b >= 0 && b < nParents) order[atomicAdd(&cursor[b], 1)] = k;
+    if (b >= 0 && b < nParents) order[atomicAdd(&cursor[(size_t)b * kVtBinStride], 1)] = k;
 }
 
-void launch_vt_bucket(int const* node, int n, VtLevel const& lv, int* cnt, int* offs, int* cursor, int* order,
-                      int orderCap, int* exactList, int* counters, hipStream_t st)
+void launch_vt_bucket(int const* node, int n, VtLevel const& lv, int* cnt, int* cursor, int* order, int orderCap,
+                      int* exactList, int* counters, hipStream_t st)
 {
-    int const g = (n + 255) / 256;
-    hipLaunchKernelGGL(k_vt_zero, dim3(std::min((lv.nParents + 255) / 256, 4096)), dim3(256), 0, st, cnt,
-                       lv.nParents, 0);
+    int const binInts = lv.nParents * kVtBinStride;
+    hipLaunchKernelGGL(k_vt_zero, dim3(std::min((binInts + 255) / 256, 4096)), dim3(256), 0, st, cnt, binInts, 0);
     hipLaunchKernelGGL(k_vt_zero, dim3(std::min((orderCap + 255) / 256, 4096)), dim3(256), 0, st, order, orderCap,
                        -1);
-    hipLaunchKernelGGL(k_vt_count, dim3(g), dim3(256), 0, st, node, n, lv.parentStart, lv.nParents, cnt, exactList,
-                       counters);
-    hipLaunchKernelGGL(k_vt_scan, dim3(1), dim3(1024), 0, st, cnt, lv.nParents, offs, cursor, counters);
-    hipLaunchKernelGGL(k_vt_scatter, dim3(g), dim3(256), 0, st, node, n, lv.parentStart, lv.nParents, cursor, order);
+    if (lv.nParents <= kVtLdsBins) {
+        int const g = (n + kVtKeysPerGroup - 1) / kVtKeysPerGroup;
+        size_t const lds = (size_t)lv.nParents * sizeof(int);
+        hipLaunchKernelGGL(k_vt_count_lds, dim3(g), dim3(256), lds, st, node, n, lv.parentStart, lv.nParents, cnt,
+                           exactList, counters);
+        hipLaunchKernelGGL(k_vt_scan, dim3(1), dim3(1024), 0, st, cnt, lv.nParents, cursor, counters);
+        hipLaunchKernelGGL(k_vt_scatter_lds, dim3(g), dim3(256), lds, st, node, n, lv.parentStart, lv.nParents,
+                           cursor, order);
+    } else {
+        int const g = (n + 255) / 256;
+        hipLaunchKernelGGL(k_vt_count, dim3(g), dim3(256), 0, st, node, n, lv.parentStart, lv.nParents, cnt,
+                           exactList, counters);
+        hipLaunchKernelGGL(k_vt_scan, dim3(1), dim3(1024), 0, st, cnt, lv.nParents, cursor, counters);
+        hipLaunchKernelGGL(k_vt_scatter, dim3(g), dim3(256), 0, st, node, n, lv.parentStart, lv.nParents, cursor,
+                           order);
+    }
 }
 
 // ------------------------------------------------------------------------------ certified scores
@@ -197,6 +279,7 @@ __global__ __launch_bounds__(256) void k_vt_exact(float const* __restrict__ keys
                                                   int const* __restrict__ exactList, int* __restrict__ node,
                                                   int* __restrict__ counters)
 {
+    __shared__ float keyRow[4][kDim];
     int const lane = threadIdx.x & 63;
     int const cnt = counters[kVtCntExact];
     for (int e = blockIdx.x * 4 + (threadIdx.x >> 6); e < cnt; e += gridDim.x * 4) {
@@ -206,13 +289,31 @@ __global__ __launch_bounds__(256) void k_vt_exact(float const* __restrict__ keys
             if (lane == 0) node[kk] = kVtNone;
             continue;
         }
-        float const* key = keys + (size_t)kk * kDim;
+        // the key in LDS (one row per wave), each child row in registers 32 floats at a
+        // time: the loads are issued together instead of one per step of the sum
+        float* kw = keyRow[threadIdx.x >> 6];
+        __builtin_amdgcn_wave_barrier();  // the previous key's reads are done
+        reinterpret_cast<float2*>(kw)[lane] = reinterpret_cast<float2 const*>(keys + (size_t)kk * kDim)[lane];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         float bestS = -INFINITY;
         int bestJ = -1;
         for (int j = lane; j < branch; j += 64) {
-            float const* c = nodes + (size_t)(c0 + j) * kDim;
+            float4 const* c = reinterpret_cast<float4 const*>(nodes + (size_t)(c0 + j) * kDim);
             float score = 0.0f;
-            for (int i = 0; i < kDim; ++i) score = add_nc(score, mul_nc(key[i], c[i]));
+            for (int q0 = 0; q0 < 32; q0 += 8) {
+                float4 v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = c[q0 + u];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    float const* kq = kw + 4 * (q0 + u);
+                    score = add_nc(score, mul_nc(kq[0], v[u].x));
+                    score = add_nc(score, mul_nc(kq[1], v[u].y));
+                    score = add_nc(score, mul_nc(kq[2], v[u].z));
+                    score = add_nc(score, mul_nc(kq[3], v[u].w));
+                }
+            }
             if (score > -1e30f && score > bestS) {
                 bestS = score;
                 bestJ = j;

@@ -111,7 +111,7 @@ int v3dm_set_precision(v3dm_handle h, int mode);
  * query (the whole selection then runs on exact FP64 scores). */
 int v3dm_set_certification(v3dm_handle h, float epsRel);
 /* Candidates per query scored exactly in the learned mode (the best k by code score;
- * default 8, at most 32). */
+ * default 6, at most 32). */
 int v3dm_set_learned_topk(v3dm_handle h, int k);
 /* 1 (default): the bf16 / fp8 / code copies of the features are kept between calls while
  * the features are unchanged; 0: every call derives them again (so that a call's time

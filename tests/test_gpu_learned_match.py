@@ -35,13 +35,20 @@ def matcher():
     m.close()
 
 
-def _recall(m, L, R, want):
+def _recall(m, L, R, want, samples=0):
+    """Recall / precision of the learned form itself: the sampled check (and with it the
+    fallback to the plain form, which would hide a defect of the learned path) is off
+    unless samples > 0."""
     m.setLeftFeatures(L.shape[0], L)
     m.setRightFeatures(R.shape[0], R)
     m.set_precision(v3dmatch.PREC_LEARNED)
+    m.set_fallback(samples, 0.995)
     got = _pairs_set(m.find_putative_matches_array(0.9, 0.8))
+    st = m.stats()
+    m.set_fallback(128, 0.995)
+    assert samples > 0 or st["nFallback"] == 0
     inter = len(got & want)
-    return inter / max(len(want), 1), inter / max(len(got), 1), m.stats()
+    return inter / max(len(want), 1), inter / max(len(got), 1), st
 
 
 @pytest.mark.parametrize("n,seed", [(1024, 3), (4096, 0), (4096, 7)])
@@ -117,7 +124,9 @@ def test_learned_batch(matcher):
     m.set_precision(v3dmatch.PREC_FP32)
     want = m.batch_match(0.9, 0.8)
     m.set_precision(v3dmatch.PREC_LEARNED)
+    m.set_fallback(0, 0.995)  # the learned batch itself, without plain reruns
     got = m.batch_match(0.9, 0.8)
+    m.set_fallback(128, 0.995)
     ws = set((p, a, b) for p in range(len(left)) for a, b in want[p].tolist())
     gs = set((p, a, b) for p in range(len(left)) for a, b in got[p].tolist())
     inter = len(ws & gs)
