@@ -6,8 +6,10 @@
 // accumulator initialised to r (the element index 0..15 of the D block) gives, for lane l
 // and element r, key = 16 * sum_k q_c q_q + r: the score with the candidate's row inside
 // the block in its low 4 bits (row 8*(r/4) + 4*(l/32) + r%4, query l%32).  Keys order as
-// (score, row), so a top-2 over keys is three integer ops per two elements (max, med3,
-// max3); the block of each kept key is tracked once per block.
+// (score, row), so the best of a set of candidates is a chain of v_max3_i32.  Each lane
+// keeps the best key of two disjoint candidate groups (elements r < 8 and r >= 8 of every
+// block), so a partition (split, lane half) lists two candidates, one per group; the block
+// of each kept key is tracked once per block.
 #include "v3dm_batch.h"
 #include "v3dm_device.h"
 #include "v3dm_learned.h"
@@ -23,44 +25,32 @@ constexpr int kCT = 64;    // candidates per tile (2 blocks of 32)
 constexpr int kCodeBytes = 32;
 
 struct KeyStateI {
-    int k1, k2;
-    int b1, b2;  // block (32-candidate) of k1 / k2
+    int k[2];    // best key of the lane's two candidate groups (elements r < 8, r >= 8)
+    int blk[2];  // block (32-candidate) of each best key
 };
 
-// One 32x32 block of keys (16 per lane) into the running top-2; blk = block index.
-// The block's accumulators come straight from the MFMA.  An instruction the compiler emits
-// itself gets the wait states that a VALU read of an MFMA result needs; inline assembly
-// does not.  So a compiler-emitted v_readfirstlane of the first accumulator (after its
-// wait, every register of the MFMA result is complete) feeds each v_med3_i32 below as an
-// operand, which orders every such read after it.
+// One 32x32 block of keys (16 per lane) into the two running maxima; blk = block index.
 template <bool kFull>
 __device__ __forceinline__ void key_block_update(KeyStateI& s, v16i const& d, int blk, int jBase, int nC, int h)
 {
-    int const o1 = s.k1, o2 = s.k2;
-    int k1 = o1, k2 = o2;
-    int const ready = __builtin_amdgcn_readfirstlane(d[0]);
 #pragma unroll
-    for (int r = 0; r < 16; r += 2) {
-        int x = d[r], y = d[r + 1];
-        if (!kFull) {
-            int const jx = jBase + 8 * (r >> 2) + 4 * h + (r & 3);
-            x = (jx < nC) ? x : INT_MIN;
-            y = (jx + 1 < nC) ? y : INT_MIN;
+    for (int g = 0; g < 2; ++g) {
+        int const o = s.k[g];
+        int x[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            int const rr = 8 * g + r;
+            x[r] = d[rr];
+            if (!kFull) x[r] = (jBase + 8 * (rr >> 2) + 4 * h + (rr & 3) < nC) ? x[r] : INT_MIN;
         }
-        // t = med3(x, y, k1), the second largest of {k1, x, y}; written as an instruction so
-        // that the max(x, y) inside the generic pattern is not shared with the k1 update,
-        // which then folds into one v_max3_i32 (`ready`: the ordering operand above)
-        int t;
-        __asm__("v_med3_i32 %0, %1, %2, %3" : "=v"(t) : "v"(x), "v"(y), "v"(k1), "s"(ready));
-        k1 = max(max(k1, x), y);
-        k2 = max(k2, t);  // two consecutive pairs fold into v_max3_i32
+        // a tree of v_max3_i32 (the keys order by score, then row)
+        int const a = max(max(x[0], x[1]), x[2]);
+        int const b = max(max(x[3], x[4]), x[5]);
+        int const c = max(max(x[6], x[7]), o);
+        int const k = max(max(a, b), c);
+        s.blk[g] = (k != o) ? blk : s.blk[g];
+        s.k[g] = k;
     }
-    bool const c1 = k1 != o1;
-    bool const c2 = k2 != o2;
-    s.b2 = (c1 && k2 == o1) ? s.b1 : (c2 ? blk : s.b2);
-    s.b1 = c1 ? blk : s.b1;
-    s.k1 = k1;
-    s.k2 = k2;
 }
 
 __device__ __forceinline__ void load_code_tile(signed char const* __restrict__ C, int t, int h, int c32, v4i (&a)[2])
@@ -70,22 +60,20 @@ __device__ __forceinline__ void load_code_tile(signed char const* __restrict__ C
         a[cb] = *reinterpret_cast<v4i const*>(C + (size_t)(t * kCT + cb * 32 + c32) * kCodeBytes + 16 * h);
 }
 
-// The four 32x32 blocks of a tile (2 candidate blocks x 2 query blocks): all four MFMAs
-// issue before the first update, so each update finds its block done.
+// The four 32x32 blocks of a tile (2 candidate blocks x 2 query blocks), one candidate
+// block at a time (two accumulators live: fewer VGPRs, six waves per SIMD).
 template <bool kFull>
 __device__ __forceinline__ void code_tile(v4i const (&a)[2], int t, v4i const (&bq)[2], v16i const& pos,
                                           KeyStateI (&st)[2], int nC, int h)
 {
-    v16i d[2][2];
-#pragma unroll
-    for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-        for (int qb = 0; qb < 2; ++qb) d[cb][qb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[cb], bq[qb], pos, 0, 0, 0);
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb) {
+        v16i d[2];
+#pragma unroll
+        for (int qb = 0; qb < 2; ++qb) d[qb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[cb], bq[qb], pos, 0, 0, 0);
         int const blk = 2 * t + cb;
 #pragma unroll
-        for (int qb = 0; qb < 2; ++qb) key_block_update<kFull>(st[qb], d[cb][qb], blk, blk * 32, nC, h);
+        for (int qb = 0; qb < 2; ++qb) key_block_update<kFull>(st[qb], d[qb], blk, blk * 32, nC, h);
     }
 }
 
@@ -113,7 +101,7 @@ __device__ __forceinline__ void code_block(signed char const* __restrict__ Q, in
     for (int r = 0; r < 16; ++r) pos[r] = r;
     KeyStateI st[2];
 #pragma unroll
-    for (int qb = 0; qb < 2; ++qb) st[qb] = KeyStateI{INT_MIN, INT_MIN, -1, -1};
+    for (int qb = 0; qb < 2; ++qb) st[qb] = KeyStateI{{INT_MIN, INT_MIN}, {-1, -1}};
 
     int const tFull = (t1 * kCT > nC) ? t1 - 1 : t1;  // the last tile may be partial
     // the next tile's A operands are loaded while the current tile is processed (the index
@@ -132,14 +120,15 @@ __device__ __forceinline__ void code_block(signed char const* __restrict__ Q, in
     for (int qb = 0; qb < 2; ++qb) {
         int const q = q0 + qb * 32 + c32;
         if (q >= nQ) continue;
+        // the list holds the best candidate of each group (two disjoint candidate sets)
         Cand2 c;
-        int const r1 = st[qb].k1 & 15, r2 = st[qb].k2 & 15;
-        bool const ok1 = st[qb].b1 >= 0 && st[qb].k1 != INT_MIN;
-        bool const ok2 = st[qb].b2 >= 0 && st[qb].k2 != INT_MIN;
-        c.v1 = ok1 ? (float)(st[qb].k1 >> 4) : -INFINITY;
-        c.v2 = ok2 ? (float)(st[qb].k2 >> 4) : -INFINITY;
-        c.j1 = ok1 ? st[qb].b1 * 32 + 8 * (r1 >> 2) + 4 * h + (r1 & 3) : -1;
-        c.j2 = ok2 ? st[qb].b2 * 32 + 8 * (r2 >> 2) + 4 * h + (r2 & 3) : -1;
+        int const r1 = st[qb].k[0] & 15, r2 = st[qb].k[1] & 15;
+        bool const ok1 = st[qb].blk[0] >= 0 && st[qb].k[0] != INT_MIN;
+        bool const ok2 = st[qb].blk[1] >= 0 && st[qb].k[1] != INT_MIN;
+        c.v1 = ok1 ? (float)(st[qb].k[0] >> 4) : -INFINITY;
+        c.v2 = ok2 ? (float)(st[qb].k[1] >> 4) : -INFINITY;
+        c.j1 = ok1 ? st[qb].blk[0] * 32 + 8 * (r1 >> 2) + 4 * h + (r1 & 3) : -1;
+        c.j2 = ok2 ? st[qb].blk[1] * 32 + 8 * (r2 >> 2) + 4 * h + (r2 & 3) : -1;
         part[((size_t)split * ldq + q) * 2 + h] = c;
     }
 }
