@@ -584,6 +584,49 @@ int v3dm_set_profiling(v3dm_handle h, int enable)
     return V3DM_OK;
 }
 
+int v3dm_time_dominant_kernel(v3dm_handle h, int reps, float* msPerLaunch)
+{
+    if (!h || !msPerLaunch || reps < 1) return fail(h, V3DM_ERR_INVALID, "bad argument");
+    if (!h->haveResult) return fail(h, V3DM_ERR_STATE, "no result: call findPutativeMatches first");
+    HIPCHECK(h, hipSetDevice(h->device));
+    int const rcf = ext_finish(h);
+    if (rcf) return rcf;
+    hipEvent_t a = nullptr, b = nullptr;
+    HIPCHECK(h, hipEventCreate(&a));
+    hipError_t e = hipEventCreate(&b);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return fail(h, V3DM_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(e));
+    }
+    int launches = 0, rc = V3DM_OK;
+    if (h->precision == V3DM_PREC_FP32) {
+        if (h->nL > 0 && h->nR > 0) {
+            FusedPlan const p = plan_fused_fp32(h->nL, h->nR, 512);
+            e = hipEventRecord(a, h->stream);
+            for (int r = 0; r < reps && e == hipSuccess; ++r)
+                launch_fused_fp32(h->dL, h->dR, h->nL, h->nR, p, h->colPart, h->capL, h->rowPart, h->capR, h->stream);
+            if (e == hipSuccess) e = hipEventRecord(b, h->stream);
+            launches = reps;
+        }
+    } else {
+        rc = ext_time_pass(h, h->precision, reps, a, b, &launches);
+    }
+    float ms = 0.0f;
+    if (rc == V3DM_OK && e == hipSuccess && launches > 0) {
+        e = hipEventSynchronize(b);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(h, V3DM_ERR_HIP, "timing failed: %s", hipGetErrorString(e));
+    if (launches == 0) return fail(h, V3DM_ERR_STATE, "no features to time");
+    *msPerLaunch = ms / (float)launches;
+    // the timing runs overwrote the partial results: the result of the last call stays in
+    // h->results, but its per-feature decisions are recomputed only by the next call
+    return V3DM_OK;
+}
+
 int v3dm_get_second_best(v3dm_handle h, int32_t* colSecond, int32_t* rowSecond)
 {
     if (!h) return V3DM_ERR_INVALID;

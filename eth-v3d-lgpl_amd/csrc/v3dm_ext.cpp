@@ -285,6 +285,45 @@ bool ext_supports(v3dm_context*, int mode)
     return mode == V3DM_PREC_FP32 || mode == V3DM_PREC_BF16 || mode == V3DM_PREC_FP8 || mode == V3DM_PREC_LEARNED;
 }
 
+// The dominant pass of a shortlist form, launched `reps` times back to back between two
+// events (the inputs of the last call: its derived copies are still in place).
+int ext_time_pass(v3dm_context* h, int mode, int reps, hipEvent_t start, hipEvent_t stop, int* launches)
+{
+    ExtState* e = ctx_ext(h);
+    int const nL = h->nL, nR = h->nR;
+    if (!e || nL <= 0 || nR <= 0) return ctx_fail(h, V3DM_ERR_STATE, "no shortlist result to time");
+    hipStream_t const st = h->stream;
+    EXTCHECK(h, hipEventRecord(start, st));
+    if (mode == V3DM_PREC_LEARNED) {
+        if (!e->cL || !e->cR) return ctx_fail(h, V3DM_ERR_STATE, "no learned codes");
+        ShortPlan const pl = plan_short(nL, nR, 1024);
+        ShortPlan const pr = plan_short(nR, nL, 1024);
+        for (int r = 0; r < reps; ++r)
+            launch_pass_code2(e->cL, nL, e->cR, nR, pl, e->partL, e->capL, pr, e->partR, e->capR, st);
+        *launches = reps;
+    } else {
+        ShortPlan const pl = plan_short(nL, nR, 512);
+        ShortPlan const pr = plan_short(nR, nL, 512);
+        if (mode == V3DM_PREC_BF16) {
+            if (!e->bL || !e->bR) return ctx_fail(h, V3DM_ERR_STATE, "no bf16 copies");
+            for (int r = 0; r < reps; ++r) {
+                launch_pass_bf16(e->bL, nL, e->bR, nR, pl, e->partL, e->capL, st);
+                launch_pass_bf16(e->bR, nR, e->bL, nL, pr, e->partR, e->capR, st);
+            }
+        } else {
+            if (!e->f8L || !e->f8R) return ctx_fail(h, V3DM_ERR_STATE, "no fp8 copies");
+            for (int r = 0; r < reps; ++r) {
+                launch_pass_fp8(e->f8L, e->eL, nL, e->f8R, e->eR, nR, pl, e->partL, e->capL, st);
+                launch_pass_fp8(e->f8R, e->eR, nR, e->f8L, e->eL, nL, pr, e->partR, e->capR, st);
+            }
+        }
+        *launches = 2 * reps;
+    }
+    EXTCHECK(h, hipEventRecord(stop, st));
+    EXTCHECK(h, hipGetLastError());
+    return V3DM_OK;
+}
+
 int ext_enqueue(v3dm_context* h, int mode, float minScore, float minRatio)
 {
     if (mode == V3DM_PREC_BF16) return enqueue_bf16(h, minScore, minRatio);

@@ -36,6 +36,8 @@ bool ext_supports(v3dm_context* h, int mode);
 int ext_enqueue(v3dm_context* h, int mode, float minScore, float minRatio);
 // before a result is read: sampled check of a learned result, plain rerun if it fails
 int ext_finish(v3dm_context* h);
+// the dominant pass of shortlist mode `mode`, reps times between the two events
+int ext_time_pass(v3dm_context* h, int mode, int reps, hipEvent_t start, hipEvent_t stop, int* launches);
 // exact second best candidate of every query of the last shortlist call (host arrays)
 int ext_second_best(v3dm_context* h, int* colSecond, int* rowSecond);
 void ext_features_changed(v3dm_context* h, bool left);

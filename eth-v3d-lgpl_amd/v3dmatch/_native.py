@@ -70,6 +70,7 @@ SIGNATURES = {
     "v3dm_set_profiling": (_I, [_P, _I]),
     "v3dm_get_stats": (_I, [_P, C.POINTER(Stats)]),
     "v3dm_get_second_best": (_I, [_P, _FP, _FP]),
+    "v3dm_time_dominant_kernel": (_I, [_P, _I, C.POINTER(C.c_float)]),
     "v3dm_batch_set_images_u8": (_I, [_P, _I, _FP, _FP]),
     "v3dm_batch_set_images": (_I, [_P, _I, _FP, _FP]),
     "v3dm_batch_set_pairs": (_I, [_P, _I, _FP, _FP]),

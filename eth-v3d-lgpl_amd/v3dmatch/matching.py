@@ -194,6 +194,13 @@ class SIFT_Matching:
         self._check(self._lib.v3dm_get_maxima(self._h, cm.ctypes.data, rm.ctypes.data))
         return cm[: self._nL], rm[: self._nR]
 
+    def time_dominant_kernel(self, reps=20):
+        """Average duration (ms) of one launch of the current mode's dominant kernel,
+        launched reps times back to back on the last call's inputs."""
+        ms = C.c_float(0)
+        self._check(self._lib.v3dm_time_dominant_kernel(self._h, int(reps), C.byref(ms)))
+        return ms.value
+
     def get_second_best(self):
         """(colSecond, rowSecond) of the last shortlist-mode call: per feature, the feature
         of the other set with the second best exact score among the rescored candidates."""

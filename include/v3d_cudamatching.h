@@ -15,6 +15,7 @@
 #define V3D_CUDA_MATCHING_H
 
 #include <algorithm>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -171,6 +172,80 @@ namespace V3D_CUDA
          int    _nMaxLeftFeatures, _nMaxRightFeatures, _nLeftFeatures, _nRightFeatures;
          v3dm_handle _h;
    }; // end struct SIFT_Matching
+
+   // Drop-in replacement of VocabularyTreeTraversal<BranchFactor, Depth>
+   // (V3D/CUDA/v3d_cudamatching.h:69-105, cu:795-869): traverseTree writes the node id of
+   // every feature's leaf, as the CUDA kernel does.  The descent follows the CPU tree's
+   // rule (first child of maximal sequential-FP32 score per level, v3d_voctree.h:63-99);
+   // the CUDA kernel's own summation order is not reproduced.  Errors throw.
+   template <int BranchFactor, int Depth>
+   struct VocabularyTreeTraversal
+   {
+         VocabularyTreeTraversal() : _nMaxFeatures(0), _t(0) { }
+         ~VocabularyTreeTraversal() { if (_t) v3dm_voctree_destroy(_t); }
+
+         VocabularyTreeTraversal(VocabularyTreeTraversal const&) = delete;
+         VocabularyTreeTraversal& operator=(VocabularyTreeTraversal const&) = delete;
+
+         void allocate(int nMaxFeatures)
+         {
+            if ((nMaxFeatures % 32) != 0)  // cu:806-811
+            {
+               std::fprintf(stderr, "VocabularyTreeTraversal::allocate() warning: nMaxFeatures should be a multiple of 32\n");
+               std::fprintf(stderr, "Rounding nFeatures down.\n");
+               nMaxFeatures = nMaxFeatures - (nMaxFeatures & 31);
+            }
+            _nMaxFeatures = nMaxFeatures;
+            if (!_t) check(v3dm_voctree_create(&_t, 0, BranchFactor, Depth));
+         }
+
+         void deallocate()
+         {
+            if (_t) v3dm_voctree_destroy(_t);
+            _t = 0;
+         }
+
+         void setNodes(float const * nodes)
+         {
+            if (!_t) check(v3dm_voctree_create(&_t, 0, BranchFactor, Depth));
+            check(v3dm_voctree_set_nodes(_t, nodes));
+         }
+
+         void traverseTree(int nFeatures, float const * features, int * leaveIds)
+         {
+            if (nFeatures > _nMaxFeatures)  // cu:843-848
+            {
+               std::fprintf(stderr, "VocabularyTreeTraversal::traverseTree() warning: nFeatures is larger than nMaxFeatures.\n");
+               std::fprintf(stderr, "Reducing nFeatures.\n");
+               nFeatures = _nMaxFeatures;
+            }
+            if (!_t) throw std::runtime_error("VocabularyTreeTraversal: setNodes() first");
+            check(v3dm_voctree_traverse(_t, nFeatures, features, leaveIds, 1));
+         }
+
+         static inline int getNodeCount()
+         {
+            int res = 0, len = 1;
+            for (int d = 0; d < Depth; ++d) { len *= BranchFactor; res += len; }
+            return res;
+         }
+
+         static inline int getLeaveCount()
+         {
+            int res = 1;
+            for (int d = 0; d < Depth; ++d) res *= BranchFactor;
+            return res;
+         }
+
+      protected:
+         static void check(int rc)
+         {
+            if (rc != V3DM_OK) throw std::runtime_error(std::string("v3dm: ") + v3dm_last_error(0));
+         }
+
+         int          _nMaxFeatures;
+         v3dm_voctree _t;
+   }; // end struct VocabularyTreeTraversal
 
 } // end namespace V3D_CUDA
 

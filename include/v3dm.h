@@ -123,6 +123,13 @@ int v3dm_set_derived_cache(v3dm_handle h, int enable);
  * minRecall, the result is replaced by the plain form's before it is returned
  * (stats.nFallback = 1).  Default 128 and 0.995. */
 int v3dm_set_fallback(v3dm_handle h, int samples, float minRecall);
+/* Measurement helper: launches the dominant kernel of the current precision mode (plain:
+ * the fused score pass; BF16 / FP8: the pass of each direction; learned: the code pass of
+ * both directions) `reps` times back to back on the inputs of the last findPutativeMatches,
+ * with one HIP event pair around all launches on the handle's stream, and returns the
+ * average duration of one launch.  Overwrites the intermediate partial results (not the
+ * match list or colMax / rowMax of the last call). */
+int v3dm_time_dominant_kernel(v3dm_handle h, int reps, float* msPerLaunch);
 /* Run the library's kernels on this HIP stream (hipStream_t, NULL = the handle's own). */
 int v3dm_set_stream(v3dm_handle h, void* hipStream);
 /* Feature upload from device memory (n*128 floats, device pointer); no host copy. */

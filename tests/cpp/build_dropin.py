@@ -1,4 +1,4 @@
-"""Compile tests/cpp/dropin_match.cpp with g++ against the in-tree libraries."""
+"""Compile the drop-in caller programs of tests/cpp/ with g++ against the in-tree libraries."""
 import os
 import subprocess
 
@@ -7,11 +7,11 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 LIB = os.path.join(ROOT, "eth-v3d-lgpl_amd", "v3dmatch", "_lib")
 
 
-def build(out_dir):
+def build(out_dir, name="dropin_match"):
     os.makedirs(out_dir, exist_ok=True)
-    exe = os.path.join(out_dir, "dropin_match")
+    exe = os.path.join(out_dir, name)
     cmd = ["g++", "-std=c++11", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
-           os.path.join(HERE, "dropin_match.cpp"), "-o", exe, "-L", LIB, "-lv3dmatch", "-lv3dsynth",
+           os.path.join(HERE, name + ".cpp"), "-o", exe, "-L", LIB, "-lv3dmatch", "-lv3dsynth",
            f"-Wl,-rpath,{LIB}"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

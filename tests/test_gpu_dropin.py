@@ -1,5 +1,5 @@
-"""The C++ drop-in class, driven like the reference's match_sequence caller, returns the
-oracle's match set on the GPU."""
+"""The C++ drop-in classes, driven like the reference's callers, return the oracle's
+results on the GPU: SIFT_Matching (match_sequence.cpp) and VocabularyTreeTraversal."""
 import os
 import subprocess
 import sys
@@ -29,3 +29,22 @@ def test_dropin_cpp_caller(tmp_path):
     L, R = synth.dequantise(qL), synth.dequantise(qR)
     want = oracle.find_putative_matches(L[:2976], R[:2496], 0.9, 0.8)
     assert np.array_equal(got, want)
+
+
+def test_dropin_cpp_voctree(tmp_path):
+    sys.path.insert(0, os.path.join(ROOT, "tests", "cpp"))
+    from build_dropin import build
+    exe = build(str(tmp_path), "dropin_voctree")
+    print("case dropin voctree <10,3>", flush=True)
+    rng = np.random.default_rng(5)
+    n_nodes = 10 + 100 + 1000
+    tree = rng.standard_normal((n_nodes, 128)).astype(np.float32)
+    keys = rng.standard_normal((3000, 128)).astype(np.float32)
+    keys[7] = tree[3]  # a key equal to a node
+    tree.tofile(tmp_path / "tree.bin")
+    keys.tofile(tmp_path / "keys.bin")
+    r = subprocess.run([exe, str(tmp_path / "tree.bin"), str(tmp_path / "keys.bin"), "3000"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    got = np.array(r.stdout.split(), np.int32)
+    assert np.array_equal(got, oracle.voctree_leaves(tree, 10, 3, keys, global_ids=True))

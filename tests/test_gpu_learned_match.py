@@ -211,3 +211,22 @@ def test_learned_batch_fallback(matcher):
         if flags[p]:
             assert np.array_equal(got[p], plain[p]), p
     other.close()
+
+
+def test_time_dominant_kernel(matcher):
+    """The measurement helper times each mode's dominant kernel and leaves the last result
+    readable."""
+    m, net = matcher
+    n = 4096
+    qL, qR, _ = synth.make_pair(n, n, 0.4, config=1, pair=13)
+    L, R = synth.dequantise(qL), synth.dequantise(qR)
+    m.setLeftFeatures(n, L)
+    m.setRightFeatures(n, R)
+    for prec in (v3dmatch.PREC_FP32, v3dmatch.PREC_BF16, v3dmatch.PREC_FP8, v3dmatch.PREC_LEARNED):
+        m.set_precision(prec)
+        want = m.find_putative_matches_array(0.9, 0.8)
+        ms = m.time_dominant_kernel(5)
+        say(f"case time dominant kernel: mode {prec}: {ms * 1e3:.1f} us per launch")
+        assert 0.0 < ms < 50.0
+        assert np.array_equal(m.get_matches(), want)
+    m.set_precision(v3dmatch.PREC_FP32)
