@@ -47,4 +47,8 @@ def test_dropin_cpp_voctree(tmp_path):
                        text=True, timeout=60)
     assert r.returncode == 0, r.stderr
     got = np.array(r.stdout.split(), np.int32)
-    assert np.array_equal(got, oracle.voctree_leaves(tree, 10, 3, keys, global_ids=True))
+    # as the reference: allocate(3000) rounds the capacity down to 2976 with a warning and
+    # traverseTree reduces nFeatures to it (cu:806-811, :843-848); the rest stays untouched
+    assert "should be a multiple of 32" in r.stderr and "Reducing nFeatures" in r.stderr
+    assert np.array_equal(got[:2976], oracle.voctree_leaves(tree, 10, 3, keys[:2976], global_ids=True))
+    assert np.all(got[2976:] == 0)
