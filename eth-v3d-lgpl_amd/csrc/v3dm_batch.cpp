@@ -467,6 +467,19 @@ int enqueue_run(v3dm_context* h, BatchState* b, int mode, float minScore, float 
             launch_batch_sample_decide(d, b->sPart, b->sMaxSplits, b->sNSplits, b->sNorm, b->sIdx, b->sCount,
                                        h->epsRel, minScore, minRatio, h->fallbackRecall, b->sel, b->pairFlag,
                                        b->pairFlag + b->nPairs, st);
+            // pairs whose sampled recall fell below the threshold get the plain form's
+            // decisions, on the device: their plain items, merge masked to their segments,
+            // exact re-check (kernels with nothing to do when no pair is flagged)
+            Plan const& pp = b->plan[0];
+            int const np = (int)pp.items.size();
+            BatchDev const d0 = dev_view(b, 0);
+            launch_batch_fb_filter(b->items[0], np, b->pairFlag, b->nPairs, b->fbItems, b->counters + kBCntFbItems,
+                                   b->segMask, st);
+            launch_batch_fused_fp32(d0, b->fbItems, np, b->counters + 0, b->partF, pp.grid, st,
+                                    b->counters + kBCntFbItems);
+            launch_batch_merge_decide(d0, b->partF, h->epsRel, minScore, minRatio, b->sel, b->flagList, b->counters,
+                                      b->segMask, st);
+            launch_batch_recheck(d0, b->flagList, b->counters, minScore, minRatio, b->sel, 1024, st);
         }
     }
     launch_batch_mutual(d, b->sel, b->leftRows, b->tmp, b->blockCount, b->out, b->pairCount, b->stripes,
@@ -547,28 +560,11 @@ int ensure_plan(v3dm_context* h, BatchState* b, int form)
     return V3DM_OK;
 }
 
-// After a learned run: pairs whose sampled recall fell below the threshold get the plain
-// form's decisions (plain items of those pairs only, merge masked to their segments,
-// re-check, mutual check of the whole batch), then the result counts are read again.
-int run_fallback(v3dm_context* h, BatchState* b)
+// The plain form for the pairs of a learned run that fail the sampled check runs inside the
+// run (enqueue_run); it needs the plain plan (without tuning when the plain form has not
+// run yet) and room for the flagged items.
+int ensure_fallback_plan(v3dm_context* h, BatchState* b)
 {
-    int const P = b->nPairs;
-    int const* flag = b->hostChk;
-    int const* cnt = b->hostChk + P;
-    std::vector<int> mask(2 * P, 0);
-    int nFall = 0, sampled = 0, missed = 0;
-    for (int p = 0; p < P; ++p) {
-        sampled += cnt[4 * p + 3];
-        missed += cnt[4 * p + 1];
-        if (flag[p]) {
-            mask[p] = mask[P + p] = 1;
-            ++nFall;
-        }
-    }
-    h->stats.nSampled = sampled;
-    h->stats.nSampleMissed = missed;
-    h->stats.nFallback = nFall;
-    if (nFall == 0) return V3DM_OK;
     int rc;
     if (!b->planValid[0]) {
         int dev = 0, cus = 256;
@@ -581,27 +577,30 @@ int run_fallback(v3dm_context* h, BatchState* b)
         b->planValid[0] = true;
         b->version += 1;
     }
-    Plan const& pl = b->plan[0];
-    std::vector<int4> items;
-    for (int4 const& w : pl.items)
-        if (mask[w.x]) items.push_back(w);
-    if ((rc = growi(h, b->fbItems, b->fbItemCap, std::max<size_t>(items.size(), 1)))) return rc;
-    hipStream_t const st = h->stream;
-    BCHECK(h, hipMemcpyAsync(b->fbItems, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, st));
-    BCHECK(h, hipMemcpyAsync(b->segMask, mask.data(), mask.size() * sizeof(int), hipMemcpyHostToDevice, st));
-    BatchDev const d = dev_view(b, 0);
-    launch_batch_reset(b->counters, b->pairCount, P, st);
-    launch_batch_fused_fp32(d, b->fbItems, (int)items.size(), b->counters + 0, b->partF, pl.grid, st);
-    launch_batch_merge_decide(d, b->partF, h->epsRel, b->lastMinScore, b->lastMinRatio, b->sel, b->flagList,
-                              b->counters, b->segMask, st);
-    launch_batch_recheck(d, b->flagList, b->counters, b->lastMinScore, b->lastMinRatio, b->sel, 1024, st);
-    launch_batch_mutual(d, b->sel, b->leftRows, b->tmp, b->blockCount, b->out, b->pairCount, b->stripes, b->counters,
-                        st);
-    BCHECK(h, hipGetLastError());
-    BCHECK(h, hipMemcpyAsync(b->hostRes, b->counters, kBatchCounters * sizeof(int), hipMemcpyDeviceToHost, st));
-    BCHECK(h, hipMemcpyAsync(b->hostRes + kBatchCounters, b->pairCount, P * sizeof(int), hipMemcpyDeviceToHost, st));
-    BCHECK(h, hipStreamSynchronize(st));  // host vectors above go out of scope
+    size_t const need = std::max<size_t>(b->plan[0].items.size(), 1);
+    if ((size_t)b->fbItemCap < need) {
+        drop_graphs(b);  // captured runs hold the old buffer
+        if ((rc = growi(h, b->fbItems, b->fbItemCap, need))) return rc;
+        b->version += 1;
+    }
     return V3DM_OK;
+}
+
+// Statistics of the sampled check of the last learned run (host copy of the flags).
+void fallback_stats(v3dm_context* h, BatchState* b)
+{
+    int const P = b->nPairs;
+    int const* flag = b->hostChk;
+    int const* cnt = b->hostChk + P;
+    int nFall = 0, sampled = 0, missed = 0;
+    for (int p = 0; p < P; ++p) {
+        sampled += cnt[4 * p + 3];
+        missed += cnt[4 * p + 1];
+        nFall += flag[p] ? 1 : 0;
+    }
+    h->stats.nSampled = sampled;
+    h->stats.nSampleMissed = missed;
+    h->stats.nFallback = nFall;
 }
 
 int check_batch(v3dm_context* h, BatchState*& b)
@@ -852,6 +851,7 @@ int v3dm_batch_match_async(v3dm_handle h, float minScore, float minRatio)
     }
     if (!h->cacheDerived && form != 0 && !b->planValid[form]) enqueue_derived(h, b, form);  // for the tuner
     if ((rc = ensure_plan(h, b, form))) return rc;
+    if (form == 2 && h->fallbackSamples > 0 && (rc = ensure_fallback_plan(h, b))) return rc;
     BCHECK(h, hipEventRecord(b->evA, st));
     if (b->useGraph) {
         BatchState::GraphKey const key{mode, minScore, minRatio, h->epsRel, b->version * 2 + (h->cacheDerived ? 1 : 0),
@@ -905,7 +905,7 @@ int v3dm_batch_get_matches(v3dm_handle h, int32_t* offsets, int32_t* pairs, int 
     h->stats.nFallback = h->stats.nSampled = h->stats.nSampleMissed = 0;
     if (b->checkPending) {
         b->checkPending = false;
-        if ((rc = run_fallback(h, b))) return rc;
+        fallback_stats(h, b);
     }
     int const total = b->hostRes[kBCntPairs];
     *nTotal = total;

@@ -27,6 +27,7 @@ constexpr int kBCntFlag = 4;
 constexpr int kBCntPairs = 6;
 constexpr int kBCntFlagDone = 7;
 constexpr int kBCntRescoredDone = 8;
+constexpr int kBCntFbItems = 10;  // plain items of the pairs a learned run hands to the plain form
 constexpr int kBatchCounters = 16;
 
 struct BatchSeg {
@@ -62,8 +63,9 @@ struct BatchDev {
 };
 
 // kernels of the plain FP32 form (v3dm_fp32.hip)
+// nItemsDev (may be null): the item count in device memory, at most nItems
 void launch_batch_fused_fp32(BatchDev const& b, int4 const* items, int nItems, int* queue, Top2* part, int grid,
-                             hipStream_t st);
+                             hipStream_t st, int const* nItemsDev = nullptr);
 // shortlist pass (v3dm_short.hip)
 void launch_batch_pass_bf16(BatchDev const& b, int4 const* items, int nItems, int* queue, Cand2* part, int grid,
                             hipStream_t st);
@@ -90,6 +92,10 @@ void launch_batch_sample_decide(BatchDev const& b, Top2 const* sPart, int maxSpl
                                 hipStream_t st);
 void launch_batch_recheck(BatchDev const& b, int const* flagList, int* counters, float minScore, float minRatio,
                           int* sel, int grid, hipStream_t st);
+// the plain items of the pairs with pairFlag set -> fbItems (count in *fbCount, which must
+// be zero), segMask[s] = flag of segment s's pair (2 * nPairs segments)
+void launch_batch_fb_filter(int4 const* items, int nItems, int const* pairFlag, int nPairs, int4* fbItems,
+                            int* fbCount, int* segMask, hipStream_t st);
 void launch_batch_mutual(BatchDev const& b, int const* sel, int leftRows, int2* tmp, int* blockCount, int2* pairs,
                          int* pairCount, int* stripes, int* counters, hipStream_t st);
 

@@ -20,6 +20,27 @@ __global__ void k_batch_reset(int* __restrict__ counters, int* __restrict__ pair
     if (t < nPairs) pairCount[t] = 0;
 }
 
+__global__ __launch_bounds__(256) void k_batch_fb_filter(int4 const* __restrict__ items, int nItems,
+                                                         int const* __restrict__ pairFlag, int nPairs,
+                                                         int4* __restrict__ fbItems, int* __restrict__ fbCount,
+                                                         int* __restrict__ segMask)
+{
+    int const t = blockIdx.x * 256 + threadIdx.x;
+    if (t < 2 * nPairs) segMask[t] = pairFlag[t < nPairs ? t : t - nPairs] != 0;
+    if (t >= nItems) return;
+    int4 const w = items[t];
+    if (pairFlag[w.x]) fbItems[atomicAdd(fbCount, 1)] = w;
+}
+
+void launch_batch_fb_filter(int4 const* items, int nItems, int const* pairFlag, int nPairs, int4* fbItems,
+                            int* fbCount, int* segMask, hipStream_t st)
+{
+    int const n = nItems > 2 * nPairs ? nItems : 2 * nPairs;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_batch_fb_filter, dim3((n + 255) / 256), dim3(256), 0, st, items, nItems, pairFlag, nPairs,
+                       fbItems, fbCount, segMask);
+}
+
 void launch_batch_reset(int* counters, int* pairCount, int nPairs, hipStream_t st)
 {
     int const n = nPairs > kBatchCounters ? nPairs : kBatchCounters;

@@ -244,12 +244,14 @@ __global__ __launch_bounds__(256, 2) void k_guided_fp32(float const* __restrict_
 // dry.  Item = (pair, left block, split, tiles per split); the left queries of pair p are
 // segment p, its right queries segment nPairs + p.
 __global__ __launch_bounds__(256, 2) void k_batch_fused_fp32(BatchDev b, int4 const* __restrict__ items, int nItems,
+                                                             int const* __restrict__ nItemsDev,
                                                              int* __restrict__ queue, Top2* __restrict__ part,
                                                              float inf)
 {
     __shared__ __attribute__((aligned(1024))) unsigned char stA[kStageBytes];
     __shared__ __attribute__((aligned(1024))) unsigned char stB[kStageBytes];
     __shared__ int sItem;
+    if (nItemsDev) nItems = __builtin_amdgcn_readfirstlane(min(nItems, *nItemsDev));  // written by an earlier kernel
     for (;;) {
         if (threadIdx.x == 0) sItem = atomicAdd(queue, 1);
         __syncthreads();  // also: every wave is done with the stages of the previous item
@@ -300,10 +302,11 @@ void launch_batch_sample_fp32(BatchDev const& b, float const* sRows, int const* 
 }
 
 void launch_batch_fused_fp32(BatchDev const& b, int4 const* items, int nItems, int* queue, Top2* part, int grid,
-                             hipStream_t st)
+                             hipStream_t st, int const* nItemsDev)
 {
     if (nItems <= 0) return;
-    hipLaunchKernelGGL(k_batch_fused_fp32, dim3(grid), dim3(256), 0, st, b, items, nItems, queue, part, INFINITY);
+    hipLaunchKernelGGL(k_batch_fused_fp32, dim3(grid), dim3(256), 0, st, b, items, nItems, nItemsDev, queue, part,
+                       INFINITY);
 }
 
 FusedPlan plan_fused_fp32(int nL, int nR, int targetGroups, int maxSplits)
