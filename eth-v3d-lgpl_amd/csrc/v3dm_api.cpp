@@ -72,8 +72,7 @@ void releaseBuffers(v3dm_context* h)
     freeDev(h->rowMax);
     freeDev(h->flagList);
     freeDev(h->rpart);
-    freeDev(h->mtmp);
-    freeDev(h->mblock);
+    freeDev(h->mstatus);
     freeDev(h->stripes);
     freeDev(h->results);
     freeDev(h->dQ);
@@ -166,7 +165,7 @@ int enqueue_plain(v3dm_context* h, float minScore, float minRatio)
         launch_recheck_exact(h->dL, h->dR, nL, nR, h->flagList, h->capL, h->results, minScore, minRatio,
                              h->colMax, h->rowMax, h->rpart, kRecheckGrid, st);
     }
-    launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mtmp, h->mblock, h->stripes,
+    launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mstatus, next_mutual_epoch(h), h->stripes,
                           reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st);
     HIPCHECK(h, hipGetLastError());
     return V3DM_OK;
@@ -206,6 +205,7 @@ void v3dm_destroy(v3dm_handle h)
     if (!h) return;
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
+    ext_quiesce(h);
     releaseBuffers(h);
     ext_destroy(h->ext);
     batch_destroy(h->batch);
@@ -226,6 +226,7 @@ int v3dm_allocate(v3dm_handle h, int nMaxLeft, int nMaxRight, int useGuidedMatch
         return fail(h, V3DM_ERR_INVALID, "allocate: maximum above 2^20 features");
     HIPCHECK(h, hipSetDevice(h->device));
     HIPCHECK(h, hipStreamSynchronize(h->stream));
+    ext_quiesce(h);
     releaseBuffers(h);
     h->useGuided = useGuidedMatching != 0;
     nMaxLeft = roundWarn(h, nMaxLeft, "allocate()", "nMaxLeftFeatures");
@@ -249,8 +250,9 @@ int v3dm_allocate(v3dm_handle h, int nMaxLeft, int nMaxRight, int useGuidedMatch
     HIPCHECK(h, hipMalloc(&h->rowMax, cr * sizeof(int)));
     HIPCHECK(h, hipMalloc(&h->flagList, (cl + cr) * sizeof(int)));
     HIPCHECK(h, hipMalloc(&h->rpart, (cl + cr) * kRecheckMaxChunks * sizeof(Top2)));
-    HIPCHECK(h, hipMalloc(&h->mtmp, cl * sizeof(int2)));
-    HIPCHECK(h, hipMalloc(&h->mblock, (cl / 1024 + 2) * sizeof(int)));
+    size_t const statusBytes = (cl / kMutualSlice + 1) * sizeof(unsigned long long);
+    HIPCHECK(h, hipMalloc(&h->mstatus, statusBytes));
+    HIPCHECK(h, hipMemsetAsync(h->mstatus, 0, statusBytes, h->stream));
     HIPCHECK(h, hipMalloc(&h->stripes, kStripes * kStripeStride * sizeof(int)));
     HIPCHECK(h, hipMemsetAsync(h->stripes, 0, kStripes * kStripeStride * sizeof(int), h->stream));
     h->pairCap = (int)std::min(cl, cr);
@@ -275,6 +277,7 @@ int v3dm_deallocate(v3dm_handle h)
     if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
     HIPCHECK(h, hipSetDevice(h->device));
     HIPCHECK(h, hipStreamSynchronize(h->stream));
+    ext_quiesce(h);
     releaseBuffers(h);
     h->nMaxL = h->nMaxR = 0;
     return V3DM_OK;
@@ -303,6 +306,7 @@ int setFeatures(v3dm_context* h, bool left, int n, const void* src, Source kind)
     n = std::min(n, left ? h->nMaxL : h->nMaxR);
     if (n > 0 && !src) return fail(h, V3DM_ERR_INVALID, "null descriptor pointer");
     HIPCHECK(h, hipSetDevice(h->device));
+    ext_quiesce(h);  // the sampled check's side stream reads the features
     float* dst = left ? h->dL : h->dR;
     size_t const bytes = (size_t)n * kDim * sizeof(float);
     if (n > 0) {
@@ -342,6 +346,8 @@ int copyResults(v3dm_context* h, int32_t* pairs, int cap, int* nPairs)
     size_t const headBytes = kNumCounters * sizeof(int);
     HIPCHECK(h, hipMemcpyAsync(h->hostStage, h->results, headBytes, hipMemcpyDeviceToHost, h->stream));
     HIPCHECK(h, hipStreamSynchronize(h->stream));
+    if (h->hostStage[kCntMutualError] == (int)h->mutualEpoch)
+        return fail(h, V3DM_ERR_HIP, "mutual check: ordering wait between work-groups ran out");
     int const n = h->hostStage[kCntPairs];
     if (n > 0)
         HIPCHECK(h, hipMemcpyAsync(h->hostStage + kNumCounters, h->results + kNumCounters,
@@ -409,7 +415,7 @@ int runGuided(v3dm_context* h, int kind, const double M[9], float distanceThresh
         launch_recheck_exact(h->dL, h->dR, nL, nR, h->flagList, h->capL, h->results, minScore, minRatio, h->colMax,
                              h->rowMax, h->rpart, kRecheckGrid, st, h->guidedBits, h->capL);
     }
-    launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mtmp, h->mblock, h->stripes,
+    launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mstatus, next_mutual_epoch(h), h->stripes,
                           reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st);
     HIPCHECK(h, hipGetLastError());
     h->lastMinScore = minScore;
@@ -484,6 +490,7 @@ int v3dm_set_stream(v3dm_handle h, void* stream)
 {
     if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
     HIPCHECK(h, hipStreamSynchronize(h->stream));
+    ext_quiesce(h);
     h->stream = stream ? (hipStream_t)stream : h->ownStream;
     return V3DM_OK;
 }
@@ -561,6 +568,7 @@ int v3dm_synchronize(v3dm_handle h)
     if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
     HIPCHECK(h, hipSetDevice(h->device));
     HIPCHECK(h, hipStreamSynchronize(h->stream));
+    ext_synchronize(h);
     return V3DM_OK;
 }
 

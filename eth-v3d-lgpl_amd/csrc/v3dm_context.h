@@ -42,8 +42,8 @@ struct v3dm_context {
     int* rowMax = nullptr;
     int* flagList = nullptr;
     v3dm::Top2* rpart = nullptr;       // re-check chunk partials: (capL + capR) x kRecheckMaxChunks
-    int2* mtmp = nullptr;        // mutual-check block slices: capL
-    int* mblock = nullptr;       // mutual-check block counts
+    unsigned long long* mstatus = nullptr;  // mutual-check status words: capL / kMutualSlice + 1
+    unsigned mutualEpoch = 0;               // epoch of the last mutual check (never 0 once run)
     int* stripes = nullptr;      // striped counts (kStripes slots), cleared by the gather kernel
     int* results = nullptr;  // counters (kNumCounters ints) followed by int2 pairs
     int* hostStage = nullptr;
@@ -75,3 +75,14 @@ struct v3dm_context {
 
     std::string err;
 };
+
+namespace v3dm {
+
+// The mutual check of the handle's next call: its status words and a fresh epoch.
+inline unsigned next_mutual_epoch(v3dm_context* h)
+{
+    if (++h->mutualEpoch == 0) ++h->mutualEpoch;
+    return h->mutualEpoch;
+}
+
+}  // namespace v3dm

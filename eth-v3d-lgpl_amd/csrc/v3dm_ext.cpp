@@ -31,14 +31,16 @@ struct ExtState {
     float* sNorm = nullptr;
     int* sIdx = nullptr;
     Top2* sColPart = nullptr;  // kMaxSplits x kSampleRows
-    int* sDec = nullptr;       // exact decisions of the samples (+1 unused right slot)
+    int* sDec = nullptr;       // exact decisions of the samples (+1 unused right slot; mapped host memory)
+    int* sDecHost = nullptr;   // the same memory as the host addresses it
     int* sFlags = nullptr;     // open-query list of the samples
     Top2* sRpart = nullptr;    // re-check partials
     int* sCounters = nullptr;  // kNumCounters
-    int* sOut = nullptr;       // accepted, missed, wrongly accepted
-    int* sHost = nullptr;      // pinned copy of sOut
+    int* sHost = nullptr;      // learned decisions of the samples (pinned, mapped, coherent)
+    int* sHostDev = nullptr;   // sHost as the kernels address it
     hipStream_t side = nullptr;  // the check runs beside the learned passes
-    hipEvent_t evFork = nullptr, evJoin = nullptr, evMain = nullptr;
+    hipEvent_t evFork = nullptr;
+    bool forkNeeded = true;    // the features were written since the side stream last waited
     bool sPending = false;
     int sSamples = 0;
     Cand2* partL = nullptr;  // kShortMaxSplits x capL x 2
@@ -119,7 +121,7 @@ int enqueue_bf16(v3dm_context* h, float minScore, float minRatio)
                        h->colMax, h->rowMax, e->secL, e->secR, h->stripes, st);
     }
     e->haveSecond = work;
-    launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mtmp, h->mblock, h->stripes,
+    launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mstatus, next_mutual_epoch(h), h->stripes,
                           reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st);
     EXTCHECK(h, hipGetLastError());
     return V3DM_OK;
@@ -143,31 +145,35 @@ int ensure_sample(v3dm_context* h, ExtState* e)
     EXTCHECK(h, hipMalloc(&e->sNorm, kSampleRows * sizeof(float)));
     EXTCHECK(h, hipMalloc(&e->sIdx, kSampleRows * sizeof(int)));
     EXTCHECK(h, hipMalloc(&e->sColPart, (size_t)kSampleMaxSplits * kSampleRows * sizeof(Top2)));
-    EXTCHECK(h, hipMalloc(&e->sDec, (kSampleRows + 1) * sizeof(int)));
+    EXTCHECK(h, hipHostMalloc(&e->sDecHost, (kSampleRows + 1) * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent));
+    EXTCHECK(h, hipHostGetDevicePointer((void**)&e->sDec, e->sDecHost, 0));
     EXTCHECK(h, hipMalloc(&e->sFlags, (kSampleRows + 1) * sizeof(int)));
     EXTCHECK(h, hipMalloc(&e->sRpart, (size_t)(kSampleRows + 1) * kRecheckMaxChunks * sizeof(Top2)));
     EXTCHECK(h, hipMalloc(&e->sCounters, kNumCounters * sizeof(int)));
-    EXTCHECK(h, hipMalloc(&e->sOut, 4 * sizeof(int)));
-    EXTCHECK(h, hipHostMalloc(&e->sHost, 4 * sizeof(int), hipHostMallocDefault));
+    EXTCHECK(h, hipHostMalloc(&e->sHost, kSampleRows * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent));
+    EXTCHECK(h, hipHostGetDevicePointer((void**)&e->sHostDev, e->sHost, 0));
     EXTCHECK(h, hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking));
     EXTCHECK(h, hipEventCreateWithFlags(&e->evFork, hipEventDisableTiming));
-    EXTCHECK(h, hipEventCreateWithFlags(&e->evJoin, hipEventDisableTiming));
-    EXTCHECK(h, hipEventCreateWithFlags(&e->evMain, hipEventDisableTiming));
+    e->forkNeeded = true;
     return V3DM_OK;
 }
 
 // The plain exact path on S sampled left features against all right features, on a side
-// stream from the fork point (it needs only the features).  The comparison with the learned
-// decisions also runs on the side stream, after the handle's stream has decided (evMain),
-// while the handle's stream runs the mutual check; the handle's stream waits for the side
-// stream at the end of the call, so later work on it is ordered after the check.
-int enqueue_sample_fork(v3dm_context* h, ExtState* e, float minScore, float minRatio)
+// stream (it reads only the features and its own buffers).  The side stream waits for the
+// handle's stream (evFork, recorded at the start of the call) only when the features were
+// written since it last did: every writer of the features first waits for the side stream
+// (ext_quiesce), so a side stream that has waited once stays ordered after the features.
+// Nothing on the handle's stream waits for the side stream: the exact decisions of the
+// samples go to mapped host memory, the mutual check writes the learned decisions of the
+// same samples there too, and the comparison runs on the host when the result is read
+// (ext_finish, after both streams are synchronised).
+int enqueue_sample_check(v3dm_context* h, ExtState* e, float minScore, float minRatio)
 {
-    int rc = ensure_sample(h, e);
-    if (rc) return rc;
-    EXTCHECK(h, hipEventRecord(e->evFork, h->stream));
-    EXTCHECK(h, hipStreamWaitEvent(e->side, e->evFork, 0));
     hipStream_t const st = e->side;
+    if (e->forkNeeded) {
+        EXTCHECK(h, hipStreamWaitEvent(st, e->evFork, 0));
+        e->forkNeeded = false;
+    }
     int const nL = h->nL, nR = h->nR;
     int const S = std::min(std::min(h->fallbackSamples, kSampleRows), nL);
     launch_sample_gather(h->dL, h->normL, nL, S, e->sL, e->sNorm, e->sIdx, e->sCounters, st);
@@ -178,25 +184,6 @@ int enqueue_sample_fork(v3dm_context* h, ExtState* e, float minScore, float minR
     launch_recheck_exact(e->sL, h->dR, S, nR, e->sFlags, kSampleRows, e->sCounters, minScore, minRatio, e->sDec,
                          e->sDec + kSampleRows, e->sRpart, kRecheckGrid, st);
     e->sSamples = S;
-    return V3DM_OK;
-}
-
-// after the learned decisions: the comparison on the side stream
-int enqueue_sample_compare(v3dm_context* h, ExtState* e)
-{
-    EXTCHECK(h, hipEventRecord(e->evMain, h->stream));
-    EXTCHECK(h, hipStreamWaitEvent(e->side, e->evMain, 0));
-    launch_sample_compare(e->sDec, e->sIdx, e->sSamples, h->colMax, e->sOut, e->side);
-    EXTCHECK(h, hipMemcpyAsync(e->sHost, e->sOut, 4 * sizeof(int), hipMemcpyDeviceToHost, e->side));
-    EXTCHECK(h, hipEventRecord(e->evJoin, e->side));
-    e->sPending = true;
-    return V3DM_OK;
-}
-
-// end of the call: the handle's stream waits for the side stream
-int enqueue_sample_join(v3dm_context* h, ExtState* e)
-{
-    EXTCHECK(h, hipStreamWaitEvent(h->stream, e->evJoin, 0));
     return V3DM_OK;
 }
 
@@ -214,6 +201,10 @@ int enqueue_learned(v3dm_context* h, float minScore, float minRatio)
     if (work) {
         if (e->cScale != scale) e->cLValid = e->cRValid = false;
         e->cScale = scale;
+        if (h->fallbackSamples > 0) {
+            if ((rc = ensure_sample(h, e))) return rc;
+            if (e->forkNeeded) EXTCHECK(h, hipEventRecord(e->evFork, st));  // the check needs only the features
+        }
         // embedding: network forward on the FP32 rows (read up to the next multiple of 128,
         // inside the allocation) with the quantisation fused; code rows past n are never
         // used (candidates past n are masked, queries past n not written)
@@ -222,7 +213,8 @@ int enqueue_learned(v3dm_context* h, float minScore, float minRatio)
         e->cLValid = e->cRValid = true;
         e->sPending = false;
         bool const check = h->fallbackSamples > 0;
-        if (check && (rc = enqueue_sample_fork(h, e, minScore, minRatio))) return rc;
+        // the main stream's kernels are enqueued first: the check's stream operations cost
+        // more host time than its kernels take, and the main stream should not wait for them
         // 1024 work-groups: the code pass runs four per CU
         ShortPlan const pl = plan_short(nL, nR, 1024);
         ShortPlan const pr = plan_short(nR, nL, 1024);
@@ -233,14 +225,18 @@ int enqueue_learned(v3dm_context* h, float minScore, float minRatio)
         launch_rescore(h->dL, h->dR, nL, nR, e->partL, pl.splits, e->capL, e->partR, pr.splits, e->capR, h->normL,
                        h->normR, h->maxNormBits, 0.0f, h->learnedTopK, minScore, minRatio,
                        h->colMax, h->rowMax, e->secL, e->secR, h->stripes, st);
-        if (check && (rc = enqueue_sample_compare(h, e))) return rc;
         e->haveSecond = true;
-        launch_mutual_compact(h->colMax, h->rowMax, nL, nR, h->mtmp, h->mblock, h->stripes,
-                              reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st);
-        if (check && (rc = enqueue_sample_join(h, e))) return rc;
+        SampleOut so;
+        if (check) {
+            if ((rc = enqueue_sample_check(h, e, minScore, minRatio))) return rc;
+            so = SampleOut{e->sSamples, e->sHostDev};
+            e->sPending = true;
+        }
+        launch_mutual_compact(h->colMax, h->rowMax, nL, nR, h->mstatus, next_mutual_epoch(h), h->stripes,
+                              reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st, so);
     } else {
         e->haveSecond = false;
-        launch_mutual_compact(h->colMax, h->rowMax, 0, nR, h->mtmp, h->mblock, h->stripes,
+        launch_mutual_compact(h->colMax, h->rowMax, 0, nR, h->mstatus, next_mutual_epoch(h), h->stripes,
                               reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st);
     }
     EXTCHECK(h, hipGetLastError());
@@ -272,7 +268,7 @@ int enqueue_fp8(v3dm_context* h, float minScore, float minRatio)
                        h->colMax, h->rowMax, e->secL, e->secR, h->stripes, st);
     }
     e->haveSecond = work;
-    launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mtmp, h->mblock, h->stripes,
+    launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mstatus, next_mutual_epoch(h), h->stripes,
                           reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st);
     EXTCHECK(h, hipGetLastError());
     return V3DM_OK;
@@ -340,7 +336,14 @@ int ext_finish(v3dm_context* h)
     if (!e || !e->sPending) return V3DM_OK;
     e->sPending = false;
     EXTCHECK(h, hipStreamSynchronize(h->stream));
-    int const accepted = e->sHost[0], missed = e->sHost[1];
+    EXTCHECK(h, hipStreamSynchronize(e->side));
+    // exactly accepted samples, and those whose learned decision differs
+    int accepted = 0, missed = 0;
+    for (int k = 0; k < e->sSamples; ++k) {
+        int const x = e->sDecHost[k], d = e->sHost[k];
+        accepted += x >= 0;
+        missed += (x >= 0 && d != x);
+    }
     h->stats.nSampled = e->sSamples;
     h->stats.nSampleMissed = missed;
     if (accepted > 0 && (float)(accepted - missed) < h->fallbackRecall * (float)accepted) {
@@ -361,6 +364,20 @@ int ext_second_best(v3dm_context* h, int* colSecond, int* rowSecond)
     if (rowSecond && h->nR > 0)
         EXTCHECK(h, hipMemcpy(rowSecond, e->secR, (size_t)h->nR * sizeof(int), hipMemcpyDeviceToHost));
     return V3DM_OK;
+}
+
+void ext_quiesce(v3dm_context* h)
+{
+    ExtState* e = ctx_ext(h);
+    if (!e || !e->side) return;
+    (void)hipStreamSynchronize(e->side);
+    e->forkNeeded = true;
+}
+
+void ext_synchronize(v3dm_context* h)
+{
+    ExtState* e = ctx_ext(h);
+    if (e && e->side) (void)hipStreamSynchronize(e->side);
 }
 
 void ext_discard_result(v3dm_context* h)
@@ -396,18 +413,19 @@ void ext_release(ExtState* e)
         if (p) (void)hipFree(p);
     e->cL = e->cR = nullptr;
     e->cLValid = e->cRValid = false;
-    for (void* p : {(void*)e->sL, (void*)e->sNorm, (void*)e->sIdx, (void*)e->sColPart,
-                    (void*)e->sDec, (void*)e->sFlags, (void*)e->sRpart, (void*)e->sCounters, (void*)e->sOut})
+    if (e->side) (void)hipStreamSynchronize(e->side);
+    for (void* p : {(void*)e->sL, (void*)e->sNorm, (void*)e->sIdx, (void*)e->sColPart, (void*)e->sFlags,
+                    (void*)e->sRpart, (void*)e->sCounters})
         if (p) (void)hipFree(p);
     if (e->sHost) (void)hipHostFree(e->sHost);
+    if (e->sDecHost) (void)hipHostFree(e->sDecHost);
     if (e->side) (void)hipStreamDestroy(e->side);
     if (e->evFork) (void)hipEventDestroy(e->evFork);
-    if (e->evJoin) (void)hipEventDestroy(e->evJoin);
-    if (e->evMain) (void)hipEventDestroy(e->evMain);
     e->side = nullptr;
-    e->evFork = e->evJoin = e->evMain = nullptr;
+    e->evFork = nullptr;
+    e->forkNeeded = true;
     e->sL = e->sNorm = nullptr;
-    e->sIdx = e->sDec = e->sFlags = e->sCounters = e->sOut = e->sHost = nullptr;
+    e->sIdx = e->sDec = e->sDecHost = e->sFlags = e->sCounters = e->sHost = e->sHostDev = nullptr;
     e->sColPart = e->sRpart = nullptr;
     e->sPending = false;
     e->f8L = e->f8R = e->eL = e->eR = nullptr;

@@ -41,6 +41,11 @@ int ext_time_pass(v3dm_context* h, int mode, int reps, hipEvent_t start, hipEven
 // exact second best candidate of every query of the last shortlist call (host arrays)
 int ext_second_best(v3dm_context* h, int* colSecond, int* rowSecond);
 void ext_features_changed(v3dm_context* h, bool left);
+// before the features (or the buffers holding them) are written: wait for the sampled
+// check's side stream, which reads them, and make its next use wait for the handle's stream
+void ext_quiesce(v3dm_context* h);
+// wait for the side stream (the handle's synchronise covers the whole call)
+void ext_synchronize(v3dm_context* h);
 // a guided call replaces the last result: drop its pending sampled check and second bests
 void ext_discard_result(v3dm_context* h);
 void ext_release(ExtState* e);

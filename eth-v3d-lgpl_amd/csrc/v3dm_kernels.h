@@ -13,7 +13,7 @@ constexpr int kCntFlag = 1;       // counters[1]: open left queries of the runni
 constexpr int kCntFlagDone = 2;   // counters[2]: queries re-checked exactly by the last call
 constexpr int kCntRescoredDone = 4;  // counters[4]: ... by the last call
 constexpr int kCntFlagR = 5;      // counters[5]: open right queries of the running call
-constexpr int kCntTicket = 6;     // counters[6]: work-groups done in the mutual-check kernel (0 between calls)
+constexpr int kCntMutualError = 6;  // counters[6]: epoch of a mutual check whose ordering wait ran out
 constexpr int kNumCounters = 8;
 // Counts summed by many work-groups go to kStripes slots one cache line apart (atomics on a
 // single address serialise in L2); the gather kernel sums and clears them.
@@ -54,9 +54,20 @@ RecheckPlan plan_recheck(int nC);
 void launch_recheck_exact(float const* L, float const* R, int nL, int nR, int const* flagList, int flagOffR,
                           int const* counters, float minScore, float minRatio, int* colMax, int* rowMax, Top2* part,
                           int grid, hipStream_t st, unsigned const* gbits = nullptr, int ldi = 0);
-// nL = 0 publishes an empty result.  tmp: nL int2, blockCount: ceil(nL/1024) ints.
-void launch_mutual_compact(int const* colMax, int const* rowMax, int nL, int nR, int2* tmp, int* blockCount,
-                           int* stripes, int2* pairs, int* counters, hipStream_t st);
+// The learned decisions of the S sampled left features of the sampled exact check:
+// out[k] = colMax[sample_row(k, nL, S)], k < S.  out may be mapped host memory.  S = 0:
+// nothing to write.
+struct SampleOut {
+    int S = 0;
+    int* out = nullptr;
+};
+// nL = 0 publishes an empty result.  status: ceil(nL / kMutualSlice) words (zero before
+// the first call); epoch: nonzero and different from the previous call's.  The sampled
+// learned decisions, when asked for, are written by work-group 0 (colMax is final by then).
+constexpr int kMutualSlice = 1024;  // left features per work-group of the mutual check
+void launch_mutual_compact(int const* colMax, int const* rowMax, int nL, int nR, unsigned long long* status,
+                           unsigned epoch, int* stripes, int2* pairs, int* counters, hipStream_t st,
+                           SampleOut const& so = {});
 // uint8 SIFT descriptors -> floats, as SIFT_Feature::normalizeDescriptor
 // (V3D/Base/v3d_feature.h:30-38): var = sum q^2, d = q * (1/sqrtf(var)), IEEE rounding.
 void launch_dequant_u8(uint8_t const* q, int n, float* out, hipStream_t st);
@@ -83,6 +94,5 @@ void launch_sample_gather(float const* L, float const* normL, int nL, int S, flo
 void launch_sample_decide(Top2 const* colPart, int nParts, int ldc, int S, float const* normS,
                           unsigned int const* maxNormBits, float epsRel, float minScore, float minRatio, int* dec,
                           int* flagList, int* counters, hipStream_t st);
-void launch_sample_compare(int const* exactS, int const* idx, int S, int const* colMax, int* out, hipStream_t st);
 
 }  // namespace v3dm

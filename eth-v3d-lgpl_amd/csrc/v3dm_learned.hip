@@ -530,11 +530,20 @@ __global__ __launch_bounds__(256, 2) void k_mlp_codes(float const* __restrict__ 
     int const wave = tid >> 6;
     int const h = lane >> 5;
     int const c32 = lane & 31;
-    // W1 -> LDS: 256 rows x 16 chunks of 16 bytes
-    for (int f = tid; f < kHid * 16; f += 256) {
-        int const row = f >> 4, ch = f & 15;
-        *reinterpret_cast<uint4*>(&W1s[row * kW1Stride + ch * 8]) =
-            reinterpret_cast<uint4 const*>(W1b + (size_t)row * 128)[ch];
+    // W1 -> LDS: 256 rows x 16 chunks of 16 bytes; all sixteen loads of a thread are issued
+    // before the first store, so their latencies overlap
+    {
+        uint4 w[kHid * 16 / 256];
+#pragma unroll
+        for (int i = 0; i < kHid * 16 / 256; ++i) {
+            int const f = tid + 256 * i;
+            w[i] = reinterpret_cast<uint4 const*>(W1b + (size_t)(f >> 4) * 128)[f & 15];
+        }
+#pragma unroll
+        for (int i = 0; i < kHid * 16 / 256; ++i) {
+            int const f = tid + 256 * i;
+            *reinterpret_cast<uint4*>(&W1s[(f >> 4) * kW1Stride + (f & 15) * 8]) = w[i];
+        }
     }
     b1s[tid] = b1[tid];
     __syncthreads();

@@ -42,34 +42,43 @@ __device__ __forceinline__ Top2x top2x_shfl_xor(Top2x s, int mask)
 
 constexpr int kPerLane = kRescoreSlots / 8;  // candidates one lane holds (kShortMaxSplits lists)
 
-__device__ __forceinline__ double exact_dot(float const* __restrict__ q, float const* __restrict__ c)
+// Exact score (the oracle's: FP64 products, summed in ascending k).  q: the query row in
+// LDS; c: the candidate row, loaded sixteen float4 at a time so that the loads overlap.
+__device__ __forceinline__ double exact_dot(float const* q, float const* __restrict__ c)
 {
     float4 const* qp = reinterpret_cast<float4 const*>(q);
     float4 const* cp = reinterpret_cast<float4 const*>(c);
     double acc = 0.0;
-#pragma unroll 8
-    for (int k4 = 0; k4 < 32; ++k4) {
-        float4 const x = qp[k4];
-        float4 const y = cp[k4];
-        acc += (double)x.x * (double)y.x;
-        acc += (double)x.y * (double)y.y;
-        acc += (double)x.z * (double)y.z;
-        acc += (double)x.w * (double)y.w;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        float4 y[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) y[u] = cp[16 * h + u];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            float4 const x = qp[16 * h + u];
+            acc += (double)x.x * (double)y[u].x;
+            acc += (double)x.y * (double)y[u].y;
+            acc += (double)x.z * (double)y[u].z;
+            acc += (double)x.w * (double)y[u].w;
+        }
     }
     return acc;
 }
 
 // Exact score against a uint8 candidate row: element k = (float)c_k * denom, the value the
-// FP32 image holds (same IEEE product).
-__device__ __forceinline__ double exact_dot_u8(float const* __restrict__ q, uint8_t const* __restrict__ c, float d)
+// FP32 image holds (same IEEE product).  The whole row (128 bytes) is loaded at once.
+__device__ __forceinline__ double exact_dot_u8(float const* q, uint8_t const* __restrict__ c, float d)
 {
     float4 const* qp = reinterpret_cast<float4 const*>(q);
     uint4 const* cp = reinterpret_cast<uint4 const*>(c);
+    uint4 w[8];
+#pragma unroll
+    for (int k16 = 0; k16 < 8; ++k16) w[k16] = cp[k16];
     double acc = 0.0;
-#pragma unroll 2
+#pragma unroll
     for (int k16 = 0; k16 < 8; ++k16) {
-        uint4 const w = cp[k16];
-        unsigned int const ws[4] = {w.x, w.y, w.z, w.w};
+        unsigned int const ws[4] = {w[k16].x, w[k16].y, w[k16].z, w[k16].w};
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             float4 const x = qp[4 * k16 + u];
@@ -83,13 +92,23 @@ __device__ __forceinline__ double exact_dot_u8(float const* __restrict__ q, uint
 }
 
 // One query, eight lanes (sub = lane & 7).  part: the query's (split 0, half 0) list; the
-// lists of split s are at part[s * ldq2 + h]; slots: the group's kRescoreSlots ints of LDS.
-// Returns the exact top-2 (all lanes) and the number of candidates scored by this lane.
+// lists of split s are at part[s * ldq2 + h]; slots: the group's kRescoreSlots ints of LDS;
+// qs: the group's 128 floats of LDS, which receive the query row (its loads are issued with
+// the list loads).  Returns the exact top-2 (all lanes) and the number of candidates scored
+// by this lane.
 __device__ __forceinline__ Top2x finish_query(Cand2 const* __restrict__ part, size_t ldq2, int nParts, float nq,
                                               float mo, float winRel, int topK, float const* __restrict__ Qd,
                                               float const* __restrict__ Cd, uint8_t const* __restrict__ Cu8,
-                                              float const* __restrict__ Cdenom, int sub, int* slots, int& nScored)
+                                              float const* __restrict__ Cdenom, int sub, int* slots, float* qs,
+                                              int& nScored)
 {
+    {
+        float4 qv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) qv[u] = reinterpret_cast<float4 const*>(Qd)[sub + 8 * u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) reinterpret_cast<float4*>(qs)[sub + 8 * u] = qv[u];
+    }
     float v[kPerLane];
     int j[kPerLane];
 #pragma unroll
@@ -174,8 +193,12 @@ __device__ __forceinline__ Top2x finish_query(Cand2 const* __restrict__ part, si
     int n = 0;
     for (int k = sub; k < total; k += 8) {
         int const jj = slots[k];
-        float const sc = Cu8 ? (float)exact_dot_u8(Qd, Cu8 + (size_t)jj * kDim, Cdenom[jj])
-                             : (float)exact_dot(Qd, Cd + (size_t)jj * kDim);
+        // an offset the compiler cannot see through: the query row is read from LDS on every
+        // candidate rather than held in 128 registers across the loop
+        int qo = 0;
+        asm volatile("" : "+v"(qo));
+        float const sc = Cu8 ? (float)exact_dot_u8(qs + qo, Cu8 + (size_t)jj * kDim, Cdenom[jj])
+                             : (float)exact_dot(qs + qo, Cd + (size_t)jj * kDim);
         if (sc == sc) s = top2x_merge(s, Top2x{sc, kInit, jj, -1});
         ++n;
     }
@@ -214,6 +237,7 @@ __global__ __launch_bounds__(256) void k_shortlist_finish(float const* __restric
                                                           int* __restrict__ stripes)
 {
     __shared__ int slots[32][kRescoreSlots];
+    __shared__ __attribute__((aligned(16))) float qrow[32][kDim];
     int const sub = threadIdx.x & 7;
     int const g = blockIdx.x * 32 + (threadIdx.x >> 3);
     int n = 0;
@@ -224,7 +248,7 @@ __global__ __launch_bounds__(256) void k_shortlist_finish(float const* __restric
                                      left ? nPartL : nPartR, left ? normL[q] : normR[q],
                                      __uint_as_float(maxNormBits[left ? 1 : 0]), winRel, topK,
                                      (left ? L : R) + (size_t)q * kDim, left ? R : L, nullptr, nullptr, sub,
-                                     slots[threadIdx.x >> 3], n);
+                                     slots[threadIdx.x >> 3], qrow[threadIdx.x >> 3], n);
         if (sub == 0) {
             (left ? colMax : rowMax)[q] = decide_exact(Top2{s.b1, s.b2, s.a1}, minScore, minRatio);
             if (colSecond) (left ? colSecond : rowSecond)[q] = s.a2;
@@ -253,6 +277,7 @@ __global__ __launch_bounds__(256) void k_batch_shortlist_finish(BatchDev b, Cand
                                                                 int* __restrict__ stripes)
 {
     __shared__ int slots[32][kRescoreSlots];
+    __shared__ __attribute__((aligned(16))) float qrow[32][kDim];
     int const sub = threadIdx.x & 7;
     int const g = blockIdx.x * 32 + (threadIdx.x >> 3);
     int n = 0;
@@ -267,7 +292,7 @@ __global__ __launch_bounds__(256) void k_batch_shortlist_finish(BatchDev b, Cand
                                          b.norm[row], __uint_as_float(b.maxNorm[sg.oImg]), winRel, topK,
                                          b.img + (size_t)row * kDim, b.img + cb * kDim,
                                          b.u8 ? b.u8 + cb * kDim : nullptr, b.u8 ? b.denom + cb : nullptr, sub,
-                                         slots[threadIdx.x >> 3], n);
+                                         slots[threadIdx.x >> 3], qrow[threadIdx.x >> 3], n);
             d = decide_exact(Top2{s.b1, s.b2, s.a1}, minScore, minRatio);
         }
         if (sub == 0) sel[g] = d;

@@ -277,37 +277,60 @@ void launch_recheck_exact(float const* L, float const* R, int nL, int nR, int co
 }
 
 // ------------------------------------------------------------------------------ mutual + compaction
-// (k, colMax[k]) is a match iff rowMax[colMax[k]] == k (cu:546-551).  Pass 1: each
-// work-group takes 1024 consecutive left features (4 per thread), compacts its matches in
-// ascending k into its own slice of tmp and writes its count.  Pass 2: one work-group
-// forms the exclusive prefix of the block counts and copies the slices into place, so
-// the output is sorted by k.  Pass 2 also publishes the re-check count and clears the
-// counter for the next call.
-// One launch: every work-group takes 1024 consecutive left features (4 per thread),
-// compacts its matches in ascending k into its own slice of tmp and writes its count; the
-// last work-group to finish (atomic ticket after a device-scope fence) forms the exclusive
-// prefix of the counts and copies the slices into place, so the output is sorted by k.  It
-// also publishes the re-check and rescoring counts and clears the per-call counters.
+// (k, colMax[k]) is a match iff rowMax[colMax[k]] == k (cu:546-551).  One launch, ordered
+// by a decoupled look-back: work-group b takes the 1024 left features [1024 b, 1024 b +
+// 1024) (4 per thread), counts its matches, publishes the count (flag 1) in its status
+// word, then adds up its predecessors' words from the nearest backwards (64 per step, one
+// per lane of wave 0) until it reaches an inclusive prefix (flag 2), publishes its own
+// inclusive prefix and writes its matches in ascending k from its exclusive prefix on, so
+// the output is sorted by k.  A status word is valid for the call whose epoch it carries,
+// so nothing is cleared between calls.  A work-group waits only for lower indices, which
+// are dispatched before it; the wait is bounded (kMutualSpinLimit sleeps), and a work-group
+// that runs out of it stores the epoch in counters[kCntMutualError] (its matches then land
+// below their place, inside the buffer).  The last work-group also publishes the match,
+// re-check and rescoring counts and clears the per-call counters.
+
+// Left feature sampled as sample k of S (evenly spaced over [0, nL)).
+__device__ __forceinline__ int sample_row(int k, int nL, int S) { return (int)(((long long)k * nL) / S); }
+
+constexpr unsigned kMutualSpinLimit = 1u << 20;
+
+__device__ __forceinline__ unsigned long long mutual_word(unsigned epoch, unsigned flag, unsigned value)
+{
+    return ((unsigned long long)epoch << 32) | ((unsigned long long)flag << 30) | value;  // value < 2^30
+}
+
 __global__ __launch_bounds__(256) void k_mutual(int const* __restrict__ colMax, int const* __restrict__ rowMax,
-                                                int nL, int nR, int2* __restrict__ tmp, int* __restrict__ blockCount,
-                                                int* __restrict__ stripes, int2* __restrict__ pairs,
-                                                int* __restrict__ counters)
+                                                int nL, int nR, unsigned long long* __restrict__ status,
+                                                unsigned epoch, int* __restrict__ stripes,
+                                                int2* __restrict__ pairs, int* __restrict__ counters, SampleOut so)
 {
     __shared__ int wsum[4];
-    __shared__ int isLast;
-    __shared__ int offs[1025];  // up to 2^20 left features: 1024 slices
+    __shared__ int sBase;
     int const tid = threadIdx.x;
     int const lane = tid & 63;
     int const wave = tid >> 6;
-    int const i0 = blockIdx.x * 1024 + 4 * tid;
+    int const b = blockIdx.x;
+    int const i0 = b * kMutualSlice + 4 * tid;
+    // all four colMax entries in one load, then all four rowMax gathers (indices clamped to
+    // [0, max(nR - 1, 0)]: in bounds, rowMax holds at least one row per allocation) before
+    // any test: two round trips
     int l[4];
+    if (i0 + 3 < nL) {
+        int4 const c = *reinterpret_cast<int4 const*>(colMax + i0);
+        l[0] = c.x, l[1] = c.y, l[2] = c.z, l[3] = c.w;
+    } else {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) l[u] = (i0 + u < nL) ? colMax[i0 + u] : -1;
+    }
+    int r[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) r[u] = rowMax[min(max(l[u], 0), max(nR - 1, 0))];
     bool m[4];
     int cnt = 0;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        int const i = i0 + u;
-        l[u] = (i < nL) ? colMax[i] : -1;
-        m[u] = (l[u] >= 0) && (l[u] < nR) && (rowMax[l[u]] == i);
+        m[u] = (l[u] >= 0) && (l[u] < nR) && (r[u] == i0 + u);
         cnt += m[u] ? 1 : 0;
     }
     int incl = cnt;
@@ -318,60 +341,86 @@ __global__ __launch_bounds__(256) void k_mutual(int const* __restrict__ colMax, 
     }
     if (lane == 63) wsum[wave] = incl;
     __syncthreads();
-    int off = 0;
-    for (int w = 0; w < wave; ++w) off += wsum[w];
-    int pos = blockIdx.x * 1024 + off + incl - cnt;
+    int off = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        off += (w < wave) ? wsum[w] : 0;
+        total += wsum[w];
+    }
+    if (wave == 0) {
+        if (lane == 0)
+            __hip_atomic_store(&status[b], mutual_word(epoch, b == 0 ? 2u : 1u, (unsigned)total), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        int excl = 0;
+        bool failed = false;
+        int top = b - 1;  // nearest predecessor not yet added (uniform over the wave)
+        unsigned spins = 0;
+        while (top >= 0) {
+            int const j = top - lane;
+            unsigned long long const w =
+                (j >= 0) ? __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                         : mutual_word(epoch, 2u, 0u);  // before work-group 0: an inclusive prefix of 0
+            unsigned const fl = ((unsigned)(w >> 32) == epoch) ? (unsigned)(w >> 30) & 3u : 0u;
+            unsigned long long const ready = __ballot(fl != 0u);
+            unsigned long long const inclusive = __ballot(fl == 2u);
+            int const stop = inclusive ? __ffsll((long long)inclusive) - 1 : 63;  // last lane to add
+            unsigned long long const need = (2ull << stop) - 1ull;                 // lanes 0 .. stop
+            if ((ready & need) == need) {
+                int v = (lane <= stop) ? (int)(w & 0x3fffffffull) : 0;
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+                excl += v;
+                if (inclusive) break;
+                top -= 64;
+            } else {
+                if (++spins > kMutualSpinLimit) {
+                    failed = true;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        if (lane == 0) {
+            if (b > 0)
+                __hip_atomic_store(&status[b], mutual_word(epoch, 2u, (unsigned)(excl + total)), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            if (failed) counters[kCntMutualError] = (int)epoch;
+            sBase = excl;
+        }
+    }
+    __syncthreads();
+    int pos = sBase + off + incl - cnt;
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-        if (m[u]) tmp[pos++] = make_int2(i0 + u, l[u]);
-    if (tid == 0) blockCount[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) isLast = (atomicAdd(&counters[kCntTicket], 1) == (int)gridDim.x - 1);
-    __syncthreads();
-    if (!isLast) return;
-    __threadfence();
-    int const nBlocks = gridDim.x;
-    int rescored = 0;
-    if (tid < kStripes) {
-        rescored = stripes[tid * kStripeStride];
-        stripes[tid * kStripeStride] = 0;
-    }
-    for (int o = 32; o > 0; o >>= 1) rescored += __shfl_xor(rescored, o);  // wave 0 holds the stripes
-    for (int b = tid; b < nBlocks; b += 256) offs[b + 1] = blockCount[b];  // loads in parallel
-    __syncthreads();
-    if (tid == 0) {
-        offs[0] = 0;
-        for (int b = 0; b < nBlocks; ++b) offs[b + 1] += offs[b];
-    }
-    __syncthreads();
-    // every output entry independently: its slice by binary search over the offsets
-    int const total = offs[nBlocks];
-    for (int i = tid; i < total; i += 256) {
-        int lo = 0, hi = nBlocks;  // offs[lo] <= i < offs[hi]
-        while (hi - lo > 1) {
-            int const mid = (lo + hi) >> 1;
-            if (offs[mid] <= i) lo = mid;
-            else hi = mid;
+        if (m[u]) pairs[pos++] = make_int2(i0 + u, l[u]);
+    // work-group 0 (off the look-back chain by now): the sampled learned decisions
+    if (b == 0 && tid < so.S) so.out[tid] = colMax[sample_row(tid, nL, so.S)];  // vector stores
+    if (b != (int)gridDim.x - 1) return;
+    // the last work-group: counts of the call
+    if (wave == 0) {
+        int rescored = 0;
+        if (tid < kStripes) {
+            rescored = stripes[tid * kStripeStride];
+            stripes[tid * kStripeStride] = 0;
         }
-        pairs[i] = tmp[(size_t)lo * 1024 + (i - offs[lo])];
-    }
-    if (tid == 0) {
-        counters[kCntPairs] = offs[nBlocks];
-        counters[kCntFlagDone] = counters[kCntFlag] + counters[kCntFlagR];
-        counters[kCntFlag] = 0;
-        counters[kCntFlagR] = 0;
-        counters[kCntRescoredDone] = rescored;
-        counters[kCntTicket] = 0;
+        for (int o = 32; o > 0; o >>= 1) rescored += __shfl_xor(rescored, o);
+        if (tid == 0) {
+            counters[kCntPairs] = sBase + total;
+            counters[kCntFlagDone] = counters[kCntFlag] + counters[kCntFlagR];
+            counters[kCntFlag] = 0;
+            counters[kCntFlagR] = 0;
+            counters[kCntRescoredDone] = rescored;
+        }
     }
 }
 
-void launch_mutual_compact(int const* colMax, int const* rowMax, int nL, int nR, int2* tmp, int* blockCount,
-                           int* stripes, int2* pairs, int* counters, hipStream_t st)
+void launch_mutual_compact(int const* colMax, int const* rowMax, int nL, int nR, unsigned long long* status,
+                           unsigned epoch, int* stripes, int2* pairs, int* counters, hipStream_t st,
+                           SampleOut const& so)
 {
-    int const nb = (nL + 1023) / 1024;
-    hipLaunchKernelGGL(k_mutual, dim3(nb > 0 ? nb : 1), dim3(256), 0, st, colMax, rowMax, nL, nR, tmp, blockCount,
-                       stripes, pairs, counters);
+    int const nb = (nL + kMutualSlice - 1) / kMutualSlice;
+    hipLaunchKernelGGL(k_mutual, dim3(nb > 0 ? nb : 1), dim3(256), 0, st, colMax, rowMax, nL, nR, status, epoch,
+                       stripes, pairs, counters, so);
 }
 
 // ------------------------------------------------------------------------------ uint8 dequantisation
@@ -405,39 +454,13 @@ __global__ __launch_bounds__(256) void k_sample_gather(float const* __restrict__
     int const lane = threadIdx.x & 63;
     int const k = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (k >= kSampleRows) return;
-    int const i = (k < S) ? (int)(((long long)k * nL) / S) : -1;
+    int const i = (k < S) ? sample_row(k, nL, S) : -1;
     float2 const v = (i >= 0) ? reinterpret_cast<float2 const*>(L + (size_t)i * kDim)[lane] : make_float2(0.f, 0.f);
     reinterpret_cast<float2*>(Ls + (size_t)k * kDim)[lane] = v;
     if (lane == 0) {
         normS[k] = (i >= 0) ? normL[i] : 0.0f;
         idx[k] = i;
     }
-}
-
-__global__ __launch_bounds__(128) void k_sample_compare(int const* __restrict__ exactS, int const* __restrict__ idx,
-                                                        int S, int const* __restrict__ colMax, int* __restrict__ out)
-{
-    __shared__ int red[2][3];
-    int const k = threadIdx.x;
-    int acc = 0, miss = 0, extra = 0;
-    if (k < S) {
-        int const e = exactS[k], d = colMax[idx[k]];
-        acc = e >= 0;
-        miss = (e >= 0 && d != e);
-        extra = (e < 0 && d >= 0);
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        acc += __shfl_xor(acc, o);
-        miss += __shfl_xor(miss, o);
-        extra += __shfl_xor(extra, o);
-    }
-    if ((k & 63) == 0) {
-        red[k >> 6][0] = acc;
-        red[k >> 6][1] = miss;
-        red[k >> 6][2] = extra;
-    }
-    __syncthreads();
-    if (k < 3) out[k] = red[0][k] + red[1][k];
 }
 
 void launch_sample_gather(float const* L, float const* normL, int nL, int S, float* Ls, float* normS, int* idx,
@@ -477,9 +500,5 @@ void launch_sample_decide(Top2 const* colPart, int nParts, int ldc, int S, float
                        maxNormBits, epsRel, minScore, minRatio, dec, flagList, counters);
 }
 
-void launch_sample_compare(int const* exactS, int const* idx, int S, int const* colMax, int* out, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_sample_compare, dim3(1), dim3(128), 0, st, exactS, idx, S, colMax, out);
-}
 
 }  // namespace v3dm

@@ -149,7 +149,8 @@ int v3dm_device_results(v3dm_handle h, const int32_t** d_pairs, const int32_t** 
 /* Per-query decisions of the last call (colMax: nLeft ints, rowMax: nRight ints, -1 = none),
  * the arrays of cu:532-533 after the selection kernels. */
 int v3dm_get_maxima(v3dm_handle h, int32_t* colMax, int32_t* rowMax);
-/* Block until all work the handle enqueued on its stream has finished. */
+/* Block until all work the handle enqueued has finished (its stream, and the side stream
+ * of the learned mode's sampled check). */
 int v3dm_synchronize(v3dm_handle h);
 /* Time the dominant kernel with HIP events on the handle's stream (1 = on, resets sums). */
 int v3dm_set_profiling(v3dm_handle h, int enable);

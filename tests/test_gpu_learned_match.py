@@ -177,6 +177,48 @@ def test_sampled_check_and_fallback(matcher):
     other.close()
 
 
+def test_sampled_check_follows_features(matcher):
+    """The sampled check reads the features on its own stream: after every change of the
+    features its counts are those of the new features.  Expected counts: the oracle's exact
+    decisions of the 128 evenly spaced left features against the learned decisions (read
+    back with the fallback threshold at 0, so that no fallback replaces them)."""
+    m, net = matcher
+    n = 2048
+    sets = []
+    for seed in (31, 32, 33):
+        qL, qR, _ = synth.make_pair(n, n, 0.4, config=1, pair=seed)
+        L, R = synth.dequantise(qL), synth.dequantise(qR)
+        sets.append((L, R, oracle.maxima(L, R, 0.9, 0.8)[0]))
+    other = v3dmatch.SIFT_Matching(0)
+    other.enableRoundingWarning(False)
+    other.allocate(n, n)
+    raw = learned.Network(other)
+    raw.set_params(learned.init_params(5))
+    raw.calibrate_code_scale(np.concatenate([sets[0][0][:1024], sets[0][1][:1024]]))
+    other.set_precision(v3dmatch.PREC_LEARNED)
+    other.set_learned_topk(2)  # a weak shortlist: misses to count
+    other.set_fallback(128, 0.0)
+    rows = (np.arange(128, dtype=np.int64) * n) // 128
+    total_missed = 0
+    for step, k in enumerate((0, 1, 0, 2, 2, 1)):
+        L, R, exact = sets[k]
+        if step == 0 or k != (0, 1, 0, 2, 2, 1)[step - 1]:
+            other.setLeftFeatures(n, L)
+            other.setRightFeatures(n, R)
+        other.find_async(0.9, 0.8)
+        learned_cm, _ = other.get_maxima()
+        st = other.stats()
+        x, d = exact[rows], learned_cm[rows]
+        want_missed = int(np.sum((x >= 0) & (d != x)))
+        say(f"case check after feature change: step {step} set {k}: sampled={st['nSampled']} "
+            f"missed={st['nSampleMissed']} (expected {want_missed})")
+        assert st["nSampled"] == 128 and st["nFallback"] == 0
+        assert st["nSampleMissed"] == want_missed
+        total_missed += want_missed
+    assert total_missed > 0
+    other.close()
+
+
 def test_learned_batch_fallback(matcher):
     """Batch: with the trained network no pair falls back; with an untrained network and a
     single candidate per query, the pairs whose sampled recall falls short get exactly the
