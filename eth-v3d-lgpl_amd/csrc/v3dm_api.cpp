@@ -469,6 +469,14 @@ int v3dm_set_learned_topk(v3dm_handle h, int k)
     return V3DM_OK;
 }
 
+int v3dm_set_learned_margin(v3dm_handle h, float marginRel)
+{
+    if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
+    if (marginRel != marginRel) return fail(h, V3DM_ERR_INVALID, "margin must be a number");
+    h->learnedMargin = marginRel;
+    return V3DM_OK;
+}
+
 int v3dm_set_fallback(v3dm_handle h, int samples, float minRecall)
 {
     if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");

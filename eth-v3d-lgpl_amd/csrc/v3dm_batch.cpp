@@ -127,12 +127,14 @@ struct BatchState {
         float minScore, minRatio, epsRel;
         int version;
         int topK;
+        float margin;
         int samples;
         float minRecall;
         bool operator<(GraphKey const& o) const
         {
-            return std::tie(mode, minScore, minRatio, epsRel, version, topK, samples, minRecall) <
-                   std::tie(o.mode, o.minScore, o.minRatio, o.epsRel, o.version, o.topK, o.samples, o.minRecall);
+            return std::tie(mode, minScore, minRatio, epsRel, version, topK, margin, samples, minRecall) <
+                   std::tie(o.mode, o.minScore, o.minRatio, o.epsRel, o.version, o.topK, o.margin, o.samples,
+                            o.minRecall);
         }
     };
     std::map<GraphKey, hipGraphExec_t> graphs;
@@ -453,10 +455,11 @@ int enqueue_run(v3dm_context* h, BatchState* b, int mode, float minScore, float 
         launch_batch_recheck(d, b->flagList, b->counters, minScore, minRatio, b->sel, 1024, st);
     } else if (form == 1) {
         launch_batch_pass_bf16(d, b->items[1], (int)pl.items.size(), b->counters + 1, b->partB, pl.grid, st);
-        launch_batch_rescore(d, b->partB, kWinRelBf16, 0, minScore, minRatio, b->sel, b->stripes, st);
+        launch_batch_rescore(d, b->partB, kWinRelBf16, 0, INFINITY, minScore, minRatio, b->sel, b->stripes, st);
     } else {
         launch_batch_pass_code(d, b->items[2], (int)pl.items.size(), b->counters + 2, b->partB, pl.grid, st);
-        launch_batch_rescore(d, b->partB, 0.0f, h->learnedTopK, minScore, minRatio, b->sel, b->stripes, st);
+        launch_batch_rescore(d, b->partB, 0.0f, h->learnedTopK, learned_margin(h), minScore, minRatio, b->sel,
+                             b->stripes, st);
         if (h->fallbackSamples > 0) {
             int dev = 0, cus = 256;
             (void)hipGetDevice(&dev);
@@ -855,8 +858,8 @@ int v3dm_batch_match_async(v3dm_handle h, float minScore, float minRatio)
     BCHECK(h, hipEventRecord(b->evA, st));
     if (b->useGraph) {
         BatchState::GraphKey const key{mode, minScore, minRatio, h->epsRel, b->version * 2 + (h->cacheDerived ? 1 : 0),
-                                       h->learnedTopK, form == 2 ? h->fallbackSamples : 0,
-                                       form == 2 ? h->fallbackRecall : 0.0f};
+                                       h->learnedTopK, form == 2 ? learned_margin(h) : 0.0f,
+                                       form == 2 ? h->fallbackSamples : 0, form == 2 ? h->fallbackRecall : 0.0f};
         auto it = b->graphs.find(key);
         if (it == b->graphs.end()) {
             hipGraph_t g = nullptr;

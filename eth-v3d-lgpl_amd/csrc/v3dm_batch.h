@@ -69,7 +69,7 @@ void launch_batch_fused_fp32(BatchDev const& b, int4 const* items, int nItems, i
 // shortlist pass (v3dm_short.hip)
 void launch_batch_pass_bf16(BatchDev const& b, int4 const* items, int nItems, int* queue, Cand2* part, int grid,
                             hipStream_t st);
-void launch_batch_rescore(BatchDev const& b, Cand2 const* part, float winRel, int topK, float minScore,
+void launch_batch_rescore(BatchDev const& b, Cand2 const* part, float winRel, int topK, float margin, float minScore,
                           float minRatio, int* sel, int* stripes, hipStream_t st);
 // learned code pass (v3dm_code.hip)
 void launch_batch_pass_code(BatchDev const& b, int4 const* items, int nItems, int* queue, Cand2* part, int grid,

@@ -22,6 +22,7 @@ struct v3dm_context {
     int precision = V3DM_PREC_FP32;
     float epsRel = v3dm::kEpsRelFp32;  // 0: every query goes to the exact re-check
     int learnedTopK = 6;               // candidates per query rescored in the learned mode
+    float learnedMargin = 0.07f;       // ... and no more than this * scale^2 below the best (< 0: off)
     int fallbackSamples = v3dm::kSampleRows;  // sampled check of learned results (0: off)
     float fallbackRecall = 0.995f;     // sampled recall below this -> the plain form is run
     float lastMinScore = 0.9f, lastMinRatio = 0.8f;
@@ -77,6 +78,14 @@ struct v3dm_context {
 };
 
 namespace v3dm {
+
+// Code-score margin of the learned mode's candidate selection: learnedMargin times the
+// square of the code scale (the code score of a code with itself is about scale^2).
+inline float learned_margin(v3dm_context* h)
+{
+    float const s = learned_code_scale(ctx_learned(h));
+    return (h->learnedMargin < 0.0f || !(s > 0.0f)) ? INFINITY : h->learnedMargin * s * s;
+}
 
 // The mutual check of the handle's next call: its status words and a fresh epoch.
 inline unsigned next_mutual_epoch(v3dm_context* h)

@@ -117,7 +117,7 @@ int enqueue_bf16(v3dm_context* h, float minScore, float minRatio)
         launch_pass_bf16(e->bR, nR, e->bL, nL, pr, e->partR, e->capR, st);
         if ((rc = prof_end(h))) return rc;
         launch_rescore(h->dL, h->dR, nL, nR, e->partL, pl.splits, e->capL, e->partR, pr.splits, e->capR, h->normL,
-                       h->normR, h->maxNormBits, kWinRelBf16, 0, minScore, minRatio,
+                       h->normR, h->maxNormBits, kWinRelBf16, 0, INFINITY, minScore, minRatio,
                        h->colMax, h->rowMax, e->secL, e->secR, h->stripes, st);
     }
     e->haveSecond = work;
@@ -223,7 +223,7 @@ int enqueue_learned(v3dm_context* h, float minScore, float minRatio)
         launch_pass_code2(e->cL, nL, e->cR, nR, pl, e->partL, e->capL, pr, e->partR, e->capR, st);
         if ((rc = prof_end(h))) return rc;
         launch_rescore(h->dL, h->dR, nL, nR, e->partL, pl.splits, e->capL, e->partR, pr.splits, e->capR, h->normL,
-                       h->normR, h->maxNormBits, 0.0f, h->learnedTopK, minScore, minRatio,
+                       h->normR, h->maxNormBits, 0.0f, h->learnedTopK, learned_margin(h), minScore, minRatio,
                        h->colMax, h->rowMax, e->secL, e->secR, h->stripes, st);
         e->haveSecond = true;
         SampleOut so;
@@ -264,7 +264,7 @@ int enqueue_fp8(v3dm_context* h, float minScore, float minRatio)
         launch_pass_fp8(e->f8R, e->eR, nR, e->f8L, e->eL, nL, pr, e->partR, e->capR, st);
         if ((rc = prof_end(h))) return rc;
         launch_rescore(h->dL, h->dR, nL, nR, e->partL, pl.splits, e->capL, e->partR, pr.splits, e->capR, h->normL,
-                       h->normR, h->maxNormBits, kWinRelFp8, 0, minScore, minRatio,
+                       h->normR, h->maxNormBits, kWinRelFp8, 0, INFINITY, minScore, minRatio,
                        h->colMax, h->rowMax, e->secL, e->secR, h->stripes, st);
     }
     e->haveSecond = work;

@@ -96,8 +96,10 @@ __device__ __forceinline__ double exact_dot_u8(float const* q, uint8_t const* __
 // qs: the group's 128 floats of LDS, which receive the query row (its loads are issued with
 // the list loads).  Returns the exact top-2 (all lanes) and the number of candidates scored
 // by this lane.
+template <bool kU8>
 __device__ __forceinline__ Top2x finish_query(Cand2 const* __restrict__ part, size_t ldq2, int nParts, float nq,
-                                              float mo, float winRel, int topK, float const* __restrict__ Qd,
+                                              float mo, float winRel, int topK, float margin,
+                                              float const* __restrict__ Qd,
                                               float const* __restrict__ Cd, uint8_t const* __restrict__ Cu8,
                                               float const* __restrict__ Cdenom, int sub, int* slots, float* qs,
                                               int& nScored)
@@ -124,11 +126,12 @@ __device__ __forceinline__ Top2x finish_query(Cand2 const* __restrict__ part, si
     float thr;
     if (topK > 0) {
         // tournament: round k takes the largest remaining value of the group; thr = the
-        // value taken in round topK (or the last finite one)
+        // value taken in round topK, or the last one within `margin` of the first (or the
+        // last finite one)
         float w[kPerLane];
 #pragma unroll
         for (int u = 0; u < kPerLane; ++u) w[u] = v[u];
-        float kth = -INFINITY;
+        float kth = -INFINITY, top = INFINITY;
         for (int k = 0; k < topK; ++k) {
             float m = w[0];
 #pragma unroll
@@ -138,6 +141,8 @@ __device__ __forceinline__ Top2x finish_query(Cand2 const* __restrict__ part, si
             g = fmaxf(g, __shfl_xor(g, 2));
             g = fmaxf(g, __shfl_xor(g, 4));
             if (g == -INFINITY) break;  // uniform over the group
+            if (k == 0) top = g;
+            else if (g < top - margin) break;  // uniform over the group
             kth = g;
             // the lowest lane holding g removes one copy
             int own = (m == g) ? sub : 8;
@@ -197,7 +202,7 @@ __device__ __forceinline__ Top2x finish_query(Cand2 const* __restrict__ part, si
         // candidate rather than held in 128 registers across the loop
         int qo = 0;
         asm volatile("" : "+v"(qo));
-        float const sc = Cu8 ? (float)exact_dot_u8(qs + qo, Cu8 + (size_t)jj * kDim, Cdenom[jj])
+        float const sc = kU8 ? (float)exact_dot_u8(qs + qo, Cu8 + (size_t)jj * kDim, Cdenom[jj])
                              : (float)exact_dot(qs + qo, Cd + (size_t)jj * kDim);
         if (sc == sc) s = top2x_merge(s, Top2x{sc, kInit, jj, -1});
         ++n;
@@ -231,7 +236,7 @@ __global__ __launch_bounds__(256) void k_shortlist_finish(float const* __restric
                                                           int ldqR, float const* __restrict__ normL,
                                                           float const* __restrict__ normR,
                                                           unsigned int const* __restrict__ maxNormBits, float winRel,
-                                                          int topK, float minScore, float minRatio,
+                                                          int topK, float margin, float minScore, float minRatio,
                                                           int* __restrict__ colMax, int* __restrict__ rowMax,
                                                           int* __restrict__ colSecond, int* __restrict__ rowSecond,
                                                           int* __restrict__ stripes)
@@ -244,9 +249,9 @@ __global__ __launch_bounds__(256) void k_shortlist_finish(float const* __restric
     if (g < nL + nR) {  // uniform over the 8 lanes of a query
         bool const left = g < nL;
         int const q = left ? g : g - nL;
-        Top2x const s = finish_query((left ? partL : partR) + (size_t)q * 2, 2 * (size_t)(left ? ldqL : ldqR),
+        Top2x const s = finish_query<false>((left ? partL : partR) + (size_t)q * 2, 2 * (size_t)(left ? ldqL : ldqR),
                                      left ? nPartL : nPartR, left ? normL[q] : normR[q],
-                                     __uint_as_float(maxNormBits[left ? 1 : 0]), winRel, topK,
+                                     __uint_as_float(maxNormBits[left ? 1 : 0]), winRel, topK, margin,
                                      (left ? L : R) + (size_t)q * kDim, left ? R : L, nullptr, nullptr, sub,
                                      slots[threadIdx.x >> 3], qrow[threadIdx.x >> 3], n);
         if (sub == 0) {
@@ -259,22 +264,25 @@ __global__ __launch_bounds__(256) void k_shortlist_finish(float const* __restric
 
 void launch_rescore(float const* L, float const* R, int nL, int nR, Cand2 const* partL, int nPartL, int ldqL,
                     Cand2 const* partR, int nPartR, int ldqR, float const* normL, float const* normR,
-                    unsigned int const* maxNormBits, float winRel, int topK, float minScore, float minRatio,
-                    int* colMax, int* rowMax, int* colSecond, int* rowSecond, int* stripes, hipStream_t st)
+                    unsigned int const* maxNormBits, float winRel, int topK, float margin, float minScore,
+                    float minRatio, int* colMax, int* rowMax, int* colSecond, int* rowSecond, int* stripes,
+                    hipStream_t st)
 {
     int const n = nL + nR;
     if (n <= 0) return;
     hipLaunchKernelGGL(k_shortlist_finish, dim3((n + 31) / 32), dim3(256), 0, st, L, R, nL, nR, partL, nPartL, ldqL,
-                       partR, nPartR, ldqR, normL, normR, maxNormBits, winRel, topK, minScore, minRatio, colMax,
-                       rowMax, colSecond, rowSecond, stripes);
+                       partR, nPartR, ldqR, normL, normR, maxNormBits, winRel, topK, margin, minScore, minRatio,
+                       colMax, rowMax, colSecond, rowSecond, stripes);
 }
 
 // ------------------------------------------------------------------------------ batch
 // Query-space rows; padding rows decide -1.
-__global__ __launch_bounds__(256) void k_batch_shortlist_finish(BatchDev b, Cand2 const* __restrict__ part,
-                                                                float winRel, int topK, float minScore,
-                                                                float minRatio, int* __restrict__ sel,
-                                                                int* __restrict__ stripes)
+// kU8: the candidate rows are read as the uint8 image (128 bytes per row) instead of the
+// FP32 one; one kernel per row format keeps each at its own register count.
+template <bool kU8>
+__device__ __forceinline__ void batch_finish(BatchDev const& b, Cand2 const* __restrict__ part, float winRel,
+                                             int topK, float margin, float minScore, float minRatio,
+                                             int* __restrict__ sel, int* __restrict__ stripes)
 {
     __shared__ int slots[32][kRescoreSlots];
     __shared__ __attribute__((aligned(16))) float qrow[32][kDim];
@@ -288,11 +296,11 @@ __global__ __launch_bounds__(256) void k_batch_shortlist_finish(BatchDev b, Cand
         if (q < sg.n && sg.nO > 0 && sg.nParts > 0) {
             int const row = b.imgBase[sg.img] + q;
             size_t const cb = (size_t)b.imgBase[sg.oImg];
-            Top2x const s = finish_query(part + ((size_t)sg.partOff + q) * 2, 2 * (size_t)sg.ld, sg.nParts,
-                                         b.norm[row], __uint_as_float(b.maxNorm[sg.oImg]), winRel, topK,
-                                         b.img + (size_t)row * kDim, b.img + cb * kDim,
-                                         b.u8 ? b.u8 + cb * kDim : nullptr, b.u8 ? b.denom + cb : nullptr, sub,
-                                         slots[threadIdx.x >> 3], qrow[threadIdx.x >> 3], n);
+            Top2x const s = finish_query<kU8>(part + ((size_t)sg.partOff + q) * 2, 2 * (size_t)sg.ld, sg.nParts,
+                                              b.norm[row], __uint_as_float(b.maxNorm[sg.oImg]), winRel, topK, margin,
+                                              b.img + (size_t)row * kDim, b.img + cb * kDim,
+                                              kU8 ? b.u8 + cb * kDim : nullptr, kU8 ? b.denom + cb : nullptr, sub,
+                                              slots[threadIdx.x >> 3], qrow[threadIdx.x >> 3], n);
             d = decide_exact(Top2{s.b1, s.b2, s.a1}, minScore, minRatio);
         }
         if (sub == 0) sel[g] = d;
@@ -300,12 +308,34 @@ __global__ __launch_bounds__(256) void k_batch_shortlist_finish(BatchDev b, Cand
     add_stripe(n, stripes);
 }
 
-void launch_batch_rescore(BatchDev const& b, Cand2 const* part, float winRel, int topK, float minScore,
+// many work-groups per query row set: six waves per SIMD (the LDS of six work-groups fits a
+// CU), which the uint8 form reaches without spilling
+__global__ __launch_bounds__(256, 6) void k_batch_shortlist_finish_u8(BatchDev b, Cand2 const* __restrict__ part,
+                                                                      float winRel, int topK, float margin,
+                                                                      float minScore, float minRatio,
+                                                                      int* __restrict__ sel, int* __restrict__ stripes)
+{
+    batch_finish<true>(b, part, winRel, topK, margin, minScore, minRatio, sel, stripes);
+}
+
+__global__ __launch_bounds__(256) void k_batch_shortlist_finish(BatchDev b, Cand2 const* __restrict__ part,
+                                                                float winRel, int topK, float margin, float minScore,
+                                                                float minRatio, int* __restrict__ sel,
+                                                                int* __restrict__ stripes)
+{
+    batch_finish<false>(b, part, winRel, topK, margin, minScore, minRatio, sel, stripes);
+}
+
+void launch_batch_rescore(BatchDev const& b, Cand2 const* part, float winRel, int topK, float margin, float minScore,
                           float minRatio, int* sel, int* stripes, hipStream_t st)
 {
     if (b.qRows <= 0) return;
-    hipLaunchKernelGGL(k_batch_shortlist_finish, dim3((b.qRows + 31) / 32), dim3(256), 0, st, b, part, winRel, topK,
-                       minScore, minRatio, sel, stripes);
+    if (b.u8)
+        hipLaunchKernelGGL(k_batch_shortlist_finish_u8, dim3((b.qRows + 31) / 32), dim3(256), 0, st, b, part, winRel,
+                           topK, margin, minScore, minRatio, sel, stripes);
+    else
+        hipLaunchKernelGGL(k_batch_shortlist_finish, dim3((b.qRows + 31) / 32), dim3(256), 0, st, b, part, winRel,
+                           topK, margin, minScore, minRatio, sel, stripes);
 }
 
 }  // namespace v3dm

@@ -42,11 +42,15 @@ void launch_pass_bf16(__bf16 const* Q, int nQ, __bf16 const* C, int nC, ShortPla
 void launch_to_fp8(float const* X, int rows, unsigned char* Y, unsigned char* E, hipStream_t st);
 void launch_pass_fp8(unsigned char const* Q, unsigned char const* EQ, int nQ, unsigned char const* C,
                      unsigned char const* EC, int nC, ShortPlan const& p, Cand2* part, int ldq, hipStream_t st);
-// Exact rescoring (v3dm_rescore.hip); colSecond / rowSecond may be NULL.
+// Exact rescoring (v3dm_rescore.hip); colSecond / rowSecond may be NULL.  topK = 0: the
+// candidates within winRel * |q| * max|c| of the second best list value (certified forms);
+// topK > 0: the topK best by list value, stopping at the first one more than `margin`
+// below the best (INFINITY: no margin).
 void launch_rescore(float const* L, float const* R, int nL, int nR, Cand2 const* partL, int nPartL, int ldqL,
                     Cand2 const* partR, int nPartR, int ldqR, float const* normL, float const* normR,
-                    unsigned int const* maxNormBits, float winRel, int topK, float minScore, float minRatio,
-                    int* colMax, int* rowMax, int* colSecond, int* rowSecond, int* stripes, hipStream_t st);
+                    unsigned int const* maxNormBits, float winRel, int topK, float margin, float minScore,
+                    float minRatio, int* colMax, int* rowMax, int* colSecond, int* rowSecond, int* stripes,
+                    hipStream_t st);
 // learned code pass (v3dm_code.hip)
 void launch_pass_code(signed char const* Q, int nQ, signed char const* C, int nC, ShortPlan const& p, Cand2* part,
                       int ldq, hipStream_t st);

@@ -56,6 +56,7 @@ SIGNATURES = {
     "v3dm_set_certification": (_I, [_P, _F]),
     "v3dm_set_stream": (_I, [_P, _P]),
     "v3dm_set_learned_topk": (_I, [_P, _I]),
+    "v3dm_set_learned_margin": (_I, [_P, _F]),
     "v3dm_set_derived_cache": (_I, [_P, _I]),
     "v3dm_set_fallback": (_I, [_P, _I, _F]),
     "v3dm_set_left_device": (_I, [_P, _I, _FP]),

@@ -155,6 +155,11 @@ class SIFT_Matching:
     def set_learned_topk(self, k):
         self._check(self._lib.v3dm_set_learned_topk(self._h, int(k)))
 
+    def set_learned_margin(self, margin_rel):
+        """Learned mode: candidates more than margin_rel * scale^2 below the best code score
+        are not rescored (negative: off)."""
+        self._check(self._lib.v3dm_set_learned_margin(self._h, C.c_float(margin_rel)))
+
     def set_fallback(self, samples=128, min_recall=0.995):
         self._check(self._lib.v3dm_set_fallback(self._h, int(samples), C.c_float(min_recall)))
 

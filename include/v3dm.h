@@ -113,6 +113,10 @@ int v3dm_set_certification(v3dm_handle h, float epsRel);
 /* Candidates per query scored exactly in the learned mode (the best k by code score;
  * default 6, at most 32). */
 int v3dm_set_learned_topk(v3dm_handle h, int k);
+/* ... of which those whose code score is more than marginRel * scale^2 below the best
+ * candidate's are not rescored (scale: the network's code scale; scale^2 is about the code
+ * score of a code with itself).  Default 0.07; a negative value turns the margin off. */
+int v3dm_set_learned_margin(v3dm_handle h, float marginRel);
 /* 1 (default): the bf16 / fp8 / code copies of the features are kept between calls while
  * the features are unchanged; 0: every call derives them again (so that a call's time
  * includes that work). */

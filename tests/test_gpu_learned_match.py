@@ -108,10 +108,29 @@ def test_topk_tradeoff(matcher):
     for k in (2, 4, 8, 16):
         m.set_learned_topk(k)
         res[k] = _recall(m, L, R, want)
-    m.set_learned_topk(8)
+    m.set_learned_topk(6)
     say("case top-k: " + " ".join(f"k={k}: recall {r[0]:.5f} rescored {r[2]['nRescored']}" for k, r in res.items()))
     assert res[16][0] >= res[2][0]
     assert res[16][2]["nRescored"] > res[2][2]["nRescored"]
+
+
+def test_margin_tradeoff(matcher):
+    """The code-score margin prunes the rescored candidates (fewer with a smaller margin)
+    and, at its default (0.07 scale^2), keeps recall and precision >= 99.5%."""
+    m, net = matcher
+    n = 4096
+    qL, qR, _ = synth.make_pair(n, n, 0.4, config=1, pair=14)
+    L, R = synth.dequantise(qL), synth.dequantise(qR)
+    want = _pairs_set(oracle.find_putative_matches(L, R, 0.9, 0.8))
+    res = {}
+    for margin in (-1.0, 0.07, 0.02):
+        m.set_learned_margin(margin)
+        res[margin] = _recall(m, L, R, want)
+    m.set_learned_margin(0.07)
+    say("case margin: " + " ".join(f"margin={k}: recall {r[0]:.5f} precision {r[1]:.5f} rescored {r[2]['nRescored']}"
+                                   for k, r in res.items()))
+    assert res[-1.0][2]["nRescored"] > res[0.07][2]["nRescored"] > res[0.02][2]["nRescored"]
+    assert res[0.07][0] >= 0.995 and res[0.07][1] >= 0.995
 
 
 def test_learned_batch(matcher):
