@@ -240,6 +240,31 @@ def bench_voctree(steps, warmup, cpu, branch=50, depth=3, n=1_000_000):
     return res
 
 
+def bench_ransac(steps, n_in=3300, n_out=1400):
+    """The match lists' consumer (computeRobustOrientationMLE, host code, one thread): a
+    synthetic two-view scene with about as many correspondences as the 16384-feature pair's
+    match list has (3300 inliers with 0.5 px noise, 1400 outliers), 1000 samples, 2 px
+    threshold, iterative refinement, seeds 1..steps."""
+    from v3dmatch import ransac, synth
+    corrs, K, R, T, is_in = synth.two_view_scene(n_in, n_out, seed=11)
+    res = None
+    t0 = time.perf_counter()
+    for s in range(1, steps + 1):
+        res = ransac.robust_orientation(corrs, K, K, 2.0, 1000, True, True, seed=s)
+    dt = (time.perf_counter() - t0) / steps
+    got = np.zeros(len(corrs), bool)
+    got[res["inliers"]] = True
+    Rr = res["rotation"]
+    rot = float(np.degrees(np.arccos(np.clip((np.trace(Rr.T @ R) - 1.0) / 2.0, -1.0, 1.0))))
+    tdir = float(np.degrees(np.arccos(np.clip(res["translation"] @ T, -1.0, 1.0))))
+    return {"workload": f"{n_in + n_out} correspondences ({n_in} inliers, 0.5 px noise, {n_out} outliers), "
+                        "1000 samples, 2 px, refinement on; host code, one thread",
+            "ms_per_call": dt * 1e3, "samples_drawn": res["samples"], "hypotheses": res["hypotheses"],
+            "rotation_error_deg": rot, "direction_error_deg": tdir,
+            "inlier_recall": float((got & is_in).sum() / is_in.sum()),
+            "inlier_precision": float((got & is_in).sum() / max(got.sum(), 1))}
+
+
 def bench_guided(L, R, steps, warmup):
     """Guided matching (runGuidedHomographyMatching) on the single pair: copies sit at H p +
     noise, distractors anywhere; each call uploads the positions, masks, rescores the pair
@@ -412,6 +437,8 @@ def main():
         out["guided"] = bench_guided(L, R, 10, 2)
         log("bench: vocabulary tree")
         out["voctree"] = bench_voctree(10, 2, not args.no_cpu)
+        log("bench: robust orientation (host)")
+        out["ransac"] = bench_ransac(5)
     print(json.dumps(out), flush=True)
     return 0
 

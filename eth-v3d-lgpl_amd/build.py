@@ -26,7 +26,7 @@ HIP_FLAGS = [
 ]
 HIP_SOURCES = ["v3dm_fp32.hip", "v3dm_select.hip", "v3dm_short.hip", "v3dm_api.cpp", "v3dm_ext.cpp", "v3dm_batch.cpp",
                "v3dm_learned.cpp", "v3dm_features.cpp",
-               "v3dm_voctree.cpp"]
+               "v3dm_voctree.cpp", "v3dm_ransac.cpp"]
 
 
 def _run(cmd):

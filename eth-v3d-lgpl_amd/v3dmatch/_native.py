@@ -57,6 +57,11 @@ SIGNATURES = {
     "v3dm_set_stream": (_I, [_P, _P]),
     "v3dm_set_learned_topk": (_I, [_P, _I]),
     "v3dm_set_learned_margin": (_I, [_P, _F]),
+    "v3dm_sampson_errors": (_I, [_FP, _FP, _I, _FP]),
+    "v3dm_essentials_from_five": (_I, [_FP, _FP, _FP, _FP, _FP, _IP]),
+    "v3dm_relative_pose_from_essential": (_I, [_FP, _I, _FP, _FP, _FP, _FP, _FP, _FP, _IP]),
+    "v3dm_refine_relative_pose": (_I, [_FP, _I, _FP, _FP, _FP, _FP, C.c_double]),
+    "v3dm_robust_orientation": (_I, [_FP, _I, _FP, _FP, C.c_double, _I, _I, _I, C.c_uint64, _P, _FP]),
     "v3dm_set_derived_cache": (_I, [_P, _I]),
     "v3dm_set_fallback": (_I, [_P, _I, _F]),
     "v3dm_set_left_device": (_I, [_P, _I, _FP]),

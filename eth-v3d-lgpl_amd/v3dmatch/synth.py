@@ -98,3 +98,27 @@ def make_batch(n_pairs, lo=500, hi=16000, frac_lo=0.3, frac_hi=0.6, config=3, ba
         row += int(nR[p])
     left = np.arange(0, 2 * n_pairs, 2, dtype=np.int32)
     return counts, desc, left, left + 1, fr
+
+
+def two_view_scene(n_in, n_out, seed=0, noise_px=0.5, focal=800.0, size=(640, 480)):
+    """Correspondences (n_in + n_out, 4) of a synthetic two-view scene in pixels (left u, v,
+    right u, v), shuffled: n_in points in front of both cameras with Gaussian pixel noise
+    and n_out uniform outliers.  Returns (corrs, K, R, T, inlier mask); x2 ~ R x1 + T."""
+    rng = np.random.default_rng(seed)
+    K = np.array([[focal, 0.0, size[0] / 2.0], [0.0, focal, size[1] / 2.0], [0.0, 0.0, 1.0]])
+    w = np.array([0.05, -0.2, 0.03]) + 0.02 * rng.standard_normal(3)
+    th = np.linalg.norm(w)
+    k = w / th
+    S = np.array([[0.0, -k[2], k[1]], [k[2], 0.0, -k[0]], [-k[1], k[0], 0.0]])
+    R = np.eye(3) + np.sin(th) * S + (1.0 - np.cos(th)) * S @ S
+    T = np.array([1.0, 0.1, 0.2]) + 0.05 * rng.standard_normal(3)
+    T /= np.linalg.norm(T)
+    X = np.c_[rng.uniform(-3, 3, (n_in, 2)), rng.uniform(4, 12, n_in)]
+    u1 = X @ K.T
+    u2 = (X @ R.T + T) @ K.T
+    u1 = u1[:, :2] / u1[:, 2:3] + rng.normal(0, noise_px, (n_in, 2))
+    u2 = u2[:, :2] / u2[:, 2:3] + rng.normal(0, noise_px, (n_in, 2))
+    o = rng.uniform([0, 0, 0, 0], [size[0], size[1], size[0], size[1]], (n_out, 4))
+    corrs = np.r_[np.c_[u1, u2], o]
+    perm = rng.permutation(len(corrs))
+    return corrs[perm], K, R, T, perm < n_in

@@ -19,16 +19,21 @@ def declared(header):
     return sorted(set(re.findall(r"\b(v3dm_[a-z0-9_]+)\s*\(", text)))
 
 
-def test_every_declared_symbol_is_exported():
+HEADERS = ["v3dm.h", "v3dm_ransac.h"]
+
+
+@pytest.mark.parametrize("header,at_least", [("v3dm.h", 20), ("v3dm_ransac.h", 5)])
+def test_every_declared_symbol_is_exported(header, at_least):
     lib = C.CDLL(_native.lib_path())
-    names = declared("v3dm.h")
-    assert len(names) >= 20
+    names = declared(header)
+    assert len(names) >= at_least
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
 
 
-def test_python_binding_covers_the_header():
-    assert set(declared("v3dm.h")) <= set(_native.SIGNATURES)
+@pytest.mark.parametrize("header", HEADERS)
+def test_python_binding_covers_the_header(header):
+    assert set(declared(header)) <= set(_native.SIGNATURES)
 
 
 def test_create_without_gpu_reports_an_error():
@@ -55,10 +60,11 @@ def test_null_handle_calls_fail_cleanly():
     lib.v3dm_destroy(None)
 
 
+@pytest.mark.parametrize("header", HEADERS)
 @pytest.mark.parametrize("compiler,std,src", [("gcc", "-std=c99", "x.c"), ("g++", "-std=c++11", "x.cpp")])
-def test_header_compiles_standalone(tmp_path, compiler, std, src):
+def test_header_compiles_standalone(tmp_path, header, compiler, std, src):
     f = tmp_path / src
-    f.write_text('#include "v3dm.h"\nint main(void) { v3dm_handle h = 0; (void)h; return 0; }\n')
+    f.write_text(f'#include "{header}"\nint main(void) {{ return 0; }}\n')
     r = subprocess.run([compiler, std, "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
                         str(f)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
