@@ -434,8 +434,10 @@ void enqueue_derived(v3dm_context* h, BatchState* b, int form)
     if (form == 1) {
         launch_to_bf16(b->img, b->imgRows, b->imgB, st);
     } else if (form == 2) {
-        launch_mlp_codes_f32(b->img, b->imgRows, *learned_weights(ctx_learned(h)), learned_code_scale(ctx_learned(h)),
-                             b->codes, st);
+        MlpWeights const& w = *learned_weights(ctx_learned(h));
+        float const scale = learned_code_scale(ctx_learned(h));
+        if (b->haveU8) launch_mlp_codes_u8(b->imgU8, b->denom, b->imgRows, w, scale, b->codes, st);
+        else launch_mlp_codes_f32(b->img, b->imgRows, w, scale, b->codes, st);
     }
 }
 

@@ -31,6 +31,10 @@ void launch_mlp_forward(__bf16 const* X, int Rp, MlpWeights const& w, __bf16* HT
 // readable), 32 bytes per row in the kernel's own (fixed) order of the code dimensions.
 void launch_mlp_codes_f32(float const* X, int rows128, MlpWeights const& w, float scale, signed char* codes,
                           hipStream_t st);
+// uint8 rows with their dequantisation scales (element = (float)q * denom[row]): the same
+// codes as the FP32 rows they dequantise to
+void launch_mlp_codes_u8(uint8_t const* Q, float const* denom, int rows128, MlpWeights const& w, float scale,
+                         signed char* codes, hipStream_t st);
 // two feature sets in one launch (row counts multiples of 128; either may be 0)
 void launch_mlp_codes2_f32(float const* X0, int rows0, signed char* codes0, float const* X1, int rows1,
                            signed char* codes1, MlpWeights const& w, float scale, hipStream_t st);

@@ -516,12 +516,75 @@ __global__ __launch_bounds__(256) void k_mlp_adam(float const* __restrict__ part
 //     element order: a fixed permutation of the code dimensions, the same for every row).
 constexpr int kW1Stride = 136;  // bf16 per LDS row of W1 (128 + 8)
 
-__global__ __launch_bounds__(256, 2) void k_mlp_codes(float const* __restrict__ X0, int nBlocks0,
-                                                      signed char* __restrict__ codes0, float const* __restrict__ X1,
-                                                      int nBlocks1, signed char* __restrict__ codes1,
-                                                      __bf16 const* __restrict__ W1b, float const* __restrict__ b1,
-                                                      __bf16 const* __restrict__ W2p, float const* __restrict__ b2,
-                                                      float scale)
+// Rows of one feature set: FP32 rows, or uint8 rows with their dequantisation scale
+// (element k = (float)q_k * denom, the FP32 value the batch image holds).
+struct MlpRows {
+    float const* f;
+    uint8_t const* q;
+    float const* denom;
+};
+
+// The inputs of one 128-row block for lane (c32, h): elements 16 ks + 8 h + 0..7, ks < 8.
+template <bool kU8>
+struct MlpRaw;
+template <>
+struct MlpRaw<false> {
+    float4 v[16];
+};
+template <>
+struct MlpRaw<true> {
+    uint2 v[8];
+    float d;
+};
+
+__device__ __forceinline__ void mlp_load(MlpRows const& in, size_t row, int h, MlpRaw<false>& r)
+{
+    float4 const* xp = reinterpret_cast<float4 const*>(in.f + row * kDim);
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+        r.v[2 * ks] = xp[4 * ks + 2 * h];
+        r.v[2 * ks + 1] = xp[4 * ks + 2 * h + 1];
+    }
+}
+
+__device__ __forceinline__ void mlp_load(MlpRows const& in, size_t row, int h, MlpRaw<true>& r)
+{
+    uint8_t const* qp = in.q + row * kDim + 8 * h;
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) r.v[ks] = *reinterpret_cast<uint2 const*>(qp + 16 * ks);
+    r.d = in.denom[row];
+}
+
+__device__ __forceinline__ void mlp_convert(MlpRaw<false> const& r, v8bf (&xb)[8])
+{
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+        float4 const u = r.v[2 * ks], v = r.v[2 * ks + 1];
+        xb[ks] = v8bf{(__bf16)u.x, (__bf16)u.y, (__bf16)u.z, (__bf16)u.w,
+                      (__bf16)v.x, (__bf16)v.y, (__bf16)v.z, (__bf16)v.w};
+    }
+}
+
+__device__ __forceinline__ void mlp_convert(MlpRaw<true> const& r, v8bf (&xb)[8])
+{
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+        unsigned int const a = r.v[ks].x, b = r.v[ks].y;
+        xb[ks] = v8bf{(__bf16)dequant_u8_value(a & 0xffu, r.d), (__bf16)dequant_u8_value((a >> 8) & 0xffu, r.d),
+                      (__bf16)dequant_u8_value((a >> 16) & 0xffu, r.d), (__bf16)dequant_u8_value(a >> 24, r.d),
+                      (__bf16)dequant_u8_value(b & 0xffu, r.d), (__bf16)dequant_u8_value((b >> 8) & 0xffu, r.d),
+                      (__bf16)dequant_u8_value((b >> 16) & 0xffu, r.d), (__bf16)dequant_u8_value(b >> 24, r.d)};
+    }
+}
+
+// kPrefetch: the next block's inputs are loaded while the current block is computed (the
+// uint8 rows take 17 registers, few enough to hold a second set).
+template <bool kU8, bool kPrefetch>
+__device__ __forceinline__ void mlp_codes_body(MlpRows const& in0, int nBlocks0, signed char* __restrict__ codes0,
+                                               MlpRows const& in1, int nBlocks1, signed char* __restrict__ codes1,
+                                               __bf16 const* __restrict__ W1b, float const* __restrict__ b1,
+                                               __bf16 const* __restrict__ W2p, float const* __restrict__ b2,
+                                               float scale)
 {
     __shared__ __attribute__((aligned(16))) __bf16 W1s[kHid * kW1Stride];
     __shared__ __attribute__((aligned(16))) float b1s[kHid];
@@ -530,6 +593,14 @@ __global__ __launch_bounds__(256, 2) void k_mlp_codes(float const* __restrict__ 
     int const wave = tid >> 6;
     int const h = lane >> 5;
     int const c32 = lane & 31;
+    int const nBlocks = nBlocks0 + nBlocks1;
+    // block blk -> (feature set, row of this lane)
+    auto rowOf = [&](int blk) -> size_t {
+        return (size_t)((blk >= nBlocks0 ? blk - nBlocks0 : blk) * 128 + wave * 32 + c32);
+    };
+    MlpRaw<kU8> raw;
+    if (kPrefetch && (int)blockIdx.x < nBlocks)
+        mlp_load((int)blockIdx.x >= nBlocks0 ? in1 : in0, rowOf(blockIdx.x), h, raw);
     // W1 -> LDS: 256 rows x 16 chunks of 16 bytes; all sixteen loads of a thread are issued
     // before the first store, so their latencies overlap
     {
@@ -552,21 +623,16 @@ __global__ __launch_bounds__(256, 2) void k_mlp_codes(float const* __restrict__ 
     for (int r = 0; r < 16; ++r) bz[r] = b2[8 * (r >> 2) + 4 * h + (r & 3)];
 
     // row blocks of the first feature set, then of the second
-    for (int blk = blockIdx.x; blk < nBlocks0 + nBlocks1; blk += gridDim.x) {
+    for (int blk = blockIdx.x; blk < nBlocks; blk += gridDim.x) {
         bool const second = blk >= nBlocks0;
-        float const* __restrict__ X = second ? X1 : X0;
         signed char* __restrict__ codes = second ? codes1 : codes0;
-        int const row = (second ? blk - nBlocks0 : blk) * 128 + wave * 32 + c32;
+        size_t const row = rowOf(blk);
+        if (!kPrefetch) mlp_load(second ? in1 : in0, row, h, raw);
         v8bf xb[8];
-        {
-            float4 const* xp = reinterpret_cast<float4 const*>(X + (size_t)row * kDim);
-#pragma unroll
-            for (int ks = 0; ks < 8; ++ks) {
-                float4 const u = xp[4 * ks + 2 * h];
-                float4 const v = xp[4 * ks + 2 * h + 1];
-                xb[ks] = v8bf{(__bf16)u.x, (__bf16)u.y, (__bf16)u.z, (__bf16)u.w,
-                              (__bf16)v.x, (__bf16)v.y, (__bf16)v.z, (__bf16)v.w};
-            }
+        mlp_convert(raw, xb);
+        if (kPrefetch) {
+            int const nxt = blk + (int)gridDim.x;
+            if (nxt < nBlocks) mlp_load(nxt >= nBlocks0 ? in1 : in0, rowOf(nxt), h, raw);
         }
         v16f z = v16f{};
 #pragma unroll 2
@@ -609,8 +675,31 @@ __global__ __launch_bounds__(256, 2) void k_mlp_codes(float const* __restrict__ 
             }
             w[g] = p;
         }
-        *reinterpret_cast<uint4*>(codes + (size_t)row * kCode + 16 * h) = make_uint4(w[0], w[1], w[2], w[3]);
+        *reinterpret_cast<uint4*>(codes + row * kCode + 16 * h) = make_uint4(w[0], w[1], w[2], w[3]);
     }
+}
+
+__global__ __launch_bounds__(256, 2) void k_mlp_codes(float const* __restrict__ X0, int nBlocks0,
+                                                      signed char* __restrict__ codes0, float const* __restrict__ X1,
+                                                      int nBlocks1, signed char* __restrict__ codes1,
+                                                      __bf16 const* __restrict__ W1b, float const* __restrict__ b1,
+                                                      __bf16 const* __restrict__ W2p, float const* __restrict__ b2,
+                                                      float scale)
+{
+    mlp_codes_body<false, false>(MlpRows{X0, nullptr, nullptr}, nBlocks0, codes0, MlpRows{X1, nullptr, nullptr},
+                                 nBlocks1, codes1, W1b, b1, W2p, b2, scale);
+}
+
+// uint8 rows (the batch image): the same values as the FP32 rows they dequantise to, read
+// at 128 bytes per row, the next block's rows loaded during the current block
+__global__ __launch_bounds__(256, 2) void k_mlp_codes_u8(uint8_t const* __restrict__ Q, float const* __restrict__ denom,
+                                                         int nBlocks, signed char* __restrict__ codes,
+                                                         __bf16 const* __restrict__ W1b, float const* __restrict__ b1,
+                                                         __bf16 const* __restrict__ W2p, float const* __restrict__ b2,
+                                                         float scale)
+{
+    MlpRows const in{nullptr, Q, denom};
+    mlp_codes_body<true, true>(in, nBlocks, codes, in, 0, codes, W1b, b1, W2p, b2, scale);
 }
 
 void launch_mlp_codes2_f32(float const* X0, int rows0, signed char* codes0, float const* X1, int rows1,
@@ -630,6 +719,19 @@ void launch_mlp_codes_f32(float const* X, int rows128, MlpWeights const& w, floa
                           hipStream_t st)
 {
     launch_mlp_codes2_f32(X, rows128, codes, nullptr, 0, nullptr, w, scale, st);
+}
+
+void launch_mlp_codes_u8(uint8_t const* Q, float const* denom, int rows128, MlpWeights const& w, float scale,
+                         signed char* codes, hipStream_t st)
+{
+    int const nb = rows128 / 128;
+    if (nb <= 0) return;
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    int const grid = nb < 2 * cus ? nb : 2 * cus;
+    hipLaunchKernelGGL(k_mlp_codes_u8, dim3(grid), dim3(256), 0, st, Q, denom, nb, codes, w.W1b, w.P + kOffB1, w.W2p,
+                       w.P + kOffB2, scale);
 }
 
 __global__ void k_fill_f32(float* __restrict__ p, size_t n, float v)
