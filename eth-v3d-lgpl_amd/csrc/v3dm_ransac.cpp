@@ -138,19 +138,24 @@ Poly poly_zero()
     return p;
 }
 
-// product of two polynomials whose degrees add up to at most 3
-Poly poly_mul(Poly const& p, Poly const& q)
-{
-    static int table[kMon][kMon];
-    static bool init = false;
-    if (!init) {
+// monomial of the product of monomials i and j (-1 above degree 3)
+struct ProductTable {
+    int t[kMon][kMon];
+    ProductTable()
+    {
         for (int i = 0; i < kMon; ++i)
             for (int j = 0; j < kMon; ++j) {
                 int const a = kExp[i][0] + kExp[j][0], b = kExp[i][1] + kExp[j][1], c = kExp[i][2] + kExp[j][2];
-                table[i][j] = (a + b + c <= 3) ? mon_index(a, b, c) : -1;
+                t[i][j] = (a + b + c <= 3) ? mon_index(a, b, c) : -1;
             }
-        init = true;
     }
+};
+
+// product of two polynomials whose degrees add up to at most 3
+Poly poly_mul(Poly const& p, Poly const& q)
+{
+    static ProductTable const tab;  // initialised once, thread-safe
+    auto const& table = tab.t;
     Poly r = poly_zero();
     for (int i = 0; i < kMon; ++i) {
         if (p.c[i] == 0.0) continue;
@@ -594,7 +599,7 @@ bool essential_svd(Mat3 const& E, Mat3& U, Mat3& V)
 }
 
 // the four decompositions: 0 (R1, t), 1 (R1, -t), 2 (R2, t), 3 (R2, -t), R1 = U D V^T,
-// R2 = U D^T V^T, t = the third column of U (relativePoseFromEssential, cu of
+// R2 = U D^T V^T, t = the third column of U (relativePoseFromEssential,
 // v3d_poseutilities.cpp:1227-1261; D as makeD_Matrix, :1183-1191)
 void pose_of(Mat3 const& U, Mat3 const& V, int conf, Mat3& R, double (&T)[3])
 {
