@@ -215,7 +215,7 @@ def test_sampled_check_follows_features(matcher):
     raw.set_params(learned.init_params(5))
     raw.calibrate_code_scale(np.concatenate([sets[0][0][:1024], sets[0][1][:1024]]))
     other.set_precision(v3dmatch.PREC_LEARNED)
-    other.set_learned_topk(2)  # a weak shortlist: misses to count
+    other.set_learned_topk(1)  # a weak shortlist: misses to count
     other.set_fallback(128, 0.0)
     rows = (np.arange(128, dtype=np.int64) * n) // 128
     total_missed = 0
