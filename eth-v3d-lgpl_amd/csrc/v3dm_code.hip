@@ -78,10 +78,12 @@ __device__ __forceinline__ void code_tile(v4i const (&a)[2], int t, v4i const (&
 }
 
 // One work unit: query block qblk (256 queries) against candidate tiles [split *
-// tilesPerSplit, ...) -> part[((split * ldq) + q) * 2 + h] (code scores as floats).
+// tilesPerSplit, ...) -> part[((split * ldq) + q) * 2 + h]: code scores as floats and
+// candidate indices (Cand2), or packed score and index inside the split (CandP).
+template <typename List>
 __device__ __forceinline__ void code_block(signed char const* __restrict__ Q, int nQ,
                                            signed char const* __restrict__ C, int nC, int qblk, int split,
-                                           int tilesPerSplit, Cand2* __restrict__ part, int ldq)
+                                           int tilesPerSplit, List* __restrict__ part, int ldq)
 {
     int const lane = threadIdx.x & 63;
     int const wave = threadIdx.x >> 6;
@@ -121,23 +123,27 @@ __device__ __forceinline__ void code_block(signed char const* __restrict__ Q, in
         int const q = q0 + qb * 32 + c32;
         if (q >= nQ) continue;
         // the list holds the best candidate of each group (two disjoint candidate sets)
-        Cand2 c;
         int const r1 = st[qb].k[0] & 15, r2 = st[qb].k[1] & 15;
         bool const ok1 = st[qb].blk[0] >= 0 && st[qb].k[0] != INT_MIN;
         bool const ok2 = st[qb].blk[1] >= 0 && st[qb].k[1] != INT_MIN;
-        c.v1 = ok1 ? (float)(st[qb].k[0] >> 4) : -INFINITY;
-        c.v2 = ok2 ? (float)(st[qb].k[1] >> 4) : -INFINITY;
-        c.j1 = ok1 ? st[qb].blk[0] * 32 + 8 * (r1 >> 2) + 4 * h + (r1 & 3) : -1;
-        c.j2 = ok2 ? st[qb].blk[1] * 32 + 8 * (r2 >> 2) + 4 * h + (r2 & 3) : -1;
-        part[((size_t)split * ldq + q) * 2 + h] = c;
+        int const j1 = st[qb].blk[0] * 32 + 8 * (r1 >> 2) + 4 * h + (r1 & 3);
+        int const j2 = st[qb].blk[1] * 32 + 8 * (r2 >> 2) + 4 * h + (r2 & 3);
+        if constexpr (sizeof(List) == sizeof(CandP)) {
+            int const j0 = t0 * kCT;  // first candidate of the split
+            unsigned int const e = (unsigned int)(kPackedEmpty & 0xffff) << 16;
+            CandP c;
+            c.a = ok1 ? ((unsigned int)(st[qb].k[0] >> 4) << 16) | (unsigned int)(j1 - j0) : e;
+            c.b = ok2 ? ((unsigned int)(st[qb].k[1] >> 4) << 16) | (unsigned int)(j2 - j0) : e;
+            part[((size_t)split * ldq + q) * 2 + h] = c;
+        } else {
+            Cand2 c;
+            c.v1 = ok1 ? (float)(st[qb].k[0] >> 4) : -INFINITY;
+            c.v2 = ok2 ? (float)(st[qb].k[1] >> 4) : -INFINITY;
+            c.j1 = ok1 ? j1 : -1;
+            c.j2 = ok2 ? j2 : -1;
+            part[((size_t)split * ldq + q) * 2 + h] = c;
+        }
     }
-}
-
-__global__ __launch_bounds__(256, 4) void k_pass_code(signed char const* __restrict__ Q, int nQ,
-                                                   signed char const* __restrict__ C, int nC, int tilesPerSplit,
-                                                   Cand2* __restrict__ part, int ldq)
-{
-    code_block(Q, nQ, C, nC, blockIdx.x, blockIdx.y, tilesPerSplit, part, ldq);
 }
 
 // Both directions of a pair in one launch: work-groups [0, queryBlocks0 * splits0) run
@@ -145,7 +151,7 @@ __global__ __launch_bounds__(256, 4) void k_pass_code(signed char const* __restr
 struct CodeDir {
     signed char const* Q;
     signed char const* C;
-    Cand2* part;
+    CandP* part;
     int nQ, nC, tilesPerSplit, ldq, queryBlocks, splits;
 };
 
@@ -187,7 +193,7 @@ void launch_batch_pass_code(BatchDev const& b, int4 const* items, int nItems, in
 }
 
 void launch_pass_code2(signed char const* A, int nA, signed char const* B, int nB, ShortPlan const& pa,
-                       Cand2* partA, int ldA, ShortPlan const& pb, Cand2* partB, int ldB, hipStream_t st)
+                       CandP* partA, int ldA, ShortPlan const& pb, CandP* partB, int ldB, hipStream_t st)
 {
     if (nA <= 0 || nB <= 0) return;
     CodeDir const d0{A, B, partA, nA, nB, pa.tilesPerSplit, ldA, pa.queryBlocks, pa.splits};
@@ -196,12 +202,5 @@ void launch_pass_code2(signed char const* A, int nA, signed char const* B, int n
                        d0, d1);
 }
 
-void launch_pass_code(signed char const* Q, int nQ, signed char const* C, int nC, ShortPlan const& p, Cand2* part,
-                      int ldq, hipStream_t st)
-{
-    if (nQ <= 0 || nC <= 0) return;
-    hipLaunchKernelGGL(k_pass_code, dim3(p.queryBlocks, p.splits), dim3(256), 0, st, Q, nQ, C, nC, p.tilesPerSplit,
-                       part, ldq);
-}
 
 }  // namespace v3dm

@@ -51,6 +51,9 @@ namespace {
 
 int roundUpTo(int n, int m) { return (n + m - 1) / m * m; }
 
+// the learned pass's packed lists live in the first half of the partition buffers
+CandP* packed(Cand2* p) { return reinterpret_cast<CandP*>(p); }
+
 #define EXTCHECK(h, call)                                                    \
     do {                                                                     \
         hipError_t const e_ = (call);                                        \
@@ -216,15 +219,16 @@ int enqueue_learned(v3dm_context* h, float minScore, float minRatio)
         // the main stream's kernels are enqueued first: the check's stream operations cost
         // more host time than its kernels take, and the main stream should not wait for them
         // 1024 work-groups: the code pass runs four per CU
-        ShortPlan const pl = plan_short(nL, nR, 1024);
-        ShortPlan const pr = plan_short(nR, nL, 1024);
+        ShortPlan const pl = plan_short_packed(nL, nR, 1024);
+        ShortPlan const pr = plan_short_packed(nR, nL, 1024);
         // both directions in one launch (the profiled "dominant kernel" covers both)
         if ((rc = prof_begin(h))) return rc;
-        launch_pass_code2(e->cL, nL, e->cR, nR, pl, e->partL, e->capL, pr, e->partR, e->capR, st);
+        launch_pass_code2(e->cL, nL, e->cR, nR, pl, packed(e->partL), e->capL, pr, packed(e->partR), e->capR, st);
         if ((rc = prof_end(h))) return rc;
-        launch_rescore(h->dL, h->dR, nL, nR, e->partL, pl.splits, e->capL, e->partR, pr.splits, e->capR, h->normL,
-                       h->normR, h->maxNormBits, 0.0f, h->learnedTopK, learned_margin(h), minScore, minRatio,
-                       h->colMax, h->rowMax, e->secL, e->secR, h->stripes, st);
+        launch_rescore_packed(h->dL, h->dR, nL, nR, packed(e->partL), pl.splits, e->capL, pl.tilesPerSplit * 64,
+                              packed(e->partR), pr.splits, e->capR, pr.tilesPerSplit * 64, h->normL, h->normR,
+                              h->maxNormBits, h->learnedTopK, learned_margin(h), minScore, minRatio, h->colMax,
+                              h->rowMax, e->secL, e->secR, h->stripes, st);
         e->haveSecond = true;
         SampleOut so;
         if (check) {
@@ -292,10 +296,10 @@ int ext_time_pass(v3dm_context* h, int mode, int reps, hipEvent_t start, hipEven
     EXTCHECK(h, hipEventRecord(start, st));
     if (mode == V3DM_PREC_LEARNED) {
         if (!e->cL || !e->cR) return ctx_fail(h, V3DM_ERR_STATE, "no learned codes");
-        ShortPlan const pl = plan_short(nL, nR, 1024);
-        ShortPlan const pr = plan_short(nR, nL, 1024);
+        ShortPlan const pl = plan_short_packed(nL, nR, 1024);
+        ShortPlan const pr = plan_short_packed(nR, nL, 1024);
         for (int r = 0; r < reps; ++r)
-            launch_pass_code2(e->cL, nL, e->cR, nR, pl, e->partL, e->capL, pr, e->partR, e->capR, st);
+            launch_pass_code2(e->cL, nL, e->cR, nR, pl, packed(e->partL), e->capL, pr, packed(e->partR), e->capR, st);
         *launches = reps;
     } else {
         ShortPlan const pl = plan_short(nL, nR, 512);

@@ -96,8 +96,8 @@ __device__ __forceinline__ double exact_dot_u8(float const* q, uint8_t const* __
 // qs: the group's 128 floats of LDS, which receive the query row (its loads are issued with
 // the list loads).  Returns the exact top-2 (all lanes) and the number of candidates scored
 // by this lane.
-template <bool kU8>
-__device__ __forceinline__ Top2x finish_query(Cand2 const* __restrict__ part, size_t ldq2, int nParts, float nq,
+template <bool kU8, typename List>
+__device__ __forceinline__ Top2x finish_query(List const* __restrict__ part, size_t ldq2, int nParts, int span, float nq,
                                               float mo, float winRel, int topK, float margin,
                                               float const* __restrict__ Qd,
                                               float const* __restrict__ Cd, uint8_t const* __restrict__ Cu8,
@@ -116,12 +116,24 @@ __device__ __forceinline__ Top2x finish_query(Cand2 const* __restrict__ part, si
 #pragma unroll
     for (int u = 0; u < kPerLane / 2; ++u) {
         int const e = sub + 8 * u;  // list (split e / 2, half e % 2)
-        Cand2 c = {-INFINITY, -1, -INFINITY, -1};
-        if (e < 2 * nParts) c = part[(size_t)(e >> 1) * ldq2 + (e & 1)];
-        v[2 * u] = c.j1 >= 0 ? c.v1 : -INFINITY;
-        j[2 * u] = c.j1;
-        v[2 * u + 1] = c.j2 >= 0 ? c.v2 : -INFINITY;
-        j[2 * u + 1] = c.j2;
+        if constexpr (sizeof(List) == sizeof(CandP)) {
+            unsigned int const empty = (unsigned int)(kPackedEmpty & 0xffff) << 16;
+            CandP c = {empty, empty};
+            if (e < 2 * nParts) c = part[(size_t)(e >> 1) * ldq2 + (e & 1)];
+            int const base = (e >> 1) * span;
+            int const s1 = (int)c.a >> 16, s2 = (int)c.b >> 16;
+            v[2 * u] = s1 != kPackedEmpty ? (float)s1 : -INFINITY;
+            j[2 * u] = s1 != kPackedEmpty ? base + (int)(c.a & 0xffffu) : -1;
+            v[2 * u + 1] = s2 != kPackedEmpty ? (float)s2 : -INFINITY;
+            j[2 * u + 1] = s2 != kPackedEmpty ? base + (int)(c.b & 0xffffu) : -1;
+        } else {
+            Cand2 c = {-INFINITY, -1, -INFINITY, -1};
+            if (e < 2 * nParts) c = part[(size_t)(e >> 1) * ldq2 + (e & 1)];
+            v[2 * u] = c.j1 >= 0 ? c.v1 : -INFINITY;
+            j[2 * u] = c.j1;
+            v[2 * u + 1] = c.j2 >= 0 ? c.v2 : -INFINITY;
+            j[2 * u + 1] = c.j2;
+        }
     }
     float thr;
     if (topK > 0) {
@@ -229,11 +241,14 @@ __device__ __forceinline__ void add_stripe(int n, int* __restrict__ stripes)
 
 // ------------------------------------------------------------------------------ one pair
 // Queries [0, nL) are left features (lists partL), [nL, nL + nR) right features; 32
-// queries per work-group.
+// queries per work-group.  List: Cand2 (BF16 / FP8 passes) or CandP (learned code pass,
+// spanL / spanR candidates per split).
+template <typename List>
 __global__ __launch_bounds__(256) void k_shortlist_finish(float const* __restrict__ L, float const* __restrict__ R,
-                                                          int nL, int nR, Cand2 const* __restrict__ partL, int nPartL,
-                                                          int ldqL, Cand2 const* __restrict__ partR, int nPartR,
-                                                          int ldqR, float const* __restrict__ normL,
+                                                          int nL, int nR, List const* __restrict__ partL, int nPartL,
+                                                          int ldqL, int spanL, List const* __restrict__ partR,
+                                                          int nPartR, int ldqR, int spanR,
+                                                          float const* __restrict__ normL,
                                                           float const* __restrict__ normR,
                                                           unsigned int const* __restrict__ maxNormBits, float winRel,
                                                           int topK, float margin, float minScore, float minRatio,
@@ -250,10 +265,10 @@ __global__ __launch_bounds__(256) void k_shortlist_finish(float const* __restric
         bool const left = g < nL;
         int const q = left ? g : g - nL;
         Top2x const s = finish_query<false>((left ? partL : partR) + (size_t)q * 2, 2 * (size_t)(left ? ldqL : ldqR),
-                                     left ? nPartL : nPartR, left ? normL[q] : normR[q],
-                                     __uint_as_float(maxNormBits[left ? 1 : 0]), winRel, topK, margin,
-                                     (left ? L : R) + (size_t)q * kDim, left ? R : L, nullptr, nullptr, sub,
-                                     slots[threadIdx.x >> 3], qrow[threadIdx.x >> 3], n);
+                                            left ? nPartL : nPartR, left ? spanL : spanR, left ? normL[q] : normR[q],
+                                            __uint_as_float(maxNormBits[left ? 1 : 0]), winRel, topK, margin,
+                                            (left ? L : R) + (size_t)q * kDim, left ? R : L, nullptr, nullptr, sub,
+                                            slots[threadIdx.x >> 3], qrow[threadIdx.x >> 3], n);
         if (sub == 0) {
             (left ? colMax : rowMax)[q] = decide_exact(Top2{s.b1, s.b2, s.a1}, minScore, minRatio);
             if (colSecond) (left ? colSecond : rowSecond)[q] = s.a2;
@@ -270,9 +285,22 @@ void launch_rescore(float const* L, float const* R, int nL, int nR, Cand2 const*
 {
     int const n = nL + nR;
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_shortlist_finish, dim3((n + 31) / 32), dim3(256), 0, st, L, R, nL, nR, partL, nPartL, ldqL,
-                       partR, nPartR, ldqR, normL, normR, maxNormBits, winRel, topK, margin, minScore, minRatio,
-                       colMax, rowMax, colSecond, rowSecond, stripes);
+    hipLaunchKernelGGL(k_shortlist_finish<Cand2>, dim3((n + 31) / 32), dim3(256), 0, st, L, R, nL, nR, partL, nPartL,
+                       ldqL, 0, partR, nPartR, ldqR, 0, normL, normR, maxNormBits, winRel, topK, margin, minScore,
+                       minRatio, colMax, rowMax, colSecond, rowSecond, stripes);
+}
+
+void launch_rescore_packed(float const* L, float const* R, int nL, int nR, CandP const* partL, int nPartL, int ldqL,
+                           int spanL, CandP const* partR, int nPartR, int ldqR, int spanR, float const* normL,
+                           float const* normR, unsigned int const* maxNormBits, int topK, float margin,
+                           float minScore, float minRatio, int* colMax, int* rowMax, int* colSecond, int* rowSecond,
+                           int* stripes, hipStream_t st)
+{
+    int const n = nL + nR;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_shortlist_finish<CandP>, dim3((n + 31) / 32), dim3(256), 0, st, L, R, nL, nR, partL, nPartL,
+                       ldqL, spanL, partR, nPartR, ldqR, spanR, normL, normR, maxNormBits, 0.0f, topK, margin,
+                       minScore, minRatio, colMax, rowMax, colSecond, rowSecond, stripes);
 }
 
 // ------------------------------------------------------------------------------ batch
@@ -296,7 +324,7 @@ __device__ __forceinline__ void batch_finish(BatchDev const& b, Cand2 const* __r
         if (q < sg.n && sg.nO > 0 && sg.nParts > 0) {
             int const row = b.imgBase[sg.img] + q;
             size_t const cb = (size_t)b.imgBase[sg.oImg];
-            Top2x const s = finish_query<kU8>(part + ((size_t)sg.partOff + q) * 2, 2 * (size_t)sg.ld, sg.nParts,
+            Top2x const s = finish_query<kU8>(part + ((size_t)sg.partOff + q) * 2, 2 * (size_t)sg.ld, sg.nParts, 0,
                                               b.norm[row], __uint_as_float(b.maxNorm[sg.oImg]), winRel, topK, margin,
                                               b.img + (size_t)row * kDim, b.img + cb * kDim,
                                               kU8 ? b.u8 + cb * kDim : nullptr, kU8 ? b.denom + cb : nullptr, sub,

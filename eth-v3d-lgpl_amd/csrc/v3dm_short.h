@@ -12,6 +12,16 @@ struct Cand2 {
     int j2;
 };
 
+// Packed lists of the single-pair learned code pass (8 bytes instead of 16): per group, the
+// code score (|score| <= 32 * 31 * 31 < 2^15) in the high 16 bits and the candidate's index
+// inside its split (< 65536: at most 1024 tiles of 64 per split) in the low 16 bits; score
+// kPackedEmpty marks a group without a candidate.
+struct CandP {
+    unsigned int a, b;
+};
+constexpr int kPackedEmpty = -32768;
+constexpr int kPackedMaxTiles = 1024;  // 64-candidate tiles per split of a packed list
+
 struct ShortPlan {
     int queryBlocks = 0;
     int splits = 0;
@@ -37,6 +47,9 @@ constexpr float kWinRelFp8 = 0.03125f;
 
 void launch_to_bf16(float const* X, int rows, __bf16* Y, hipStream_t st);
 ShortPlan plan_short(int nQ, int nC, int targetGroups);
+// plan_short with at most kPackedMaxTiles tiles per split (every index inside a split fits
+// 16 bits; at most 2^20 candidates need 16 splits)
+ShortPlan plan_short_packed(int nQ, int nC, int targetGroups);
 void launch_pass_bf16(__bf16 const* Q, int nQ, __bf16 const* C, int nC, ShortPlan const& p, Cand2* part, int ldq,
                       hipStream_t st);
 void launch_to_fp8(float const* X, int rows, unsigned char* Y, unsigned char* E, hipStream_t st);
@@ -51,11 +64,17 @@ void launch_rescore(float const* L, float const* R, int nL, int nR, Cand2 const*
                     unsigned int const* maxNormBits, float winRel, int topK, float margin, float minScore,
                     float minRatio, int* colMax, int* rowMax, int* colSecond, int* rowSecond, int* stripes,
                     hipStream_t st);
-// learned code pass (v3dm_code.hip)
-void launch_pass_code(signed char const* Q, int nQ, signed char const* C, int nC, ShortPlan const& p, Cand2* part,
-                      int ldq, hipStream_t st);
-// both directions in one launch: A as queries against B (plan pa -> partA), B against A
+// The same over packed lists (the learned code pass): spanL / spanR = candidates per split
+// of the left / right queries' lists (the split's first candidate is split * span).
+void launch_rescore_packed(float const* L, float const* R, int nL, int nR, CandP const* partL, int nPartL, int ldqL,
+                           int spanL, CandP const* partR, int nPartR, int ldqR, int spanR, float const* normL,
+                           float const* normR, unsigned int const* maxNormBits, int topK, float margin,
+                           float minScore, float minRatio, int* colMax, int* rowMax, int* colSecond, int* rowSecond,
+                           int* stripes, hipStream_t st);
+// learned code pass (v3dm_code.hip), both directions in one launch: A as queries against B
+// (plan pa -> partA), B against A; packed lists (CandP); plans with at most 1024 tiles per
+// split
 void launch_pass_code2(signed char const* A, int nA, signed char const* B, int nB, ShortPlan const& pa,
-                       Cand2* partA, int ldA, ShortPlan const& pb, Cand2* partB, int ldB, hipStream_t st);
+                       CandP* partA, int ldA, ShortPlan const& pb, CandP* partB, int ldB, hipStream_t st);
 
 }  // namespace v3dm

@@ -270,6 +270,18 @@ ShortPlan plan_short(int nQ, int nC, int targetGroups)
     return p;
 }
 
+ShortPlan plan_short_packed(int nQ, int nC, int targetGroups)
+{
+    ShortPlan p = plan_short(nQ, nC, targetGroups);
+    if (p.tilesPerSplit > kPackedMaxTiles) {  // more splits, so that a split's index fits 16 bits
+        int const nTiles = (nC + kST - 1) / kST;
+        int const splits = (nTiles + kPackedMaxTiles - 1) / kPackedMaxTiles;
+        p.tilesPerSplit = (nTiles + splits - 1) / splits;
+        p.splits = (nTiles + p.tilesPerSplit - 1) / p.tilesPerSplit;
+    }
+    return p;
+}
+
 void launch_pass_bf16(__bf16 const* Q, int nQ, __bf16 const* C, int nC, ShortPlan const& p, Cand2* part, int ldq,
                       hipStream_t st)
 {

@@ -77,6 +77,28 @@ def test_learned_recall_full_size(matcher):
     assert rec >= 0.995 and prec >= 0.995
 
 
+def test_learned_large_candidate_sets():
+    """Many queries and candidates: the default plan would put more than 1024 candidate tiles
+    in one split, past what a packed list index holds, so the plan takes more splits.  The
+    learned form's result against the plain form's (the exact set) on a 262144 x 131072 pair."""
+    nL, nR = 262144, 131072
+    say(f"case learned {nL}x{nR}")
+    qL, qR, _ = synth.make_pair(nL, nR, 0.4, config=1, pair=17)
+    L, R = synth.dequantise(qL), synth.dequantise(qR)
+    m = v3dmatch.SIFT_Matching(0)
+    m.enableRoundingWarning(False)
+    m.allocate(nL, nR)
+    learned.Network(m).load(learned.WEIGHTS_PATH)
+    m.setLeftFeatures(nL, L)
+    m.setRightFeatures(nR, R)
+    m.set_precision(v3dmatch.PREC_FP32)
+    want = _pairs_set(m.find_putative_matches_array(0.9, 0.8))
+    rec, prec, st = _recall(m, L, R, want)
+    m.close()
+    say(f"  exact={len(want)} recall={rec:.5f} precision={prec:.5f}")
+    assert rec >= 0.99 and prec >= 0.995
+
+
 def test_training_gain(matcher):
     """The trained network against the same pipeline with its initial (untrained) weights."""
     m, net = matcher
