@@ -166,23 +166,21 @@ __device__ __forceinline__ void run_tiles_fp8(unsigned char const* __restrict__ 
     }
 }
 
-__global__ __launch_bounds__(256, 2) void k_pass_fp8(unsigned char const* __restrict__ Q,
-                                                     unsigned char const* __restrict__ EQ, int nQ,
-                                                     unsigned char const* __restrict__ C,
-                                                     unsigned char const* __restrict__ EC, int nC, int tilesPerSplit,
-                                                     Cand2* __restrict__ part, int ldq, float inf)
+// One work unit: query block qblk against the candidate tiles of split `split`.
+__device__ __forceinline__ void pass_fp8_block(unsigned char const* __restrict__ Q,
+                                               unsigned char const* __restrict__ EQ, int nQ,
+                                               unsigned char const* __restrict__ C,
+                                               unsigned char const* __restrict__ EC, int nC, int qblk, int split,
+                                               int tilesPerSplit, Cand2* __restrict__ part, int ldq,
+                                               unsigned char* stA, unsigned char* stB, float inf)
 {
-    __shared__ __attribute__((aligned(1024))) unsigned char stA[kStageBytesF];
-    __shared__ __attribute__((aligned(1024))) unsigned char stB[kStageBytesF];
-
     int const tid = threadIdx.x;
     int const lane = tid & 63;
     int const wave = tid >> 6;
     int const wv = __builtin_amdgcn_readfirstlane(wave);
     int const h = lane >> 5;
     int const c32 = lane & 31;
-    int const q0 = blockIdx.x * kFQ + wave * 64;
-    int const split = blockIdx.y;
+    int const q0 = qblk * kFQ + wave * 64;
     int const nTiles = (nC + kFT - 1) / kFT;
     int const t0 = split * tilesPerSplit;
     int const t1 = min(t0 + tilesPerSplit, nTiles);
@@ -228,12 +226,40 @@ __global__ __launch_bounds__(256, 2) void k_pass_fp8(unsigned char const* __rest
     }
 }
 
-void launch_pass_fp8(unsigned char const* Q, unsigned char const* EQ, int nQ, unsigned char const* C,
-                     unsigned char const* EC, int nC, ShortPlan const& p, Cand2* part, int ldq, hipStream_t st)
+// Both directions of a pair in one launch: work-groups [0, queryBlocks0 * splits0) run
+// direction 0 (query block fastest), the rest direction 1.
+struct Fp8Dir {
+    unsigned char const* Q;
+    unsigned char const* EQ;
+    unsigned char const* C;
+    unsigned char const* EC;
+    Cand2* part;
+    int nQ, nC, tilesPerSplit, ldq, queryBlocks, splits;
+};
+
+__global__ __launch_bounds__(256, 2) void k_pass_fp8(Fp8Dir d0, Fp8Dir d1, float inf)
 {
-    if (nQ <= 0 || nC <= 0) return;
-    hipLaunchKernelGGL(k_pass_fp8, dim3(p.queryBlocks, p.splits), dim3(256), 0, st, Q, EQ, nQ, C, EC, nC,
-                       p.tilesPerSplit, part, ldq, INFINITY);
+    __shared__ __attribute__((aligned(1024))) unsigned char stA[kStageBytesF];
+    __shared__ __attribute__((aligned(1024))) unsigned char stB[kStageBytesF];
+    int b = blockIdx.x;
+    int const n0 = d0.queryBlocks * d0.splits;
+    bool const second = b >= n0;  // uniform per work-group
+    Fp8Dir const d = second ? d1 : d0;
+    if (second) b -= n0;
+    int const split = b / d.queryBlocks;
+    pass_fp8_block(d.Q, d.EQ, d.nQ, d.C, d.EC, d.nC, b - split * d.queryBlocks, split, d.tilesPerSplit, d.part,
+                   d.ldq, stA, stB, inf);
+}
+
+void launch_pass_fp8_2(unsigned char const* A, unsigned char const* EA, int nA, unsigned char const* B,
+                       unsigned char const* EB, int nB, ShortPlan const& pa, Cand2* partA, int ldA, ShortPlan const& pb,
+                       Cand2* partB, int ldB, hipStream_t st)
+{
+    if (nA <= 0 || nB <= 0) return;
+    Fp8Dir const d0{A, EA, B, EB, partA, nA, nB, pa.tilesPerSplit, ldA, pa.queryBlocks, pa.splits};
+    Fp8Dir const d1{B, EB, A, EA, partB, nB, nA, pb.tilesPerSplit, ldB, pb.queryBlocks, pb.splits};
+    hipLaunchKernelGGL(k_pass_fp8, dim3(pa.queryBlocks * pa.splits + pb.queryBlocks * pb.splits), dim3(256), 0, st,
+                       d0, d1, INFINITY);
 }
 
 }  // namespace v3dm

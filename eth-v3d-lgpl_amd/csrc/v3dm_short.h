@@ -50,11 +50,14 @@ ShortPlan plan_short(int nQ, int nC, int targetGroups);
 // plan_short with at most kPackedMaxTiles tiles per split (every index inside a split fits
 // 16 bits; at most 2^20 candidates need 16 splits)
 ShortPlan plan_short_packed(int nQ, int nC, int targetGroups);
-void launch_pass_bf16(__bf16 const* Q, int nQ, __bf16 const* C, int nC, ShortPlan const& p, Cand2* part, int ldq,
-                      hipStream_t st);
+// both directions of a pair in one launch: A as queries against B (plan pa -> partA), B
+// against A (pb -> partB)
+void launch_pass_bf16_2(__bf16 const* A, int nA, __bf16 const* B, int nB, ShortPlan const& pa, Cand2* partA,
+                        int ldA, ShortPlan const& pb, Cand2* partB, int ldB, hipStream_t st);
 void launch_to_fp8(float const* X, int rows, unsigned char* Y, unsigned char* E, hipStream_t st);
-void launch_pass_fp8(unsigned char const* Q, unsigned char const* EQ, int nQ, unsigned char const* C,
-                     unsigned char const* EC, int nC, ShortPlan const& p, Cand2* part, int ldq, hipStream_t st);
+void launch_pass_fp8_2(unsigned char const* A, unsigned char const* EA, int nA, unsigned char const* B,
+                       unsigned char const* EB, int nB, ShortPlan const& pa, Cand2* partA, int ldA, ShortPlan const& pb,
+                       Cand2* partB, int ldB, hipStream_t st);
 // Exact rescoring (v3dm_rescore.hip); colSecond / rowSecond may be NULL.  topK = 0: the
 // candidates within winRel * |q| * max|c| of the second best list value (certified forms);
 // topK > 0: the topK best by list value, stopping at the first one more than `margin`

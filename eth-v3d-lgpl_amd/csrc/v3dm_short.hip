@@ -217,13 +217,27 @@ __device__ __forceinline__ void pass_bf16_block(__bf16 const* __restrict__ Q, in
     }
 }
 
-__global__ __launch_bounds__(256, 2) void k_pass_bf16(__bf16 const* __restrict__ Q, int nQ,
-                                                      __bf16 const* __restrict__ C, int nC, int tilesPerSplit,
-                                                      Cand2* __restrict__ part, int ldq, float inf)
+// Both directions of a pair in one launch: work-groups [0, queryBlocks0 * splits0) run
+// direction 0 (query block fastest), the rest direction 1.
+struct Bf16Dir {
+    __bf16 const* Q;
+    __bf16 const* C;
+    Cand2* part;
+    int nQ, nC, tilesPerSplit, ldq, queryBlocks, splits;
+};
+
+__global__ __launch_bounds__(256, 2) void k_pass_bf16(Bf16Dir d0, Bf16Dir d1, float inf)
 {
     __shared__ __attribute__((aligned(1024))) unsigned char stA[kStageBytesB];
     __shared__ __attribute__((aligned(1024))) unsigned char stB[kStageBytesB];
-    pass_bf16_block(Q, nQ, C, nC, blockIdx.x, blockIdx.y, tilesPerSplit, part, ldq, stA, stB, inf);
+    int b = blockIdx.x;
+    int const n0 = d0.queryBlocks * d0.splits;
+    bool const second = b >= n0;  // uniform per work-group
+    Bf16Dir const d = second ? d1 : d0;
+    if (second) b -= n0;
+    int const split = b / d.queryBlocks;
+    pass_bf16_block(d.Q, d.nQ, d.C, d.nC, b - split * d.queryBlocks, split, d.tilesPerSplit, d.part, d.ldq, stA, stB,
+                    inf);
 }
 
 // Persistent form over a batch: each work-group takes items from the queue until it runs
@@ -282,12 +296,14 @@ ShortPlan plan_short_packed(int nQ, int nC, int targetGroups)
     return p;
 }
 
-void launch_pass_bf16(__bf16 const* Q, int nQ, __bf16 const* C, int nC, ShortPlan const& p, Cand2* part, int ldq,
-                      hipStream_t st)
+void launch_pass_bf16_2(__bf16 const* A, int nA, __bf16 const* B, int nB, ShortPlan const& pa, Cand2* partA,
+                        int ldA, ShortPlan const& pb, Cand2* partB, int ldB, hipStream_t st)
 {
-    if (nQ <= 0 || nC <= 0) return;
-    hipLaunchKernelGGL(k_pass_bf16, dim3(p.queryBlocks, p.splits), dim3(256), 0, st, Q, nQ, C, nC, p.tilesPerSplit,
-                       part, ldq, INFINITY);
+    if (nA <= 0 || nB <= 0) return;
+    Bf16Dir const d0{A, B, partA, nA, nB, pa.tilesPerSplit, ldA, pa.queryBlocks, pa.splits};
+    Bf16Dir const d1{B, A, partB, nB, nA, pb.tilesPerSplit, ldB, pb.queryBlocks, pb.splits};
+    hipLaunchKernelGGL(k_pass_bf16, dim3(pa.queryBlocks * pa.splits + pb.queryBlocks * pb.splits), dim3(256), 0, st,
+                       d0, d1, INFINITY);
 }
 
 

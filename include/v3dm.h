@@ -128,8 +128,8 @@ int v3dm_set_derived_cache(v3dm_handle h, int enable);
  * (stats.nFallback = 1).  Default 128 and 0.995. */
 int v3dm_set_fallback(v3dm_handle h, int samples, float minRecall);
 /* Measurement helper: launches the dominant kernel of the current precision mode (plain:
- * the fused score pass; BF16 / FP8: the pass of each direction; learned: the code pass of
- * both directions) `reps` times back to back on the inputs of the last findPutativeMatches,
+ * the fused score pass; BF16 / FP8 / learned: the pass over both directions) `reps` times
+ * back to back on the inputs of the last findPutativeMatches,
  * with one HIP event pair around all launches on the handle's stream, and returns the
  * average duration of one launch.  Overwrites the intermediate partial results (not the
  * match list or colMax / rowMax of the last call). */
