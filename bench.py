@@ -287,10 +287,10 @@ def bench_guided(L, R, steps, warmup):
     m.setRightFeatures(N_FEAT, R)
     put = m.find_putative_matches_array(MIN_SCORE, MIN_RATIO)
     for _ in range(warmup):
-        m.runGuidedHomographyMatching(H, 3.0, lxy, rxy, MIN_SCORE, MIN_RATIO)
+        m.run_guided_array(True, H, 3.0, lxy, rxy, MIN_SCORE, MIN_RATIO)
     t0 = time.perf_counter()
     for _ in range(steps):
-        g = m.runGuidedHomographyMatching(H, 3.0, lxy, rxy, MIN_SCORE, MIN_RATIO)
+        g = m.run_guided_array(True, H, 3.0, lxy, rxy, MIN_SCORE, MIN_RATIO)
     dt = (time.perf_counter() - t0) / steps
     m.close()
     return {"workload": f"{N_FEAT}x{N_FEAT} pair, homography mask, 3 px", "ms_per_call_host_inclusive": dt * 1e3,

@@ -138,12 +138,21 @@ class SIFT_Matching:
         return self._pairs_out(self._lib.v3dm_find_putative_matches, C.c_float(minScore), C.c_float(minRatio))
 
     def _guided(self, fn, M, thr, lpos, rpos, minScore, minRatio):
+        return [tuple(p) for p in self._guided_array(fn, M, thr, lpos, rpos, minScore, minRatio).tolist()]
+
+    def _guided_array(self, fn, M, thr, lpos, rpos, minScore, minRatio):
         m = np.ascontiguousarray(np.asarray(M, dtype=np.float64).reshape(3, 3))
         lp = _xy(lpos, self._nL)
         rp = _xy(rpos, self._nR)
-        pairs = self._pairs_out(fn, m.ctypes.data, C.c_float(thr), lp.ctypes.data, rp.ctypes.data,
-                                C.c_float(minScore), C.c_float(minRatio))
-        return [tuple(p) for p in pairs.tolist()]
+        return self._pairs_out(fn, m.ctypes.data, C.c_float(thr), lp.ctypes.data, rp.ctypes.data,
+                               C.c_float(minScore), C.c_float(minRatio))
+
+    def run_guided_array(self, homography, M, distanceThreshold, leftPositions, rightPositions, minScore=0.9,
+                         minRatio=0.8):
+        """runGuidedHomographyMatching (homography=True) or runGuidedMatching with the matches
+        as an (n, 2) int32 array instead of a list of tuples."""
+        fn = self._lib.v3dm_run_guided_H if homography else self._lib.v3dm_run_guided_F
+        return self._guided_array(fn, M, distanceThreshold, leftPositions, rightPositions, minScore, minRatio)
 
     def set_precision(self, mode):
         self._check(self._lib.v3dm_set_precision(self._h, int(mode)))
