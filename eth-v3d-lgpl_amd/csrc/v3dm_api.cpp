@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -485,6 +486,22 @@ int v3dm_set_fallback(v3dm_handle h, int samples, float minRecall)
     h->fallbackSamples = samples;
     h->fallbackRecall = minRecall;
     return V3DM_OK;
+}
+
+int v3dm_fallback_allowed_misses(int accepted, float minRecall)
+{
+    if (accepted <= 0 || !(minRecall < 1.0f)) return 0;
+    if (!(minRecall > 0.0f)) return accepted;
+    // binomial tail by the pmf recurrence P[X = k + 1] = P[X = k] (a - k) / (k + 1) q / r
+    double const r = minRecall, q = 1.0 - r;
+    double pmf = std::pow(r, accepted), cdf = pmf;
+    int k = 0;
+    while (k < accepted && 1.0 - cdf > kFallbackAlpha) {
+        pmf *= (double)(accepted - k) / (double)(k + 1) * (q / r);
+        cdf += pmf;
+        ++k;
+    }
+    return k;
 }
 
 int v3dm_set_derived_cache(v3dm_handle h, int enable)

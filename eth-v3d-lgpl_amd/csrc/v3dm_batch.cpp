@@ -28,6 +28,13 @@ void launch_batch_load_f32(float const* src, int const* srcRow, int const* nImg,
 
 namespace {
 
+MissLimits miss_limits(float minRecall)
+{
+    MissLimits lim;
+    for (int a = 0; a <= kSampleRows; ++a) lim.allowed[a] = (unsigned char)v3dm_fallback_allowed_misses(a, minRecall);
+    return lim;
+}
+
 int roundUp(int n, int m) { return (n + m - 1) / m * m; }
 
 struct Plan {
@@ -470,7 +477,8 @@ int enqueue_run(v3dm_context* h, BatchState* b, int mode, float minScore, float 
             launch_batch_sample_fp32(d, b->sRows, b->sCount, b->sItems, b->nSItems, b->counters + 3, b->sPart,
                                      b->sMaxSplits, 2 * cus, st);
             launch_batch_sample_decide(d, b->sPart, b->sMaxSplits, b->sNSplits, b->sNorm, b->sIdx, b->sCount,
-                                       h->epsRel, minScore, minRatio, h->fallbackRecall, b->sel, b->pairFlag,
+                                       h->epsRel, minScore, minRatio, miss_limits(h->fallbackRecall), b->sel,
+                                       b->pairFlag,
                                        b->pairFlag + b->nPairs, st);
             // pairs whose sampled recall fell below the threshold get the plain form's
             // decisions, on the device: their plain items, merge masked to their segments,

@@ -47,6 +47,12 @@ struct BatchSeg {
 // Shortlist item:  x = segment, y = query block (256 rows), z = first tile, w = last tile
 // (64-row tiles of candidates in both forms).
 
+// Misses the sampled check tolerates per number of exactly accepted samples (passed by
+// value: allowed[a] = v3dm_fallback_allowed_misses(a, minRecall)).
+struct MissLimits {
+    unsigned char allowed[kSampleRows + 1];
+};
+
 struct BatchDev {
     float const* img;
     __bf16 const* imgB;
@@ -88,7 +94,7 @@ void launch_batch_sample_fp32(BatchDev const& b, float const* sRows, int const* 
                               int nItems, int* queue, Top2* sPart, int maxSplits, int grid, hipStream_t st);
 void launch_batch_sample_decide(BatchDev const& b, Top2 const* sPart, int maxSplits, int const* nSplits,
                                 float const* sNorm, int const* sIdx, int const* sCount, float epsRel, float minScore,
-                                float minRatio, float minRecall, int const* sel, int* pairFlag, int* counts,
+                                float minRatio, MissLimits const& lim, int const* sel, int* pairFlag, int* counts,
                                 hipStream_t st);
 void launch_batch_recheck(BatchDev const& b, int const* flagList, int* counters, float minScore, float minRatio,
                           int* sel, int grid, hipStream_t st);

@@ -418,15 +418,15 @@ __global__ __launch_bounds__(256) void k_batch_sample_gather(BatchDev b, int sam
 // One work-group (kSampleRows threads) per pair: merge the sample's split partials, take
 // the certified FP32 decision (samples whose decision is not certified -- near ties, well
 // under 1% -- are left out of the estimate), compare with the shortlist decision sel of the
-// same left feature.  pairFlag[p] = 1 when the sampled recall (accepted - missed) /
-// accepted falls below minRecall; counts[4p..] = accepted, missed, left out.
+// same left feature.  pairFlag[p] = 1 when the misses exceed lim.allowed[accepted];
+// counts[4p..] = accepted, missed, left out.
 __global__ __launch_bounds__(kSampleRows) void k_batch_sample_decide(BatchDev b, Top2 const* __restrict__ sPart,
                                                                      int maxSplits, int const* __restrict__ nSplits,
                                                                      float const* __restrict__ sNorm,
                                                                      int const* __restrict__ sIdx,
                                                                      int const* __restrict__ sCount, float epsRel,
                                                                      float minScore, float minRatio,
-                                                                     float minRecall, int const* __restrict__ sel,
+                                                                     MissLimits lim, int const* __restrict__ sel,
                                                                      int* __restrict__ pairFlag,
                                                                      int* __restrict__ counts)
 {
@@ -467,7 +467,7 @@ __global__ __launch_bounds__(kSampleRows) void k_batch_sample_decide(BatchDev b,
         counts[4 * p + 1] = m;
         counts[4 * p + 2] = x;
         counts[4 * p + 3] = S;
-        pairFlag[p] = (a > 0 && (float)(a - m) < minRecall * (float)a) ? 1 : 0;
+        pairFlag[p] = (m > (int)lim.allowed[a]) ? 1 : 0;
     }
 }
 
@@ -482,12 +482,12 @@ void launch_batch_sample_gather(BatchDev const& b, int samples, float* sRows, fl
 
 void launch_batch_sample_decide(BatchDev const& b, Top2 const* sPart, int maxSplits, int const* nSplits,
                                 float const* sNorm, int const* sIdx, int const* sCount, float epsRel, float minScore,
-                                float minRatio, float minRecall, int const* sel, int* pairFlag, int* counts,
+                                float minRatio, MissLimits const& lim, int const* sel, int* pairFlag, int* counts,
                                 hipStream_t st)
 {
     if (b.nPairs <= 0) return;
     hipLaunchKernelGGL(k_batch_sample_decide, dim3(b.nPairs), dim3(kSampleRows), 0, st, b, sPart, maxSplits, nSplits,
-                       sNorm, sIdx, sCount, epsRel, minScore, minRatio, minRecall, sel, pairFlag, counts);
+                       sNorm, sIdx, sCount, epsRel, minScore, minRatio, lim, sel, pairFlag, counts);
 }
 
 }  // namespace v3dm

@@ -64,6 +64,7 @@ SIGNATURES = {
     "v3dm_robust_orientation": (_I, [_FP, _I, _FP, _FP, C.c_double, _I, _I, _I, C.c_uint64, _P, _FP]),
     "v3dm_set_derived_cache": (_I, [_P, _I]),
     "v3dm_set_fallback": (_I, [_P, _I, _F]),
+    "v3dm_fallback_allowed_misses": (_I, [_I, _F]),
     "v3dm_set_left_device": (_I, [_P, _I, _FP]),
     "v3dm_set_right_device": (_I, [_P, _I, _FP]),
     "v3dm_set_left_u8": (_I, [_P, _I, _FP]),

@@ -172,6 +172,12 @@ class SIFT_Matching:
     def set_fallback(self, samples=128, min_recall=0.995):
         self._check(self._lib.v3dm_set_fallback(self._h, int(samples), C.c_float(min_recall)))
 
+    @staticmethod
+    def fallback_allowed_misses(accepted, min_recall=0.995):
+        """Misses a sampled check of `accepted` exactly accepted features tolerates before
+        the plain form is run (one-sided binomial test at 5%)."""
+        return int(load_library().v3dm_fallback_allowed_misses(int(accepted), C.c_float(min_recall)))
+
     def set_derived_cache(self, enable=True):
         self._check(self._lib.v3dm_set_derived_cache(self._h, int(bool(enable))))
 

@@ -122,11 +122,17 @@ int v3dm_set_learned_margin(v3dm_handle h, float marginRel);
  * includes that work). */
 int v3dm_set_derived_cache(v3dm_handle h, int enable);
 /* Learned mode: every call also decides `samples` evenly spaced left features (at most
- * 128; 0 = off) with the exact plain path; when the recall of the learned decisions on them
- * (exactly accepted features whose learned decision agrees / exactly accepted) is below
- * minRecall, the result is replaced by the plain form's before it is returned
- * (stats.nFallback = 1).  Default 128 and 0.995. */
+ * 128; 0 = off) with the exact plain path.  Of the a sampled features that the exact path
+ * accepts, m have a different learned decision (missed); when m is more than a recall of
+ * minRecall explains -- m > v3dm_fallback_allowed_misses(a, minRecall) -- the result is
+ * replaced by the plain form's before it is returned (stats.nFallback = 1).  minRecall 1:
+ * any miss falls back; 0: never.  Default 128 and 0.995. */
 int v3dm_set_fallback(v3dm_handle h, int samples, float minRecall);
+/* The misses a sampled check of `accepted` exactly accepted features tolerates at recall
+ * minRecall: the smallest k with P[X > k] <= 0.05 for X ~ Binomial(accepted, 1 - minRecall)
+ * (a one-sided test at 5%: a result whose sample misses more is taken to fall short of
+ * minRecall).  0 for minRecall >= 1, `accepted` for minRecall <= 0. */
+int v3dm_fallback_allowed_misses(int accepted, float minRecall);
 /* Measurement helper: launches the dominant kernel of the current precision mode (plain:
  * the fused score pass; BF16 / FP8 / learned: the pass over both directions) `reps` times
  * back to back on the inputs of the last findPutativeMatches,
@@ -252,8 +258,8 @@ int v3dm_batch_get_tuning(v3dm_handle h, int mode, int* tilesPerItem, int* grid,
 /* 1 (default): replay batch runs from a captured hipGraph; 0: launch kernel by kernel. */
 int v3dm_batch_use_graph(v3dm_handle h, int enable);
 /* Learned batch runs decide min(samples, nLeft) evenly spaced left features of every pair
- * exactly as well (v3dm_set_fallback); a pair whose sampled recall is below the threshold
- * gets the plain form's matches.  flags[p] = 1 for the pairs of the last run that did. */
+ * exactly as well (v3dm_set_fallback); a pair whose sampled misses exceed
+ * v3dm_fallback_allowed_misses gets the plain form's matches.  flags[p] = 1 for the pairs of the last run that did. */
 int v3dm_batch_get_fallback(v3dm_handle h, int32_t* flags);
 /* GPU time of the last batch run (HIP events around the whole run), ms. */
 int v3dm_batch_last_ms(v3dm_handle h, float* ms);
