@@ -342,18 +342,28 @@ int setFeatures(v3dm_context* h, bool left, int n, const void* src, Source kind)
 int copyResults(v3dm_context* h, int32_t* pairs, int cap, int* nPairs)
 {
     if (!h->haveResult) return fail(h, V3DM_ERR_STATE, "no result: call findPutativeMatches first");
+    // One copy of the counters and of as many pairs as the previous result held (plus a
+    // margin), enqueued before the wait for the sampled check so that both end in one round
+    // trip; the rest only when the result is larger.
+    size_t const headBytes = kNumCounters * sizeof(int);
+    size_t const prev = (size_t)std::max(h->stats.nMatches, 0);
+    size_t const guess = std::min((size_t)h->pairCap, prev + prev / 4 + 1024);
+    HIPCHECK(h, hipMemcpyAsync(h->hostStage, h->results, headBytes + guess * 2 * sizeof(int), hipMemcpyDeviceToHost,
+                               h->stream));
     int const rcf = ext_finish(h);
     if (rcf) return rcf;
-    size_t const headBytes = kNumCounters * sizeof(int);
-    HIPCHECK(h, hipMemcpyAsync(h->hostStage, h->results, headBytes, hipMemcpyDeviceToHost, h->stream));
+    if (h->stats.nFallback)  // the plain form's result replaced the learned one
+        HIPCHECK(h, hipMemcpyAsync(h->hostStage, h->results, headBytes + guess * 2 * sizeof(int),
+                                   hipMemcpyDeviceToHost, h->stream));
     HIPCHECK(h, hipStreamSynchronize(h->stream));
     if (h->hostStage[kCntMutualError] == (int)h->mutualEpoch)
         return fail(h, V3DM_ERR_HIP, "mutual check: ordering wait between work-groups ran out");
     int const n = h->hostStage[kCntPairs];
-    if (n > 0)
-        HIPCHECK(h, hipMemcpyAsync(h->hostStage + kNumCounters, h->results + kNumCounters,
-                                   (size_t)n * 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
-    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    if ((size_t)n > guess) {
+        HIPCHECK(h, hipMemcpyAsync(h->hostStage + kNumCounters + 2 * guess, h->results + kNumCounters + 2 * guess,
+                                   ((size_t)n - guess) * 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        HIPCHECK(h, hipStreamSynchronize(h->stream));
+    }
     h->stats.nMatches = n;
     h->stats.nRechecked = h->hostStage[kCntFlagDone];
     h->stats.nRescored = h->hostStage[kCntRescoredDone];
