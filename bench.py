@@ -69,6 +69,17 @@ def time_form(m, steps, warmup):
     return ms_step, kernel_ms, st
 
 
+def time_read_back(m, calls):
+    """ms per call for a caller that reads every result before the next call (the
+    reference's synchronous findPutativeMatches + match copy): findPutativeMatches with the
+    match list copied to host memory, `calls` times."""
+    m.find_putative_matches_array(MIN_SCORE, MIN_RATIO)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        m.find_putative_matches_array(MIN_SCORE, MIN_RATIO)
+    return (time.perf_counter() - t0) * 1e3 / calls
+
+
 def copy_calibration(nbytes):
     """One hipMemcpy device-to-device of nbytes (the runtime's copy kernel reads and writes
     exactly nbytes), used to scale the FETCH_SIZE / WRITE_SIZE counters."""
@@ -352,7 +363,8 @@ def main():
                             "kernel": "k_fused_fp32", "kernel_ms": kms_plain,
                             "kernel_ms_in_step": st_plain["kernel_ms_in_step"],
                             "kernel_tflops": flop / (kms_plain * 1e-3) / 1e12,
-                            "matches": int(len(exact)), "rechecked_queries": int(st_plain["nRechecked"])}}
+                            "matches": int(len(exact)), "rechecked_queries": int(st_plain["nRechecked"]),
+                            "ms_per_call_read_back": time_read_back(m, args.steps)}}
 
     ex = set(map(tuple, exact.tolist()))
 
@@ -369,6 +381,9 @@ def main():
                        "dtype": dtype, "peak": peak, "matches": len(gs),
                        "recall": inter / max(len(ex), 1), "precision": inter / max(len(gs), 1),
                        "rescored_candidates": int(st["nRescored"]), "speedup_vs_plain": ms_plain / ms}
+        m.set_precision(prec)
+        forms[name]["ms_per_call_read_back"] = time_read_back(m, args.steps)
+        m.set_precision(v3dmatch.PREC_FP32)
         if prec == v3dmatch.PREC_LEARNED:
             forms[name]["sampled_check"] = {"samples": int(st["nSampled"]), "missed": int(st["nSampleMissed"]),
                                             "fallback_to_plain": int(st["nFallback"])}
