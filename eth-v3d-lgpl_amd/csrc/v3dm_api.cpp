@@ -500,15 +500,19 @@ int v3dm_set_fallback(v3dm_handle h, int samples, float minRecall)
 
 int v3dm_fallback_allowed_misses(int accepted, float minRecall)
 {
-    if (accepted <= 0 || !(minRecall < 1.0f)) return 0;
-    if (!(minRecall > 0.0f)) return accepted;
-    // binomial tail by the pmf recurrence P[X = k + 1] = P[X = k] (a - k) / (k + 1) q / r
+    if (accepted < 0 || accepted > kFallbackMaxAccepted || !(minRecall >= 0.0f) || !(minRecall <= 1.0f)) return -1;
+    if (accepted == 0 || minRecall == 1.0f) return 0;
+    if (minRecall == 0.0f) return accepted;
+    // binomial tail by the pmf recurrence P[X = k + 1] = P[X = k] (a - k) / (k + 1) q / r,
+    // in logarithms (r^a underflows for large a)
     double const r = minRecall, q = 1.0 - r;
-    double pmf = std::pow(r, accepted), cdf = pmf;
+    double const lqr = std::log(q / r);
+    double lpmf = accepted * std::log(r);
+    double cdf = std::exp(lpmf);
     int k = 0;
     while (k < accepted && 1.0 - cdf > kFallbackAlpha) {
-        pmf *= (double)(accepted - k) / (double)(k + 1) * (q / r);
-        cdf += pmf;
+        lpmf += std::log((double)(accepted - k) / (double)(k + 1)) + lqr;
+        cdf += std::exp(lpmf);
         ++k;
     }
     return k;

@@ -19,7 +19,8 @@ constexpr int kNumCounters = 8;
 // single address serialise in L2); the gather kernel sums and clears them.
 constexpr int kSampleRows = 128;  // sampled check of a shortlist result: at most one left block
 constexpr int kSampleMaxSplits = 256;
-constexpr double kFallbackAlpha = 0.05;  // level of the sampled check's one-sided test  // ... its right features split into up to 256 work-groups
+constexpr double kFallbackAlpha = 0.05;  // level of the sampled check's one-sided test
+constexpr int kFallbackMaxAccepted = 1 << 24;  // largest count v3dm_fallback_allowed_misses takes  // ... its right features split into up to 256 work-groups
 constexpr int kStripes = 64;
 constexpr int kStripeStride = 32;  // ints (128 bytes)
 constexpr int kRecheckMaxChunks = 64;  // candidate chunks per re-checked query

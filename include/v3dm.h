@@ -131,7 +131,8 @@ int v3dm_set_fallback(v3dm_handle h, int samples, float minRecall);
 /* The misses a sampled check of `accepted` exactly accepted features tolerates at recall
  * minRecall: the smallest k with P[X > k] <= 0.05 for X ~ Binomial(accepted, 1 - minRecall)
  * (a one-sided test at 5%: a result whose sample misses more is taken to fall short of
- * minRecall).  0 for minRecall >= 1, `accepted` for minRecall <= 0. */
+ * minRecall).  0 for minRecall 1, `accepted` for minRecall 0; -1 when accepted is outside
+ * [0, 2^24] or minRecall outside [0, 1]. */
 int v3dm_fallback_allowed_misses(int accepted, float minRecall);
 /* Measurement helper: launches the dominant kernel of the current precision mode (plain:
  * the fused score pass; BF16 / FP8 / learned: the pass over both directions) `reps` times

@@ -35,3 +35,11 @@ def test_allowed_misses_edges():
     assert SIFT_Matching.fallback_allowed_misses(60, 0.995) == 1
     assert SIFT_Matching.fallback_allowed_misses(128, 0.995) == 2
     assert SIFT_Matching.fallback_allowed_misses(10, 0.995) == 0
+    # large counts (r^a underflows in plain arithmetic) and invalid arguments
+    for a in (10 ** 5, 10 ** 6):
+        k = SIFT_Matching.fallback_allowed_misses(a, 0.995)
+        p_miss = 1.0 - float(np.float32(0.995))
+        assert binom.sf(k, a, p_miss) <= ALPHA + 1e-9 and binom.sf(k - 1, a, p_miss) > ALPHA - 1e-9, (a, k)
+    assert SIFT_Matching.fallback_allowed_misses(-1, 0.995) == -1
+    assert SIFT_Matching.fallback_allowed_misses(1 << 25, 0.995) == -1
+    assert SIFT_Matching.fallback_allowed_misses(10, 1.5) == -1
