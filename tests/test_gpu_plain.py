@@ -126,3 +126,19 @@ def test_every_query_rechecked_exactly(matcher, nL, nR, seed):
         assert matcher.stats()["nRechecked"] == nL + nR
     finally:
         matcher.set_certification(2.0 ** -15)
+
+
+def test_result_larger_than_previous(matcher):
+    """The match list is copied in one piece sized from the previous result; a result much
+    larger than the previous one (here: 32 features, then 4096 features matched against
+    copies of themselves) is completed by a second copy."""
+    m = matcher
+    L, R, _ = _pair(32, 32, pair=21)
+    _check(m, L, R, 32, 32)
+    small = m.stats()["nMatches"]
+    rng = np.random.default_rng(21)
+    q = rng.integers(0, 60, (4096, 128)).astype(np.uint8)
+    X = synth.dequantise(q)
+    got = _check(m, X, X.copy(), 4096, 4096)
+    say(f"  previous result {small} matches, this one {len(got)}")
+    assert len(got) > 1024 + small + small // 4  # larger than the first copy
