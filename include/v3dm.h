@@ -260,7 +260,8 @@ int v3dm_batch_get_tuning(v3dm_handle h, int mode, int* tilesPerItem, int* grid,
 int v3dm_batch_use_graph(v3dm_handle h, int enable);
 /* Learned batch runs decide min(samples, nLeft) evenly spaced left features of every pair
  * exactly as well (v3dm_set_fallback); a pair whose sampled misses exceed
- * v3dm_fallback_allowed_misses gets the plain form's matches.  flags[p] = 1 for the pairs of the last run that did. */
+ * v3dm_fallback_allowed_misses gets the plain form's matches.  flags[p] = 1 for the pairs
+ * of the last run that did. */
 int v3dm_batch_get_fallback(v3dm_handle h, int32_t* flags);
 /* GPU time of the last batch run (HIP events around the whole run), ms. */
 int v3dm_batch_last_ms(v3dm_handle h, float* ms);
