@@ -338,7 +338,8 @@ int upload_plan(v3dm_context* h, BatchState* b, int form, Plan const& pl)
     return V3DM_OK;
 }
 
-// Items of the sampled check: (pair, 0, split, 8 tiles of 64 right features).
+// Items of the sampled check: (pair, 0, split, 8 tiles of 64 features of the image the
+// samples scan: the pair's smaller one, as sample_side).
 int ensure_sample_plan(v3dm_context* h, BatchState* b)
 {
     int const P = b->nPairs;
@@ -351,7 +352,7 @@ int ensure_sample_plan(v3dm_context* h, BatchState* b)
     for (int p = 0; p < P; ++p) {
         int const nL = seg[p].n, nR = seg[P + p].n;
         if (nL <= 0 || nR <= 0) continue;
-        int const nTiles = (nR + 63) / 64;
+        int const nTiles = ((nL >= nR ? nR : nL) + 63) / 64;
         int const splits = (nTiles + tps - 1) / tps;
         nSplits[p] = splits;
         maxS = std::max(maxS, splits);

@@ -68,6 +68,14 @@ struct BatchDev {
     int qRows;                    // size of the query space
 };
 
+// The side whose features the sampled check of pair p draws (0: left, 1: right): the one
+// with more features, so that each sample scans the smaller image (left on a tie).
+__device__ __forceinline__ int sample_side(BatchDev const& b, int p)
+{
+    return b.seg[p].n >= b.seg[b.nPairs + p].n ? 0 : 1;
+}
+
+
 // kernels of the plain FP32 form (v3dm_fp32.hip)
 // nItemsDev (may be null): the item count in device memory, at most nItems
 void launch_batch_fused_fp32(BatchDev const& b, int4 const* items, int nItems, int* queue, Top2* part, int grid,

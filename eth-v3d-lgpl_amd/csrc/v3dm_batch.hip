@@ -401,7 +401,7 @@ __global__ __launch_bounds__(256) void k_batch_sample_gather(BatchDev b, int sam
     int const r = blockIdx.x * 4 + (threadIdx.x >> 6);
     int const p = r / kSampleRows, k = r - p * kSampleRows;
     if (p >= b.nPairs) return;
-    BatchSeg const sl = b.seg[p];
+    BatchSeg const sl = b.seg[sample_side(b, p) * b.nPairs + p];
     int const S = min(samples, sl.n);
     int const i = (k < S) ? (int)(((long long)k * sl.n) / S) : -1;
     int const row = b.imgBase[sl.img] + (i >= 0 ? i : 0);
@@ -433,7 +433,7 @@ __global__ __launch_bounds__(kSampleRows) void k_batch_sample_decide(BatchDev b,
     __shared__ int red[2][3];
     int const p = blockIdx.x;
     int const k = threadIdx.x;
-    BatchSeg const sl = b.seg[p];
+    BatchSeg const sl = b.seg[sample_side(b, p) * b.nPairs + p];
     int const S = sCount[p];
     int acc = 0, miss = 0, out = 0;
     if (k < S && sl.nO > 0) {

@@ -269,7 +269,8 @@ __global__ __launch_bounds__(256, 2) void k_batch_fused_fp32(BatchDev b, int4 co
 }
 
 // Sampled check of a batch: item = (pair, -, split, tiles per split); the left block is the
-// pair's gathered sample rows (kSampleRows), left decisions only.
+// pair's gathered sample rows (kSampleRows, from the side sample_side picks) against the
+// other image, the samples' decisions only.
 __global__ __launch_bounds__(256, 2) void k_batch_sample_fp32(BatchDev b, float const* __restrict__ sRows,
                                                               int const* __restrict__ sCount,
                                                               int4 const* __restrict__ items, int nItems,
@@ -286,7 +287,7 @@ __global__ __launch_bounds__(256, 2) void k_batch_sample_fp32(BatchDev b, float 
         __syncthreads();
         if (it >= nItems) break;
         int4 const w = items[it];
-        BatchSeg const sr = b.seg[b.nPairs + w.x];
+        BatchSeg const sr = b.seg[(1 - sample_side(b, w.x)) * b.nPairs + w.x];  // the image the samples scan
         float const* R = b.img + (size_t)b.imgBase[sr.img] * kDim;
         fused_block(sRows + (size_t)w.x * kSampleRows * kDim, R, sCount[w.x], sr.n, 0, w.z, w.w,
                     sPart + (size_t)w.x * maxSplits * kSampleRows, kSampleRows, nullptr, 0, stA, stB, inf);

@@ -258,8 +258,8 @@ int v3dm_batch_set_tuning(v3dm_handle h, int autotune, int mode, int tilesPerIte
 int v3dm_batch_get_tuning(v3dm_handle h, int mode, int* tilesPerItem, int* grid, int* nItems, float* passMs);
 /* 1 (default): replay batch runs from a captured hipGraph; 0: launch kernel by kernel. */
 int v3dm_batch_use_graph(v3dm_handle h, int enable);
-/* Learned batch runs decide min(samples, nLeft) evenly spaced left features of every pair
- * exactly as well (v3dm_set_fallback); a pair whose sampled misses exceed
+/* Learned batch runs decide min(samples, n) evenly spaced features of the larger image of
+ * every pair (left on a tie) exactly as well (v3dm_set_fallback); a pair whose sampled misses exceed
  * v3dm_fallback_allowed_misses gets the plain form's matches.  flags[p] = 1 for the pairs
  * of the last run that did. */
 int v3dm_batch_get_fallback(v3dm_handle h, int32_t* flags);
