@@ -498,7 +498,25 @@ int v3dm_set_fallback(v3dm_handle h, int samples, float minRecall)
     return V3DM_OK;
 }
 
+int v3dm_set_fallback_rule(v3dm_handle h, int rule)
+{
+    if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
+    if (rule != V3DM_FALLBACK_THRESHOLD && rule != V3DM_FALLBACK_BINOMIAL)
+        return fail(h, V3DM_ERR_INVALID, "unknown fallback rule %d", rule);
+    h->fallbackRule = rule;
+    return V3DM_OK;
+}
+
 int v3dm_fallback_allowed_misses(int accepted, float minRecall)
+{
+    if (accepted < 0 || accepted > kFallbackMaxAccepted || !(minRecall >= 0.0f) || !(minRecall <= 1.0f)) return -1;
+    // the largest m whose sampled recall (a - m) / a, formed in float, is not below minRecall
+    int m = 0;
+    while (m < accepted && !((float)(accepted - m - 1) / (float)accepted < minRecall)) ++m;
+    return m;
+}
+
+int v3dm_fallback_allowed_misses_binomial(int accepted, float minRecall)
 {
     if (accepted < 0 || accepted > kFallbackMaxAccepted || !(minRecall >= 0.0f) || !(minRecall <= 1.0f)) return -1;
     if (accepted == 0 || minRecall == 1.0f) return 0;
@@ -717,3 +735,14 @@ int v3dm_run_guided_H(v3dm_handle h, const double H[9], float distanceThreshold,
 }
 
 }  // extern "C"
+
+namespace v3dm {
+
+int fallback_allowed(v3dm_context const* h, int accepted)
+{
+    return h->fallbackRule == V3DM_FALLBACK_BINOMIAL ? v3dm_fallback_allowed_misses_binomial(accepted, h->fallbackRecall)
+                                                     : v3dm_fallback_allowed_misses(accepted, h->fallbackRecall);
+}
+
+}  // namespace v3dm
+

@@ -22,16 +22,17 @@
 namespace v3dm {
 
 void launch_batch_load_u8(uint8_t const* src, int const* srcRow, int const* nImg, int const* blkImg,
-                          int const* imgBase, int rows, float* img, uint8_t* imgU8, float* denom, hipStream_t st);
+                          int const* imgBase, int rows, float* img, uint8_t* imgU8, float* denom, signed char* img8,
+                          int* rowC, float* rowS, hipStream_t st);
 void launch_batch_load_f32(float const* src, int const* srcRow, int const* nImg, int const* blkImg,
                            int const* imgBase, int rows, float* img, hipStream_t st);
 
 namespace {
 
-MissLimits miss_limits(float minRecall)
+MissLimits miss_limits(v3dm_context const* h)
 {
     MissLimits lim;
-    for (int a = 0; a <= kSampleRows; ++a) lim.allowed[a] = (unsigned char)v3dm_fallback_allowed_misses(a, minRecall);
+    for (int a = 0; a <= kSampleRows; ++a) lim.allowed[a] = (unsigned char)fallback_allowed(h, a);
     return lim;
 }
 
@@ -60,6 +61,9 @@ struct BatchState {
     signed char* codes = nullptr;  // learned codes of every image row
     uint8_t* imgU8 = nullptr;      // uint8 rows (uint8 uploads), aligned layout
     float* denom = nullptr;
+    signed char* img8 = nullptr;   // sampled check of uint8 batches: rows ^ 0x80, rowC, rowS
+    int* rowC = nullptr;
+    float* rowS = nullptr;
     bool haveU8 = false;
     bool codesValid = false;
     float codesScale = 0.0f;
@@ -119,6 +123,16 @@ struct BatchState {
     Top2* sPart = nullptr;
     int4* sItems = nullptr;
     int nSItems = 0, sMaxSplits = 1;
+    // uint8 batches: the two phases of the mutual sampled check (v3dm_check.hip)
+    int4* cItems[2] = {nullptr, nullptr};
+    int nCItems[2] = {0, 0};
+    int* cNSplits = nullptr;
+    int4* cPart = nullptr;
+    int cMaxSplits = 1;
+    int* cQB = nullptr;
+    int* cSlotB = nullptr;
+    int* cCntB = nullptr;
+    int* cDecA = nullptr;
     int* pairFlag = nullptr;  // nPairs flags followed by 4 counts per pair
     int* hostChk = nullptr;   // pinned copy
     int4* fbItems = nullptr;
@@ -208,6 +222,23 @@ BatchDev dev_view(BatchState* b, int form)
     d.nPairs = b->nPairs;
     d.qRows = b->qRows;
     return d;
+}
+
+CheckDev check_view(BatchState* b, int samples)
+{
+    CheckDev c;
+    c.img8 = b->img8;
+    c.rowC = b->rowC;
+    c.rowS = b->rowS;
+    c.part = b->cPart;
+    c.maxSplits = b->cMaxSplits;
+    c.nSplits = b->cNSplits;
+    c.qB = b->cQB;
+    c.slotB = b->cSlotB;
+    c.cntB = b->cCntB;
+    c.decA = b->cDecA;
+    c.samples = samples;
+    return c;
 }
 
 // Query space and the segment skeleton (partials filled in by the plans).
@@ -338,50 +369,109 @@ int upload_plan(v3dm_context* h, BatchState* b, int form, Plan const& pl)
     return V3DM_OK;
 }
 
-// Items of the sampled check: (pair, 0, split, 8 tiles of 64 features of the image the
-// samples scan: the pair's smaller one, as sample_side).
+// Items of the sampled check.  uint8 images (the mutual check of v3dm_check.hip): per phase
+// (pair, -, split, kCheckTiles 32-row tiles) of the image the phase scans (phase 0: the
+// pair's smaller image, phase 1: the larger).  FP32 images: (pair, 0, split, 8 tiles of 64
+// features of the smaller image) for the samples' own decisions.
+constexpr int kCheckTiles = 16;
+
 int ensure_sample_plan(v3dm_context* h, BatchState* b)
 {
     int const P = b->nPairs;
     std::vector<BatchSeg> seg;
     layout_segments(b, seg);
-    std::vector<std::pair<int, int4>> work;
-    std::vector<int> nSplits(P, 0);
-    int const tps = 8;
-    int maxS = 1;
-    for (int p = 0; p < P; ++p) {
-        int const nL = seg[p].n, nR = seg[P + p].n;
-        if (nL <= 0 || nR <= 0) continue;
-        int const nTiles = ((nL >= nR ? nR : nL) + 63) / 64;
-        int const splits = (nTiles + tps - 1) / tps;
-        nSplits[p] = splits;
-        maxS = std::max(maxS, splits);
-        for (int s = 0; s < splits; ++s) work.push_back({std::min(tps, nTiles - s * tps), make_int4(p, 0, s, tps)});
-    }
-    std::stable_sort(work.begin(), work.end(),
-                     [](std::pair<int, int4> const& x, std::pair<int, int4> const& y) { return x.first > y.first; });
-    std::vector<int4> items;
-    for (auto& w : work) items.push_back(w.second);
     for (void* ptr : {(void*)b->sRows, (void*)b->sNorm, (void*)b->sIdx, (void*)b->sCount, (void*)b->sNSplits,
-                      (void*)b->sPart, (void*)b->sItems, (void*)b->pairFlag, (void*)b->segMask})
+                      (void*)b->sPart, (void*)b->sItems, (void*)b->pairFlag, (void*)b->segMask, (void*)b->cItems[0],
+                      (void*)b->cItems[1], (void*)b->cNSplits, (void*)b->cPart, (void*)b->cQB, (void*)b->cSlotB,
+                      (void*)b->cCntB, (void*)b->cDecA})
         if (ptr) (void)hipFree(ptr);
+    b->sRows = b->sNorm = nullptr;
+    b->sIdx = b->sCount = b->sNSplits = nullptr;
+    b->sPart = nullptr;
+    b->sItems = nullptr;
+    b->cItems[0] = b->cItems[1] = nullptr;
+    b->cNSplits = b->cQB = b->cSlotB = b->cCntB = b->cDecA = nullptr;
+    b->cPart = nullptr;
+    b->nSItems = b->nCItems[0] = b->nCItems[1] = 0;
     if (b->hostChk) (void)hipHostFree(b->hostChk);
-    size_t const rows = (size_t)P * kSampleRows;
-    BCHECK(h, hipMalloc(&b->sRows, rows * kDim * sizeof(float)));
-    BCHECK(h, hipMalloc(&b->sNorm, rows * sizeof(float)));
-    BCHECK(h, hipMalloc(&b->sIdx, rows * sizeof(int)));
-    BCHECK(h, hipMalloc(&b->sCount, P * sizeof(int)));
-    BCHECK(h, hipMalloc(&b->sNSplits, P * sizeof(int)));
-    BCHECK(h, hipMalloc(&b->sPart, rows * maxS * sizeof(Top2)));
-    BCHECK(h, hipMalloc(&b->sItems, std::max<size_t>(items.size(), 1) * sizeof(int4)));
+    b->hostChk = nullptr;
+    auto sorted_items = [](std::vector<std::pair<int, int4>>& work) {
+        std::stable_sort(work.begin(), work.end(),
+                         [](std::pair<int, int4> const& x, std::pair<int, int4> const& y) { return x.first > y.first; });
+        std::vector<int4> items;
+        for (auto& w : work) items.push_back(w.second);
+        return items;
+    };
+    if (b->haveU8) {
+        std::vector<int> nSplits(2 * P, 0);
+        int maxS = 1;
+        std::vector<int4> items[2];
+        for (int phase = 0; phase < 2; ++phase) {
+            std::vector<std::pair<int, int4>> work;
+            for (int p = 0; p < P; ++p) {
+                int const nL = seg[p].n, nR = seg[P + p].n;
+                if (nL <= 0 || nR <= 0) continue;
+                int const nA = std::max(nL, nR), nB = std::min(nL, nR);
+                int const nTiles = (phase == 0 ? nB : nA) / 32;
+                int const splits = (nTiles + kCheckTiles - 1) / kCheckTiles;
+                nSplits[phase * P + p] = splits;
+                maxS = std::max(maxS, splits);
+                for (int sp = 0; sp < splits; ++sp)
+                    work.push_back({std::min(kCheckTiles, nTiles - sp * kCheckTiles), make_int4(p, 0, sp, kCheckTiles)});
+            }
+            items[phase] = sorted_items(work);
+        }
+        size_t const slots = (size_t)P * kSampleRows;
+        for (int phase = 0; phase < 2; ++phase) {
+            BCHECK(h, hipMalloc(&b->cItems[phase], std::max<size_t>(items[phase].size(), 1) * sizeof(int4)));
+            if (!items[phase].empty())
+                BCHECK(h, hipMemcpy(b->cItems[phase], items[phase].data(), items[phase].size() * sizeof(int4),
+                                    hipMemcpyHostToDevice));
+            b->nCItems[phase] = (int)items[phase].size();
+        }
+        BCHECK(h, hipMalloc(&b->cNSplits, 2 * P * sizeof(int)));
+        BCHECK(h, hipMemcpy(b->cNSplits, nSplits.data(), 2 * P * sizeof(int), hipMemcpyHostToDevice));
+        BCHECK(h, hipMalloc(&b->cPart, 2 * slots * maxS * sizeof(int4)));
+        BCHECK(h, hipMalloc(&b->cQB, slots * sizeof(int)));
+        BCHECK(h, hipMalloc(&b->cSlotB, slots * sizeof(int)));
+        BCHECK(h, hipMalloc(&b->cDecA, slots * sizeof(int)));
+        BCHECK(h, hipMalloc(&b->cCntB, P * sizeof(int)));
+        b->cMaxSplits = maxS;
+    } else {
+        std::vector<std::pair<int, int4>> work;
+        std::vector<int> nSplits(P, 0);
+        int const tps = 8;
+        int maxS = 1;
+        for (int p = 0; p < P; ++p) {
+            int const nL = seg[p].n, nR = seg[P + p].n;
+            if (nL <= 0 || nR <= 0) continue;
+            int const nTiles = ((nL >= nR ? nR : nL) + 63) / 64;
+            int const splits = (nTiles + tps - 1) / tps;
+            nSplits[p] = splits;
+            maxS = std::max(maxS, splits);
+            for (int sp = 0; sp < splits; ++sp)
+                work.push_back({std::min(tps, nTiles - sp * tps), make_int4(p, 0, sp, tps)});
+        }
+        std::vector<int4> items = sorted_items(work);
+        size_t const rows = (size_t)P * kSampleRows;
+        BCHECK(h, hipMalloc(&b->sRows, rows * kDim * sizeof(float)));
+        BCHECK(h, hipMalloc(&b->sNorm, rows * sizeof(float)));
+        BCHECK(h, hipMalloc(&b->sIdx, rows * sizeof(int)));
+        BCHECK(h, hipMalloc(&b->sCount, P * sizeof(int)));
+        BCHECK(h, hipMalloc(&b->sNSplits, P * sizeof(int)));
+        BCHECK(h, hipMalloc(&b->sPart, rows * maxS * sizeof(Top2)));
+        BCHECK(h, hipMalloc(&b->sItems, std::max<size_t>(items.size(), 1) * sizeof(int4)));
+        BCHECK(h, hipMemcpy(b->sNSplits, nSplits.data(), P * sizeof(int), hipMemcpyHostToDevice));
+        if (!items.empty())
+            BCHECK(h, hipMemcpy(b->sItems, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+        b->nSItems = (int)items.size();
+        b->sMaxSplits = maxS;
+    }
     BCHECK(h, hipMalloc(&b->pairFlag, 5 * P * sizeof(int)));
     BCHECK(h, hipMalloc(&b->segMask, 2 * P * sizeof(int)));
+    BCHECK(h, hipMemset(b->pairFlag, 0, 5 * P * sizeof(int)));
     BCHECK(h, hipHostMalloc(&b->hostChk, 5 * P * sizeof(int), hipHostMallocDefault));
-    BCHECK(h, hipMemcpy(b->sNSplits, nSplits.data(), P * sizeof(int), hipMemcpyHostToDevice));
-    if (!items.empty())
-        BCHECK(h, hipMemcpy(b->sItems, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
-    b->nSItems = (int)items.size();
-    b->sMaxSplits = maxS;
+    std::memset(b->hostChk, 0, 5 * P * sizeof(int));
     b->checkPending = false;
     return V3DM_OK;
 }
@@ -465,25 +555,38 @@ int enqueue_run(v3dm_context* h, BatchState* b, int mode, float minScore, float 
         launch_batch_recheck(d, b->flagList, b->counters, minScore, minRatio, b->sel, 1024, st);
     } else if (form == 1) {
         launch_batch_pass_bf16(d, b->items[1], (int)pl.items.size(), b->counters + 1, b->partB, pl.grid, st);
-        launch_batch_rescore(d, b->partB, kWinRelBf16, 0, INFINITY, minScore, minRatio, b->sel, b->stripes, st);
+        launch_batch_rescore(d, b->partB, kWinRelBf16, 0, INFINITY, minScore, minRatio, h->epsRel > 0.0f, b->sel,
+                             b->stripes, st);
     } else {
         launch_batch_pass_code(d, b->items[2], (int)pl.items.size(), b->counters + 2, b->partB, pl.grid, st);
-        launch_batch_rescore(d, b->partB, 0.0f, h->learnedTopK, learned_margin(h), minScore, minRatio, b->sel,
-                             b->stripes, st);
+        launch_batch_rescore(d, b->partB, 0.0f, h->learnedTopK, learned_margin(h), minScore, minRatio,
+                             h->epsRel > 0.0f, b->sel, b->stripes, st);
         if (h->fallbackSamples > 0) {
             int dev = 0, cus = 256;
             (void)hipGetDevice(&dev);
             (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-            launch_batch_sample_gather(d, h->fallbackSamples, b->sRows, b->sNorm, b->sIdx, b->sCount, st);
-            launch_batch_sample_fp32(d, b->sRows, b->sCount, b->sItems, b->nSItems, b->counters + 3, b->sPart,
-                                     b->sMaxSplits, 2 * cus, st);
-            launch_batch_sample_decide(d, b->sPart, b->sMaxSplits, b->sNSplits, b->sNorm, b->sIdx, b->sCount,
-                                       h->epsRel, minScore, minRatio, miss_limits(h->fallbackRecall), b->sel,
-                                       b->pairFlag,
-                                       b->pairFlag + b->nPairs, st);
-            // pairs whose sampled recall fell below the threshold get the plain form's
-            // decisions, on the device: their plain items, merge masked to their segments,
-            // exact re-check (kernels with nothing to do when no pair is flagged)
+            MissLimits const lim = miss_limits(h);
+            if (b->haveU8) {
+                // exact mutual matches among the samples against the learned result
+                CheckDev const c = check_view(b, h->fallbackSamples);
+                launch_batch_check_probe(d, c, 0, b->cItems[0], b->nCItems[0], b->counters + kBQueueCheck0, 2 * cus, st);
+                launch_batch_check_decide(d, c, 0, minScore, minRatio, lim, b->sel, b->pairFlag,
+                                          b->pairFlag + b->nPairs, st);
+                launch_batch_check_probe(d, c, 1, b->cItems[1], b->nCItems[1], b->counters + kBQueueCheck1, 2 * cus, st);
+                launch_batch_check_decide(d, c, 1, minScore, minRatio, lim, b->sel, b->pairFlag,
+                                          b->pairFlag + b->nPairs, st);
+            } else {
+                // FP32 images: the samples' own decisions against the learned ones
+                launch_batch_sample_gather(d, h->fallbackSamples, b->sRows, b->sNorm, b->sIdx, b->sCount, st);
+                launch_batch_sample_fp32(d, b->sRows, b->sCount, b->sItems, b->nSItems, b->counters + kBQueueCheck0, b->sPart,
+                                         b->sMaxSplits, 2 * cus, st);
+                launch_batch_sample_decide(d, b->sPart, b->sMaxSplits, b->sNSplits, b->sNorm, b->sIdx, b->sCount,
+                                           h->epsRel, minScore, minRatio, lim, b->sel, b->pairFlag,
+                                           b->pairFlag + b->nPairs, st);
+            }
+            // pairs whose sampled recall fell short get the plain form's decisions, on the
+            // device: their plain items, merge masked to their segments, exact re-check
+            // (kernels with nothing to do when no pair is flagged)
             Plan const& pp = b->plan[0];
             int const np = (int)pp.items.size();
             BatchDev const d0 = dev_view(b, 0);
@@ -664,10 +767,13 @@ int set_images(v3dm_context* h, int nImages, int32_t const* counts, void const* 
         if (b->norm) (void)hipFree(b->norm);
         if (b->codes) (void)hipFree(b->codes);  // sized with the image rows: allocated again on use
         b->codes = nullptr;
-        if (b->imgU8) (void)hipFree(b->imgU8);
-        if (b->denom) (void)hipFree(b->denom);
+        for (void* ptr : {(void*)b->imgU8, (void*)b->denom, (void*)b->img8, (void*)b->rowC, (void*)b->rowS})
+            if (ptr) (void)hipFree(ptr);
         b->imgU8 = nullptr;
         b->denom = nullptr;
+        b->img8 = nullptr;
+        b->rowC = nullptr;
+        b->rowS = nullptr;
         b->img = nullptr;
         b->imgB = nullptr;
         b->norm = nullptr;
@@ -721,11 +827,14 @@ int set_images(v3dm_context* h, int nImages, int32_t const* counts, void const* 
     if (u8 && !b->imgU8) {
         BCHECK(h, hipMalloc(&b->imgU8, (size_t)b->imgCapRows * kDim));
         BCHECK(h, hipMalloc(&b->denom, (size_t)b->imgCapRows * sizeof(float)));
+        BCHECK(h, hipMalloc(&b->img8, (size_t)b->imgCapRows * kDim));
+        BCHECK(h, hipMalloc(&b->rowC, (size_t)b->imgCapRows * sizeof(int)));
+        BCHECK(h, hipMalloc(&b->rowS, (size_t)b->imgCapRows * sizeof(float)));
     }
     b->haveU8 = u8;
     if (u8)
         launch_batch_load_u8(static_cast<uint8_t const*>(b->upload), b->srcRow, b->nImgD, b->blkImg, b->imgBase, rows,
-                             b->img, b->imgU8, b->denom, st);
+                             b->img, b->imgU8, b->denom, b->img8, b->rowC, b->rowS, st);
     else
         launch_batch_load_f32(static_cast<float const*>(b->upload), b->srcRow, b->nImgD, b->blkImg, b->imgBase, rows,
                               b->img, st);
@@ -748,7 +857,9 @@ void batch_destroy(BatchState*& b)
                     (void*)b->items[0], (void*)b->items[1], (void*)b->segD[0], (void*)b->segD[1], (void*)b->partF,
                     (void*)b->partB, (void*)b->sel, (void*)b->flagList, (void*)b->sRows, (void*)b->sNorm, (void*)b->sIdx, (void*)b->sCount,
                     (void*)b->sNSplits, (void*)b->sPart, (void*)b->sItems, (void*)b->pairFlag, (void*)b->fbItems,
-                    (void*)b->segMask,  (void*)b->counters, (void*)b->stripes, (void*)b->pairCount,
+                    (void*)b->segMask, (void*)b->img8, (void*)b->rowC, (void*)b->rowS, (void*)b->cItems[0],
+                    (void*)b->cItems[1], (void*)b->cNSplits, (void*)b->cPart, (void*)b->cQB, (void*)b->cSlotB,
+                    (void*)b->cCntB, (void*)b->cDecA, (void*)b->counters, (void*)b->stripes, (void*)b->pairCount,
                     (void*)b->tmp, (void*)b->blockCount, (void*)b->out})
         if (p) (void)hipFree(p);
     if (b->pinned) (void)hipHostFree(b->pinned);
@@ -955,6 +1066,15 @@ int v3dm_batch_get_fallback(v3dm_handle h, int32_t* flags)
     BatchState* b = ctx_batch(h);
     if (!b || b->nPairs <= 0 || !b->hostChk) return ctx_fail(h, V3DM_ERR_STATE, "no batch");
     for (int p = 0; p < b->nPairs; ++p) flags[p] = b->hostChk[p];
+    return V3DM_OK;
+}
+
+int v3dm_batch_get_check(v3dm_handle h, int32_t* counts)
+{
+    if (!h || !counts) return V3DM_ERR_INVALID;
+    BatchState* b = ctx_batch(h);
+    if (!b || b->nPairs <= 0 || !b->hostChk) return ctx_fail(h, V3DM_ERR_STATE, "no batch");
+    for (int k = 0; k < 4 * b->nPairs; ++k) counts[k] = b->hostChk[b->nPairs + k];
     return V3DM_OK;
 }
 

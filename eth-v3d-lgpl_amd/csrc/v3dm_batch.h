@@ -28,6 +28,8 @@ constexpr int kBCntPairs = 6;
 constexpr int kBCntFlagDone = 7;
 constexpr int kBCntRescoredDone = 8;
 constexpr int kBCntFbItems = 10;  // plain items of the pairs a learned run hands to the plain form
+constexpr int kBQueueCheck0 = 3;  // work counters of the sampled check (phase 0 / FP32 samples, phase 1)
+constexpr int kBQueueCheck1 = 11;
 constexpr int kBatchCounters = 16;
 
 struct BatchSeg {
@@ -68,6 +70,26 @@ struct BatchDev {
     int qRows;                    // size of the query space
 };
 
+// Sampled check of uint8 batches (v3dm_check.hip).  Per image row: img8 = the uint8 row ^
+// 0x80 (int8 u - 128), rowC = 128 * sum of the uint8 components, rowS = the row's 1/sqrt
+// scale; rows that are no candidate (all-zero rows, padding) have rowC = kDegenerateC and
+// rowS = 1.
+constexpr int kDegenerateC = -(1 << 30);
+
+struct CheckDev {
+    signed char const* img8;
+    int const* rowC;
+    float const* rowS;
+    int4* part;          // keys per (phase, pair, split, query slot)
+    int maxSplits;
+    int const* nSplits;  // per (phase, pair)
+    int* qB;             // phase 1 queries: matched features of B, per pair kSampleRows slots
+    int* slotB;          // ... and the sample slot that matched each
+    int* cntB;           // ... and their number
+    int* decA;           // exact decisions of the phase 0 samples
+    int samples;
+};
+
 // The side whose features the sampled check of pair p draws (0: left, 1: right): the one
 // with more features, so that each sample scans the smaller image (left on a tie).
 __device__ __forceinline__ int sample_side(BatchDev const& b, int p)
@@ -83,8 +105,10 @@ void launch_batch_fused_fp32(BatchDev const& b, int4 const* items, int nItems, i
 // shortlist pass (v3dm_short.hip)
 void launch_batch_pass_bf16(BatchDev const& b, int4 const* items, int nItems, int* queue, Cand2* part, int grid,
                             hipStream_t st);
+// uint8 images: certified integer scores, open decisions scored exactly (certify = false:
+// every decision scored exactly); FP32 images: exact scores
 void launch_batch_rescore(BatchDev const& b, Cand2 const* part, float winRel, int topK, float margin, float minScore,
-                          float minRatio, int* sel, int* stripes, hipStream_t st);
+                          float minRatio, bool certify, int* sel, int* stripes, hipStream_t st);
 // learned code pass (v3dm_code.hip)
 void launch_batch_pass_code(BatchDev const& b, int4 const* items, int nItems, int* queue, Cand2* part, int grid,
                             hipStream_t st);
@@ -104,6 +128,13 @@ void launch_batch_sample_decide(BatchDev const& b, Top2 const* sPart, int maxSpl
                                 float const* sNorm, int const* sIdx, int const* sCount, float epsRel, float minScore,
                                 float minRatio, MissLimits const& lim, int const* sel, int* pairFlag, int* counts,
                                 hipStream_t st);
+// uint8 batches: phase 0 (samples of the larger image) and phase 1 (their exact partners);
+// the phase 1 decide sets pairFlag[p] and counts[4p..] = exact matches among the samples,
+// missed, decisions taken exactly, samples
+void launch_batch_check_probe(BatchDev const& b, CheckDev const& c, int phase, int4 const* items, int nItems,
+                              int* queue, int grid, hipStream_t st);
+void launch_batch_check_decide(BatchDev const& b, CheckDev const& c, int phase, float minScore, float minRatio,
+                               MissLimits const& lim, int const* sel, int* pairFlag, int* counts, hipStream_t st);
 void launch_batch_recheck(BatchDev const& b, int const* flagList, int* counters, float minScore, float minRatio,
                           int* sel, int grid, hipStream_t st);
 // the plain items of the pairs with pairFlag set -> fbItems (count in *fbCount, which must

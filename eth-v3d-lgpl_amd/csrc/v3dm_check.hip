@@ -1,0 +1,349 @@
+// Sampled exact check of a learned batch run over uint8 images: the sampled recall of the
+// pair's exact match set.
+//
+// Per pair p, side A is the image with more features (left on a tie) and B the other.
+// Phase 0 decides S = min(samples, nA) evenly spaced features i_k of A exactly against B
+// (the per-feature rule of V3D/CUDA/v3d_cudamatching.cu:133-210); phase 1 decides, against
+// A, every feature j_k of B that phase 0 matched.  Sample k is an exact match when phase 1
+// returns i_k (the mutual check of cu:546-551 restricted to the samples), and the learned
+// result holds it when its decisions of i_k and j_k are j_k and i_k.  The pair falls back
+// to the plain form when its misses exceed the limit for its number of exact matches.
+//
+// Scores: the rows are kept as int8 (u ^ 0x80 = u - 128), so one v_mfma_i32_32x32x32_i8
+// chain over 128 components gives D = sum_k (qa_k - 128)(qb_k - 128), and with the MFMA's
+// accumulator input set to 128 * sum(qb) the integer I = sum_k qa_k qb_k = D + 128 * sum(qa)
+// - 2^21 is exact.  t = fl32(rb * I) and s~ = fl32(ra * t) (ra, rb the rows' 1/sqrt
+// scales) differ from the exact score fl32(sum_k fl32(qa_k ra) fl32(qb_k rb)) by at most
+// 5.01 * 2^-24 * s~ (every term is nonnegative); the candidates' keys keep t with its five
+// low mantissa bits replaced by the row inside the 32-row block (2^-18 relative), so a
+// decision from the keys is certified with eps = 2^-17 * s~1.  A decision the bound leaves
+// open is taken exactly: the candidates whose score could be in the exact top-2 (s~ >=
+// s~2 - 2 eps) are found again from the integer scores and scored with the oracle's
+// arithmetic (FP64 sum of the FP32 products, ascending k).
+#include <climits>
+
+#include "v3dm_batch.h"
+#include "v3dm_device.h"
+
+namespace v3dm {
+
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+constexpr int kChkTile = 32;       // candidate rows per MFMA block
+constexpr int kChkCand = 256;      // candidates an open decision scores exactly (more: all of them)
+constexpr float kCertChk = 7.62939453125e-06f;  // 2^-17
+
+__device__ __forceinline__ int chk_nq(BatchDev const& b, CheckDev const& c, int phase, int p, BatchSeg const& sA)
+{
+    return phase == 0 ? min(c.samples, sA.n) : c.cntB[p];
+}
+
+// image-local row of query slot k (phase 0: the k-th evenly spaced feature of A)
+__device__ __forceinline__ int chk_query(CheckDev const& c, int phase, int p, int k, int nQ, int nA)
+{
+    return phase == 0 ? (int)(((long long)k * nA) / nQ) : c.qB[p * kSampleRows + k];
+}
+
+__device__ __forceinline__ float key_value(int key) { return __int_as_float(key & ~31); }
+
+// exact score of image rows q and c (the oracle's arithmetic on the dequantised floats)
+__device__ __forceinline__ float exact_score(float const* __restrict__ q, float const* __restrict__ c)
+{
+    float4 const* qp = reinterpret_cast<float4 const*>(q);
+    float4 const* cp = reinterpret_cast<float4 const*>(c);
+    double acc = 0.0;
+#pragma unroll 8
+    for (int u = 0; u < 32; ++u) {
+        float4 const x = qp[u], y = cp[u];
+        acc += (double)x.x * (double)y.x;
+        acc += (double)x.y * (double)y.y;
+        acc += (double)x.z * (double)y.z;
+        acc += (double)x.w * (double)y.w;
+    }
+    return (float)acc;
+}
+
+// Top2 of a work-group of 128 threads (two waves), the order-independent merge
+__device__ __forceinline__ Top2 wg_top2(Top2 s, Top2* red)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s = top2_merge(s, top2_shfl_xor(s, o));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    return top2_merge(red[0], red[1]);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------- probe
+// Persistent over items (pair, -, split, tiles per split): the pair's (up to 128) query
+// rows of this phase, 32 per wave, against the split's 32-row candidate tiles.  Per lane
+// (query l % 32, candidate rows 8 (r / 4) + 4 (l / 32) + r % 4 of each block) the two best
+// keys and the tile of the best; the lane halves merge at the end.  part: int4 (key1, key2,
+// tile of key1, 0) per (phase, pair, split, query slot).
+__global__ __launch_bounds__(256) void k_batch_check_probe(BatchDev b, CheckDev c, int phase,
+                                                           int4 const* __restrict__ items, int nItems,
+                                                           int* __restrict__ queue)
+{
+    __shared__ int sItem;
+    int const lane = threadIdx.x & 63;
+    int const wave = threadIdx.x >> 6;
+    int const h = lane >> 5;
+    int const c32 = lane & 31;
+    for (;;) {
+        if (threadIdx.x == 0) sItem = atomicAdd(queue, 1);
+        __syncthreads();
+        int const it = __builtin_amdgcn_readfirstlane(sItem);
+        __syncthreads();
+        if (it >= nItems) break;
+        int4 const w = items[it];
+        int const p = w.x;
+        int const sa = sample_side(b, p);
+        BatchSeg const sA = b.seg[sa * b.nPairs + p];
+        BatchSeg const sB = b.seg[(1 - sa) * b.nPairs + p];
+        int const nQ = chk_nq(b, c, phase, p, sA);
+        if (wave * 32 >= nQ) continue;  // uniform over the wave; no barrier is skipped
+        int const k = wave * 32 + c32;
+        int const qLocal = chk_query(c, phase, p, min(k, nQ - 1), nQ, sA.n);
+        size_t const qRow = (size_t)b.imgBase[phase == 0 ? sA.img : sB.img] + qLocal;
+        int const cImg = phase == 0 ? sB.img : sA.img;
+        int const nC = phase == 0 ? sB.n : sA.n;
+        v4i bq[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) bq[m] = *reinterpret_cast<v4i const*>(c.img8 + qRow * kDim + 32 * m + 16 * h);
+        int const cq = c.rowC[qRow] - (1 << 21);
+        size_t const cBase = (size_t)b.imgBase[cImg];
+        int const t0 = w.z * w.w;
+        int const t1 = min(t0 + w.w, nC / kChkTile);
+        int k1 = INT_MIN, k2 = INT_MIN, blk1 = -1;
+        for (int t = t0; t < t1; ++t) {
+            size_t const r0 = cBase + (size_t)t * kChkTile;
+            v4i a[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                a[m] = *reinterpret_cast<v4i const*>(c.img8 + (r0 + c32) * kDim + 32 * m + 16 * h);
+            v16i cv;
+            float sv[16];
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                int4 const ci = *reinterpret_cast<int4 const*>(c.rowC + r0 + 8 * g + 4 * h);
+                float4 const si = *reinterpret_cast<float4 const*>(c.rowS + r0 + 8 * g + 4 * h);
+                cv[4 * g + 0] = ci.x;
+                cv[4 * g + 1] = ci.y;
+                cv[4 * g + 2] = ci.z;
+                cv[4 * g + 3] = ci.w;
+                sv[4 * g + 0] = si.x;
+                sv[4 * g + 1] = si.y;
+                sv[4 * g + 2] = si.z;
+                sv[4 * g + 3] = si.w;
+            }
+            v16i d = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[0], bq[0], cv, 0, 0, 0);
+            d = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[1], bq[1], d, 0, 0, 0);
+            d = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[2], bq[2], d, 0, 0, 0);
+            d = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[3], bq[3], d, 0, 0, 0);
+            int const o = k1;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                float const tv = sv[r] * (float)(d[r] + cq);  // degenerate rows: negative
+                int const key = (__float_as_int(tv) & ~31) | (8 * (r >> 2) + 4 * h + (r & 3));
+                k2 = max(k2, min(k1, key));
+                k1 = max(k1, key);
+            }
+            blk1 = (k1 != o) ? t : blk1;
+        }
+        int const ok1 = __shfl_xor(k1, 32), ok2 = __shfl_xor(k2, 32), ob = __shfl_xor(blk1, 32);
+        bool const takeO = ok1 > k1 || (ok1 == k1 && ob >= 0 && (blk1 < 0 || ob < blk1));
+        int const n1 = max(k1, ok1);
+        int const n2 = max(min(k1, ok1), max(k2, ok2));
+        if (h == 0 && k < nQ)
+            c.part[(((size_t)phase * b.nPairs + p) * c.maxSplits + w.z) * kSampleRows + k] =
+                make_int4(n1, n2, takeO ? ob : blk1, 0);
+    }
+}
+
+// ---------------------------------------------------------------------------- decide
+// One work-group (128 threads, thread = query slot) per pair: merge the splits' keys, the
+// certified decision, the exact decision of open queries (the whole work-group per open
+// query), then phase 0: the decisions and the list of matched B features for phase 1;
+// phase 1: the comparison with the learned decisions and the pair's fallback flag.
+__global__ __launch_bounds__(128) void k_batch_check_decide(BatchDev b, CheckDev c, int phase, float minScore,
+                                                            float minRatio, MissLimits lim,
+                                                            int const* __restrict__ sel,
+                                                            int* __restrict__ pairFlag, int* __restrict__ counts)
+{
+    __shared__ int openSlot[kSampleRows];
+    __shared__ float openThr[kSampleRows];
+    __shared__ int openDec[kSampleRows];
+    __shared__ int cand[kChkCand];
+    __shared__ int nOpen, nCand;
+    __shared__ Top2 red[2];
+    __shared__ int wcnt[2][3];
+    int const p = blockIdx.x;
+    int const k = threadIdx.x;
+    int const sa = sample_side(b, p);
+    BatchSeg const sA = b.seg[sa * b.nPairs + p];
+    BatchSeg const sB = b.seg[(1 - sa) * b.nPairs + p];
+    int const nQ = (sA.n > 0 && sB.n > 0) ? chk_nq(b, c, phase, p, sA) : 0;
+    int const qImg = phase == 0 ? sA.img : sB.img;
+    int const cImg = phase == 0 ? sB.img : sA.img;
+    int const nC = phase == 0 ? sB.n : sA.n;
+    size_t const cBase = (size_t)b.imgBase[cImg];
+    if (k == 0) nOpen = 0;
+    __syncthreads();
+    int d = -1;
+    int qLocal = -1;
+    if (k < nQ) {
+        qLocal = chk_query(c, phase, p, k, nQ, sA.n);
+        float const ra = b.denom[(size_t)b.imgBase[qImg] + qLocal];
+        if (ra > 0.0f && ra < INFINITY) {  // else an all-zero row: every exact score is NaN
+            int K1 = INT_MIN, K2 = INT_MIN, B1 = -1;
+            int const ns = c.nSplits[phase * b.nPairs + p];
+            int4 const* pp = c.part + ((size_t)phase * b.nPairs + p) * c.maxSplits * kSampleRows + k;
+            for (int s = 0; s < ns; ++s) {
+                int4 const e = pp[(size_t)s * kSampleRows];
+                bool const take = e.x > K1;  // splits in ascending order: equal keys keep the first
+                K2 = max(max(K2, e.y), min(K1, e.x));
+                B1 = take ? e.z : B1;
+                K1 = max(K1, e.x);
+            }
+            float const s1 = K1 >= 0 ? ra * key_value(K1) : kInit;
+            float const s2 = K2 >= 0 ? ra * key_value(K2) : kInit;
+            int const a1 = K1 >= 0 ? B1 * kChkTile + (K1 & 31) : -1;
+            float const eps = kCertChk * fmaxf(s1, 0.0f) + kEpsAbs;
+            d = decide_certified(Top2{s1, s2, a1}, eps, minScore, minRatio);
+            if (d == kFlag) {
+                int const o = atomicAdd(&nOpen, 1);
+                openSlot[o] = k;
+                openThr[o] = s2 - 2.0f * eps;
+            }
+        }
+    }
+    __syncthreads();
+    // exact decisions of the open queries
+    int const nO = nOpen;
+    for (int o = 0; o < nO; ++o) {
+        int const kq = openSlot[o];
+        int const ql = chk_query(c, phase, p, kq, nQ, sA.n);
+        size_t const qRow = (size_t)b.imgBase[qImg] + ql;
+        float const ra = b.denom[qRow];
+        float const thr = openThr[o];
+        if (k == 0) nCand = 0;
+        __syncthreads();
+        uint4 qv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) qv[u] = reinterpret_cast<uint4 const*>(b.u8 + qRow * kDim)[u];
+        for (int j = k; j < nC; j += blockDim.x) {
+            size_t const row = cBase + j;
+            if (c.rowC[row] == kDegenerateC) continue;
+            uint4 const* cr = reinterpret_cast<uint4 const*>(b.u8 + row * kDim);
+            unsigned int acc = 0;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                uint4 const y = cr[u];
+                acc = __builtin_amdgcn_udot4(qv[u].x, y.x, acc, false);
+                acc = __builtin_amdgcn_udot4(qv[u].y, y.y, acc, false);
+                acc = __builtin_amdgcn_udot4(qv[u].z, y.z, acc, false);
+                acc = __builtin_amdgcn_udot4(qv[u].w, y.w, acc, false);
+            }
+            float const sv = ra * (c.rowS[row] * (float)acc);
+            if (sv >= thr) {
+                int const e = atomicAdd(&nCand, 1);
+                if (e < kChkCand) cand[e] = j;
+            }
+        }
+        __syncthreads();
+        int const nc = nCand;
+        float const* qf = b.img + qRow * kDim;
+        Top2 s = top2_init();
+        if (nc <= kChkCand) {
+            for (int e = k; e < nc; e += blockDim.x) {
+                int const j = cand[e];
+                float const sc = exact_score(qf, b.img + (cBase + j) * kDim);
+                if (sc == sc) s = top2_merge(s, Top2{sc, kInit, j});
+            }
+        } else {  // a pile of near ties: every candidate exactly
+            for (int j = k; j < nC; j += blockDim.x) {
+                float const sc = exact_score(qf, b.img + (cBase + j) * kDim);
+                if (sc == sc) s = top2_merge(s, Top2{sc, kInit, j});
+            }
+        }
+        Top2 const t = wg_top2(s, red);
+        if (k == 0) openDec[o] = decide_exact(t, minScore, minRatio);
+        __syncthreads();
+    }
+    if (d == kFlag) {
+        for (int o = 0; o < nO; ++o)
+            if (openSlot[o] == k) d = openDec[o];
+    }
+    int const lane = k & 63, wv = k >> 6;
+    if (phase == 0) {
+        // decisions, and the matched B features in slot order for phase 1
+        if (k < kSampleRows) c.decA[p * kSampleRows + k] = (k < nQ) ? d : -1;
+        bool const m = k < nQ && d >= 0;
+        unsigned long long const bal = __ballot(m);
+        int const below = __popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) wcnt[wv][0] = __popcll(bal);
+        __syncthreads();
+        int const pos = (wv == 1 ? wcnt[0][0] : 0) + below;
+        if (m) {
+            c.qB[p * kSampleRows + pos] = d;
+            c.slotB[p * kSampleRows + pos] = k;
+        }
+        if (k == 0) {
+            c.cntB[p] = wcnt[0][0] + wcnt[1][0];
+            counts[4 * p + 2] = nO;
+        }
+    } else {
+        int acc = 0, miss = 0;
+        if (k < nQ) {
+            int const slot = c.slotB[p * kSampleRows + k];
+            int const S = min(c.samples, sA.n);
+            int const i = (int)(((long long)slot * sA.n) / S);
+            int const j = qLocal;
+            if (d == i) {
+                acc = 1;
+                bool const held = sel[sA.qOff + i] == j && sel[sB.qOff + j] == i;
+                miss = held ? 0 : 1;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            acc += __shfl_xor(acc, o);
+            miss += __shfl_xor(miss, o);
+        }
+        if (lane == 0) {
+            wcnt[wv][0] = acc;
+            wcnt[wv][1] = miss;
+        }
+        __syncthreads();
+        if (k == 0) {
+            int const a = wcnt[0][0] + wcnt[1][0], mm = wcnt[0][1] + wcnt[1][1];
+            int const S = (sA.n > 0 && sB.n > 0) ? min(c.samples, sA.n) : 0;
+            counts[4 * p + 0] = a;
+            counts[4 * p + 1] = mm;
+            counts[4 * p + 2] += nO;
+            counts[4 * p + 3] = S;
+            pairFlag[p] = (mm > (int)lim.allowed[a]) ? 1 : 0;
+        }
+    }
+}
+
+void launch_batch_check_probe(BatchDev const& b, CheckDev const& c, int phase, int4 const* items, int nItems,
+                              int* queue, int grid, hipStream_t st)
+{
+    if (nItems <= 0) return;
+    hipLaunchKernelGGL(k_batch_check_probe, dim3(grid), dim3(256), 0, st, b, c, phase, items, nItems, queue);
+}
+
+void launch_batch_check_decide(BatchDev const& b, CheckDev const& c, int phase, float minScore, float minRatio,
+                               MissLimits const& lim, int const* sel, int* pairFlag, int* counts, hipStream_t st)
+{
+    if (b.nPairs <= 0) return;
+    hipLaunchKernelGGL(k_batch_check_decide, dim3(b.nPairs), dim3(kSampleRows), 0, st, b, c, phase, minScore,
+                       minRatio, lim, sel, pairFlag, counts);
+}
+
+}  // namespace v3dm

@@ -25,6 +25,7 @@ struct v3dm_context {
     float learnedMargin = 0.07f;       // ... and no more than this * scale^2 below the best (< 0: off)
     int fallbackSamples = v3dm::kSampleRows;  // sampled check of learned results (0: off)
     float fallbackRecall = 0.995f;     // sampled recall below this -> the plain form is run
+    int fallbackRule = V3DM_FALLBACK_THRESHOLD;  // or V3DM_FALLBACK_BINOMIAL
     float lastMinScore = 0.9f, lastMinRatio = 0.8f;
     bool cacheDerived = true;          // keep bf16 / fp8 / code copies while features are unchanged
 
@@ -86,6 +87,9 @@ inline float learned_margin(v3dm_context* h)
     float const s = learned_code_scale(ctx_learned(h));
     return (h->learnedMargin < 0.0f || !(s > 0.0f)) ? INFINITY : h->learnedMargin * s * s;
 }
+
+// Misses a sampled check of `accepted` exact matches tolerates under the handle's rule.
+int fallback_allowed(v3dm_context const* h, int accepted);
 
 // The mutual check of the handle's next call: its status words and a fresh epoch.
 inline unsigned next_mutual_epoch(v3dm_context* h)

@@ -141,6 +141,25 @@ __device__ __forceinline__ float2 dequant_u8_wave(uchar2 v)
     return make_float2(dequant_u8_value(v.x, denom), dequant_u8_value(v.y, denom));
 }
 
+// Wait until this wave's direct global->LDS loads (global_load_lds_dwordx4) have written
+// LDS: s_waitcnt vmcnt(0).  A double-buffered stage is published to the other waves by a
+// barrier; the compiler does not insert this wait on every path into that barrier (the
+// loop back edge of the stage pipelines lacked it), so the stage loops call it before each
+// such barrier.  The loads were issued a whole tile earlier, so the wait rarely stalls.
+__device__ __forceinline__ void wait_lds_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Work-groups of a launch are dispatched to the eight XCDs round-robin (work-group b runs on
+// XCD b % 8).  xcd_block(b, G) renumbers the G work-groups so that each XCD takes one
+// contiguous range of logical blocks: neighbouring blocks, which read the same rows,
+// share that XCD's L2.  A bijection of [0, G).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int G)
+{
+    int const per = G / kXcds, rem = G % kXcds;
+    int const x = b % kXcds, i = b / kXcds;
+    return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + i;
+}
+
 typedef float v16f __attribute__((ext_vector_type(16)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 

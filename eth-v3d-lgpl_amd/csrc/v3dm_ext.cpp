@@ -342,7 +342,7 @@ int ext_finish(v3dm_context* h)
     }
     h->stats.nSampled = e->sSamples;
     h->stats.nSampleMissed = missed;
-    if (missed > v3dm_fallback_allowed_misses(accepted, h->fallbackRecall)) {
+    if (missed > fallback_allowed(h, accepted)) {
         int const rc = enqueue_plain(h, h->lastMinScore, h->lastMinRatio);
         if (rc) return rc;
         h->stats.nFallback = 1;

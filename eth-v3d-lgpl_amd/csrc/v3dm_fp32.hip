@@ -152,10 +152,12 @@ __device__ __forceinline__ void run_tiles_fp32(float const* __restrict__ R, unsi
 {
     // two tiles per trip so that each stage is a distinct LDS object
     for (int t = t0; t < t1; t += 2) {
+        wait_lds_loads();
         __syncthreads();  // tile t is in stA; every wave is done with stB
         if (t + 1 < t1) stage_tile_fp32(R, t + 1, stB, src, wv);
         tile_fp32<!kMasked, !kMasked, kGuided>(stA, bfrag, colS, rowPart, ldr, lb, t, nL, nR, i0, tid, inf, gb);
         if (t + 1 >= t1) break;
+        wait_lds_loads();
         __syncthreads();  // tile t+1 is in stB; every wave is done with stA
         if (t + 2 < t1) stage_tile_fp32(R, t + 2, stA, src, wv);
         tile_fp32<!kMasked, !kMasked, kGuided>(stB, bfrag, colS, rowPart, ldr, lb, t + 1, nL, nR, i0, tid, inf,

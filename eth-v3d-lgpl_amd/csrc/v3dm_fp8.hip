@@ -156,10 +156,12 @@ __device__ __forceinline__ void run_tiles_fp8(unsigned char const* __restrict__ 
                                               int t1, int nC, int h, int c32, int wv, float inf)
 {
     for (int t = t0; t < t1; t += 2) {
+        wait_lds_loads();
         __syncthreads();  // tile t is in stA; every wave is done with stB
         if (t + 1 < t1) stage_tile_fp8(C, E, t + 1, stB, src, wv);
         tile_fp8<kFull>(stA, bq, sq, st, t, nC, h, c32, inf);
         if (t + 1 >= t1) break;
+        wait_lds_loads();
         __syncthreads();  // tile t+1 is in stB; every wave is done with stA
         if (t + 2 < t1) stage_tile_fp8(C, E, t + 2, stA, src, wv);
         tile_fp8<kFull>(stB, bq, sq, st, t + 1, nC, h, c32, inf);

@@ -91,26 +91,17 @@ __device__ __forceinline__ double exact_dot_u8(float const* q, uint8_t const* __
     return acc;
 }
 
-// One query, eight lanes (sub = lane & 7).  part: the query's (split 0, half 0) list; the
-// lists of split s are at part[s * ldq2 + h]; slots: the group's kRescoreSlots ints of LDS;
-// qs: the group's 128 floats of LDS, which receive the query row (its loads are issued with
-// the list loads).  Returns the exact top-2 (all lanes) and the number of candidates scored
-// by this lane.
-template <bool kU8, typename List>
-__device__ __forceinline__ Top2x finish_query(List const* __restrict__ part, size_t ldq2, int nParts, int span, float nq,
-                                              float mo, float winRel, int topK, float margin,
-                                              float const* __restrict__ Qd,
-                                              float const* __restrict__ Cd, uint8_t const* __restrict__ Cu8,
-                                              float const* __restrict__ Cdenom, int sub, int* slots, float* qs,
-                                              int& nScored)
+// Candidate selection of one query, eight lanes (sub = lane & 7).  part: the query's
+// (split 0, half 0) list; the lists of split s are at part[s * ldq2 + h]; slots: the
+// group's kRescoreSlots ints of LDS.  The group fixes the threshold (window below the
+// approximate second best, or the topK tournament with the margin), compacts the
+// candidates at or above it into its slot row (in lane order) and returns their count
+// (uniform over the group).
+template <typename List>
+__device__ __forceinline__ int select_candidates(List const* __restrict__ part, size_t ldq2, int nParts, int span,
+                                                 float nq, float mo, float winRel, int topK, float margin, int sub,
+                                                 int* slots)
 {
-    {
-        float4 qv[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) qv[u] = reinterpret_cast<float4 const*>(Qd)[sub + 8 * u];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) reinterpret_cast<float4*>(qs)[sub + 8 * u] = qv[u];
-    }
     float v[kPerLane];
     int j[kPerLane];
 #pragma unroll
@@ -187,8 +178,6 @@ __device__ __forceinline__ Top2x finish_query(List const* __restrict__ part, siz
         }
         thr = a2 - winRel * nq * mo;
     }
-    // compact the group's candidates at or above the threshold into its LDS slot row (in
-    // lane order), then lane sub scores slots sub, sub + 8, ... (balanced over the lanes)
     int mine = 0;
 #pragma unroll
     for (int u = 0; u < kPerLane; ++u) mine += (j[u] >= 0 && v[u] >= thr) ? 1 : 0;
@@ -206,6 +195,17 @@ __device__ __forceinline__ Top2x finish_query(List const* __restrict__ part, siz
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // the group's slot writes are visible
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return total;
+}
+
+// Exact scores of the group's `total` slot candidates: lane sub scores slots sub, sub + 8,
+// ... (the oracle's arithmetic); qs: the query row (128 floats of LDS).  Returns the exact
+// top-2 (all lanes) and the number of candidates scored by this lane.
+template <bool kU8>
+__device__ __forceinline__ Top2x score_exact(float const* qs, float const* __restrict__ Cd,
+                                             uint8_t const* __restrict__ Cu8, float const* __restrict__ Cdenom,
+                                             int const* slots, int total, int sub, int& nScored)
+{
     Top2x s = {kInit, kInit, -1, -1};
     int n = 0;
     for (int k = sub; k < total; k += 8) {
@@ -223,6 +223,69 @@ __device__ __forceinline__ Top2x finish_query(List const* __restrict__ part, siz
     s = top2x_merge(s, top2x_shfl_xor(s, 2));
     s = top2x_merge(s, top2x_shfl_xor(s, 4));
     nScored = n;
+    return s;
+}
+
+// One query, eight lanes: candidate selection, then exact scores of the candidates.  The
+// query row is staged in the group's LDS row qs (its loads issued with the list loads).
+template <bool kU8, typename List>
+__device__ __forceinline__ Top2x finish_query(List const* __restrict__ part, size_t ldq2, int nParts, int span, float nq,
+                                              float mo, float winRel, int topK, float margin,
+                                              float const* __restrict__ Qd,
+                                              float const* __restrict__ Cd, uint8_t const* __restrict__ Cu8,
+                                              float const* __restrict__ Cdenom, int sub, int* slots, float* qs,
+                                              int& nScored)
+{
+    {
+        float4 qv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) qv[u] = reinterpret_cast<float4 const*>(Qd)[sub + 8 * u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) reinterpret_cast<float4*>(qs)[sub + 8 * u] = qv[u];
+    }
+    int const total = select_candidates(part, ldq2, nParts, span, nq, mo, winRel, topK, margin, sub, slots);
+    return score_exact<kU8>(qs, Cd, Cu8, Cdenom, slots, total, sub, nScored);
+}
+
+// Certified scores of uint8 rows.  With a_k = fl32(qa_k * ra) and b_k = fl32(qb_k * rb)
+// (qa, qb the uint8 components, ra, rb the rows' 1/sqrt scales), the exact score is
+// s = fl32(sum_k a_k * b_k).  The integer I = sum_k qa_k * qb_k is exact, and since every
+// term is nonnegative, s~ = fl32(fl32(ra * rb) * I) satisfies |s~ - s| <= 5.01 * 2^-24 * s~
+// (two rounded dequantisations, the FP64 sum, the FP32 roundings of s and s~), which
+// kCertU8 = 2^-21 covers.  The eight lanes of a query hold 16 bytes each of the query row
+// (qv) and of every candidate row, so a candidate is one 128-byte segment; four
+// v_dot4_u32_u8 per lane and three xor shuffles form I.  All lanes end with the same top-2.
+constexpr float kCertU8 = 4.76837158203125e-07f;  // 2^-21
+
+__device__ __forceinline__ Top2x score_u8_certified(uint4 qv, float ra, uint8_t const* __restrict__ Cu8,
+                                                    float const* __restrict__ Cdenom, int const* slots, int total,
+                                                    int sub)
+{
+    Top2x s = {kInit, kInit, -1, -1};
+    for (int k = 0; k < total; k += 4) {
+        uint4 cv[4];
+        float rb[4];
+        int jj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            jj[u] = (k + u < total) ? slots[k + u] : -1;
+            int const jr = max(jj[u], 0);  // row 0 of the candidate image stands in (not used)
+            cv[u] = *reinterpret_cast<uint4 const*>(Cu8 + (size_t)jr * kDim + 16 * sub);
+            rb[u] = Cdenom[jr];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            unsigned int acc = __builtin_amdgcn_udot4(qv.x, cv[u].x, 0u, false);
+            acc = __builtin_amdgcn_udot4(qv.y, cv[u].y, acc, false);
+            acc = __builtin_amdgcn_udot4(qv.z, cv[u].z, acc, false);
+            acc = __builtin_amdgcn_udot4(qv.w, cv[u].w, acc, false);
+            acc += __shfl_xor(acc, 1);
+            acc += __shfl_xor(acc, 2);
+            acc += __shfl_xor(acc, 4);
+            float const sc = mul_nc(mul_nc(ra, rb[u]), (float)acc);
+            if (jj[u] >= 0 && sc == sc) s = top2x_merge(s, Top2x{sc, kInit, jj[u], -1});
+        }
+    }
     return s;
 }
 
@@ -304,10 +367,7 @@ void launch_rescore_packed(float const* L, float const* R, int nL, int nR, CandP
 }
 
 // ------------------------------------------------------------------------------ batch
-// Query-space rows; padding rows decide -1.
-// kU8: the candidate rows are read as the uint8 image (128 bytes per row) instead of the
-// FP32 one; one kernel per row format keeps each at its own register count.
-template <bool kU8>
+// Query-space rows; padding rows decide -1.  FP32 images: exact scores of every candidate.
 __device__ __forceinline__ void batch_finish(BatchDev const& b, Cand2 const* __restrict__ part, float winRel,
                                              int topK, float margin, float minScore, float minRatio,
                                              int* __restrict__ sel, int* __restrict__ stripes)
@@ -324,11 +384,10 @@ __device__ __forceinline__ void batch_finish(BatchDev const& b, Cand2 const* __r
         if (q < sg.n && sg.nO > 0 && sg.nParts > 0) {
             int const row = b.imgBase[sg.img] + q;
             size_t const cb = (size_t)b.imgBase[sg.oImg];
-            Top2x const s = finish_query<kU8>(part + ((size_t)sg.partOff + q) * 2, 2 * (size_t)sg.ld, sg.nParts, 0,
-                                              b.norm[row], __uint_as_float(b.maxNorm[sg.oImg]), winRel, topK, margin,
-                                              b.img + (size_t)row * kDim, b.img + cb * kDim,
-                                              kU8 ? b.u8 + cb * kDim : nullptr, kU8 ? b.denom + cb : nullptr, sub,
-                                              slots[threadIdx.x >> 3], qrow[threadIdx.x >> 3], n);
+            Top2x const s = finish_query<false>(part + ((size_t)sg.partOff + q) * 2, 2 * (size_t)sg.ld, sg.nParts, 0,
+                                                b.norm[row], __uint_as_float(b.maxNorm[sg.oImg]), winRel, topK,
+                                                margin, b.img + (size_t)row * kDim, b.img + cb * kDim, nullptr,
+                                                nullptr, sub, slots[threadIdx.x >> 3], qrow[threadIdx.x >> 3], n);
             d = decide_exact(Top2{s.b1, s.b2, s.a1}, minScore, minRatio);
         }
         if (sub == 0) sel[g] = d;
@@ -336,14 +395,57 @@ __device__ __forceinline__ void batch_finish(BatchDev const& b, Cand2 const* __r
     add_stripe(n, stripes);
 }
 
-// many work-groups per query row set: six waves per SIMD (the LDS of six work-groups fits a
-// CU), which the uint8 form reaches without spilling
-__global__ __launch_bounds__(256, 6) void k_batch_shortlist_finish_u8(BatchDev b, Cand2 const* __restrict__ part,
-                                                                      float winRel, int topK, float margin,
-                                                                      float minScore, float minRatio,
-                                                                      int* __restrict__ sel, int* __restrict__ stripes)
+// uint8 images: certified integer scores (score_u8_certified) and the certified decision;
+// a query whose decision the bound leaves open (near ties and thresholds) is scored again
+// exactly from its FP32 row, staged in LDS (certify = false: every query is).  Blocks run in XCD order (xcd_block), so the
+// queries of one segment, which read the same candidate image, share an L2.
+__global__ __launch_bounds__(256) void k_batch_rescore_u8(BatchDev b, Cand2 const* __restrict__ part, float winRel,
+                                                          int topK, float margin, float minScore, float minRatio,
+                                                          bool certify, int* __restrict__ sel,
+                                                          int* __restrict__ stripes)
 {
-    batch_finish<true>(b, part, winRel, topK, margin, minScore, minRatio, sel, stripes);
+    __shared__ int slots[32][kRescoreSlots];
+    __shared__ __attribute__((aligned(16))) float qrow[32][kDim];
+    int const sub = threadIdx.x & 7;
+    int const grp = threadIdx.x >> 3;
+    int const g = xcd_block(blockIdx.x, gridDim.x) * 32 + grp;
+    int n = 0;
+    if (g < b.qRows) {
+        BatchSeg const sg = b.seg[b.segOfBlock[g / kBatchRowAlign]];
+        int const q = g - sg.qOff;
+        int d = -1;
+        if (q < sg.n && sg.nO > 0 && sg.nParts > 0) {
+            int const row = b.imgBase[sg.img] + q;
+            size_t const cb = (size_t)b.imgBase[sg.oImg];
+            uint4 const qv = reinterpret_cast<uint4 const*>(b.u8 + (size_t)row * kDim)[sub];
+            float const ra = b.denom[row];
+            int const total = select_candidates(part + ((size_t)sg.partOff + q) * 2, 2 * (size_t)sg.ld, sg.nParts, 0,
+                                                b.norm[row], __uint_as_float(b.maxNorm[sg.oImg]), winRel, topK,
+                                                margin, sub, slots[grp]);
+            n = (sub == 0) ? total : 0;
+            Top2x s;
+            d = kFlag;
+            if (certify) {
+                s = score_u8_certified(qv, ra, b.u8 + cb * kDim, b.denom + cb, slots[grp], total, sub);
+                d = decide_certified(Top2{s.b1, s.b2, s.a1}, kCertU8 * fmaxf(s.b1, 0.0f) + kEpsAbs, minScore,
+                                     minRatio);
+            }
+            if (d == kFlag) {  // uniform over the group
+                float4 const* qd = reinterpret_cast<float4 const*>(b.img + (size_t)row * kDim);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) reinterpret_cast<float4*>(qrow[grp])[sub + 8 * u] = qd[sub + 8 * u];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                int scored = 0;
+                s = score_exact<true>(qrow[grp], nullptr, b.u8 + cb * kDim, b.denom + cb, slots[grp], total, sub,
+                                      scored);
+                d = decide_exact(Top2{s.b1, s.b2, s.a1}, minScore, minRatio);
+            }
+        }
+        if (sub == 0) sel[g] = d;
+    }
+    add_stripe(n, stripes);
 }
 
 __global__ __launch_bounds__(256) void k_batch_shortlist_finish(BatchDev b, Cand2 const* __restrict__ part,
@@ -351,16 +453,16 @@ __global__ __launch_bounds__(256) void k_batch_shortlist_finish(BatchDev b, Cand
                                                                 float minRatio, int* __restrict__ sel,
                                                                 int* __restrict__ stripes)
 {
-    batch_finish<false>(b, part, winRel, topK, margin, minScore, minRatio, sel, stripes);
+    batch_finish(b, part, winRel, topK, margin, minScore, minRatio, sel, stripes);
 }
 
 void launch_batch_rescore(BatchDev const& b, Cand2 const* part, float winRel, int topK, float margin, float minScore,
-                          float minRatio, int* sel, int* stripes, hipStream_t st)
+                          float minRatio, bool certify, int* sel, int* stripes, hipStream_t st)
 {
     if (b.qRows <= 0) return;
     if (b.u8)
-        hipLaunchKernelGGL(k_batch_shortlist_finish_u8, dim3((b.qRows + 31) / 32), dim3(256), 0, st, b, part, winRel,
-                           topK, margin, minScore, minRatio, sel, stripes);
+        hipLaunchKernelGGL(k_batch_rescore_u8, dim3((b.qRows + 31) / 32), dim3(256), 0, st, b, part, winRel, topK,
+                           margin, minScore, minRatio, certify, sel, stripes);
     else
         hipLaunchKernelGGL(k_batch_shortlist_finish, dim3((b.qRows + 31) / 32), dim3(256), 0, st, b, part, winRel,
                            topK, margin, minScore, minRatio, sel, stripes);

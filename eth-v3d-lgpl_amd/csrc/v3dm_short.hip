@@ -139,10 +139,12 @@ __device__ __forceinline__ void run_tiles_bf16(__bf16 const* __restrict__ C, uns
                                                float inf)
 {
     for (int t = t0; t < t1; t += 2) {
+        wait_lds_loads();
         __syncthreads();  // tile t is in stA; every wave is done with stB
         if (t + 1 < t1) stage_tile_bf16(C, t + 1, stB, src, wv);
         tile_bf16<kFull>(stA, bq, st, t, nC, h, c32, inf);
         if (t + 1 >= t1) break;
+        wait_lds_loads();
         __syncthreads();  // tile t+1 is in stB; every wave is done with stA
         if (t + 2 < t1) stage_tile_bf16(C, t + 2, stA, src, wv);
         tile_bf16<kFull>(stB, bq, st, t + 1, nC, h, c32, inf);

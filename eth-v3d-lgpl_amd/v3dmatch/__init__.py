@@ -6,6 +6,6 @@
 """
 from ._native import V3DMError, lib_path, load_library  # noqa: F401
 from .matching import (  # noqa: F401
-    PREC_BF16, PREC_FP8, PREC_FP32, PREC_LEARNED, SIFT_Matching,
+    FALLBACK_BINOMIAL, FALLBACK_THRESHOLD, PREC_BF16, PREC_FP8, PREC_FP32, PREC_LEARNED, SIFT_Matching,
 )
 from . import synth  # noqa: F401

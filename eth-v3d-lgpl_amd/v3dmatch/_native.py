@@ -65,6 +65,8 @@ SIGNATURES = {
     "v3dm_set_derived_cache": (_I, [_P, _I]),
     "v3dm_set_fallback": (_I, [_P, _I, _F]),
     "v3dm_fallback_allowed_misses": (_I, [_I, _F]),
+    "v3dm_fallback_allowed_misses_binomial": (_I, [_I, _F]),
+    "v3dm_set_fallback_rule": (_I, [_P, _I]),
     "v3dm_set_left_device": (_I, [_P, _I, _FP]),
     "v3dm_set_right_device": (_I, [_P, _I, _FP]),
     "v3dm_set_left_u8": (_I, [_P, _I, _FP]),
@@ -89,6 +91,7 @@ SIGNATURES = {
     "v3dm_batch_use_graph": (_I, [_P, _I]),
     "v3dm_batch_last_ms": (_I, [_P, C.POINTER(C.c_float)]),
     "v3dm_batch_get_fallback": (_I, [_P, _FP]),
+    "v3dm_batch_get_check": (_I, [_P, _FP]),
     "v3dm_learned_set_state": (_I, [_P, _FP, _FP, _FP, _I]),
     "v3dm_learned_get_state": (_I, [_P, _FP, _FP, _FP, _IP]),
     "v3dm_learned_set_code_scale": (_I, [_P, _F]),

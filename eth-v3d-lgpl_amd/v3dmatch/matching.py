@@ -18,6 +18,8 @@ PREC_FP32 = 0
 PREC_BF16 = 1
 PREC_FP8 = 2
 PREC_LEARNED = 3
+FALLBACK_THRESHOLD = 0
+FALLBACK_BINOMIAL = 1
 
 
 def _f32_rows(desc, n):
@@ -172,11 +174,19 @@ class SIFT_Matching:
     def set_fallback(self, samples=128, min_recall=0.995):
         self._check(self._lib.v3dm_set_fallback(self._h, int(samples), C.c_float(min_recall)))
 
+    def set_fallback_rule(self, rule):
+        """FALLBACK_THRESHOLD (default: the plain form when the sampled recall is below
+        min_recall) or FALLBACK_BINOMIAL (a one-sided binomial test at 5%)."""
+        self._check(self._lib.v3dm_set_fallback_rule(self._h, int(rule)))
+
     @staticmethod
-    def fallback_allowed_misses(accepted, min_recall=0.995):
-        """Misses a sampled check of `accepted` exactly accepted features tolerates before
-        the plain form is run (one-sided binomial test at 5%)."""
-        return int(load_library().v3dm_fallback_allowed_misses(int(accepted), C.c_float(min_recall)))
+    def fallback_allowed_misses(accepted, min_recall=0.995, rule=0):
+        """Misses a sampled check of `accepted` exact matches tolerates before the plain
+        form is run: recall (accepted - m) / accepted not below min_recall (rule 0), or a
+        one-sided binomial test at 5% (rule 1)."""
+        fn = load_library().v3dm_fallback_allowed_misses_binomial if rule == 1 else \
+            load_library().v3dm_fallback_allowed_misses
+        return int(fn(int(accepted), C.c_float(min_recall)))
 
     def set_derived_cache(self, enable=True):
         self._check(self._lib.v3dm_set_derived_cache(self._h, int(bool(enable))))
@@ -301,6 +311,13 @@ class SIFT_Matching:
         plain form."""
         out = np.zeros(self._bpairs, dtype=np.int32)
         self._check(self._lib.v3dm_batch_get_fallback(self._h, out.ctypes.data))
+        return out
+
+    def batch_get_check(self):
+        """Per pair of the last learned batch run: (exact matches among the samples, missed,
+        decided by exact FP64 scores, samples), an int32 (nPairs, 4) array."""
+        out = np.zeros((self._bpairs, 4), dtype=np.int32)
+        self._check(self._lib.v3dm_batch_get_check(self._h, out.ctypes.data))
         return out
 
     def batch_last_ms(self):

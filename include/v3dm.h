@@ -121,19 +121,27 @@ int v3dm_set_learned_margin(v3dm_handle h, float marginRel);
  * the features are unchanged; 0: every call derives them again (so that a call's time
  * includes that work). */
 int v3dm_set_derived_cache(v3dm_handle h, int enable);
-/* Learned mode: every call also decides `samples` evenly spaced left features (at most
- * 128; 0 = off) with the exact plain path.  Of the a sampled features that the exact path
- * accepts, m have a different learned decision (missed); when m is more than a recall of
- * minRecall explains -- m > v3dm_fallback_allowed_misses(a, minRecall) -- the result is
- * replaced by the plain form's before it is returned (stats.nFallback = 1).  minRecall 1:
- * any miss falls back; 0: never.  Default 128 and 0.995. */
+/* Learned mode: every call also decides `samples` evenly spaced features (at most 128;
+ * 0 = off) with the exact plain path.  Of the a sampled features that the exact path
+ * accepts, m have a different learned decision (missed); when the sampled recall
+ * (a - m) / a falls below minRecall (V3DM_FALLBACK_THRESHOLD, the default rule), or when m
+ * is more than a recall of minRecall explains at the 5% level (V3DM_FALLBACK_BINOMIAL), the
+ * result is replaced by the plain form's before it is returned (stats.nFallback = 1).
+ * minRecall 1: any miss falls back; 0: never.  Default 128 and 0.995.  The batch form
+ * samples exact mutual matches instead (v3dm_batch_match_async). */
 int v3dm_set_fallback(v3dm_handle h, int samples, float minRecall);
-/* The misses a sampled check of `accepted` exactly accepted features tolerates at recall
- * minRecall: the smallest k with P[X > k] <= 0.05 for X ~ Binomial(accepted, 1 - minRecall)
- * (a one-sided test at 5%: a result whose sample misses more is taken to fall short of
- * minRecall).  0 for minRecall 1, `accepted` for minRecall 0; -1 when accepted is outside
- * [0, 2^24] or minRecall outside [0, 1]. */
+#define V3DM_FALLBACK_THRESHOLD 0
+#define V3DM_FALLBACK_BINOMIAL 1
+int v3dm_set_fallback_rule(v3dm_handle h, int rule);
+/* The misses a sampled check of `accepted` exact matches tolerates at recall minRecall
+ * under the threshold rule: the largest m with (float)(accepted - m) / (float)accepted not
+ * below minRecall (0 for minRecall 1, `accepted` for minRecall 0); -1 when accepted is
+ * outside [0, 2^24] or minRecall outside [0, 1]. */
 int v3dm_fallback_allowed_misses(int accepted, float minRecall);
+/* ... under the binomial rule: the smallest k with P[X > k] <= 0.05 for
+ * X ~ Binomial(accepted, 1 - minRecall) (a one-sided test at 5%: a result whose sample
+ * misses more is taken to fall short of minRecall). */
+int v3dm_fallback_allowed_misses_binomial(int accepted, float minRecall);
 /* Measurement helper: launches the dominant kernel of the current precision mode (plain:
  * the fused score pass; BF16 / FP8 / learned: the pass over both directions) `reps` times
  * back to back on the inputs of the last findPutativeMatches,
@@ -258,11 +266,18 @@ int v3dm_batch_set_tuning(v3dm_handle h, int autotune, int mode, int tilesPerIte
 int v3dm_batch_get_tuning(v3dm_handle h, int mode, int* tilesPerItem, int* grid, int* nItems, float* passMs);
 /* 1 (default): replay batch runs from a captured hipGraph; 0: launch kernel by kernel. */
 int v3dm_batch_use_graph(v3dm_handle h, int enable);
-/* Learned batch runs decide min(samples, n) evenly spaced features of the larger image of
- * every pair (left on a tie) exactly as well (v3dm_set_fallback); a pair whose sampled misses exceed
- * v3dm_fallback_allowed_misses gets the plain form's matches.  flags[p] = 1 for the pairs
- * of the last run that did. */
+/* Learned batch runs check every pair against its exact match set (v3dm_set_fallback):
+ * S = min(samples, nA) evenly spaced features of the pair's larger image A (left on a tie)
+ * are decided exactly, and so are the features of the other image they match; an exact
+ * mutual match among the samples that the learned result lacks is a miss.  A pair whose
+ * sampled recall falls short under the handle's rule (v3dm_set_fallback_rule) gets the
+ * plain form's matches, in the same run.  (Batches uploaded as FP32 compare the samples'
+ * own decisions instead.)  flags[p] = 1 for the pairs of the last run that did. */
 int v3dm_batch_get_fallback(v3dm_handle h, int32_t* flags);
+/* Per pair of the last learned batch run, 4 ints: exact matches among the samples,
+ * missed by the learned result, sampled decisions the certified scores left to the exact
+ * FP64 scores, samples. */
+int v3dm_batch_get_check(v3dm_handle h, int32_t* counts);
 /* GPU time of the last batch run (HIP events around the whole run), ms. */
 int v3dm_batch_last_ms(v3dm_handle h, float* ms);
 

@@ -8,7 +8,7 @@ import pytest
 
 import v3dmatch
 from oracle import oracle
-from v3dmatch import synth
+from v3dmatch import learned, synth
 
 pytestmark = pytest.mark.gpu
 
@@ -66,3 +66,29 @@ def test_bf16_no_ratio_and_edges(matcher):
     assert len(got & want) / max(len(want), 1) >= 0.995
     matcher.setLeftFeatures(31, L)
     assert matcher.findPutativeMatches() == []
+
+
+@pytest.mark.parametrize("prec", [v3dmatch.PREC_BF16, v3dmatch.PREC_FP8, v3dmatch.PREC_LEARNED])
+def test_repeated_calls_are_identical(matcher, prec):
+    """The shortlist passes stream candidate tiles through double-buffered LDS stages; a
+    stage read before its loads have landed would change the lists from call to call.  Five
+    calls on the 16384^2 pair give identical decisions in both directions."""
+    say(f"case repeated calls mode {prec}")
+    qL, qR, _ = synth.make_pair(16384, 16384, 0.4, config=2, pair=0)
+    L, R = synth.dequantise(qL), synth.dequantise(qR)
+    matcher.setLeftFeatures(16384, L)
+    matcher.setRightFeatures(16384, R)
+    if prec == v3dmatch.PREC_LEARNED:
+        learned.Network(matcher).load(learned.WEIGHTS_PATH)
+    matcher.set_precision(prec)
+    matcher.set_fallback(0, 0.995)
+    first = None
+    for _ in range(5):
+        matcher.find_async(0.9, 0.8)
+        cm, rm = matcher.get_maxima()
+        if first is None:
+            first = (cm.copy(), rm.copy())
+        else:
+            assert np.array_equal(cm, first[0]) and np.array_equal(rm, first[1])
+    matcher.set_fallback(128, 0.995)
+    matcher.set_precision(v3dmatch.PREC_BF16)
