@@ -23,7 +23,7 @@ namespace v3dm {
 
 void launch_batch_load_u8(uint8_t const* src, int const* srcRow, int const* nImg, int const* blkImg,
                           int const* imgBase, int rows, float* img, uint8_t* imgU8, float* denom, signed char* img8,
-                          int* rowC, float* rowS, hipStream_t st);
+                          int2* rowCS, hipStream_t st);
 void launch_batch_load_f32(float const* src, int const* srcRow, int const* nImg, int const* blkImg,
                            int const* imgBase, int rows, float* img, hipStream_t st);
 
@@ -37,6 +37,8 @@ MissLimits miss_limits(v3dm_context const* h)
 }
 
 int roundUp(int n, int m) { return (n + m - 1) / m * m; }
+
+constexpr int kCheckTiles = 8;  // 32-row candidate tiles per item of the sampled check
 
 struct Plan {
     std::vector<int4> items;
@@ -61,9 +63,8 @@ struct BatchState {
     signed char* codes = nullptr;  // learned codes of every image row
     uint8_t* imgU8 = nullptr;      // uint8 rows (uint8 uploads), aligned layout
     float* denom = nullptr;
-    signed char* img8 = nullptr;   // sampled check of uint8 batches: rows ^ 0x80, rowC, rowS
-    int* rowC = nullptr;
-    float* rowS = nullptr;
+    signed char* img8 = nullptr;   // sampled check of uint8 batches: rows ^ 0x80 and rowCS
+    int2* rowCS = nullptr;
     bool haveU8 = false;
     bool codesValid = false;
     float codesScale = 0.0f;
@@ -88,12 +89,14 @@ struct BatchState {
     int segBlkCap = 0;
 
     // plans (0 = plain FP32, 1 = BF16 shortlist, 2 = learned codes) and their device copies
-    Plan plan[3];
-    bool planValid[3] = {false, false, false};
-    int4* items[3] = {nullptr, nullptr, nullptr};
-    int itemCap[3] = {0, 0, 0};
-    BatchSeg* segD[3] = {nullptr, nullptr, nullptr};
-    int segCap[3] = {0, 0, 0};
+    // (3 = the plain plan of the pairs a learned run hands to the plain form: short items,
+    // so that a few pairs still spread over the whole GPU)
+    Plan plan[4];
+    bool planValid[4] = {false, false, false, false};
+    int4* items[4] = {nullptr, nullptr, nullptr, nullptr};
+    int itemCap[4] = {0, 0, 0, 0};
+    BatchSeg* segD[4] = {nullptr, nullptr, nullptr, nullptr};
+    int segCap[4] = {0, 0, 0, 0};
     Top2* partF = nullptr;
     size_t partFCap = 0;
     Cand2* partB = nullptr;
@@ -102,6 +105,7 @@ struct BatchState {
     // selection / results
     int* sel = nullptr;
     int* flagList = nullptr;
+    int* openList = nullptr;  // open decisions of the shortlist rescoring
     int selCap = 0;
     int* counters = nullptr;  // kBatchCounters ints
     int* stripes = nullptr;   // striped counts, cleared by the gather kernel
@@ -228,10 +232,10 @@ CheckDev check_view(BatchState* b, int samples)
 {
     CheckDev c;
     c.img8 = b->img8;
-    c.rowC = b->rowC;
-    c.rowS = b->rowS;
+    c.rowCS = b->rowCS;
     c.part = b->cPart;
     c.maxSplits = b->cMaxSplits;
+    c.tilesPerSplit = kCheckTiles;
     c.nSplits = b->cNSplits;
     c.qB = b->cQB;
     c.slotB = b->cSlotB;
@@ -353,7 +357,7 @@ int upload_plan(v3dm_context* h, BatchState* b, int form, Plan const& pl)
     int rc;
     if ((rc = growi(h, b->items[form], b->itemCap[form], pl.items.size()))) return rc;
     if ((rc = growi(h, b->segD[form], b->segCap[form], pl.seg.size()))) return rc;
-    if (form == 0) {
+    if (form == 0 || form == 3) {
         if ((rc = grow(h, b->partF, b->partFCap, pl.partSlots))) return rc;
     } else {
         if ((rc = grow(h, b->partB, b->partBCap, pl.partSlots * 2))) return rc;
@@ -370,11 +374,9 @@ int upload_plan(v3dm_context* h, BatchState* b, int form, Plan const& pl)
 }
 
 // Items of the sampled check.  uint8 images (the mutual check of v3dm_check.hip): per phase
-// (pair, -, split, kCheckTiles 32-row tiles) of the image the phase scans (phase 0: the
+// (pair, -, split, kCheckTiles 32-row tiles, see below) of the image the phase scans (phase 0: the
 // pair's smaller image, phase 1: the larger).  FP32 images: (pair, 0, split, 8 tiles of 64
 // features of the smaller image) for the samples' own decisions.
-constexpr int kCheckTiles = 16;
-
 int ensure_sample_plan(v3dm_context* h, BatchState* b)
 {
     int const P = b->nPairs;
@@ -490,8 +492,10 @@ int ensure_query_space(v3dm_context* h, BatchState* b)
     if ((rc = growi(h, b->segOfBlock, b->segBlkCap, nBlk))) return rc;
     if ((rc = growi(h, b->sel, b->selCap, b->qRows))) return rc;
     if (b->flagList) (void)hipFree(b->flagList);
-    b->flagList = nullptr;
+    if (b->openList) (void)hipFree(b->openList);
+    b->flagList = b->openList = nullptr;
     BCHECK(h, hipMalloc(&b->flagList, (size_t)std::max(b->qRows, 1) * sizeof(int)));
+    BCHECK(h, hipMalloc(&b->openList, (size_t)std::max(b->qRows, 1) * sizeof(int)));
     int const nbMutual = (b->leftRows + 1023) / 1024;
     if (b->tmp) (void)hipFree(b->tmp);
     if (b->blockCount) (void)hipFree(b->blockCount);
@@ -550,17 +554,16 @@ int enqueue_run(v3dm_context* h, BatchState* b, int mode, float minScore, float 
     if (!h->cacheDerived) enqueue_derived(h, b, form);
     if (form == 0) {
         launch_batch_fused_fp32(d, b->items[0], (int)pl.items.size(), b->counters + 0, b->partF, pl.grid, st);
-        launch_batch_merge_decide(d, b->partF, h->epsRel, minScore, minRatio, b->sel, b->flagList, b->counters,
-                                  nullptr, st);
+        launch_batch_merge_decide(d, b->partF, h->epsRel, minScore, minRatio, b->sel, b->flagList, b->counters, st);
         launch_batch_recheck(d, b->flagList, b->counters, minScore, minRatio, b->sel, 1024, st);
     } else if (form == 1) {
         launch_batch_pass_bf16(d, b->items[1], (int)pl.items.size(), b->counters + 1, b->partB, pl.grid, st);
         launch_batch_rescore(d, b->partB, kWinRelBf16, 0, INFINITY, minScore, minRatio, h->epsRel > 0.0f, b->sel,
-                             b->stripes, st);
+                             b->openList, b->counters + kBCntOpenRescore, b->stripes, st);
     } else {
         launch_batch_pass_code(d, b->items[2], (int)pl.items.size(), b->counters + 2, b->partB, pl.grid, st);
         launch_batch_rescore(d, b->partB, 0.0f, h->learnedTopK, learned_margin(h), minScore, minRatio,
-                             h->epsRel > 0.0f, b->sel, b->stripes, st);
+                             h->epsRel > 0.0f, b->sel, b->openList, b->counters + kBCntOpenRescore, b->stripes, st);
         if (h->fallbackSamples > 0) {
             int dev = 0, cus = 256;
             (void)hipGetDevice(&dev);
@@ -569,10 +572,10 @@ int enqueue_run(v3dm_context* h, BatchState* b, int mode, float minScore, float 
             if (b->haveU8) {
                 // exact mutual matches among the samples against the learned result
                 CheckDev const c = check_view(b, h->fallbackSamples);
-                launch_batch_check_probe(d, c, 0, b->cItems[0], b->nCItems[0], b->counters + kBQueueCheck0, 2 * cus, st);
+                launch_batch_check_probe(d, c, 0, b->cItems[0], b->nCItems[0], b->counters + kBQueueCheck0, 3 * cus, st);
                 launch_batch_check_decide(d, c, 0, minScore, minRatio, lim, b->sel, b->pairFlag,
                                           b->pairFlag + b->nPairs, st);
-                launch_batch_check_probe(d, c, 1, b->cItems[1], b->nCItems[1], b->counters + kBQueueCheck1, 2 * cus, st);
+                launch_batch_check_probe(d, c, 1, b->cItems[1], b->nCItems[1], b->counters + kBQueueCheck1, 3 * cus, st);
                 launch_batch_check_decide(d, c, 1, minScore, minRatio, lim, b->sel, b->pairFlag,
                                           b->pairFlag + b->nPairs, st);
             } else {
@@ -587,15 +590,15 @@ int enqueue_run(v3dm_context* h, BatchState* b, int mode, float minScore, float 
             // pairs whose sampled recall fell short get the plain form's decisions, on the
             // device: their plain items, merge masked to their segments, exact re-check
             // (kernels with nothing to do when no pair is flagged)
-            Plan const& pp = b->plan[0];
+            Plan const& pp = b->plan[3];
             int const np = (int)pp.items.size();
-            BatchDev const d0 = dev_view(b, 0);
-            launch_batch_fb_filter(b->items[0], np, b->pairFlag, b->nPairs, b->fbItems, b->counters + kBCntFbItems,
+            BatchDev const d0 = dev_view(b, 3);
+            launch_batch_fb_filter(b->items[3], np, b->pairFlag, b->nPairs, b->fbItems, b->counters + kBCntFbItems,
                                    b->segMask, st);
             launch_batch_fused_fp32(d0, b->fbItems, np, b->counters + 0, b->partF, pp.grid, st,
                                     b->counters + kBCntFbItems);
-            launch_batch_merge_decide(d0, b->partF, h->epsRel, minScore, minRatio, b->sel, b->flagList, b->counters,
-                                      b->segMask, st);
+            launch_batch_merge_masked(d0, b->partF, h->epsRel, minScore, minRatio, b->sel, b->flagList, b->counters,
+                                      b->segMask, 2 * cus, st);
             launch_batch_recheck(d0, b->flagList, b->counters, minScore, minRatio, b->sel, 1024, st);
         }
     }
@@ -678,23 +681,25 @@ int ensure_plan(v3dm_context* h, BatchState* b, int form)
 }
 
 // The plain form for the pairs of a learned run that fail the sampled check runs inside the
-// run (enqueue_run); it needs the plain plan (without tuning when the plain form has not
-// run yet) and room for the flagged items.
+// run (enqueue_run) on its own plan: items of kFallbackTiles 64-row tiles, so that even a
+// single small pair spreads over the GPU; and room for the flagged items.
+constexpr int kFallbackTiles = 2;
+
 int ensure_fallback_plan(v3dm_context* h, BatchState* b)
 {
     int rc;
-    if (!b->planValid[0]) {
+    if (!b->planValid[3]) {
         int dev = 0, cus = 256;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         drop_graphs(b);
-        Plan pl = plan_plain(b, 32, 2 * cus);
-        if ((rc = upload_plan(h, b, 0, pl))) return rc;
-        b->plan[0] = std::move(pl);
-        b->planValid[0] = true;
+        Plan pl = plan_plain(b, kFallbackTiles, 2 * cus);
+        if ((rc = upload_plan(h, b, 3, pl))) return rc;
+        b->plan[3] = std::move(pl);
+        b->planValid[3] = true;
         b->version += 1;
     }
-    size_t const need = std::max<size_t>(b->plan[0].items.size(), 1);
+    size_t const need = std::max<size_t>(b->plan[3].items.size(), 1);
     if ((size_t)b->fbItemCap < need) {
         drop_graphs(b);  // captured runs hold the old buffer
         if ((rc = growi(h, b->fbItems, b->fbItemCap, need))) return rc;
@@ -752,7 +757,7 @@ int set_images(v3dm_context* h, int nImages, int32_t const* counts, void const* 
     }
     if (totalRaw > (size_t)0x7fffffff / kDim) return ctx_fail(h, V3DM_ERR_INVALID, "batch too large");
     drop_graphs(b);
-    b->planValid[0] = b->planValid[1] = b->planValid[2] = false;
+    b->planValid[0] = b->planValid[1] = b->planValid[2] = b->planValid[3] = false;
     b->nPairs = 0;
     b->pending = false;
     b->nImages = nImages;
@@ -767,13 +772,12 @@ int set_images(v3dm_context* h, int nImages, int32_t const* counts, void const* 
         if (b->norm) (void)hipFree(b->norm);
         if (b->codes) (void)hipFree(b->codes);  // sized with the image rows: allocated again on use
         b->codes = nullptr;
-        for (void* ptr : {(void*)b->imgU8, (void*)b->denom, (void*)b->img8, (void*)b->rowC, (void*)b->rowS})
+        for (void* ptr : {(void*)b->imgU8, (void*)b->denom, (void*)b->img8, (void*)b->rowCS})
             if (ptr) (void)hipFree(ptr);
         b->imgU8 = nullptr;
         b->denom = nullptr;
         b->img8 = nullptr;
-        b->rowC = nullptr;
-        b->rowS = nullptr;
+        b->rowCS = nullptr;
         b->img = nullptr;
         b->imgB = nullptr;
         b->norm = nullptr;
@@ -828,13 +832,12 @@ int set_images(v3dm_context* h, int nImages, int32_t const* counts, void const* 
         BCHECK(h, hipMalloc(&b->imgU8, (size_t)b->imgCapRows * kDim));
         BCHECK(h, hipMalloc(&b->denom, (size_t)b->imgCapRows * sizeof(float)));
         BCHECK(h, hipMalloc(&b->img8, (size_t)b->imgCapRows * kDim));
-        BCHECK(h, hipMalloc(&b->rowC, (size_t)b->imgCapRows * sizeof(int)));
-        BCHECK(h, hipMalloc(&b->rowS, (size_t)b->imgCapRows * sizeof(float)));
+        BCHECK(h, hipMalloc(&b->rowCS, (size_t)b->imgCapRows * sizeof(int2)));
     }
     b->haveU8 = u8;
     if (u8)
         launch_batch_load_u8(static_cast<uint8_t const*>(b->upload), b->srcRow, b->nImgD, b->blkImg, b->imgBase, rows,
-                             b->img, b->imgU8, b->denom, b->img8, b->rowC, b->rowS, st);
+                             b->img, b->imgU8, b->denom, b->img8, b->rowCS, st);
     else
         launch_batch_load_f32(static_cast<float const*>(b->upload), b->srcRow, b->nImgD, b->blkImg, b->imgBase, rows,
                               b->img, st);
@@ -854,10 +857,11 @@ void batch_destroy(BatchState*& b)
     drop_graphs(b);
     for (void* p : {(void*)b->img, (void*)b->imgB, (void*)b->codes, (void*)b->imgU8, (void*)b->denom, (void*)b->imgBase, (void*)b->blkImg, (void*)b->nImgD,
                     (void*)b->srcRow, (void*)b->norm, (void*)b->maxNorm, b->upload, (void*)b->segOfBlock,
-                    (void*)b->items[0], (void*)b->items[1], (void*)b->segD[0], (void*)b->segD[1], (void*)b->partF,
-                    (void*)b->partB, (void*)b->sel, (void*)b->flagList, (void*)b->sRows, (void*)b->sNorm, (void*)b->sIdx, (void*)b->sCount,
+                    (void*)b->items[0], (void*)b->items[1], (void*)b->items[2], (void*)b->items[3],
+                    (void*)b->segD[0], (void*)b->segD[1], (void*)b->segD[2], (void*)b->segD[3], (void*)b->partF,
+                    (void*)b->partB, (void*)b->sel, (void*)b->flagList, (void*)b->openList, (void*)b->sRows, (void*)b->sNorm, (void*)b->sIdx, (void*)b->sCount,
                     (void*)b->sNSplits, (void*)b->sPart, (void*)b->sItems, (void*)b->pairFlag, (void*)b->fbItems,
-                    (void*)b->segMask, (void*)b->img8, (void*)b->rowC, (void*)b->rowS, (void*)b->cItems[0],
+                    (void*)b->segMask, (void*)b->img8, (void*)b->rowCS, (void*)b->cItems[0],
                     (void*)b->cItems[1], (void*)b->cNSplits, (void*)b->cPart, (void*)b->cQB, (void*)b->cSlotB,
                     (void*)b->cCntB, (void*)b->cDecA, (void*)b->counters, (void*)b->stripes, (void*)b->pairCount,
                     (void*)b->tmp, (void*)b->blockCount, (void*)b->out})
@@ -897,7 +901,7 @@ int v3dm_batch_set_pairs(v3dm_handle h, int nPairs, const int32_t* left, const i
         if (left[p] < 0 || left[p] >= b->nImages || right[p] < 0 || right[p] >= b->nImages)
             return ctx_fail(h, V3DM_ERR_INVALID, "pair refers to an image outside the batch");
     drop_graphs(b);
-    b->planValid[0] = b->planValid[1] = b->planValid[2] = false;
+    b->planValid[0] = b->planValid[1] = b->planValid[2] = b->planValid[3] = false;
     b->pending = false;
     b->nPairs = nPairs;
     b->pa.assign(left, left + nPairs);

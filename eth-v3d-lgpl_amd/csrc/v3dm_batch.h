@@ -30,6 +30,7 @@ constexpr int kBCntRescoredDone = 8;
 constexpr int kBCntFbItems = 10;  // plain items of the pairs a learned run hands to the plain form
 constexpr int kBQueueCheck0 = 3;  // work counters of the sampled check (phase 0 / FP32 samples, phase 1)
 constexpr int kBQueueCheck1 = 11;
+constexpr int kBCntOpenRescore = 12;  // open decisions of the shortlist rescoring
 constexpr int kBatchCounters = 16;
 
 struct BatchSeg {
@@ -71,17 +72,18 @@ struct BatchDev {
 };
 
 // Sampled check of uint8 batches (v3dm_check.hip).  Per image row: img8 = the uint8 row ^
-// 0x80 (int8 u - 128), rowC = 128 * sum of the uint8 components, rowS = the row's 1/sqrt
-// scale; rows that are no candidate (all-zero rows, padding) have rowC = kDegenerateC and
-// rowS = 1.
+// 0x80 (int8 u - 128), rowCS = (128 * sum of the uint8 components, bits of the row's
+// 1/sqrt scale); rows that are no candidate (all-zero rows, padding) have rowCS =
+// (kDegenerateC, bits of 1).
 constexpr int kDegenerateC = -(1 << 30);
+constexpr int kChkMaxSplits = 1024;  // splits of one check query (64 Ki candidates at 8 tiles per split)
 
 struct CheckDev {
     signed char const* img8;
-    int const* rowC;
-    float const* rowS;
+    int2 const* rowCS;
     int4* part;          // keys per (phase, pair, split, query slot)
     int maxSplits;
+    int tilesPerSplit;   // 32-row candidate tiles per split
     int const* nSplits;  // per (phase, pair)
     int* qB;             // phase 1 queries: matched features of B, per pair kSampleRows slots
     int* slotB;          // ... and the sample slot that matched each
@@ -105,10 +107,11 @@ void launch_batch_fused_fp32(BatchDev const& b, int4 const* items, int nItems, i
 // shortlist pass (v3dm_short.hip)
 void launch_batch_pass_bf16(BatchDev const& b, int4 const* items, int nItems, int* queue, Cand2* part, int grid,
                             hipStream_t st);
-// uint8 images: certified integer scores, open decisions scored exactly (certify = false:
-// every decision scored exactly); FP32 images: exact scores
+// uint8 images: certified integer scores, the open decisions (listed in `open`, count in
+// *nOpen, which must be zero) scored exactly (certify = false: every decision scored
+// exactly); FP32 images: exact scores
 void launch_batch_rescore(BatchDev const& b, Cand2 const* part, float winRel, int topK, float margin, float minScore,
-                          float minRatio, bool certify, int* sel, int* stripes, hipStream_t st);
+                          float minRatio, bool certify, int* sel, int* open, int* nOpen, int* stripes, hipStream_t st);
 // learned code pass (v3dm_code.hip)
 void launch_batch_pass_code(BatchDev const& b, int4 const* items, int nItems, int* queue, Cand2* part, int grid,
                             hipStream_t st);
@@ -116,9 +119,11 @@ void launch_batch_pass_code(BatchDev const& b, int4 const* items, int nItems, in
 void launch_batch_reset(int* counters, int* pairCount, int nPairs, hipStream_t st);
 void launch_batch_norms(float const* img, int const* rowImg, int rows, float* norm, unsigned int* maxNorm,
                         hipStream_t st);
-// segMask (may be null): segments whose mask is 0 keep their decisions
 void launch_batch_merge_decide(BatchDev const& b, Top2 const* part, float epsRel, float minScore, float minRatio,
-                               int* sel, int* flagList, int* counters, int const* segMask, hipStream_t st);
+                               int* sel, int* flagList, int* counters, hipStream_t st);
+// the segments with segMask set only (the others keep their decisions); persistent grid
+void launch_batch_merge_masked(BatchDev const& b, Top2 const* part, float epsRel, float minScore, float minRatio,
+                               int* sel, int* flagList, int* counters, int const* segMask, int grid, hipStream_t st);
 // sampled exact check of a shortlist run (v3dm_batch.hip, v3dm_fp32.hip)
 void launch_batch_sample_gather(BatchDev const& b, int samples, float* sRows, float* sNorm, int* sIdx, int* sCount,
                                 hipStream_t st);

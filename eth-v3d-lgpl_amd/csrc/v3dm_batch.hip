@@ -83,18 +83,14 @@ void launch_batch_norms(float const* img, int const* blkImg, int rows, float* no
     hipLaunchKernelGGL(k_batch_norms, dim3((rows + 63) / 64), dim3(256), 0, st, img, blkImg, rows, norm, maxNorm);
 }
 
-// 8 lanes per query row: merge the segment's partials, then the certified decision (as
-// the single-pair merge).  Open queries are appended to flagList (query-space rows).
-__global__ __launch_bounds__(256) void k_batch_merge_decide(BatchDev b, Top2 const* __restrict__ part, float epsRel,
-                                                            float minScore, float minRatio, int* __restrict__ sel,
-                                                            int* __restrict__ flagList, int* __restrict__ counters,
-                                                            int const* __restrict__ segMask)
+// 8 lanes (sub = lane & 7) per query row g of segment si: merge the segment's partials,
+// then the certified decision (as the single-pair merge).  Open queries are appended to
+// flagList (query-space rows).
+__device__ __forceinline__ void merge_decide_row(BatchDev const& b, Top2 const* __restrict__ part, int si, int g,
+                                                 int sub, float epsRel, float minScore, float minRatio,
+                                                 int* __restrict__ sel, int* __restrict__ flagList,
+                                                 int* __restrict__ counters)
 {
-    int const sub = threadIdx.x & 7;
-    int const g = (int)((blockIdx.x * 256u + threadIdx.x) >> 3);
-    if (g >= b.qRows) return;  // uniform over the 8 lanes of a query
-    int const si = b.segOfBlock[g / kBatchRowAlign];
-    if (segMask && !segMask[si]) return;  // a segment the run leaves as it is
     BatchSeg const sg = b.seg[si];
     int const q = g - sg.qOff;
     int const np = (q < sg.n) ? sg.nParts : 0;
@@ -105,7 +101,7 @@ __global__ __launch_bounds__(256) void k_batch_merge_decide(BatchDev b, Top2 con
     s = top2_merge(s, top2_shfl_xor(s, 1));
     s = top2_merge(s, top2_shfl_xor(s, 2));
     s = top2_merge(s, top2_shfl_xor(s, 4));
-    if (sub != 0) return;
+    if (sub != 0) return;  // the group's result is in every lane; its first lane writes it
     if (q >= sg.n || sg.nO <= 0) {
         sel[g] = -1;
         return;
@@ -118,12 +114,65 @@ __global__ __launch_bounds__(256) void k_batch_merge_decide(BatchDev b, Top2 con
     sel[g] = d;
 }
 
+__global__ __launch_bounds__(256) void k_batch_merge_decide(BatchDev b, Top2 const* __restrict__ part, float epsRel,
+                                                            float minScore, float minRatio, int* __restrict__ sel,
+                                                            int* __restrict__ flagList, int* __restrict__ counters)
+{
+    int const g = (int)((blockIdx.x * 256u + threadIdx.x) >> 3);
+    if (g >= b.qRows) return;  // uniform over the 8 lanes of a query
+    merge_decide_row(b, part, b.segOfBlock[g / kBatchRowAlign], g, threadIdx.x & 7, epsRel, minScore, minRatio, sel,
+                     flagList, counters);
+}
+
+// The same for the segments with segMask set only: a persistent grid reads the segment
+// masks 1024 at a time into LDS (a list of the flagged ones) and shares each flagged
+// segment's rows (32 per work-group step) out, so that a run with few or no flagged pairs
+// costs a few microseconds.
+__global__ __launch_bounds__(256) void k_batch_merge_masked(BatchDev b, Top2 const* __restrict__ part, float epsRel,
+                                                            float minScore, float minRatio, int* __restrict__ sel,
+                                                            int* __restrict__ flagList, int* __restrict__ counters,
+                                                            int const* __restrict__ segMask)
+{
+    __shared__ int list[1024];
+    __shared__ int nList;
+    int const sub = threadIdx.x & 7;
+    int const r = threadIdx.x >> 3;
+    int const nSeg = 2 * b.nPairs;
+    for (int base = 0; base < nSeg; base += 1024) {
+        __syncthreads();  // the previous chunk's list is consumed
+        if (threadIdx.x == 0) nList = 0;
+        __syncthreads();
+        for (int u = threadIdx.x; u < 1024 && base + u < nSeg; u += 256)
+            if (segMask[base + u]) list[atomicAdd(&nList, 1)] = base + u;
+        __syncthreads();
+        int const n = nList;
+        for (int e = 0; e < n; ++e) {
+            int const si = list[e];
+            BatchSeg const sg = b.seg[si];
+            int const chunks = (sg.n + 31) / 32;
+            for (int c = blockIdx.x; c < chunks; c += gridDim.x) {
+                int const g = sg.qOff + c * 32 + r;
+                if (g - sg.qOff < sg.n)
+                    merge_decide_row(b, part, si, g, sub, epsRel, minScore, minRatio, sel, flagList, counters);
+            }
+        }
+    }
+}
+
 void launch_batch_merge_decide(BatchDev const& b, Top2 const* part, float epsRel, float minScore, float minRatio,
-                               int* sel, int* flagList, int* counters, int const* segMask, hipStream_t st)
+                               int* sel, int* flagList, int* counters, hipStream_t st)
 {
     if (b.qRows <= 0) return;
     hipLaunchKernelGGL(k_batch_merge_decide, dim3((b.qRows + 31) / 32), dim3(256), 0, st, b, part, epsRel, minScore,
-                       minRatio, sel, flagList, counters, segMask);
+                       minRatio, sel, flagList, counters);
+}
+
+void launch_batch_merge_masked(BatchDev const& b, Top2 const* part, float epsRel, float minScore, float minRatio,
+                               int* sel, int* flagList, int* counters, int const* segMask, int grid, hipStream_t st)
+{
+    if (b.nPairs <= 0) return;
+    hipLaunchKernelGGL(k_batch_merge_masked, dim3(grid), dim3(256), 0, st, b, part, epsRel, minScore, minRatio, sel,
+                       flagList, counters, segMask);
 }
 
 // Exact decision of one open query per work-group (grid-stride over the flag list).
@@ -334,14 +383,15 @@ void launch_batch_mutual(BatchDev const& b, int const* sel, int leftRows, int2* 
 // a feature when below nImg[m] (the count rounded down to 32) and zero padding otherwise.
 // The uint8 rows are also kept in the aligned layout with their scale (imgU8, denom): the
 // exact rescoring of the shortlist forms reads 128 bytes per candidate instead of 512 and
-// forms the same float values.  The sampled check's rows: img8 (u ^ 0x80), rowC (128 * the
-// component sum, kDegenerateC for all-zero and padding rows) and rowS (the scale, or 1).
+// forms the same float values.  The sampled check's rows: img8 (u ^ 0x80), rowCS.x (128 * the
+// component sum, kDegenerateC for all-zero and padding rows) and rowCS.y (the bits of the
+// scale, or of 1).
 __global__ __launch_bounds__(256) void k_batch_load_u8(uint8_t const* __restrict__ src, int const* __restrict__ srcRow,
                                                        int const* __restrict__ nImg, int const* __restrict__ blkImg,
                                                        int const* __restrict__ imgBase, int rows,
                                                        float* __restrict__ img, uint8_t* __restrict__ imgU8,
                                                        float* __restrict__ denom, signed char* __restrict__ img8,
-                                                       int* __restrict__ rowC, float* __restrict__ rowS)
+                                                       int2* __restrict__ rowCS)
 {
     int const lane = threadIdx.x & 63;
     int const row = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -364,8 +414,7 @@ __global__ __launch_bounds__(256) void k_batch_load_u8(uint8_t const* __restrict
     if (lane == 0) {
         bool const valid = d > 0.0f && d < INFINITY;  // a feature with a nonzero component
         denom[row] = d;
-        rowC[row] = valid ? 128 * sum : kDegenerateC;
-        rowS[row] = valid ? d : 1.0f;
+        rowCS[row] = make_int2(valid ? 128 * sum : kDegenerateC, __float_as_int(valid ? d : 1.0f));
     }
 }
 
@@ -386,11 +435,11 @@ __global__ __launch_bounds__(256) void k_batch_load_f32(float const* __restrict_
 
 void launch_batch_load_u8(uint8_t const* src, int const* srcRow, int const* nImg, int const* blkImg,
                           int const* imgBase, int rows, float* img, uint8_t* imgU8, float* denom, signed char* img8,
-                          int* rowC, float* rowS, hipStream_t st)
+                          int2* rowCS, hipStream_t st)
 {
     if (rows <= 0) return;
     hipLaunchKernelGGL(k_batch_load_u8, dim3((rows + 3) / 4), dim3(256), 0, st, src, srcRow, nImg, blkImg, imgBase,
-                       rows, img, imgU8, denom, img8, rowC, rowS);
+                       rows, img, imgU8, denom, img8, rowCS);
 }
 
 void launch_batch_load_f32(float const* src, int const* srcRow, int const* nImg, int const* blkImg,

@@ -35,6 +35,7 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 constexpr int kChkTile = 32;       // candidate rows per MFMA block
 constexpr int kChkCand = 256;      // candidates an open decision scores exactly (more: all of them)
 constexpr float kCertChk = 7.62939453125e-06f;  // 2^-17
+constexpr float kKeySlack = 1.0000076293945312f;  // 1 + 2^-17
 
 __device__ __forceinline__ int chk_nq(BatchDev const& b, CheckDev const& c, int phase, int p, BatchSeg const& sA)
 {
@@ -48,6 +49,15 @@ __device__ __forceinline__ int chk_query(CheckDev const& c, int phase, int p, in
 }
 
 __device__ __forceinline__ float key_value(int key) { return __int_as_float(key & ~31); }
+
+// Waves of a probe work-group: query block (32 rows) w % qbp of the ceil(nQ / 32) blocks,
+// tile part w / qbp of the item's tiles (qbp = the block count rounded up to 1, 2 or 4), so
+// that a phase with few queries still keeps the four waves busy; parts = 4 / qbp.
+__device__ __forceinline__ int probe_qbp(int nQ)
+{
+    int const nqb = (nQ + 31) / 32;
+    return nqb <= 1 ? 1 : (nqb <= 2 ? 2 : 4);
+}
 
 // exact score of image rows q and c (the oracle's arithmetic on the dequantised floats)
 __device__ __forceinline__ float exact_score(float const* __restrict__ q, float const* __restrict__ c)
@@ -81,22 +91,75 @@ __device__ __forceinline__ Top2 wg_top2(Top2 s, Top2* red)
 
 // ---------------------------------------------------------------------------- probe
 // Persistent over items (pair, -, split, tiles per split): the pair's (up to 128) query
-// rows of this phase, 32 per wave, against the split's 32-row candidate tiles.  Per lane
-// (query l % 32, candidate rows 8 (r / 4) + 4 (l / 32) + r % 4 of each block) the two best
-// keys and the tile of the best; the lane halves merge at the end.  part: int4 (key1, key2,
-// tile of key1, 0) per (phase, pair, split, query slot).
+// rows of this phase against the split's 32-row candidate tiles.  Wave w takes query block
+// w % qbp and, of every group of `parts` consecutive tiles (probe_qbp), tile w / qbp.  The
+// work-group stages each group through LDS (rows padded to 144 bytes, so the lanes' 16-byte
+// reads of 32 rows spread over the banks), loading the next group into registers while the
+// current one is multiplied.  Per lane (query l % 32, candidate rows 8 (r / 4) + 4 (l / 32)
+// + r % 4 of each block) the two best keys and the tile of the best; the lane halves merge
+// by a shuffle, the waves of a query block through LDS (equal keys keep the smaller tile).
+// part: int4 (key1, key2, tile of key1, 0) per (phase, pair, split, query slot).
+__device__ __forceinline__ void key_merge(int& k1, int& k2, int& b1, int o1, int o2, int ob)
+{
+    bool const take = o1 > k1 || (o1 == k1 && ob >= 0 && (b1 < 0 || ob < b1));
+    k2 = max(min(k1, o1), max(k2, o2));
+    b1 = take ? ob : b1;
+    k1 = max(k1, o1);
+}
+
+constexpr int kRowPadB = 144;                 // LDS bytes per staged 128-byte row
+constexpr int kTileB = kChkTile * kRowPadB;   // 4608
+
+// group `grp` of the item's tiles [t0, t1) -> registers: thread tid holds chunk tid % 8 of
+// row tid / 8 of each tile g < parts, and rowCS of row tid % 32 when tid < 32
+struct StageRegs {
+    uint4 v[4];
+    int2 cs[4];
+};
+
+__device__ __forceinline__ void stage_load(CheckDev const& c, size_t cBase, int t0, int t1, int grp, int parts,
+                                           StageRegs& r)
+{
+    int const tid = threadIdx.x;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        int const t = t0 + grp * parts + g;
+        if (g < parts && t < t1) {
+            size_t const row = cBase + (size_t)t * kChkTile;
+            r.v[g] = *reinterpret_cast<uint4 const*>(c.img8 + (row + (tid >> 3)) * kDim + 16 * (tid & 7));
+            if (tid < kChkTile) r.cs[g] = c.rowCS[row + tid];
+        }
+    }
+}
+
+__device__ __forceinline__ void stage_store(unsigned char* st, int2* stCS, int parts, StageRegs const& r)
+{
+    int const tid = threadIdx.x;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        if (g < parts) {
+            *reinterpret_cast<uint4*>(st + g * kTileB + (tid >> 3) * kRowPadB + 16 * (tid & 7)) = r.v[g];
+            if (tid < kChkTile) stCS[g * kChkTile + tid] = r.cs[g];
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_batch_check_probe(BatchDev b, CheckDev c, int phase,
                                                            int4 const* __restrict__ items, int nItems,
                                                            int* __restrict__ queue)
 {
     __shared__ int sItem;
-    int const lane = threadIdx.x & 63;
-    int const wave = threadIdx.x >> 6;
+    __shared__ int4 red[4][32];
+    __shared__ __attribute__((aligned(16))) unsigned char st[2][4 * kTileB];
+    __shared__ __attribute__((aligned(16))) int2 stCS[2][4 * kChkTile];
+    int const tid = threadIdx.x;
+    int const lane = tid & 63;
+    int const wave = tid >> 6;
     int const h = lane >> 5;
     int const c32 = lane & 31;
     for (;;) {
-        if (threadIdx.x == 0) sItem = atomicAdd(queue, 1);
-        __syncthreads();
+        if (tid == 0) sItem = atomicAdd(queue, 1);
+        __syncthreads();  // also: the previous item's stages and LDS merge are done
         int const it = __builtin_amdgcn_readfirstlane(sItem);
         __syncthreads();
         if (it >= nItems) break;
@@ -106,62 +169,87 @@ __global__ __launch_bounds__(256) void k_batch_check_probe(BatchDev b, CheckDev 
         BatchSeg const sA = b.seg[sa * b.nPairs + p];
         BatchSeg const sB = b.seg[(1 - sa) * b.nPairs + p];
         int const nQ = chk_nq(b, c, phase, p, sA);
-        if (wave * 32 >= nQ) continue;  // uniform over the wave; no barrier is skipped
-        int const k = wave * 32 + c32;
-        int const qLocal = chk_query(c, phase, p, min(k, nQ - 1), nQ, sA.n);
-        size_t const qRow = (size_t)b.imgBase[phase == 0 ? sA.img : sB.img] + qLocal;
+        int const qbp = probe_qbp(nQ), parts = 4 / qbp;
+        int const qb = wave % qbp, part = wave / qbp;
+        bool const active = qb * 32 < nQ;  // uniform over the wave
+        int const k = qb * 32 + c32;
         int const cImg = phase == 0 ? sB.img : sA.img;
         int const nC = phase == 0 ? sB.n : sA.n;
-        v4i bq[4];
-#pragma unroll
-        for (int m = 0; m < 4; ++m) bq[m] = *reinterpret_cast<v4i const*>(c.img8 + qRow * kDim + 32 * m + 16 * h);
-        int const cq = c.rowC[qRow] - (1 << 21);
         size_t const cBase = (size_t)b.imgBase[cImg];
         int const t0 = w.z * w.w;
         int const t1 = min(t0 + w.w, nC / kChkTile);
-        int k1 = INT_MIN, k2 = INT_MIN, blk1 = -1;
-        for (int t = t0; t < t1; ++t) {
-            size_t const r0 = cBase + (size_t)t * kChkTile;
-            v4i a[4];
+        int const nGroups = (t1 - t0 + parts - 1) / parts;
+        v4i bq[4];
+        int cq = 0;
+        if (active) {
+            int const qLocal = chk_query(c, phase, p, min(k, nQ - 1), nQ, sA.n);
+            size_t const qRow = (size_t)b.imgBase[phase == 0 ? sA.img : sB.img] + qLocal;
 #pragma unroll
             for (int m = 0; m < 4; ++m)
-                a[m] = *reinterpret_cast<v4i const*>(c.img8 + (r0 + c32) * kDim + 32 * m + 16 * h);
-            v16i cv;
-            float sv[16];
-#pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                int4 const ci = *reinterpret_cast<int4 const*>(c.rowC + r0 + 8 * g + 4 * h);
-                float4 const si = *reinterpret_cast<float4 const*>(c.rowS + r0 + 8 * g + 4 * h);
-                cv[4 * g + 0] = ci.x;
-                cv[4 * g + 1] = ci.y;
-                cv[4 * g + 2] = ci.z;
-                cv[4 * g + 3] = ci.w;
-                sv[4 * g + 0] = si.x;
-                sv[4 * g + 1] = si.y;
-                sv[4 * g + 2] = si.z;
-                sv[4 * g + 3] = si.w;
-            }
-            v16i d = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[0], bq[0], cv, 0, 0, 0);
-            d = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[1], bq[1], d, 0, 0, 0);
-            d = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[2], bq[2], d, 0, 0, 0);
-            d = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[3], bq[3], d, 0, 0, 0);
-            int const o = k1;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                float const tv = sv[r] * (float)(d[r] + cq);  // degenerate rows: negative
-                int const key = (__float_as_int(tv) & ~31) | (8 * (r >> 2) + 4 * h + (r & 3));
-                k2 = max(k2, min(k1, key));
-                k1 = max(k1, key);
-            }
-            blk1 = (k1 != o) ? t : blk1;
+                bq[m] = *reinterpret_cast<v4i const*>(c.img8 + qRow * kDim + 32 * m + 16 * h);
+            cq = c.rowCS[qRow].x - (1 << 21);
         }
-        int const ok1 = __shfl_xor(k1, 32), ok2 = __shfl_xor(k2, 32), ob = __shfl_xor(blk1, 32);
-        bool const takeO = ok1 > k1 || (ok1 == k1 && ob >= 0 && (blk1 < 0 || ob < blk1));
-        int const n1 = max(k1, ok1);
-        int const n2 = max(min(k1, ok1), max(k2, ok2));
-        if (h == 0 && k < nQ)
+        StageRegs r;
+        stage_load(c, cBase, t0, t1, 0, parts, r);
+        stage_store(st[0], stCS[0], parts, r);
+        __syncthreads();
+        int k1 = INT_MIN, k2 = INT_MIN, blk1 = -1;
+        for (int gi = 0; gi < nGroups; ++gi) {
+            int const buf = gi & 1;
+            if (gi + 1 < nGroups) stage_load(c, cBase, t0, t1, gi + 1, parts, r);
+            int const t = t0 + gi * parts + part;
+            if (active && t < t1) {
+                unsigned char const* tp = st[buf] + part * kTileB;
+                v4i a[4];
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+                    a[m] = *reinterpret_cast<v4i const*>(tp + c32 * kRowPadB + 32 * m + 16 * h);
+                v16i cv;
+                float sv[16];
+                int2 const* cs = stCS[buf] + part * kChkTile;
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    int4 const x0 = *reinterpret_cast<int4 const*>(cs + 8 * g + 4 * h);
+                    int4 const x1 = *reinterpret_cast<int4 const*>(cs + 8 * g + 4 * h + 2);
+                    cv[4 * g + 0] = x0.x;
+                    sv[4 * g + 0] = __int_as_float(x0.y);
+                    cv[4 * g + 1] = x0.z;
+                    sv[4 * g + 1] = __int_as_float(x0.w);
+                    cv[4 * g + 2] = x1.x;
+                    sv[4 * g + 2] = __int_as_float(x1.y);
+                    cv[4 * g + 3] = x1.z;
+                    sv[4 * g + 3] = __int_as_float(x1.w);
+                }
+                v16i d = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[0], bq[0], cv, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[1], bq[1], d, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[2], bq[2], d, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[3], bq[3], d, 0, 0, 0);
+                int const o = k1;
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    float const tv = sv[q] * (float)(d[q] + cq);  // degenerate rows: negative
+                    int const key = (__float_as_int(tv) & ~31) | (8 * (q >> 2) + 4 * h + (q & 3));
+                    k2 = max(k2, min(k1, key));
+                    k1 = max(k1, key);
+                }
+                blk1 = (k1 != o) ? t : blk1;
+            }
+            if (gi + 1 < nGroups) stage_store(st[buf ^ 1], stCS[buf ^ 1], parts, r);
+            __syncthreads();
+        }
+        if (active) key_merge(k1, k2, blk1, __shfl_xor(k1, 32), __shfl_xor(k2, 32), __shfl_xor(blk1, 32));
+        if (parts > 1) {  // uniform over the work-group
+            if (active && h == 0) red[wave][c32] = make_int4(k1, k2, blk1, 0);
+            __syncthreads();
+            if (active && part == 0 && h == 0)
+                for (int q = 1; q < parts; ++q) {
+                    int4 const e = red[qb + q * qbp][c32];
+                    key_merge(k1, k2, blk1, e.x, e.y, e.z);
+                }
+        }
+        if (active && part == 0 && h == 0 && k < nQ)
             c.part[(((size_t)phase * b.nPairs + p) * c.maxSplits + w.z) * kSampleRows + k] =
-                make_int4(n1, n2, takeO ? ob : blk1, 0);
+                make_int4(k1, k2, blk1, 0);
     }
 }
 
@@ -179,7 +267,8 @@ __global__ __launch_bounds__(128) void k_batch_check_decide(BatchDev b, CheckDev
     __shared__ float openThr[kSampleRows];
     __shared__ int openDec[kSampleRows];
     __shared__ int cand[kChkCand];
-    __shared__ int nOpen, nCand;
+    __shared__ int scanSplit[kChkMaxSplits];
+    __shared__ int nOpen, nCand, nScan;
     __shared__ Top2 red[2];
     __shared__ int wcnt[2][3];
     int const p = blockIdx.x;
@@ -203,6 +292,7 @@ __global__ __launch_bounds__(128) void k_batch_check_decide(BatchDev b, CheckDev
             int K1 = INT_MIN, K2 = INT_MIN, B1 = -1;
             int const ns = c.nSplits[phase * b.nPairs + p];
             int4 const* pp = c.part + ((size_t)phase * b.nPairs + p) * c.maxSplits * kSampleRows + k;
+#pragma unroll 8
             for (int s = 0; s < ns; ++s) {
                 int4 const e = pp[(size_t)s * kSampleRows];
                 bool const take = e.x > K1;  // splits in ascending order: equal keys keep the first
@@ -223,43 +313,71 @@ __global__ __launch_bounds__(128) void k_batch_check_decide(BatchDev b, CheckDev
         }
     }
     __syncthreads();
-    // exact decisions of the open queries
+    // exact decisions of the open queries.  The exact top-2 lies among the candidates with
+    // s~ >= thr = s~2 - 2 eps.  A split whose second key stays below thr holds at most one
+    // of them, its best (listed from the key); a split whose second key reaches thr may
+    // hold more and is scanned again with the integer scores.
     int const nO = nOpen;
+    int const ns = c.nSplits[phase * b.nPairs + p];
+    int const span = c.tilesPerSplit * kChkTile;  // candidates per split
     for (int o = 0; o < nO; ++o) {
         int const kq = openSlot[o];
         int const ql = chk_query(c, phase, p, kq, nQ, sA.n);
         size_t const qRow = (size_t)b.imgBase[qImg] + ql;
         float const ra = b.denom[qRow];
         float const thr = openThr[o];
-        if (k == 0) nCand = 0;
+        if (k == 0) nCand = nScan = 0;
         __syncthreads();
+        int4 const* pp = c.part + ((size_t)phase * b.nPairs + p) * c.maxSplits * kSampleRows + kq;
+        for (int sp = k; sp < ns; sp += blockDim.x) {
+            int4 const e = pp[(size_t)sp * kSampleRows];
+            // keys truncate t by less than 2^-18 relative: the slack keeps every candidate
+            // whose untruncated s~ may reach thr
+            bool const two = e.y >= 0 && ra * key_value(e.y) * kKeySlack >= thr;
+            bool const one = e.x >= 0 && ra * key_value(e.x) * kKeySlack >= thr;
+            if (two) {
+                int const q = atomicAdd(&nScan, 1);
+                if (q < kChkMaxSplits) scanSplit[q] = sp;
+            } else if (one) {
+                int const q = atomicAdd(&nCand, 1);
+                if (q < kChkCand) cand[q] = e.z * kChkTile + (e.x & 31);
+            }
+        }
+        __syncthreads();
+        bool const overflow = nScan > kChkMaxSplits;  // every candidate exactly below
+        int const nsc = overflow ? 0 : nScan;
         uint4 qv[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) qv[u] = reinterpret_cast<uint4 const*>(b.u8 + qRow * kDim)[u];
-        for (int j = k; j < nC; j += blockDim.x) {
-            size_t const row = cBase + j;
-            if (c.rowC[row] == kDegenerateC) continue;
-            uint4 const* cr = reinterpret_cast<uint4 const*>(b.u8 + row * kDim);
-            unsigned int acc = 0;
+        for (int e = 0; e < nsc; ++e) {
+            int const j0 = scanSplit[e] * span;
+            int const j1 = min(j0 + span, nC);
+            for (int j = j0 + k; j < j1; j += blockDim.x) {
+                size_t const row = cBase + j;
+                int2 const csr = c.rowCS[row];
+                if (csr.x == kDegenerateC) continue;
+                uint4 const* cr = reinterpret_cast<uint4 const*>(b.u8 + row * kDim);
+                unsigned int acc = 0;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                uint4 const y = cr[u];
-                acc = __builtin_amdgcn_udot4(qv[u].x, y.x, acc, false);
-                acc = __builtin_amdgcn_udot4(qv[u].y, y.y, acc, false);
-                acc = __builtin_amdgcn_udot4(qv[u].z, y.z, acc, false);
-                acc = __builtin_amdgcn_udot4(qv[u].w, y.w, acc, false);
-            }
-            float const sv = ra * (c.rowS[row] * (float)acc);
-            if (sv >= thr) {
-                int const e = atomicAdd(&nCand, 1);
-                if (e < kChkCand) cand[e] = j;
+                for (int u = 0; u < 8; ++u) {
+                    uint4 const y = cr[u];
+                    acc = __builtin_amdgcn_udot4(qv[u].x, y.x, acc, false);
+                    acc = __builtin_amdgcn_udot4(qv[u].y, y.y, acc, false);
+                    acc = __builtin_amdgcn_udot4(qv[u].z, y.z, acc, false);
+                    acc = __builtin_amdgcn_udot4(qv[u].w, y.w, acc, false);
+                }
+                float const sv = ra * (__int_as_float(csr.y) * (float)acc);
+                if (sv * kKeySlack >= thr) {
+                    int const q = atomicAdd(&nCand, 1);
+                    if (q < kChkCand) cand[q] = j;
+                }
             }
         }
         __syncthreads();
         int const nc = nCand;
         float const* qf = b.img + qRow * kDim;
         Top2 s = top2_init();
-        if (nc <= kChkCand) {
+        if (nc <= kChkCand && !overflow) {
             for (int e = k; e < nc; e += blockDim.x) {
                 int const j = cand[e];
                 float const sc = exact_score(qf, b.img + (cBase + j) * kDim);

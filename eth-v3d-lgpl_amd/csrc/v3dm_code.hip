@@ -53,11 +53,14 @@ __device__ __forceinline__ void key_block_update(KeyStateI& s, v16i const& d, in
     }
 }
 
-__device__ __forceinline__ void load_code_tile(signed char const* __restrict__ C, int t, int h, int c32, v4i (&a)[2])
+// The A operands of tile t: the tile's address is uniform (scalar registers), the lane's
+// offset inside it (row c32 of each 32-row block, bytes 16h..16h+15) is fixed, so the
+// loads take a scalar base and a constant vector offset.
+__device__ __forceinline__ void load_code_tile(signed char const* __restrict__ C, int t, int lo, v4i (&a)[2])
 {
+    signed char const* tp = C + (size_t)t * (kCT * kCodeBytes);
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb)
-        a[cb] = *reinterpret_cast<v4i const*>(C + (size_t)(t * kCT + cb * 32 + c32) * kCodeBytes + 16 * h);
+    for (int cb = 0; cb < 2; ++cb) a[cb] = *reinterpret_cast<v4i const*>(tp + lo + cb * 32 * kCodeBytes);
 }
 
 // The four 32x32 blocks of a tile (2 candidate blocks x 2 query blocks), one candidate
@@ -101,6 +104,11 @@ __device__ __forceinline__ void code_block(signed char const* __restrict__ Q, in
     v16i pos;
 #pragma unroll
     for (int r = 0; r < 16; ++r) pos[r] = r;
+    // held in vector registers across the tile loop (as a constant it is rebuilt from
+    // scalar registers for every MFMA)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) __asm__ volatile("" : "+v"(pos[r]));
+    int const lo = c32 * kCodeBytes + 16 * h;
     KeyStateI st[2];
 #pragma unroll
     for (int qb = 0; qb < 2; ++qb) st[qb] = KeyStateI{{INT_MIN, INT_MIN}, {-1, -1}};
@@ -109,9 +117,9 @@ __device__ __forceinline__ void code_block(signed char const* __restrict__ Q, in
     // the next tile's A operands are loaded while the current tile is processed (the index
     // is clamped to the split's last tile, so every load stays in range)
     v4i a[2], an[2];
-    load_code_tile(C, t0, h, c32, a);
+    load_code_tile(C, t0, lo, a);
     for (int t = t0; t < tFull; ++t) {
-        load_code_tile(C, min(t + 1, t1 - 1), h, c32, an);
+        load_code_tile(C, min(t + 1, t1 - 1), lo, an);
         code_tile<true>(a, t, bq, pos, st, nC, h);
         a[0] = an[0];
         a[1] = an[1];

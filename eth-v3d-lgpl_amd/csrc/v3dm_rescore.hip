@@ -127,7 +127,28 @@ __device__ __forceinline__ int select_candidates(List const* __restrict__ part, 
         }
     }
     float thr;
+    int within = kRescoreSlots + 1;
     if (topK > 0) {
+        // the group's best value; when at most topK candidates lie within `margin` of it,
+        // they are the tournament's result below (it takes them all, in rounds)
+        float m = v[0];
+#pragma unroll
+        for (int u = 1; u < kPerLane; ++u) m = fmaxf(m, v[u]);
+        m = fmaxf(m, __shfl_xor(m, 1));
+        m = fmaxf(m, __shfl_xor(m, 2));
+        m = fmaxf(m, __shfl_xor(m, 4));
+        thr = m - margin;
+        int c = 0;
+#pragma unroll
+        for (int u = 0; u < kPerLane; ++u) c += (j[u] >= 0 && !(v[u] < thr)) ? 1 : 0;
+        c += __shfl_xor(c, 1);
+        c += __shfl_xor(c, 2);
+        c += __shfl_xor(c, 4);
+        within = (m == -INFINITY) ? 0 : c;
+    }
+    if (topK > 0 && within <= topK) {
+        // thr as set above (no candidate: nothing is at or above it)
+    } else if (topK > 0) {
         // tournament: round k takes the largest remaining value of the group; thr = the
         // value taken in round topK, or the last one within `margin` of the first (or the
         // last finite one)
@@ -395,17 +416,20 @@ __device__ __forceinline__ void batch_finish(BatchDev const& b, Cand2 const* __r
     add_stripe(n, stripes);
 }
 
-// uint8 images: certified integer scores (score_u8_certified) and the certified decision;
-// a query whose decision the bound leaves open (near ties and thresholds) is scored again
-// exactly from its FP32 row, staged in LDS (certify = false: every query is).  Blocks run in XCD order (xcd_block), so the
-// queries of one segment, which read the same candidate image, share an L2.
-__global__ __launch_bounds__(256) void k_batch_rescore_u8(BatchDev b, Cand2 const* __restrict__ part, float winRel,
-                                                          int topK, float margin, float minScore, float minRatio,
-                                                          bool certify, int* __restrict__ sel,
-                                                          int* __restrict__ stripes)
+// uint8 images: certified integer scores (score_u8_certified) and the certified decision.
+// A query whose decision the bound leaves open (near ties and thresholds; certify = false:
+// every query) is appended to `open` (count in *nOpen) and decided by
+// k_batch_rescore_open.  Blocks run in XCD order (xcd_block), so the queries of one
+// segment, which read the same candidate image, share an L2.  No exact code here: few
+// registers, eight waves per SIMD to cover the chain of dependent loads (segment, lists,
+// candidate rows).
+__global__ __launch_bounds__(256, 8) void k_batch_rescore_u8(BatchDev b, Cand2 const* __restrict__ part, float winRel,
+                                                             int topK, float margin, float minScore, float minRatio,
+                                                             bool certify, int* __restrict__ sel,
+                                                             int* __restrict__ open, int* __restrict__ nOpen,
+                                                             int* __restrict__ stripes)
 {
     __shared__ int slots[32][kRescoreSlots];
-    __shared__ __attribute__((aligned(16))) float qrow[32][kDim];
     int const sub = threadIdx.x & 7;
     int const grp = threadIdx.x >> 3;
     int const g = xcd_block(blockIdx.x, gridDim.x) * 32 + grp;
@@ -423,29 +447,44 @@ __global__ __launch_bounds__(256) void k_batch_rescore_u8(BatchDev b, Cand2 cons
                                                 b.norm[row], __uint_as_float(b.maxNorm[sg.oImg]), winRel, topK,
                                                 margin, sub, slots[grp]);
             n = (sub == 0) ? total : 0;
-            Top2x s;
             d = kFlag;
             if (certify) {
-                s = score_u8_certified(qv, ra, b.u8 + cb * kDim, b.denom + cb, slots[grp], total, sub);
+                Top2x const s = score_u8_certified(qv, ra, b.u8 + cb * kDim, b.denom + cb, slots[grp], total, sub);
                 d = decide_certified(Top2{s.b1, s.b2, s.a1}, kCertU8 * fmaxf(s.b1, 0.0f) + kEpsAbs, minScore,
                                      minRatio);
             }
-            if (d == kFlag) {  // uniform over the group
-                float4 const* qd = reinterpret_cast<float4 const*>(b.img + (size_t)row * kDim);
-#pragma unroll
-                for (int u = 0; u < 4; ++u) reinterpret_cast<float4*>(qrow[grp])[sub + 8 * u] = qd[sub + 8 * u];
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                int scored = 0;
-                s = score_exact<true>(qrow[grp], nullptr, b.u8 + cb * kDim, b.denom + cb, slots[grp], total, sub,
-                                      scored);
-                d = decide_exact(Top2{s.b1, s.b2, s.a1}, minScore, minRatio);
-            }
+            if (d == kFlag && sub == 0) open[atomicAdd(nOpen, 1)] = g;
         }
         if (sub == 0) sel[g] = d;
     }
     add_stripe(n, stripes);
+}
+
+// The open queries of k_batch_rescore_u8, eight lanes each (grid-stride over the list): the
+// same candidate selection, exact scores (the oracle's arithmetic on the FP32 rows).
+__global__ __launch_bounds__(256) void k_batch_rescore_open(BatchDev b, Cand2 const* __restrict__ part, float winRel,
+                                                            int topK, float margin, float minScore, float minRatio,
+                                                            int* __restrict__ sel, int const* __restrict__ open,
+                                                            int const* __restrict__ nOpen)
+{
+    __shared__ int slots[32][kRescoreSlots];
+    int const sub = threadIdx.x & 7;
+    int const grp = threadIdx.x >> 3;
+    int const cnt = *nOpen;
+    for (int w = blockIdx.x * 32 + grp; w < cnt; w += gridDim.x * 32) {  // uniform over the group
+        int const g = open[w];
+        BatchSeg const sg = b.seg[b.segOfBlock[g / kBatchRowAlign]];
+        int const q = g - sg.qOff;
+        int const row = b.imgBase[sg.img] + q;
+        size_t const cb = (size_t)b.imgBase[sg.oImg];
+        int const total = select_candidates(part + ((size_t)sg.partOff + q) * 2, 2 * (size_t)sg.ld, sg.nParts, 0,
+                                            b.norm[row], __uint_as_float(b.maxNorm[sg.oImg]), winRel, topK, margin,
+                                            sub, slots[grp]);
+        int scored = 0;
+        Top2x const s = score_exact<true>(b.img + (size_t)row * kDim, nullptr, b.u8 + cb * kDim, b.denom + cb,
+                                          slots[grp], total, sub, scored);
+        if (sub == 0) sel[g] = decide_exact(Top2{s.b1, s.b2, s.a1}, minScore, minRatio);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_batch_shortlist_finish(BatchDev b, Cand2 const* __restrict__ part,
@@ -457,15 +496,18 @@ __global__ __launch_bounds__(256) void k_batch_shortlist_finish(BatchDev b, Cand
 }
 
 void launch_batch_rescore(BatchDev const& b, Cand2 const* part, float winRel, int topK, float margin, float minScore,
-                          float minRatio, bool certify, int* sel, int* stripes, hipStream_t st)
+                          float minRatio, bool certify, int* sel, int* open, int* nOpen, int* stripes, hipStream_t st)
 {
     if (b.qRows <= 0) return;
-    if (b.u8)
+    if (b.u8) {
         hipLaunchKernelGGL(k_batch_rescore_u8, dim3((b.qRows + 31) / 32), dim3(256), 0, st, b, part, winRel, topK,
-                           margin, minScore, minRatio, certify, sel, stripes);
-    else
+                           margin, minScore, minRatio, certify, sel, open, nOpen, stripes);
+        hipLaunchKernelGGL(k_batch_rescore_open, dim3(256), dim3(256), 0, st, b, part, winRel, topK, margin, minScore,
+                           minRatio, sel, open, nOpen);
+    } else {
         hipLaunchKernelGGL(k_batch_shortlist_finish, dim3((b.qRows + 31) / 32), dim3(256), 0, st, b, part, winRel,
                            topK, margin, minScore, minRatio, sel, stripes);
+    }
 }
 
 }  // namespace v3dm
