@@ -22,8 +22,8 @@
 namespace v3dm {
 
 void launch_batch_load_u8(uint8_t const* src, int const* srcRow, int const* nImg, int const* blkImg,
-                          int const* imgBase, int rows, float* img, uint8_t* imgU8, float* denom, signed char* img8,
-                          int2* rowCS, hipStream_t st);
+                          int const* imgBase, int rows, float* img, uint8_t* imgU8, float* denom, int2* rowCS,
+                          hipStream_t st);
 void launch_batch_load_f32(float const* src, int const* srcRow, int const* nImg, int const* blkImg,
                            int const* imgBase, int rows, float* img, hipStream_t st);
 
@@ -38,7 +38,9 @@ MissLimits miss_limits(v3dm_context const* h)
 
 int roundUp(int n, int m) { return (n + m - 1) / m * m; }
 
-constexpr int kCheckTiles = 8;  // 32-row candidate tiles per item of the sampled check
+// 32-row candidate tiles per item of the sampled check's phases (phase 1 has few queries per
+// pair: longer items)
+constexpr int kCheckTiles[2] = {8, 32};
 
 struct Plan {
     std::vector<int4> items;
@@ -63,8 +65,7 @@ struct BatchState {
     signed char* codes = nullptr;  // learned codes of every image row
     uint8_t* imgU8 = nullptr;      // uint8 rows (uint8 uploads), aligned layout
     float* denom = nullptr;
-    signed char* img8 = nullptr;   // sampled check of uint8 batches: rows ^ 0x80 and rowCS
-    int2* rowCS = nullptr;
+    int2* rowCS = nullptr;          // sampled check of uint8 batches: row constants
     bool haveU8 = false;
     bool codesValid = false;
     float codesScale = 0.0f;
@@ -231,11 +232,12 @@ BatchDev dev_view(BatchState* b, int form)
 CheckDev check_view(BatchState* b, int samples)
 {
     CheckDev c;
-    c.img8 = b->img8;
+    c.u8 = b->imgU8;
     c.rowCS = b->rowCS;
     c.part = b->cPart;
     c.maxSplits = b->cMaxSplits;
-    c.tilesPerSplit = kCheckTiles;
+    c.tilesPerSplit[0] = kCheckTiles[0];
+    c.tilesPerSplit[1] = kCheckTiles[1];
     c.nSplits = b->cNSplits;
     c.qB = b->cQB;
     c.slotB = b->cSlotB;
@@ -415,11 +417,12 @@ int ensure_sample_plan(v3dm_context* h, BatchState* b)
                 if (nL <= 0 || nR <= 0) continue;
                 int const nA = std::max(nL, nR), nB = std::min(nL, nR);
                 int const nTiles = (phase == 0 ? nB : nA) / 32;
-                int const splits = (nTiles + kCheckTiles - 1) / kCheckTiles;
+                int const tps = kCheckTiles[phase];
+                int const splits = (nTiles + tps - 1) / tps;
                 nSplits[phase * P + p] = splits;
                 maxS = std::max(maxS, splits);
                 for (int sp = 0; sp < splits; ++sp)
-                    work.push_back({std::min(kCheckTiles, nTiles - sp * kCheckTiles), make_int4(p, 0, sp, kCheckTiles)});
+                    work.push_back({std::min(tps, nTiles - sp * tps), make_int4(p, 0, sp, tps)});
             }
             items[phase] = sorted_items(work);
         }
@@ -572,10 +575,10 @@ int enqueue_run(v3dm_context* h, BatchState* b, int mode, float minScore, float 
             if (b->haveU8) {
                 // exact mutual matches among the samples against the learned result
                 CheckDev const c = check_view(b, h->fallbackSamples);
-                launch_batch_check_probe(d, c, 0, b->cItems[0], b->nCItems[0], b->counters + kBQueueCheck0, 3 * cus, st);
+                launch_batch_check_probe(d, c, 0, b->cItems[0], b->nCItems[0], b->counters + kBQueueCheck0, cus, st);
                 launch_batch_check_decide(d, c, 0, minScore, minRatio, lim, b->sel, b->pairFlag,
                                           b->pairFlag + b->nPairs, st);
-                launch_batch_check_probe(d, c, 1, b->cItems[1], b->nCItems[1], b->counters + kBQueueCheck1, 3 * cus, st);
+                launch_batch_check_probe(d, c, 1, b->cItems[1], b->nCItems[1], b->counters + kBQueueCheck1, cus, st);
                 launch_batch_check_decide(d, c, 1, minScore, minRatio, lim, b->sel, b->pairFlag,
                                           b->pairFlag + b->nPairs, st);
             } else {
@@ -772,11 +775,10 @@ int set_images(v3dm_context* h, int nImages, int32_t const* counts, void const* 
         if (b->norm) (void)hipFree(b->norm);
         if (b->codes) (void)hipFree(b->codes);  // sized with the image rows: allocated again on use
         b->codes = nullptr;
-        for (void* ptr : {(void*)b->imgU8, (void*)b->denom, (void*)b->img8, (void*)b->rowCS})
+        for (void* ptr : {(void*)b->imgU8, (void*)b->denom, (void*)b->rowCS})
             if (ptr) (void)hipFree(ptr);
         b->imgU8 = nullptr;
         b->denom = nullptr;
-        b->img8 = nullptr;
         b->rowCS = nullptr;
         b->img = nullptr;
         b->imgB = nullptr;
@@ -831,13 +833,12 @@ int set_images(v3dm_context* h, int nImages, int32_t const* counts, void const* 
     if (u8 && !b->imgU8) {
         BCHECK(h, hipMalloc(&b->imgU8, (size_t)b->imgCapRows * kDim));
         BCHECK(h, hipMalloc(&b->denom, (size_t)b->imgCapRows * sizeof(float)));
-        BCHECK(h, hipMalloc(&b->img8, (size_t)b->imgCapRows * kDim));
         BCHECK(h, hipMalloc(&b->rowCS, (size_t)b->imgCapRows * sizeof(int2)));
     }
     b->haveU8 = u8;
     if (u8)
         launch_batch_load_u8(static_cast<uint8_t const*>(b->upload), b->srcRow, b->nImgD, b->blkImg, b->imgBase, rows,
-                             b->img, b->imgU8, b->denom, b->img8, b->rowCS, st);
+                             b->img, b->imgU8, b->denom, b->rowCS, st);
     else
         launch_batch_load_f32(static_cast<float const*>(b->upload), b->srcRow, b->nImgD, b->blkImg, b->imgBase, rows,
                               b->img, st);
@@ -861,7 +862,7 @@ void batch_destroy(BatchState*& b)
                     (void*)b->segD[0], (void*)b->segD[1], (void*)b->segD[2], (void*)b->segD[3], (void*)b->partF,
                     (void*)b->partB, (void*)b->sel, (void*)b->flagList, (void*)b->openList, (void*)b->sRows, (void*)b->sNorm, (void*)b->sIdx, (void*)b->sCount,
                     (void*)b->sNSplits, (void*)b->sPart, (void*)b->sItems, (void*)b->pairFlag, (void*)b->fbItems,
-                    (void*)b->segMask, (void*)b->img8, (void*)b->rowCS, (void*)b->cItems[0],
+                    (void*)b->segMask, (void*)b->rowCS, (void*)b->cItems[0],
                     (void*)b->cItems[1], (void*)b->cNSplits, (void*)b->cPart, (void*)b->cQB, (void*)b->cSlotB,
                     (void*)b->cCntB, (void*)b->cDecA, (void*)b->counters, (void*)b->stripes, (void*)b->pairCount,
                     (void*)b->tmp, (void*)b->blockCount, (void*)b->out})

@@ -71,19 +71,18 @@ struct BatchDev {
     int qRows;                    // size of the query space
 };
 
-// Sampled check of uint8 batches (v3dm_check.hip).  Per image row: img8 = the uint8 row ^
-// 0x80 (int8 u - 128), rowCS = (128 * sum of the uint8 components, bits of the row's
-// 1/sqrt scale); rows that are no candidate (all-zero rows, padding) have rowCS =
-// (kDegenerateC, bits of 1).
+// Sampled check of uint8 batches (v3dm_check.hip).  Per image row: rowCS = (128 * sum of
+// the uint8 components, bits of the row's 1/sqrt scale); rows that are no candidate
+// (all-zero rows, padding) have rowCS = (kDegenerateC, bits of 1).
 constexpr int kDegenerateC = -(1 << 30);
 constexpr int kChkMaxSplits = 1024;  // splits of one check query (64 Ki candidates at 8 tiles per split)
 
 struct CheckDev {
-    signed char const* img8;
+    uint8_t const* u8;   // the batch's uint8 rows
     int2 const* rowCS;
     int4* part;          // keys per (phase, pair, split, query slot)
     int maxSplits;
-    int tilesPerSplit;   // 32-row candidate tiles per split
+    int tilesPerSplit[2];  // 32-row candidate tiles per split, per phase
     int const* nSplits;  // per (phase, pair)
     int* qB;             // phase 1 queries: matched features of B, per pair kSampleRows slots
     int* slotB;          // ... and the sample slot that matched each
@@ -137,7 +136,7 @@ void launch_batch_sample_decide(BatchDev const& b, Top2 const* sPart, int maxSpl
 // the phase 1 decide sets pairFlag[p] and counts[4p..] = exact matches among the samples,
 // missed, decisions taken exactly, samples
 void launch_batch_check_probe(BatchDev const& b, CheckDev const& c, int phase, int4 const* items, int nItems,
-                              int* queue, int grid, hipStream_t st);
+                              int* queue, int cus, hipStream_t st);
 void launch_batch_check_decide(BatchDev const& b, CheckDev const& c, int phase, float minScore, float minRatio,
                                MissLimits const& lim, int const* sel, int* pairFlag, int* counts, hipStream_t st);
 void launch_batch_recheck(BatchDev const& b, int const* flagList, int* counters, float minScore, float minRatio,

@@ -383,15 +383,14 @@ void launch_batch_mutual(BatchDev const& b, int const* sel, int leftRows, int2* 
 // a feature when below nImg[m] (the count rounded down to 32) and zero padding otherwise.
 // The uint8 rows are also kept in the aligned layout with their scale (imgU8, denom): the
 // exact rescoring of the shortlist forms reads 128 bytes per candidate instead of 512 and
-// forms the same float values.  The sampled check's rows: img8 (u ^ 0x80), rowCS.x (128 * the
+// forms the same float values.  The sampled check's row constants: rowCS.x (128 * the
 // component sum, kDegenerateC for all-zero and padding rows) and rowCS.y (the bits of the
 // scale, or of 1).
 __global__ __launch_bounds__(256) void k_batch_load_u8(uint8_t const* __restrict__ src, int const* __restrict__ srcRow,
                                                        int const* __restrict__ nImg, int const* __restrict__ blkImg,
                                                        int const* __restrict__ imgBase, int rows,
                                                        float* __restrict__ img, uint8_t* __restrict__ imgU8,
-                                                       float* __restrict__ denom, signed char* __restrict__ img8,
-                                                       int2* __restrict__ rowCS)
+                                                       float* __restrict__ denom, int2* __restrict__ rowCS)
 {
     int const lane = threadIdx.x & 63;
     int const row = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -410,7 +409,6 @@ __global__ __launch_bounds__(256) void k_batch_load_u8(uint8_t const* __restrict
     for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
     reinterpret_cast<float2*>(img + (size_t)row * kDim)[lane] = v;
     reinterpret_cast<uchar2*>(imgU8 + (size_t)row * kDim)[lane] = q;
-    reinterpret_cast<uchar2*>(img8 + (size_t)row * kDim)[lane] = make_uchar2(q.x ^ 0x80u, q.y ^ 0x80u);
     if (lane == 0) {
         bool const valid = d > 0.0f && d < INFINITY;  // a feature with a nonzero component
         denom[row] = d;
@@ -434,12 +432,12 @@ __global__ __launch_bounds__(256) void k_batch_load_f32(float const* __restrict_
 }
 
 void launch_batch_load_u8(uint8_t const* src, int const* srcRow, int const* nImg, int const* blkImg,
-                          int const* imgBase, int rows, float* img, uint8_t* imgU8, float* denom, signed char* img8,
-                          int2* rowCS, hipStream_t st)
+                          int const* imgBase, int rows, float* img, uint8_t* imgU8, float* denom, int2* rowCS,
+                          hipStream_t st)
 {
     if (rows <= 0) return;
     hipLaunchKernelGGL(k_batch_load_u8, dim3((rows + 3) / 4), dim3(256), 0, st, src, srcRow, nImg, blkImg, imgBase,
-                       rows, img, imgU8, denom, img8, rowCS);
+                       rows, img, imgU8, denom, rowCS);
 }
 
 void launch_batch_load_f32(float const* src, int const* srcRow, int const* nImg, int const* blkImg,

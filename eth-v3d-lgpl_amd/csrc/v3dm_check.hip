@@ -9,7 +9,7 @@
 // result holds it when its decisions of i_k and j_k are j_k and i_k.  The pair falls back
 // to the plain form when its misses exceed the limit for its number of exact matches.
 //
-// Scores: the rows are kept as int8 (u ^ 0x80 = u - 128), so one v_mfma_i32_32x32x32_i8
+// Scores: the uint8 rows are taken as int8 (u ^ 0x80 = u - 128), so one v_mfma_i32_32x32x32_i8
 // chain over 128 components gives D = sum_k (qa_k - 128)(qb_k - 128), and with the MFMA's
 // accumulator input set to 128 * sum(qb) the integer I = sum_k qa_k qb_k = D + 128 * sum(qa)
 // - 2^21 is exact.  t = fl32(rb * I) and s~ = fl32(ra * t) (ra, rb the rows' 1/sqrt
@@ -126,7 +126,7 @@ __device__ __forceinline__ void stage_load(CheckDev const& c, size_t cBase, int 
         int const t = t0 + grp * parts + g;
         if (g < parts && t < t1) {
             size_t const row = cBase + (size_t)t * kChkTile;
-            r.v[g] = *reinterpret_cast<uint4 const*>(c.img8 + (row + (tid >> 3)) * kDim + 16 * (tid & 7));
+            r.v[g] = *reinterpret_cast<uint4 const*>(c.u8 + (row + (tid >> 3)) * kDim + 16 * (tid & 7));
             if (tid < kChkTile) r.cs[g] = c.rowCS[row + tid];
         }
     }
@@ -138,7 +138,9 @@ __device__ __forceinline__ void stage_store(unsigned char* st, int2* stCS, int p
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
         if (g < parts) {
-            *reinterpret_cast<uint4*>(st + g * kTileB + (tid >> 3) * kRowPadB + 16 * (tid & 7)) = r.v[g];
+            uint4 const v = r.v[g];  // uint8 -> int8 u - 128
+            *reinterpret_cast<uint4*>(st + g * kTileB + (tid >> 3) * kRowPadB + 16 * (tid & 7)) =
+                make_uint4(v.x ^ 0x80808080u, v.y ^ 0x80808080u, v.z ^ 0x80808080u, v.w ^ 0x80808080u);
             if (tid < kChkTile) stCS[g * kChkTile + tid] = r.cs[g];
         }
     }
@@ -185,8 +187,10 @@ __global__ __launch_bounds__(256) void k_batch_check_probe(BatchDev b, CheckDev 
             int const qLocal = chk_query(c, phase, p, min(k, nQ - 1), nQ, sA.n);
             size_t const qRow = (size_t)b.imgBase[phase == 0 ? sA.img : sB.img] + qLocal;
 #pragma unroll
-            for (int m = 0; m < 4; ++m)
-                bq[m] = *reinterpret_cast<v4i const*>(c.img8 + qRow * kDim + 32 * m + 16 * h);
+            for (int m = 0; m < 4; ++m) {
+                v4i const u = *reinterpret_cast<v4i const*>(c.u8 + qRow * kDim + 32 * m + 16 * h);
+                bq[m] = u ^ (int)0x80808080;
+            }
             cq = c.rowCS[qRow].x - (1 << 21);
         }
         StageRegs r;
@@ -319,7 +323,7 @@ __global__ __launch_bounds__(128) void k_batch_check_decide(BatchDev b, CheckDev
     // hold more and is scanned again with the integer scores.
     int const nO = nOpen;
     int const ns = c.nSplits[phase * b.nPairs + p];
-    int const span = c.tilesPerSplit * kChkTile;  // candidates per split
+    int const span = c.tilesPerSplit[phase] * kChkTile;  // candidates per split
     for (int o = 0; o < nO; ++o) {
         int const kq = openSlot[o];
         int const ql = chk_query(c, phase, p, kq, nQ, sA.n);
@@ -450,10 +454,11 @@ __global__ __launch_bounds__(128) void k_batch_check_decide(BatchDev b, CheckDev
 }
 
 void launch_batch_check_probe(BatchDev const& b, CheckDev const& c, int phase, int4 const* items, int nItems,
-                              int* queue, int grid, hipStream_t st)
+                              int* queue, int cus, hipStream_t st)
 {
     if (nItems <= 0) return;
-    hipLaunchKernelGGL(k_batch_check_probe, dim3(grid), dim3(256), 0, st, b, c, phase, items, nItems, queue);
+    // work-groups resident per CU (LDS and registers)
+    hipLaunchKernelGGL(k_batch_check_probe, dim3(3 * cus), dim3(256), 0, st, b, c, phase, items, nItems, queue);
 }
 
 void launch_batch_check_decide(BatchDev const& b, CheckDev const& c, int phase, float minScore, float minRatio,
