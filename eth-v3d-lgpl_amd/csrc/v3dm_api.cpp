@@ -557,6 +557,11 @@ int v3dm_find_putative_matches_async(v3dm_handle h, float minScore, float minRat
     if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
     if (!h->allocated) return fail(h, V3DM_ERR_STATE, "findPutativeMatches before allocate");
     HIPCHECK(h, hipSetDevice(h->device));
+    // a pending sampled check of the previous (learned) call is evaluated before this call
+    // replaces its result -- a failing check reruns the plain form for it -- so that every
+    // learned call pays for its check and any fallback, read or not
+    int const rcf = ext_finish(h);
+    if (rcf) return rcf;
     h->lastMinScore = minScore;
     h->lastMinRatio = minRatio;
     h->stats.nFallback = 0;
@@ -646,6 +651,29 @@ int v3dm_set_profiling(v3dm_handle h, int enable)
     h->evPending = 0;
     h->kernelMs = 0.0;
     h->kernelLaunches = 0;
+    return V3DM_OK;
+}
+
+int v3dm_score_block(v3dm_handle h, int mode, int nRows, int nCols, float* scores)
+{
+    if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
+    if (!h->allocated) return fail(h, V3DM_ERR_STATE, "allocate first");
+    if (mode != V3DM_PREC_FP32 && mode != V3DM_PREC_BF16 && mode != V3DM_PREC_FP8)
+        return fail(h, V3DM_ERR_INVALID, "score blocks exist for the FP32, BF16 and FP8 passes");
+    if (nRows < 0 || nRows > h->nL || nCols < 0 || nCols > h->nR)
+        return fail(h, V3DM_ERR_INVALID, "block %d x %d outside the %d x %d feature sets", nRows, nCols, h->nL, h->nR);
+    if (nRows == 0 || nCols == 0) return V3DM_OK;
+    if (!scores) return fail(h, V3DM_ERR_INVALID, "null output");
+    HIPCHECK(h, hipSetDevice(h->device));
+    float* d = nullptr;
+    size_t const bytes = (size_t)nRows * nCols * sizeof(float);
+    HIPCHECK(h, hipMalloc(&d, bytes));
+    launch_score_block(mode, h->dL, h->dR, nRows, nCols, d, h->stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(scores, d, bytes, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(h, V3DM_ERR_HIP, "score block: %s", hipGetErrorString(e));
     return V3DM_OK;
 }
 

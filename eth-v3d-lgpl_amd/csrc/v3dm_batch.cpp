@@ -1074,6 +1074,38 @@ int v3dm_batch_get_fallback(v3dm_handle h, int32_t* flags)
     return V3DM_OK;
 }
 
+int v3dm_batch_time_dominant_kernel(v3dm_handle h, int reps, float* msPerLaunch)
+{
+    BatchState* b = nullptr;
+    int rc = check_batch(h, b);
+    if (rc) return rc;
+    if (reps < 1 || !msPerLaunch) return ctx_fail(h, V3DM_ERR_INVALID, "bad argument");
+    if (!b->pending) return ctx_fail(h, V3DM_ERR_STATE, "no batch run yet");
+    int const form = form_of(b->lastMode);
+    if (!b->planValid[form]) return ctx_fail(h, V3DM_ERR_STATE, "no plan for the last run's mode");
+    Plan const& pl = b->plan[form];
+    hipStream_t const st = h->stream;
+    BatchDev const d = dev_view(b, form);
+    int* q = nullptr;  // one zeroed work counter per launch
+    BCHECK(h, hipMalloc(&q, (size_t)reps * sizeof(int)));
+    hipError_t e = hipMemsetAsync(q, 0, (size_t)reps * sizeof(int), st);
+    if (e == hipSuccess) e = hipEventRecord(b->evA, st);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+        int const n = (int)pl.items.size();
+        if (form == 0) launch_batch_fused_fp32(d, b->items[0], n, q + r, b->partF, pl.grid, st);
+        else if (form == 1) launch_batch_pass_bf16(d, b->items[1], n, q + r, b->partB, pl.grid, st);
+        else launch_batch_pass_code(d, b->items[2], n, q + r, b->partB, pl.grid, st);
+    }
+    if (e == hipSuccess) e = hipEventRecord(b->evB, st);
+    if (e == hipSuccess) e = hipEventSynchronize(b->evB);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, b->evA, b->evB);
+    (void)hipFree(q);
+    if (e != hipSuccess) return ctx_fail(h, V3DM_ERR_HIP, "timing the batch pass failed");
+    *msPerLaunch = ms / reps;
+    return V3DM_OK;
+}
+
 int v3dm_batch_get_check(v3dm_handle h, int32_t* counts)
 {
     if (!h || !counts) return V3DM_ERR_INVALID;

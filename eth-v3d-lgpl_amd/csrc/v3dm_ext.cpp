@@ -212,7 +212,6 @@ int enqueue_learned(v3dm_context* h, float minScore, float minRatio)
         launch_mlp_codes2_f32(h->dL, e->cLValid ? 0 : (nL + 127) / 128 * 128, e->cL, h->dR,
                               e->cRValid ? 0 : (nR + 127) / 128 * 128, e->cR, *w, scale, st);
         e->cLValid = e->cRValid = true;
-        e->sPending = false;
         bool const check = h->fallbackSamples > 0;
         // the main stream's kernels are enqueued first: the check's stream operations cost
         // more host time than its kernels take, and the main stream should not wait for them
@@ -388,6 +387,7 @@ void ext_features_changed(v3dm_context* h, bool left)
 {
     ExtState* e = ctx_ext(h);
     if (!e) return;
+    e->sPending = false;  // the last result is of the old features: its check is moot
     if (left) e->bLValid = e->f8LValid = e->cLValid = false;
     else e->bRValid = e->f8RValid = e->cRValid = false;
 }

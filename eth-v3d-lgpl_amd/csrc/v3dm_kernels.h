@@ -88,6 +88,10 @@ void launch_guided_mask(int kind, float const (&m)[9], float sqrThreshold, float
 void launch_guided_fp32(float const* L, float const* R, int nL, int nR, FusedPlan const& p, Top2* colPart, int ldc,
                         Top2* rowPart, int ldr, unsigned const* bits, int ldi, hipStream_t st);
 
+// measurement: the approximate scores of a form's pass, out[i * nCols + j] (v3dm_scores.hip;
+// mode 0 FP32, 1 BF16, 2 FP8)
+void launch_score_block(int mode, float const* L, float const* R, int nRows, int nCols, float* out, hipStream_t st);
+
 // sampled exact check of a shortlist result (v3dm_select.hip)
 // (also zeroes the check's counters)
 void launch_sample_gather(float const* L, float const* normL, int nL, int S, float* Ls, float* normS, int* idx,

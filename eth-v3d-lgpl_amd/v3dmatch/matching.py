@@ -231,6 +231,13 @@ class SIFT_Matching:
         self._check(self._lib.v3dm_time_dominant_kernel(self._h, int(reps), C.byref(ms)))
         return ms.value
 
+    def score_block(self, mode, n_rows, n_cols):
+        """The approximate scores of mode's pass (PREC_FP32 / BF16 / FP8) for the first
+        n_rows left x n_cols right features, float32 (n_rows, n_cols)."""
+        out = np.empty((max(n_rows, 1), max(n_cols, 1)), dtype=np.float32)
+        self._check(self._lib.v3dm_score_block(self._h, int(mode), int(n_rows), int(n_cols), out.ctypes.data))
+        return out[:n_rows, :n_cols]
+
     def get_second_best(self):
         """(colSecond, rowSecond) of the last shortlist-mode call: per feature, the feature
         of the other set with the second best exact score among the rescored candidates."""
@@ -319,6 +326,13 @@ class SIFT_Matching:
         out = np.zeros((self._bpairs, 4), dtype=np.int32)
         self._check(self._lib.v3dm_batch_get_check(self._h, out.ctypes.data))
         return out
+
+    def batch_time_dominant_kernel(self, reps=10):
+        """Average ms of one launch of the last batch run's pass kernel (launched reps times
+        back to back between one event pair)."""
+        ms = C.c_float(0)
+        self._check(self._lib.v3dm_batch_time_dominant_kernel(self._h, int(reps), C.byref(ms)))
+        return ms.value
 
     def batch_last_ms(self):
         ms = C.c_float(0)

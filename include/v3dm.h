@@ -149,6 +149,12 @@ int v3dm_fallback_allowed_misses_binomial(int accepted, float minRecall);
  * average duration of one launch.  Overwrites the intermediate partial results (not the
  * match list or colMax / rowMax of the last call). */
 int v3dm_time_dominant_kernel(v3dm_handle h, int reps, float* msPerLaunch);
+/* Measurement helper (no reference analogue): the approximate scores the pass of mode
+ * V3DM_PREC_FP32 / BF16 / FP8 computes for the first nRows left x nCols right features of
+ * the current sets (same instruction, operand rounding and summation order as the pass,
+ * which never writes its scores), row-major into the host array scores[nRows * nCols].
+ * Synchronous. */
+int v3dm_score_block(v3dm_handle h, int mode, int nRows, int nCols, float* scores);
 /* Run the library's kernels on this HIP stream (hipStream_t, NULL = the handle's own). */
 int v3dm_set_stream(v3dm_handle h, void* hipStream);
 /* Feature upload from device memory (n*128 floats, device pointer); no host copy. */
@@ -274,6 +280,11 @@ int v3dm_batch_use_graph(v3dm_handle h, int enable);
  * plain form's matches, in the same run.  (Batches uploaded as FP32 compare the samples'
  * own decisions instead.)  flags[p] = 1 for the pairs of the last run that did. */
 int v3dm_batch_get_fallback(v3dm_handle h, int32_t* flags);
+/* Measurement helper: the pass kernel of the last batch run's mode (plain: the fused FP32
+ * pass; BF16: the BF16 pass; learned: the code pass), launched `reps` times back to back
+ * with its plan between one HIP event pair on the handle's stream; average ms per launch.
+ * Leaves the last run's matches readable. */
+int v3dm_batch_time_dominant_kernel(v3dm_handle h, int reps, float* msPerLaunch);
 /* Per pair of the last learned batch run, 4 ints: exact matches among the samples,
  * missed by the learned result, sampled decisions the certified scores left to the exact
  * FP64 scores, samples. */
