@@ -94,6 +94,8 @@ SIGNATURES = {
     "v3dm_batch_get_check": (_I, [_P, _FP]),
     "v3dm_batch_time_dominant_kernel": (_I, [_P, _I, C.POINTER(C.c_float)]),
     "v3dm_score_block": (_I, [_P, _I, _I, _I, _FP]),
+    "v3dm_match_device_noalloc": (_I, [_FP, _I, _FP, _I, _F, _F, _FP, _FP, _P]),
+    "v3dm_noalloc_last_error": (C.c_char_p, []),
     "v3dm_learned_set_state": (_I, [_P, _FP, _FP, _FP, _I]),
     "v3dm_learned_get_state": (_I, [_P, _FP, _FP, _FP, _IP]),
     "v3dm_learned_set_code_scale": (_I, [_P, _F]),

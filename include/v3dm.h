@@ -149,6 +149,24 @@ int v3dm_fallback_allowed_misses_binomial(int accepted, float minRecall);
  * average duration of one launch.  Overwrites the intermediate partial results (not the
  * match list or colMax / rowMax of the last call). */
 int v3dm_time_dominant_kernel(v3dm_handle h, int reps, float* msPerLaunch);
+/* findPutativeMatches of the plain FP32 form on device-resident descriptors, allocating
+ * nothing.  No counterpart in the reference: for callers that cannot allocate device
+ * memory (HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * leftDesc / rightDesc: device arrays of nLeft / nRight x 128 floats (counts rounded down
+ * to multiples of 32, as setLeftFeatures does, cu:493-518; each at most
+ * V3DM_NOALLOC_MAX_FEATURES); pairs: device array of at least 2 * min(nLeft, nRight) ints
+ * receiving (left, right) sorted by left; count: device int receiving the number of
+ * matches (-1: the mutual check's ordering wait ran out).  Enqueued on `stream` (a
+ * hipStream_t; NULL = the null stream), asynchronous; the match set equals
+ * v3dm_find_putative_matches's.  The work buffers are one file-scope device array of
+ * about 32 MiB per GPU (held until the process ends), shared by every call on that GPU:
+ * calls that overlap on one GPU race.  Errors: V3DM_ERR_INVALID (counts outside [0,
+ * V3DM_NOALLOC_MAX_FEATURES], null pointers), V3DM_ERR_HIP; message in
+ * v3dm_noalloc_last_error(). */
+#define V3DM_NOALLOC_MAX_FEATURES 8192
+int v3dm_match_device_noalloc(const float* leftDesc, int nLeft, const float* rightDesc, int nRight, float minScore,
+                              float minRatio, int32_t* pairs, int32_t* count, void* stream);
+const char* v3dm_noalloc_last_error(void);
 /* Measurement helper (no reference analogue): the approximate scores the pass of mode
  * V3DM_PREC_FP32 / BF16 / FP8 computes for the first nRows left x nCols right features of
  * the current sets (same instruction, operand rounding and summation order as the pass,
