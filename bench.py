@@ -521,20 +521,20 @@ def bench_guided(L, R, steps, warmup):
     exactly and returns the match list to the host."""
     import v3dmatch
     from v3dmatch import synth
-    _, _, truth = synth.make_pair(N_FEAT, N_FEAT, 0.4, config=2, pair=0)
+    _, _, truth = synth.make_pair(N_PAIR, N_PAIR, 0.4, config=2, pair=0)
     rng = np.random.default_rng(5)
     H = np.array([[1.02, 0.03, 12.0], [-0.02, 0.99, -7.0], [1e-5, -2e-5, 1.0]])
-    lxy = np.stack([rng.uniform(0, 1024, N_FEAT), rng.uniform(0, 768, N_FEAT)], 1)
-    rxy = np.stack([rng.uniform(0, 1024, N_FEAT), rng.uniform(0, 768, N_FEAT)], 1)
+    lxy = np.stack([rng.uniform(0, 1024, N_PAIR), rng.uniform(0, 768, N_PAIR)], 1)
+    rxy = np.stack([rng.uniform(0, 1024, N_PAIR), rng.uniform(0, 768, N_PAIR)], 1)
     cp = truth >= 0
     hp = (H @ np.c_[lxy[truth[cp]], np.ones(cp.sum())].T).T
     rxy[cp] = hp[:, :2] / hp[:, 2:3] + rng.standard_normal((cp.sum(), 2))
     lxy, rxy = lxy.astype(np.float32), rxy.astype(np.float32)
     m = v3dmatch.SIFT_Matching(0)
     m.enableRoundingWarning(False)
-    m.allocate(N_FEAT, N_FEAT, True)
-    m.setLeftFeatures(N_FEAT, L)
-    m.setRightFeatures(N_FEAT, R)
+    m.allocate(N_PAIR, N_PAIR, True)
+    m.setLeftFeatures(N_PAIR, L)
+    m.setRightFeatures(N_PAIR, R)
     put = m.find_putative_matches_array(MIN_SCORE, MIN_RATIO)
     for _ in range(warmup):
         m.run_guided_array(True, H, 3.0, lxy, rxy, MIN_SCORE, MIN_RATIO)
@@ -543,7 +543,7 @@ def bench_guided(L, R, steps, warmup):
         g = m.run_guided_array(True, H, 3.0, lxy, rxy, MIN_SCORE, MIN_RATIO)
     dt = (time.perf_counter() - t0) / steps
     m.close()
-    return {"workload": f"{N_FEAT}x{N_FEAT} pair, homography mask, 3 px", "ms_per_call_host_inclusive": dt * 1e3,
+    return {"workload": f"{N_PAIR}x{N_PAIR} pair, homography mask, 3 px", "ms_per_call_host_inclusive": dt * 1e3,
             "calls_per_s": 1.0 / dt, "putative_matches": int(len(put)), "guided_matches": len(g)}
 
 
@@ -579,6 +579,10 @@ def main():
 
     batch, cpu = bench_batch(args)
     lf = batch["learned_shortlist"]
+    log(f"bench: batch ms per batch plain {batch['plain_fp32']['ms_per_batch']:.3f} bf16 "
+        f"{batch['bf16_shortlist']['ms_per_batch']:.3f} learned {lf['ms_per_batch']:.3f} "
+        f"(x{lf['speedup_vs_plain']:.2f}, recall {lf['recall']:.5f}, fallback pairs "
+        f"{lf['sampled_check']['fallback_pairs']})")
     ops = lf["ops_per_launch"]
     achieved = ops / (lf["kernel_ms"] * 1e-3)
     out = {
