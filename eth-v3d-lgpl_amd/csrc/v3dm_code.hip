@@ -7,9 +7,10 @@
 // and element r, key = 16 * sum_k q_c q_q + r: the score with the candidate's row inside
 // the block in its low 4 bits (row 8*(r/4) + 4*(l/32) + r%4, query l%32).  Keys order as
 // (score, row), so the best of a set of candidates is a chain of v_max3_i32.  Each lane
-// keeps the best key of two disjoint candidate groups (elements r < 8 and r >= 8 of every
-// block), so a partition (split, lane half) lists two candidates, one per group; the block
-// of each kept key is tracked once per block.
+// keeps the best key of two disjoint candidate groups: group g takes the elements with
+// r / 8 == g of a tile's first candidate block and those with r / 8 == 1 - g of its second,
+// so a key's bit 3 tells the block (g ^ bit 3) and the tile of each kept key is tracked
+// once per tile.  A partition (split, lane half) lists two candidates, one per group.
 #include "v3dm_batch.h"
 #include "v3dm_device.h"
 #include "v3dm_learned.h"
@@ -25,30 +26,40 @@ constexpr int kCT = 64;    // candidates per tile (2 blocks of 32)
 constexpr int kCodeBytes = 32;
 
 struct KeyStateI {
-    int k[2];    // best key of the lane's two candidate groups (elements r < 8, r >= 8)
-    int blk[2];  // block (32-candidate) of each best key
+    int k[2];  // best key of the lane's two candidate groups
+    int t[2];  // tile of each best key
 };
 
-// One 32x32 block of keys (16 per lane) into the two running maxima; blk = block index.
+// The two candidate blocks of tile t against one query block (d[cb], 16 keys per lane) into
+// the two running maxima: 16 keys and the old maximum per group, eight v_max3_i32.
 template <bool kFull>
-__device__ __forceinline__ void key_block_update(KeyStateI& s, v16i const& d, int blk, int jBase, int nC, int h)
+__device__ __forceinline__ void key_tile_update(KeyStateI& s, v16i const (&d)[2], int t, int jBase, int nC, int h)
 {
 #pragma unroll
     for (int g = 0; g < 2; ++g) {
         int const o = s.k[g];
-        int x[8];
+        int x[16];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            int const rr = 8 * g + r;
-            x[r] = d[rr];
-            if (!kFull) x[r] = (jBase + 8 * (rr >> 2) + 4 * h + (rr & 3) < nC) ? x[r] : INT_MIN;
-        }
-        // a tree of v_max3_i32 (the keys order by score, then row)
-        int const a = max(max(x[0], x[1]), x[2]);
-        int const b = max(max(x[3], x[4]), x[5]);
-        int const c = max(max(x[6], x[7]), o);
-        int const k = max(max(a, b), c);
-        s.blk[g] = (k != o) ? blk : s.blk[g];
+        for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                int const rr = 8 * (g ^ cb) + r;
+                int v = d[cb][rr];
+                if (!kFull) v = (jBase + 32 * cb + 8 * (rr >> 2) + 4 * h + (rr & 3) < nC) ? v : INT_MIN;
+                x[8 * cb + r] = v;
+            }
+        int const a0 = max(max(x[0], x[1]), x[2]);
+        int const a1 = max(max(x[3], x[4]), x[5]);
+        int const a2 = max(max(x[6], x[7]), x[8]);
+        int const a3 = max(max(x[9], x[10]), x[11]);
+        int const a4 = max(max(x[12], x[13]), x[14]);
+        int const b0 = max(max(a0, a1), a2);
+        int const b1 = max(max(a3, a4), x[15]);
+        int k = max(max(b0, b1), o);
+        // opaque to the optimiser: the test below stays k != o (one compare) instead of
+        // being rewritten as max(b0, b1) > o, which costs a separate max for k
+        __asm__ volatile("" : "+v"(k));
+        s.t[g] = (k != o) ? t : s.t[g];
         s.k[g] = k;
     }
 }
@@ -63,20 +74,17 @@ __device__ __forceinline__ void load_code_tile(signed char const* __restrict__ C
     for (int cb = 0; cb < 2; ++cb) a[cb] = *reinterpret_cast<v4i const*>(tp + lo + cb * 32 * kCodeBytes);
 }
 
-// The four 32x32 blocks of a tile (2 candidate blocks x 2 query blocks), one candidate
-// block at a time (two accumulators live: fewer VGPRs, six waves per SIMD).
+// The four 32x32 blocks of a tile, one query block at a time (two accumulators live).
 template <bool kFull>
 __device__ __forceinline__ void code_tile(v4i const (&a)[2], int t, v4i const (&bq)[2], v16i const& pos,
                                           KeyStateI (&st)[2], int nC, int h)
 {
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb) {
+    for (int qb = 0; qb < 2; ++qb) {
         v16i d[2];
 #pragma unroll
-        for (int qb = 0; qb < 2; ++qb) d[qb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[cb], bq[qb], pos, 0, 0, 0);
-        int const blk = 2 * t + cb;
-#pragma unroll
-        for (int qb = 0; qb < 2; ++qb) key_block_update<kFull>(st[qb], d[qb], blk, blk * 32, nC, h);
+        for (int cb = 0; cb < 2; ++cb) d[cb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[cb], bq[qb], pos, 0, 0, 0);
+        key_tile_update<kFull>(st[qb], d, t, t * kCT, nC, h);
     }
 }
 
@@ -118,24 +126,33 @@ __device__ __forceinline__ void code_block(signed char const* __restrict__ Q, in
     // is clamped to the split's last tile, so every load stays in range)
     v4i a[2], an[2];
     load_code_tile(C, t0, lo, a);
-    for (int t = t0; t < tFull; ++t) {
+    int t = t0;
+    for (; t + 1 < tFull; t += 2) {  // two tiles a round: the buffers alternate, no copies
+        load_code_tile(C, t + 1, lo, an);
+        code_tile<true>(a, t, bq, pos, st, nC, h);
+        load_code_tile(C, min(t + 2, t1 - 1), lo, a);
+        code_tile<true>(an, t + 1, bq, pos, st, nC, h);
+    }
+    // `a` holds tile t here
+    if (t < tFull) {
         load_code_tile(C, min(t + 1, t1 - 1), lo, an);
         code_tile<true>(a, t, bq, pos, st, nC, h);
-        a[0] = an[0];
-        a[1] = an[1];
+        if (tFull < t1) code_tile<false>(an, tFull, bq, pos, st, nC, h);
+    } else if (tFull < t1) {
+        code_tile<false>(a, tFull, bq, pos, st, nC, h);
     }
-    if (tFull < t1) code_tile<false>(a, tFull, bq, pos, st, nC, h);
 
 #pragma unroll
     for (int qb = 0; qb < 2; ++qb) {
         int const q = q0 + qb * 32 + c32;
         if (q >= nQ) continue;
-        // the list holds the best candidate of each group (two disjoint candidate sets)
+        // the list holds the best candidate of each group (two disjoint candidate sets):
+        // element r of block 2 t + ((r >> 3) ^ g)
         int const r1 = st[qb].k[0] & 15, r2 = st[qb].k[1] & 15;
-        bool const ok1 = st[qb].blk[0] >= 0 && st[qb].k[0] != INT_MIN;
-        bool const ok2 = st[qb].blk[1] >= 0 && st[qb].k[1] != INT_MIN;
-        int const j1 = st[qb].blk[0] * 32 + 8 * (r1 >> 2) + 4 * h + (r1 & 3);
-        int const j2 = st[qb].blk[1] * 32 + 8 * (r2 >> 2) + 4 * h + (r2 & 3);
+        bool const ok1 = st[qb].t[0] >= 0 && st[qb].k[0] != INT_MIN;
+        bool const ok2 = st[qb].t[1] >= 0 && st[qb].k[1] != INT_MIN;
+        int const j1 = (2 * st[qb].t[0] + (r1 >> 3)) * 32 + 8 * (r1 >> 2) + 4 * h + (r1 & 3);
+        int const j2 = (2 * st[qb].t[1] + ((r2 >> 3) ^ 1)) * 32 + 8 * (r2 >> 2) + 4 * h + (r2 & 3);
         if constexpr (sizeof(List) == sizeof(CandP)) {
             int const j0 = t0 * kCT;  // first candidate of the split
             unsigned int const e = (unsigned int)(kPackedEmpty & 0xffff) << 16;

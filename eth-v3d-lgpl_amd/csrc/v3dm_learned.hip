@@ -504,17 +504,29 @@ __global__ __launch_bounds__(256) void k_mlp_adam(float const* __restrict__ part
 }
 
 // ------------------------------------------------------------------------------ inference codes
-// Codes of descriptor rows, persistent over 128-row blocks (grid-stride).  W1 is staged once
-// per work-group in LDS (rows padded to 272 bytes: the 32 rows one A-operand read touches
-// fall on distinct bank groups).  Each wave takes 32 rows:
-//   layer 1, transposed: D1[unit][row] = W1 . x (A = W1 rows from LDS, B = the wave's rows,
-//     converted to bf16 once per block and kept in registers), one 32-unit block at a time;
-//   bias + ReLU in registers; the 16 hidden units a lane holds are the B operand of layer 2
-//     as they are (k order permuted), with W2 read in the same permutation (W2p);
-//   layer 2: D2[code][row]; lane (row, half) holds 16 of the row's codes: bias, norm over
-//     the two halves, quantisation, one 16-byte store (bytes 16*half .. +15 of the row, in
-//     element order: a fixed permutation of the code dimensions, the same for every row).
+// Codes of descriptor rows.  A work-group of kMlpWaves waves stages the network once in LDS
+// and its waves then take 32-row units independently (unit u of the grid-stride sequence):
+//   W1 rows padded to 272 bytes (the 32 rows one A-operand read touches fall on distinct
+//     bank groups); the pad holds the first layer's bias split into bf16 (bh, bh, bl);
+//   W2 in the lane order of the second layer's A operands (W2p), 16 contiguous bytes a lane.
+// Per unit:
+//   layer 1, transposed: D1[unit][row] = W1 . x + s b1 (A = W1 rows, B = the wave's rows in
+//     bf16, kept in registers; the bias as a ninth product whose B operand is (sh, sl, sh):
+//     bh sh + bh sl + bl sh = s b1 to about 2^-16), one 32-unit block at a time;
+//   ReLU on the bf16 values (a packed int16 max); the 16 hidden units a lane holds are the B
+//     operand of layer 2 as they are (k order permuted), with W2 read in the same permutation;
+//   layer 2: D2[code][row] + s b2; lane (row, half) holds 16 of the row's codes: norm over the
+//     two halves, quantisation, one 16-byte store (bytes 16*half .. +15 of the row, in element
+//     order: a fixed permutation of the code dimensions, the same for every row).
+// s is 1 for FP32 rows.  uint8 rows enter as the integers q (exact in bf16) with s = 1/d: the
+// network's output on q d is z = z'/s with z' = W2 relu(W1 q + s b1) + s b2, and the code is
+// z normalised, so z' gives it.
+// The LDS operands of a unit (88 reads of 16 bytes) pass through a ring of registers kRing
+// reads ahead of their MFMA, and the next 32-unit block's first-layer products are issued
+// before the current block's ReLU, so LDS latency and the VALU work overlap the MFMA chain.
 constexpr int kW1Stride = 136;  // bf16 per LDS row of W1 (128 + 8)
+constexpr int kMlpWaves = 8;    // waves per work-group (LDS: 85 KB, one group per CU)
+constexpr int kMlpRing = 8;     // LDS operand reads in flight per wave
 
 // Rows of one feature set: FP32 rows, or uint8 rows with their dequantisation scale
 // (element k = (float)q_k * denom, the FP32 value the batch image holds).
@@ -524,7 +536,7 @@ struct MlpRows {
     float const* denom;
 };
 
-// The inputs of one 128-row block for lane (c32, h): elements 16 ks + 8 h + 0..7, ks < 8.
+// The inputs of one 32-row unit for lane (c32, h): elements 16 ks + 8 h + 0..7, ks < 8.
 template <bool kU8>
 struct MlpRaw;
 template <>
@@ -565,104 +577,185 @@ __device__ __forceinline__ void mlp_convert(MlpRaw<false> const& r, v8bf (&xb)[8
     }
 }
 
+// uint8 rows: the integers themselves (at most 8 significant bits: exact in bf16)
 __device__ __forceinline__ void mlp_convert(MlpRaw<true> const& r, v8bf (&xb)[8])
 {
 #pragma unroll
     for (int ks = 0; ks < 8; ++ks) {
         unsigned int const a = r.v[ks].x, b = r.v[ks].y;
-        xb[ks] = v8bf{(__bf16)dequant_u8_value(a & 0xffu, r.d), (__bf16)dequant_u8_value((a >> 8) & 0xffu, r.d),
-                      (__bf16)dequant_u8_value((a >> 16) & 0xffu, r.d), (__bf16)dequant_u8_value(a >> 24, r.d),
-                      (__bf16)dequant_u8_value(b & 0xffu, r.d), (__bf16)dequant_u8_value((b >> 8) & 0xffu, r.d),
-                      (__bf16)dequant_u8_value((b >> 16) & 0xffu, r.d), (__bf16)dequant_u8_value(b >> 24, r.d)};
+        xb[ks] = v8bf{(__bf16)(float)(a & 0xffu), (__bf16)(float)((a >> 8) & 0xffu),
+                      (__bf16)(float)((a >> 16) & 0xffu), (__bf16)(float)(a >> 24),
+                      (__bf16)(float)(b & 0xffu), (__bf16)(float)((b >> 8) & 0xffu),
+                      (__bf16)(float)((b >> 16) & 0xffu), (__bf16)(float)(b >> 24)};
     }
 }
 
-// kPrefetch: the next block's inputs are loaded while the current block is computed (the
+// s of the header: 1 for FP32 rows, 1/d for uint8 rows (0 for an all-zero row, d = inf)
+__device__ __forceinline__ float mlp_bias_scale(MlpRaw<false> const&) { return 1.0f; }
+__device__ __forceinline__ float mlp_bias_scale(MlpRaw<true> const& r) { return 1.0f / r.d; }
+
+typedef short v2s16 __attribute__((ext_vector_type(2)));
+typedef __bf16 v2bf16 __attribute__((ext_vector_type(2)));
+typedef float v2f32 __attribute__((ext_vector_type(2)));
+
+// max(x, 0) of two values rounded to bf16 (one v_cvt_pk_bf16_f32): a bf16 is negative
+// exactly when its bits are negative as int16, so one packed int16 max clears both lanes
+__device__ __forceinline__ unsigned relu_bf16x2(float x0, float x1)
+{
+    v2bf16 const p = __builtin_convertvector((v2f32){x0, x1}, v2bf16);
+    return __builtin_bit_cast(unsigned, __builtin_elementwise_max(__builtin_bit_cast(v2s16, p), (v2s16){0, 0}));
+}
+
+// LDS operand t of a unit, t < 88, in the order the unit uses them: for block ub, its nine
+// layer-1 operands (W1 ks = 0..7, the bias column) are read at 9 ub + j (ub = 0..7), its two
+// layer-2 operands at 72 + 2 ub + s
+__device__ __forceinline__ __bf16 const* mlp_operand(__bf16 const* W1s, __bf16 const* W2s, int t, int c32, int h,
+                                                     int lane)
+{
+    if (t < 72) {
+        int const ub = t / 9, j = t % 9;
+        return W1s + (ub * 32 + c32) * kW1Stride + (j < 8 ? 16 * j + 8 * h : 128);
+    }
+    return W2s + ((t - 72) * 64 + lane) * 8;
+}
+
+// kPrefetch: the next unit's inputs are loaded while the current unit is computed (the
 // uint8 rows take 17 registers, few enough to hold a second set).
 template <bool kU8, bool kPrefetch>
-__device__ __forceinline__ void mlp_codes_body(MlpRows const& in0, int nBlocks0, signed char* __restrict__ codes0,
-                                               MlpRows const& in1, int nBlocks1, signed char* __restrict__ codes1,
+__device__ __forceinline__ void mlp_codes_body(MlpRows const& in0, int nUnits0, signed char* __restrict__ codes0,
+                                               MlpRows const& in1, int nUnits1, signed char* __restrict__ codes1,
                                                __bf16 const* __restrict__ W1b, float const* __restrict__ b1,
                                                __bf16 const* __restrict__ W2p, float const* __restrict__ b2,
                                                float scale)
 {
     __shared__ __attribute__((aligned(16))) __bf16 W1s[kHid * kW1Stride];
-    __shared__ __attribute__((aligned(16))) float b1s[kHid];
+    __shared__ __attribute__((aligned(16))) __bf16 W2s[kHid * kCode];
     int const tid = threadIdx.x;
     int const lane = tid & 63;
     int const wave = tid >> 6;
     int const h = lane >> 5;
     int const c32 = lane & 31;
-    int const nBlocks = nBlocks0 + nBlocks1;
-    // block blk -> (feature set, row of this lane)
-    auto rowOf = [&](int blk) -> size_t {
-        return (size_t)((blk >= nBlocks0 ? blk - nBlocks0 : blk) * 128 + wave * 32 + c32);
-    };
+    int const nUnits = nUnits0 + nUnits1;
+    int const stride = (int)gridDim.x * kMlpWaves;
+    // unit -> row of this lane in its feature set
+    auto rowOf = [&](int u) -> size_t { return (size_t)((u >= nUnits0 ? u - nUnits0 : u) * 32 + c32); };
+    int unit = (int)blockIdx.x * kMlpWaves + wave;
     MlpRaw<kU8> raw;
-    if (kPrefetch && (int)blockIdx.x < nBlocks)
-        mlp_load((int)blockIdx.x >= nBlocks0 ? in1 : in0, rowOf(blockIdx.x), h, raw);
-    // W1 -> LDS: 256 rows x 16 chunks of 16 bytes; all sixteen loads of a thread are issued
-    // before the first store, so their latencies overlap
+    if (kPrefetch && unit < nUnits) mlp_load(unit >= nUnits0 ? in1 : in0, rowOf(unit), h, raw);
+    // the network -> LDS; all loads of a thread are issued before its first store
     {
-        uint4 w[kHid * 16 / 256];
+        constexpr int kW1Chunks = kHid * 16 / (64 * kMlpWaves), kW2Chunks = kHid * kCode / 8 / (64 * kMlpWaves);
+        uint4 w[kW1Chunks], w2[kW2Chunks];
 #pragma unroll
-        for (int i = 0; i < kHid * 16 / 256; ++i) {
-            int const f = tid + 256 * i;
+        for (int i = 0; i < kW1Chunks; ++i) {
+            int const f = tid + 64 * kMlpWaves * i;
             w[i] = reinterpret_cast<uint4 const*>(W1b + (size_t)(f >> 4) * 128)[f & 15];
         }
 #pragma unroll
-        for (int i = 0; i < kHid * 16 / 256; ++i) {
-            int const f = tid + 256 * i;
+        for (int i = 0; i < kW2Chunks; ++i) w2[i] = reinterpret_cast<uint4 const*>(W2p)[tid + 64 * kMlpWaves * i];
+        float const bv = tid < kHid ? b1[tid] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < kW1Chunks; ++i) {
+            int const f = tid + 64 * kMlpWaves * i;
             *reinterpret_cast<uint4*>(&W1s[(f >> 4) * kW1Stride + (f & 15) * 8]) = w[i];
         }
+#pragma unroll
+        for (int i = 0; i < kW2Chunks; ++i) reinterpret_cast<uint4*>(W2s)[tid + 64 * kMlpWaves * i] = w2[i];
+        if (tid < kHid) {
+            __bf16 const bh = (__bf16)bv;
+            __bf16 const bl = (__bf16)(bv - (float)bh);
+            __bf16 const zz = (__bf16)0.0f;
+            *reinterpret_cast<v8bf*>(&W1s[tid * kW1Stride + 128]) = v8bf{bh, bh, bl, zz, zz, zz, zz, zz};
+        }
     }
-    b1s[tid] = b1[tid];
     __syncthreads();
     float bz[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) bz[r] = b2[8 * (r >> 2) + 4 * h + (r & 3)];
 
-    // row blocks of the first feature set, then of the second
-    for (int blk = blockIdx.x; blk < nBlocks; blk += gridDim.x) {
-        bool const second = blk >= nBlocks0;
+    for (; unit < nUnits; unit += stride) {
+        bool const second = unit >= nUnits0;
         signed char* __restrict__ codes = second ? codes1 : codes0;
-        size_t const row = rowOf(blk);
+        size_t const row = rowOf(unit);
         if (!kPrefetch) mlp_load(second ? in1 : in0, row, h, raw);
         v8bf xb[8];
         mlp_convert(raw, xb);
+        float const sc = mlp_bias_scale(raw);
+        __bf16 const sh = (__bf16)sc;
+        __bf16 const sl = (__bf16)(sc - (float)sh);
+        __bf16 const zz = (__bf16)0.0f;
+        v8bf const xbias = h == 0 ? v8bf{sh, sl, sh, zz, zz, zz, zz, zz} : v8bf{zz, zz, zz, zz, zz, zz, zz, zz};
         if (kPrefetch) {
-            int const nxt = blk + (int)gridDim.x;
-            if (nxt < nBlocks) mlp_load(nxt >= nBlocks0 ? in1 : in0, rowOf(nxt), h, raw);
+            int const nxt = unit + stride;
+            if (nxt < nUnits) mlp_load(nxt >= nUnits0 ? in1 : in0, rowOf(nxt), h, raw);
         }
+        // LDS operand ring: operand i (in order of use) sits in ring[i % kMlpRing] from its
+        // read, issued kMlpRing operands ahead, until its MFMA.  Order of use: L1(0), then
+        // per ub: L1(ub + 1) (ub < 7), L2(ub).  Each step (one MFMA, the read it frees room
+        // for, a share of the ReLU work) is closed by a scheduling barrier, so the compiler
+        // keeps this interleaving instead of hoisting every read to the top.
+        v8bf ring[kMlpRing];
+        auto order = [](int i) -> int {  // i-th operand used -> its LDS operand index
+            if (i < 9) return i;
+            int const k = i - 9, ub = k / 11, j = k % 11;
+            if (ub < 7) return j < 9 ? 9 * (ub + 1) + j : 72 + 2 * ub + (j - 9);
+            return 72 + 14 + (k - 77);
+        };
+#pragma unroll
+        for (int i = 0; i < kMlpRing; ++i) ring[i] = load8(mlp_operand(W1s, W2s, order(i), c32, h, lane));
+        int used = 0;
+        auto next = [&]() -> v8bf {
+            v8bf const op = ring[used % kMlpRing];
+            if (used + kMlpRing < 88)
+                ring[used % kMlpRing] = load8(mlp_operand(W1s, W2s, order(used + kMlpRing), c32, h, lane));
+            ++used;
+            return op;
+        };
+        __builtin_amdgcn_sched_barrier(0);
         v16f z = v16f{};
-#pragma unroll 2
+        v16f a = v16f{};
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            v8bf const op = next();
+            a = mfma_bf16(op, j < 8 ? xb[j] : xbias, a);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
         for (int ub = 0; ub < 8; ++ub) {
-            v16f a = v16f{};
-            __bf16 const* wrow = &W1s[(ub * 32 + c32) * kW1Stride + 8 * h];
+            unsigned w[8];
+            if (ub < 7) {
+                v16f an = v16f{};
 #pragma unroll
-            for (int ks = 0; ks < 8; ++ks)
-                a = mfma_bf16(*reinterpret_cast<v8bf const*>(wrow + 16 * ks), xb[ks], a);
-            float4 const* bp = reinterpret_cast<float4 const*>(&b1s[ub * 32 + 4 * h]);
-            v8bf hb[2];
+                for (int j = 0; j < 9; ++j) {
+                    v8bf const op = next();
+                    an = mfma_bf16(op, j < 8 ? xb[j] : xbias, an);
+                    if (j < 8) w[j] = relu_bf16x2(a[2 * j], a[2 * j + 1]);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                v8bf const hb0 = __builtin_bit_cast(v8bf, (uint4){w[0], w[1], w[2], w[3]});
+                v8bf const hb1 = __builtin_bit_cast(v8bf, (uint4){w[4], w[5], w[6], w[7]});
+                z = mfma_bf16(next(), hb0, z);
+                __builtin_amdgcn_sched_barrier(0);
+                z = mfma_bf16(next(), hb1, z);
+                __builtin_amdgcn_sched_barrier(0);
+                a = an;
+            } else {
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                float4 const bb = bp[2 * g];  // units ub*32 + 8g + 4h + 0..3
-                hb[g >> 1][4 * (g & 1) + 0] = (__bf16)fmaxf(a[4 * g + 0] + bb.x, 0.0f);
-                hb[g >> 1][4 * (g & 1) + 1] = (__bf16)fmaxf(a[4 * g + 1] + bb.y, 0.0f);
-                hb[g >> 1][4 * (g & 1) + 2] = (__bf16)fmaxf(a[4 * g + 2] + bb.z, 0.0f);
-                hb[g >> 1][4 * (g & 1) + 3] = (__bf16)fmaxf(a[4 * g + 3] + bb.w, 0.0f);
+                for (int g = 0; g < 8; ++g) w[g] = relu_bf16x2(a[2 * g], a[2 * g + 1]);
+                v8bf const hb0 = __builtin_bit_cast(v8bf, (uint4){w[0], w[1], w[2], w[3]});
+                v8bf const hb1 = __builtin_bit_cast(v8bf, (uint4){w[4], w[5], w[6], w[7]});
+                z = mfma_bf16(next(), hb0, z);
+                z = mfma_bf16(next(), hb1, z);
             }
-#pragma unroll
-            for (int s = 0; s < 2; ++s) z = mfma_bf16(load8(W2p + ((ub * 2 + s) * 64 + lane) * 8), hb[s], z);
         }
         float s2 = 0.0f;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            z[r] += bz[r];
+            z[r] = fmaf(bz[r], sc, z[r]);
             s2 += z[r] * z[r];
         }
         s2 += __shfl_xor(s2, 32);
-        float const inv = scale / fmaxf(sqrtf(s2), 1e-12f);
+        float const inv = scale / fmaxf(sqrtf(s2), 1e-30f);
         unsigned int w[4];
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
@@ -679,40 +772,44 @@ __device__ __forceinline__ void mlp_codes_body(MlpRows const& in0, int nBlocks0,
     }
 }
 
-__global__ __launch_bounds__(256, 2) void k_mlp_codes(float const* __restrict__ X0, int nBlocks0,
-                                                      signed char* __restrict__ codes0, float const* __restrict__ X1,
-                                                      int nBlocks1, signed char* __restrict__ codes1,
-                                                      __bf16 const* __restrict__ W1b, float const* __restrict__ b1,
-                                                      __bf16 const* __restrict__ W2p, float const* __restrict__ b2,
-                                                      float scale)
+__global__ __launch_bounds__(64 * kMlpWaves, 1) void k_mlp_codes(
+    float const* __restrict__ X0, int nUnits0, signed char* __restrict__ codes0, float const* __restrict__ X1,
+    int nUnits1, signed char* __restrict__ codes1, __bf16 const* __restrict__ W1b, float const* __restrict__ b1,
+    __bf16 const* __restrict__ W2p, float const* __restrict__ b2, float scale)
 {
-    mlp_codes_body<false, false>(MlpRows{X0, nullptr, nullptr}, nBlocks0, codes0, MlpRows{X1, nullptr, nullptr},
-                                 nBlocks1, codes1, W1b, b1, W2p, b2, scale);
+    mlp_codes_body<false, false>(MlpRows{X0, nullptr, nullptr}, nUnits0, codes0, MlpRows{X1, nullptr, nullptr},
+                                 nUnits1, codes1, W1b, b1, W2p, b2, scale);
 }
 
-// uint8 rows (the batch image): the same values as the FP32 rows they dequantise to, read
-// at 128 bytes per row, the next block's rows loaded during the current block
-__global__ __launch_bounds__(256, 2) void k_mlp_codes_u8(uint8_t const* __restrict__ Q, float const* __restrict__ denom,
-                                                         int nBlocks, signed char* __restrict__ codes,
-                                                         __bf16 const* __restrict__ W1b, float const* __restrict__ b1,
-                                                         __bf16 const* __restrict__ W2p, float const* __restrict__ b2,
-                                                         float scale)
+// uint8 rows (the batch image): read at 128 bytes per row, the next unit's rows loaded during
+// the current unit
+__global__ __launch_bounds__(64 * kMlpWaves, 1) void k_mlp_codes_u8(
+    uint8_t const* __restrict__ Q, float const* __restrict__ denom, int nUnits, signed char* __restrict__ codes,
+    __bf16 const* __restrict__ W1b, float const* __restrict__ b1, __bf16 const* __restrict__ W2p,
+    float const* __restrict__ b2, float scale)
 {
     MlpRows const in{nullptr, Q, denom};
-    mlp_codes_body<true, true>(in, nBlocks, codes, in, 0, codes, W1b, b1, W2p, b2, scale);
+    mlp_codes_body<true, true>(in, nUnits, codes, in, 0, codes, W1b, b1, W2p, b2, scale);
 }
+
+namespace {
+int mlp_grid(int nUnits)
+{
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    int const groups = (nUnits + kMlpWaves - 1) / kMlpWaves;
+    return groups < cus ? groups : cus;
+}
+}  // namespace
 
 void launch_mlp_codes2_f32(float const* X0, int rows0, signed char* codes0, float const* X1, int rows1,
                            signed char* codes1, MlpWeights const& w, float scale, hipStream_t st)
 {
-    int const nb0 = rows0 / 128, nb1 = rows1 / 128;
-    if (nb0 + nb1 <= 0) return;
-    int dev = 0, cus = 256;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    int const grid = nb0 + nb1 < 2 * cus ? nb0 + nb1 : 2 * cus;
-    hipLaunchKernelGGL(k_mlp_codes, dim3(grid), dim3(256), 0, st, X0, nb0, codes0, X1, nb1, codes1, w.W1b,
-                       w.P + kOffB1, w.W2p, w.P + kOffB2, scale);
+    int const nu0 = rows0 / 32, nu1 = rows1 / 32;
+    if (nu0 + nu1 <= 0) return;
+    hipLaunchKernelGGL(k_mlp_codes, dim3(mlp_grid(nu0 + nu1)), dim3(64 * kMlpWaves), 0, st, X0, nu0, codes0, X1, nu1,
+                       codes1, w.W1b, w.P + kOffB1, w.W2p, w.P + kOffB2, scale);
 }
 
 void launch_mlp_codes_f32(float const* X, int rows128, MlpWeights const& w, float scale, signed char* codes,
@@ -724,14 +821,10 @@ void launch_mlp_codes_f32(float const* X, int rows128, MlpWeights const& w, floa
 void launch_mlp_codes_u8(uint8_t const* Q, float const* denom, int rows128, MlpWeights const& w, float scale,
                          signed char* codes, hipStream_t st)
 {
-    int const nb = rows128 / 128;
-    if (nb <= 0) return;
-    int dev = 0, cus = 256;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    int const grid = nb < 2 * cus ? nb : 2 * cus;
-    hipLaunchKernelGGL(k_mlp_codes_u8, dim3(grid), dim3(256), 0, st, Q, denom, nb, codes, w.W1b, w.P + kOffB1, w.W2p,
-                       w.P + kOffB2, scale);
+    int const nu = rows128 / 32;
+    if (nu <= 0) return;
+    hipLaunchKernelGGL(k_mlp_codes_u8, dim3(mlp_grid(nu)), dim3(64 * kMlpWaves), 0, st, Q, denom, nu, codes, w.W1b,
+                       w.P + kOffB1, w.W2p, w.P + kOffB2, scale);
 }
 
 __global__ void k_fill_f32(float* __restrict__ p, size_t n, float v)

@@ -460,6 +460,119 @@ __global__ __launch_bounds__(256, 8) void k_batch_rescore_u8(BatchDev b, Cand2 c
     add_stripe(n, stripes);
 }
 
+// The same for the learned lists (topK in [1, kLaneTopK]) with one lane per query, the
+// work-group's 256 queries one segment (kBatchRowAlign = 256).  Candidate selection in two
+// passes over the query's lists: the first keeps the kLaneTopK largest list values
+// (multiplicity counted) and sets thr = max(m - margin, the topK-th largest), m the
+// largest -- the eight-lane selection's threshold in both of its cases (at most topK
+// values within the margin: m - margin, which is then >= the topK-th largest; else the
+// topK-th largest, which is then >= m - margin); the second scores every listed candidate
+// with value >= thr, as the eight-lane form does: the same candidates, the same integer
+// scores and rounding, the same top-2 order (score, then smaller index) and decision.
+constexpr int kLaneTopK = 8;
+
+__device__ __forceinline__ void topk_insert(float (&t)[kLaneTopK], float x)
+{
+#pragma unroll
+    for (int i = 0; i < kLaneTopK; ++i) {  // t descending: keep the larger, carry the smaller
+        float const hi = fmaxf(t[i], x);
+        x = fminf(t[i], x);
+        t[i] = hi;
+    }
+}
+
+// exact integer dot product of two uint8 rows held as eight uint4
+__device__ __forceinline__ unsigned dot_u8_row(uint4 const (&a)[8], uint4 const (&c)[8])
+{
+    unsigned acc = 0u;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        acc = __builtin_amdgcn_udot4(a[k].x, c[k].x, acc, false);
+        acc = __builtin_amdgcn_udot4(a[k].y, c[k].y, acc, false);
+        acc = __builtin_amdgcn_udot4(a[k].z, c[k].z, acc, false);
+        acc = __builtin_amdgcn_udot4(a[k].w, c[k].w, acc, false);
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(256) void k_batch_rescore_lane(BatchDev b, Cand2 const* __restrict__ part, int topK,
+                                                            float margin, float minScore, float minRatio,
+                                                            bool certify, int* __restrict__ sel,
+                                                            int* __restrict__ open, int* __restrict__ nOpen,
+                                                            int* __restrict__ stripes)
+{
+    static_assert(kBatchRowAlign == 256, "a work-group's queries lie in one segment");
+    int const blk = xcd_block(blockIdx.x, gridDim.x);
+    int const g = blk * 256 + (int)threadIdx.x;
+    int n = 0;
+    if (g < b.qRows) {
+        BatchSeg const sg = b.seg[b.segOfBlock[blk]];
+        int const q = g - sg.qOff;
+        int d = -1;
+        if (q < sg.n && sg.nO > 0 && sg.nParts > 0) {
+            int const row = b.imgBase[sg.img] + q;
+            uint8_t const* __restrict__ cRows = b.u8 + (size_t)b.imgBase[sg.oImg] * kDim;
+            float const* __restrict__ cDen = b.denom + b.imgBase[sg.oImg];
+            uint4 qv[8];
+            if (certify) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) qv[k] = reinterpret_cast<uint4 const*>(b.u8 + (size_t)row * kDim)[k];
+            }
+            float const ra = b.denom[row];
+            Cand2 const* __restrict__ lp = part + ((size_t)sg.partOff + q) * 2;
+            size_t const ld2 = 2 * (size_t)sg.ld;
+            int const nLists = 2 * sg.nParts;
+            float t[kLaneTopK];
+#pragma unroll
+            for (int i = 0; i < kLaneTopK; ++i) t[i] = -INFINITY;
+            for (int e = 0; e < nLists; ++e) {
+                Cand2 const c = lp[(size_t)(e >> 1) * ld2 + (e & 1)];
+                topk_insert(t, c.j1 >= 0 ? c.v1 : -INFINITY);
+                topk_insert(t, c.j2 >= 0 ? c.v2 : -INFINITY);
+            }
+            float kth = t[0];
+#pragma unroll
+            for (int i = 1; i < kLaneTopK; ++i) kth = (i == topK - 1) ? t[i] : kth;
+            float const thr = fmaxf(t[0] - margin, kth);
+            Top2x s = {kInit, kInit, -1, -1};
+            for (int e = 0; e < nLists; ++e) {
+                Cand2 const c = lp[(size_t)(e >> 1) * ld2 + (e & 1)];
+                bool const take1 = c.j1 >= 0 && c.v1 >= thr;
+                bool const take2 = c.j2 >= 0 && c.v2 >= thr;
+                n += (take1 ? 1 : 0) + (take2 ? 1 : 0);
+                if (!certify) continue;
+                uint4 r1[8], r2[8];
+                float rb1 = 0.0f, rb2 = 0.0f;
+                if (take1) {
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) r1[k] = reinterpret_cast<uint4 const*>(cRows + (size_t)c.j1 * kDim)[k];
+                    rb1 = cDen[c.j1];
+                }
+                if (take2) {
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) r2[k] = reinterpret_cast<uint4 const*>(cRows + (size_t)c.j2 * kDim)[k];
+                    rb2 = cDen[c.j2];
+                }
+                if (take1) {
+                    float const sc = mul_nc(mul_nc(ra, rb1), (float)dot_u8_row(qv, r1));
+                    if (sc == sc) s = top2x_merge(s, Top2x{sc, kInit, c.j1, -1});
+                }
+                if (take2) {
+                    float const sc = mul_nc(mul_nc(ra, rb2), (float)dot_u8_row(qv, r2));
+                    if (sc == sc) s = top2x_merge(s, Top2x{sc, kInit, c.j2, -1});
+                }
+            }
+            d = kFlag;
+            if (certify)
+                d = decide_certified(Top2{s.b1, s.b2, s.a1}, kCertU8 * fmaxf(s.b1, 0.0f) + kEpsAbs, minScore,
+                                     minRatio);
+            if (d == kFlag) open[atomicAdd(nOpen, 1)] = g;
+        }
+        sel[g] = d;
+    }
+    add_stripe(n, stripes);
+}
+
 // The open queries of k_batch_rescore_u8, eight lanes each (grid-stride over the list): the
 // same candidate selection, exact scores (the oracle's arithmetic on the FP32 rows).
 __global__ __launch_bounds__(256) void k_batch_rescore_open(BatchDev b, Cand2 const* __restrict__ part, float winRel,
@@ -499,7 +612,12 @@ void launch_batch_rescore(BatchDev const& b, Cand2 const* part, float winRel, in
                           float minRatio, bool certify, int* sel, int* open, int* nOpen, int* stripes, hipStream_t st)
 {
     if (b.qRows <= 0) return;
-    if (b.u8) {
+    if (b.u8 && topK >= 1 && topK <= kLaneTopK) {
+        hipLaunchKernelGGL(k_batch_rescore_lane, dim3((b.qRows + 255) / 256), dim3(256), 0, st, b, part, topK, margin,
+                           minScore, minRatio, certify, sel, open, nOpen, stripes);
+        hipLaunchKernelGGL(k_batch_rescore_open, dim3(256), dim3(256), 0, st, b, part, winRel, topK, margin, minScore,
+                           minRatio, sel, open, nOpen);
+    } else if (b.u8) {
         hipLaunchKernelGGL(k_batch_rescore_u8, dim3((b.qRows + 31) / 32), dim3(256), 0, st, b, part, winRel, topK,
                            margin, minScore, minRatio, certify, sel, open, nOpen, stripes);
         hipLaunchKernelGGL(k_batch_rescore_open, dim3(256), dim3(256), 0, st, b, part, winRel, topK, margin, minScore,
