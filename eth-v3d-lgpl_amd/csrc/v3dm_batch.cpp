@@ -45,6 +45,7 @@ constexpr int kCheckTiles[2] = {8, 32};
 struct Plan {
     std::vector<int4> items;
     std::vector<BatchSeg> seg;
+    std::vector<BatchBlock> blocks;  // shortlist plans: query blocks pair by pair
     size_t partSlots = 0;
     int tiles = 0;  // tiles per item asked for
     int grid = 0;
@@ -98,6 +99,8 @@ struct BatchState {
     int itemCap[4] = {0, 0, 0, 0};
     BatchSeg* segD[4] = {nullptr, nullptr, nullptr, nullptr};
     int segCap[4] = {0, 0, 0, 0};
+    BatchBlock* blkInfo[4] = {nullptr, nullptr, nullptr, nullptr};
+    int blkInfoCap[4] = {0, 0, 0, 0};
     Top2* partF = nullptr;
     size_t partFCap = 0;
     Cand2* partB = nullptr;
@@ -107,6 +110,8 @@ struct BatchState {
     int* sel = nullptr;
     int* flagList = nullptr;
     int* openList = nullptr;  // open decisions of the shortlist rescoring
+    int* selCnt = nullptr;    // learned rescoring: selected candidates per query
+    int* selSlots = nullptr;  // and their indices, 8 per query
     int selCap = 0;
     int* counters = nullptr;  // kBatchCounters ints
     int* stripes = nullptr;   // striped counts, cleared by the gather kernel
@@ -224,6 +229,7 @@ BatchDev dev_view(BatchState* b, int form)
     d.maxNorm = b->maxNorm;
     d.seg = b->segD[form];
     d.segOfBlock = b->segOfBlock;
+    d.blkInfo = b->blkInfo[form];
     d.nPairs = b->nPairs;
     d.qRows = b->qRows;
     return d;
@@ -317,7 +323,9 @@ Plan plan_plain(BatchState* b, int tiles, int grid)
 
 // Shortlist plan: items (segment, query block, split, tiles) with at least `tiles` 64-row
 // candidate tiles per item and at most maxSplits (<= kShortMaxSplits) splits per query.
-Plan plan_short_batch(BatchState* b, int tiles, int grid, int maxSplits)
+// wholeRows: one item per (segment, query block) that runs every split of the row in turn
+// (the learned code pass); else one item per split.
+Plan plan_short_batch(BatchState* b, int tiles, int grid, int maxSplits, bool wholeRows)
 {
     Plan pl;
     pl.tiles = tiles;
@@ -340,17 +348,44 @@ Plan plan_short_batch(BatchState* b, int tiles, int grid, int maxSplits)
         s.partOff = (int)off;
         off += (size_t)splits * s.ld;
         int const qBlocks = (s.n + 255) / 256;
-        for (int qb = 0; qb < qBlocks; ++qb)
+        for (int qb = 0; qb < qBlocks; ++qb) {
+            if (wholeRows) {
+                work.push_back({nTiles, make_int4(si, qb, 0, tps)});
+                continue;
+            }
             for (int sp = 0; sp < splits; ++sp) {
                 int const cost = std::min(tps, nTiles - sp * tps);
                 work.push_back({cost, make_int4(si, qb, sp, tps)});
             }
+        }
     }
     std::stable_sort(work.begin(), work.end(),
                      [](std::pair<int, int4> const& x, std::pair<int, int4> const& y) { return x.first > y.first; });
     pl.items.reserve(work.size());
     for (auto& w : work) pl.items.push_back(w.second);
     pl.partSlots = off;
+    // the query blocks pair by pair, both sides of a pair next to each other: the learned
+    // rescoring's queries of one side are the other side's candidates, so a pair's rows are
+    // fetched into an XCD's L2 once
+    int const P = b->nPairs;
+    for (int p = 0; p < P; ++p)
+        for (int side = 0; side < 2; ++side) {
+            BatchSeg const& sg = pl.seg[side * P + p];
+            int const nb = roundUp(std::max(sg.n, 1), kBatchRowAlign) / kBatchRowAlign;
+            for (int k = 0; k < nb; ++k) {
+                BatchBlock bb{};
+                bb.blk = sg.qOff / kBatchRowAlign + k;
+                bb.q0 = k * kBatchRowAlign;
+                bb.n = sg.n;
+                bb.nO = sg.nO;
+                bb.nParts = sg.nParts;
+                bb.partOff = sg.partOff;
+                bb.ld = sg.ld;
+                bb.rowBase = b->imgBaseH[sg.img] + bb.q0;
+                bb.cBase = b->imgBaseH[sg.oImg];
+                pl.blocks.push_back(bb);
+            }
+        }
     return pl;
 }
 
@@ -359,6 +394,7 @@ int upload_plan(v3dm_context* h, BatchState* b, int form, Plan const& pl)
     int rc;
     if ((rc = growi(h, b->items[form], b->itemCap[form], pl.items.size()))) return rc;
     if ((rc = growi(h, b->segD[form], b->segCap[form], pl.seg.size()))) return rc;
+    if ((rc = growi(h, b->blkInfo[form], b->blkInfoCap[form], pl.blocks.size()))) return rc;
     if (form == 0 || form == 3) {
         if ((rc = grow(h, b->partF, b->partFCap, pl.partSlots))) return rc;
     } else {
@@ -370,6 +406,9 @@ int upload_plan(v3dm_context* h, BatchState* b, int form, Plan const& pl)
                                  hipMemcpyHostToDevice, st));
     if (!pl.seg.empty())
         BCHECK(h, hipMemcpyAsync(b->segD[form], pl.seg.data(), pl.seg.size() * sizeof(BatchSeg),
+                                 hipMemcpyHostToDevice, st));
+    if (!pl.blocks.empty())
+        BCHECK(h, hipMemcpyAsync(b->blkInfo[form], pl.blocks.data(), pl.blocks.size() * sizeof(BatchBlock),
                                  hipMemcpyHostToDevice, st));
     BCHECK(h, hipStreamSynchronize(st));  // host vectors may change after return
     return V3DM_OK;
@@ -496,9 +535,13 @@ int ensure_query_space(v3dm_context* h, BatchState* b)
     if ((rc = growi(h, b->sel, b->selCap, b->qRows))) return rc;
     if (b->flagList) (void)hipFree(b->flagList);
     if (b->openList) (void)hipFree(b->openList);
-    b->flagList = b->openList = nullptr;
+    if (b->selCnt) (void)hipFree(b->selCnt);
+    if (b->selSlots) (void)hipFree(b->selSlots);
+    b->flagList = b->openList = b->selCnt = b->selSlots = nullptr;
     BCHECK(h, hipMalloc(&b->flagList, (size_t)std::max(b->qRows, 1) * sizeof(int)));
     BCHECK(h, hipMalloc(&b->openList, (size_t)std::max(b->qRows, 1) * sizeof(int)));
+    BCHECK(h, hipMalloc(&b->selCnt, (size_t)std::max(b->qRows, 1) * sizeof(int)));
+    BCHECK(h, hipMalloc(&b->selSlots, (size_t)std::max(b->qRows, 1) * 8 * sizeof(int)));
     int const nbMutual = (b->leftRows + 1023) / 1024;
     if (b->tmp) (void)hipFree(b->tmp);
     if (b->blockCount) (void)hipFree(b->blockCount);
@@ -523,9 +566,10 @@ int ensure_query_space(v3dm_context* h, BatchState* b)
         b->hostResCap = kBatchCounters + std::max(b->nPairs, 1);
         BCHECK(h, hipHostMalloc(&b->hostRes, b->hostResCap * sizeof(int), hipHostMallocDefault));
     }
-    if (nBlk > 0)
+    if (nBlk > 0) {
         BCHECK(h, hipMemcpyAsync(b->segOfBlock, b->segOfBlockH.data(), nBlk * sizeof(int), hipMemcpyHostToDevice,
                                  h->stream));
+    }
     BCHECK(h, hipStreamSynchronize(h->stream));
     return V3DM_OK;
 }
@@ -562,11 +606,12 @@ int enqueue_run(v3dm_context* h, BatchState* b, int mode, float minScore, float 
     } else if (form == 1) {
         launch_batch_pass_bf16(d, b->items[1], (int)pl.items.size(), b->counters + 1, b->partB, pl.grid, st);
         launch_batch_rescore(d, b->partB, kWinRelBf16, 0, INFINITY, minScore, minRatio, h->epsRel > 0.0f, b->sel,
-                             b->openList, b->counters + kBCntOpenRescore, b->stripes, st);
+                             b->openList, b->counters + kBCntOpenRescore, nullptr, nullptr, b->stripes, st);
     } else {
         launch_batch_pass_code(d, b->items[2], (int)pl.items.size(), b->counters + 2, b->partB, pl.grid, st);
         launch_batch_rescore(d, b->partB, 0.0f, h->learnedTopK, learned_margin(h), minScore, minRatio,
-                             h->epsRel > 0.0f, b->sel, b->openList, b->counters + kBCntOpenRescore, b->stripes, st);
+                             h->epsRel > 0.0f, b->sel, b->openList, b->counters + kBCntOpenRescore, b->selCnt,
+                             b->selSlots, b->stripes, st);
         if (h->fallbackSamples > 0) {
             int dev = 0, cus = 256;
             (void)hipGetDevice(&dev);
@@ -649,12 +694,17 @@ int ensure_plan(v3dm_context* h, BatchState* b, int form)
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     auto make = [&](int tiles, int grid) {
-        return form == 0 ? plan_plain(b, tiles, grid) : plan_short_batch(b, tiles, grid, form == 2 ? 8 : 16);
+        return form == 0 ? plan_plain(b, tiles, grid)
+                         : plan_short_batch(b, form == 2 ? kCodeSplitTiles : tiles, grid, form == 2 ? 8 : 16, form == 2);
     };
-    std::vector<int> tileCands = form == 0 ? std::vector<int>{8, 16, 32, 64} : std::vector<int>{4, 8, 16, 32};
+    // the learned lists' split size is fixed (the result does not depend on tuning): only the
+    // grid is tuned for the code pass
+    std::vector<int> tileCands = form == 0   ? std::vector<int>{8, 16, 32, 64}
+                                 : form == 1 ? std::vector<int>{4, 8, 16, 32}
+                                             : std::vector<int>{kCodeSplitTiles};
     // the code pass runs 4 work-groups per CU, the others 2
     std::vector<int> gridCands = form == 2 ? std::vector<int>{2 * cus, 4 * cus} : std::vector<int>{cus, 2 * cus};
-    int bestT = form == 0 ? 16 : 8, bestG = 2 * cus;
+    int bestT = form == 0 ? 16 : form == 1 ? 8 : kCodeSplitTiles, bestG = 2 * cus;
     float bestMs = -1.0f;
     if (b->forceTiles[form] > 0 || b->forceGrid[form] > 0) {
         bestT = b->forceTiles[form] > 0 ? b->forceTiles[form] : bestT;
@@ -859,8 +909,9 @@ void batch_destroy(BatchState*& b)
     for (void* p : {(void*)b->img, (void*)b->imgB, (void*)b->codes, (void*)b->imgU8, (void*)b->denom, (void*)b->imgBase, (void*)b->blkImg, (void*)b->nImgD,
                     (void*)b->srcRow, (void*)b->norm, (void*)b->maxNorm, b->upload, (void*)b->segOfBlock,
                     (void*)b->items[0], (void*)b->items[1], (void*)b->items[2], (void*)b->items[3],
-                    (void*)b->segD[0], (void*)b->segD[1], (void*)b->segD[2], (void*)b->segD[3], (void*)b->partF,
-                    (void*)b->partB, (void*)b->sel, (void*)b->flagList, (void*)b->openList, (void*)b->sRows, (void*)b->sNorm, (void*)b->sIdx, (void*)b->sCount,
+                    (void*)b->segD[0], (void*)b->segD[1], (void*)b->segD[2], (void*)b->segD[3], (void*)b->blkInfo[0],
+                    (void*)b->blkInfo[1], (void*)b->blkInfo[2], (void*)b->blkInfo[3], (void*)b->partF,
+                    (void*)b->partB, (void*)b->sel, (void*)b->flagList, (void*)b->openList, (void*)b->selCnt, (void*)b->selSlots, (void*)b->sRows, (void*)b->sNorm, (void*)b->sIdx, (void*)b->sCount,
                     (void*)b->sNSplits, (void*)b->sPart, (void*)b->sItems, (void*)b->pairFlag, (void*)b->fbItems,
                     (void*)b->segMask, (void*)b->rowCS, (void*)b->cItems[0],
                     (void*)b->cItems[1], (void*)b->cNSplits, (void*)b->cPart, (void*)b->cQB, (void*)b->cSlotB,

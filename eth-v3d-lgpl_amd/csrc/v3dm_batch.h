@@ -20,6 +20,7 @@
 namespace v3dm {
 
 constexpr int kBatchRowAlign = 256;
+constexpr int kCodeSplitTiles = 32;  // learned lists: candidates per split, in 64-row tiles
 constexpr int kBatchQueues = 4;  // work counters: 0 plain FP32 pass, 1 BF16 / learned pass
 // Counters of a batch (an int array of kBatchCounters): queue heads, open-query count,
 // rescored-candidate count, totals published by the gather kernel.
@@ -47,13 +48,25 @@ struct BatchSeg {
 };
 
 // Plain FP32 item: x = pair, y = left block (128 rows), z = first tile, w = last tile (excl.)
-// Shortlist item:  x = segment, y = query block (256 rows), z = first tile, w = last tile
-// (64-row tiles of candidates in both forms).
+// Shortlist item:  x = segment, y = query block (256 rows), z = split, w = tiles per split
+// (64-row tiles of candidates); the learned code pass runs splits z .. nParts - 1.
 
 // Misses the sampled check tolerates per number of exactly accepted samples (passed by
 // value: allowed[a] = v3dm_fallback_allowed_misses(a, minRecall)).
 struct MissLimits {
     unsigned char allowed[kSampleRows + 1];
+};
+
+// One 256-row query block of a shortlist plan, in pair-by-pair order (both sides of a pair
+// in turn): everything the learned rescoring needs in one scalar load.
+struct BatchBlock {
+    int blk;      // the block: queries blk * 256 ..
+    int q0;       // first query of the block within its segment
+    int n, nO;    // the segment's queries and candidates
+    int nParts, partOff, ld;
+    int rowBase;  // image row of query q0
+    int cBase;    // image row of candidate 0
+    int pad0, pad1, pad2;
 };
 
 struct BatchDev {
@@ -67,6 +80,7 @@ struct BatchDev {
     unsigned int const* maxNorm;  // per image (float bits)
     BatchSeg const* seg;
     int const* segOfBlock;
+    BatchBlock const* blkInfo;    // shortlist plans: the 256-row query blocks, pair by pair
     int nPairs;
     int qRows;                    // size of the query space
 };
@@ -108,9 +122,11 @@ void launch_batch_pass_bf16(BatchDev const& b, int4 const* items, int nItems, in
                             hipStream_t st);
 // uint8 images: certified integer scores, the open decisions (listed in `open`, count in
 // *nOpen, which must be zero) scored exactly (certify = false: every decision scored
-// exactly); FP32 images: exact scores
+// exactly); FP32 images: exact scores.  selCnt / selSlots (qRows and 8 qRows ints, or null):
+// work buffers of the learned lists' two-kernel rescoring.
 void launch_batch_rescore(BatchDev const& b, Cand2 const* part, float winRel, int topK, float margin, float minScore,
-                          float minRatio, bool certify, int* sel, int* open, int* nOpen, int* stripes, hipStream_t st);
+                          float minRatio, bool certify, int* sel, int* open, int* nOpen, int* selCnt, int* selSlots,
+                          int* stripes, hipStream_t st);
 // learned code pass (v3dm_code.hip)
 void launch_batch_pass_code(BatchDev const& b, int4 const* items, int nItems, int* queue, Cand2* part, int grid,
                             hipStream_t st);

@@ -191,7 +191,8 @@ __global__ __launch_bounds__(256, 4) void k_pass_code2(CodeDir d0, CodeDir d1)
     code_block(d.Q, d.nQ, d.C, d.nC, b - split * d.queryBlocks, split, d.tilesPerSplit, d.part, d.ldq);
 }
 
-// Persistent form over a batch: item = (segment, query block, split, tiles per split).
+// Persistent form over a batch: item = (segment, query block, first split, tiles per split);
+// a work-group runs the splits of the item's row from the first one on.
 __global__ __launch_bounds__(256, 4) void k_batch_pass_code(BatchDev b, int4 const* __restrict__ items, int nItems,
                                                             int* __restrict__ queue, Cand2* __restrict__ part)
 {
@@ -206,7 +207,8 @@ __global__ __launch_bounds__(256, 4) void k_batch_pass_code(BatchDev b, int4 con
         BatchSeg const sg = b.seg[w.x];
         signed char const* Q = b.codes + (size_t)b.imgBase[sg.img] * kCodeBytes;
         signed char const* C = b.codes + (size_t)b.imgBase[sg.oImg] * kCodeBytes;
-        code_block(Q, sg.n, C, sg.nO, w.y, w.z, w.w, part + (size_t)sg.partOff * 2, sg.ld);
+        for (int sp = w.z; sp < sg.nParts; ++sp)  // the item: every split of its row, in turn
+            code_block(Q, sg.n, C, sg.nO, w.y, sp, w.w, part + (size_t)sg.partOff * 2, sg.ld);
     }
 }
 

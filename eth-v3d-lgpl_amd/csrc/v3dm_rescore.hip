@@ -460,117 +460,227 @@ __global__ __launch_bounds__(256, 8) void k_batch_rescore_u8(BatchDev b, Cand2 c
     add_stripe(n, stripes);
 }
 
-// The same for the learned lists (topK in [1, kLaneTopK]) with one lane per query, the
-// work-group's 256 queries one segment (kBatchRowAlign = 256).  Candidate selection in two
-// passes over the query's lists: the first keeps the kLaneTopK largest list values
-// (multiplicity counted) and sets thr = max(m - margin, the topK-th largest), m the
-// largest -- the eight-lane selection's threshold in both of its cases (at most topK
-// values within the margin: m - margin, which is then >= the topK-th largest; else the
-// topK-th largest, which is then >= m - margin); the second scores every listed candidate
-// with value >= thr, as the eight-lane form does: the same candidates, the same integer
-// scores and rounding, the same top-2 order (score, then smaller index) and decision.
-constexpr int kLaneTopK = 8;
+// The learned lists of uint8 images (topK in [1, kSelK]) in two kernels.
+//
+// k_batch_select_lane, one lane per query (a work-group's 256 queries lie in one segment,
+// kBatchRowAlign = 256), one pass over the query's lists: the kSelK largest list values
+// with their places in the lists (sorted, multiplicity counted), then
+// thr = max(m - margin, the topK-th largest) with m the largest -- the eight-lane
+// selection's threshold in both of its cases (at most topK values within the margin:
+// m - margin, which is then >= the topK-th largest; else the topK-th largest, which is
+// then >= m - margin).  The candidates at or above thr are a prefix of the kept ones unless
+// all kSelK kept values are >= thr (ties at the threshold may continue past them): such a
+// query, like every query when certify is off, goes to the open list (exact path).  The
+// prefix goes to slots[g][0 .. cnt) with cnt[g].
+//
+// k_batch_score8, eight lanes per query: lane k loads bytes 16k .. 16k+15 of the query row
+// and of every selected candidate row (coalesced 128-byte rows), forms the eight partial
+// integer dot products, and a transposed reduction (three exchange steps over the group)
+// leaves lane k with candidate k's whole product.  Score fl(fl(ra rb) I) as the eight-lane
+// form forms it, the top-2 (score, then smaller index) merged over the group, the
+// certified decision.  Same candidates, scores, order and decision as the eight-lane form.
+constexpr int kSelK = 8;
 
-__device__ __forceinline__ void topk_insert(float (&t)[kLaneTopK], float x)
+// The learned lists' values are integer code scores (|s| < 2^15): key = s * 64 + e with e
+// the candidate's place in the query's lists (2 * list + 0/1, < 64) orders by value (then
+// place) and fits an int32; INT_MIN marks no candidate.
+__device__ __forceinline__ void topk_insert_key(int (&t)[kSelK], int x)
 {
 #pragma unroll
-    for (int i = 0; i < kLaneTopK; ++i) {  // t descending: keep the larger, carry the smaller
-        float const hi = fmaxf(t[i], x);
-        x = fminf(t[i], x);
+    for (int i = 0; i < kSelK; ++i) {  // t descending: keep the larger, carry the smaller
+        int const hi = max(t[i], x);
+        x = min(t[i], x);
         t[i] = hi;
     }
 }
 
-// exact integer dot product of two uint8 rows held as eight uint4
-__device__ __forceinline__ unsigned dot_u8_row(uint4 const (&a)[8], uint4 const (&c)[8])
-{
-    unsigned acc = 0u;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        acc = __builtin_amdgcn_udot4(a[k].x, c[k].x, acc, false);
-        acc = __builtin_amdgcn_udot4(a[k].y, c[k].y, acc, false);
-        acc = __builtin_amdgcn_udot4(a[k].z, c[k].z, acc, false);
-        acc = __builtin_amdgcn_udot4(a[k].w, c[k].w, acc, false);
-    }
-    return acc;
-}
+__device__ __forceinline__ float key_value(int key) { return key == INT_MIN ? -INFINITY : (float)(key >> 6); }
 
-__global__ __launch_bounds__(256) void k_batch_rescore_lane(BatchDev b, Cand2 const* __restrict__ part, int topK,
-                                                            float margin, float minScore, float minRatio,
-                                                            bool certify, int* __restrict__ sel,
-                                                            int* __restrict__ open, int* __restrict__ nOpen,
-                                                            int* __restrict__ stripes)
+__global__ __launch_bounds__(256) void k_batch_select_lane(BatchDev b, Cand2 const* __restrict__ part, int topK,
+                                                           float margin, bool certify, int* __restrict__ sel,
+                                                           int* __restrict__ cnt, int4* __restrict__ slots,
+                                                           int* __restrict__ open, int* __restrict__ nOpen,
+                                                           int* __restrict__ stripes)
 {
     static_assert(kBatchRowAlign == 256, "a work-group's queries lie in one segment");
-    int const blk = xcd_block(blockIdx.x, gridDim.x);
-    int const g = blk * 256 + (int)threadIdx.x;
+    static_assert(2 * 2 * kShortMaxSplits <= 64, "a candidate's place fits 6 bits");
+    BatchBlock const bi = b.blkInfo[xcd_block(blockIdx.x, gridDim.x)];  // one scalar load
+    int const g = bi.blk * 256 + (int)threadIdx.x;
     int n = 0;
     if (g < b.qRows) {
-        BatchSeg const sg = b.seg[b.segOfBlock[blk]];
-        int const q = g - sg.qOff;
+        int const q = bi.q0 + (int)threadIdx.x;
+        int c = 0;
         int d = -1;
-        if (q < sg.n && sg.nO > 0 && sg.nParts > 0) {
-            int const row = b.imgBase[sg.img] + q;
-            uint8_t const* __restrict__ cRows = b.u8 + (size_t)b.imgBase[sg.oImg] * kDim;
-            float const* __restrict__ cDen = b.denom + b.imgBase[sg.oImg];
-            uint4 qv[8];
-            if (certify) {
+        int jj[kSelK];
 #pragma unroll
-                for (int k = 0; k < 8; ++k) qv[k] = reinterpret_cast<uint4 const*>(b.u8 + (size_t)row * kDim)[k];
-            }
-            float const ra = b.denom[row];
-            Cand2 const* __restrict__ lp = part + ((size_t)sg.partOff + q) * 2;
-            size_t const ld2 = 2 * (size_t)sg.ld;
-            int const nLists = 2 * sg.nParts;
-            float t[kLaneTopK];
+        for (int i = 0; i < kSelK; ++i) jj[i] = -1;
+        if (q < bi.n && bi.nO > 0 && bi.nParts > 0) {
+            Cand2 const* __restrict__ lp = part + ((size_t)bi.partOff + q) * 2;
+            size_t const ld2 = 2 * (size_t)bi.ld;
+            int const nLists = 2 * bi.nParts;
+            int t[kSelK];
 #pragma unroll
-            for (int i = 0; i < kLaneTopK; ++i) t[i] = -INFINITY;
-            for (int e = 0; e < nLists; ++e) {
-                Cand2 const c = lp[(size_t)(e >> 1) * ld2 + (e & 1)];
-                topk_insert(t, c.j1 >= 0 ? c.v1 : -INFINITY);
-                topk_insert(t, c.j2 >= 0 ? c.v2 : -INFINITY);
-            }
-            float kth = t[0];
+            for (int i = 0; i < kSelK; ++i) t[i] = INT_MIN;
+            int total = 0;  // listed candidates
+            for (int e0 = 0; e0 < nLists; e0 += 4) {
+                Cand2 cs[4];
 #pragma unroll
-            for (int i = 1; i < kLaneTopK; ++i) kth = (i == topK - 1) ? t[i] : kth;
-            float const thr = fmaxf(t[0] - margin, kth);
-            Top2x s = {kInit, kInit, -1, -1};
-            for (int e = 0; e < nLists; ++e) {
-                Cand2 const c = lp[(size_t)(e >> 1) * ld2 + (e & 1)];
-                bool const take1 = c.j1 >= 0 && c.v1 >= thr;
-                bool const take2 = c.j2 >= 0 && c.v2 >= thr;
-                n += (take1 ? 1 : 0) + (take2 ? 1 : 0);
-                if (!certify) continue;
-                uint4 r1[8], r2[8];
-                float rb1 = 0.0f, rb2 = 0.0f;
-                if (take1) {
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) r1[k] = reinterpret_cast<uint4 const*>(cRows + (size_t)c.j1 * kDim)[k];
-                    rb1 = cDen[c.j1];
+                for (int u = 0; u < 4; ++u) {  // four lists in flight
+                    int const e = e0 + u;
+                    cs[u] = Cand2{0.0f, -1, 0.0f, -1};
+                    if (e < nLists) cs[u] = lp[(size_t)(e >> 1) * ld2 + (e & 1)];
                 }
-                if (take2) {
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) r2[k] = reinterpret_cast<uint4 const*>(cRows + (size_t)c.j2 * kDim)[k];
-                    rb2 = cDen[c.j2];
-                }
-                if (take1) {
-                    float const sc = mul_nc(mul_nc(ra, rb1), (float)dot_u8_row(qv, r1));
-                    if (sc == sc) s = top2x_merge(s, Top2x{sc, kInit, c.j1, -1});
-                }
-                if (take2) {
-                    float const sc = mul_nc(mul_nc(ra, rb2), (float)dot_u8_row(qv, r2));
-                    if (sc == sc) s = top2x_merge(s, Top2x{sc, kInit, c.j2, -1});
+                for (int u = 0; u < 4; ++u) {
+                    int const e = e0 + u;
+                    if (cs[u].j1 >= 0) {
+                        topk_insert_key(t, (int)cs[u].v1 * 64 + 2 * e);
+                        ++total;
+                    }
+                    if (cs[u].j2 >= 0) {
+                        topk_insert_key(t, (int)cs[u].v2 * 64 + 2 * e + 1);
+                        ++total;
+                    }
                 }
             }
-            d = kFlag;
-            if (certify)
-                d = decide_certified(Top2{s.b1, s.b2, s.a1}, kCertU8 * fmaxf(s.b1, 0.0f) + kEpsAbs, minScore,
-                                     minRatio);
-            if (d == kFlag) open[atomicAdd(nOpen, 1)] = g;
+            float kth = key_value(t[0]);
+#pragma unroll
+            for (int i = 1; i < kSelK; ++i) kth = (i == topK - 1) ? key_value(t[i]) : kth;
+            float const thr = fmaxf(key_value(t[0]) - margin, kth);
+#pragma unroll
+            for (int i = 0; i < kSelK; ++i) c += (t[i] != INT_MIN && key_value(t[i]) >= thr) ? 1 : 0;
+            n = c;  // candidates scored here (an overflowing query's further ties uncounted)
+            bool const overflow = c == kSelK && total > kSelK;
+            if (!certify || overflow) {
+                d = kFlag;
+                c = 0;
+                open[atomicAdd(nOpen, 1)] = g;
+            } else if (c > 0) {
+                d = kFlag;  // decided by k_batch_score8
+#pragma unroll
+                for (int i = 0; i < kSelK; ++i)  // the kept candidates' indices (cache hits)
+                    if (i < c) {
+                        int const e = (t[i] & 63) >> 1;
+                        Cand2 const* cp = lp + (size_t)(e >> 1) * ld2 + (e & 1);
+                        jj[i] = (t[i] & 1) ? cp->j2 : cp->j1;
+                    }
+            }
         }
         sel[g] = d;
+        cnt[g] = c;
+        // slots past cnt repeat slot 0 (read, not used, by k_batch_score8)
+#pragma unroll
+        for (int i = 1; i < kSelK; ++i) jj[i] = i < c ? jj[i] : jj[0];
+        slots[2 * (size_t)g] = make_int4(jj[0], jj[1], jj[2], jj[3]);
+        slots[2 * (size_t)g + 1] = make_int4(jj[4], jj[5], jj[6], jj[7]);
     }
     add_stripe(n, stripes);
+}
+
+// DPP exchanges inside a group of eight lanes (lane k = lane & 7):
+//   kDppXor1 / kDppXor2: quad_perm [1,0,3,2] / [2,3,0,1] (lane k ^ 1, k ^ 2);
+//   kDppMirror8: row_half_mirror (lane 7 - k: the other half of the group).
+constexpr int kDppXor1 = 0xB1;
+constexpr int kDppXor2 = 0x4E;
+constexpr int kDppMirror8 = 0x141;
+
+template <int kCtrl>
+__device__ __forceinline__ unsigned dpp_u(unsigned v)
+{
+    return (unsigned)__builtin_amdgcn_mov_dpp((int)v, kCtrl, 0xF, 0xF, false);
+}
+
+// Best score, its candidate and the second best score (the decision needs no second
+// index): the merge keeps the larger best (equal: the smaller index, -1 as the largest) and
+// the second largest of the four values.
+struct Top2s {
+    float b1, b2;
+    int a1;
+};
+
+__device__ __forceinline__ Top2s top2s_merge(Top2s a, Top2s b)
+{
+    bool const takeB = (b.b1 > a.b1) || (b.b1 == a.b1 && (unsigned)b.a1 < (unsigned)a.a1);
+    return Top2s{fmaxf(a.b1, b.b1), fmaxf(fminf(a.b1, b.b1), fmaxf(a.b2, b.b2)), takeB ? b.a1 : a.a1};
+}
+
+template <int kCtrl>
+__device__ __forceinline__ Top2s top2s_dpp(Top2s s)
+{
+    return Top2s{__uint_as_float(dpp_u<kCtrl>(__float_as_uint(s.b1))),
+                 __uint_as_float(dpp_u<kCtrl>(__float_as_uint(s.b2))), (int)dpp_u<kCtrl>((unsigned)s.a1)};
+}
+
+__global__ __launch_bounds__(256) void k_batch_score8(BatchDev b, int const* __restrict__ cnt,
+                                                      int4 const* __restrict__ slots, float minScore, float minRatio,
+                                                      int* __restrict__ sel, int* __restrict__ open,
+                                                      int* __restrict__ nOpen)
+{
+    int const k = threadIdx.x & 7;
+    int const lb = xcd_block(blockIdx.x, gridDim.x);  // 32-query blocks, in blkInfo's order
+    BatchBlock const bi = b.blkInfo[lb >> 3];         // one scalar load
+    int const sub = (lb & 7) * 32 + (int)(threadIdx.x >> 3);  // query inside the 256-row block
+    int const g = bi.blk * 256 + sub;
+    int const c = cnt[g];                    // uniform over the group
+    if (c <= 0) return;                      // whole groups leave: the exchanges stay inside groups
+    int const row = bi.rowBase + sub;
+    uint8_t const* __restrict__ cRows = b.u8 + (size_t)bi.cBase * kDim;  // uniform
+    float const* __restrict__ cDen = b.denom + bi.cBase;
+    int4 const s0 = slots[2 * (size_t)g], s1 = slots[2 * (size_t)g + 1];
+    int const js[kSelK] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+    uint4 const qv = reinterpret_cast<uint4 const*>(b.u8 + (size_t)row * kDim)[k];
+    float const ra = b.denom[row];
+    // chunk k of each slot's row (32-bit offsets from the uniform image base); slots past
+    // cnt repeat slot 0's row (cache hits), and their sums only reach lanes that are not used
+    uint4 cv[kSelK];
+#pragma unroll
+    for (int i = 0; i < kSelK; ++i)
+        cv[i] = *reinterpret_cast<uint4 const*>(cRows + ((unsigned)js[i] * (unsigned)kDim + 16u * k));
+    int const jk = k < c ? js[k] : -1;  // the candidate this lane finishes
+    float const rb = jk >= 0 ? cDen[jk] : 0.0f;
+    unsigned p[kSelK];
+#pragma unroll
+    for (int i = 0; i < kSelK; ++i) {
+        unsigned acc = __builtin_amdgcn_udot4(qv.x, cv[i].x, 0u, false);
+        acc = __builtin_amdgcn_udot4(qv.y, cv[i].y, acc, false);
+        acc = __builtin_amdgcn_udot4(qv.z, cv[i].z, acc, false);
+        p[i] = __builtin_amdgcn_udot4(qv.w, cv[i].w, acc, false);
+    }
+    // transposed reduction: step 1 with lane 7 - k (other half: k < 4 keeps candidates
+    // 0..3), step 2 with k ^ 2 (bit 1 of k picks the pair), step 3 with k ^ 1
+    unsigned r4[4], r2[2];
+    bool const lo4 = k < 4;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        unsigned const keep = lo4 ? p[i] : p[4 + i];
+        unsigned const send = lo4 ? p[4 + i] : p[i];
+        r4[i] = keep + dpp_u<kDppMirror8>(send);
+    }
+    bool const lo2 = (k & 2) == 0;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        unsigned const keep = lo2 ? r4[i] : r4[2 + i];
+        unsigned const send = lo2 ? r4[2 + i] : r4[i];
+        r2[i] = keep + dpp_u<kDppXor2>(send);
+    }
+    bool const lo1 = (k & 1) == 0;
+    unsigned const keep = lo1 ? r2[0] : r2[1];
+    unsigned const send = lo1 ? r2[1] : r2[0];
+    unsigned const dot = keep + dpp_u<kDppXor1>(send);  // candidate k's product
+    Top2s s = {kInit, kInit, -1};
+    if (jk >= 0) {
+        float const sc = mul_nc(mul_nc(ra, rb), (float)dot);
+        if (sc == sc) s = Top2s{sc, kInit, jk};
+    }
+    s = top2s_merge(s, top2s_dpp<kDppXor1>(s));
+    s = top2s_merge(s, top2s_dpp<kDppXor2>(s));
+    s = top2s_merge(s, top2s_dpp<kDppMirror8>(s));
+    if (k == 0) {
+        int const d = decide_certified(Top2{s.b1, s.b2, s.a1}, kCertU8 * fmaxf(s.b1, 0.0f) + kEpsAbs, minScore,
+                                       minRatio);
+        sel[g] = d;
+        if (d == kFlag) open[atomicAdd(nOpen, 1)] = g;
+    }
 }
 
 // The open queries of k_batch_rescore_u8, eight lanes each (grid-stride over the list): the
@@ -609,12 +719,15 @@ __global__ __launch_bounds__(256) void k_batch_shortlist_finish(BatchDev b, Cand
 }
 
 void launch_batch_rescore(BatchDev const& b, Cand2 const* part, float winRel, int topK, float margin, float minScore,
-                          float minRatio, bool certify, int* sel, int* open, int* nOpen, int* stripes, hipStream_t st)
+                          float minRatio, bool certify, int* sel, int* open, int* nOpen, int* selCnt, int* selSlots,
+                          int* stripes, hipStream_t st)
 {
     if (b.qRows <= 0) return;
-    if (b.u8 && topK >= 1 && topK <= kLaneTopK) {
-        hipLaunchKernelGGL(k_batch_rescore_lane, dim3((b.qRows + 255) / 256), dim3(256), 0, st, b, part, topK, margin,
-                           minScore, minRatio, certify, sel, open, nOpen, stripes);
+    if (b.u8 && topK >= 1 && topK <= kSelK && selCnt && selSlots) {
+        hipLaunchKernelGGL(k_batch_select_lane, dim3((b.qRows + 255) / 256), dim3(256), 0, st, b, part, topK, margin,
+                           certify, sel, selCnt, reinterpret_cast<int4*>(selSlots), open, nOpen, stripes);
+        hipLaunchKernelGGL(k_batch_score8, dim3((b.qRows + 31) / 32), dim3(256), 0, st, b, selCnt,
+                           reinterpret_cast<int4 const*>(selSlots), minScore, minRatio, sel, open, nOpen);
         hipLaunchKernelGGL(k_batch_rescore_open, dim3(256), dim3(256), 0, st, b, part, winRel, topK, margin, minScore,
                            minRatio, sel, open, nOpen);
     } else if (b.u8) {
