@@ -167,7 +167,7 @@ int enqueue_plain(v3dm_context* h, float minScore, float minRatio)
                              h->colMax, h->rowMax, h->rpart, kRecheckGrid, st);
     }
     launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mstatus, next_mutual_epoch(h), h->stripes,
-                          reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st);
+                          reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st, {}, h->mutualWait);
     HIPCHECK(h, hipGetLastError());
     return V3DM_OK;
 }
@@ -427,7 +427,7 @@ int runGuided(v3dm_context* h, int kind, const double M[9], float distanceThresh
                              h->rowMax, h->rpart, kRecheckGrid, st, h->guidedBits, h->capL);
     }
     launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mstatus, next_mutual_epoch(h), h->stripes,
-                          reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st);
+                          reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st, {}, h->mutualWait);
     HIPCHECK(h, hipGetLastError());
     h->lastMinScore = minScore;
     h->lastMinRatio = minRatio;
@@ -504,6 +504,14 @@ int v3dm_set_fallback_rule(v3dm_handle h, int rule)
     if (rule != V3DM_FALLBACK_THRESHOLD && rule != V3DM_FALLBACK_BINOMIAL)
         return fail(h, V3DM_ERR_INVALID, "unknown fallback rule %d", rule);
     h->fallbackRule = rule;
+    return V3DM_OK;
+}
+
+int v3dm_debug_mutual_wait(v3dm_handle h, int spinLimit, int neverReady)
+{
+    if (!h) return fail(nullptr, V3DM_ERR_INVALID, "null handle");
+    h->mutualWait.spinLimit = spinLimit < 0 ? kMutualSpinLimit : (unsigned)spinLimit;
+    h->mutualWait.readXor = neverReady ? 0x80000000u : 0u;
     return V3DM_OK;
 }
 

@@ -46,6 +46,7 @@ struct v3dm_context {
     v3dm::Top2* rpart = nullptr;       // re-check chunk partials: (capL + capR) x kRecheckMaxChunks
     unsigned long long* mstatus = nullptr;  // mutual-check status words: capL / kMutualSlice + 1
     unsigned mutualEpoch = 0;               // epoch of the last mutual check (never 0 once run)
+    v3dm::MutualWait mutualWait;            // the look-back's wait (v3dm_debug_mutual_wait)
     int* stripes = nullptr;      // striped counts (kStripes slots), cleared by the gather kernel
     int* results = nullptr;  // counters (kNumCounters ints) followed by int2 pairs
     int* hostStage = nullptr;

@@ -123,7 +123,7 @@ int enqueue_bf16(v3dm_context* h, float minScore, float minRatio)
     }
     e->haveSecond = work;
     launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mstatus, next_mutual_epoch(h), h->stripes,
-                          reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st);
+                          reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st, {}, h->mutualWait);
     EXTCHECK(h, hipGetLastError());
     return V3DM_OK;
 }
@@ -234,11 +234,11 @@ int enqueue_learned(v3dm_context* h, float minScore, float minRatio)
             e->sPending = true;
         }
         launch_mutual_compact(h->colMax, h->rowMax, nL, nR, h->mstatus, next_mutual_epoch(h), h->stripes,
-                              reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st, so);
+                              reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st, so, h->mutualWait);
     } else {
         e->haveSecond = false;
         launch_mutual_compact(h->colMax, h->rowMax, 0, nR, h->mstatus, next_mutual_epoch(h), h->stripes,
-                              reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st);
+                              reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st, {}, h->mutualWait);
     }
     EXTCHECK(h, hipGetLastError());
     return V3DM_OK;
@@ -267,7 +267,7 @@ int enqueue_fp8(v3dm_context* h, float minScore, float minRatio)
     }
     e->haveSecond = work;
     launch_mutual_compact(h->colMax, h->rowMax, work ? nL : 0, nR, h->mstatus, next_mutual_epoch(h), h->stripes,
-                          reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st);
+                          reinterpret_cast<int2*>(h->results + kNumCounters), h->results, st, {}, h->mutualWait);
     EXTCHECK(h, hipGetLastError());
     return V3DM_OK;
 }

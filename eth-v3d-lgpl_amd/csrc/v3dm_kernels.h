@@ -67,9 +67,17 @@ struct SampleOut {
 // the first call); epoch: nonzero and different from the previous call's.  The sampled
 // learned decisions, when asked for, are written by work-group 0 (colMax is final by then).
 constexpr int kMutualSlice = 1024;  // left features per work-group of the mutual check
+// The look-back's bounded wait: at most spinLimit sleeps; a predecessor's status word counts
+// when its epoch equals epoch ^ readXor (0 in every call but the test of the time-out path,
+// v3dm_debug_mutual_wait, which makes no word count so that the wait runs out).
+constexpr unsigned kMutualSpinLimit = 1u << 20;
+struct MutualWait {
+    unsigned spinLimit = kMutualSpinLimit;
+    unsigned readXor = 0;
+};
 void launch_mutual_compact(int const* colMax, int const* rowMax, int nL, int nR, unsigned long long* status,
                            unsigned epoch, int* stripes, int2* pairs, int* counters, hipStream_t st,
-                           SampleOut const& so = {});
+                           SampleOut const& so = {}, MutualWait const& mw = {});
 // uint8 SIFT descriptors -> floats, as SIFT_Feature::normalizeDescriptor
 // (V3D/Base/v3d_feature.h:30-38): var = sum q^2, d = q * (1/sqrtf(var)), IEEE rounding.
 void launch_dequant_u8(uint8_t const* q, int n, float* out, hipStream_t st);

@@ -285,15 +285,13 @@ void launch_recheck_exact(float const* L, float const* R, int nL, int nR, int co
 // inclusive prefix and writes its matches in ascending k from its exclusive prefix on, so
 // the output is sorted by k.  A status word is valid for the call whose epoch it carries,
 // so nothing is cleared between calls.  A work-group waits only for lower indices, which
-// are dispatched before it; the wait is bounded (kMutualSpinLimit sleeps), and a work-group
+// are dispatched before it; the wait is bounded (mw.spinLimit sleeps), and a work-group
 // that runs out of it stores the epoch in counters[kCntMutualError] (its matches then land
 // below their place, inside the buffer).  The last work-group also publishes the match,
 // re-check and rescoring counts and clears the per-call counters.
 
 // Left feature sampled as sample k of S (evenly spaced over [0, nL)).
 __device__ __forceinline__ int sample_row(int k, int nL, int S) { return (int)(((long long)k * nL) / S); }
-
-constexpr unsigned kMutualSpinLimit = 1u << 20;
 
 __device__ __forceinline__ unsigned long long mutual_word(unsigned epoch, unsigned flag, unsigned value)
 {
@@ -303,7 +301,8 @@ __device__ __forceinline__ unsigned long long mutual_word(unsigned epoch, unsign
 __global__ __launch_bounds__(256) void k_mutual(int const* __restrict__ colMax, int const* __restrict__ rowMax,
                                                 int nL, int nR, unsigned long long* __restrict__ status,
                                                 unsigned epoch, int* __restrict__ stripes,
-                                                int2* __restrict__ pairs, int* __restrict__ counters, SampleOut so)
+                                                int2* __restrict__ pairs, int* __restrict__ counters, SampleOut so,
+                                                MutualWait mw)
 {
     __shared__ int wsum[4];
     __shared__ int sBase;
@@ -360,7 +359,7 @@ __global__ __launch_bounds__(256) void k_mutual(int const* __restrict__ colMax, 
             unsigned long long const w =
                 (j >= 0) ? __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                          : mutual_word(epoch, 2u, 0u);  // before work-group 0: an inclusive prefix of 0
-            unsigned const fl = ((unsigned)(w >> 32) == epoch) ? (unsigned)(w >> 30) & 3u : 0u;
+            unsigned const fl = ((unsigned)(w >> 32) == (epoch ^ mw.readXor)) ? (unsigned)(w >> 30) & 3u : 0u;
             unsigned long long const ready = __ballot(fl != 0u);
             unsigned long long const inclusive = __ballot(fl == 2u);
             int const stop = inclusive ? __ffsll((long long)inclusive) - 1 : 63;  // last lane to add
@@ -373,7 +372,7 @@ __global__ __launch_bounds__(256) void k_mutual(int const* __restrict__ colMax, 
                 if (inclusive) break;
                 top -= 64;
             } else {
-                if (++spins > kMutualSpinLimit) {
+                if (++spins > mw.spinLimit) {
                     failed = true;
                     break;
                 }
@@ -416,11 +415,11 @@ __global__ __launch_bounds__(256) void k_mutual(int const* __restrict__ colMax, 
 
 void launch_mutual_compact(int const* colMax, int const* rowMax, int nL, int nR, unsigned long long* status,
                            unsigned epoch, int* stripes, int2* pairs, int* counters, hipStream_t st,
-                           SampleOut const& so)
+                           SampleOut const& so, MutualWait const& mw)
 {
     int const nb = (nL + kMutualSlice - 1) / kMutualSlice;
     hipLaunchKernelGGL(k_mutual, dim3(nb > 0 ? nb : 1), dim3(256), 0, st, colMax, rowMax, nL, nR, status, epoch,
-                       stripes, pairs, counters, so);
+                       stripes, pairs, counters, so, mw);
 }
 
 // ------------------------------------------------------------------------------ uint8 dequantisation

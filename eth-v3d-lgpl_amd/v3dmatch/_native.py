@@ -67,6 +67,7 @@ SIGNATURES = {
     "v3dm_fallback_allowed_misses": (_I, [_I, _F]),
     "v3dm_fallback_allowed_misses_binomial": (_I, [_I, _F]),
     "v3dm_set_fallback_rule": (_I, [_P, _I]),
+    "v3dm_debug_mutual_wait": (_I, [_P, _I, _I]),
     "v3dm_set_left_device": (_I, [_P, _I, _FP]),
     "v3dm_set_right_device": (_I, [_P, _I, _FP]),
     "v3dm_set_left_u8": (_I, [_P, _I, _FP]),

@@ -179,6 +179,11 @@ class SIFT_Matching:
         min_recall) or FALLBACK_BINOMIAL (a one-sided binomial test at 5%)."""
         self._check(self._lib.v3dm_set_fallback_rule(self._h, int(rule)))
 
+    def debug_mutual_wait(self, spin_limit=-1, never_ready=False):
+        """Test hook of the mutual check's wait-ran-out path (v3dm_debug_mutual_wait):
+        debug_mutual_wait(8, True) makes the next calls fail; debug_mutual_wait() restores."""
+        self._check(self._lib.v3dm_debug_mutual_wait(self._h, int(spin_limit), int(bool(never_ready))))
+
     @staticmethod
     def fallback_allowed_misses(accepted, min_recall=0.995, rule=0):
         """Misses a sampled check of `accepted` exact matches tolerates before the plain

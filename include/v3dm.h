@@ -133,6 +133,14 @@ int v3dm_set_fallback(v3dm_handle h, int samples, float minRecall);
 #define V3DM_FALLBACK_THRESHOLD 0
 #define V3DM_FALLBACK_BINOMIAL 1
 int v3dm_set_fallback_rule(v3dm_handle h, int rule);
+/* Test hook of the failure path of the mutual check's ordered compaction (no counterpart
+ * in the reference; for tests only).  Each work-group of the single-pair mutual check waits,
+ * boundedly, for its predecessors' counts; when the wait runs out the call fails with
+ * V3DM_ERR_HIP ("ordering wait ... ran out") and the handle stays usable.  spinLimit: the
+ * sleeps before giving up (< 0: the default, 2^20); neverReady = 1 makes no predecessor
+ * count visible, so every work-group but the first runs out of its wait (with several
+ * work-groups, i.e. more than 1024 left features).  0, -1 restores normal operation. */
+int v3dm_debug_mutual_wait(v3dm_handle h, int spinLimit, int neverReady);
 /* The misses a sampled check of `accepted` exact matches tolerates at recall minRecall
  * under the threshold rule: the largest m with (float)(accepted - m) / (float)accepted not
  * below minRecall (0 for minRecall 1, `accepted` for minRecall 0); -1 when accepted is
