@@ -449,8 +449,10 @@ void launch_batch_load_f32(float const* src, int const* srcRow, int const* nImg,
 }
 
 // ------------------------------------------------------------------------------ sampled check
-// Per pair p, S_p = min(samples, nL_p) evenly spaced left features: rows, norms and indices
-// gathered into kSampleRows-row blocks (zero rows past S_p).  One wave per sample row.
+// FP32 images (uint8 batches use v3dm_check.hip).  Per pair p, S_p = min(samples, n) evenly
+// spaced features of the side sample_side picks (its larger image, left on a tie): rows,
+// norms and indices gathered into kSampleRows-row blocks (zero rows past S_p).  One wave per
+// sample row.
 __global__ __launch_bounds__(256) void k_batch_sample_gather(BatchDev b, int samples, float* __restrict__ sRows,
                                                              float* __restrict__ sNorm, int* __restrict__ sIdx,
                                                              int* __restrict__ sCount)
@@ -476,8 +478,8 @@ __global__ __launch_bounds__(256) void k_batch_sample_gather(BatchDev b, int sam
 // One work-group (kSampleRows threads) per pair: merge the sample's split partials, take
 // the certified FP32 decision (samples whose decision is not certified -- near ties, well
 // under 1% -- are left out of the estimate), compare with the shortlist decision sel of the
-// same left feature.  pairFlag[p] = 1 when the misses exceed lim.allowed[accepted];
-// counts[4p..] = accepted, missed, left out.
+// same feature of the sampled side.  pairFlag[p] = 1 when the misses exceed
+// lim.allowed[accepted]; counts[4p..] = accepted, missed, left out.
 __global__ __launch_bounds__(kSampleRows) void k_batch_sample_decide(BatchDev b, Top2 const* __restrict__ sPart,
                                                                      int maxSplits, int const* __restrict__ nSplits,
                                                                      float const* __restrict__ sNorm,
