@@ -110,8 +110,6 @@ struct BatchState {
     int* sel = nullptr;
     int* flagList = nullptr;
     int* openList = nullptr;  // open decisions of the shortlist rescoring
-    int* selCnt = nullptr;    // learned rescoring: selected candidates per query
-    int* selSlots = nullptr;  // and their indices, 8 per query
     int selCap = 0;
     int* counters = nullptr;  // kBatchCounters ints
     int* stripes = nullptr;   // striped counts, cleared by the gather kernel
@@ -535,13 +533,9 @@ int ensure_query_space(v3dm_context* h, BatchState* b)
     if ((rc = growi(h, b->sel, b->selCap, b->qRows))) return rc;
     if (b->flagList) (void)hipFree(b->flagList);
     if (b->openList) (void)hipFree(b->openList);
-    if (b->selCnt) (void)hipFree(b->selCnt);
-    if (b->selSlots) (void)hipFree(b->selSlots);
-    b->flagList = b->openList = b->selCnt = b->selSlots = nullptr;
+    b->flagList = b->openList = nullptr;
     BCHECK(h, hipMalloc(&b->flagList, (size_t)std::max(b->qRows, 1) * sizeof(int)));
     BCHECK(h, hipMalloc(&b->openList, (size_t)std::max(b->qRows, 1) * sizeof(int)));
-    BCHECK(h, hipMalloc(&b->selCnt, (size_t)std::max(b->qRows, 1) * sizeof(int)));
-    BCHECK(h, hipMalloc(&b->selSlots, (size_t)std::max(b->qRows, 1) * 8 * sizeof(int)));
     int const nbMutual = (b->leftRows + 1023) / 1024;
     if (b->tmp) (void)hipFree(b->tmp);
     if (b->blockCount) (void)hipFree(b->blockCount);
@@ -606,12 +600,11 @@ int enqueue_run(v3dm_context* h, BatchState* b, int mode, float minScore, float 
     } else if (form == 1) {
         launch_batch_pass_bf16(d, b->items[1], (int)pl.items.size(), b->counters + 1, b->partB, pl.grid, st);
         launch_batch_rescore(d, b->partB, kWinRelBf16, 0, INFINITY, minScore, minRatio, h->epsRel > 0.0f, b->sel,
-                             b->openList, b->counters + kBCntOpenRescore, nullptr, nullptr, b->stripes, st);
+                             b->openList, b->counters + kBCntOpenRescore, b->stripes, st);
     } else {
         launch_batch_pass_code(d, b->items[2], (int)pl.items.size(), b->counters + 2, b->partB, pl.grid, st);
         launch_batch_rescore(d, b->partB, 0.0f, h->learnedTopK, learned_margin(h), minScore, minRatio,
-                             h->epsRel > 0.0f, b->sel, b->openList, b->counters + kBCntOpenRescore, b->selCnt,
-                             b->selSlots, b->stripes, st);
+                             h->epsRel > 0.0f, b->sel, b->openList, b->counters + kBCntOpenRescore, b->stripes, st);
         if (h->fallbackSamples > 0) {
             int dev = 0, cus = 256;
             (void)hipGetDevice(&dev);
@@ -911,7 +904,7 @@ void batch_destroy(BatchState*& b)
                     (void*)b->items[0], (void*)b->items[1], (void*)b->items[2], (void*)b->items[3],
                     (void*)b->segD[0], (void*)b->segD[1], (void*)b->segD[2], (void*)b->segD[3], (void*)b->blkInfo[0],
                     (void*)b->blkInfo[1], (void*)b->blkInfo[2], (void*)b->blkInfo[3], (void*)b->partF,
-                    (void*)b->partB, (void*)b->sel, (void*)b->flagList, (void*)b->openList, (void*)b->selCnt, (void*)b->selSlots, (void*)b->sRows, (void*)b->sNorm, (void*)b->sIdx, (void*)b->sCount,
+                    (void*)b->partB, (void*)b->sel, (void*)b->flagList, (void*)b->openList, (void*)b->sRows, (void*)b->sNorm, (void*)b->sIdx, (void*)b->sCount,
                     (void*)b->sNSplits, (void*)b->sPart, (void*)b->sItems, (void*)b->pairFlag, (void*)b->fbItems,
                     (void*)b->segMask, (void*)b->rowCS, (void*)b->cItems[0],
                     (void*)b->cItems[1], (void*)b->cNSplits, (void*)b->cPart, (void*)b->cQB, (void*)b->cSlotB,

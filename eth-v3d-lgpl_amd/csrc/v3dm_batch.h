@@ -122,11 +122,9 @@ void launch_batch_pass_bf16(BatchDev const& b, int4 const* items, int nItems, in
                             hipStream_t st);
 // uint8 images: certified integer scores, the open decisions (listed in `open`, count in
 // *nOpen, which must be zero) scored exactly (certify = false: every decision scored
-// exactly); FP32 images: exact scores.  selCnt / selSlots (qRows and 8 qRows ints, or null):
-// work buffers of the learned lists' two-kernel rescoring.
+// exactly); FP32 images: exact scores
 void launch_batch_rescore(BatchDev const& b, Cand2 const* part, float winRel, int topK, float margin, float minScore,
-                          float minRatio, bool certify, int* sel, int* open, int* nOpen, int* selCnt, int* selSlots,
-                          int* stripes, hipStream_t st);
+                          float minRatio, bool certify, int* sel, int* open, int* nOpen, int* stripes, hipStream_t st);
 // learned code pass (v3dm_code.hip)
 void launch_batch_pass_code(BatchDev const& b, int4 const* items, int nItems, int* queue, Cand2* part, int grid,
                             hipStream_t st);

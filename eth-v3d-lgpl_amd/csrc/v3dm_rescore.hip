@@ -460,129 +460,46 @@ __global__ __launch_bounds__(256, 8) void k_batch_rescore_u8(BatchDev b, Cand2 c
     add_stripe(n, stripes);
 }
 
-// The learned lists of uint8 images (topK in [1, kSelK]) in two kernels.
+// The learned lists of uint8 images (topK in [1, kSelK]): k_batch_rescore_learned, one
+// work-group per 64 queries of one segment (BatchBlock table, pair by pair).
 //
-// k_batch_select_lane, one lane per query (a work-group's 256 queries lie in one segment,
-// kBatchRowAlign = 256), one pass over the query's lists: the kSelK largest list values
-// with their places in the lists (sorted, multiplicity counted), then
-// thr = max(m - margin, the topK-th largest) with m the largest -- the eight-lane
-// selection's threshold in both of its cases (at most topK values within the margin:
-// m - margin, which is then >= the topK-th largest; else the topK-th largest, which is
-// then >= m - margin).  The candidates at or above thr are a prefix of the kept ones unless
-// all kSelK kept values are >= thr (ties at the threshold may continue past them): such a
-// query, like every query when certify is off, goes to the open list (exact path).  The
-// prefix goes to slots[g][0 .. cnt) with cnt[g].
+// Selection, one lane per query (wave 0), one pass over the query's lists with four lists in
+// flight: the kSelK largest list values with their candidates (sorted, multiplicity
+// counted), then thr = max(m - margin, the topK-th largest) with m the largest -- the
+// eight-lane selection's threshold in both of its cases (at most topK values within the
+// margin: m - margin, which is then >= the topK-th largest; else the topK-th largest, which
+// is then >= m - margin).  The candidates at or above thr are a prefix of the kept ones
+// unless all kSelK kept values are >= thr (ties at the threshold may continue past them):
+// such a query, like every query when certify is off, goes to the open list (exact path).
+// The prefix goes to LDS.
 //
-// k_batch_score8, eight lanes per query: lane k loads bytes 16k .. 16k+15 of the query row
-// and of every selected candidate row (coalesced 128-byte rows), forms the eight partial
-// integer dot products, and a transposed reduction (three exchange steps over the group)
-// leaves lane k with candidate k's whole product.  Score fl(fl(ra rb) I) as the eight-lane
-// form forms it, the top-2 (score, then smaller index) merged over the group, the
-// certified decision.  Same candidates, scores, order and decision as the eight-lane form.
+// Scoring, four lanes per query (a quad): lane k holds bytes 32k .. 32k+31 of the query row
+// and of every selected candidate row, eight partial integer products per lane; a
+// transposed reduction in two quad exchanges (DPP) leaves lane k with the whole products of
+// candidates 2k and 2k+1; score fl(fl(ra rb) I) as the eight-lane form forms it, the top-2
+// (score, then smaller index) merged over the quad, the certified decision (eps = 2^-21
+// best: the integer product is exact, the two roundings give |s~ - s| <= 5.01 2^-24 s~).
+// Same candidates, scores, order and decision as the eight-lane form, in three dependent
+// memory round trips (block table, lists, rows).
 constexpr int kSelK = 8;
 
-// The learned lists' values are integer code scores (|s| < 2^15): key = s * 64 + e with e
-// the candidate's place in the query's lists (2 * list + 0/1, < 64) orders by value (then
-// place) and fits an int32; INT_MIN marks no candidate.
-__device__ __forceinline__ void topk_insert_key(int (&t)[kSelK], int x)
+__device__ __forceinline__ void topk_insert_pair(float (&t)[kSelK], int (&jj)[kSelK], float x, int jx)
 {
 #pragma unroll
-    for (int i = 0; i < kSelK; ++i) {  // t descending: keep the larger, carry the smaller
-        int const hi = max(t[i], x);
-        x = min(t[i], x);
-        t[i] = hi;
+    for (int i = 0; i < kSelK; ++i) {  // t descending; a new value goes behind equal ones
+        bool const sw = x > t[i];
+        float const tv = sw ? x : t[i];
+        int const tj = sw ? jx : jj[i];
+        x = sw ? t[i] : x;
+        jx = sw ? jj[i] : jx;
+        t[i] = tv;
+        jj[i] = tj;
     }
 }
 
-__device__ __forceinline__ float key_value(int key) { return key == INT_MIN ? -INFINITY : (float)(key >> 6); }
-
-__global__ __launch_bounds__(256) void k_batch_select_lane(BatchDev b, Cand2 const* __restrict__ part, int topK,
-                                                           float margin, bool certify, int* __restrict__ sel,
-                                                           int* __restrict__ cnt, int4* __restrict__ slots,
-                                                           int* __restrict__ open, int* __restrict__ nOpen,
-                                                           int* __restrict__ stripes)
-{
-    static_assert(kBatchRowAlign == 256, "a work-group's queries lie in one segment");
-    static_assert(2 * 2 * kShortMaxSplits <= 64, "a candidate's place fits 6 bits");
-    BatchBlock const bi = b.blkInfo[xcd_block(blockIdx.x, gridDim.x)];  // one scalar load
-    int const g = bi.blk * 256 + (int)threadIdx.x;
-    int n = 0;
-    if (g < b.qRows) {
-        int const q = bi.q0 + (int)threadIdx.x;
-        int c = 0;
-        int d = -1;
-        int jj[kSelK];
-#pragma unroll
-        for (int i = 0; i < kSelK; ++i) jj[i] = -1;
-        if (q < bi.n && bi.nO > 0 && bi.nParts > 0) {
-            Cand2 const* __restrict__ lp = part + ((size_t)bi.partOff + q) * 2;
-            size_t const ld2 = 2 * (size_t)bi.ld;
-            int const nLists = 2 * bi.nParts;
-            int t[kSelK];
-#pragma unroll
-            for (int i = 0; i < kSelK; ++i) t[i] = INT_MIN;
-            int total = 0;  // listed candidates
-            for (int e0 = 0; e0 < nLists; e0 += 4) {
-                Cand2 cs[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {  // four lists in flight
-                    int const e = e0 + u;
-                    cs[u] = Cand2{0.0f, -1, 0.0f, -1};
-                    if (e < nLists) cs[u] = lp[(size_t)(e >> 1) * ld2 + (e & 1)];
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    int const e = e0 + u;
-                    if (cs[u].j1 >= 0) {
-                        topk_insert_key(t, (int)cs[u].v1 * 64 + 2 * e);
-                        ++total;
-                    }
-                    if (cs[u].j2 >= 0) {
-                        topk_insert_key(t, (int)cs[u].v2 * 64 + 2 * e + 1);
-                        ++total;
-                    }
-                }
-            }
-            float kth = key_value(t[0]);
-#pragma unroll
-            for (int i = 1; i < kSelK; ++i) kth = (i == topK - 1) ? key_value(t[i]) : kth;
-            float const thr = fmaxf(key_value(t[0]) - margin, kth);
-#pragma unroll
-            for (int i = 0; i < kSelK; ++i) c += (t[i] != INT_MIN && key_value(t[i]) >= thr) ? 1 : 0;
-            n = c;  // candidates scored here (an overflowing query's further ties uncounted)
-            bool const overflow = c == kSelK && total > kSelK;
-            if (!certify || overflow) {
-                d = kFlag;
-                c = 0;
-                open[atomicAdd(nOpen, 1)] = g;
-            } else if (c > 0) {
-                d = kFlag;  // decided by k_batch_score8
-#pragma unroll
-                for (int i = 0; i < kSelK; ++i)  // the kept candidates' indices (cache hits)
-                    if (i < c) {
-                        int const e = (t[i] & 63) >> 1;
-                        Cand2 const* cp = lp + (size_t)(e >> 1) * ld2 + (e & 1);
-                        jj[i] = (t[i] & 1) ? cp->j2 : cp->j1;
-                    }
-            }
-        }
-        sel[g] = d;
-        cnt[g] = c;
-        // slots past cnt repeat slot 0 (read, not used, by k_batch_score8)
-#pragma unroll
-        for (int i = 1; i < kSelK; ++i) jj[i] = i < c ? jj[i] : jj[0];
-        slots[2 * (size_t)g] = make_int4(jj[0], jj[1], jj[2], jj[3]);
-        slots[2 * (size_t)g + 1] = make_int4(jj[4], jj[5], jj[6], jj[7]);
-    }
-    add_stripe(n, stripes);
-}
-
-// DPP exchanges inside a group of eight lanes (lane k = lane & 7):
-//   kDppXor1 / kDppXor2: quad_perm [1,0,3,2] / [2,3,0,1] (lane k ^ 1, k ^ 2);
-//   kDppMirror8: row_half_mirror (lane 7 - k: the other half of the group).
+// DPP exchanges inside a quad: kDppXor1 / kDppXor2 = quad_perm [1,0,3,2] / [2,3,0,1]
 constexpr int kDppXor1 = 0xB1;
 constexpr int kDppXor2 = 0x4E;
-constexpr int kDppMirror8 = 0x141;
 
 template <int kCtrl>
 __device__ __forceinline__ unsigned dpp_u(unsigned v)
@@ -611,70 +528,146 @@ __device__ __forceinline__ Top2s top2s_dpp(Top2s s)
                  __uint_as_float(dpp_u<kCtrl>(__float_as_uint(s.b2))), (int)dpp_u<kCtrl>((unsigned)s.a1)};
 }
 
-__global__ __launch_bounds__(256) void k_batch_score8(BatchDev b, int const* __restrict__ cnt,
-                                                      int4 const* __restrict__ slots, float minScore, float minRatio,
-                                                      int* __restrict__ sel, int* __restrict__ open,
-                                                      int* __restrict__ nOpen)
+__global__ __launch_bounds__(256) void k_batch_rescore_learned(BatchDev b, Cand2 const* __restrict__ part, int topK,
+                                                               float margin, float minScore, float minRatio,
+                                                               bool certify, int* __restrict__ sel,
+                                                               int* __restrict__ open, int* __restrict__ nOpen,
+                                                               int* __restrict__ stripes)
 {
-    int const k = threadIdx.x & 7;
-    int const lb = xcd_block(blockIdx.x, gridDim.x);  // 32-query blocks, in blkInfo's order
-    BatchBlock const bi = b.blkInfo[lb >> 3];         // one scalar load
-    int const sub = (lb & 7) * 32 + (int)(threadIdx.x >> 3);  // query inside the 256-row block
-    int const g = bi.blk * 256 + sub;
-    int const c = cnt[g];                    // uniform over the group
-    if (c <= 0) return;                      // whole groups leave: the exchanges stay inside groups
-    int const row = bi.rowBase + sub;
+    static_assert(kBatchRowAlign == 256, "a work-group's queries lie in one segment");
+    __shared__ int sCnt[64];
+    __shared__ int sJ[64][kSelK];
+    int const tid = threadIdx.x;
+    int const lb = xcd_block(blockIdx.x, gridDim.x);  // 64-query blocks, in blkInfo's order
+    BatchBlock const bi = b.blkInfo[lb >> 2];         // one scalar load
+    int const base = (lb & 3) * 64;                   // first query of the work-group in its block
+    int n = 0;
+    if (tid < 64) {
+        // ---- selection: query base + tid
+        int const q = bi.q0 + base + tid;
+        int c = 0;
+        int d = -1;
+        int jj[kSelK];
+#pragma unroll
+        for (int i = 0; i < kSelK; ++i) jj[i] = -1;
+        if (q < bi.n && bi.nO > 0 && bi.nParts > 0) {
+            Cand2 const* __restrict__ lp = part + ((size_t)bi.partOff + q) * 2;
+            size_t const ld2 = 2 * (size_t)bi.ld;
+            int const nLists = 2 * bi.nParts;
+            float t[kSelK];
+#pragma unroll
+            for (int i = 0; i < kSelK; ++i) t[i] = -INFINITY;
+            int total = 0;  // listed candidates
+            for (int e0 = 0; e0 < nLists; e0 += 4) {
+                Cand2 cs[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {  // four lists in flight
+                    int const e = e0 + u;
+                    cs[u] = Cand2{-INFINITY, -1, -INFINITY, -1};
+                    if (e < nLists) cs[u] = lp[(size_t)(e >> 1) * ld2 + (e & 1)];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (cs[u].j1 >= 0) {
+                        topk_insert_pair(t, jj, cs[u].v1, cs[u].j1);
+                        ++total;
+                    }
+                    if (cs[u].j2 >= 0) {
+                        topk_insert_pair(t, jj, cs[u].v2, cs[u].j2);
+                        ++total;
+                    }
+                }
+            }
+            float kth = t[0];
+#pragma unroll
+            for (int i = 1; i < kSelK; ++i) kth = (i == topK - 1) ? t[i] : kth;
+            float const thr = fmaxf(t[0] - margin, kth);
+#pragma unroll
+            for (int i = 0; i < kSelK; ++i) c += (jj[i] >= 0 && t[i] >= thr) ? 1 : 0;
+            n = c;  // candidates scored here (an overflowing query's further ties uncounted)
+            if (!certify || (c == kSelK && total > kSelK)) {
+                d = kFlag;
+                c = 0;
+                open[atomicAdd(nOpen, 1)] = bi.blk * 256 + base + tid;
+            } else if (c > 0) {
+                d = kFlag;  // decided by the scoring below
+            }
+        }
+        if (c == 0) sel[bi.blk * 256 + base + tid] = d;
+        // slots past c repeat slot 0 (read, not used, by the scoring)
+        sCnt[tid] = c;
+#pragma unroll
+        for (int i = 0; i < kSelK; ++i) sJ[tid][i] = i < c ? jj[i] : jj[0];
+    }
+    __syncthreads();  // the selections in LDS
+    add_stripe(n, stripes);
+    // ---- scoring: four lanes per query
+    int const k = tid & 3;
+    int const qi = tid >> 2;  // query base + qi
+    int const c = sCnt[qi];   // uniform over the quad
+    if (c <= 0) return;       // whole quads leave: the exchanges stay inside quads
+    int const g = bi.blk * 256 + base + qi;
+    int const row = bi.rowBase + base + qi;
     uint8_t const* __restrict__ cRows = b.u8 + (size_t)bi.cBase * kDim;  // uniform
     float const* __restrict__ cDen = b.denom + bi.cBase;
-    int4 const s0 = slots[2 * (size_t)g], s1 = slots[2 * (size_t)g + 1];
-    int const js[kSelK] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
-    uint4 const qv = reinterpret_cast<uint4 const*>(b.u8 + (size_t)row * kDim)[k];
-    float const ra = b.denom[row];
-    // chunk k of each slot's row (32-bit offsets from the uniform image base); slots past
-    // cnt repeat slot 0's row (cache hits), and their sums only reach lanes that are not used
-    uint4 cv[kSelK];
+    int js[kSelK];
 #pragma unroll
-    for (int i = 0; i < kSelK; ++i)
-        cv[i] = *reinterpret_cast<uint4 const*>(cRows + ((unsigned)js[i] * (unsigned)kDim + 16u * k));
-    int const jk = k < c ? js[k] : -1;  // the candidate this lane finishes
-    float const rb = jk >= 0 ? cDen[jk] : 0.0f;
+    for (int i = 0; i < kSelK; ++i) js[i] = sJ[qi][i];
+    uint4 const* qp = reinterpret_cast<uint4 const*>(b.u8 + (size_t)row * kDim + 32 * k);
+    uint4 const q0 = qp[0], q1 = qp[1];
+    float const ra = b.denom[row];
+    uint4 cv[kSelK][2];
+#pragma unroll
+    for (int i = 0; i < kSelK; ++i) {
+        uint4 const* cp = reinterpret_cast<uint4 const*>(cRows + ((unsigned)js[i] * (unsigned)kDim + 32u * k));
+        cv[i][0] = cp[0];
+        cv[i][1] = cp[1];
+    }
+    int const ja = 2 * k < c ? js[2 * k] : -1;  // the candidates this lane finishes
+    int const jb = 2 * k + 1 < c ? js[2 * k + 1] : -1;
+    float const rba = ja >= 0 ? cDen[ja] : 0.0f;
+    float const rbb = jb >= 0 ? cDen[jb] : 0.0f;
     unsigned p[kSelK];
 #pragma unroll
     for (int i = 0; i < kSelK; ++i) {
-        unsigned acc = __builtin_amdgcn_udot4(qv.x, cv[i].x, 0u, false);
-        acc = __builtin_amdgcn_udot4(qv.y, cv[i].y, acc, false);
-        acc = __builtin_amdgcn_udot4(qv.z, cv[i].z, acc, false);
-        p[i] = __builtin_amdgcn_udot4(qv.w, cv[i].w, acc, false);
+        unsigned acc = __builtin_amdgcn_udot4(q0.x, cv[i][0].x, 0u, false);
+        acc = __builtin_amdgcn_udot4(q0.y, cv[i][0].y, acc, false);
+        acc = __builtin_amdgcn_udot4(q0.z, cv[i][0].z, acc, false);
+        acc = __builtin_amdgcn_udot4(q0.w, cv[i][0].w, acc, false);
+        acc = __builtin_amdgcn_udot4(q1.x, cv[i][1].x, acc, false);
+        acc = __builtin_amdgcn_udot4(q1.y, cv[i][1].y, acc, false);
+        acc = __builtin_amdgcn_udot4(q1.z, cv[i][1].z, acc, false);
+        p[i] = __builtin_amdgcn_udot4(q1.w, cv[i][1].w, acc, false);
     }
-    // transposed reduction: step 1 with lane 7 - k (other half: k < 4 keeps candidates
-    // 0..3), step 2 with k ^ 2 (bit 1 of k picks the pair), step 3 with k ^ 1
-    unsigned r4[4], r2[2];
-    bool const lo4 = k < 4;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        unsigned const keep = lo4 ? p[i] : p[4 + i];
-        unsigned const send = lo4 ? p[4 + i] : p[i];
-        r4[i] = keep + dpp_u<kDppMirror8>(send);
-    }
+    // step 1 with lane k ^ 2 (bit 1 of k: candidates 0..3 or 4..7), step 2 with k ^ 1
+    // (bit 0: the first or the second pair of those): lane k ends with candidates 2k, 2k+1
+    unsigned r4[4];
     bool const lo2 = (k & 2) == 0;
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        unsigned const keep = lo2 ? r4[i] : r4[2 + i];
-        unsigned const send = lo2 ? r4[2 + i] : r4[i];
-        r2[i] = keep + dpp_u<kDppXor2>(send);
+    for (int i = 0; i < 4; ++i) {
+        unsigned const keep = lo2 ? p[i] : p[4 + i];
+        unsigned const send = lo2 ? p[4 + i] : p[i];
+        r4[i] = keep + dpp_u<kDppXor2>(send);
     }
     bool const lo1 = (k & 1) == 0;
-    unsigned const keep = lo1 ? r2[0] : r2[1];
-    unsigned const send = lo1 ? r2[1] : r2[0];
-    unsigned const dot = keep + dpp_u<kDppXor1>(send);  // candidate k's product
+    unsigned dot[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        unsigned const keep = lo1 ? r4[i] : r4[2 + i];
+        unsigned const send = lo1 ? r4[2 + i] : r4[i];
+        dot[i] = keep + dpp_u<kDppXor1>(send);
+    }
     Top2s s = {kInit, kInit, -1};
-    if (jk >= 0) {
-        float const sc = mul_nc(mul_nc(ra, rb), (float)dot);
-        if (sc == sc) s = Top2s{sc, kInit, jk};
+    if (ja >= 0) {
+        float const sc = mul_nc(mul_nc(ra, rba), (float)dot[0]);
+        if (sc == sc) s = Top2s{sc, kInit, ja};
+    }
+    if (jb >= 0) {
+        float const sc = mul_nc(mul_nc(ra, rbb), (float)dot[1]);
+        if (sc == sc) s = top2s_merge(s, Top2s{sc, kInit, jb});
     }
     s = top2s_merge(s, top2s_dpp<kDppXor1>(s));
     s = top2s_merge(s, top2s_dpp<kDppXor2>(s));
-    s = top2s_merge(s, top2s_dpp<kDppMirror8>(s));
     if (k == 0) {
         int const d = decide_certified(Top2{s.b1, s.b2, s.a1}, kCertU8 * fmaxf(s.b1, 0.0f) + kEpsAbs, minScore,
                                        minRatio);
@@ -719,15 +712,12 @@ __global__ __launch_bounds__(256) void k_batch_shortlist_finish(BatchDev b, Cand
 }
 
 void launch_batch_rescore(BatchDev const& b, Cand2 const* part, float winRel, int topK, float margin, float minScore,
-                          float minRatio, bool certify, int* sel, int* open, int* nOpen, int* selCnt, int* selSlots,
-                          int* stripes, hipStream_t st)
+                          float minRatio, bool certify, int* sel, int* open, int* nOpen, int* stripes, hipStream_t st)
 {
     if (b.qRows <= 0) return;
-    if (b.u8 && topK >= 1 && topK <= kSelK && selCnt && selSlots) {
-        hipLaunchKernelGGL(k_batch_select_lane, dim3((b.qRows + 255) / 256), dim3(256), 0, st, b, part, topK, margin,
-                           certify, sel, selCnt, reinterpret_cast<int4*>(selSlots), open, nOpen, stripes);
-        hipLaunchKernelGGL(k_batch_score8, dim3((b.qRows + 31) / 32), dim3(256), 0, st, b, selCnt,
-                           reinterpret_cast<int4 const*>(selSlots), minScore, minRatio, sel, open, nOpen);
+    if (b.u8 && topK >= 1 && topK <= kSelK) {
+        hipLaunchKernelGGL(k_batch_rescore_learned, dim3((b.qRows + 63) / 64), dim3(256), 0, st, b, part, topK, margin,
+                           minScore, minRatio, certify, sel, open, nOpen, stripes);
         hipLaunchKernelGGL(k_batch_rescore_open, dim3(256), dim3(256), 0, st, b, part, winRel, topK, margin, minScore,
                            minRatio, sel, open, nOpen);
     } else if (b.u8) {
