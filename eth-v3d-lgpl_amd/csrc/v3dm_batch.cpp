@@ -687,11 +687,11 @@ int ensure_plan(v3dm_context* h, BatchState* b, int form)
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     auto make = [&](int tiles, int grid) {
-        return form == 0 ? plan_plain(b, tiles, grid)
-                         : plan_short_batch(b, form == 2 ? kCodeSplitTiles : tiles, grid, form == 2 ? 8 : 16, form == 2);
+        return form == 0 ? plan_plain(b, tiles, grid) : plan_short_batch(b, tiles, grid, form == 2 ? 8 : 16, form == 2);
     };
-    // the learned lists' split size is fixed (the result does not depend on tuning): only the
-    // grid is tuned for the code pass
+    // the learned lists' split size (at least kCodeSplitTiles, or the size set with
+    // v3dm_batch_set_tuning) is not tuned: it sets which candidates are listed, so the
+    // result would depend on tuning; only the grid is tuned for the code pass
     std::vector<int> tileCands = form == 0   ? std::vector<int>{8, 16, 32, 64}
                                  : form == 1 ? std::vector<int>{4, 8, 16, 32}
                                              : std::vector<int>{kCodeSplitTiles};

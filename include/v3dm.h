@@ -292,7 +292,9 @@ int v3dm_batch_match(v3dm_handle h, float minScore, float minRatio, int32_t* off
                      int* nTotal);
 /* Work-item tuning of the persistent kernels: autotune != 0 measures the candidate
  * (tiles per item, grid) settings on the batch at its first run; tilesPerItem / grid > 0
- * fix them for the form of `mode` instead. */
+ * fix them for the form of `mode` instead.  In the learned form tilesPerItem is the minimum
+ * split size of the candidate lists (default 32 tiles of 64 candidates, at most 8 splits
+ * per query): it changes which candidates are listed, so it is never tuned. */
 int v3dm_batch_set_tuning(v3dm_handle h, int autotune, int mode, int tilesPerItem, int grid);
 /* The setting in use for `mode` and the autotuner's measured pass time (-1 if not tuned). */
 int v3dm_batch_get_tuning(v3dm_handle h, int mode, int* tilesPerItem, int* grid, int* nItems, float* passMs);
