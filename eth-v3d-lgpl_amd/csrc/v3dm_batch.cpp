@@ -49,6 +49,7 @@ struct Plan {
     size_t partSlots = 0;
     int tiles = 0;  // tiles per item asked for
     int grid = 0;
+    int qRows = kBatchRowAlign;  // queries per work-group (shortlist plans: the query blocks)
     float passMs = -1.0f;  // autotuner measurement
 };
 
@@ -150,7 +151,7 @@ struct BatchState {
 
     // autotuning and graphs
     bool autotune = true;
-    int forceTiles[3] = {0, 0, 0}, forceGrid[3] = {0, 0, 0};
+    int forceTiles[3] = {0, 0, 0}, forceGrid[3] = {0, 0, 0}, forceQRows[3] = {0, 0, 0};
     struct GraphKey {
         int mode;
         float minScore, minRatio, epsRel;
@@ -323,11 +324,12 @@ Plan plan_plain(BatchState* b, int tiles, int grid)
 // candidate tiles per item and at most maxSplits (<= kShortMaxSplits) splits per query.
 // wholeRows: one item per (segment, query block) that runs every split of the row in turn
 // (the learned code pass); else one item per split.
-Plan plan_short_batch(BatchState* b, int tiles, int grid, int maxSplits, bool wholeRows)
+Plan plan_short_batch(BatchState* b, int tiles, int grid, int maxSplits, bool wholeRows, int qRows)
 {
     Plan pl;
     pl.tiles = tiles;
     pl.grid = grid;
+    pl.qRows = qRows;
     layout_segments(b, pl.seg);
     size_t off = 0;
     std::vector<std::pair<int, int4>> work;
@@ -345,7 +347,7 @@ Plan plan_short_batch(BatchState* b, int tiles, int grid, int maxSplits, bool wh
         s.nParts = splits;
         s.partOff = (int)off;
         off += (size_t)splits * s.ld;
-        int const qBlocks = (s.n + 255) / 256;
+        int const qBlocks = (s.n + qRows - 1) / qRows;  // qRows divides kBatchRowAlign: blocks stay in ld
         for (int qb = 0; qb < qBlocks; ++qb) {
             if (wholeRows) {
                 work.push_back({nTiles, make_int4(si, qb, 0, tps)});
@@ -602,7 +604,8 @@ int enqueue_run(v3dm_context* h, BatchState* b, int mode, float minScore, float 
         launch_batch_rescore(d, b->partB, kWinRelBf16, 0, INFINITY, minScore, minRatio, h->epsRel > 0.0f, b->sel,
                              b->openList, b->counters + kBCntOpenRescore, b->stripes, st);
     } else {
-        launch_batch_pass_code(d, b->items[2], (int)pl.items.size(), b->counters + 2, b->partB, pl.grid, st);
+        launch_batch_pass_code(d, b->items[2], (int)pl.items.size(), b->counters + 2, b->partB, pl.grid, pl.qRows,
+                               st);
         launch_batch_rescore(d, b->partB, 0.0f, h->learnedTopK, learned_margin(h), minScore, minRatio,
                              h->epsRel > 0.0f, b->sel, b->openList, b->counters + kBCntOpenRescore, b->stripes, st);
         if (h->fallbackSamples > 0) {
@@ -666,7 +669,8 @@ int time_pass(v3dm_context* h, BatchState* b, int form, Plan const& pl, float& m
         else if (form == 1)
             launch_batch_pass_bf16(d, b->items[1], (int)pl.items.size(), b->counters + 1, b->partB, pl.grid, st);
         else
-            launch_batch_pass_code(d, b->items[2], (int)pl.items.size(), b->counters + 2, b->partB, pl.grid, st);
+            launch_batch_pass_code(d, b->items[2], (int)pl.items.size(), b->counters + 2, b->partB, pl.grid, pl.qRows,
+                               st);
         BCHECK(h, hipEventRecord(b->evB, st));
         BCHECK(h, hipEventSynchronize(b->evB));
         float t = 0.0f;
@@ -677,8 +681,9 @@ int time_pass(v3dm_context* h, BatchState* b, int form, Plan const& pl, float& m
     return V3DM_OK;
 }
 
-// Choose (tiles per item, persistent grid) for a form: the given override, or the fastest
-// candidate measured on this batch, or the default when tuning is off.
+// Choose (tiles per item, persistent grid, and for the code pass the work-group's tile shape:
+// 128 or 256 queries against 64-candidate tiles) for a form: the given override, or the
+// fastest candidate measured on this batch, or the default when tuning is off.
 int ensure_plan(v3dm_context* h, BatchState* b, int form)
 {
     if (b->planValid[form]) return V3DM_OK;
@@ -686,37 +691,46 @@ int ensure_plan(v3dm_context* h, BatchState* b, int form)
     (void)hipGetDevice(&dev);
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    auto make = [&](int tiles, int grid) {
-        return form == 0 ? plan_plain(b, tiles, grid) : plan_short_batch(b, tiles, grid, form == 2 ? 8 : 16, form == 2);
+    auto make = [&](int tiles, int grid, int qRows) {
+        return form == 0 ? plan_plain(b, tiles, grid)
+                         : plan_short_batch(b, tiles, grid, form == 2 ? 8 : 16, form == 2, qRows);
     };
     // the learned lists' split size (at least kCodeSplitTiles, or the size set with
     // v3dm_batch_set_tuning) is not tuned: it sets which candidates are listed, so the
-    // result would depend on tuning; only the grid is tuned for the code pass
+    // result would depend on tuning; the code pass's tile shape and grid are
     std::vector<int> tileCands = form == 0   ? std::vector<int>{8, 16, 32, 64}
                                  : form == 1 ? std::vector<int>{4, 8, 16, 32}
                                              : std::vector<int>{kCodeSplitTiles};
-    // the code pass runs 4 work-groups per CU, the others 2
-    std::vector<int> gridCands = form == 2 ? std::vector<int>{2 * cus, 4 * cus} : std::vector<int>{cus, 2 * cus};
-    int bestT = form == 0 ? 16 : form == 1 ? 8 : kCodeSplitTiles, bestG = 2 * cus;
+    std::vector<int> shapeCands = form == 2 ? std::vector<int>{256, 128} : std::vector<int>{kBatchRowAlign};
+    // persistent grids: the code pass runs 4 work-groups of 4 waves per CU (8 of 2 waves),
+    // the others 2
+    auto gridCands = [&](int qRows) {
+        int const perCU = form == 2 ? (qRows == 128 ? 4 : 2) : 1;
+        return std::vector<int>{perCU * cus, 2 * perCU * cus};
+    };
+    int bestT = form == 0 ? 16 : form == 1 ? 8 : kCodeSplitTiles, bestG = 2 * cus, bestQ = kBatchRowAlign;
     float bestMs = -1.0f;
-    if (b->forceTiles[form] > 0 || b->forceGrid[form] > 0) {
+    if (b->forceTiles[form] > 0 || b->forceGrid[form] > 0 || b->forceQRows[form] > 0) {
         bestT = b->forceTiles[form] > 0 ? b->forceTiles[form] : bestT;
-        bestG = b->forceGrid[form] > 0 ? b->forceGrid[form] : bestG;
+        bestQ = b->forceQRows[form] > 0 ? b->forceQRows[form] : bestQ;
+        bestG = b->forceGrid[form] > 0 ? b->forceGrid[form] : gridCands(bestQ)[0];
     } else if (b->autotune) {
-        for (int t : tileCands)
-            for (int g : gridCands) {
-                float ms = 0.0f;
-                int const rc = time_pass(h, b, form, make(t, g), ms);
-                if (rc) return rc;
-                if (bestMs < 0.0f || ms < bestMs) {
-                    bestMs = ms;
-                    bestT = t;
-                    bestG = g;
+        for (int q : shapeCands)
+            for (int t : tileCands)
+                for (int g : gridCands(q)) {
+                    float ms = 0.0f;
+                    int const rc = time_pass(h, b, form, make(t, g, q), ms);
+                    if (rc) return rc;
+                    if (bestMs < 0.0f || ms < bestMs) {
+                        bestMs = ms;
+                        bestT = t;
+                        bestG = g;
+                        bestQ = q;
+                    }
                 }
-            }
     }
     drop_graphs(b);
-    Plan pl = make(bestT, bestG);
+    Plan pl = make(bestT, bestG, bestQ);
     pl.passMs = bestMs;
     int const rc = upload_plan(h, b, form, pl);
     if (rc) return rc;
@@ -972,6 +986,35 @@ int v3dm_batch_set_tuning(v3dm_handle h, int autotune, int mode, int tilesPerIte
     return V3DM_OK;
 }
 
+int v3dm_batch_set_tile_shape(v3dm_handle h, int mode, int queryRows)
+{
+    if (!h) return V3DM_ERR_INVALID;
+    BatchState* b = ctx_batch(h);
+    if (!b) return ctx_fail(h, V3DM_ERR_STATE, "set the batch images first");
+    if (mode < 0 || mode > V3DM_PREC_LEARNED) return ctx_fail(h, V3DM_ERR_INVALID, "invalid mode");
+    int const form = form_of(mode);
+    bool const ok = queryRows == 0 || (form == 2 ? (queryRows == 128 || queryRows == 256)
+                                                  : queryRows == (form == 0 ? 128 : 256));
+    if (!ok) return ctx_fail(h, V3DM_ERR_INVALID, "tile shape not available for this mode");
+    b->forceQRows[form] = form == 2 ? queryRows : 0;
+    b->planValid[form] = false;
+    drop_graphs(b);
+    return V3DM_OK;
+}
+
+int v3dm_batch_get_tile_shape(v3dm_handle h, int mode, int* queryRows, int* candidateRows)
+{
+    if (!h) return V3DM_ERR_INVALID;
+    BatchState* b = ctx_batch(h);
+    if (!b) return ctx_fail(h, V3DM_ERR_STATE, "no batch");
+    int const form = form_of(mode);
+    if (!b->planValid[form]) return ctx_fail(h, V3DM_ERR_STATE, "no plan for this mode yet");
+    // plain: 128 left features x 64-row right tiles; shortlists: query block x 64 candidates
+    if (queryRows) *queryRows = form == 0 ? 128 : b->plan[form].qRows;
+    if (candidateRows) *candidateRows = 64;
+    return V3DM_OK;
+}
+
 int v3dm_batch_get_tuning(v3dm_handle h, int mode, int* tilesPerItem, int* grid, int* nItems, float* passMs)
 {
     if (!h) return V3DM_ERR_INVALID;
@@ -1138,7 +1181,7 @@ int v3dm_batch_time_dominant_kernel(v3dm_handle h, int reps, float* msPerLaunch)
         int const n = (int)pl.items.size();
         if (form == 0) launch_batch_fused_fp32(d, b->items[0], n, q + r, b->partF, pl.grid, st);
         else if (form == 1) launch_batch_pass_bf16(d, b->items[1], n, q + r, b->partB, pl.grid, st);
-        else launch_batch_pass_code(d, b->items[2], n, q + r, b->partB, pl.grid, st);
+        else launch_batch_pass_code(d, b->items[2], n, q + r, b->partB, pl.grid, pl.qRows, st);
     }
     if (e == hipSuccess) e = hipEventRecord(b->evB, st);
     if (e == hipSuccess) e = hipEventSynchronize(b->evB);

@@ -125,9 +125,10 @@ void launch_batch_pass_bf16(BatchDev const& b, int4 const* items, int nItems, in
 // exactly); FP32 images: exact scores
 void launch_batch_rescore(BatchDev const& b, Cand2 const* part, float winRel, int topK, float margin, float minScore,
                           float minRatio, bool certify, int* sel, int* open, int* nOpen, int* stripes, hipStream_t st);
-// learned code pass (v3dm_code.hip)
+// learned code pass (v3dm_code.hip); queryRows: queries per work-group, 128 or 256 (the
+// plan's query blocks)
 void launch_batch_pass_code(BatchDev const& b, int4 const* items, int nItems, int* queue, Cand2* part, int grid,
-                            hipStream_t st);
+                            int queryRows, hipStream_t st);
 // selection and bookkeeping (v3dm_batch.hip)
 void launch_batch_reset(int* counters, int* pairCount, int nPairs, hipStream_t st);
 void launch_batch_norms(float const* img, int const* rowImg, int rows, float* norm, unsigned int* maxNorm,

@@ -21,7 +21,7 @@ namespace v3dm {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-constexpr int kCQ = 256;   // queries per work-group (4 waves x 2 blocks of 32)
+constexpr int kCQ = 256;   // queries per work-group of the single-pair pass (4 waves x 2 blocks of 32)
 constexpr int kCT = 64;    // candidates per tile (2 blocks of 32)
 constexpr int kCodeBytes = 32;
 
@@ -91,7 +91,7 @@ __device__ __forceinline__ void code_tile(v4i const (&a)[2], int t, v4i const (&
 // One work unit: query block qblk (256 queries) against candidate tiles [split *
 // tilesPerSplit, ...) -> part[((split * ldq) + q) * 2 + h]: code scores as floats and
 // candidate indices (Cand2), or packed score and index inside the split (CandP).
-template <typename List>
+template <int kQueries = kCQ, typename List>
 __device__ __forceinline__ void code_block(signed char const* __restrict__ Q, int nQ,
                                            signed char const* __restrict__ C, int nC, int qblk, int split,
                                            int tilesPerSplit, List* __restrict__ part, int ldq)
@@ -100,7 +100,7 @@ __device__ __forceinline__ void code_block(signed char const* __restrict__ Q, in
     int const wave = threadIdx.x >> 6;
     int const h = lane >> 5;
     int const c32 = lane & 31;
-    int const q0 = qblk * kCQ + wave * 64;
+    int const q0 = qblk * kQueries + wave * 64;
     int const nTiles = (nC + kCT - 1) / kCT;
     int const t0 = split * tilesPerSplit;
     int const t1 = min(t0 + tilesPerSplit, nTiles);
@@ -192,9 +192,12 @@ __global__ __launch_bounds__(256, 4) void k_pass_code2(CodeDir d0, CodeDir d1)
 }
 
 // Persistent form over a batch: item = (segment, query block, first split, tiles per split);
-// a work-group runs the splits of the item's row from the first one on.
-__global__ __launch_bounds__(256, 4) void k_batch_pass_code(BatchDev b, int4 const* __restrict__ items, int nItems,
-                                                            int* __restrict__ queue, Cand2* __restrict__ part)
+// a work-group runs the splits of the item's row from the first one on.  kQW waves of 64
+// queries per work-group: the query block is kQW * 64 rows (the plan's tile shape).
+template <int kQW>
+__global__ __launch_bounds__(64 * kQW, 4) void k_batch_pass_code(BatchDev b, int4 const* __restrict__ items,
+                                                                        int nItems, int* __restrict__ queue,
+                                                                        Cand2* __restrict__ part)
 {
     __shared__ int sItem;
     for (;;) {
@@ -208,15 +211,18 @@ __global__ __launch_bounds__(256, 4) void k_batch_pass_code(BatchDev b, int4 con
         signed char const* Q = b.codes + (size_t)b.imgBase[sg.img] * kCodeBytes;
         signed char const* C = b.codes + (size_t)b.imgBase[sg.oImg] * kCodeBytes;
         for (int sp = w.z; sp < sg.nParts; ++sp)  // the item: every split of its row, in turn
-            code_block(Q, sg.n, C, sg.nO, w.y, sp, w.w, part + (size_t)sg.partOff * 2, sg.ld);
+            code_block<kQW * 64>(Q, sg.n, C, sg.nO, w.y, sp, w.w, part + (size_t)sg.partOff * 2, sg.ld);
     }
 }
 
 void launch_batch_pass_code(BatchDev const& b, int4 const* items, int nItems, int* queue, Cand2* part, int grid,
-                            hipStream_t st)
+                            int queryRows, hipStream_t st)
 {
     if (nItems <= 0) return;
-    hipLaunchKernelGGL(k_batch_pass_code, dim3(grid), dim3(256), 0, st, b, items, nItems, queue, part);
+    if (queryRows == 128)
+        hipLaunchKernelGGL(k_batch_pass_code<2>, dim3(grid), dim3(128), 0, st, b, items, nItems, queue, part);
+    else
+        hipLaunchKernelGGL(k_batch_pass_code<4>, dim3(grid), dim3(256), 0, st, b, items, nItems, queue, part);
 }
 
 void launch_pass_code2(signed char const* A, int nA, signed char const* B, int nB, ShortPlan const& pa,

@@ -67,6 +67,8 @@ SIGNATURES = {
     "v3dm_fallback_allowed_misses": (_I, [_I, _F]),
     "v3dm_fallback_allowed_misses_binomial": (_I, [_I, _F]),
     "v3dm_set_fallback_rule": (_I, [_P, _I]),
+    "v3dm_batch_set_tile_shape": (_I, [_P, _I, _I]),
+    "v3dm_batch_get_tile_shape": (_I, [_P, _I, _IP, _IP]),
     "v3dm_debug_mutual_wait": (_I, [_P, _I, _I]),
     "v3dm_set_left_device": (_I, [_P, _I, _FP]),
     "v3dm_set_right_device": (_I, [_P, _I, _FP]),

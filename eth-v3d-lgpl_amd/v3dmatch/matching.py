@@ -309,6 +309,16 @@ class SIFT_Matching:
         self._check(self._lib.v3dm_batch_set_tuning(self._h, int(bool(autotune)), int(mode), int(tiles_per_item),
                                                      int(grid)))
 
+    def batch_set_tile_shape(self, mode, query_rows=0):
+        """The learned code pass's work-group shape: 128 or 256 queries per 64-candidate tile
+        (0: chosen by the autotuner)."""
+        self._check(self._lib.v3dm_batch_set_tile_shape(self._h, int(mode), int(query_rows)))
+
+    def batch_get_tile_shape(self, mode=0):
+        q, c = C.c_int(0), C.c_int(0)
+        self._check(self._lib.v3dm_batch_get_tile_shape(self._h, int(mode), C.byref(q), C.byref(c)))
+        return {"query_rows": q.value, "candidate_rows": c.value}
+
     def batch_get_tuning(self, mode=0):
         t, g, n, ms = C.c_int(0), C.c_int(0), C.c_int(0), C.c_float(0)
         self._check(self._lib.v3dm_batch_get_tuning(self._h, int(mode), C.byref(t), C.byref(g), C.byref(n),

@@ -177,6 +177,35 @@ def test_learned_batch(matcher):
     m.set_precision(v3dmatch.PREC_FP32)
 
 
+def test_learned_batch_tile_shapes(matcher):
+    """Every work-group shape of the learned code pass (128 or 256 queries per 64-candidate
+    tile) lists the same candidates: identical match sets, with the check on; the shape the
+    tuner chose is reported."""
+    m, net = matcher
+    counts, desc, left, right, _ = synth.make_batch(24, lo=300, hi=6000, config=43)
+    m.batch_set_images(counts, desc)
+    m.batch_set_pairs(left, right)
+    m.set_precision(v3dmatch.PREC_LEARNED)
+    res = {}
+    for q in (256, 128):
+        m.batch_set_tile_shape(v3dmatch.PREC_LEARNED, q)
+        res[q] = m.batch_match(0.9, 0.8)
+        shape = m.batch_get_tile_shape(v3dmatch.PREC_LEARNED)
+        say(f"case learned tile shape {q}: {shape}, {sum(len(r) for r in res[q])} matches")
+        assert shape == {"query_rows": q, "candidate_rows": 64}
+    assert all(np.array_equal(a, b) for a, b in zip(res[256], res[128]))
+    m.batch_set_tile_shape(v3dmatch.PREC_LEARNED, 0)
+    m.batch_set_tuning(autotune=True)
+    tuned = m.batch_match(0.9, 0.8)
+    say(f"case learned tile shape tuned: {m.batch_get_tile_shape(v3dmatch.PREC_LEARNED)} "
+        f"{m.batch_get_tuning(v3dmatch.PREC_LEARNED)}")
+    assert m.batch_get_tile_shape(v3dmatch.PREC_LEARNED)["query_rows"] in (128, 256)
+    assert all(np.array_equal(a, b) for a, b in zip(res[256], tuned))
+    with pytest.raises(v3dmatch.V3DMError):
+        m.batch_set_tile_shape(v3dmatch.PREC_LEARNED, 512)
+    m.set_precision(v3dmatch.PREC_FP32)
+
+
 def test_sampled_check_and_fallback(matcher):
     """Every learned call decides 128 sampled left features exactly; with the trained
     network the check passes (no fallback), with a poor shortlist (untrained network, one
