@@ -619,6 +619,62 @@ __device__ __forceinline__ __bf16 const* mlp_operand(__bf16 const* W1s, __bf16 c
     return W2s + ((t - 72) * 64 + lane) * 8;
 }
 
+// The network -> LDS (one work-group of kMlpWaves waves): W1 rows padded to kW1Stride with
+// the bias split (bh, bh, bl) in the pad, W2 in its operand order.  All loads of a thread are
+// issued before its first store.
+__device__ __forceinline__ void mlp_stage_network(__bf16* W1s, __bf16* W2s, __bf16 const* __restrict__ W1b,
+                                                  float const* __restrict__ b1, __bf16 const* __restrict__ W2p,
+                                                  int tid)
+{
+    constexpr int kW1Chunks = kHid * 16 / (64 * kMlpWaves), kW2Chunks = kHid * kCode / 8 / (64 * kMlpWaves);
+    uint4 w[kW1Chunks], w2[kW2Chunks];
+#pragma unroll
+    for (int i = 0; i < kW1Chunks; ++i) {
+        int const f = tid + 64 * kMlpWaves * i;
+        w[i] = reinterpret_cast<uint4 const*>(W1b + (size_t)(f >> 4) * 128)[f & 15];
+    }
+#pragma unroll
+    for (int i = 0; i < kW2Chunks; ++i) w2[i] = reinterpret_cast<uint4 const*>(W2p)[tid + 64 * kMlpWaves * i];
+    float const bv = tid < kHid ? b1[tid] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < kW1Chunks; ++i) {
+        int const f = tid + 64 * kMlpWaves * i;
+        *reinterpret_cast<uint4*>(&W1s[(f >> 4) * kW1Stride + (f & 15) * 8]) = w[i];
+    }
+#pragma unroll
+    for (int i = 0; i < kW2Chunks; ++i) reinterpret_cast<uint4*>(W2s)[tid + 64 * kMlpWaves * i] = w2[i];
+    if (tid < kHid) {
+        __bf16 const bh = (__bf16)bv;
+        __bf16 const bl = (__bf16)(bv - (float)bh);
+        __bf16 const zz = (__bf16)0.0f;
+        *reinterpret_cast<v8bf*>(&W1s[tid * kW1Stride + 128]) = v8bf{bh, bh, bl, zz, zz, zz, zz, zz};
+    }
+}
+
+// Normalise and quantise the 16 codes a lane holds (z: the unit's output plus bias) and store
+// them: bytes 16 h .. 16 h + 15 of the row's 32.
+__device__ __forceinline__ void mlp_store_codes(v16f const& z, float scale, int h, signed char* __restrict__ dst)
+{
+    float s2 = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) s2 += z[r] * z[r];
+    s2 += __shfl_xor(s2, 32);
+    float const inv = scale / fmaxf(sqrtf(s2), 1e-30f);
+    unsigned int w[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        unsigned int p = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            float const c = z[4 * g + e] * inv;
+            int const q = (c == c) ? (int)rintf(fminf(fmaxf(c, -31.0f), 31.0f)) : 0;
+            p |= ((unsigned int)(4 * q) & 0xffu) << (8 * e);
+        }
+        w[g] = p;
+    }
+    *reinterpret_cast<uint4*>(dst + 16 * h) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
 // kPrefetch: the next unit's inputs are loaded while the current unit is computed (the
 // uint8 rows take 17 registers, few enough to hold a second set).
 template <bool kU8, bool kPrefetch>
@@ -642,32 +698,7 @@ __device__ __forceinline__ void mlp_codes_body(MlpRows const& in0, int nUnits0, 
     int unit = (int)blockIdx.x * kMlpWaves + wave;
     MlpRaw<kU8> raw;
     if (kPrefetch && unit < nUnits) mlp_load(unit >= nUnits0 ? in1 : in0, rowOf(unit), h, raw);
-    // the network -> LDS; all loads of a thread are issued before its first store
-    {
-        constexpr int kW1Chunks = kHid * 16 / (64 * kMlpWaves), kW2Chunks = kHid * kCode / 8 / (64 * kMlpWaves);
-        uint4 w[kW1Chunks], w2[kW2Chunks];
-#pragma unroll
-        for (int i = 0; i < kW1Chunks; ++i) {
-            int const f = tid + 64 * kMlpWaves * i;
-            w[i] = reinterpret_cast<uint4 const*>(W1b + (size_t)(f >> 4) * 128)[f & 15];
-        }
-#pragma unroll
-        for (int i = 0; i < kW2Chunks; ++i) w2[i] = reinterpret_cast<uint4 const*>(W2p)[tid + 64 * kMlpWaves * i];
-        float const bv = tid < kHid ? b1[tid] : 0.0f;
-#pragma unroll
-        for (int i = 0; i < kW1Chunks; ++i) {
-            int const f = tid + 64 * kMlpWaves * i;
-            *reinterpret_cast<uint4*>(&W1s[(f >> 4) * kW1Stride + (f & 15) * 8]) = w[i];
-        }
-#pragma unroll
-        for (int i = 0; i < kW2Chunks; ++i) reinterpret_cast<uint4*>(W2s)[tid + 64 * kMlpWaves * i] = w2[i];
-        if (tid < kHid) {
-            __bf16 const bh = (__bf16)bv;
-            __bf16 const bl = (__bf16)(bv - (float)bh);
-            __bf16 const zz = (__bf16)0.0f;
-            *reinterpret_cast<v8bf*>(&W1s[tid * kW1Stride + 128]) = v8bf{bh, bh, bl, zz, zz, zz, zz, zz};
-        }
-    }
+    mlp_stage_network(W1s, W2s, W1b, b1, W2p, tid);
     __syncthreads();
     float bz[16];
 #pragma unroll
@@ -748,27 +779,9 @@ __device__ __forceinline__ void mlp_codes_body(MlpRows const& in0, int nUnits0, 
                 z = mfma_bf16(next(), hb1, z);
             }
         }
-        float s2 = 0.0f;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            z[r] = fmaf(bz[r], sc, z[r]);
-            s2 += z[r] * z[r];
-        }
-        s2 += __shfl_xor(s2, 32);
-        float const inv = scale / fmaxf(sqrtf(s2), 1e-30f);
-        unsigned int w[4];
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            unsigned int p = 0;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                float const c = z[4 * g + e] * inv;
-                int const q = (c == c) ? (int)rintf(fminf(fmaxf(c, -31.0f), 31.0f)) : 0;
-                p |= ((unsigned int)(4 * q) & 0xffu) << (8 * e);
-            }
-            w[g] = p;
-        }
-        *reinterpret_cast<uint4*>(codes + row * kCode + 16 * h) = make_uint4(w[0], w[1], w[2], w[3]);
+        for (int r = 0; r < 16; ++r) z[r] = fmaf(bz[r], sc, z[r]);
+        mlp_store_codes(z, scale, h, codes + row * kCode);
     }
 }
 
@@ -781,15 +794,116 @@ __global__ __launch_bounds__(64 * kMlpWaves, 1) void k_mlp_codes(
                                  nUnits1, codes1, W1b, b1, W2p, b2, scale);
 }
 
-// uint8 rows (the batch image): read at 128 bytes per row, the next unit's rows loaded during
-// the current unit
+// uint8 rows (the batch image), 64-row units: the wave's two 32-row groups share every LDS
+// operand read (88 reads of 16 bytes per 64 rows, half the LDS traffic of 32-row units).  Per
+// 32-unit block: the first layer's 18 MFMAs (each operand read once, used for both groups),
+// the ReLU of group 0, then the second layer's MFMAs of group 0 interleaved with group 1's
+// ReLU, then group 1's.  The next unit's rows are loaded during the current unit.
+__device__ __forceinline__ void mlp_codes_u8_body(MlpRows const& in, int nUnits, signed char* __restrict__ codes,
+                                                  __bf16 const* __restrict__ W1b, float const* __restrict__ b1,
+                                                  __bf16 const* __restrict__ W2p, float const* __restrict__ b2,
+                                                  float scale)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 W1s[kHid * kW1Stride];
+    __shared__ __attribute__((aligned(16))) __bf16 W2s[kHid * kCode];
+    __shared__ __attribute__((aligned(16))) float b2s[kCode];
+    int const tid = threadIdx.x;
+    int const lane = tid & 63;
+    int const wave = tid >> 6;
+    int const h = lane >> 5;
+    int const c32 = lane & 31;
+    int const stride = (int)gridDim.x * kMlpWaves;
+    int unit = (int)blockIdx.x * kMlpWaves + wave;
+    MlpRaw<true> raw[2];
+    if (unit < nUnits) {
+#pragma unroll
+        for (int g = 0; g < 2; ++g) mlp_load(in, (size_t)unit * 64 + 32 * g + c32, h, raw[g]);
+    }
+    mlp_stage_network(W1s, W2s, W1b, b1, W2p, tid);
+    if (tid < kCode) b2s[tid] = b2[tid];
+    __syncthreads();
+
+    for (; unit < nUnits; unit += stride) {
+        v8bf xb[2][8];
+        v8bf xbias[2];
+        float sc[2];
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+            mlp_convert(raw[g], xb[g]);
+            sc[g] = mlp_bias_scale(raw[g]);
+            __bf16 const sh = (__bf16)sc[g];
+            __bf16 const sl = (__bf16)(sc[g] - (float)sh);
+            __bf16 const zz = (__bf16)0.0f;
+            xbias[g] = h == 0 ? v8bf{sh, sl, sh, zz, zz, zz, zz, zz} : v8bf{zz, zz, zz, zz, zz, zz, zz, zz};
+        }
+        int const nxt = unit + stride;
+        if (nxt < nUnits) {
+#pragma unroll
+            for (int g = 0; g < 2; ++g) mlp_load(in, (size_t)nxt * 64 + 32 * g + c32, h, raw[g]);
+        }
+        // LDS operand ring in the order of use: per ub, the nine first-layer operands (W1
+        // ks = 0..7, the bias column), then the two second-layer ones; each step (the MFMAs of
+        // one operand, the read it frees room for, a share of the VALU work) is closed by a
+        // scheduling barrier so that the compiler keeps the interleaving
+        v8bf ring[kMlpRing];
+        auto order = [](int i) -> int {  // i-th operand used -> its LDS operand index
+            int const ub = i / 11, j = i % 11;
+            return j < 9 ? 9 * ub + j : 72 + 2 * ub + (j - 9);
+        };
+#pragma unroll
+        for (int i = 0; i < kMlpRing; ++i) ring[i] = load8(mlp_operand(W1s, W2s, order(i), c32, h, lane));
+        int used = 0;
+        auto next = [&]() -> v8bf {
+            v8bf const op = ring[used % kMlpRing];
+            if (used + kMlpRing < 88)
+                ring[used % kMlpRing] = load8(mlp_operand(W1s, W2s, order(used + kMlpRing), c32, h, lane));
+            ++used;
+            return op;
+        };
+        __builtin_amdgcn_sched_barrier(0);
+        v16f z[2] = {v16f{}, v16f{}};
+#pragma unroll
+        for (int ub = 0; ub < 8; ++ub) {
+            v16f a[2] = {v16f{}, v16f{}};
+#pragma unroll
+            for (int j = 0; j < 9; ++j) {
+                v8bf const op = next();
+                a[0] = mfma_bf16(op, j < 8 ? xb[0][j] : xbias[0], a[0]);
+                a[1] = mfma_bf16(op, j < 8 ? xb[1][j] : xbias[1], a[1]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            unsigned w0[8], w1[8];
+#pragma unroll
+            for (int g = 0; g < 8; ++g) w0[g] = relu_bf16x2(a[0][2 * g], a[0][2 * g + 1]);
+            v8bf const op0 = next();
+            v8bf const op1 = next();
+            z[0] = mfma_bf16(op0, __builtin_bit_cast(v8bf, (uint4){w0[0], w0[1], w0[2], w0[3]}), z[0]);
+#pragma unroll
+            for (int g = 0; g < 4; ++g) w1[g] = relu_bf16x2(a[1][2 * g], a[1][2 * g + 1]);
+            __builtin_amdgcn_sched_barrier(0);
+            z[0] = mfma_bf16(op1, __builtin_bit_cast(v8bf, (uint4){w0[4], w0[5], w0[6], w0[7]}), z[0]);
+#pragma unroll
+            for (int g = 4; g < 8; ++g) w1[g] = relu_bf16x2(a[1][2 * g], a[1][2 * g + 1]);
+            __builtin_amdgcn_sched_barrier(0);
+            z[1] = mfma_bf16(op0, __builtin_bit_cast(v8bf, (uint4){w1[0], w1[1], w1[2], w1[3]}), z[1]);
+            z[1] = mfma_bf16(op1, __builtin_bit_cast(v8bf, (uint4){w1[4], w1[5], w1[6], w1[7]}), z[1]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) z[g][r] = fmaf(b2s[8 * (r >> 2) + 4 * h + (r & 3)], sc[g], z[g][r]);
+            mlp_store_codes(z[g], scale, h, codes + ((size_t)unit * 64 + 32 * g + c32) * kCode);
+        }
+    }
+}
+
 __global__ __launch_bounds__(64 * kMlpWaves, 1) void k_mlp_codes_u8(
     uint8_t const* __restrict__ Q, float const* __restrict__ denom, int nUnits, signed char* __restrict__ codes,
     __bf16 const* __restrict__ W1b, float const* __restrict__ b1, __bf16 const* __restrict__ W2p,
     float const* __restrict__ b2, float scale)
 {
-    MlpRows const in{nullptr, Q, denom};
-    mlp_codes_body<true, true>(in, nUnits, codes, in, 0, codes, W1b, b1, W2p, b2, scale);
+    mlp_codes_u8_body(MlpRows{nullptr, Q, denom}, nUnits, codes, W1b, b1, W2p, b2, scale);
 }
 
 namespace {
@@ -821,7 +935,7 @@ void launch_mlp_codes_f32(float const* X, int rows128, MlpWeights const& w, floa
 void launch_mlp_codes_u8(uint8_t const* Q, float const* denom, int rows128, MlpWeights const& w, float scale,
                          signed char* codes, hipStream_t st)
 {
-    int const nu = rows128 / 32;
+    int const nu = rows128 / 64;  // 64-row units
     if (nu <= 0) return;
     hipLaunchKernelGGL(k_mlp_codes_u8, dim3(mlp_grid(nu)), dim3(64 * kMlpWaves), 0, st, Q, denom, nu, codes, w.W1b,
                        w.P + kOffB1, w.W2p, w.P + kOffB2, scale);
