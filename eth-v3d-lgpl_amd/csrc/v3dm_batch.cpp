@@ -276,11 +276,13 @@ void layout_segments(BatchState* b, std::vector<BatchSeg>& seg)
 
 // Plain FP32 plan: items (pair, left block, split, tiles) with `tiles` 64-row right tiles
 // per item; partials: left queries one per split, right queries one per left block.
-Plan plan_plain(BatchState* b, int tiles, int grid)
+// Plain plan: items (pair, left block of leftRows features, split, tiles per split).
+Plan plan_plain(BatchState* b, int tiles, int grid, int leftRows = 128)
 {
     Plan pl;
     pl.tiles = tiles;
     pl.grid = grid;
+    pl.qRows = leftRows;
     layout_segments(b, pl.seg);
     int const P = b->nPairs;
     size_t off = 0;
@@ -299,7 +301,7 @@ Plan plan_plain(BatchState* b, int tiles, int grid)
         }
         int const nTiles = (nR + 63) / 64;
         int const splits = (nTiles + tiles - 1) / tiles;
-        int const blocks = (nL + 127) / 128;
+        int const blocks = (nL + leftRows - 1) / leftRows;  // leftRows divides kBatchRowAlign
         sl.nParts = splits;
         sl.partOff = (int)off;
         off += (size_t)splits * padL;
@@ -596,11 +598,13 @@ int enqueue_run(v3dm_context* h, BatchState* b, int mode, float minScore, float 
     launch_batch_reset(b->counters, b->pairCount, b->nPairs, st);
     if (!h->cacheDerived) enqueue_derived(h, b, form);
     if (form == 0) {
-        launch_batch_fused_fp32(d, b->items[0], (int)pl.items.size(), b->counters + 0, b->partF, pl.grid, st);
+        launch_batch_fused_fp32(d, b->items[0], (int)pl.items.size(), b->counters + 0, b->partF, pl.grid, pl.qRows,
+                                st);
         launch_batch_merge_decide(d, b->partF, h->epsRel, minScore, minRatio, b->sel, b->flagList, b->counters, st);
         launch_batch_recheck(d, b->flagList, b->counters, minScore, minRatio, b->sel, 1024, st);
     } else if (form == 1) {
-        launch_batch_pass_bf16(d, b->items[1], (int)pl.items.size(), b->counters + 1, b->partB, pl.grid, st);
+        launch_batch_pass_bf16(d, b->items[1], (int)pl.items.size(), b->counters + 1, b->partB, pl.grid, pl.qRows,
+                               st);
         launch_batch_rescore(d, b->partB, kWinRelBf16, 0, INFINITY, minScore, minRatio, h->epsRel > 0.0f, b->sel,
                              b->openList, b->counters + kBCntOpenRescore, b->stripes, st);
     } else {
@@ -639,7 +643,7 @@ int enqueue_run(v3dm_context* h, BatchState* b, int mode, float minScore, float 
             BatchDev const d0 = dev_view(b, 3);
             launch_batch_fb_filter(b->items[3], np, b->pairFlag, b->nPairs, b->fbItems, b->counters + kBCntFbItems,
                                    b->segMask, st);
-            launch_batch_fused_fp32(d0, b->fbItems, np, b->counters + 0, b->partF, pp.grid, st,
+            launch_batch_fused_fp32(d0, b->fbItems, np, b->counters + 0, b->partF, pp.grid, pp.qRows, st,
                                     b->counters + kBCntFbItems);
             launch_batch_merge_masked(d0, b->partF, h->epsRel, minScore, minRatio, b->sel, b->flagList, b->counters,
                                       b->segMask, 2 * cus, st);
@@ -665,9 +669,11 @@ int time_pass(v3dm_context* h, BatchState* b, int form, Plan const& pl, float& m
         launch_batch_reset(b->counters, b->pairCount, b->nPairs, st);
         BCHECK(h, hipEventRecord(b->evA, st));
         if (form == 0)
-            launch_batch_fused_fp32(d, b->items[0], (int)pl.items.size(), b->counters + 0, b->partF, pl.grid, st);
+            launch_batch_fused_fp32(d, b->items[0], (int)pl.items.size(), b->counters + 0, b->partF, pl.grid,
+                                    pl.qRows, st);
         else if (form == 1)
-            launch_batch_pass_bf16(d, b->items[1], (int)pl.items.size(), b->counters + 1, b->partB, pl.grid, st);
+            launch_batch_pass_bf16(d, b->items[1], (int)pl.items.size(), b->counters + 1, b->partB, pl.grid,
+                                   pl.qRows, st);
         else
             launch_batch_pass_code(d, b->items[2], (int)pl.items.size(), b->counters + 2, b->partB, pl.grid, pl.qRows,
                                st);
@@ -681,9 +687,10 @@ int time_pass(v3dm_context* h, BatchState* b, int form, Plan const& pl, float& m
     return V3DM_OK;
 }
 
-// Choose (tiles per item, persistent grid, and for the code pass the work-group's tile shape:
-// 128 or 256 queries against 64-candidate tiles) for a form: the given override, or the
-// fastest candidate measured on this batch, or the default when tuning is off.
+// Choose (tiles per item, persistent grid, and the work-group's tile shape: 128 or 256 query
+// rows against 64-candidate tiles, i.e. 4 or 8 waves for the plain form, 2 or 4 waves for
+// the BF16 and code passes) for a form: the given override, or the fastest candidate
+// measured on this batch, or the default when tuning is off.
 int ensure_plan(v3dm_context* h, BatchState* b, int form)
 {
     if (b->planValid[form]) return V3DM_OK;
@@ -692,7 +699,7 @@ int ensure_plan(v3dm_context* h, BatchState* b, int form)
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     auto make = [&](int tiles, int grid, int qRows) {
-        return form == 0 ? plan_plain(b, tiles, grid)
+        return form == 0 ? plan_plain(b, tiles, grid, qRows)
                          : plan_short_batch(b, tiles, grid, form == 2 ? 8 : 16, form == 2, qRows);
     };
     // the learned lists' split size (at least kCodeSplitTiles, or the size set with
@@ -701,14 +708,15 @@ int ensure_plan(v3dm_context* h, BatchState* b, int form)
     std::vector<int> tileCands = form == 0   ? std::vector<int>{8, 16, 32, 64}
                                  : form == 1 ? std::vector<int>{4, 8, 16, 32}
                                              : std::vector<int>{kCodeSplitTiles};
-    std::vector<int> shapeCands = form == 2 ? std::vector<int>{256, 128} : std::vector<int>{kBatchRowAlign};
+    std::vector<int> shapeCands = form == 0 ? std::vector<int>{128, 256} : std::vector<int>{256, 128};
     // persistent grids: the code pass runs 4 work-groups of 4 waves per CU (8 of 2 waves),
-    // the others 2
+    // the BF16 pass 2 of 4 waves (4 of 2), the plain form 2 of 4 waves (1 of 8)
     auto gridCands = [&](int qRows) {
-        int const perCU = form == 2 ? (qRows == 128 ? 4 : 2) : 1;
+        int const perCU = form == 2 ? (qRows == 128 ? 4 : 2) : form == 1 ? (qRows == 128 ? 2 : 1) : 1;
         return std::vector<int>{perCU * cus, 2 * perCU * cus};
     };
-    int bestT = form == 0 ? 16 : form == 1 ? 8 : kCodeSplitTiles, bestG = 2 * cus, bestQ = kBatchRowAlign;
+    int bestT = form == 0 ? 16 : form == 1 ? 8 : kCodeSplitTiles, bestG = 2 * cus;
+    int bestQ = form == 0 ? 128 : kBatchRowAlign;
     float bestMs = -1.0f;
     if (b->forceTiles[form] > 0 || b->forceGrid[form] > 0 || b->forceQRows[form] > 0) {
         bestT = b->forceTiles[form] > 0 ? b->forceTiles[form] : bestT;
@@ -993,10 +1001,10 @@ int v3dm_batch_set_tile_shape(v3dm_handle h, int mode, int queryRows)
     if (!b) return ctx_fail(h, V3DM_ERR_STATE, "set the batch images first");
     if (mode < 0 || mode > V3DM_PREC_LEARNED) return ctx_fail(h, V3DM_ERR_INVALID, "invalid mode");
     int const form = form_of(mode);
-    bool const ok = queryRows == 0 || (form == 2 ? (queryRows == 128 || queryRows == 256)
-                                                  : queryRows == (form == 0 ? 128 : 256));
+    // every form: 128 or 256 query rows per work-group (plain: left features)
+    bool const ok = queryRows == 0 || queryRows == 128 || queryRows == 256;
     if (!ok) return ctx_fail(h, V3DM_ERR_INVALID, "tile shape not available for this mode");
-    b->forceQRows[form] = form == 2 ? queryRows : 0;
+    b->forceQRows[form] = queryRows;
     b->planValid[form] = false;
     drop_graphs(b);
     return V3DM_OK;
@@ -1009,8 +1017,8 @@ int v3dm_batch_get_tile_shape(v3dm_handle h, int mode, int* queryRows, int* cand
     if (!b) return ctx_fail(h, V3DM_ERR_STATE, "no batch");
     int const form = form_of(mode);
     if (!b->planValid[form]) return ctx_fail(h, V3DM_ERR_STATE, "no plan for this mode yet");
-    // plain: 128 left features x 64-row right tiles; shortlists: query block x 64 candidates
-    if (queryRows) *queryRows = form == 0 ? 128 : b->plan[form].qRows;
+    // plain: left block x 64-row right tiles; shortlists: query block x 64 candidates
+    if (queryRows) *queryRows = b->plan[form].qRows;
     if (candidateRows) *candidateRows = 64;
     return V3DM_OK;
 }
@@ -1179,8 +1187,8 @@ int v3dm_batch_time_dominant_kernel(v3dm_handle h, int reps, float* msPerLaunch)
     if (e == hipSuccess) e = hipEventRecord(b->evA, st);
     for (int r = 0; r < reps && e == hipSuccess; ++r) {
         int const n = (int)pl.items.size();
-        if (form == 0) launch_batch_fused_fp32(d, b->items[0], n, q + r, b->partF, pl.grid, st);
-        else if (form == 1) launch_batch_pass_bf16(d, b->items[1], n, q + r, b->partB, pl.grid, st);
+        if (form == 0) launch_batch_fused_fp32(d, b->items[0], n, q + r, b->partF, pl.grid, pl.qRows, st);
+        else if (form == 1) launch_batch_pass_bf16(d, b->items[1], n, q + r, b->partB, pl.grid, pl.qRows, st);
         else launch_batch_pass_code(d, b->items[2], n, q + r, b->partB, pl.grid, pl.qRows, st);
     }
     if (e == hipSuccess) e = hipEventRecord(b->evB, st);

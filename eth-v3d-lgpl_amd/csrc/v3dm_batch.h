@@ -113,13 +113,15 @@ __device__ __forceinline__ int sample_side(BatchDev const& b, int p)
 }
 
 
-// kernels of the plain FP32 form (v3dm_fp32.hip)
-// nItemsDev (may be null): the item count in device memory, at most nItems
+// kernels of the plain FP32 form (v3dm_fp32.hip); leftRows: left features per work-group,
+// 128 (4 waves) or 256 (8 waves), the plan's left blocks; nItemsDev (may be null): the item
+// count in device memory, at most nItems
 void launch_batch_fused_fp32(BatchDev const& b, int4 const* items, int nItems, int* queue, Top2* part, int grid,
-                             hipStream_t st, int const* nItemsDev = nullptr);
-// shortlist pass (v3dm_short.hip)
+                             int leftRows, hipStream_t st, int const* nItemsDev = nullptr);
+// shortlist pass (v3dm_short.hip); queryRows: queries per work-group, 128 (2 waves) or 256
+// (4 waves), the plan's query blocks
 void launch_batch_pass_bf16(BatchDev const& b, int4 const* items, int nItems, int* queue, Cand2* part, int grid,
-                            hipStream_t st);
+                            int queryRows, hipStream_t st);
 // uint8 images: certified integer scores, the open decisions (listed in `open`, count in
 // *nOpen, which must be zero) scored exactly (certify = false: every decision scored
 // exactly); FP32 images: exact scores

@@ -30,12 +30,28 @@ namespace v3dm {
 // each lane fetches whatever chunk belongs at its slot (pad slots refetch a neighbour).
 // After the multiply the stage is reused for the transposed tile S[j][i] (row stride 132
 // floats, conflict-free for the writes and the per-right reads).
+//
+// The batch form also has an 8-wave work-group (256 left features): each staged tile then
+// serves twice as many left features, and the transposed tile (64 x 264 floats) needs a
+// larger stage.  Shape<kW> holds the sizes of a kW-wave work-group.
 constexpr int kBI = 128;
 constexpr int kBJ = 64;
 constexpr int kSeg = 272;             // bytes per stage segment (256 data + 16 pad)
-constexpr int kStageLoads = 9;        // 1 KiB direct loads per wave per tile (4 waves)
-constexpr int kStageBytes = kStageLoads * 4 * 1024;
-constexpr int kSRow = 132;            // floats per row of the transposed tile
+constexpr int kStageData = 34;        // KiB of segments in a stage (2 * kBJ segments)
+
+template <int kW>
+struct Shape {
+    static constexpr int kRows = 32 * kW;                          // left features per work-group
+    static constexpr int kLoads = (kStageData + kW - 1) / kW;      // 1 KiB direct loads per wave per tile
+    static constexpr int kSRow = kRows + kW;                       // floats per row of the transposed tile
+    static constexpr int kLoadBytes = kLoads * kW * 1024;
+    static constexpr int kTBytes = (kBJ * kSRow * 4 + 1023) / 1024 * 1024;
+    static constexpr int kStage = kLoadBytes > kTBytes ? kLoadBytes : kTBytes;
+};
+constexpr int kStageLoads = Shape<4>::kLoads;
+constexpr int kStageBytes = Shape<4>::kStage;
+static_assert(kStageLoads == 9 && kStageBytes == 9 * 4 * 1024, "4-wave stage: 36 KiB");
+static_assert(Shape<8>::kStage <= 80 * 1024, "two 8-wave stages fit one CU's LDS");
 
 typedef __attribute__((address_space(3))) void lds_void;
 
@@ -48,13 +64,14 @@ __device__ __forceinline__ int stage_src_fp32(int slot)
     return row * 512 + (h * 16 + (within < 16 ? within : 15)) * 16;
 }
 
+template <int kW>
 __device__ __forceinline__ void stage_tile_fp32(float const* __restrict__ R, int t, unsigned char* stage,
-                                                int const (&src)[kStageLoads], int wv)
+                                                int const* src, int wv)
 {
     unsigned char const* tile = reinterpret_cast<unsigned char const*>(R + (size_t)t * kBJ * kDim);
 #pragma unroll
-    for (int p = 0; p < kStageLoads; ++p)
-        __builtin_amdgcn_global_load_lds(tile + src[p], (lds_void*)(stage + (4 * p + wv) * 1024), 16, 0, 0);
+    for (int p = 0; p < Shape<kW>::kLoads; ++p)
+        __builtin_amdgcn_global_load_lds(tile + src[p], (lds_void*)(stage + (kW * p + wv) * 1024), 16, 0, 0);
 }
 
 // Removed (left, right) pairs of guided matching: bit (j & 31) of bits[(j >> 5) * ldi + i]
@@ -64,7 +81,7 @@ struct GuidedBits {
     int ldi;
 };
 
-template <bool kFullJ, bool kFullI, bool kGuided>
+template <bool kFullJ, bool kFullI, bool kGuided, int kW>
 __device__ __forceinline__ void tile_fp32(unsigned char* stage, float const (&bfrag)[64], Top2& colS, Top2* rowPart,
                                           int ldr, int lb, int t, int nL, int nR, int i0, int tid, float inf,
                                           GuidedBits gb)
@@ -119,6 +136,7 @@ __device__ __forceinline__ void tile_fp32(unsigned char* stage, float const (&bf
     }
     if (!rowPart) return;  // left decisions only (uniform: a kernel argument)
     __syncthreads();  // every wave is done reading the stage as the A operand
+    constexpr int kSRow = Shape<kW>::kSRow;
     float* S = reinterpret_cast<float*>(stage);
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
@@ -127,26 +145,28 @@ __device__ __forceinline__ void tile_fp32(unsigned char* stage, float const (&bf
         S[(jl + 32) * kSRow + wave * 32 + c32] = acc1[r];
     }
     __syncthreads();
-    int const rq = tid & 3;   // quarter of the 128 left features
-    int const rj = tid >> 2;  // tile row
+    // kW threads per tile row, each over every kW-th left feature; the row stride (kRows +
+    // kW floats) puts the rows that 32 lanes read on distinct banks
+    int const rq = tid % kW;
+    int const rj = tid / kW;  // tile row
     Top2 rs = top2_init();
 #pragma unroll
     for (int c = 0; c < 32; ++c) {
-        int const il = c * 4 + rq;
+        int const il = c * kW + rq;
         float v = S[rj * kSRow + il];
         if (!kFullI) v = (i0 + il < nL) ? v : -INFINITY;
         top2_push_fast(rs, v, i0 + il, inf);
     }
-    rs = top2_merge(rs, top2_shfl_xor(rs, 1));
-    rs = top2_merge(rs, top2_shfl_xor(rs, 2));
+#pragma unroll
+    for (int x = 1; x < kW; x *= 2) rs = top2_merge(rs, top2_shfl_xor(rs, x));
     if (rq == 0 && jt + rj < nR) rowPart[(size_t)lb * ldr + jt + rj] = rs;
 }
 
 // The tiles [t0, t1) of one work-group, double-buffered; kMasked handles a partial last
 // tile of right features or a partial block of left features.
-template <bool kMasked, bool kGuided>
+template <bool kMasked, bool kGuided, int kW>
 __device__ __forceinline__ void run_tiles_fp32(float const* __restrict__ R, unsigned char* stA, unsigned char* stB,
-                                               int const (&src)[kStageLoads], float const (&bfrag)[64], Top2& colS,
+                                               int const* src, float const (&bfrag)[64], Top2& colS,
                                                Top2* rowPart, int ldr, int lb, int t0, int t1, int nL, int nR, int i0,
                                                int tid, int wv, float inf, GuidedBits gb)
 {
@@ -154,20 +174,20 @@ __device__ __forceinline__ void run_tiles_fp32(float const* __restrict__ R, unsi
     for (int t = t0; t < t1; t += 2) {
         wait_lds_loads();
         __syncthreads();  // tile t is in stA; every wave is done with stB
-        if (t + 1 < t1) stage_tile_fp32(R, t + 1, stB, src, wv);
-        tile_fp32<!kMasked, !kMasked, kGuided>(stA, bfrag, colS, rowPart, ldr, lb, t, nL, nR, i0, tid, inf, gb);
+        if (t + 1 < t1) stage_tile_fp32<kW>(R, t + 1, stB, src, wv);
+        tile_fp32<!kMasked, !kMasked, kGuided, kW>(stA, bfrag, colS, rowPart, ldr, lb, t, nL, nR, i0, tid, inf, gb);
         if (t + 1 >= t1) break;
         wait_lds_loads();
         __syncthreads();  // tile t+1 is in stB; every wave is done with stA
-        if (t + 2 < t1) stage_tile_fp32(R, t + 2, stA, src, wv);
-        tile_fp32<!kMasked, !kMasked, kGuided>(stB, bfrag, colS, rowPart, ldr, lb, t + 1, nL, nR, i0, tid, inf,
-                                                gb);
+        if (t + 2 < t1) stage_tile_fp32<kW>(R, t + 2, stA, src, wv);
+        tile_fp32<!kMasked, !kMasked, kGuided, kW>(stB, bfrag, colS, rowPart, ldr, lb, t + 1, nL, nR, i0, tid, inf,
+                                                    gb);
     }
 }
 
-// One work unit: left block `lb` (128 features) against right tiles [split * tilesPerSplit,
-// ...) -> colPart[split * ldc + i], rowPart[lb * ldr + j].
-template <bool kGuided = false>
+// One work unit: left block `lb` (32 * kW features) against right tiles [split *
+// tilesPerSplit, ...) -> colPart[split * ldc + i], rowPart[lb * ldr + j].
+template <bool kGuided = false, int kW = 4>
 __device__ __forceinline__ void fused_block(float const* __restrict__ L, float const* __restrict__ R, int nL, int nR,
                                             int lb, int split, int tilesPerSplit, Top2* __restrict__ colPart,
                                             int ldc, Top2* __restrict__ rowPart, int ldr, unsigned char* stA,
@@ -182,15 +202,15 @@ __device__ __forceinline__ void fused_block(float const* __restrict__ L, float c
     int const wv = __builtin_amdgcn_readfirstlane(wave);
     int const h = lane >> 5;
     int const c32 = lane & 31;
-    int const i0 = lb * kBI;
+    int const i0 = lb * Shape<kW>::kRows;
     int const nTiles = (nR + kBJ - 1) / kBJ;
     int const t0 = split * tilesPerSplit;
     int const t1 = min(t0 + tilesPerSplit, nTiles);
 
-    int src[kStageLoads];
+    int src[Shape<kW>::kLoads];
 #pragma unroll
-    for (int p = 0; p < kStageLoads; ++p) src[p] = stage_src_fp32((4 * p + wv) * 64 + lane);
-    if (t0 < t1) stage_tile_fp32(R, t0, stA, src, wv);
+    for (int p = 0; p < Shape<kW>::kLoads; ++p) src[p] = stage_src_fp32((kW * p + wv) * 64 + lane);
+    if (t0 < t1) stage_tile_fp32<kW>(R, t0, stA, src, wv);
 
     int const myI = i0 + wave * 32 + c32;
     float bfrag[64];
@@ -207,13 +227,13 @@ __device__ __forceinline__ void fused_block(float const* __restrict__ L, float c
     }
 
     Top2 colS = top2_init();
-    bool const masked = (i0 + kBI > nL) || (t1 * kBJ > nR);
+    bool const masked = (i0 + Shape<kW>::kRows > nL) || (t1 * kBJ > nR);
     if (masked)
-        run_tiles_fp32<true, kGuided>(R, stA, stB, src, bfrag, colS, rowPart, ldr, lb, t0, t1, nL, nR, i0, tid, wv,
-                                      inf, gb);
+        run_tiles_fp32<true, kGuided, kW>(R, stA, stB, src, bfrag, colS, rowPart, ldr, lb, t0, t1, nL, nR, i0, tid,
+                                          wv, inf, gb);
     else
-        run_tiles_fp32<false, kGuided>(R, stA, stB, src, bfrag, colS, rowPart, ldr, lb, t0, t1, nL, nR, i0, tid, wv,
-                                       inf, gb);
+        run_tiles_fp32<false, kGuided, kW>(R, stA, stB, src, bfrag, colS, rowPart, ldr, lb, t0, t1, nL, nR, i0, tid,
+                                           wv, inf, gb);
 
     colS = top2_merge(colS, top2_shfl_xor(colS, 32));
     if (h == 0 && myI < nL) colPart[(size_t)split * ldc + myI] = colS;
@@ -243,15 +263,17 @@ __global__ __launch_bounds__(256, 2) void k_guided_fp32(float const* __restrict_
 }
 
 // Persistent form over a batch: each work-group takes items from the queue until it runs
-// dry.  Item = (pair, left block, split, tiles per split); the left queries of pair p are
-// segment p, its right queries segment nPairs + p.
-__global__ __launch_bounds__(256, 2) void k_batch_fused_fp32(BatchDev b, int4 const* __restrict__ items, int nItems,
-                                                             int const* __restrict__ nItemsDev,
-                                                             int* __restrict__ queue, Top2* __restrict__ part,
-                                                             float inf)
+// dry.  Item = (pair, left block of 32 * kW features, split, tiles per split); the left
+// queries of pair p are segment p, its right queries segment nPairs + p.  kW = 4: two
+// work-groups per CU; kW = 8: one (132 KiB of LDS).
+template <int kW>
+__global__ __launch_bounds__(64 * kW, 2) void k_batch_fused_fp32(BatchDev b, int4 const* __restrict__ items,
+                                                                 int nItems, int const* __restrict__ nItemsDev,
+                                                                 int* __restrict__ queue, Top2* __restrict__ part,
+                                                                 float inf)
 {
-    __shared__ __attribute__((aligned(1024))) unsigned char stA[kStageBytes];
-    __shared__ __attribute__((aligned(1024))) unsigned char stB[kStageBytes];
+    __shared__ __attribute__((aligned(1024))) unsigned char stA[Shape<kW>::kStage];
+    __shared__ __attribute__((aligned(1024))) unsigned char stB[Shape<kW>::kStage];
     __shared__ int sItem;
     if (nItemsDev) nItems = __builtin_amdgcn_readfirstlane(min(nItems, *nItemsDev));  // written by an earlier kernel
     for (;;) {
@@ -265,8 +287,8 @@ __global__ __launch_bounds__(256, 2) void k_batch_fused_fp32(BatchDev b, int4 co
         BatchSeg const sr = b.seg[b.nPairs + w.x];
         float const* L = b.img + (size_t)b.imgBase[sl.img] * kDim;
         float const* R = b.img + (size_t)b.imgBase[sr.img] * kDim;
-        fused_block(L, R, sl.n, sr.n, w.y, w.z, w.w, part + sl.partOff, sl.ld, part + sr.partOff, sr.ld, stA, stB,
-                    inf);
+        fused_block<false, kW>(L, R, sl.n, sr.n, w.y, w.z, w.w, part + sl.partOff, sl.ld, part + sr.partOff, sr.ld,
+                               stA, stB, inf);
     }
 }
 
@@ -305,11 +327,15 @@ void launch_batch_sample_fp32(BatchDev const& b, float const* sRows, int const* 
 }
 
 void launch_batch_fused_fp32(BatchDev const& b, int4 const* items, int nItems, int* queue, Top2* part, int grid,
-                             hipStream_t st, int const* nItemsDev)
+                             int leftRows, hipStream_t st, int const* nItemsDev)
 {
     if (nItems <= 0) return;
-    hipLaunchKernelGGL(k_batch_fused_fp32, dim3(grid), dim3(256), 0, st, b, items, nItems, nItemsDev, queue, part,
-                       INFINITY);
+    if (leftRows == Shape<8>::kRows)
+        hipLaunchKernelGGL(k_batch_fused_fp32<8>, dim3(grid), dim3(512), 0, st, b, items, nItems, nItemsDev, queue,
+                           part, INFINITY);
+    else
+        hipLaunchKernelGGL(k_batch_fused_fp32<4>, dim3(grid), dim3(256), 0, st, b, items, nItems, nItemsDev, queue,
+                           part, INFINITY);
 }
 
 FusedPlan plan_fused_fp32(int nL, int nR, int targetGroups, int maxSplits)

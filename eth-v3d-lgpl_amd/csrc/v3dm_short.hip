@@ -68,8 +68,18 @@ __device__ __forceinline__ void key_tile_update(KeyState& s, float o1, float o2,
 // across bank groups.  Each lane fetches the chunk that belongs at its slot; pad slots
 // refetch a neighbour.
 constexpr int kSegB = 144;
-constexpr int kStageLoadsB = 5;  // 1 KiB direct loads per wave per tile (4 waves)
-constexpr int kStageBytesB = kStageLoadsB * 4 * 1024;
+constexpr int kStageDataB = 18;  // KiB of segments in a stage (2 * kST segments)
+
+// Sizes of a work-group of kQW waves (64 queries each): 1 KiB direct loads per wave per
+// tile, and the stage they fill.
+template <int kQW>
+struct ShapeB {
+    static constexpr int kLoads = (kStageDataB + kQW - 1) / kQW;
+    static constexpr int kStage = kLoads * kQW * 1024;
+};
+constexpr int kStageLoadsB = ShapeB<4>::kLoads;
+constexpr int kStageBytesB = ShapeB<4>::kStage;
+static_assert(kStageLoadsB == 5 && kSQ == 4 * 64, "single-pair pass: 4 waves, 20 KiB stages");
 
 __device__ __forceinline__ int stage_src_bf16(int slot)
 {
@@ -79,13 +89,14 @@ __device__ __forceinline__ int stage_src_bf16(int slot)
     return row * 256 + (h * 8 + (within < 8 ? within : 7)) * 16;
 }
 
+template <int kQW>
 __device__ __forceinline__ void stage_tile_bf16(__bf16 const* __restrict__ C, int t, unsigned char* stage,
-                                                int const (&src)[kStageLoadsB], int wv)
+                                                int const* src, int wv)
 {
     unsigned char const* tile = reinterpret_cast<unsigned char const*>(C + (size_t)t * kST * kDim);
 #pragma unroll
-    for (int p = 0; p < kStageLoadsB; ++p)
-        __builtin_amdgcn_global_load_lds(tile + src[p], (lds_void*)(stage + (4 * p + wv) * 1024), 16, 0, 0);
+    for (int p = 0; p < ShapeB<kQW>::kLoads; ++p)
+        __builtin_amdgcn_global_load_lds(tile + src[p], (lds_void*)(stage + (kQW * p + wv) * 1024), 16, 0, 0);
 }
 
 template <bool kFull>
@@ -132,21 +143,21 @@ __device__ __forceinline__ void tile_bf16(unsigned char const* stage, v8bf const
 
 // The tiles [t0, t1) of one work-group, double-buffered (two tiles per trip so that each
 // stage is a distinct LDS object); kFull = no tile of the range is partial.
-template <bool kFull>
+template <bool kFull, int kQW>
 __device__ __forceinline__ void run_tiles_bf16(__bf16 const* __restrict__ C, unsigned char* stA, unsigned char* stB,
-                                               int const (&src)[kStageLoadsB], v8bf const (&bq)[2][8],
+                                               int const* src, v8bf const (&bq)[2][8],
                                                KeyState (&st)[2], int t0, int t1, int nC, int h, int c32, int wv,
                                                float inf)
 {
     for (int t = t0; t < t1; t += 2) {
         wait_lds_loads();
         __syncthreads();  // tile t is in stA; every wave is done with stB
-        if (t + 1 < t1) stage_tile_bf16(C, t + 1, stB, src, wv);
+        if (t + 1 < t1) stage_tile_bf16<kQW>(C, t + 1, stB, src, wv);
         tile_bf16<kFull>(stA, bq, st, t, nC, h, c32, inf);
         if (t + 1 >= t1) break;
         wait_lds_loads();
         __syncthreads();  // tile t+1 is in stB; every wave is done with stA
-        if (t + 2 < t1) stage_tile_bf16(C, t + 2, stA, src, wv);
+        if (t + 2 < t1) stage_tile_bf16<kQW>(C, t + 2, stA, src, wv);
         tile_bf16<kFull>(stB, bq, st, t + 1, nC, h, c32, inf);
     }
 }
@@ -161,8 +172,10 @@ __device__ __forceinline__ void run_tiles_bf16(__bf16 const* __restrict__ C, uns
 // with its 6 low mantissa bits replaced by the position (block*16 + r) inside the tile,
 // so that a med3 pair updates value and position together; the tile of each of the two
 // keys is tracked once per tile.  Masked candidates score -1e30.
-// One work unit: query block `qblk` (256 queries) against candidate tiles [split *
-// tilesPerSplit, ...) -> part[((split * ldq) + q) * 2 + h].
+// One work unit: query block `qblk` (64 * kQW queries; the single-pair pass: 4 waves, 256
+// queries) against candidate tiles [split * tilesPerSplit, ...) -> part[((split * ldq) + q)
+// * 2 + h].
+template <int kQW = 4>
 __device__ __forceinline__ void pass_bf16_block(__bf16 const* __restrict__ Q, int nQ, __bf16 const* __restrict__ C,
                                                 int nC, int qblk, int split, int tilesPerSplit,
                                                 Cand2* __restrict__ part, int ldq, unsigned char* stA,
@@ -174,15 +187,15 @@ __device__ __forceinline__ void pass_bf16_block(__bf16 const* __restrict__ Q, in
     int const wv = __builtin_amdgcn_readfirstlane(wave);
     int const h = lane >> 5;
     int const c32 = lane & 31;
-    int const q0 = qblk * kSQ + wave * 64;
+    int const q0 = qblk * (64 * kQW) + wave * 64;
     int const nTiles = (nC + kST - 1) / kST;
     int const t0 = split * tilesPerSplit;
     int const t1 = min(t0 + tilesPerSplit, nTiles);
 
-    int src[kStageLoadsB];
+    int src[ShapeB<kQW>::kLoads];
 #pragma unroll
-    for (int p = 0; p < kStageLoadsB; ++p) src[p] = stage_src_bf16((4 * p + wv) * 64 + lane);
-    if (t0 < t1) stage_tile_bf16(C, t0, stA, src, wv);
+    for (int p = 0; p < ShapeB<kQW>::kLoads; ++p) src[p] = stage_src_bf16((kQW * p + wv) * 64 + lane);
+    if (t0 < t1) stage_tile_bf16<kQW>(C, t0, stA, src, wv);
 
     v8bf bq[2][8];
 #pragma unroll
@@ -199,8 +212,8 @@ __device__ __forceinline__ void pass_bf16_block(__bf16 const* __restrict__ Q, in
 #pragma unroll
     for (int qb = 0; qb < 2; ++qb) st[qb] = KeyState{-INFINITY, -INFINITY, -1, -1};
 
-    if (t1 * kST > nC) run_tiles_bf16<false>(C, stA, stB, src, bq, st, t0, t1, nC, h, c32, wv, inf);
-    else run_tiles_bf16<true>(C, stA, stB, src, bq, st, t0, t1, nC, h, c32, wv, inf);
+    if (t1 * kST > nC) run_tiles_bf16<false, kQW>(C, stA, stB, src, bq, st, t0, t1, nC, h, c32, wv, inf);
+    else run_tiles_bf16<true, kQW>(C, stA, stB, src, bq, st, t0, t1, nC, h, c32, wv, inf);
 
 #pragma unroll
     for (int qb = 0; qb < 2; ++qb) {
@@ -243,13 +256,14 @@ __global__ __launch_bounds__(256, 2) void k_pass_bf16(Bf16Dir d0, Bf16Dir d1, fl
 }
 
 // Persistent form over a batch: each work-group takes items from the queue until it runs
-// dry.  Item = (segment, query block, split, tiles per split).
-__global__ __launch_bounds__(256, 2) void k_batch_pass_bf16(BatchDev b, int4 const* __restrict__ items, int nItems,
-                                                            int* __restrict__ queue, Cand2* __restrict__ part,
-                                                            float inf)
+// dry.  Item = (segment, query block of 64 * kQW queries, split, tiles per split).
+template <int kQW>
+__global__ __launch_bounds__(64 * kQW, 2) void k_batch_pass_bf16(BatchDev b, int4 const* __restrict__ items,
+                                                                 int nItems, int* __restrict__ queue,
+                                                                 Cand2* __restrict__ part, float inf)
 {
-    __shared__ __attribute__((aligned(1024))) unsigned char stA[kStageBytesB];
-    __shared__ __attribute__((aligned(1024))) unsigned char stB[kStageBytesB];
+    __shared__ __attribute__((aligned(1024))) unsigned char stA[ShapeB<kQW>::kStage];
+    __shared__ __attribute__((aligned(1024))) unsigned char stB[ShapeB<kQW>::kStage];
     __shared__ int sItem;
     for (;;) {
         if (threadIdx.x == 0) sItem = atomicAdd(queue, 1);
@@ -261,15 +275,18 @@ __global__ __launch_bounds__(256, 2) void k_batch_pass_bf16(BatchDev b, int4 con
         BatchSeg const sg = b.seg[w.x];
         __bf16 const* Q = b.imgB + (size_t)b.imgBase[sg.img] * kDim;
         __bf16 const* C = b.imgB + (size_t)b.imgBase[sg.oImg] * kDim;
-        pass_bf16_block(Q, sg.n, C, sg.nO, w.y, w.z, w.w, part + (size_t)sg.partOff * 2, sg.ld, stA, stB, inf);
+        pass_bf16_block<kQW>(Q, sg.n, C, sg.nO, w.y, w.z, w.w, part + (size_t)sg.partOff * 2, sg.ld, stA, stB, inf);
     }
 }
 
 void launch_batch_pass_bf16(BatchDev const& b, int4 const* items, int nItems, int* queue, Cand2* part, int grid,
-                            hipStream_t st)
+                            int queryRows, hipStream_t st)
 {
     if (nItems <= 0) return;
-    hipLaunchKernelGGL(k_batch_pass_bf16, dim3(grid), dim3(256), 0, st, b, items, nItems, queue, part, INFINITY);
+    if (queryRows == 128)
+        hipLaunchKernelGGL(k_batch_pass_bf16<2>, dim3(grid), dim3(128), 0, st, b, items, nItems, queue, part, INFINITY);
+    else
+        hipLaunchKernelGGL(k_batch_pass_bf16<4>, dim3(grid), dim3(256), 0, st, b, items, nItems, queue, part, INFINITY);
 }
 
 ShortPlan plan_short(int nQ, int nC, int targetGroups)

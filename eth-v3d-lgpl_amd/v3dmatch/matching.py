@@ -310,8 +310,8 @@ class SIFT_Matching:
                                                      int(grid)))
 
     def batch_set_tile_shape(self, mode, query_rows=0):
-        """The learned code pass's work-group shape: 128 or 256 queries per 64-candidate tile
-        (0: chosen by the autotuner)."""
+        """The work-group shape of the pass of `mode`'s form: 128 or 256 query rows (plain:
+        left features) per 64-candidate tile; 0: chosen by the autotuner."""
         self._check(self._lib.v3dm_batch_set_tile_shape(self._h, int(mode), int(query_rows)))
 
     def batch_get_tile_shape(self, mode=0):

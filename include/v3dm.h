@@ -297,10 +297,12 @@ int v3dm_batch_match(v3dm_handle h, float minScore, float minRatio, int32_t* off
  * per query): it changes which candidates are listed, so it is never tuned. */
 int v3dm_batch_set_tuning(v3dm_handle h, int autotune, int mode, int tilesPerItem, int grid);
 /* The work-group tile shape of a form's pass: queries (plain: left features) per work-group
- * x candidate rows per tile.  Plain FP32 128 x 64, BF16 256 x 64, fixed; the learned code
- * pass 128 x 64 or 256 x 64, chosen by the autotuner with the grid (every shape lists the
- * same candidates: the result does not depend on it).  set: queryRows > 0 fixes the learned
- * shape, 0 lets the tuner choose; get: the shape of the form's current plan. */
+ * x candidate rows per tile, 128 x 64 or 256 x 64 in every form (plain FP32: 4 or 8 waves;
+ * BF16 and learned codes: 2 or 4 waves), chosen by the autotuner with the tiles per item and
+ * the grid (no shape changes the result: the plain form's match set is the exact one in
+ * every shape, and the shortlist forms list the same candidates).  set: queryRows 128 or 256
+ * fixes the shape of the form of `mode`, 0 lets the tuner choose (default without tuning:
+ * plain 128, BF16 and learned 256); get: the shape of the form's current plan. */
 int v3dm_batch_set_tile_shape(v3dm_handle h, int mode, int queryRows);
 int v3dm_batch_get_tile_shape(v3dm_handle h, int mode, int* queryRows, int* candidateRows);
 /* The setting in use for `mode` and the autotuner's measured pass time (-1 if not tuned). */

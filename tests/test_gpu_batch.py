@@ -124,6 +124,45 @@ def test_batch_bf16_recall(matcher, small_batch):
     matcher.batch_set_tuning(autotune=True, mode=v3dmatch.PREC_BF16)
 
 
+def test_batch_tile_shapes(matcher, small_batch):
+    """Both work-group shapes of the plain pass (128 left features on 4 waves, 256 on 8)
+    give the oracle's match set under several work-item tunings; both shapes of the BF16
+    pass (256 queries on 4 waves, 128 on 2) list the same candidates, so their match sets
+    are identical; the shape in use is reported, and other shapes are refused."""
+    counts, desc, left, right, want = small_batch
+    m = matcher
+    m.batch_set_images(counts, desc)
+    m.batch_set_pairs(left, right)
+    m.set_precision(v3dmatch.PREC_FP32)
+    for q in (256, 128):
+        m.batch_set_tile_shape(v3dmatch.PREC_FP32, q)
+        for tiles, grid in [(0, 0), (1, 7), (5, 300), (64, 1024)]:
+            m.batch_set_tuning(autotune=False, mode=v3dmatch.PREC_FP32, tiles_per_item=tiles, grid=grid)
+            say(f"case batch fp32 shape {q} tiles {tiles} grid {grid}")
+            got = m.batch_match(0.9, 0.8)
+            assert m.batch_get_tile_shape(v3dmatch.PREC_FP32) == {"query_rows": q, "candidate_rows": 64}
+            for p in range(len(left)):
+                assert np.array_equal(got[p], want[p]), (q, tiles, grid, p)
+    m.set_precision(v3dmatch.PREC_BF16)
+    res = {}
+    for q in (256, 128):
+        m.batch_set_tile_shape(v3dmatch.PREC_BF16, q)
+        m.batch_set_tuning(autotune=False, mode=v3dmatch.PREC_BF16, tiles_per_item=4, grid=300)
+        say(f"case batch bf16 shape {q}")
+        res[q] = m.batch_match(0.9, 0.8)
+        assert m.batch_get_tile_shape(v3dmatch.PREC_BF16) == {"query_rows": q, "candidate_rows": 64}
+    assert all(np.array_equal(a, b) for a, b in zip(res[256], res[128]))
+    for mode in (v3dmatch.PREC_FP32, v3dmatch.PREC_BF16):
+        with pytest.raises(v3dmatch.V3DMError):
+            m.batch_set_tile_shape(mode, 64)
+        m.batch_set_tile_shape(mode, 0)
+        m.batch_set_tuning(autotune=True, mode=mode)
+    m.set_precision(v3dmatch.PREC_FP32)
+    m.batch_match(0.9, 0.8)
+    say(f"  tuned fp32 shape {m.batch_get_tile_shape(v3dmatch.PREC_FP32)} "
+        f"{m.batch_get_tuning(v3dmatch.PREC_FP32)}")
+
+
 def test_batch_full_size_against_single_pair(matcher):
     """256 pairs, 500-16000 features per image (log-uniform), 30-60% copies: the batched
     FP32 result of every pair equals the single-pair GPU path's (itself identical to the
@@ -146,7 +185,15 @@ def test_batch_full_size_against_single_pair(matcher):
     single.close()
     for p in range(256):
         assert np.array_equal(got[p], want[p]), p
-    say(f"  fp32 batch identical to single-pair on 256 pairs, {sum(len(w) for w in want)} matches")
+    say(f"  fp32 batch identical to single-pair on 256 pairs, {sum(len(w) for w in want)} matches "
+        f"(shape {matcher.batch_get_tile_shape(v3dmatch.PREC_FP32)})")
+    for q in (128, 256):  # both work-group shapes of the plain pass at full size
+        matcher.batch_set_tile_shape(v3dmatch.PREC_FP32, q)
+        gq = matcher.batch_match(0.9, 0.8)
+        for p in range(256):
+            assert np.array_equal(gq[p], want[p]), (q, p)
+        say(f"  fp32 shape {q}: identical")
+    matcher.batch_set_tile_shape(v3dmatch.PREC_FP32, 0)
     matcher.set_precision(v3dmatch.PREC_BF16)
     gb = matcher.batch_match(0.9, 0.8)
     ws = set((p, a, b) for p in range(256) for a, b in want[p].tolist())
