@@ -58,42 +58,57 @@ Mat3 transpose3(Mat3 const& a)
     return t;
 }
 
-bool invert3(Mat3 const& a, Mat3& out)
+// The adjugate scaled by 1 / det, det expanded as a e i + b f g + c d h - c e g - b d i -
+// a f h (invertedMatrix, V3D/Math/v3d_linear.h:558-575, in its operation order)
+bool invert3(Mat3 const& A, Mat3& out)
 {
-    double const c00 = a(1, 1) * a(2, 2) - a(1, 2) * a(2, 1);
-    double const c01 = a(1, 2) * a(2, 0) - a(1, 0) * a(2, 2);
-    double const c02 = a(1, 0) * a(2, 1) - a(1, 1) * a(2, 0);
-    double const det = a(0, 0) * c00 + a(0, 1) * c01 + a(0, 2) * c02;
+    double const a = A(0, 0), b = A(0, 1), c = A(0, 2);
+    double const d = A(1, 0), e = A(1, 1), f = A(1, 2);
+    double const g = A(2, 0), h = A(2, 1), i = A(2, 2);
+    double const det = a * e * i + b * f * g + c * d * h - c * e * g - b * d * i - a * f * h;
     if (!(std::fabs(det) > 0.0)) return false;
-    double const id = 1.0 / det;
-    out(0, 0) = c00 * id;
-    out(1, 0) = c01 * id;
-    out(2, 0) = c02 * id;
-    out(0, 1) = (a(0, 2) * a(2, 1) - a(0, 1) * a(2, 2)) * id;
-    out(1, 1) = (a(0, 0) * a(2, 2) - a(0, 2) * a(2, 0)) * id;
-    out(2, 1) = (a(0, 1) * a(2, 0) - a(0, 0) * a(2, 1)) * id;
-    out(0, 2) = (a(0, 1) * a(1, 2) - a(0, 2) * a(1, 1)) * id;
-    out(1, 2) = (a(0, 2) * a(1, 0) - a(0, 0) * a(1, 2)) * id;
-    out(2, 2) = (a(0, 0) * a(1, 1) - a(0, 1) * a(1, 0)) * id;
+    double const s = 1.0 / det;
+    out(0, 0) = (e * i - f * h) * s, out(0, 1) = (c * h - b * i) * s, out(0, 2) = (b * f - c * e) * s;
+    out(1, 0) = (f * g - d * i) * s, out(1, 1) = (a * i - c * g) * s, out(1, 2) = (c * d - a * f) * s;
+    out(2, 0) = (d * h - e * g) * s, out(2, 1) = (b * g - a * h) * s, out(2, 2) = (a * e - b * d) * s;
     return true;
 }
 
-// K^-1 (u, v, 1), divided by its third component (multiply_A_v_projective)
-void normalise_point(Mat3 const& iK, double u, double v, double& x, double& y)
+// K^-1 (u, v, 1), divided by its third component, each sum started from the last column
+// (multiply_A_v_projective, V3D/Math/v3d_linearbase.h:1030-1046); Acc = double, or float
+// where the reference stores the result in a float vector (its PointMeasurement positions)
+template <typename Acc>
+void normalise_point_t(Mat3 const& iK, double u, double v, double& x, double& y)
 {
-    double const w = iK(2, 0) * u + iK(2, 1) * v + iK(2, 2);
-    x = (iK(0, 0) * u + iK(0, 1) * v + iK(0, 2)) / w;
-    y = (iK(1, 0) * u + iK(1, 1) * v + iK(1, 2)) / w;
+    Acc w = (Acc)iK(2, 2);
+    w = (Acc)(w + iK(2, 0) * u);
+    w = (Acc)(w + iK(2, 1) * v);
+    Acc d0 = (Acc)iK(0, 2), d1 = (Acc)iK(1, 2);
+    d0 = (Acc)(d0 + iK(0, 0) * u);
+    d0 = (Acc)(d0 + iK(0, 1) * v);
+    d1 = (Acc)(d1 + iK(1, 0) * u);
+    d1 = (Acc)(d1 + iK(1, 1) * v);
+    d0 /= w;
+    d1 /= w;
+    x = d0;
+    y = d1;
 }
 
-// sampsonEpipolarError (reconstruction_common.h:588-608), same operations
+void normalise_point(Mat3 const& iK, double u, double v, double& x, double& y)
+{
+    normalise_point_t<double>(iK, u, v, x, y);
+}
+
+// (A (x, y, 1))_r summed from the last column on (multiply_A_v_affine, v3d_linearbase.h:
+// 1048-1061); At: the same with A transposed
+double affine_row(Mat3 const& A, int r, double x, double y) { return A(r, 2) + A(r, 0) * x + A(r, 1) * y; }
+double affine_col(Mat3 const& A, int c, double x, double y) { return A(2, c) + A(0, c) * x + A(1, c) * y; }
+
+// sampsonEpipolarError (reconstruction_common.h:588-608), same operations in the same order
 double sampson(Mat3 const& F, double px, double py, double qx, double qy)
 {
-    double const fp0 = F(0, 0) * px + F(0, 1) * py + F(0, 2);
-    double const fp1 = F(1, 0) * px + F(1, 1) * py + F(1, 2);
-    double const fp2 = F(2, 0) * px + F(2, 1) * py + F(2, 2);
-    double const ftq0 = F(0, 0) * qx + F(1, 0) * qy + F(2, 0);
-    double const ftq1 = F(0, 1) * qx + F(1, 1) * qy + F(2, 1);
+    double const fp0 = affine_row(F, 0, px, py), fp1 = affine_row(F, 1, px, py), fp2 = affine_row(F, 2, px, py);
+    double const ftq0 = affine_col(F, 0, qx, qy), ftq1 = affine_col(F, 1, qx, qy);
     double const tmp = qx * fp0 + qy * fp1 + fp2;
     double const num = tmp * tmp;
     double const denom = fp0 * fp0 + fp1 * fp1 + ftq0 * ftq0 + ftq1 * ftq1;
@@ -523,39 +538,10 @@ int five_point(double const x1[5], double const y1[5], double const x2[5], doubl
 }
 
 // ---------------------------------------------------------------------------- pose from E
-// Symmetric 3x3 eigen-decomposition (cyclic Jacobi): s = V diag(d) V^T, columns of V
-void jacobi3(Mat3 s, double (&d)[3], Mat3& V)
-{
-    V = zero3();
-    V(0, 0) = V(1, 1) = V(2, 2) = 1.0;
-    for (int sweep = 0; sweep < 50; ++sweep) {
-        double off = std::fabs(s(0, 1)) + std::fabs(s(0, 2)) + std::fabs(s(1, 2));
-        if (off == 0.0) break;
-        for (int p = 0; p < 2; ++p)
-            for (int q = p + 1; q < 3; ++q) {
-                if (s(p, q) == 0.0) continue;
-                double const theta = (s(q, q) - s(p, p)) / (2.0 * s(p, q));
-                double const t = sign_of(1.0, theta) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-                double const c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
-                for (int k = 0; k < 3; ++k) {  // s <- J^T s J
-                    double const a = s(k, p), b = s(k, q);
-                    s(k, p) = c * a - sn * b;
-                    s(k, q) = sn * a + c * b;
-                }
-                for (int k = 0; k < 3; ++k) {
-                    double const a = s(p, k), b = s(q, k);
-                    s(p, k) = c * a - sn * b;
-                    s(q, k) = sn * a + c * b;
-                }
-                for (int k = 0; k < 3; ++k) {
-                    double const a = V(k, p), b = V(k, q);
-                    V(k, p) = c * a - sn * b;
-                    V(k, q) = sn * a + c * b;
-                }
-            }
-    }
-    for (int i = 0; i < 3; ++i) d[i] = s(i, i);
-}
+// Nister's procedure ("An efficient solution to the five-point relative pose problem",
+// PAMI 2004, section 3.3), in the reference's arithmetic order so that every per-point
+// decision is the reference's (computeEssentialSVD, getExactConfiguration,
+// relativePoseFromEssential, triangulatePointExact: v3d_poseutilities.cpp:1108-1291).
 
 void cross3(double const a[3], double const b[3], double (&c)[3])
 {
@@ -566,94 +552,109 @@ void cross3(double const a[3], double const b[3], double (&c)[3])
 
 double norm3(double const a[3]) { return std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
 
-// E = U diag(s, s, 0) V^T with U, V rotations (computeEssentialSVD)
+void unit3(double (&a)[3])
+{
+    double const n = norm3(a);
+    for (int i = 0; i < 3; ++i) a[i] /= n;
+}
+
+void mul3v(Mat3 const& E, double const v[3], double (&out)[3])
+{
+    for (int r = 0; r < 3; ++r) out[r] = E(r, 0) * v[0] + E(r, 1) * v[1] + E(r, 2) * v[2];
+}
+
+// E = U diag(s, s, 0) V^T without an iterative SVD: the third right singular vector is the
+// longest cross product of two rows of E (the null direction), the first is the first row
+// of that pair, the second completes the frame; U's columns are E v / |E v| and their cross
+// product.  false for a rank < 2 matrix (no cross product of non-zero length).
 bool essential_svd(Mat3 const& E, Mat3& U, Mat3& V)
 {
-    double d[3];
-    Mat3 W;
-    jacobi3(mul3(transpose3(E), E), d, W);
-    int order[3] = {0, 1, 2};
-    std::sort(order, order + 3, [&](int a, int b) { return d[a] > d[b]; });
-    double v[3][3], u[3][3];
-    for (int k = 0; k < 2; ++k)
-        for (int i = 0; i < 3; ++i) v[k][i] = W(i, order[k]);
-    cross3(v[0], v[1], v[2]);
-    for (int k = 0; k < 2; ++k) {
-        for (int i = 0; i < 3; ++i) u[k][i] = E(i, 0) * v[k][0] + E(i, 1) * v[k][1] + E(i, 2) * v[k][2];
+    double rows[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) rows[r][c] = E(r, c);
+    double x01[3], x02[3], x12[3];
+    cross3(rows[0], rows[1], x01);
+    cross3(rows[0], rows[2], x02);
+    cross3(rows[1], rows[2], x12);
+    double const* x = x01;
+    double const* y = rows[0];
+    double maxLen = norm3(x01);
+    double len = norm3(x02);
+    if (len > maxLen) maxLen = len, x = x02, y = rows[0];
+    len = norm3(x12);
+    if (len > maxLen) maxLen = len, x = x12, y = rows[1];
+    if (!(maxLen > 0.0)) return false;
+    double const inv = 1.0 / maxLen;
+    double vc[3] = {inv * x[0], inv * x[1], inv * x[2]};
+    double va[3] = {y[0], y[1], y[2]};
+    unit3(va);
+    double vb[3];
+    cross3(vc, va, vb);
+    double ua[3], ub[3], uc[3];
+    mul3v(E, va, ua);
+    mul3v(E, vb, ub);
+    unit3(ua);
+    unit3(ub);
+    cross3(ua, ub, uc);
+    for (int i = 0; i < 3; ++i) {
+        V(i, 0) = va[i], V(i, 1) = vb[i], V(i, 2) = vc[i];
+        U(i, 0) = ua[i], U(i, 1) = ub[i], U(i, 2) = uc[i];
     }
-    double n0 = norm3(u[0]);
-    if (!(n0 > 0.0)) return false;
-    for (int i = 0; i < 3; ++i) u[0][i] /= n0;
-    double const dp = u[0][0] * u[1][0] + u[0][1] * u[1][1] + u[0][2] * u[1][2];
-    for (int i = 0; i < 3; ++i) u[1][i] -= dp * u[0][i];
-    double n1 = norm3(u[1]);
-    if (!(n1 > 0.0)) return false;
-    for (int i = 0; i < 3; ++i) u[1][i] /= n1;
-    cross3(u[0], u[1], u[2]);
-    for (int k = 0; k < 3; ++k)
-        for (int i = 0; i < 3; ++i) {
-            U(i, k) = u[k][i];
-            V(i, k) = v[k][i];
-        }
     return true;
 }
 
-// the four decompositions: 0 (R1, t), 1 (R1, -t), 2 (R2, t), 3 (R2, -t), R1 = U D V^T,
-// R2 = U D^T V^T, t = the third column of U (relativePoseFromEssential,
-// v3d_poseutilities.cpp:1227-1261; D as makeD_Matrix, :1183-1191)
-void pose_of(Mat3 const& U, Mat3 const& V, int conf, Mat3& R, double (&T)[3])
+Mat3 twist_d()
 {
     Mat3 D = zero3();
     D(0, 1) = 1.0;
     D(1, 0) = -1.0;
     D(2, 2) = 1.0;
+    return D;
+}
+
+// the four decompositions: 0 (R1, t), 1 (R1, -t), 2 (R2, t), 3 (R2, -t), R1 = U D V^T,
+// R2 = U D^T V^T, t = the third column of U, normalised
+void pose_of(Mat3 const& U, Mat3 const& V, int conf, Mat3& R, double (&T)[3])
+{
+    Mat3 const D = twist_d();
     R = mul3(mul3(U, conf < 2 ? D : transpose3(D)), transpose3(V));
-    double const sgn = (conf == 0 || conf == 2) ? 1.0 : -1.0;
-    for (int i = 0; i < 3; ++i) T[i] = sgn * U(i, 2);
-    double const n = norm3(T);
-    for (int i = 0; i < 3; ++i) T[i] /= n;
+    for (int i = 0; i < 3; ++i) T[i] = (conf == 0 || conf == 2) ? U(i, 2) : -1.0 * U(i, 2);
+    unit3(T);
 }
 
-// depths (d1, d2) of the point seen at q1 = (x1, y1, 1) and q2 = (x2, y2, 1) under x2 ~ R
-// x1 + T: least squares of d1 R q1 - d2 q2 = -T
-void depths(Mat3 const& R, double const T[3], double x1, double y1, double x2, double y2, double& d1, double& d2)
+// The decomposition a correspondence p = (x1, y1, 1) <-> q = (x2, y2, 1) selects: the point
+// triangulated exactly under P_A = [R1 | t] (the plane through the camera-2 ray and the
+// baseline, C = P_A^T (q x diag(1, 1, 0) E p), meets the camera-1 ray at (C_4 p, -p.C_123));
+// in front of both cameras -> (R1, t), behind both -> (R1, -t); else the twisted pair
+// H_t = [I 0; -2 v_c^T -1] decides between (R2, t) and (R2, -t).
+int configuration(Mat3 const& E, Mat3 const& U, Mat3 const& V, double x1, double y1, double x2, double y2)
 {
-    double const a[3] = {R(0, 0) * x1 + R(0, 1) * y1 + R(0, 2), R(1, 0) * x1 + R(1, 1) * y1 + R(1, 2),
-                         R(2, 0) * x1 + R(2, 1) * y1 + R(2, 2)};
-    double const b[3] = {-x2, -y2, -1.0};
-    double const aa = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
-    double const ab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
-    double const bb = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
-    double const at = -(a[0] * T[0] + a[1] * T[1] + a[2] * T[2]);
-    double const bt = -(b[0] * T[0] + b[1] * T[1] + b[2] * T[2]);
-    double const det = aa * bb - ab * ab;
-    if (!(std::fabs(det) > 0.0)) {
-        d1 = d2 = 0.0;
-        return;
+    double const p[3] = {x1, y1, 1.0}, q[3] = {x2, y2, 1.0};
+    Mat3 const R1 = mul3(mul3(U, twist_d()), transpose3(V));
+    double const t1[3] = {U(0, 2), U(1, 2), U(2, 2)};
+    double Pa[3][4];
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) Pa[r][c] = R1(r, c);
+        Pa[r][3] = t1[r];
     }
-    d1 = (bb * at - ab * bt) / det;
-    d2 = (aa * bt - ab * at) / det;
-}
-
-// the decomposition under which the point lies in front of both cameras (the largest
-// smaller depth of the four)
-int configuration(Mat3 const& U, Mat3 const& V, double x1, double y1, double x2, double y2)
-{
-    int best = 0;
-    double bestDepth = -INFINITY;
-    for (int conf = 0; conf < 4; ++conf) {
-        Mat3 R;
-        double T[3];
-        pose_of(U, V, conf, R, T);
-        double d1, d2;
-        depths(R, T, x1, y1, x2, y2, d1, d2);
-        double const m = std::min(d1, d2);
-        if (m > bestDepth) {
-            bestDepth = m;
-            best = conf;
-        }
-    }
-    return best;
+    double const p1[3] = {E(0, 0) * p[0] + E(0, 1) * p[1] + E(0, 2) * p[2],
+                          E(1, 0) * p[0] + E(1, 1) * p[1] + E(1, 2) * p[2], 0.0};
+    double cpl[3];
+    cross3(q, p1, cpl);
+    double C[4];
+    for (int j = 0; j < 4; ++j) C[j] = Pa[0][j] * cpl[0] + Pa[1][j] * cpl[1] + Pa[2][j] * cpl[2];
+    double Q[4] = {p[0] * C[3], p[1] * C[3], p[2] * C[3], -(p[0] * C[0] + p[1] * C[1] + p[2] * C[2])};
+    Q[0] /= Q[3];
+    Q[1] /= Q[3];
+    Q[2] /= Q[3];
+    Q[3] = 1.0;
+    double const c1 = Q[2] * Q[3];
+    double const c = Pa[2][0] * Q[0] + Pa[2][1] * Q[1] + Pa[2][2] * Q[2] + Pa[2][3] * Q[3];
+    double const c2 = c * Q[3];
+    if (c1 >= 0 && c2 >= 0) return 0;
+    if (c1 < 0 && c2 < 0) return 1;
+    double const ct = -(2 * V(0, 2) * Q[0] + 2 * V(1, 2) * Q[1] + 2 * V(2, 2) * Q[2] + Q[3]);
+    return Q[2] * ct >= 0 ? 2 : 3;
 }
 
 bool pose_from_essential(Mat3 const& E, int n, double const* x1, double const* y1, double const* x2,
@@ -661,26 +662,25 @@ bool pose_from_essential(Mat3 const& E, int n, double const* x1, double const* y
 {
     Mat3 U, V;
     if (!essential_svd(E, U, V)) return false;
-    int const conf0 = configuration(U, V, x1[0], y1[0], x2[0], y2[0]);
+    int const conf0 = configuration(E, U, V, x1[0], y1[0], x2[0], y2[0]);
     for (int i = 1; i < n; ++i)
-        if (configuration(U, V, x1[i], y1[i], x2[i], y2[i]) != conf0) return false;
+        if (configuration(E, U, V, x1[i], y1[i], x2[i], y2[i]) != conf0) return false;
     pose_of(U, V, conf0, R, T);
     return true;
 }
 
 // ---------------------------------------------------------------------------- refinement
+// exp([w]x) = I + sin(th)/th [w]x + (1 - cos(th))/th^2 [w]x^2, th = |w|; a rotation of at
+// most 1e-6 rad is taken as the identity, as the reference's optimiser does
+// (createRotationMatrixRodrigues, V3D/Math/v3d_mathutilities.h:170-190): its refinement
+// drops such rotation steps and moves only T, and this refinement follows it
 Mat3 rodrigues(double const w[3])
 {
-    double const th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
-    double const th = std::sqrt(th2);
-    double a, b;  // sin(th) / th, (1 - cos(th)) / th^2
-    if (th < 1e-8) {
-        a = 1.0 - th2 / 6.0;
-        b = 0.5 - th2 / 24.0;
-    } else {
-        a = std::sin(th) / th;
-        b = (1.0 - std::cos(th)) / th2;
-    }
+    double const th = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    Mat3 R = zero3();
+    R(0, 0) = R(1, 1) = R(2, 2) = 1.0;
+    if (!(std::fabs(th) > 1e-6)) return R;
+    double const a = std::sin(th) / th, b = (1.0 - std::cos(th)) / (th * th);
     Mat3 K = zero3();
     K(0, 1) = -w[2];
     K(0, 2) = w[1];
@@ -689,11 +689,7 @@ Mat3 rodrigues(double const w[3])
     K(2, 0) = -w[1];
     K(2, 1) = w[0];
     Mat3 const K2 = mul3(K, K);
-    Mat3 R = zero3();
-    for (int i = 0; i < 9; ++i) R.m[i] = a * K.m[i] + b * K2.m[i];
-    R(0, 0) += 1.0;
-    R(1, 1) += 1.0;
-    R(2, 2) += 1.0;
+    for (int i = 0; i < 9; ++i) R.m[i] += a * K.m[i] + b * K2.m[i];
     return R;
 }
 
@@ -709,11 +705,8 @@ struct PoseRefiner {
         size_t const n = nl.size() / 2;
         for (size_t k = 0; k < n; ++k) {
             double const px = nl[2 * k], py = nl[2 * k + 1], qx = nr[2 * k], qy = nr[2 * k + 1];
-            double const ep0 = E(0, 0) * px + E(0, 1) * py + E(0, 2);
-            double const ep1 = E(1, 0) * px + E(1, 1) * py + E(1, 2);
-            double const ep2 = E(2, 0) * px + E(2, 1) * py + E(2, 2);
-            double const etq0 = E(0, 0) * qx + E(1, 0) * qy + E(2, 0);
-            double const etq1 = E(0, 1) * qx + E(1, 1) * qy + E(2, 1);
+            double const ep0 = affine_row(E, 0, px, py), ep1 = affine_row(E, 1, px, py), ep2 = affine_row(E, 2, px, py);
+            double const etq0 = affine_col(E, 0, qx, qy), etq1 = affine_col(E, 1, qx, qy);
             double const num = qx * ep0 + qy * ep1 + ep2;
             f[k] = num / std::sqrt(ep0 * ep0 + ep1 * ep1 + etq0 * etq0 + etq1 * etq1);
         }
@@ -726,11 +719,8 @@ struct PoseRefiner {
         size_t const n = pl.size() / 2;
         for (size_t k = 0; k < n; ++k) {
             double const px = pl[2 * k], py = pl[2 * k + 1], qx = pr[2 * k], qy = pr[2 * k + 1];
-            double const fp0 = F(0, 0) * px + F(0, 1) * py + F(0, 2);
-            double const fp1 = F(1, 0) * px + F(1, 1) * py + F(1, 2);
-            double const fp2 = F(2, 0) * px + F(2, 1) * py + F(2, 2);
-            double const ftq0 = F(0, 0) * qx + F(1, 0) * qy + F(2, 0);
-            double const ftq1 = F(0, 1) * qx + F(1, 1) * qy + F(2, 1);
+            double const fp0 = affine_row(F, 0, px, py), fp1 = affine_row(F, 1, px, py), fp2 = affine_row(F, 2, px, py);
+            double const ftq0 = affine_col(F, 0, qx, qy), ftq1 = affine_col(F, 1, qx, qy);
             double const e = (qx * fp0 + qy * fp1 + fp2) / std::sqrt(fp0 * fp0 + fp1 * fp1 + ftq0 * ftq0 + ftq1 * ftq1);
             w[k] = (std::fabs(e) < thr) ? 1.0 : std::sqrt(thr / std::fabs(e));
         }
@@ -758,18 +748,23 @@ struct PoseRefiner {
             double const p[3] = {nl[2 * k], nl[2 * k + 1], 1.0}, q[3] = {nr[2 * k], nr[2 * k + 1], 1.0};
             double ep[3], etq[3];
             for (int r = 0; r < 3; ++r) {
-                ep[r] = E(r, 0) * p[0] + E(r, 1) * p[1] + E(r, 2) * p[2];
-                etq[r] = E(0, r) * q[0] + E(1, r) * q[1] + E(2, r) * q[2];
+                ep[r] = affine_row(E, r, p[0], p[1]);
+                etq[r] = affine_col(E, r, q[0], q[1]);
             }
             double const num = q[0] * ep[0] + q[1] * ep[1] + ep[2];
             double const S = ep[0] * ep[0] + ep[1] * ep[1] + etq[0] * etq[0] + etq[1] * etq[1];
             double const sq = std::sqrt(S);
+            double const invSq = 1.0 / sq, invS = 1.0 / S;
+            // d num / dE = q p^T; d S / dE: 2 (Ep)_r p_c (rows 0, 1) + 2 (E^T q)_c q_r (columns
+            // 0, 1), the entries with a factor 1 or 0 written as the reference writes them
+            double const dS[9] = {2 * ep[0] * p[0] + 2 * etq[0] * q[0], 2 * ep[0] * p[1] + 2 * etq[1] * q[0], 2 * ep[0],
+                                  2 * ep[1] * p[0] + 2 * etq[0] * q[1], 2 * ep[1] * p[1] + 2 * etq[1] * q[1], 2 * ep[1],
+                                  2 * etq[0], 2 * etq[1], 0.0};
             double outer[9];
             for (int r = 0; r < 3; ++r)
                 for (int c = 0; c < 3; ++c) {
                     double const dn = q[r] * p[c];
-                    double const dS = (r < 2 ? 2.0 * ep[r] * p[c] : 0.0) + (c < 2 ? 2.0 * etq[c] * q[r] : 0.0);
-                    outer[3 * r + c] = (dn * sq - num * 0.5 / sq * dS) / S;
+                    outer[3 * r + c] = (dn * sq - num * 0.5 * invSq * dS[3 * r + c]) * invS;
                 }
             for (int c = 0; c < 6; ++c) {
                 double s = 0.0;
@@ -780,30 +775,35 @@ struct PoseRefiner {
     }
 };
 
+// L L^T x = b, the Cholesky-Banachiewicz order of the public-domain JAMA/TNT routine the
+// reference's optimiser calls (V3D/Math/v3d_linear_tnt.h:93-171): L_jk = (N_jk - sum_i<k
+// L_ki L_ji) / L_kk, L_jj = sqrt(N_jj - sum_k<j L_jk^2); false when N is not positive
+// definite
 bool cholesky_solve6(double (&N)[6][6], double const (&b)[6], double (&x)[6])
 {
     double L[6][6] = {};
-    for (int i = 0; i < 6; ++i)
-        for (int j = 0; j <= i; ++j) {
-            double s = N[i][j];
-            for (int k = 0; k < j; ++k) s -= L[i][k] * L[j][k];
-            if (i == j) {
-                if (!(s > 0.0)) return false;
-                L[i][i] = std::sqrt(s);
-            } else {
-                L[i][j] = s / L[j][j];
-            }
+    bool spd = true;
+    for (int j = 0; j < 6; ++j) {
+        double d = 0.0;
+        for (int k = 0; k < j; ++k) {
+            double s = 0.0;
+            for (int i = 0; i < k; ++i) s += L[k][i] * L[j][i];
+            L[j][k] = s = (N[j][k] - s) / L[k][k];
+            d = d + s * s;
         }
-    double y[6];
-    for (int i = 0; i < 6; ++i) {
-        double s = b[i];
-        for (int k = 0; k < i; ++k) s -= L[i][k] * y[k];
-        y[i] = s / L[i][i];
+        d = N[j][j] - d;
+        spd = spd && d > 0.0;
+        L[j][j] = std::sqrt(d > 0.0 ? d : 0.0);
     }
-    for (int i = 5; i >= 0; --i) {
-        double s = y[i];
-        for (int k = i + 1; k < 6; ++k) s -= L[k][i] * x[k];
-        x[i] = s / L[i][i];
+    if (!spd) return false;
+    for (int k = 0; k < 6; ++k) x[k] = b[k];
+    for (int k = 0; k < 6; ++k) {
+        for (int i = 0; i < k; ++i) x[k] -= x[i] * L[k][i];
+        x[k] /= L[k][k];
+    }
+    for (int k = 5; k >= 0; --k) {
+        for (int i = k + 1; i < 6; ++i) x[k] -= x[i] * L[i][k];
+        x[k] /= L[k][k];
     }
     return true;
 }
@@ -855,9 +855,9 @@ void refine_pose(PoseRefiner const& pr, Mat3& R, double (&T)[3])
             for (int b = 0; b < 6; ++b) N[a][b] = JtJ[a][b] + (a == b ? lambda : 0.0);
         double delta[6];
         if (!cholesky_solve6(N, Jte, delta)) return;
-        double dl = 0.0;
-        for (int a = 0; a < 6; ++a) dl += delta[a] * delta[a];
-        dl = std::sqrt(dl);
+        double dl2 = 0.0;
+        for (int a = 0; a < 6; ++a) dl2 += delta[a] * delta[a];
+        double const dl = std::sqrt(dl2);
         if (it >= 10 && dl < 1e-8 * (4.0 + 1e-8)) return;
         Mat3 const savedR = R;
         double const savedT[3] = {T[0], T[1], T[2]};
@@ -871,7 +871,7 @@ void refine_pose(PoseRefiner const& pr, Mat3& R, double (&T)[3])
         for (size_t k = 0; k < n; ++k) errNew += (w[k] * f[k]) * (w[k] * f[k]);
         double dJ = 0.0;
         for (int a = 0; a < 6; ++a) dJ += delta[a] * Jte[a];
-        double const rho = (err - errNew) / (lambda * dl * dl + dJ);
+        double const rho = (err - errNew) / (dl2 * lambda + dJ);
         if (rho > 0.0) {
             lambda = std::max(1e-10, lambda * 0.1);
             derivs = true;
@@ -925,7 +925,7 @@ struct Rng {
 // getShuffleCount (v3d_homography.cpp:14-18)
 int shuffle_count(int N, int nRounds, int minSampleSize)
 {
-    return std::max(1, (int)((float)nRounds * (float)minSampleSize / (float)N + 0.5f));
+    return std::max(1, (int)((float)nRounds * (float)minSampleSize / (float)N + 0.5));
 }
 
 // ransacNSamples(nNeededSamples, outlierRatio, wantedPsucces) (reconstruction_common.h:642-650)
@@ -946,9 +946,9 @@ int v3dm_sampson_errors(const double F[9], const double* corrs, int n, double* o
 {
     if (!F || n < 0 || (n > 0 && (!corrs || !out))) return V3DM_ERR_INVALID;
     Mat3 const f = load3(F);
-    for (int i = 0; i < n; ++i) {
+    for (int i = 0; i < n; ++i) {  // positions as the reference's PointMeasurement holds them: float
         double const* c = corrs + 4 * (size_t)i;
-        out[i] = sampson(f, c[0], c[1], c[2], c[3]);
+        out[i] = sampson(f, (float)c[0], (float)c[1], (float)c[2], (float)c[3]);
     }
     return V3DM_OK;
 }
@@ -996,16 +996,22 @@ int v3dm_refine_relative_pose(const double* corrs, int n, const double K1[9], co
     return V3DM_OK;
 }
 
-int v3dm_robust_orientation(const double* corrs, int n, const double K1[9], const double K2[9],
+int v3dm_robust_orientation(const double* raw, int n, const double K1[9], const double K2[9],
                             double inlierThreshold, int nSamples, int reportInliers, int iterativeRefinement,
                             uint64_t seed, v3dm_orientation* res, int32_t* inliers)
 {
     int const minSize = 5;
+    double const* corrs = raw;
     if (!res || !K1 || !K2 || (n > 0 && !corrs)) return V3DM_ERR_INVALID;
     if (n < minSize) return V3DM_ERR_INVALID;  // "At least 5 point correspondences are required."
     if (reportInliers && !inliers) return V3DM_ERR_INVALID;
     std::memset(res, 0, sizeof(*res));
     res->error = INFINITY;
+    // the reference's correspondences hold float positions (PointMeasurement), and it keeps
+    // the normalised ones in the same type: the samples' coordinates are float-rounded
+    std::vector<double> cf(4 * (size_t)n);
+    for (size_t i = 0; i < cf.size(); ++i) cf[i] = (float)raw[i];
+    corrs = cf.data();
     double const sqrThr = inlierThreshold * inlierThreshold;
     Mat3 const k1 = load3(K1), k2 = load3(K2);
     Mat3 iK1, iK2;
@@ -1014,8 +1020,8 @@ int v3dm_robust_orientation(const double* corrs, int n, const double K1[9], cons
     std::vector<double> nx1(n), ny1(n), nx2(n), ny2(n);
     for (int i = 0; i < n; ++i) {
         double const* c = corrs + 4 * (size_t)i;
-        normalise_point(iK1, c[0], c[1], nx1[i], ny1[i]);
-        normalise_point(iK2, c[2], c[3], nx2[i], ny2[i]);
+        normalise_point_t<float>(iK1, c[0], c[1], nx1[i], ny1[i]);
+        normalise_point_t<float>(iK2, c[2], c[3], nx2[i], ny2[i]);
     }
     unsigned const nShuffles = (unsigned)shuffle_count(n, nSamples, minSize);
     double minError = 1e30;

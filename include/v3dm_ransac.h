@@ -12,7 +12,12 @@
  *
  * Conventions (the reference's): x2 ~ R x1 + T for normalised image points x = K^-1 (u, v,
  * 1); E = [T]x R; F = K2^-T E K1^-1; matrices row-major; T of unit length.  Correspondences
- * are rows of four doubles (left u, left v, right u, right v) in pixels.
+ * are rows of four doubles (left u, left v, right u, right v) in pixels; where the reference
+ * takes them as PointCorrespondence (v3dm_sampson_errors, v3dm_robust_orientation) they are
+ * used as it holds them, rounded to float (PointMeasurement::pos is a Vector2f,
+ * V3D/Geometry/v3d_mviewutilities.h:14-17), and the arithmetic follows the reference's
+ * operation order, so that the results are the reference's to rounding for the same
+ * samples.
  *
  * Return codes are the V3DM_* codes of v3dm.h; messages are not kept (no handle). */
 #ifndef V3DM_RANSAC_H
@@ -51,13 +56,18 @@ int v3dm_essentials_from_five(const double x1[5], const double y1[5], const doub
 
 /* relativePoseFromEssential(E, n, x1, y1, x2, y2, R, T) (v3d_poseutilities.cpp:1263-1291):
  * the decomposition of E whose triangulated points lie in front of both cameras for all n
- * correspondences.  *ok = 0 (R, T zero) when the points disagree on the configuration. */
+ * correspondences, decided point by point as the reference does (Nister's singular vectors
+ * from cross products of the rows of E, exact triangulation and the twisted-pair test).
+ * *ok = 0 (R, T zero) when the points disagree on the configuration. */
 int v3dm_relative_pose_from_essential(const double E[9], int n, const double* x1, const double* y1,
                                       const double* x2, const double* y2, double R[9], double T[3], int* ok);
 
 /* refineRelativePose (v3d_poseutilities.cpp:1063-1099): Levenberg-Marquardt over (R, T) on
  * the signed Sampson residuals of the normalised points, reweighted by the pixel-space
- * Sampson error against sampsonInlierThreshold; R and T are updated in place. */
+ * Sampson error against sampsonInlierThreshold; R and T are updated in place.  The
+ * reference's optimiser schedule, including its Rodrigues map, which takes a rotation step
+ * of at most 1e-6 rad as the identity (V3D/Math/v3d_mathutilities.h:170-190).  Positions as
+ * given (the reference takes them as Vector2d here). */
 int v3dm_refine_relative_pose(const double* corrs, int n, const double K1[9], const double K2[9], double R[9],
                               double T[3], double sampsonInlierThreshold);
 

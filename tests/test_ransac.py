@@ -38,11 +38,13 @@ def test_sampson_errors_match_reference_formula():
     got = ransac.sampson_errors(F, corrs)
     want = OR.sampson_error(F, corrs)
     assert np.array_equal(got, want)  # same operations in the same order
-    # a correspondence on its epipolar line scores 0
+    # a correspondence on its epipolar line scores 0, up to the single-precision rounding of
+    # the positions (the reference's PointMeasurement holds floats)
     R, T, x1, x2 = _scene(rng, 3)
     E = OR.essential_from_pose(R, T)
     c = np.c_[x1, x2]
-    assert np.abs(ransac.sampson_errors(E, c)).max() < 1e-24
+    assert np.abs(ransac.sampson_errors(E, c)).max() < 1e-14
+    assert np.abs(ransac.sampson_errors(E, c.astype(np.float32).astype(np.float64))).max() < 1e-14
 
 
 def test_five_point_contains_true_essential():
@@ -142,7 +144,8 @@ def test_sample_counts_follow_reference_formulas():
     assert OR.ransac_n_samples(5, 0.0, 1.0 - 10e-10) == 0
     corrs, _ = OR.synthetic_two_view(rng, 200, 0, K, R, T, 0.0)
     res = ransac.robust_orientation(corrs, K, K, 1.0, 1000, True, False, seed=1)
-    assert res["samples"] == 1 and res["n_inliers"] == 200 and res["error"] < 1e-12
+    # (the error is what rounding the positions to float, as the reference stores them, leaves)
+    assert res["samples"] == 1 and res["n_inliers"] == 200 and res["error"] < 1e-8
     # random correspondences: no early stop; getShuffleCount shuffles of floor((N - 1) / 5)
     # quintuples each, or nSamples + 1 samples, whichever is smaller
     for n, n_samples in ((12, 1000), (100, 7), (37, 40)):
