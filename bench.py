@@ -93,8 +93,10 @@ def time_batch(m, prec, steps, warmup, read_back_steps=3):
     for _ in range(read_back_steps):
         res = m.batch_match(MIN_SCORE, MIN_RATIO)
     ms_rb = (time.perf_counter() - t0) * 1e3 / read_back_steps
+    tuning = m.batch_get_tuning(prec)
+    tuning["shape"] = m.batch_get_tile_shape(prec)  # query (plain: left) rows x candidate rows per work-group
     return res, {"ms_per_batch": ms, "pairs_per_s": BATCH_PAIRS / (ms * 1e-3), "gpu_ms_last_run": gpu_ms,
-                 "ms_per_batch_read_back": ms_rb, "tuning": m.batch_get_tuning(prec)}
+                 "ms_per_batch_read_back": ms_rb, "tuning": tuning}
 
 
 def cpu_baseline_batch(images, left, right):
