@@ -177,6 +177,65 @@ def test_learned_batch(matcher):
     m.set_precision(v3dmatch.PREC_FP32)
 
 
+def _oracle_pairs(images, left, right):
+    out = []
+    for a, b in zip(left, right):
+        L, R = synth.dequantise(images[a]), synth.dequantise(images[b])
+        eL, eR = oracle.round32(L.shape[0]), oracle.round32(R.shape[0])
+        out.append(np.zeros((0, 2), np.int32) if eL == 0 or eR == 0
+                   else np.asarray(oracle.find_putative_matches(L[:eL], R[:eR], 0.9, 0.8), np.int32).reshape(-1, 2))
+    return out
+
+
+def test_learned_batch_ragged_images(matcher):
+    """Every batch form on ragged images: an image without features, images below the
+    32-feature granularity (rounded to none) or just above it, an image paired with itself,
+    one used by several pairs and an empty image paired with itself.  Pairs without features
+    on a side return no matches and have zero check counts; the plain form returns the
+    oracle's set pair by pair; the BF16 form and the learned form (check off and on) keep
+    >= 99.5% recall and precision against it."""
+    m, net = matcher
+    counts0, desc0, left0, right0, _ = synth.make_batch(10, lo=40, hi=1500, config=51)
+    x1 = synth.make_pair(20, 700, 0.4, config=52, pair=0)
+    x2 = synth.make_pair(33, 64, 0.5, config=53, pair=1)
+    n = len(counts0)
+    E, T, B, C, D = n, n + 1, n + 2, n + 3, n + 4
+    counts = np.concatenate([counts0, [0, 20, 700, 33, 64]]).astype(np.int32)
+    desc = np.concatenate([desc0, x1[0], x1[1], x2[0], x2[1]])
+    left = np.concatenate([left0, [E, 0, T, B, C, D, B, 0, E]]).astype(np.int32)
+    right = np.concatenate([right0, [0, E, B, T, D, C, B, 2, E]]).astype(np.int32)
+    offs = np.concatenate([[0], np.cumsum(counts)])
+    images = [desc[offs[k]:offs[k + 1]] for k in range(len(counts))]
+    want = _oracle_pairs(images, left, right)
+    empty = [p for p in range(len(left)) if oracle.round32(int(counts[left[p]])) == 0
+             or oracle.round32(int(counts[right[p]])) == 0]
+    say(f"case ragged batch: {len(left)} pairs, {len(empty)} without features on a side")
+    m.batch_set_images(counts, desc)
+    m.batch_set_pairs(left, right)
+    m.set_precision(v3dmatch.PREC_FP32)
+    got = m.batch_match(0.9, 0.8)
+    for p in range(len(left)):
+        assert np.array_equal(got[p], want[p]), p
+    ws = set((p, a, b) for p in range(len(left)) for a, b in want[p].tolist())
+    for mode, samples in ((v3dmatch.PREC_BF16, 0), (v3dmatch.PREC_LEARNED, 0), (v3dmatch.PREC_LEARNED, 128)):
+        m.set_precision(mode)
+        m.set_fallback(samples, 0.995)
+        got = m.batch_match(0.9, 0.8)
+        for p in empty:
+            assert len(got[p]) == 0, (mode, p)
+        gs = set((p, a, b) for p in range(len(left)) for a, b in got[p].tolist())
+        inter = len(ws & gs)
+        say(f"  mode {mode} samples {samples}: recall {inter / len(ws):.5f} precision {inter / max(len(gs), 1):.5f}")
+        assert inter >= 0.995 * len(ws) and inter >= 0.995 * len(gs)
+        if mode == v3dmatch.PREC_LEARNED and samples:
+            chk = m.batch_get_check()
+            for p in empty:
+                assert not chk[p].any(), (p, chk[p])
+            assert chk[[p for p in range(len(left)) if p not in empty], 3].min() > 0
+    m.set_fallback(128, 0.995)
+    m.set_precision(v3dmatch.PREC_FP32)
+
+
 def test_learned_batch_tile_shapes(matcher):
     """Every work-group shape of the learned code pass (128 or 256 queries per 64-candidate
     tile) lists the same candidates: identical match sets, with the check on; the shape the
