@@ -4,6 +4,10 @@
   before a match) and leave the handle usable;
 * a result larger than the caller's buffer gives V3DM_ERR_CAPACITY with the full count in
   *nPairs, and the pairs that fit are the first ones of the full result;
+* invalid sizes and arguments (negative or above-limit maxima, negative feature counts, a pair
+  naming an image outside the batch, unknown modes and tile shapes, a batch result larger
+  than the caller's buffer) give V3DM_ERR_INVALID / V3DM_ERR_CAPACITY and leave the handle
+  usable;
 * the mutual check's ordered compaction waits, boundedly, for the counts of the work-groups
   before it; the time-out path is forced from the host (v3dm_debug_mutual_wait: no
   predecessor count is ever seen, a limit of 8 sleeps), so every kernel still ends
@@ -22,6 +26,7 @@ from v3dmatch._native import load_library
 
 pytestmark = pytest.mark.gpu
 
+V3DM_ERR_INVALID = 1
 V3DM_ERR_STATE = 2
 V3DM_ERR_CAPACITY = 3
 V3DM_ERR_HIP = 4
@@ -118,3 +123,61 @@ def test_mutual_wait_runs_out(pair):
         assert np.array_equal(np.asarray(got, np.int32).reshape(-1, 2), np.asarray(want, np.int32).reshape(-1, 2))
     finally:
         m.close()
+
+
+def test_invalid_arguments():
+    lib = load_library()
+    h = _handle(lib)
+    try:
+        say("case invalid: allocate sizes")
+        assert lib.v3dm_allocate(h, -1, 64, 0) == V3DM_ERR_INVALID
+        assert lib.v3dm_allocate(h, (1 << 20) + 32, 64, 0) == V3DM_ERR_INVALID
+        assert b"2^20" in lib.v3dm_last_error(h)
+        assert lib.v3dm_allocate(h, 64, 64, 0) == 0  # usable afterwards
+        say("case invalid: batch images and pairs")
+        desc = np.zeros((100, 128), np.uint8)
+        bad = np.array([60, -1], np.int32)
+        assert lib.v3dm_batch_set_images_u8(h, 2, _addr(bad), _addr(desc)) == V3DM_ERR_INVALID
+        good = np.array([60, 40], np.int32)
+        assert lib.v3dm_batch_set_images_u8(h, 2, _addr(good), _addr(desc)) == 0
+        left, right = np.array([0, 5], np.int32), np.array([1, 0], np.int32)
+        assert lib.v3dm_batch_set_pairs(h, 2, _addr(left), _addr(right)) == V3DM_ERR_INVALID
+        assert b"outside" in lib.v3dm_last_error(h)
+        assert lib.v3dm_batch_set_tuning(h, 1, 7, 0, 0) == V3DM_ERR_INVALID
+        assert lib.v3dm_batch_set_tile_shape(h, 0, 64) == V3DM_ERR_INVALID
+        assert lib.v3dm_batch_set_tile_shape(h, 9, 128) == V3DM_ERR_INVALID
+    finally:
+        lib.v3dm_destroy(h)
+
+
+def test_batch_capacity_error():
+    """A batch result larger than the caller's buffer: V3DM_ERR_CAPACITY with the full count;
+    the same run read again with room for it returns every pair, equal to the result of a
+    normal call."""
+    from v3dmatch import synth
+    lib = load_library()
+    h = _handle(lib)
+    try:
+        counts, desc, left, right, _ = synth.make_batch(6, lo=300, hi=2000, config=61)
+        desc = np.ascontiguousarray(desc)
+        assert lib.v3dm_batch_set_images_u8(h, len(counts), _addr(counts), _addr(desc)) == 0
+        assert lib.v3dm_batch_set_pairs(h, len(left), _addr(left), _addr(right)) == 0
+        offs = np.zeros(len(left) + 1, np.int32)
+        small = np.zeros(2 * 4, np.int32)
+        n = C.c_int(-1)
+        say("case batch capacity: cap 4")
+        rc = lib.v3dm_batch_match(h, C.c_float(0.9), C.c_float(0.8), _addr(offs), _addr(small), 4, C.byref(n))
+        assert rc == V3DM_ERR_CAPACITY and n.value > 4 and offs[-1] == n.value
+        full = np.zeros(2 * n.value, np.int32)
+        n2 = C.c_int(-1)
+        assert lib.v3dm_batch_get_matches(h, _addr(offs), _addr(full), n.value, C.byref(n2)) == 0
+        assert n2.value == n.value
+        ref = np.zeros(2 * n.value, np.int32)
+        n3 = C.c_int(-1)
+        assert lib.v3dm_batch_match(h, C.c_float(0.9), C.c_float(0.8), _addr(offs), _addr(ref), n.value,
+                                    C.byref(n3)) == 0
+        assert n3.value == n.value and np.array_equal(full, ref)
+        ms = C.c_float(0)
+        assert lib.v3dm_batch_time_dominant_kernel(h, 0, C.byref(ms)) == V3DM_ERR_INVALID
+    finally:
+        lib.v3dm_destroy(h)
