@@ -236,6 +236,39 @@ def test_learned_batch_ragged_images(matcher):
     m.set_precision(v3dmatch.PREC_FP32)
 
 
+def test_batch_graph_replays_equal_eager(matcher):
+    """Every form's batch run replayed from its captured hipGraph several times, with eager
+    work of the same handle (a single-pair learned call) and synchronisations between the
+    replays, equals the run launched kernel by kernel: match sets, and for the learned form
+    (sampled check and plain fallback on) the per-pair check counts and fallback flags."""
+    m, net = matcher
+    counts, desc, left, right, _ = synth.make_batch(24, lo=300, hi=6000, config=47)
+    qL, qR, _ = synth.make_pair(2048, 2048, 0.4, config=1, pair=21)
+    L, R = synth.dequantise(qL), synth.dequantise(qR)
+    m.batch_set_images(counts, desc)
+    m.batch_set_pairs(left, right)
+    m.set_fallback(128, 0.995)
+    for mode in (v3dmatch.PREC_FP32, v3dmatch.PREC_BF16, v3dmatch.PREC_LEARNED):
+        m.set_precision(mode)
+        m.batch_use_graph(False)
+        eager = m.batch_match(0.9, 0.8)
+        chk = m.batch_get_check() if mode == v3dmatch.PREC_LEARNED else None
+        flags = m.batch_get_fallback() if mode == v3dmatch.PREC_LEARNED else None
+        m.batch_use_graph(True)
+        for rep in range(3):
+            got = m.batch_match(0.9, 0.8)
+            say(f"case graph replay: mode {mode} replay {rep}: {sum(len(g) for g in got)} matches")
+            assert all(np.array_equal(a, b) for a, b in zip(got, eager)), (mode, rep)
+            if chk is not None:
+                assert np.array_equal(m.batch_get_check(), chk) and np.array_equal(m.batch_get_fallback(), flags)
+            # eager work of the same handle between replays
+            m.setLeftFeatures(2048, L)
+            m.setRightFeatures(2048, R)
+            m.find_putative_matches_array(0.9, 0.8)
+            m.synchronize()
+    m.set_precision(v3dmatch.PREC_FP32)
+
+
 def test_learned_batch_tile_shapes(matcher):
     """Every work-group shape of the learned code pass (128 or 256 queries per 64-candidate
     tile) lists the same candidates: identical match sets, with the check on; the shape the
