@@ -42,7 +42,7 @@ PEAK_INT8_MFMA = 5.0e15
 PEAK_HBM = 8.0e12
 N_PAIR = 16384
 N_FP8 = 32768
-N_C1 = 4096
+N_SMALL = 4096
 BATCH_PAIRS = 256
 MIN_SCORE, MIN_RATIO = 0.9, 0.8
 METRIC = "image pairs matched/s @\u226599.5% match recall, learned/BF16 vs plain FP32"
@@ -346,23 +346,23 @@ def bench_fp8(args):
             "recall_reference": "the plain FP32 form's set (identical to the oracle's)"}
 
 
-def bench_c1(args):
+def bench_small_pair(args):
     """The CPU-runnable configuration: one 4096 x 4096 pair, plain form, GPU and CPU."""
     import v3dmatch
     from v3dmatch import synth
-    qL, qR, _ = synth.make_pair(N_C1, N_C1, 0.4, config=0, pair=0)
+    qL, qR, _ = synth.make_pair(N_SMALL, N_SMALL, 0.4, config=0, pair=0)
     L, R = synth.dequantise(qL), synth.dequantise(qR)
     m = v3dmatch.SIFT_Matching(0)
     m.enableRoundingWarning(False)
-    m.allocate(N_C1, N_C1)
-    m.setLeftFeatures(N_C1, L)
-    m.setRightFeatures(N_C1, R)
+    m.allocate(N_SMALL, N_SMALL)
+    m.setLeftFeatures(N_SMALL, L)
+    m.setRightFeatures(N_SMALL, R)
     ms, kms, _ = time_form(m, args.steps, args.warmup)
     got = m.get_matches()
     rb = time_read_back(m, args.steps)
     m.close()
-    flop = 2.0 * N_C1 * N_C1 * 128
-    out = {"workload": f"one pair of {N_C1}x{N_C1} descriptors, 40% noisy copies, plain FP32 form",
+    flop = 2.0 * N_SMALL * N_SMALL * 128
+    out = {"workload": f"one pair of {N_SMALL}x{N_SMALL} descriptors, 40% noisy copies, plain FP32 form",
            "ms_per_step": ms, "pairs_per_s": 1e3 / ms, "ms_per_call_read_back": rb, "kernel": "k_fused_fp32",
            "kernel_ms": kms, "frac_fp32_peak": flop / (kms * 1e-3) / PEAK_FP32_MFMA, "matches": int(len(got))}
     if not args.no_cpu:
@@ -385,7 +385,7 @@ def bench_score_error():
     m = v3dmatch.SIFT_Matching(0)
     m.enableRoundingWarning(False)
     m.allocate(1024, 1024)
-    for config, n in ((0, N_C1), (2, N_PAIR), (4, N_FP8)):
+    for config, n in ((0, N_SMALL), (2, N_PAIR), (4, N_FP8)):
         qL, qR, _ = synth.make_pair(n, n, 0.4, config=config, pair=3)
         L, R = synth.dequantise(qL[:1024]), synth.dequantise(qR[:1024])
         m.setLeftFeatures(1024, L)
@@ -620,7 +620,7 @@ def main():
         out["pair_16384"] = bench_pair(args)
         out["fp8_32768"] = bench_fp8(args)
         log("bench: 4096 pair")
-        out["pair_4096"] = bench_c1(args)
+        out["pair_4096"] = bench_small_pair(args)
         log("bench: score errors")
         out["score_error"] = bench_score_error()
         if not args.no_extras:

@@ -107,11 +107,11 @@ def test_u8_upload_equals_float(matcher):
     assert np.array_equal(a, b)
 
 
-def test_c2_full_size(matcher):
-    say("case C2 16384x16384")
+def test_pair_16384_full_size(matcher):
+    say("case pair 16384x16384")
     L, R, _ = _pair(16384, 16384, pair=0, config=2)
     got = _check(matcher, L, R, 16384, 16384)
-    say(f"  C2 matches={len(got)} rechecked={matcher.stats()['nRechecked']}")
+    say(f"  16384x16384 matches={len(got)} rechecked={matcher.stats()['nRechecked']}")
 
 
 @pytest.mark.parametrize("nL,nR,seed", [(512, 480, 51), (2048, 1024, 52), (4096, 4096, 53)])
