@@ -579,7 +579,8 @@ void enqueue_derived(v3dm_context* h, BatchState* b, int form)
 {
     hipStream_t const st = h->stream;
     if (form == 1) {
-        launch_to_bf16(b->img, b->imgRows, b->imgB, st);
+        if (b->haveU8) launch_batch_u8_to_bf16(b->imgU8, b->denom, b->imgRows, b->imgB, st);
+        else launch_to_bf16(b->img, b->imgRows, b->imgB, st);
     } else if (form == 2) {
         MlpWeights const& w = *learned_weights(ctx_learned(h));
         float const scale = learned_code_scale(ctx_learned(h));

@@ -113,6 +113,8 @@ __device__ __forceinline__ int sample_side(BatchDev const& b, int p)
 }
 
 
+// the BF16 copy of a uint8 batch from its uint8 rows and row scales (v3dm_batch.hip)
+void launch_batch_u8_to_bf16(uint8_t const* u8, float const* denom, int rows, __bf16* Y, hipStream_t st);
 // kernels of the plain FP32 form (v3dm_fp32.hip); leftRows: left features per work-group,
 // 128 (4 waves) or 256 (8 waves), the plan's left blocks; nItemsDev (may be null): the item
 // count in device memory, at most nItems

@@ -440,6 +440,34 @@ void launch_batch_load_u8(uint8_t const* src, int const* srcRow, int const* nImg
                        rows, img, imgU8, denom, rowCS);
 }
 
+// The BF16 copy of a uint8 batch from the uint8 rows and their scales (128 + 4 bytes read per
+// row instead of the 512 of the FP32 rows): each element is formed as k_batch_load_u8 forms
+// it (dequant_u8_value, the same IEEE product) and rounded to bf16 (nearest even), so the copy
+// equals the conversion of the FP32 image bit for bit.  Eight elements per thread.
+__global__ __launch_bounds__(256) void k_batch_u8_to_bf16(uint8_t const* __restrict__ u8,
+                                                          float const* __restrict__ denom, int rows,
+                                                          __bf16* __restrict__ Y)
+{
+    int const t = blockIdx.x * 256 + threadIdx.x;
+    int const row = t >> 4;
+    if (row >= rows) return;
+    uint2 const q = reinterpret_cast<uint2 const*>(u8)[t];  // elements 8 (t & 15) .. +7 of the row
+    float const d = denom[row];
+    __bf16 out[8];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        out[k] = (__bf16)dequant_u8_value((q.x >> (8 * k)) & 0xffu, d);
+        out[4 + k] = (__bf16)dequant_u8_value((q.y >> (8 * k)) & 0xffu, d);
+    }
+    reinterpret_cast<uint4*>(Y)[t] = *reinterpret_cast<uint4 const*>(out);
+}
+
+void launch_batch_u8_to_bf16(uint8_t const* u8, float const* denom, int rows, __bf16* Y, hipStream_t st)
+{
+    if (rows <= 0) return;
+    hipLaunchKernelGGL(k_batch_u8_to_bf16, dim3((rows * 16 + 255) / 256), dim3(256), 0, st, u8, denom, rows, Y);
+}
+
 void launch_batch_load_f32(float const* src, int const* srcRow, int const* nImg, int const* blkImg,
                            int const* imgBase, int rows, float* img, hipStream_t st)
 {

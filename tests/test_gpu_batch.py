@@ -124,6 +124,27 @@ def test_batch_bf16_recall(matcher, small_batch):
     matcher.batch_set_tuning(autotune=True, mode=v3dmatch.PREC_BF16)
 
 
+def test_bf16_copy_from_uint8_rows(matcher, small_batch):
+    """The BF16 copy of a uint8 batch is formed from the uint8 rows and their scales; it
+    must equal the conversion of the dequantised FP32 rows bit for bit: the BF16 form's
+    match sets are identical between the uint8 upload and the FP32 upload of the same
+    dequantised descriptors (whose copy is converted from the FP32 rows), under the same
+    fixed tuning: the same lists, the same windows, and decisions that equal the exact ones
+    on both paths."""
+    counts, desc, left, right, _ = small_batch
+    res = []
+    for d in (desc, synth.dequantise(desc)):
+        matcher.batch_set_images(counts, d)
+        matcher.batch_set_pairs(left, right)
+        matcher.set_precision(v3dmatch.PREC_BF16)
+        matcher.batch_set_tuning(autotune=False, mode=v3dmatch.PREC_BF16, tiles_per_item=4, grid=300)
+        res.append(matcher.batch_match(0.9, 0.8))
+    say(f"case bf16 copy: {sum(len(r) for r in res[0])} matches from uint8 rows")
+    assert all(np.array_equal(a, b) for a, b in zip(res[0], res[1]))
+    matcher.batch_set_tuning(autotune=True, mode=v3dmatch.PREC_BF16)
+    matcher.set_precision(v3dmatch.PREC_FP32)
+
+
 def test_batch_tile_shapes(matcher, small_batch):
     """Both work-group shapes of the plain pass (128 left features on 4 waves, 256 on 8)
     give the oracle's match set under several work-item tunings; both shapes of the BF16
