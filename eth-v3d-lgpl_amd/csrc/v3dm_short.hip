@@ -47,18 +47,45 @@ __device__ __forceinline__ float pack_key(float v, int pos)
     return __uint_as_float((__float_as_uint(v) & ~63u) | (unsigned int)pos);
 }
 
+// Running best key of the lane's two candidate groups (k1: group 0, k2: group 1) and the
+// tile each was found in.
 struct KeyState {
     float k1, k2;
     int t1, t2;
 };
 
-__device__ __forceinline__ void key_tile_update(KeyState& s, float o1, float o2, int t)
+// max of three keys in one v_max3_f32 (fmaxf would first canonicalise each key, which the
+// compiler cannot prove canonical after the bit packing).  The keys come from VALU
+// instructions, never straight from an MFMA result.
+__device__ __forceinline__ float key_max3(float a, float b, float c)
 {
-    bool const c1 = s.k1 != o1;
-    bool const c2 = s.k2 != o2;
-    int const nt2 = (c1 && s.k2 == o1) ? s.t1 : (c2 ? t : s.t2);
-    s.t1 = c1 ? t : s.t1;
-    s.t2 = nt2;
+    float r;
+    __asm__("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+// Group g of a tile against one query block: elements r with r / 8 == g of the first
+// candidate block and r / 8 == 1 - g of the second (16 keys), folded into the running best
+// by a tree of eight max3; the group's tile changes when its best does.
+template <bool kFull>
+__device__ __forceinline__ float group_best(v16f const& a0, v16f const& a1, int g, float o, int jt, int nC, int h)
+{
+    float x[16];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+        for (int rr = 0; rr < 8; ++rr) {
+            int const r = 8 * (g ^ cb) + rr;
+            float v = cb == 0 ? a0[r] : a1[r];
+            if (!kFull) {
+                int const jl = cb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+                v = (jt + jl < nC) ? v : -1e30f;
+            }
+            x[8 * cb + rr] = pack_key(v, cb * 16 + r);
+        }
+    float const b0 = key_max3(key_max3(x[0], x[1], x[2]), key_max3(x[3], x[4], x[5]), key_max3(x[6], x[7], x[8]));
+    float const b1 = key_max3(key_max3(x[9], x[10], x[11]), key_max3(x[12], x[13], x[14]), x[15]);
+    return key_max3(b0, b1, o);
 }
 
 // Tile t of the candidates -> one LDS stage by direct global->LDS loads
@@ -120,24 +147,12 @@ __device__ __forceinline__ void tile_bf16(unsigned char const* stage, v8bf const
 #pragma unroll
     for (int qb = 0; qb < 2; ++qb) {
         float const o1 = st[qb].k1, o2 = st[qb].k2;
-        float k1 = o1, k2 = o2;
-#pragma unroll
-        for (int cb = 0; cb < 2; ++cb) {
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                float v = acc[cb][qb][r];
-                if (!kFull) {
-                    int const jl = cb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-                    v = (jt + jl < nC) ? v : -1e30f;
-                }
-                float const key = pack_key(v, cb * 16 + r);
-                k2 = __builtin_amdgcn_fmed3f(k1, k2, key);
-                k1 = fast_max(k1, key, inf);
-            }
-        }
+        float const k1 = group_best<kFull>(acc[0][qb], acc[1][qb], 0, o1, jt, nC, h);
+        float const k2 = group_best<kFull>(acc[0][qb], acc[1][qb], 1, o2, jt, nC, h);
+        st[qb].t1 = (k1 != o1) ? t : st[qb].t1;
+        st[qb].t2 = (k2 != o2) ? t : st[qb].t2;
         st[qb].k1 = k1;
         st[qb].k2 = k2;
-        key_tile_update(st[qb], o1, o2, t);
     }
 }
 
@@ -168,10 +183,12 @@ __device__ __forceinline__ void run_tiles_bf16(__bf16 const* __restrict__ C, uns
 // tiles of 64 (two 32-row blocks, A operand with the same k map): while tile t is
 // multiplied, tile t+1 is already in flight into the other stage.  Lane l then holds, per
 // 32x32 block, the scores of query l%32 against candidate rows jl = 8*(r/4) + 4*h + r%4,
-// r = 0..15.  Per query and lane half the running top-2 is kept on keys: the FP32 score
-// with its 6 low mantissa bits replaced by the position (block*16 + r) inside the tile,
-// so that a med3 pair updates value and position together; the tile of each of the two
-// keys is tracked once per tile.  Masked candidates score -1e30.
+// r = 0..15.  Per query and lane half the best key of each of two disjoint candidate
+// groups is kept (the group maxima stand in for a top-2: the rescoring's window is set from
+// the largest two listed values): the FP32 score with its 6 low mantissa bits replaced by
+// the position (block*16 + r) inside the tile, so that one max3 tree updates value and
+// position together; the tile of each group's key is tracked once per tile.  Masked
+// candidates score -1e30.
 // One work unit: query block `qblk` (64 * kQW queries; the single-pair pass: 4 waves, 256
 // queries) against candidate tiles [split * tilesPerSplit, ...) -> part[((split * ldq) + q)
 // * 2 + h].
