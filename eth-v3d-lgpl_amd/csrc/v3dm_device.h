@@ -161,6 +161,48 @@ __device__ __forceinline__ int xcd_block(int b, int G)
 }
 
 typedef float v16f __attribute__((ext_vector_type(16)));
+
+// Shortlist keys (BF16 and FP8 passes): the FP32 score with its 6 low mantissa bits
+// replaced by the position (block * 16 + r) of the score inside a 64-candidate tile.
+__device__ __forceinline__ float pack_key(float v, int pos)
+{
+    return __uint_as_float((__float_as_uint(v) & ~63u) | (unsigned int)pos);
+}
+
+// max of three keys in one v_max3_f32 (fmaxf would first canonicalise each key, which the
+// compiler cannot prove canonical after the bit packing).  The keys come from VALU
+// instructions, never straight from an MFMA result.
+__device__ __forceinline__ float key_max3(float a, float b, float c)
+{
+    float r;
+    __asm__("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+// Group g of a tile against one query block: elements r with r / 8 == g of the first
+// candidate block and r / 8 == 1 - g of the second (16 keys), folded into the running best
+// o by a tree of eight max3 (the caller tracks the tile the best comes from).
+template <bool kFull>
+__device__ __forceinline__ float group_best(v16f const& a0, v16f const& a1, int g, float o, int jt, int nC, int h)
+{
+    float x[16];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+        for (int rr = 0; rr < 8; ++rr) {
+            int const r = 8 * (g ^ cb) + rr;
+            float v = cb == 0 ? a0[r] : a1[r];
+            if (!kFull) {
+                int const jl = cb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+                v = (jt + jl < nC) ? v : -1e30f;
+            }
+            x[8 * cb + rr] = pack_key(v, cb * 16 + r);
+        }
+    float const b0 = key_max3(key_max3(x[0], x[1], x[2]), key_max3(x[3], x[4], x[5]), key_max3(x[6], x[7], x[8]));
+    float const b1 = key_max3(key_max3(x[9], x[10], x[11]), key_max3(x[12], x[13], x[14]), x[15]);
+    return key_max3(b0, b1, o);
+}
+
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 }  // namespace v3dm

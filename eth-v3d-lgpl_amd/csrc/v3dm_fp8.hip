@@ -51,7 +51,8 @@ void launch_to_fp8(float const* X, int rows, unsigned char* Y, unsigned char* E,
 // candidates per tile through two LDS stages filled by direct global->LDS loads.  Stage
 // layout: row r, half h (64 bytes) is segment h*64 + r at 80*(h*64 + r) (16-byte pad:
 // consecutive rows fall on different bank groups); the tile's 64 candidate scale bytes
-// follow at byte 10240.  The top-2 per query and lane half runs on index-packed keys.
+// follow at byte 10240.  Per query and lane half the best index-packed key of each of two
+// disjoint candidate groups is kept, as in the BF16 pass.
 constexpr int kFQ = 256;
 constexpr int kFT = 64;
 constexpr int kSegF = 80;
@@ -59,24 +60,12 @@ constexpr int kStageLoadsF = 3;  // 1 KiB direct loads per wave per tile (4 wave
 constexpr int kStageBytesF = kStageLoadsF * 4 * 1024;
 constexpr int kScaleOffF = 640 * 16;
 
+// Running best key of the lane's two candidate groups (k1: group 0, k2: group 1) and the
+// tile each was found in.
 struct KeyStateF {
     float k1, k2;
     int t1, t2;
 };
-
-__device__ __forceinline__ float pack_key_f(float v, int pos)
-{
-    return __uint_as_float((__float_as_uint(v) & ~63u) | (unsigned int)pos);
-}
-
-__device__ __forceinline__ void key_tile_update_f(KeyStateF& s, float o1, float o2, int t)
-{
-    bool const c1 = s.k1 != o1;
-    bool const c2 = s.k2 != o2;
-    int const nt2 = (c1 && s.k2 == o1) ? s.t1 : (c2 ? t : s.t2);
-    s.t1 = c1 ? t : s.t1;
-    s.t2 = nt2;
-}
 
 // Source of stage slot `slot`: descriptor bytes (offset within the tile) or, for the four
 // slots after the 640 data slots, the tile's scale bytes (flag bit 30 set).
@@ -128,24 +117,12 @@ __device__ __forceinline__ void tile_fp8(unsigned char const* stage, v8i const (
 #pragma unroll
     for (int qb = 0; qb < 2; ++qb) {
         float const o1 = st[qb].k1, o2 = st[qb].k2;
-        float k1 = o1, k2 = o2;
-#pragma unroll
-        for (int cb = 0; cb < 2; ++cb) {
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                float v = acc[cb][qb][r];
-                if (!kFull) {
-                    int const jl = cb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-                    v = (jt + jl < nC) ? v : -1e30f;
-                }
-                float const key = pack_key_f(v, cb * 16 + r);
-                k2 = __builtin_amdgcn_fmed3f(k1, k2, key);
-                k1 = fast_max(k1, key, inf);
-            }
-        }
+        float const k1 = group_best<kFull>(acc[0][qb], acc[1][qb], 0, o1, jt, nC, h);
+        float const k2 = group_best<kFull>(acc[0][qb], acc[1][qb], 1, o2, jt, nC, h);
+        st[qb].t1 = (k1 != o1) ? t : st[qb].t1;
+        st[qb].t2 = (k2 != o2) ? t : st[qb].t2;
         st[qb].k1 = k1;
         st[qb].k2 = k2;
-        key_tile_update_f(st[qb], o1, o2, t);
     }
 }
 

@@ -42,51 +42,10 @@ constexpr int kST = 64;           // candidates per tile (16 KB of bf16)
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-__device__ __forceinline__ float pack_key(float v, int pos)
-{
-    return __uint_as_float((__float_as_uint(v) & ~63u) | (unsigned int)pos);
-}
-
-// Running best key of the lane's two candidate groups (k1: group 0, k2: group 1) and the
-// tile each was found in.
 struct KeyState {
     float k1, k2;
     int t1, t2;
 };
-
-// max of three keys in one v_max3_f32 (fmaxf would first canonicalise each key, which the
-// compiler cannot prove canonical after the bit packing).  The keys come from VALU
-// instructions, never straight from an MFMA result.
-__device__ __forceinline__ float key_max3(float a, float b, float c)
-{
-    float r;
-    __asm__("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
-    return r;
-}
-
-// Group g of a tile against one query block: elements r with r / 8 == g of the first
-// candidate block and r / 8 == 1 - g of the second (16 keys), folded into the running best
-// by a tree of eight max3; the group's tile changes when its best does.
-template <bool kFull>
-__device__ __forceinline__ float group_best(v16f const& a0, v16f const& a1, int g, float o, int jt, int nC, int h)
-{
-    float x[16];
-#pragma unroll
-    for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-        for (int rr = 0; rr < 8; ++rr) {
-            int const r = 8 * (g ^ cb) + rr;
-            float v = cb == 0 ? a0[r] : a1[r];
-            if (!kFull) {
-                int const jl = cb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-                v = (jt + jl < nC) ? v : -1e30f;
-            }
-            x[8 * cb + rr] = pack_key(v, cb * 16 + r);
-        }
-    float const b0 = key_max3(key_max3(x[0], x[1], x[2]), key_max3(x[3], x[4], x[5]), key_max3(x[6], x[7], x[8]));
-    float const b1 = key_max3(key_max3(x[9], x[10], x[11]), key_max3(x[12], x[13], x[14]), x[15]);
-    return key_max3(b0, b1, o);
-}
 
 // Tile t of the candidates -> one LDS stage by direct global->LDS loads
 // (global_load_lds_dwordx4, lane L of a load landing at +16L).  Stage layout: row r, half
