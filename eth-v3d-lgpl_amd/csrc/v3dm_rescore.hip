@@ -541,6 +541,14 @@ __global__ __launch_bounds__(256) void k_batch_rescore_learned(BatchDev b, Cand2
     int const lb = xcd_block(blockIdx.x, gridDim.x);  // 64-query blocks, in blkInfo's order
     BatchBlock const bi = b.blkInfo[lb >> 2];         // one scalar load
     int const base = (lb & 3) * 64;                   // first query of the work-group in its block
+    // the scoring's query rows, loaded while wave 0 selects: lane k of quad qi holds bytes
+    // 32k .. 32k+31 of query row base + qi (rows of the padded image, always in range)
+    int const k = tid & 3;
+    int const qi = tid >> 2;  // query base + qi
+    int const row = bi.rowBase + base + qi;
+    uint4 const* qp = reinterpret_cast<uint4 const*>(b.u8 + (size_t)row * kDim + 32 * k);
+    uint4 const q0 = qp[0], q1 = qp[1];
+    float const ra = b.denom[row];
     int n = 0;
     if (tid < 64) {
         // ---- selection: query base + tid
@@ -602,20 +610,14 @@ __global__ __launch_bounds__(256) void k_batch_rescore_learned(BatchDev b, Cand2
     __syncthreads();  // the selections in LDS
     add_stripe(n, stripes);
     // ---- scoring: four lanes per query
-    int const k = tid & 3;
-    int const qi = tid >> 2;  // query base + qi
     int const c = sCnt[qi];   // uniform over the quad
     if (c <= 0) return;       // whole quads leave: the exchanges stay inside quads
     int const g = bi.blk * 256 + base + qi;
-    int const row = bi.rowBase + base + qi;
     uint8_t const* __restrict__ cRows = b.u8 + (size_t)bi.cBase * kDim;  // uniform
     float const* __restrict__ cDen = b.denom + bi.cBase;
     int js[kSelK];
 #pragma unroll
     for (int i = 0; i < kSelK; ++i) js[i] = sJ[qi][i];
-    uint4 const* qp = reinterpret_cast<uint4 const*>(b.u8 + (size_t)row * kDim + 32 * k);
-    uint4 const q0 = qp[0], q1 = qp[1];
-    float const ra = b.denom[row];
     uint4 cv[kSelK][2];
 #pragma unroll
     for (int i = 0; i < kSelK; ++i) {
