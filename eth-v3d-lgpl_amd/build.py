@@ -24,6 +24,9 @@ HIP_FLAGS = [
     # packed FP32 instructions are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
+# per-source additions: the sampled check's probe keeps its MFMA accumulators in ordinary
+# VGPRs (the default form moved every accumulator through AGPRs, 32 extra moves per tile)
+HIP_SOURCE_FLAGS = {"v3dm_check.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 HIP_SOURCES = ["v3dm_fp32.hip", "v3dm_select.hip", "v3dm_short.hip", "v3dm_api.cpp", "v3dm_ext.cpp", "v3dm_batch.cpp",
                "v3dm_learned.cpp", "v3dm_features.cpp",
                "v3dm_voctree.cpp", "v3dm_ransac.cpp"]
@@ -58,7 +61,7 @@ def build_hip(jobs=8):
     def compile_one(src):
         obj = os.path.join(OBJ, src + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
-        _run([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
+        _run([HIPCC] + HIP_FLAGS + HIP_SOURCE_FLAGS.get(src, []) + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
         return obj
 
     with ThreadPoolExecutor(max_workers=jobs) as ex:

@@ -417,9 +417,10 @@ int upload_plan(v3dm_context* h, BatchState* b, int form, Plan const& pl)
 }
 
 // Items of the sampled check.  uint8 images (the mutual check of v3dm_check.hip): per phase
-// (pair, -, split, kCheckTiles 32-row tiles, see below) of the image the phase scans (phase 0: the
-// pair's smaller image, phase 1: the larger).  FP32 images: (pair, 0, split, 8 tiles of 64
-// features of the smaller image) for the samples' own decisions.
+// up to kCheckTiles 32-row tiles of the image the phase scans (phase 0: the pair's smaller
+// image, phase 1: the larger), each item as two int4 with every row offset resolved here.
+// FP32 images: (pair, 0, split, 8 tiles of 64 features of the smaller image) for the
+// samples' own decisions.
 int ensure_sample_plan(v3dm_context* h, BatchState* b)
 {
     int const P = b->nPairs;
@@ -452,20 +453,33 @@ int ensure_sample_plan(v3dm_context* h, BatchState* b)
         int maxS = 1;
         std::vector<int4> items[2];
         for (int phase = 0; phase < 2; ++phase) {
-            std::vector<std::pair<int, int4>> work;
+            // two int4 per item (the probe reads them in one go): (pair, split, first tile,
+            // tiles) and (row of the first candidate tile, first row of the query image, nA, 0)
+            std::vector<std::pair<int, int4>> work, rec;
             for (int p = 0; p < P; ++p) {
                 int const nL = seg[p].n, nR = seg[P + p].n;
                 if (nL <= 0 || nR <= 0) continue;
                 int const nA = std::max(nL, nR), nB = std::min(nL, nR);
+                int const sa = nL >= nR ? 0 : 1;  // the device's sample_side
+                int const imgA = seg[sa * P + p].img, imgB = seg[(1 - sa) * P + p].img;
+                int const cBase = b->imgBaseH[phase == 0 ? imgB : imgA];
+                int const qBase = b->imgBaseH[phase == 0 ? imgA : imgB];
                 int const nTiles = (phase == 0 ? nB : nA) / 32;
                 int const tps = kCheckTiles[phase];
                 int const splits = (nTiles + tps - 1) / tps;
                 nSplits[phase * P + p] = splits;
                 maxS = std::max(maxS, splits);
-                for (int sp = 0; sp < splits; ++sp)
-                    work.push_back({std::min(tps, nTiles - sp * tps), make_int4(p, 0, sp, tps)});
+                for (int sp = 0; sp < splits; ++sp) {
+                    int const t0 = sp * tps, nT = std::min(tps, nTiles - t0);
+                    work.push_back({nT, make_int4(p, sp, t0, nT)});
+                    rec.push_back({nT, make_int4(cBase + 32 * t0, qBase, nA, 0)});
+                }
             }
-            items[phase] = sorted_items(work);
+            std::vector<int4> const first = sorted_items(work), second = sorted_items(rec);
+            for (size_t i = 0; i < first.size(); ++i) {
+                items[phase].push_back(first[i]);
+                items[phase].push_back(second[i]);
+            }
         }
         size_t const slots = (size_t)P * kSampleRows;
         for (int phase = 0; phase < 2; ++phase) {
@@ -473,7 +487,7 @@ int ensure_sample_plan(v3dm_context* h, BatchState* b)
             if (!items[phase].empty())
                 BCHECK(h, hipMemcpy(b->cItems[phase], items[phase].data(), items[phase].size() * sizeof(int4),
                                     hipMemcpyHostToDevice));
-            b->nCItems[phase] = (int)items[phase].size();
+            b->nCItems[phase] = (int)items[phase].size() / 2;
         }
         BCHECK(h, hipMalloc(&b->cNSplits, 2 * P * sizeof(int)));
         BCHECK(h, hipMemcpy(b->cNSplits, nSplits.data(), 2 * P * sizeof(int), hipMemcpyHostToDevice));
@@ -621,10 +635,10 @@ int enqueue_run(v3dm_context* h, BatchState* b, int mode, float minScore, float 
             if (b->haveU8) {
                 // exact mutual matches among the samples against the learned result
                 CheckDev const c = check_view(b, h->fallbackSamples);
-                launch_batch_check_probe(d, c, 0, b->cItems[0], b->nCItems[0], b->counters + kBQueueCheck0, cus, st);
+                launch_batch_check_probe(d, c, 0, b->cItems[0], b->nCItems[0], cus, st);
                 launch_batch_check_decide(d, c, 0, minScore, minRatio, lim, b->sel, b->pairFlag,
                                           b->pairFlag + b->nPairs, st);
-                launch_batch_check_probe(d, c, 1, b->cItems[1], b->nCItems[1], b->counters + kBQueueCheck1, cus, st);
+                launch_batch_check_probe(d, c, 1, b->cItems[1], b->nCItems[1], cus, st);
                 launch_batch_check_decide(d, c, 1, minScore, minRatio, lim, b->sel, b->pairFlag,
                                           b->pairFlag + b->nPairs, st);
             } else {

@@ -29,8 +29,7 @@ constexpr int kBCntPairs = 6;
 constexpr int kBCntFlagDone = 7;
 constexpr int kBCntRescoredDone = 8;
 constexpr int kBCntFbItems = 10;  // plain items of the pairs a learned run hands to the plain form
-constexpr int kBQueueCheck0 = 3;  // work counters of the sampled check (phase 0 / FP32 samples, phase 1)
-constexpr int kBQueueCheck1 = 11;
+constexpr int kBQueueCheck0 = 3;  // work counter of the sampled check of FP32 batches
 constexpr int kBCntOpenRescore = 12;  // open decisions of the shortlist rescoring
 constexpr int kBatchCounters = 16;
 
@@ -155,7 +154,7 @@ void launch_batch_sample_decide(BatchDev const& b, Top2 const* sPart, int maxSpl
 // the phase 1 decide sets pairFlag[p] and counts[4p..] = exact matches among the samples,
 // missed, decisions taken exactly, samples
 void launch_batch_check_probe(BatchDev const& b, CheckDev const& c, int phase, int4 const* items, int nItems,
-                              int* queue, int cus, hipStream_t st);
+                              int cus, hipStream_t st);
 void launch_batch_check_decide(BatchDev const& b, CheckDev const& c, int phase, float minScore, float minRatio,
                                MissLimits const& lim, int const* sel, int* pairFlag, int* counts, hipStream_t st);
 void launch_batch_recheck(BatchDev const& b, int const* flagList, int* counters, float minScore, float minRatio,

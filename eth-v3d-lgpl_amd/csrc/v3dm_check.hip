@@ -11,8 +11,8 @@
 //
 // Scores: the uint8 rows are taken as int8 (u ^ 0x80 = u - 128), so one v_mfma_i32_32x32x32_i8
 // chain over 128 components gives D = sum_k (qa_k - 128)(qb_k - 128), and with the MFMA's
-// accumulator input set to 128 * sum(qb) the integer I = sum_k qa_k qb_k = D + 128 * sum(qa)
-// - 2^21 is exact.  t = fl32(rb * I) and s~ = fl32(ra * t) (ra, rb the rows' 1/sqrt
+// accumulator input set to 128 * sum(qb) + 128 * sum(qa) - 2^21 its result is the exact
+// integer I = sum_k qa_k qb_k.  t = fl32(rb * I) and s~ = fl32(ra * t) (ra, rb the rows' 1/sqrt
 // scales) differ from the exact score fl32(sum_k fl32(qa_k ra) fl32(qb_k rb)) by at most
 // 5.01 * 2^-24 * s~ (every term is nonnegative); the candidates' keys keep t with its five
 // low mantissa bits replaced by the row inside the 32-row block (2^-18 relative), so a
@@ -49,6 +49,15 @@ __device__ __forceinline__ int chk_query(CheckDev const& c, int phase, int p, in
 }
 
 __device__ __forceinline__ float key_value(int key) { return __int_as_float(key & ~31); }
+
+// The second of the two largest after `key` joins (k1, k2), k1 >= k2: max(k2, min(k1, key)),
+// which is the median of the three, in one v_med3_i32.
+__device__ __forceinline__ int top2_second(int k1, int k2, int key)
+{
+    int r;
+    __asm__("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(k1), "v"(k2), "v"(key));
+    return r;
+}
 
 // Waves of a probe work-group: query block (32 rows) w % qbp of the ceil(nQ / 32) blocks,
 // tile part w / qbp of the item's tiles (qbp = the block count rounded up to 1, 2 or 4), so
@@ -90,14 +99,20 @@ __device__ __forceinline__ Top2 wg_top2(Top2 s, Top2* red)
 }  // namespace
 
 // ---------------------------------------------------------------------------- probe
-// Persistent over items (pair, -, split, tiles per split): the pair's (up to 128) query
-// rows of this phase against the split's 32-row candidate tiles.  Wave w takes query block
-// w % qbp and, of every group of `parts` consecutive tiles (probe_qbp), tile w / qbp.  The
-// work-group stages each group through LDS (rows padded to 144 bytes, so the lanes' 16-byte
-// reads of 32 rows spread over the banks), loading the next group into registers while the
-// current one is multiplied.  Per lane (query l % 32, candidate rows 8 (r / 4) + 4 (l / 32)
-// + r % 4 of each block) the two best keys and the tile of the best; the lane halves merge
-// by a shuffle, the waves of a query block through LDS (equal keys keep the smaller tile).
+// Persistent over items (two int4 each, every row offset resolved by the host: pair, split,
+// first tile, tiles; row of the first tile, first row of the query image, nA): the pair's
+// (up to 128) query rows of this phase against the item's 32-row candidate tiles.  Wave w
+// takes query block w % qbp (qbp = the block count rounded up to 1, 2 or 4) and, of every
+// group of four consecutive tiles, the tiles w / qbp + j qbp... (parts = 4 / qbp of them:
+// with few queries the waves split the tiles instead).  The work-group stages each group
+// through LDS (rows padded to 144 bytes, so the lanes' 16-byte reads of 32 rows spread over
+// the banks), loading the next group into registers while the current one is multiplied.
+// An item's loads depend only on its record, so they start together: the group's tiles,
+// and the query rows (phase 1: after the pair's list of matched features); the next item's
+// index is fetched while the current one runs.  Per lane (query l % 32, candidate rows
+// 8 (r / 4) + 4 (l / 32) + r % 4 of each block) the two best keys and the tile of the best;
+// the lane halves merge by a shuffle, the waves of a query block through LDS (equal keys
+// keep the smaller tile).
 // part: int4 (key1, key2, tile of key1, 0) per (phase, pair, split, query slot).
 __device__ __forceinline__ void key_merge(int& k1, int& k2, int& b1, int o1, int o2, int ob)
 {
@@ -109,22 +124,23 @@ __device__ __forceinline__ void key_merge(int& k1, int& k2, int& b1, int o1, int
 
 constexpr int kRowPadB = 144;                 // LDS bytes per staged 128-byte row
 constexpr int kTileB = kChkTile * kRowPadB;   // 4608
+constexpr int kGroupTiles = 4;                // tiles per stage
+constexpr int kProbeGroupsPerCU = 3;
 
 // group `grp` of the item's tiles [t0, t1) -> registers: thread tid holds chunk tid % 8 of
-// row tid / 8 of each tile g < parts, and rowCS of row tid % 32 when tid < 32
+// row tid / 8 of each of the group's tiles, and rowCS of row tid % 32 when tid < 32
 struct StageRegs {
-    uint4 v[4];
-    int2 cs[4];
+    uint4 v[kGroupTiles];
+    int2 cs[kGroupTiles];
 };
 
-__device__ __forceinline__ void stage_load(CheckDev const& c, size_t cBase, int t0, int t1, int grp, int parts,
-                                           StageRegs& r)
+__device__ __forceinline__ void stage_load(CheckDev const& c, size_t cBase, int t0, int t1, int grp, StageRegs& r)
 {
     int const tid = threadIdx.x;
 #pragma unroll
-    for (int g = 0; g < 4; ++g) {
-        int const t = t0 + grp * parts + g;
-        if (g < parts && t < t1) {
+    for (int g = 0; g < kGroupTiles; ++g) {
+        int const t = t0 + grp * kGroupTiles + g;
+        if (t < t1) {
             size_t const row = cBase + (size_t)t * kChkTile;
             r.v[g] = *reinterpret_cast<uint4 const*>(c.u8 + (row + (tid >> 3)) * kDim + 16 * (tid & 7));
             if (tid < kChkTile) r.cs[g] = c.rowCS[row + tid];
@@ -132,12 +148,13 @@ __device__ __forceinline__ void stage_load(CheckDev const& c, size_t cBase, int 
     }
 }
 
-__device__ __forceinline__ void stage_store(unsigned char* st, int2* stCS, int parts, StageRegs const& r)
+__device__ __forceinline__ void stage_store(unsigned char* st, int2* stCS, int t0, int t1, int grp,
+                                            StageRegs const& r)
 {
     int const tid = threadIdx.x;
 #pragma unroll
-    for (int g = 0; g < 4; ++g) {
-        if (g < parts) {
+    for (int g = 0; g < kGroupTiles; ++g) {
+        if (t0 + grp * kGroupTiles + g < t1) {
             uint4 const v = r.v[g];  // uint8 -> int8 u - 128
             *reinterpret_cast<uint4*>(st + g * kTileB + (tid >> 3) * kRowPadB + 16 * (tid & 7)) =
                 make_uint4(v.x ^ 0x80808080u, v.y ^ 0x80808080u, v.z ^ 0x80808080u, v.w ^ 0x80808080u);
@@ -146,46 +163,91 @@ __device__ __forceinline__ void stage_store(unsigned char* st, int2* stCS, int p
     }
 }
 
-__global__ __launch_bounds__(256) void k_batch_check_probe(BatchDev b, CheckDev c, int phase,
-                                                           int4 const* __restrict__ items, int nItems,
-                                                           int* __restrict__ queue)
+// One staged 32-row tile against this wave's query block: keys of the 32 x 32 block into
+// (k1, k2) and tile t as the tile of k1 when it holds it
+__device__ __forceinline__ void probe_tile(unsigned char const* tp, int2 const* cs, v4i const (&bq)[4], int cq, int t,
+                                           int h, int c32, int& k1, int& k2, int& blk1)
 {
-    __shared__ int sItem;
+    v4i a[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) a[m] = *reinterpret_cast<v4i const*>(tp + c32 * kRowPadB + 32 * m + 16 * h);
+    v16i cv;
+    float sv[16];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {  // accumulator input: 128 sum(qb) + 128 sum(qa) - 2^21
+        int4 const x0 = *reinterpret_cast<int4 const*>(cs + 8 * g + 4 * h);
+        int4 const x1 = *reinterpret_cast<int4 const*>(cs + 8 * g + 4 * h + 2);
+        cv[4 * g + 0] = x0.x + cq;
+        sv[4 * g + 0] = __int_as_float(x0.y);
+        cv[4 * g + 1] = x0.z + cq;
+        sv[4 * g + 1] = __int_as_float(x0.w);
+        cv[4 * g + 2] = x1.x + cq;
+        sv[4 * g + 2] = __int_as_float(x1.y);
+        cv[4 * g + 3] = x1.z + cq;
+        sv[4 * g + 3] = __int_as_float(x1.w);
+    }
+    v16i d = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[0], bq[0], cv, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[1], bq[1], d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[2], bq[2], d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[3], bq[3], d, 0, 0, 0);
+    int const o = k1;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        float const tv = sv[q] * (float)d[q];  // degenerate rows: negative
+        int const key = (__float_as_int(tv) & ~31) | (8 * (q >> 2) + 4 * h + (q & 3));
+        k2 = top2_second(k1, k2, key);
+        k1 = max(k1, key);
+    }
+    blk1 = (k1 != o) ? t : blk1;
+}
+
+__global__ __launch_bounds__(256) void k_batch_check_probe(BatchDev b, CheckDev c, int phase,
+                                                           int4 const* __restrict__ items, int nItems)
+{
     __shared__ int4 red[4][32];
-    __shared__ __attribute__((aligned(16))) unsigned char st[2][4 * kTileB];
-    __shared__ __attribute__((aligned(16))) int2 stCS[2][4 * kChkTile];
+    __shared__ __attribute__((aligned(16))) unsigned char st[2][kGroupTiles * kTileB];
+    __shared__ __attribute__((aligned(16))) int2 stCS[2][kGroupTiles * kChkTile];
     int const tid = threadIdx.x;
     int const lane = tid & 63;
     int const wave = tid >> 6;
     int const h = lane >> 5;
     int const c32 = lane & 31;
-    for (;;) {
-        if (tid == 0) sItem = atomicAdd(queue, 1);
-        __syncthreads();  // also: the previous item's stages and LDS merge are done
-        int const it = __builtin_amdgcn_readfirstlane(sItem);
-        __syncthreads();
-        if (it >= nItems) break;
-        int4 const w = items[it];
+    for (int it = blockIdx.x; it < nItems; it += gridDim.x) {
+        int4 const w = items[2 * it];       // (pair, split, first tile, tiles)
+        int4 const wr = items[2 * it + 1];  // (row of the first tile, first row of the query image, nA, 0)
         int const p = w.x;
-        int const sa = sample_side(b, p);
-        BatchSeg const sA = b.seg[sa * b.nPairs + p];
-        BatchSeg const sB = b.seg[(1 - sa) * b.nPairs + p];
-        int const nQ = chk_nq(b, c, phase, p, sA);
+        int const t0 = w.z, t1 = w.z + w.w;
+        size_t const cBase = (size_t)wr.x - (size_t)kChkTile * t0;  // tile t starts at row cBase + 32 t
+        int const nGroups = (t1 - t0 + kGroupTiles - 1) / kGroupTiles;
+        StageRegs r;
+        stage_load(c, cBase, t0, t1, 0, r);
+        // phase 1: the pair's matched features of B, every slot a wave may take (c32 + 32 j)
+        int qs[4] = {0, 0, 0, 0};
+        int nQ;
+        if (phase == 0) {
+            nQ = min(c.samples, wr.z);
+        } else {
+            nQ = c.cntB[p];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) qs[j] = c.qB[p * kSampleRows + 32 * j + c32];
+        }
         int const qbp = probe_qbp(nQ), parts = 4 / qbp;
         int const qb = wave % qbp, part = wave / qbp;
         bool const active = qb * 32 < nQ;  // uniform over the wave
         int const k = qb * 32 + c32;
-        int const cImg = phase == 0 ? sB.img : sA.img;
-        int const nC = phase == 0 ? sB.n : sA.n;
-        size_t const cBase = (size_t)b.imgBase[cImg];
-        int const t0 = w.z * w.w;
-        int const t1 = min(t0 + w.w, nC / kChkTile);
-        int const nGroups = (t1 - t0 + parts - 1) / parts;
+        // image-local row of query slot kk (phase 0: the kk-th evenly spaced feature of A),
+        // formed by every wave so that the slot loads above are always consumed here
+        int const kk = max(min(k, nQ - 1), 0);
+        int qLocal = 0;
+        if (phase == 0)
+            qLocal = nQ > 0 ? (int)(((long long)kk * wr.z) / nQ) : 0;
+        else
+            qLocal = kk == k ? (qb == 0 ? qs[0] : qb == 1 ? qs[1] : qb == 2 ? qs[2] : qs[3])
+                             : c.qB[p * kSampleRows + kk];
         v4i bq[4];
         int cq = 0;
         if (active) {
-            int const qLocal = chk_query(c, phase, p, min(k, nQ - 1), nQ, sA.n);
-            size_t const qRow = (size_t)b.imgBase[phase == 0 ? sA.img : sB.img] + qLocal;
+            size_t const qRow = (size_t)wr.y + qLocal;
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
                 v4i const u = *reinterpret_cast<v4i const*>(c.u8 + qRow * kDim + 32 * m + 16 * h);
@@ -193,52 +255,18 @@ __global__ __launch_bounds__(256) void k_batch_check_probe(BatchDev b, CheckDev 
             }
             cq = c.rowCS[qRow].x - (1 << 21);
         }
-        StageRegs r;
-        stage_load(c, cBase, t0, t1, 0, parts, r);
-        stage_store(st[0], stCS[0], parts, r);
+        stage_store(st[0], stCS[0], t0, t1, 0, r);
         __syncthreads();
         int k1 = INT_MIN, k2 = INT_MIN, blk1 = -1;
         for (int gi = 0; gi < nGroups; ++gi) {
             int const buf = gi & 1;
-            if (gi + 1 < nGroups) stage_load(c, cBase, t0, t1, gi + 1, parts, r);
-            int const t = t0 + gi * parts + part;
-            if (active && t < t1) {
-                unsigned char const* tp = st[buf] + part * kTileB;
-                v4i a[4];
-#pragma unroll
-                for (int m = 0; m < 4; ++m)
-                    a[m] = *reinterpret_cast<v4i const*>(tp + c32 * kRowPadB + 32 * m + 16 * h);
-                v16i cv;
-                float sv[16];
-                int2 const* cs = stCS[buf] + part * kChkTile;
-#pragma unroll
-                for (int g = 0; g < 4; ++g) {
-                    int4 const x0 = *reinterpret_cast<int4 const*>(cs + 8 * g + 4 * h);
-                    int4 const x1 = *reinterpret_cast<int4 const*>(cs + 8 * g + 4 * h + 2);
-                    cv[4 * g + 0] = x0.x;
-                    sv[4 * g + 0] = __int_as_float(x0.y);
-                    cv[4 * g + 1] = x0.z;
-                    sv[4 * g + 1] = __int_as_float(x0.w);
-                    cv[4 * g + 2] = x1.x;
-                    sv[4 * g + 2] = __int_as_float(x1.y);
-                    cv[4 * g + 3] = x1.z;
-                    sv[4 * g + 3] = __int_as_float(x1.w);
+            if (gi + 1 < nGroups) stage_load(c, cBase, t0, t1, gi + 1, r);
+            if (active)
+                for (int j = part; j < kGroupTiles; j += parts) {
+                    int const t = t0 + gi * kGroupTiles + j;
+                    if (t < t1) probe_tile(st[buf] + j * kTileB, stCS[buf] + j * kChkTile, bq, cq, t, h, c32, k1, k2, blk1);
                 }
-                v16i d = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[0], bq[0], cv, 0, 0, 0);
-                d = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[1], bq[1], d, 0, 0, 0);
-                d = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[2], bq[2], d, 0, 0, 0);
-                d = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[3], bq[3], d, 0, 0, 0);
-                int const o = k1;
-#pragma unroll
-                for (int q = 0; q < 16; ++q) {
-                    float const tv = sv[q] * (float)(d[q] + cq);  // degenerate rows: negative
-                    int const key = (__float_as_int(tv) & ~31) | (8 * (q >> 2) + 4 * h + (q & 3));
-                    k2 = max(k2, min(k1, key));
-                    k1 = max(k1, key);
-                }
-                blk1 = (k1 != o) ? t : blk1;
-            }
-            if (gi + 1 < nGroups) stage_store(st[buf ^ 1], stCS[buf ^ 1], parts, r);
+            if (gi + 1 < nGroups) stage_store(st[buf ^ 1], stCS[buf ^ 1], t0, t1, gi + 1, r);
             __syncthreads();
         }
         if (active) key_merge(k1, k2, blk1, __shfl_xor(k1, 32), __shfl_xor(k2, 32), __shfl_xor(blk1, 32));
@@ -252,8 +280,9 @@ __global__ __launch_bounds__(256) void k_batch_check_probe(BatchDev b, CheckDev 
                 }
         }
         if (active && part == 0 && h == 0 && k < nQ)
-            c.part[(((size_t)phase * b.nPairs + p) * c.maxSplits + w.z) * kSampleRows + k] =
+            c.part[(((size_t)phase * b.nPairs + p) * c.maxSplits + w.y) * kSampleRows + k] =
                 make_int4(k1, k2, blk1, 0);
+        __syncthreads();  // the stages and the LDS merge are done
     }
 }
 
@@ -454,11 +483,13 @@ __global__ __launch_bounds__(128) void k_batch_check_decide(BatchDev b, CheckDev
 }
 
 void launch_batch_check_probe(BatchDev const& b, CheckDev const& c, int phase, int4 const* items, int nItems,
-                              int* queue, int cus, hipStream_t st)
+                              int cus, hipStream_t st)
 {
     if (nItems <= 0) return;
-    // work-groups resident per CU (LDS and registers)
-    hipLaunchKernelGGL(k_batch_check_probe, dim3(3 * cus), dim3(256), 0, st, b, c, phase, items, nItems, queue);
+    // kProbeGroupsPerCU work-groups per CU (40 KiB of LDS and 128 VGPRs each: four would
+    // fit; three measured as fast or faster); items are taken in a fixed stride, largest first
+    hipLaunchKernelGGL(k_batch_check_probe, dim3(kProbeGroupsPerCU * cus), dim3(256), 0, st, b, c, phase, items,
+                       nItems);
 }
 
 void launch_batch_check_decide(BatchDev const& b, CheckDev const& c, int phase, float minScore, float minRatio,
