@@ -620,10 +620,13 @@ __global__ __launch_bounds__(256) void k_batch_rescore_learned(BatchDev b, Cand2
     for (int i = 0; i < kSelK; ++i) js[i] = sJ[qi][i];
     uint4 cv[kSelK][2];
 #pragma unroll
-    for (int i = 0; i < kSelK; ++i) {
-        uint4 const* cp = reinterpret_cast<uint4 const*>(cRows + ((unsigned)js[i] * (unsigned)kDim + 32u * k));
-        cv[i][0] = cp[0];
-        cv[i][1] = cp[1];
+    for (int i = 0; i < kSelK; ++i) {  // only the c selected rows are fetched (the rest: unused)
+        cv[i][0] = cv[i][1] = make_uint4(0u, 0u, 0u, 0u);
+        if (i < c) {
+            uint4 const* cp = reinterpret_cast<uint4 const*>(cRows + ((unsigned)js[i] * (unsigned)kDim + 32u * k));
+            cv[i][0] = cp[0];
+            cv[i][1] = cp[1];
+        }
     }
     int const ja = 2 * k < c ? js[2 * k] : -1;  // the candidates this lane finishes
     int const jb = 2 * k + 1 < c ? js[2 * k + 1] : -1;
