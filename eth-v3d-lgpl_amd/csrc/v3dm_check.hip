@@ -99,20 +99,20 @@ __device__ __forceinline__ Top2 wg_top2(Top2 s, Top2* red)
 }  // namespace
 
 // ---------------------------------------------------------------------------- probe
-// Persistent over items (two int4 each, every row offset resolved by the host: pair, split,
-// first tile, tiles; row of the first tile, first row of the query image, nA): the pair's
-// (up to 128) query rows of this phase against the item's 32-row candidate tiles.  Wave w
-// takes query block w % qbp (qbp = the block count rounded up to 1, 2 or 4) and, of every
-// group of four consecutive tiles, the tiles w / qbp + j qbp... (parts = 4 / qbp of them:
-// with few queries the waves split the tiles instead).  The work-group stages each group
-// through LDS (rows padded to 144 bytes, so the lanes' 16-byte reads of 32 rows spread over
-// the banks), loading the next group into registers while the current one is multiplied.
-// An item's loads depend only on its record, so they start together: the group's tiles,
-// and the query rows (phase 1: after the pair's list of matched features); the next item's
-// index is fetched while the current one runs.  Per lane (query l % 32, candidate rows
-// 8 (r / 4) + 4 (l / 32) + r % 4 of each block) the two best keys and the tile of the best;
-// the lane halves merge by a shuffle, the waves of a query block through LDS (equal keys
-// keep the smaller tile).
+// Each work-group takes the items blockIdx.x, blockIdx.x + gridDim.x, ... (largest first; two
+// int4 each, every row offset resolved by the host: pair, split, first tile, tiles; row of
+// the first tile, first row of the query image, nA): the pair's (up to 128) query rows of
+// this phase against the item's 32-row candidate tiles.  Wave w takes query block w % qbp
+// (qbp = the block count rounded up to 1, 2 or 4) and, of every group of four consecutive
+// tiles, tiles w / qbp, w / qbp + parts, ... (parts = 4 / qbp: with few queries the waves
+// split the tiles instead).  The work-group stages each group through LDS (rows padded to
+// 144 bytes, so the lanes' 16-byte reads of 32 rows spread over the banks), loading the
+// next group into registers while the current one is multiplied.  An item's loads depend
+// only on its record, so they start together: the first group's tiles, and the query rows
+// (phase 1: after the pair's list of matched features).  Per lane (query l % 32, candidate
+// rows 8 (r / 4) + 4 (l / 32) + r % 4 of each block) the two best keys and the tile of the
+// best; the lane halves merge by a shuffle, the waves of a query block through LDS (equal
+// keys keep the smaller tile).
 // part: int4 (key1, key2, tile of key1, 0) per (phase, pair, split, query slot).
 __device__ __forceinline__ void key_merge(int& k1, int& k2, int& b1, int o1, int o2, int ob)
 {
