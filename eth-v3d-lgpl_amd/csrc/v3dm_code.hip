@@ -123,14 +123,19 @@ __device__ __forceinline__ void code_block(signed char const* __restrict__ Q, in
 
     int const tFull = (t1 * kCT > nC) ? t1 - 1 : t1;  // the last tile may be partial
     // the next tile's A operands are loaded while the current tile is processed (the index
-    // is clamped to the split's last tile, so every load stays in range)
+    // is clamped to the split's last tile, so every load stays in range); the scheduling
+    // barriers keep each load ahead of the tile before it (left free, the compiler moved it
+    // behind that tile's MFMAs into the same registers, so that every tile waited for its
+    // own operands)
     v4i a[2], an[2];
     load_code_tile(C, t0, lo, a);
     int t = t0;
     for (; t + 1 < tFull; t += 2) {  // two tiles a round: the buffers alternate, no copies
         load_code_tile(C, t + 1, lo, an);
+        __builtin_amdgcn_sched_barrier(0);
         code_tile<true>(a, t, bq, pos, st, nC, h);
         load_code_tile(C, min(t + 2, t1 - 1), lo, a);
+        __builtin_amdgcn_sched_barrier(0);
         code_tile<true>(an, t + 1, bq, pos, st, nC, h);
     }
     // `a` holds tile t here
