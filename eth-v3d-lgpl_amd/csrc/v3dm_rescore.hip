@@ -463,8 +463,8 @@ __global__ __launch_bounds__(256, 8) void k_batch_rescore_u8(BatchDev b, Cand2 c
 // The learned lists of uint8 images (topK in [1, kSelK]): k_batch_rescore_learned, one
 // work-group per 64 queries of one segment (BatchBlock table, pair by pair).
 //
-// Selection, one lane per query (wave 0), one pass over the query's lists with four lists in
-// flight: the kSelK largest list values with their candidates (sorted, multiplicity
+// Selection, one lane per query (wave 0), one pass over the query's lists with eight lists in
+// flight (the same 90 registers as four; sixteen would cost a work-group per CU): the kSelK largest list values with their candidates (sorted, multiplicity
 // counted), then thr = max(m - margin, the topK-th largest) with m the largest -- the
 // eight-lane selection's threshold in both of its cases (at most topK values within the
 // margin: m - margin, which is then >= the topK-th largest; else the topK-th largest, which
@@ -566,16 +566,16 @@ __global__ __launch_bounds__(256) void k_batch_rescore_learned(BatchDev b, Cand2
 #pragma unroll
             for (int i = 0; i < kSelK; ++i) t[i] = -INFINITY;
             int total = 0;  // listed candidates
-            for (int e0 = 0; e0 < nLists; e0 += 4) {
-                Cand2 cs[4];
+            for (int e0 = 0; e0 < nLists; e0 += 8) {
+                Cand2 cs[8];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {  // four lists in flight
+                for (int u = 0; u < 8; ++u) {  // eight lists in flight
                     int const e = e0 + u;
                     cs[u] = Cand2{-INFINITY, -1, -INFINITY, -1};
                     if (e < nLists) cs[u] = lp[(size_t)(e >> 1) * ld2 + (e & 1)];
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < 8; ++u) {
                     if (cs[u].j1 >= 0) {
                         topk_insert_pair(t, jj, cs[u].v1, cs[u].j1);
                         ++total;
